@@ -1,0 +1,242 @@
+"""ctypes mirror of include/fdtd_acoustics.h and the loader of the HIP library.
+
+The product path is the C-ABI library ``acoustics_amd/lib/libfdtd_acoustics.so``
+built from ``acoustics_amd/csrc``.  Loading fails loudly when it is missing:
+there is no CPU fallback anywhere in the package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libfdtd_acoustics.so")
+
+FDTD_BOUNDARY_RASTERIZE = 0
+FDTD_BOUNDARY_FULLY_COUPLED = 1
+FDTD_OBJ_SPHERE = 0
+FDTD_OBJ_ELLIPSOID = 1
+
+STATUS_NAMES = {
+    0: "FDTD_OK", 1: "FDTD_ERR_INVALID", 2: "FDTD_ERR_NO_DEVICE", 3: "FDTD_ERR_DEVICE",
+    4: "FDTD_ERR_OOM", 5: "FDTD_ERR_PARSE", 6: "FDTD_ERR_UNSUPPORTED", 7: "FDTD_ERR_NONFINITE",
+    8: "FDTD_ERR_BUSY", 9: "FDTD_ERR_CANCELLED", 10: "FDTD_ERR_NOT_FOUND",
+}
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("sound_speed", C.c_double), ("density", C.c_double), ("cell_size", C.c_double),
+        ("time_step", C.c_double), ("stop_time", C.c_double), ("pml_width", C.c_double),
+        ("pml_strength", C.c_double), ("air_viscosity_alpha", C.c_double), ("start_time", C.c_double),
+        ("boundary_condition_preset", C.c_int), ("boundary_handling", C.c_int), ("use_mesh", C.c_int),
+        ("divisions", C.c_int * 3), ("min_bound", C.c_double * 3),
+    ]
+
+
+class Object(C.Structure):
+    _fields_ = [
+        ("type", C.c_int), ("reserved", C.c_int), ("center", C.c_double * 3),
+        ("semi_axes", C.c_double * 3), ("rotation", C.c_double * 9),
+    ]
+
+
+class ImpulseSource(C.Structure):
+    _fields_ = [("object", C.c_int), ("reserved", C.c_int), ("amplitude", C.c_double),
+                ("width", C.c_double), ("onset", C.c_double)]
+
+
+class PressureSource(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("width_time", C.c_double), ("width_space", C.c_double),
+                ("offset_time", C.c_double), ("normalize_constant", C.c_double)]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("settings", Settings),
+        ("n_objects", C.c_int), ("objects", C.POINTER(Object)),
+        ("n_impulse_sources", C.c_int), ("impulse_sources", C.POINTER(ImpulseSource)),
+        ("n_pressure_sources", C.c_int), ("pressure_sources", C.POINTER(PressureSource)),
+        ("n_listeners", C.c_int), ("listeners", C.POINTER(C.c_double)),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [("stream", C.c_void_p), ("device", C.c_int), ("steps_per_graph", C.c_int),
+                ("series_capacity", C.c_int), ("tile_config", C.c_int)]
+
+
+class Info(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("n_pml_cells", C.c_int64), ("n_ghost_cells", C.c_int64),
+                ("n_solid_cells", C.c_int64), ("n_listeners", C.c_int64), ("step_index", C.c_int64),
+                ("time", C.c_double), ("bytes_per_step", C.c_double), ("device_bytes", C.c_int64)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("stream", C.c_void_p), ("steps_per_graph", C.c_int), ("tile_config", C.c_int),
+                ("persistent_groups", C.c_int)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int), ("steps_per_block", C.c_int),
+                ("device", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("cancelled", C.c_int64), ("active", C.c_int64), ("queued", C.c_int64),
+                ("blocks_run", C.c_int64), ("peak_resident_bytes", C.c_int64)]
+
+
+# Every symbol include/fdtd_acoustics.h declares, with (restype, argtypes).
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_D = C.POINTER(C.c_double)
+SIGNATURES = {
+    "fdtd_status_string": (C.c_char_p, [C.c_int]),
+    "fdtd_settings_default": (None, [C.POINTER(Settings)]),
+    "fdtd_parse_xml": (C.c_int, [C.c_char_p, _PP]),
+    "fdtd_parsed_scene": (C.POINTER(Scene), [_P]),
+    "fdtd_parsed_num_steps": (C.c_int, [_P]),
+    "fdtd_parsed_free": (None, [_P]),
+    "fdtd_last_error": (C.c_char_p, []),
+    "fdtd_options_default": (None, [C.POINTER(Options)]),
+    "fdtd_create": (C.c_int, [C.POINTER(Scene), C.POINTER(Options), _PP]),
+    "fdtd_create_from_xml": (C.c_int, [C.c_char_p, C.POINTER(Options), _PP]),
+    "fdtd_destroy": (C.c_int, [_P]),
+    "fdtd_step": (C.c_int, [_P, C.c_int]),
+    "fdtd_synchronize": (C.c_int, [_P]),
+    "fdtd_listen": (C.c_int, [_P, _D]),
+    "fdtd_read_listener_series": (C.c_int, [_P, C.c_int64, C.c_int64, _D]),
+    "fdtd_read_pressure": (C.c_int, [_P, _D]),
+    "fdtd_get_info": (C.c_int, [_P, C.POINTER(Info)]),
+    "fdtd_set_timing": (C.c_int, [_P, C.c_int]),
+    "fdtd_last_kernel_ms": (C.c_int, [_P, _D, _D]),
+    "fdtd_batch_options_default": (None, [C.POINTER(BatchOptions)]),
+    "fdtd_batch_create": (C.c_int, [C.POINTER(Scene), C.POINTER(C.c_int), C.c_int,
+                                    C.POINTER(BatchOptions), _PP]),
+    "fdtd_batch_run": (C.c_int, [_P]),
+    "fdtd_batch_step": (C.c_int, [_P, C.c_int]),
+    "fdtd_batch_steps_done": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
+    "fdtd_batch_read_listener_series": (C.c_int, [_P, C.c_int, _D]),
+    "fdtd_batch_destroy": (C.c_int, [_P]),
+    "fdtd_server_options_default": (None, [C.POINTER(ServerOptions)]),
+    "fdtd_server_create": (C.c_int, [C.POINTER(ServerOptions), _PP]),
+    "fdtd_server_submit": (C.c_int, [_P, C.POINTER(Scene), C.c_int, C.POINTER(C.c_int64)]),
+    "fdtd_server_poll": (C.c_int, [_P, C.c_int64]),
+    "fdtd_server_wait": (C.c_int, [_P, C.c_int64, C.c_int]),
+    "fdtd_server_take_result": (C.c_int, [_P, C.c_int64, _D, _D]),
+    "fdtd_server_cancel": (C.c_int, [_P, C.c_int64]),
+    "fdtd_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "fdtd_scene_device_bytes": (C.c_int64, [C.POINTER(Scene)]),
+    "fdtd_server_destroy": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+class FDTDError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        name = STATUS_NAMES.get(status, str(status))
+        detail = ""
+        if _lib is not None:
+            msg = _lib.fdtd_last_error()
+            if msg:
+                detail = ": " + msg.decode(errors="replace")
+        super().__init__(f"{what} failed with {name}{detail}")
+
+
+def load_library(path: str | None = None):
+    """Loads the HIP C-ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"HIP library {p} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != 0:
+        raise FDTDError(status, what)
+
+
+class SceneBuffers:
+    """Owns the ctypes arrays a Scene points at (keeps them alive)."""
+
+    def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None):
+        self.settings_dict = dict(settings)
+        s = Settings()
+        for k, v in settings.items():
+            if k in ("divisions", "min_bound"):
+                arr = getattr(s, k)
+                for i in range(3):
+                    arr[i] = v[i]
+            else:
+                setattr(s, k, v)
+        self.objects_list = list(objects)
+        self.impulse_list = list(impulse_sources)
+        self.pressure_list = list(pressure_sources)
+        self._objs = (Object * max(1, len(self.objects_list)))()
+        for i, o in enumerate(self.objects_list):
+            self._objs[i].type = o["type"]
+            for d in range(3):
+                self._objs[i].center[d] = o["center"][d]
+                self._objs[i].semi_axes[d] = o["semi_axes"][d]
+            rot = np.asarray(o.get("rotation", np.eye(3)), dtype=np.float64).reshape(9)
+            for q in range(9):
+                self._objs[i].rotation[q] = rot[q]
+        self._isrc = (ImpulseSource * max(1, len(self.impulse_list)))()
+        for i, s_ in enumerate(self.impulse_list):
+            self._isrc[i].object = s_["object"]
+            self._isrc[i].amplitude = s_["amplitude"]
+            self._isrc[i].width = s_["width"]
+            self._isrc[i].onset = s_["onset"]
+        self._psrc = (PressureSource * max(1, len(self.pressure_list)))()
+        for i, s_ in enumerate(self.pressure_list):
+            for d in range(3):
+                self._psrc[i].position[d] = s_["position"][d]
+            self._psrc[i].width_time = s_["width_time"]
+            self._psrc[i].width_space = s_["width_space"]
+            self._psrc[i].offset_time = s_["offset_time"]
+            self._psrc[i].normalize_constant = s_["normalize_constant"]
+        lst = np.zeros((0, 3)) if listeners is None else np.asarray(listeners, dtype=np.float64).reshape(-1, 3)
+        self.listeners_np = np.ascontiguousarray(lst)
+        self.scene = Scene()
+        self.scene.settings = s
+        self.scene.n_objects = len(self.objects_list)
+        self.scene.objects = C.cast(self._objs, C.POINTER(Object))
+        self.scene.n_impulse_sources = len(self.impulse_list)
+        self.scene.impulse_sources = C.cast(self._isrc, C.POINTER(ImpulseSource))
+        self.scene.n_pressure_sources = len(self.pressure_list)
+        self.scene.pressure_sources = C.cast(self._psrc, C.POINTER(PressureSource))
+        self.scene.n_listeners = self.listeners_np.shape[0]
+        self.scene.listeners = self.listeners_np.ctypes.data_as(C.POINTER(C.c_double))
+
+    @property
+    def n_listeners(self):
+        return self.listeners_np.shape[0]
+
+    @property
+    def divisions(self):
+        return tuple(self.settings_dict["divisions"])
+
+    @property
+    def n_cells(self):
+        d = self.divisions
+        return d[0] * d[1] * d[2]
+
+    def ref(self):
+        return C.byref(self.scene)

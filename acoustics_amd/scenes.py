@@ -1,0 +1,198 @@
+"""Seeded synthetic scenes for the five workloads of the FDTD engine.
+
+Every scene follows the reference's static solver policy: a cubic (or, for
+batches and serving, box-shaped) MAC grid centred at the origin
+(``BoundingBox(h, n, center)``, src/geometry/BoundingBox.h:90-100), sound
+speed 343 m/s, density 1.2 kg/m^3, dt = 0.5 h / (c sqrt 3), PML strength 1e6,
+preset 0.  All randomness comes from ``numpy.random.default_rng(seed)``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .capi import (FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_ELLIPSOID,
+                   FDTD_OBJ_SPHERE, SceneBuffers)
+
+C_SOUND = 343.0
+RHO = 1.2
+PML_STRENGTH = 1.0e6
+
+
+def cfl_dt(h, c=C_SOUND, cfl=0.5):
+    return cfl * h / (c * math.sqrt(3.0))
+
+
+def base_settings(div, h, pml_width, boundary=FDTD_BOUNDARY_RASTERIZE, c=C_SOUND, rho=RHO, dt=None):
+    div = tuple(int(x) for x in div)
+    dt = cfl_dt(h, c) if dt is None else dt
+    # BoundingBox(cellSize, cellDivisions, center) puts min = center - h*n/2
+    min_bound = tuple(0.0 - h * float(n) / 2.0 for n in div)
+    return dict(sound_speed=c, density=rho, cell_size=h, time_step=dt, stop_time=0.0,
+                pml_width=float(pml_width), pml_strength=PML_STRENGTH, air_viscosity_alpha=0.0,
+                start_time=0.0, boundary_condition_preset=0, boundary_handling=boundary, use_mesh=1,
+                divisions=div, min_bound=min_bound)
+
+
+def interior_bounds(settings, margin_cells):
+    """Lower/upper corner of the region >= margin_cells inside the PML."""
+    h = settings["cell_size"]
+    w = settings["pml_width"]
+    lo = np.array(settings["min_bound"]) + (w + margin_cells) * h
+    hi = np.array(settings["min_bound"]) + (np.array(settings["divisions"]) - w - margin_cells) * h
+    return lo, hi
+
+
+def random_rotation(rng):
+    """Uniform random rotation from a uniform unit quaternion (Shoemake)."""
+    u1, u2, u3 = rng.random(3)
+    q = np.array([math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                  math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3)])
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def fibonacci_shell(center, radius, n):
+    pts = []
+    ga = math.pi * (3.0 - math.sqrt(5.0))
+    for i in range(n):
+        z = 1.0 - 2.0 * (i + 0.5) / n
+        r = math.sqrt(max(0.0, 1.0 - z * z))
+        th = ga * i
+        pts.append([center[0] + radius * r * math.cos(th), center[1] + radius * r * math.sin(th),
+                    center[2] + radius * z])
+    return np.array(pts)
+
+
+def gaussian_pressure_source(position, width_time, c=C_SOUND, offset=None):
+    ws = c * width_time
+    return dict(position=tuple(position), width_time=width_time, width_space=ws,
+                offset_time=4.0 * width_time if offset is None else offset,
+                normalize_constant=1.0 / (math.sqrt(2.0 * math.pi) * ws) ** 3)
+
+
+def _object_clear(p, objects, h, cells):
+    for o in objects:
+        r = max(o["semi_axes"])
+        if np.linalg.norm(np.asarray(p) - np.asarray(o["center"])) < r + cells * h:
+            return False
+    return True
+
+
+def _place_ellipsoids(rng, settings, count, axis_lo, axis_hi, margin_cells=2, gap_cells=2, tries=10000):
+    h = settings["cell_size"]
+    lo, hi = interior_bounds(settings, margin_cells)
+    objs = []
+    for _ in range(tries):
+        if len(objs) == count:
+            break
+        axes = rng.uniform(axis_lo, axis_hi, size=3)
+        r = axes.max()
+        if np.any(hi - lo <= 2 * r):
+            continue
+        c = rng.uniform(lo + r, hi - r)
+        if all(np.linalg.norm(c - np.asarray(o["center"])) > r + max(o["semi_axes"]) + gap_cells * h for o in objs):
+            objs.append(dict(type=FDTD_OBJ_ELLIPSOID, center=tuple(c), semi_axes=tuple(axes),
+                             rotation=random_rotation(rng)))
+    if len(objs) != count:
+        raise RuntimeError("could not place non-overlapping ellipsoids")
+    return objs
+
+
+def _random_listeners(rng, settings, objects, count, margin_cells=2, clear_cells=2):
+    h = settings["cell_size"]
+    lo, hi = interior_bounds(settings, margin_cells)
+    pts = []
+    while len(pts) < count:
+        p = rng.uniform(lo, hi)
+        if _object_clear(p, objects, h, clear_cells):
+            pts.append(p)
+    return np.array(pts)
+
+
+def config1(seed=1, n=64, steps=500):
+    """Free field: n^3 grid, h = 5 mm, 10-cell PML, one Gaussian pressure pulse
+    (width 2e-4 s) at a seeded interior point, 8 seeded listeners."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), 0.005, 10 if n >= 40 else max(2, n // 6))
+    lo, hi = interior_bounds(s, 2)
+    src = gaussian_pressure_source(rng.uniform(lo, hi), 2e-4)
+    listeners = rng.uniform(lo, hi, size=(8, 3))
+    return SceneBuffers(s, pressure_sources=[src], listeners=listeners), steps
+
+
+def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
+            n_listeners=64, shell=0.2):
+    """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse
+    normal acceleration (1e3 m/s^2, 1e-4 s, onset 4e-4 s); listeners on a shell."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml, boundary)
+    c = rng.uniform(-0.05, 0.05, size=3) * (n * h / 0.64)
+    obj = dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(radius, radius, radius), rotation=np.eye(3))
+    src = dict(object=0, amplitude=1e3, width=1e-4, onset=4e-4)
+    listeners = fibonacci_shell(c, shell, n_listeners)
+    return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners), steps
+
+
+def config3(seed=3, n=256, steps=4000, boundary=FDTD_BOUNDARY_RASTERIZE, h=0.0025, pml=16, n_obj=5,
+            n_listeners=128, axis_lo=0.01, axis_hi=0.04):
+    """Five rigid ellipsoids (semi-axes U(0.01, 0.04) m, random orientation,
+    non-overlapping) each with its own impulse at a seeded onset in [0, 2 ms]."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml, boundary)
+    objs = _place_ellipsoids(rng, s, n_obj, axis_lo, axis_hi)
+    srcs = [dict(object=i, amplitude=1e3, width=1e-4, onset=float(rng.uniform(0, 2e-3))) for i in range(n_obj)]
+    listeners = _random_listeners(rng, s, objs, n_listeners)
+    return SceneBuffers(s, objects=objs, impulse_sources=srcs, listeners=listeners), steps
+
+
+def _small_object_scene(rng, div, h, pml, n_obj, n_listeners, boundary=FDTD_BOUNDARY_RASTERIZE,
+                        sphere_or_ellipsoid=True):
+    s = base_settings(div, h, pml, boundary)
+    lo, hi = interior_bounds(s, 2)
+    half = float(np.min(hi - lo)) / 2.0
+    amax = min(0.04, half / 4.0)
+    amin = min(0.01, 0.5 * amax)
+    objs = []
+    if sphere_or_ellipsoid and n_obj == 1 and rng.random() < 0.5:
+        r = float(rng.uniform(amin, amax))
+        c = rng.uniform(lo + r, hi - r)
+        objs.append(dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(r, r, r), rotation=np.eye(3)))
+    else:
+        objs = _place_ellipsoids(rng, s, n_obj, amin, amax)
+    srcs = [dict(object=i, amplitude=1e3, width=1e-4, onset=float(rng.uniform(0, 1e-3))) for i in range(n_obj)]
+    listeners = _random_listeners(rng, s, objs, n_listeners)
+    return SceneBuffers(s, objects=objs, impulse_sources=srcs, listeners=listeners)
+
+
+def config4(seed=4, n_scenes=64, edge_lo=32, edge_hi=128, steps_lo=1000, steps_hi=3000):
+    """Batch: independent scenes with per-axis edges U{edge_lo..edge_hi},
+    h = 5 mm, 8-cell PML, one sphere or ellipsoid with an impulse, 16 listeners."""
+    rng = np.random.default_rng(seed)
+    scenes, steps = [], []
+    for _ in range(n_scenes):
+        div = tuple(int(x) for x in rng.integers(edge_lo, edge_hi + 1, size=3))
+        scenes.append(_small_object_scene(rng, div, 0.005, 8, 1, 16))
+        steps.append(int(rng.integers(steps_lo, steps_hi + 1)))
+    return scenes, steps
+
+
+def config5_stream(seed=5, n_requests=64, rate=200.0, edge_lo=48, edge_hi=192, dur_lo=0.01, dur_hi=0.05):
+    """Serving: Poisson arrivals (rate req/s) of impact-sound requests with
+    1-3 ellipsoids, 32 listeners, simulated duration U(dur_lo, dur_hi) s."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    h = 0.005
+    dt = cfl_dt(h)
+    for _ in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        div = tuple(int(x) for x in rng.integers(edge_lo, edge_hi + 1, size=3))
+        n_obj = int(rng.integers(1, 4))
+        sc = _small_object_scene(rng, div, h, 8, n_obj, 32, sphere_or_ellipsoid=False)
+        dur = float(rng.uniform(dur_lo, dur_hi))
+        out.append((t, sc, int(math.ceil(dur / dt))))
+    return out

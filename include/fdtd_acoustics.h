@@ -1,0 +1,267 @@
+/*
+ * fdtd_acoustics.h -- C-ABI of the MI355X FDTD acoustic wave engine.
+ *
+ * This is the drop-in boundary for the time-stepping path of the wavesolver
+ * (MAC_Grid / PML_WaveSolver / FDTD_AcousticSimulator).  Plain C types only:
+ * pointers, sizes and PODs.  All device memory is owned by the handles; all
+ * host output buffers are owned by the caller.
+ *
+ * Reference interfaces each group replaces (paths relative to the reference
+ * repository root):
+ *   fdtd_settings        <- PML_WaveSolver_Settings
+ *                           (src/wavesolver/PML_WaveSolver_Settings.h:17-99)
+ *   fdtd_object          <- FDTD_RigidObject geometry queries
+ *                           (src/wavesolver/FDTD_RigidObject.h:135-147); analytic
+ *                           sphere / ellipsoid instead of a mesh + distance field
+ *   fdtd_impulse_source  <- VibrationalSource::Evaluate
+ *                           (src/wavesolver/VibrationalSource.h:17-37); new
+ *                           Gaussian-impulse normal acceleration
+ *   fdtd_pressure_source <- GaussianPressureSource
+ *                           (src/wavesolver/GaussianPressureSource.h:31-39)
+ *   fdtd_create          <- PML_WaveSolver(bbox, settings, objects)
+ *                           (src/wavesolver/PML_WaveSolver.cpp:78-100) and
+ *                           FDTD_AcousticSimulator::InitializeSolver
+ *                           (src/wavesolver/FDTD_AcousticSimulator.cpp:435-453);
+ *                           opaque-handle style of wave_sim_init
+ *                           (src/wavesolver/gpusolver/cuda/cuda_PAT_wave_3d.h:11-53)
+ *   fdtd_create_from_xml <- FDTD_AcousticSimulator(configFile) + InitializeSolver()
+ *                           (src/wavesolver/FDTD_AcousticSimulator.cpp:419-431)
+ *   fdtd_step            <- FDTD_AcousticSimulator::RunForSteps
+ *                           (src/wavesolver/FDTD_AcousticSimulator.cpp:528-544)
+ *                           -> PML_WaveSolver::stepSystem / stepCollocated
+ *                           (src/wavesolver/PML_WaveSolver.cpp:579-609, 775-826)
+ *   fdtd_listen          <- PML_WaveSolver::FetchPressureData
+ *                           (src/wavesolver/PML_WaveSolver.cpp:319-373)
+ *   fdtd_read_listener_series
+ *                        <- AudioOutput::AccumulateBuffer/WriteAndResetBuffer
+ *                           (src/wavesolver/AudioOutput.cpp:26-64): FP64 [step][L]
+ *   fdtd_read_pressure   <- PML_WaveSolver::vertexPressure / pressure field
+ *                           (src/wavesolver/PML_WaveSolver.cpp:641-651)
+ *   fdtd_destroy         <- wave_sim_free
+ *                           (src/wavesolver/gpusolver/cuda/cuda_PAT_wave_3d.h:11-53)
+ *
+ * Errors are status codes; nothing here calls exit() (the reference's CUDA
+ * helpers do, src/wavesolver/gpusolver/cuda/cuda_helper.h:4-12) and nothing
+ * throws across this boundary.
+ */
+#ifndef FDTD_ACOUSTICS_H
+#define FDTD_ACOUSTICS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* status codes                                                              */
+/* ------------------------------------------------------------------------- */
+typedef enum fdtd_status {
+    FDTD_OK              = 0,
+    FDTD_ERR_INVALID     = 1,  /* bad argument or scene                     */
+    FDTD_ERR_NO_DEVICE   = 2,  /* no usable gfx950 device                   */
+    FDTD_ERR_DEVICE      = 3,  /* HIP runtime error                         */
+    FDTD_ERR_OOM         = 4,  /* device allocation failed / over budget    */
+    FDTD_ERR_PARSE       = 5,  /* XML scene could not be parsed             */
+    FDTD_ERR_UNSUPPORTED = 6,  /* scene uses a feature this engine lacks    */
+    FDTD_ERR_NONFINITE   = 7,  /* NaN/Inf detected in a listener signal     */
+    FDTD_ERR_BUSY        = 8,  /* request not finished yet                  */
+    FDTD_ERR_CANCELLED   = 9,  /* request cancelled or evicted              */
+    FDTD_ERR_NOT_FOUND   = 10  /* unknown request id                        */
+} fdtd_status;
+
+const char *fdtd_status_string(fdtd_status s);
+
+/* ------------------------------------------------------------------------- */
+/* scene description                                                         */
+/* ------------------------------------------------------------------------- */
+enum { FDTD_BOUNDARY_RASTERIZE = 0, FDTD_BOUNDARY_FULLY_COUPLED = 1 };
+enum { FDTD_OBJ_SPHERE = 0, FDTD_OBJ_ELLIPSOID = 1 };
+
+/* Solver settings, field by field from PML_WaveSolver_Settings and the XML
+ * <solver> / <solver_control_policy type="static"> attributes
+ * (src/parser/ImpulseResponseParser.cpp:492-641). */
+typedef struct fdtd_settings {
+    double sound_speed;               /* sound_speed (m/s), default 343        */
+    double density;                   /* density (kg/m^3)                      */
+    double cell_size;                 /* cellsize h (m)                        */
+    double time_step;                 /* 1 / timestepfrequency (s)             */
+    double stop_time;                 /* stop_time (s); <= 0: no limit         */
+    double pml_width;                 /* PML_width in cells                    */
+    double pml_strength;              /* PML_strength                          */
+    double air_viscosity_alpha;       /* air_viscosity_alpha (0: lossless)     */
+    double start_time;                /* fast_forward_to_event_time            */
+    int    boundary_condition_preset; /* 0 no wall, 1, 2 (MAC_Grid::InsidePML) */
+    int    boundary_handling;         /* FDTD_BOUNDARY_*                       */
+    int    use_mesh;                  /* must be 1: builds the PML lists       */
+    int    divisions[3];              /* pressure cells per axis               */
+    double min_bound[3];              /* minimum corner of the pressure grid   */
+} fdtd_settings;
+
+/* Rigid object with an analytic signed distance.  The rotation maps object
+ * coordinates to world coordinates (row-major 3x3, orthonormal). */
+typedef struct fdtd_object {
+    int    type;          /* FDTD_OBJ_SPHERE | FDTD_OBJ_ELLIPSOID        */
+    int    reserved;
+    double center[3];     /* world position of the centre (m)            */
+    double semi_axes[3];  /* sphere: radius in semi_axes[0]              */
+    double rotation[9];   /* world-from-object rotation, row-major       */
+} fdtd_object;
+
+/* Normal boundary acceleration a_n(t) = A exp(-(t - t_on)^2 / (2 w^2)) on the
+ * whole surface of one object; the acceleration vector is a_n * outward
+ * normal at the boundary point. */
+typedef struct fdtd_impulse_source {
+    int    object;        /* index into fdtd_scene.objects               */
+    int    reserved;
+    double amplitude;     /* A (m/s^2)                                   */
+    double width;         /* w (s)                                       */
+    double onset;         /* t_on (s)                                    */
+} fdtd_impulse_source;
+
+/* GaussianPressureSource: exp(-|x-x0|^2/(2 ws^2)) exp(-((t-t0)/wt)^2/2) K/3.6836,
+ * zero outside the open box |x - x0|_inf < 2 ws. */
+typedef struct fdtd_pressure_source {
+    double position[3];
+    double width_time;          /* source_width_time wt                  */
+    double width_space;         /* source_width_space ws (default c*wt)  */
+    double offset_time;         /* source_offset_time t0                 */
+    double normalize_constant;  /* K (default 1/(sqrt(2 pi) ws)^3)       */
+} fdtd_pressure_source;
+
+typedef struct fdtd_scene {
+    fdtd_settings                settings;
+    int                          n_objects;
+    const fdtd_object           *objects;
+    int                          n_impulse_sources;
+    const fdtd_impulse_source   *impulse_sources;
+    int                          n_pressure_sources;
+    const fdtd_pressure_source  *pressure_sources;
+    int                          n_listeners;
+    const double                *listeners;          /* [n_listeners][3] */
+} fdtd_scene;
+
+/* Fills every field of a settings block with the parser defaults
+ * (src/parser/ImpulseResponseParser.cpp:506-556). */
+void fdtd_settings_default(fdtd_settings *s);
+
+/* Parsed XML scene (owns the arrays that fdtd_parsed_scene points at). */
+typedef struct fdtd_parsed fdtd_parsed;
+fdtd_status        fdtd_parse_xml(const char *path, fdtd_parsed **out);
+const fdtd_scene  *fdtd_parsed_scene(const fdtd_parsed *p);
+int                fdtd_parsed_num_steps(const fdtd_parsed *p); /* number_of_timesteps */
+void               fdtd_parsed_free(fdtd_parsed *p);
+/* Error text of the last failed call on this thread (parse errors name the
+ * element and attribute). */
+const char        *fdtd_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* single scene                                                              */
+/* ------------------------------------------------------------------------- */
+typedef struct fdtd_handle fdtd_handle;
+
+typedef struct fdtd_options {
+    void *stream;            /* hipStream_t to run on; NULL: handle-owned stream */
+    int   device;            /* HIP device ordinal (-1: current)               */
+    int   steps_per_graph;   /* >0: capture blocks of this many steps in a hipGraph */
+    int   series_capacity;   /* device rows of the listener series (0: 1024)   */
+    int   tile_config;       /* kernel tile id from the autotuner (-1: default) */
+} fdtd_options;
+
+void fdtd_options_default(fdtd_options *o);
+
+typedef struct fdtd_info {
+    int64_t n_cells;           /* N = nx*ny*nz                                 */
+    int64_t n_pml_cells;       /* cells inside the PML                         */
+    int64_t n_ghost_cells;     /* ghost-cell entries (owner, fluid neighbour)  */
+    int64_t n_solid_cells;     /* cells occupied by objects                    */
+    int64_t n_listeners;
+    int64_t step_index;        /* steps taken since fdtd_create                */
+    double  time;              /* simulation time (s)                          */
+    double  bytes_per_step;    /* algorithmic HBM bytes: 12 N + 44 N_pml      */
+    int64_t device_bytes;      /* device memory held by the handle             */
+} fdtd_info;
+
+fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt, fdtd_handle **out);
+fdtd_status fdtd_create_from_xml(const char *path, const fdtd_options *opt, fdtd_handle **out);
+fdtd_status fdtd_destroy(fdtd_handle *h);
+/* Advances nsteps time steps.  Asynchronous on the handle's stream. */
+fdtd_status fdtd_step(fdtd_handle *h, int nsteps);
+fdtd_status fdtd_synchronize(fdtd_handle *h);
+/* Listener pressures after the newest step (FP64, n_listeners values). */
+fdtd_status fdtd_listen(fdtd_handle *h, double *out);
+/* Listener time series rows [first_step, first_step + nsteps): FP64 [nsteps][L].
+ * Row s holds the listener pressures after step s + 1. */
+fdtd_status fdtd_read_listener_series(fdtd_handle *h, int64_t first_step, int64_t nsteps, double *out);
+/* Current pressure field (FP64 copy of the FP32 field, z fastest). */
+fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out);
+fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *out);
+/* Mean device duration (ms) of the stencil kernel over the last fdtd_step
+ * call, measured with HIP events on the handle's stream (0 if unmeasured). */
+fdtd_status fdtd_set_timing(fdtd_handle *h, int enable);
+fdtd_status fdtd_last_kernel_ms(fdtd_handle *h, double *stencil_ms, double *step_ms);
+
+/* ------------------------------------------------------------------------- */
+/* batches of independent scenes (one launch per step for the whole batch)    */
+/* ------------------------------------------------------------------------- */
+typedef struct fdtd_batch fdtd_batch;
+
+typedef struct fdtd_batch_options {
+    void *stream;            /* NULL: batch-owned stream                          */
+    int   steps_per_graph;   /* hipGraph step block (0: 32)                       */
+    int   tile_config;       /* -1: autotune once, cached per process             */
+    int   persistent_groups; /* 0: 2 * #CUs workgroups                            */
+} fdtd_batch_options;
+
+void fdtd_batch_options_default(fdtd_batch_options *o);
+fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, int n_scenes,
+                              const fdtd_batch_options *opt, fdtd_batch **out);
+/* Runs step blocks until every scene has taken its num_steps steps. */
+fdtd_status fdtd_batch_run(fdtd_batch *b);
+/* Advances every unfinished scene by up to nsteps steps. */
+fdtd_status fdtd_batch_step(fdtd_batch *b, int nsteps);
+fdtd_status fdtd_batch_steps_done(fdtd_batch *b, int scene, int64_t *steps);
+fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *out); /* [num_steps][L] */
+fdtd_status fdtd_batch_destroy(fdtd_batch *b);
+
+/* ------------------------------------------------------------------------- */
+/* serving: continuous batching of impact-sound solve requests                */
+/* ------------------------------------------------------------------------- */
+typedef struct fdtd_server fdtd_server;
+
+typedef struct fdtd_server_options {
+    int64_t hbm_budget_bytes;  /* admission budget for resident request state     */
+    int     max_active;        /* slot cap (0: no cap beyond the budget)          */
+    int     steps_per_block;   /* steps advanced per batched launch block (0: 64) */
+    int     device;            /* -1: current                                     */
+} fdtd_server_options;
+
+typedef struct fdtd_server_stats {
+    int64_t submitted, completed, failed, cancelled, active, queued;
+    int64_t blocks_run;
+    int64_t peak_resident_bytes;
+} fdtd_server_stats;
+
+void fdtd_server_options_default(fdtd_server_options *o);
+fdtd_status fdtd_server_create(const fdtd_server_options *opt, fdtd_server **out);
+/* Queues a request (thread-safe).  The scene is copied. */
+fdtd_status fdtd_server_submit(fdtd_server *s, const fdtd_scene *scene, int num_steps, int64_t *request_id);
+/* FDTD_OK when finished (result ready), FDTD_ERR_BUSY while queued/running,
+ * or the request's error status. */
+fdtd_status fdtd_server_poll(fdtd_server *s, int64_t request_id);
+/* Blocks until the request finishes, or timeout_ms passes (<0: no timeout). */
+fdtd_status fdtd_server_wait(fdtd_server *s, int64_t request_id, int timeout_ms);
+/* Copies the FP64 [num_steps][L] listener series of a finished request and
+ * releases it; elapsed_ms gets submit-to-finish wall time. */
+fdtd_status fdtd_server_take_result(fdtd_server *s, int64_t request_id, double *out, double *elapsed_ms);
+fdtd_status fdtd_server_cancel(fdtd_server *s, int64_t request_id);
+fdtd_status fdtd_server_get_stats(fdtd_server *s, fdtd_server_stats *out);
+/* Bytes of device state a scene needs when admitted (used by admission). */
+int64_t     fdtd_scene_device_bytes(const fdtd_scene *scene);
+fdtd_status fdtd_server_destroy(fdtd_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTD_ACOUSTICS_H */

@@ -1,0 +1,1271 @@
+/*
+ * fdtd_oracle.c -- FP64 CPU restatement of the reference's collocated
+ * time-stepping path.  TEST INFRASTRUCTURE ONLY (the checker, never the
+ * product); see fdtd_oracle.h for the list of reference lines it follows.
+ *
+ * Parity status of each part is recorded beside the function; every part is
+ * pinned by a test in tests/test_oracle_*.py (closed forms, invariants, the
+ * reference's own known-answer inputs) unless it says "parity unpinned".
+ */
+#define _GNU_SOURCE
+#include "fdtd_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static __thread char g_err[512];
+static void set_err(const char *msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+const char *oracle_error(void) { return g_err; }
+
+/* Wavesolver_ConstantsAndTypes.h:72-98, config.h:41-46 */
+#define SMALL_PML_BUFFER 1e-12        /* InsidePML detection buffer, MAC_Grid.cpp:4031 */
+#define GAUSSIAN_CHECK_BOUND 2.0
+#define CONDN_LIM 25.0
+#define REF_EPS 1e-10                 /* EPS in config.h */
+#define MLS_RADIUS 0.018              /* MLSConstantsAndTypes.h:8 */
+
+/* ======================================================================== */
+/* small vector helpers                                                      */
+/* ======================================================================== */
+static inline double dot3(const double *a, const double *b) { return a[0]*b[0] + a[1]*b[1] + a[2]*b[2]; }
+static inline double norm3(const double *a) { return sqrt(dot3(a, a)); }
+
+/* ======================================================================== */
+/* dense linear algebra: one-sided Jacobi SVD (Hestenes), pseudo-inverse     */
+/* solve with Eigen 3.3 JacobiSVD's rank threshold                           */
+/* ======================================================================== */
+/* A is m x n row-major, m >= n.  On return U (m x n, row-major, columns are
+ * left singular vectors scaled out), sv[n] (unsorted), V (n x n row-major). */
+static void jacobi_svd(int m, int n, const double *A, double *U, double *sv, double *V)
+{
+    memcpy(U, A, sizeof(double) * m * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) V[i*n + j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double alpha = 0, beta = 0, gamma = 0;
+                for (int i = 0; i < m; ++i) {
+                    const double up = U[i*n + p], uq = U[i*n + q];
+                    alpha += up * up; beta += uq * uq; gamma += up * uq;
+                }
+                if (gamma == 0.0) continue;
+                const double conv = fabs(gamma) / sqrt(alpha * beta);
+                if (!(conv > 1e-300)) continue;
+                if (conv > off) off = conv;
+                if (conv < 1e-17) continue;
+                const double zeta = (beta - alpha) / (2.0 * gamma);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                for (int i = 0; i < m; ++i) {
+                    const double up = U[i*n + p], uq = U[i*n + q];
+                    U[i*n + p] = c * up - s * uq;
+                    U[i*n + q] = s * up + c * uq;
+                }
+                for (int i = 0; i < n; ++i) {
+                    const double vp = V[i*n + p], vq = V[i*n + q];
+                    V[i*n + p] = c * vp - s * vq;
+                    V[i*n + q] = s * vp + c * vq;
+                }
+            }
+        if (off < 1e-16) break;
+    }
+    for (int j = 0; j < n; ++j) {
+        double s2 = 0;
+        for (int i = 0; i < m; ++i) s2 += U[i*n + j] * U[i*n + j];
+        sv[j] = sqrt(s2);
+        if (sv[j] > 0)
+            for (int i = 0; i < m; ++i) U[i*n + j] /= sv[j];
+    }
+}
+
+int oracle_svd_values(int m, int n, const double *A, double *sv)
+{
+    if (m < n || n <= 0 || n > 16 || m > 256) return -1;
+    double U[256 * 16], V[16 * 16], s[16];
+    jacobi_svd(m, n, A, U, s, V);
+    /* sort decreasing */
+    for (int i = 0; i < n; ++i) sv[i] = s[i];
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+            if (sv[j] > sv[i]) { double t = sv[i]; sv[i] = sv[j]; sv[j] = t; }
+    return 0;
+}
+
+/* x = pinv(A) b with Eigen's threshold: rank counts sv > max(sv) * min(m,n) * eps
+ * (Eigen/src/SVD/SVDBase.h threshold(), JacobiSVD::_solve_impl). */
+static int lsq_solve_internal(int m, int n, const double *A, const double *b, double *x, double *cond_out)
+{
+    double U[256 * 16], V[16 * 16], s[16];
+    jacobi_svd(m, n, A, U, s, V);
+    double smax = 0, smin = INFINITY;
+    for (int j = 0; j < n; ++j) { if (s[j] > smax) smax = s[j]; if (s[j] < smin) smin = s[j]; }
+    if (cond_out) *cond_out = smax / smin;
+    double thr = smax * (double)(m < n ? m : n) * DBL_EPSILON;
+    if (thr < DBL_MIN) thr = DBL_MIN;
+    int rank = 0;
+    double tmp[16];
+    for (int j = 0; j < n; ++j) {
+        if (s[j] > thr) {
+            double acc = 0;
+            for (int i = 0; i < m; ++i) acc += U[i*n + j] * b[i];
+            tmp[j] = acc / s[j];
+            ++rank;
+        } else tmp[j] = 0.0;
+    }
+    for (int i = 0; i < n; ++i) {
+        double acc = 0;
+        for (int j = 0; j < n; ++j) acc += V[i*n + j] * tmp[j];
+        x[i] = acc;
+    }
+    return rank;
+}
+
+int oracle_lsq_solve(int m, int n, const double *A, const double *b, double *x)
+{
+    if (m < n || n <= 0 || n > 16 || m > 256) return -1;
+    return lsq_solve_internal(m, n, A, b, x, NULL);
+}
+
+/* ======================================================================== */
+/* MLS lookup, order 1, Fleishman weights                                    */
+/* MLSModeInterpolator.hpp:282-429 (lookup), 588-597 (distanceWeight),       */
+/* 712-738 (initializeWeights); PolyGen<3> rows [1, dx, dy, dz] with         */
+/* d = point - p.  Solve: JacobiSVD(A) min-norm, output c(0).               */
+/* ======================================================================== */
+double oracle_mls_lookup(const double *p, int npts, const double *pts, const double *vals)
+{
+    double A[256 * 4], b[256], c[4];
+    if (npts > 256) npts = 256;
+    const double r2 = MLS_RADIUS * MLS_RADIUS;
+    for (int i = 0; i < npts; ++i) {
+        const double dx = pts[3*i+0] - p[0], dy = pts[3*i+1] - p[1], dz = pts[3*i+2] - p[2];
+        const double d2 = (p[0]-pts[3*i+0])*(p[0]-pts[3*i+0]) + (p[1]-pts[3*i+1])*(p[1]-pts[3*i+1])
+                        + (p[2]-pts[3*i+2])*(p[2]-pts[3*i+2]);
+        const double w = exp(-(d2) / (2. * r2));
+        A[i*4 + 0] = 1.0 * w; A[i*4 + 1] = dx * w; A[i*4 + 2] = dy * w; A[i*4 + 3] = dz * w;
+        b[i] = vals[i] * w;
+    }
+    lsq_solve_internal(npts, 4, A, b, c, NULL);
+    return c[0];
+}
+
+/* ======================================================================== */
+/* analytic geometry (replaces FDTD_RigidObject's mesh + ADF + kd-tree)      */
+/* ======================================================================== */
+static void to_object(const fdtd_object *o, const double *x, double *y)
+{
+    const double d[3] = {x[0]-o->center[0], x[1]-o->center[1], x[2]-o->center[2]};
+    /* y = R^T d */
+    for (int i = 0; i < 3; ++i) y[i] = o->rotation[0*3+i]*d[0] + o->rotation[1*3+i]*d[1] + o->rotation[2*3+i]*d[2];
+}
+static void to_world_vec(const fdtd_object *o, const double *y, double *x)
+{
+    for (int i = 0; i < 3; ++i) x[i] = o->rotation[i*3+0]*y[0] + o->rotation[i*3+1]*y[1] + o->rotation[i*3+2]*y[2];
+}
+
+/* Eberly, "Distance from a Point to an Ellipse, an Ellipsoid, or a
+ * Hyperellipsoid" (2013): bisection on the largest root of
+ * F(s) = sum (r_i z_i / (s + r_i))^2 - 1, z_i = y_i / e_i, r_i = (e_i/e_min)^2,
+ * first-octant point, axes sorted decreasing. */
+static double robust_len2(double a, double b)
+{
+    const double m = fmax(fabs(a), fabs(b));
+    if (m == 0) return 0;
+    const double x = a / m, y = b / m;
+    return m * sqrt(x*x + y*y);
+}
+static double robust_len3(double a, double b, double c)
+{
+    const double m = fmax(fabs(a), fmax(fabs(b), fabs(c)));
+    if (m == 0) return 0;
+    const double x = a / m, y = b / m, z = c / m;
+    return m * sqrt(x*x + y*y + z*z);
+}
+static double get_root2(double r0, double z0, double z1, double g)
+{
+    const double n0 = r0 * z0;
+    double s0 = z1 - 1.0, s1 = (g < 0 ? 0.0 : robust_len2(n0, z1) - 1.0), s = 0;
+    for (int it = 0; it < 2200; ++it) {
+        s = 0.5 * (s0 + s1);
+        if (s == s0 || s == s1) break;
+        const double a = n0 / (s + r0), b = z1 / (s + 1.0);
+        g = a*a + b*b - 1.0;
+        if (g > 0) s0 = s; else if (g < 0) s1 = s; else break;
+    }
+    return s;
+}
+static double get_root3(double r0, double r1, double z0, double z1, double z2, double g)
+{
+    const double n0 = r0 * z0, n1 = r1 * z1;
+    double s0 = z2 - 1.0, s1 = (g < 0 ? 0.0 : robust_len3(n0, n1, z2) - 1.0), s = 0;
+    for (int it = 0; it < 2200; ++it) {
+        s = 0.5 * (s0 + s1);
+        if (s == s0 || s == s1) break;
+        const double a = n0 / (s + r0), b = n1 / (s + r1), c = z2 / (s + 1.0);
+        g = a*a + b*b + c*c - 1.0;
+        if (g > 0) s0 = s; else if (g < 0) s1 = s; else break;
+    }
+    return s;
+}
+/* 2D: e0 >= e1 > 0, y0, y1 >= 0 */
+static void ellipse_closest(double e0, double e1, double y0, double y1, double *x0, double *x1)
+{
+    if (y1 > 0) {
+        if (y0 > 0) {
+            const double z0 = y0 / e0, z1 = y1 / e1, g = z0*z0 + z1*z1 - 1.0;
+            if (g != 0) {
+                const double r0 = (e0 / e1) * (e0 / e1);
+                const double sb = get_root2(r0, z0, z1, g);
+                *x0 = r0 * y0 / (sb + r0); *x1 = y1 / (sb + 1.0);
+            } else { *x0 = y0; *x1 = y1; }
+        } else { *x0 = 0; *x1 = e1; }
+    } else {
+        const double numer0 = e0 * y0, denom0 = e0*e0 - e1*e1;
+        if (numer0 < denom0) {
+            const double xde0 = numer0 / denom0;
+            *x0 = e0 * xde0; *x1 = e1 * sqrt(fmax(0.0, 1.0 - xde0*xde0));
+        } else { *x0 = e0; *x1 = 0; }
+    }
+}
+/* 3D: e0 >= e1 >= e2 > 0, y >= 0 */
+static void ellipsoid_closest_sorted(const double *e, const double *y, double *x)
+{
+    if (y[2] > 0) {
+        if (y[1] > 0) {
+            if (y[0] > 0) {
+                const double z0 = y[0]/e[0], z1 = y[1]/e[1], z2 = y[2]/e[2];
+                const double g = z0*z0 + z1*z1 + z2*z2 - 1.0;
+                if (g != 0) {
+                    const double r0 = (e[0]/e[2])*(e[0]/e[2]), r1 = (e[1]/e[2])*(e[1]/e[2]);
+                    const double sb = get_root3(r0, r1, z0, z1, z2, g);
+                    x[0] = r0 * y[0] / (sb + r0); x[1] = r1 * y[1] / (sb + r1); x[2] = y[2] / (sb + 1.0);
+                } else { x[0] = y[0]; x[1] = y[1]; x[2] = y[2]; }
+            } else { x[0] = 0; ellipse_closest(e[1], e[2], y[1], y[2], &x[1], &x[2]); }
+        } else {
+            if (y[0] > 0) { x[1] = 0; ellipse_closest(e[0], e[2], y[0], y[2], &x[0], &x[2]); }
+            else { x[0] = 0; x[1] = 0; x[2] = e[2]; }
+        }
+    } else {
+        const double d0 = e[0]*e[0] - e[2]*e[2], d1 = e[1]*e[1] - e[2]*e[2];
+        const double n0 = e[0]*y[0], n1 = e[1]*y[1];
+        int done = 0;
+        if (n0 < d0 && n1 < d1) {
+            const double a = n0 / d0, b = n1 / d1, disc = 1.0 - a*a - b*b;
+            if (disc > 0) { x[0] = e[0]*a; x[1] = e[1]*b; x[2] = e[2]*sqrt(disc); done = 1; }
+        }
+        if (!done) { x[2] = 0; ellipse_closest(e[0], e[1], y[0], y[1], &x[0], &x[1]); }
+    }
+}
+
+/* Closest surface point xb, outward unit normal n at xb, signed distance
+ * (negative inside); the semantics of FDTD_RigidObject::DistanceToMesh and
+ * NormalToMesh (FDTD_RigidObject.cpp:256-327) for analytic shapes. */
+double oracle_object_closest(const fdtd_object *obj, const double *x, double *xb, double *n)
+{
+    if (obj->type == FDTD_OBJ_SPHERE) {
+        const double R = obj->semi_axes[0];
+        double d[3] = {x[0]-obj->center[0], x[1]-obj->center[1], x[2]-obj->center[2]};
+        double r = norm3(d);
+        double u[3];
+        if (r > 0) { u[0] = d[0]/r; u[1] = d[1]/r; u[2] = d[2]/r; }
+        else { u[0] = 1; u[1] = 0; u[2] = 0; }
+        for (int i = 0; i < 3; ++i) { xb[i] = obj->center[i] + R*u[i]; n[i] = u[i]; }
+        return r - R;
+    }
+    double y[3];
+    to_object(obj, x, y);
+    /* reflect into the first octant and sort the axes decreasing */
+    double sgn[3], ya[3];
+    for (int i = 0; i < 3; ++i) { sgn[i] = (y[i] < 0) ? -1.0 : 1.0; ya[i] = fabs(y[i]); }
+    int perm[3] = {0, 1, 2};
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (obj->semi_axes[perm[j]] > obj->semi_axes[perm[i]]) { int t = perm[i]; perm[i] = perm[j]; perm[j] = t; }
+    double es[3], ys[3], xs[3];
+    for (int i = 0; i < 3; ++i) { es[i] = obj->semi_axes[perm[i]]; ys[i] = ya[perm[i]]; }
+    ellipsoid_closest_sorted(es, ys, xs);
+    double xo[3];
+    for (int i = 0; i < 3; ++i) xo[perm[i]] = xs[i] * sgn[perm[i]];
+    const double f = (y[0]/obj->semi_axes[0])*(y[0]/obj->semi_axes[0]) + (y[1]/obj->semi_axes[1])*(y[1]/obj->semi_axes[1])
+                   + (y[2]/obj->semi_axes[2])*(y[2]/obj->semi_axes[2]) - 1.0;
+    const double dd[3] = {xo[0]-y[0], xo[1]-y[1], xo[2]-y[2]};
+    double dist = norm3(dd);
+    if (f < 0) dist = -dist;
+    double no[3] = {xo[0]/(obj->semi_axes[0]*obj->semi_axes[0]), xo[1]/(obj->semi_axes[1]*obj->semi_axes[1]),
+                    xo[2]/(obj->semi_axes[2]*obj->semi_axes[2])};
+    const double nn = norm3(no);
+    for (int i = 0; i < 3; ++i) no[i] /= nn;
+    double xw[3];
+    to_world_vec(obj, xo, xw);
+    for (int i = 0; i < 3; ++i) xb[i] = xw[i] + obj->center[i];
+    to_world_vec(obj, no, n);
+    return dist;
+}
+
+/* inside test: distance < DISTANCE_TOLERANCE (= 0), FDTD_Objects.cpp:35-49 */
+static int object_occupies(const fdtd_object *obj, const double *x)
+{
+    if (obj->type == FDTD_OBJ_SPHERE) {
+        const double d[3] = {x[0]-obj->center[0], x[1]-obj->center[1], x[2]-obj->center[2]};
+        return sqrt(dot3(d, d)) - obj->semi_axes[0] < 0.0;
+    }
+    double y[3];
+    to_object(obj, x, y);
+    const double f = (y[0]/obj->semi_axes[0])*(y[0]/obj->semi_axes[0]) + (y[1]/obj->semi_axes[1])*(y[1]/obj->semi_axes[1])
+                   + (y[2]/obj->semi_axes[2])*(y[2]/obj->semi_axes[2]) - 1.0;
+    return f < 0.0;
+}
+
+static double object_distance(const fdtd_object *obj, const double *x)
+{
+    double xb[3], n[3];
+    return oracle_object_closest(obj, x, xb, n);
+}
+
+/* FDTD_RigidObject::ReflectAgainstBoundary (FDTD_RigidObject.cpp:429-536):
+ * inside: erected = xb - x, image = xb + erected; outside: erected = x - xb,
+ * image = x + erected; on the surface: erected = surface normal, image = x. */
+/* oracle_reflect plus the outward surface normal at the boundary point */
+static double reflect_normal(const fdtd_object *obj, const double *x, double *image, double *xb, double *erected, double *n)
+{
+    const double dist = oracle_object_closest(obj, x, xb, n);
+    const int inside = dist < 0;
+    const double diff[3] = {xb[0]-x[0], xb[1]-x[1], xb[2]-x[2]};
+    if (norm3(diff) > REF_EPS) {
+        if (inside) for (int i = 0; i < 3; ++i) { erected[i] = xb[i] - x[i]; image[i] = xb[i] + erected[i]; }
+        else        for (int i = 0; i < 3; ++i) { erected[i] = -xb[i] + x[i]; image[i] = x[i] + erected[i]; }
+    } else {
+        for (int i = 0; i < 3; ++i) { erected[i] = n[i]; image[i] = x[i]; }
+    }
+    const double en = norm3(erected);
+    for (int i = 0; i < 3; ++i) erected[i] /= en;
+    return dist;
+}
+
+double oracle_reflect(const fdtd_object *obj, const double *x, double *image, double *xb, double *erected)
+{
+    double n[3];
+    return reflect_normal(obj, x, image, xb, erected, n);
+}
+
+/* tight world AABB of an object (stands in for GetUnionBBox) */
+static void object_aabb(const fdtd_object *o, double *mn, double *mx)
+{
+    for (int i = 0; i < 3; ++i) {
+        double ext;
+        if (o->type == FDTD_OBJ_SPHERE) ext = o->semi_axes[0];
+        else {
+            ext = 0;
+            for (int j = 0; j < 3; ++j) ext += (o->rotation[i*3+j]*o->semi_axes[j]) * (o->rotation[i*3+j]*o->semi_axes[j]);
+            ext = sqrt(ext);
+        }
+        mn[i] = o->center[i] - ext; mx[i] = o->center[i] + ext;
+    }
+}
+
+/* ======================================================================== */
+/* Vandermonde fill (MAC_Grid.cpp:4444-4510) and the fully-coupled weights   */
+/* ======================================================================== */
+static void fill_regular(const double *np, double *row)
+{
+    const double x = np[0], y = np[1], z = np[2];
+    row[0] = x*y*z; row[1] = x*y; row[2] = x*z; row[3] = y*z;
+    row[4] = x; row[5] = y; row[6] = z; row[7] = 1;
+}
+static void fill_boundary(const double *np, const double *nr, double *row)
+{
+    const double x = np[0], y = np[1], z = np[2], nx = nr[0], ny = nr[1], nz = nr[2];
+    row[0] = nx*y*z + ny*x*z + nz*x*y; row[1] = nx*y + ny*x; row[2] = nx*z + nz*x;
+    row[3] = ny*z + nz*y; row[4] = nx; row[5] = ny; row[6] = nz; row[7] = 0;
+}
+static void scaled(const double *p, const double *origin, double h, double *np)
+{
+    for (int i = 0; i < 3; ++i) { np[i] = p[i]; np[i] -= origin[i]; np[i] /= h; np[i] += 0.5; }
+}
+
+/* V rows = sample rows; solve V^T w = v (JacobiSVD of V^T), return cond. */
+double oracle_vandermonde_weights(const double *rows, int s_i, const double *bp, const double *nrm,
+                                  const double *image, const double *origin, double h, double *w)
+{
+    double V[64], VT[64], v[8], np[3];
+    for (int r = 0; r < 8; ++r) {
+        if (r == s_i) { scaled(bp, origin, h, np); fill_boundary(np, nrm, &V[r*8]); }
+        else          { scaled(&rows[3*r], origin, h, np); fill_regular(np, &V[r*8]); }
+    }
+    scaled(image, origin, h, np);
+    fill_regular(np, v);
+    for (int i = 0; i < 8; ++i) for (int j = 0; j < 8; ++j) VT[i*8 + j] = V[j*8 + i];
+    double cond;
+    lsq_solve_internal(8, 8, VT, v, w, &cond);
+    return cond;
+}
+
+/* ======================================================================== */
+/* PML profile (MAC_Grid.cpp:4027-4102) and coefficients (MAC_Grid.h:528-571)*/
+/* ======================================================================== */
+static void pressure_bbox(const fdtd_settings *s, double *bmin, double *bmax)
+{
+    for (int d = 0; d < 3; ++d) {
+        bmin[d] = s->min_bound[d];
+        bmax[d] = bmin[d];
+        bmax[d] += s->cell_size * (double)s->divisions[d];   /* ScalarField.cpp:43-46 */
+    }
+}
+
+static int inside_pml(const fdtd_settings *s, const double *x, double W)
+{
+    double bmin[3], bmax[3];
+    pressure_bbox(s, bmin, bmax);
+    const double pw = W + SMALL_PML_BUFFER;
+    for (int d = 0; d < 3; ++d) {
+        const double hMin = x[d] - bmin[d], hMax = bmax[d] - x[d];
+        switch (s->boundary_condition_preset) {
+        case 0: if (hMin <= pw || hMax <= pw) return 1; break;
+        case 1: if (hMin <= pw || (d == 1 && hMax <= pw)) return 1; break;
+        case 2: if (d == 2 && hMax <= pw) return 1; break;
+        default: break;
+        }
+    }
+    return 0;
+}
+
+static double pml_sigma(const fdtd_settings *s, const double *x, double W, int d)
+{
+    double bmin[3], bmax[3];
+    pressure_bbox(s, bmin, bmax);
+    const double S = s->pml_strength;
+    const double hMin = x[d] - bmin[d], hMax = bmax[d] - x[d];
+    const double a2 = W * W, distMin = W - hMin, distMax = W - hMax;
+    switch (s->boundary_condition_preset) {
+    case 0:
+        if (hMin <= W) return S * pow(distMin, 2) / a2;
+        else if (hMax <= W) return S * pow(distMax, 2) / a2;
+        return 0.0;
+    case 1:
+        if (hMin <= W) return S * pow(distMin, 2) / a2;
+        else if (d == 1 && hMax <= W) return S * pow(distMax, 2) / a2;
+        return 0.0;
+    case 2:
+        if (d != 2 || hMin <= W) return 0.0;
+        else if (d == 2 && hMax <= W) return S * pow(distMax, 2) / a2;
+        return 0.0;
+    default: return 0.0;
+    }
+}
+
+double oracle_pml_sigma(const fdtd_settings *s, const double *x, int dim)
+{
+    return pml_sigma(s, x, s->pml_width * s->cell_size, dim);
+}
+int oracle_inside_pml(const fdtd_settings *s, const double *x)
+{
+    return inside_pml(s, x, s->pml_width * s->cell_size);
+}
+
+static inline double f_vel_update(double sg, double dt) { double c = 1.0/dt - sg/2.0; c /= 1.0/dt + sg/2.0; return c; }
+static inline double f_grad(double sg, double dt, double h, double rho) { double c = 1.0/dt + sg/2.0; c *= rho*h; return -1.0/c; }
+static inline double f_dir(double sg, double dt) { return 1.0/dt + sg/2.0; }
+static inline double f_pupd(double sg, double dt, double D) { return (1.0/dt - sg/2.0) / D; }
+static inline double f_div(double rho, double c, double D) { return -1.0*rho*c*c / D; }
+
+/* ======================================================================== */
+/* solver state                                                               */
+/* ======================================================================== */
+typedef struct { int index; double upd[3], div[3]; int vl[3], vr[3]; } pml_pcell;
+typedef struct { int index, dim; double upd, grad; int pr, pl; } pml_vcell;
+
+typedef struct {
+    int owner, neighbour, topology, object;
+    uint64_t key;
+    double position[3];
+    double distance;
+    double pressure;
+} ghost_t;
+
+struct oracle {
+    fdtd_scene sc;
+    fdtd_object *objs;
+    fdtd_impulse_source *isrc;
+    fdtd_pressure_source *psrc;
+    double *listeners;
+    double src_bmin[64][3], src_bmax[64][3], src_pos[64][3];
+
+    int n[3];
+    long long N, Nv[3];
+    double h, W, dt, c, rho;
+    double bmin[3], bmax[3];
+
+    double *pc[3], *lap[2], *pdir[3], *v[3];
+    unsigned char *bulk, *ghostc, *pmlc, *cavity, *history, *classified;
+    int pc_ind, lap_ind;
+
+    pml_pcell *pp; long long npp;
+    pml_vcell *pv; long long npv;
+
+    ghost_t *gh; int ngh, capgh;
+    /* open-addressing hash: key -> ghost index */
+    uint64_t *hkeys; int *hvals; int hcap;
+    /* owner -> children range (ghost indices sorted by owner then key) */
+    int *child_order;
+    int reclass;
+    int n_coupled;
+
+    double t;
+    long long step;
+};
+
+static inline long long pidx(const oracle *o, int i, int j, int k) { return ((long long)i * o->n[1] + j) * o->n[2] + k; }
+static inline long long vidx(const oracle *o, int d, int i, int j, int k)
+{
+    const int ny = o->n[1] + (d == 1), nz = o->n[2] + (d == 2);
+    return ((long long)i * ny + j) * nz + k;
+}
+static inline void pidx3(const oracle *o, long long c, int *ijk)
+{
+    ijk[0] = (int)(c / ((long long)o->n[1] * o->n[2]));
+    c -= (long long)ijk[0] * o->n[1] * o->n[2];
+    ijk[1] = (int)(c / o->n[2]);
+    ijk[2] = (int)(c - (long long)ijk[1] * o->n[2]);
+}
+/* ScalarField::cellPosition (ScalarField.cpp:126-136) */
+static inline void ppos(const oracle *o, const int *ijk, double *x)
+{
+    for (int d = 0; d < 3; ++d) { x[d] = o->bmin[d]; x[d] += (0.5 + (double)ijk[d]) * o->h; }
+}
+/* velocity field d: min = pmin - h/2 e_d (MAC_Grid.cpp:103-113) */
+static inline void vpos(const oracle *o, int d, const int *ijk, double *x)
+{
+    for (int e = 0; e < 3; ++e) {
+        double mn = o->bmin[e] - (e == d ? o->h / 2.0 : 0.0);
+        x[e] = mn; x[e] += (0.5 + (double)ijk[e]) * o->h;
+    }
+}
+
+/* GhostCell::MakeKey, Szudzik pairing (MAC_Grid.h:108-114) */
+static inline uint64_t szudzik(long long a_, long long b_)
+{
+    const uint64_t a = (uint64_t)(a_ < b_ ? a_ : b_), b = (uint64_t)(a_ < b_ ? b_ : a_);
+    return (a >= b ? a*a + a + b : a + b*b);
+}
+
+static int hash_find(const oracle *o, uint64_t key)
+{
+    if (!o->hcap) return -1;
+    uint64_t hsh = key * 0x9E3779B97F4A7C15ull;
+    int pos = (int)(hsh >> 40) & (o->hcap - 1);
+    while (o->hvals[pos] >= 0) {
+        if (o->hkeys[pos] == key) return o->hvals[pos];
+        pos = (pos + 1) & (o->hcap - 1);
+    }
+    return -1;
+}
+static void hash_build(oracle *o)
+{
+    int cap = 16;
+    while (cap < 2 * o->ngh + 16) cap <<= 1;
+    if (cap != o->hcap) {
+        free(o->hkeys); free(o->hvals);
+        o->hkeys = (uint64_t *)malloc(sizeof(uint64_t) * cap);
+        o->hvals = (int *)malloc(sizeof(int) * cap);
+        o->hcap = cap;
+    }
+    for (int i = 0; i < cap; ++i) o->hvals[i] = -1;
+    for (int g = 0; g < o->ngh; ++g) {
+        uint64_t hsh = o->gh[g].key * 0x9E3779B97F4A7C15ull;
+        int pos = (int)(hsh >> 40) & (cap - 1);
+        while (o->hvals[pos] >= 0) pos = (pos + 1) & (cap - 1);
+        o->hkeys[pos] = o->gh[g].key; o->hvals[pos] = g;
+    }
+}
+
+static double ghost_pressure(const oracle *o, long long a, long long b)
+{
+    const int g = hash_find(o, szudzik(a, b));
+    if (g < 0) { fprintf(stderr, "oracle: missing ghost key\n"); abort(); }
+    return o->gh[g].pressure;
+}
+
+/* ======================================================================== */
+/* MAC_Grid::classifyCells (MAC_Grid.cpp:2397-2693): PML lists.              */
+/* The analytic-object scenes go through FDTD_Objects, never _boundaryFields,*/
+/* so every cell is bulk here and the interfacial lists stay empty.          */
+/* ======================================================================== */
+static void classify_static(oracle *o)
+{
+    const fdtd_settings *s = &o->sc.settings;
+    const double dt = o->dt, dx = o->h, c = o->c, rho = o->rho;
+    o->npp = 0; o->npv = 0;
+    for (long long ci = 0; ci < o->N; ++ci) {
+        int ijk[3]; double x[3];
+        pidx3(o, ci, ijk); ppos(o, ijk, x);
+        o->bulk[ci] = 1;
+        o->pmlc[ci] = 0;
+        if (inside_pml(s, x, o->W)) {
+            pml_pcell *pc = &o->pp[o->npp++];
+            pc->index = (int)ci;
+            for (int d = 0; d < 3; ++d) {
+                const double sg = pml_sigma(s, x, o->W, d);
+                const double D = f_dir(sg, dt);
+                pc->upd[d] = f_pupd(sg, dt, D);
+                pc->div[d] = f_div(rho, c, D);
+                int t[3] = {ijk[0], ijk[1], ijk[2]};
+                pc->vl[d] = (int)vidx(o, d, t[0], t[1], t[2]);
+                t[d] += 1;
+                pc->vr[d] = (int)vidx(o, d, t[0], t[1], t[2]);
+            }
+            o->pmlc[ci] = 1;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        const int vn[3] = {o->n[0] + (d == 0), o->n[1] + (d == 1), o->n[2] + (d == 2)};
+        for (int i = 0; i < vn[0]; ++i)
+        for (int j = 0; j < vn[1]; ++j)
+        for (int k = 0; k < vn[2]; ++k) {
+            int ijk[3] = {i, j, k};
+            if (ijk[d] == 0 || ijk[d] == vn[d] - 1) continue;  /* domain walls stay 0 */
+            double x[3];
+            vpos(o, d, ijk, x);
+            if (!inside_pml(s, x, o->W)) continue;
+            const double sg = pml_sigma(s, x, o->W, d);
+            pml_vcell *pv = &o->pv[o->npv++];
+            pv->index = (int)vidx(o, d, i, j, k);
+            pv->dim = d;
+            pv->upd = f_vel_update(sg, dt);
+            pv->grad = f_grad(sg, dt, dx, rho);
+            pv->pr = (int)pidx(o, ijk[0], ijk[1], ijk[2]);
+            ijk[d] -= 1;
+            pv->pl = (int)pidx(o, ijk[0], ijk[1], ijk[2]);
+            if (!o->pmlc[pv->pr] && !o->pmlc[pv->pl]) {
+                fprintf(stderr, "oracle: pml face with no pml neighbour\n"); abort();
+            }
+        }
+    }
+}
+
+/* ScalarField::GetIterationBox (ScalarField.cpp:288-314) */
+static void iteration_box(const oracle *o, const double *in_min, const double *in_max, int *start, int *range)
+{
+    double mn[3] = {in_min[0], in_min[1], in_min[2]}, mx[3] = {in_max[0], in_max[1], in_max[2]};
+    int inside_min = 1, inside_max = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (!(in_min[d] >= o->bmin[d] && in_min[d] <= o->bmax[d])) inside_min = 0;
+        if (!(in_max[d] >= o->bmin[d] && in_max[d] <= o->bmax[d])) inside_max = 0;
+    }
+    if (!inside_min) for (int d = 0; d < 3; ++d) mn[d] = fmax(o->bmin[d], mn[d]);
+    if (!inside_max) for (int d = 0; d < 3; ++d) mx[d] = fmin(o->bmax[d], mx[d]);
+    for (int d = 0; d < 3; ++d) {
+        start[d] = (int)floor((mn[d] - o->bmin[d]) / o->h);
+        const int end = (int)ceil((mx[d] - o->bmin[d]) / o->h);
+        range[d] = end - start[d] > 1 ? end - start[d] : 1;
+    }
+}
+
+static int occupy_by_object(const oracle *o, const double *x)
+{
+    for (int i = 0; i < o->sc.n_objects; ++i)
+        if (object_occupies(&o->objs[i], x)) return i;
+    return -1;
+}
+
+/* FDTD_Objects::LowestObjectDistance (FDTD_Objects.cpp:240-257) */
+static void lowest_object_distance(const oracle *o, const double *x, double *dist, int *id)
+{
+    *dist = DBL_MAX; *id = 0;
+    for (int i = 0; i < o->sc.n_objects; ++i) {
+        const double d = object_distance(&o->objs[i], x);
+        if (fabs(d) < fabs(*dist)) { *dist = d; *id = i; }
+    }
+}
+
+static int cmp_ghost(const void *a, const void *b)
+{
+    const ghost_t *x = (const ghost_t *)a, *y = (const ghost_t *)b;
+    if (x->owner != y->owner) return x->owner < y->owner ? -1 : 1;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return 0;
+}
+
+/* MAC_Grid::classifyCells_FAST (MAC_Grid.cpp:3455-3891) for rigid objects:
+ * bulk = all 7 samples (centre, +-h/4 per axis) outside every object; solid
+ * cells with a bulk 6-neighbour become ghost cells, one ghost entry per
+ * (solid owner, bulk neighbour) pair via MakeGhostCell (4310-4340). */
+static void classify_fast(oracle *o)
+{
+    const int nobj = o->sc.n_objects;
+    memcpy(o->history, o->bulk, (size_t)o->N);
+    memset(o->bulk, 1, (size_t)o->N);
+    memset(o->cavity, 0, (size_t)o->N);
+    memset(o->ghostc, 0, (size_t)o->N);
+    memset(o->classified, 0, (size_t)o->N);
+    o->ngh = 0;
+    if (nobj == 0) { hash_build(o); return; }
+    const double q = 0.25 * o->h;
+    long long nsolid = 0;
+    long long *solid = NULL; long long capsolid = 0;
+    for (int ob = 0; ob < nobj; ++ob) {
+        double mn[3], mx[3]; int st[3], rg[3];
+        object_aabb(&o->objs[ob], mn, mx);
+        iteration_box(o, mn, mx, st, rg);
+        for (int i = st[0]; i < st[0] + rg[0]; ++i)
+        for (int j = st[1]; j < st[1] + rg[1]; ++j)
+        for (int k = st[2]; k < st[2] + rg[2]; ++k) {
+            if (i < 0 || j < 0 || k < 0 || i >= o->n[0] || j >= o->n[1] || k >= o->n[2]) continue;
+            const long long ci = pidx(o, i, j, k);
+            if (o->classified[ci]) continue;
+            int ijk[3] = {i, j, k}; double x[3];
+            ppos(o, ijk, x);
+            int fluid = occupy_by_object(o, x) < 0;
+            for (int d = 0; d < 3 && fluid; ++d)
+                for (int sgn = -1; sgn <= 1 && fluid; sgn += 2) {
+                    double y[3] = {x[0], x[1], x[2]};
+                    y[d] += (double)sgn * q;
+                    if (occupy_by_object(o, y) >= 0) fluid = 0;
+                }
+            if (!fluid) {
+                o->bulk[ci] = 0;
+                if (nsolid == capsolid) { capsolid = capsolid ? 2*capsolid : 1024; solid = (long long *)realloc(solid, sizeof(long long)*capsolid); }
+                solid[nsolid++] = ci;
+            }
+            o->classified[ci] = 1;
+        }
+    }
+    /* ghost entries, MakeGhostCell + LowestObjectDistance */
+    for (long long s = 0; s < nsolid; ++s) {
+        const long long ci = solid[s];
+        o->pc[0][ci] = 0.0; o->pc[1][ci] = 0.0; o->pc[2][ci] = 0.0;
+        int ijk[3]; pidx3(o, ci, ijk);
+        int isg = 0;
+        for (int d = 0; d < 3; ++d)
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                int t[3] = {ijk[0], ijk[1], ijk[2]};
+                t[d] += sgn;
+                if (t[d] < 0 || t[d] >= o->n[d]) continue;
+                const long long nb = pidx(o, t[0], t[1], t[2]);
+                if (!o->bulk[nb]) continue;
+                isg = 1;
+                if (o->ngh == o->capgh) { o->capgh = o->capgh ? 2*o->capgh : 1024; o->gh = (ghost_t *)realloc(o->gh, sizeof(ghost_t)*o->capgh); }
+                ghost_t *g = &o->gh[o->ngh++];
+                g->owner = (int)ci; g->neighbour = (int)nb; g->topology = sgn * (d + 1);
+                g->pressure = 0.0;
+                double x[3]; ppos(o, ijk, x);
+                lowest_object_distance(o, x, &g->distance, &g->object);
+                g->position[0] = x[0]; g->position[1] = x[1]; g->position[2] = x[2];
+                g->position[d] += (g->topology > 0 ? 1.0 : -1.0) * 0.25 * o->h;
+                g->key = szudzik(ci, nb);
+            }
+        o->ghostc[ci] = (unsigned char)isg;
+    }
+    free(solid);
+    qsort(o->gh, o->ngh, sizeof(ghost_t), cmp_ghost);
+    hash_build(o);
+}
+
+/* boundary acceleration vector at a surface point: sum of the object's
+ * Gaussian-impulse sources times the outward normal */
+static double object_an(const oracle *o, int obj, double t)
+{
+    double a = 0.0;
+    for (int i = 0; i < o->sc.n_impulse_sources; ++i) {
+        const fdtd_impulse_source *s = &o->isrc[i];
+        if (s->object != obj) continue;
+        const double u = t - s->onset;
+        a += s->amplitude * exp(-(u * u) / (2.0 * s->width * s->width));
+    }
+    return a;
+}
+
+/* ScalarField::enclosingNeighbours (ScalarField.cpp:261-286): sorted unique */
+static int enclosing(const oracle *o, const double *p, long long *out)
+{
+    double diff[3];
+    for (int d = 0; d < 3; ++d) { diff[d] = p[d] - o->bmin[d]; diff[d] -= o->h / 2.0; }
+    int a[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+        int v0 = (int)(diff[d] / o->h);
+        if (v0 < 0) v0 = 0;
+        if (v0 > o->n[d] - 1) v0 = o->n[d] - 1;
+        a[d] = v0;
+        b[d] = diff[d] < 0.0 ? 0 : (v0 + 1 < o->n[d] - 1 ? v0 + 1 : o->n[d] - 1);
+    }
+    long long tmp[8]; int m = 0;
+    for (int x = 0; x < 2; ++x) for (int y = 0; y < 2; ++y) for (int z = 0; z < 2; ++z)
+        tmp[m++] = pidx(o, x ? b[0] : a[0], y ? b[1] : a[1], z ? b[2] : a[2]);
+    /* sort + unique */
+    for (int i = 0; i < 8; ++i) for (int j = i + 1; j < 8; ++j) if (tmp[j] < tmp[i]) { long long t = tmp[i]; tmp[i] = tmp[j]; tmp[j] = t; }
+    int u = 0;
+    for (int i = 0; i < 8; ++i) if (u == 0 || tmp[i] != out[u-1]) out[u++] = tmp[i];
+    return u;
+}
+
+static int find_children(const oracle *o, int owner, int *first)
+{
+    /* gh sorted by owner: binary search */
+    int lo = 0, hi = o->ngh;
+    while (lo < hi) { int mid = (lo + hi) / 2; if (o->gh[mid].owner < owner) lo = mid + 1; else hi = mid; }
+    *first = lo;
+    int cnt = 0;
+    while (lo + cnt < o->ngh && o->gh[lo + cnt].owner == owner) ++cnt;
+    return cnt;
+}
+
+/* ======================================================================== */
+/* MAC_Grid::PML_pressureUpdateGhostCells (MAC_Grid.cpp:1131-1441)          */
+/* ======================================================================== */
+typedef struct { int row, col; double val; } trip_t;
+
+static void sparse_matvec(int n, const int *rp, const int *ci, const double *vv, const double *x, double *y)
+{
+    for (int r = 0; r < n; ++r) {
+        double acc = 0;
+        for (int q = rp[r]; q < rp[r+1]; ++q) acc += vv[q] * x[ci[q]];
+        y[r] = acc;
+    }
+}
+
+/* BiCGSTAB with a diagonal (Jacobi) preconditioner, Eigen's defaults:
+ * tolerance eps, at most 2n iterations (Eigen/src/IterativeLinearSolvers). */
+static void bicgstab(int n, const int *rp, const int *ci, const double *vv, const double *b, double *x)
+{
+    double *dinv = (double *)malloc(sizeof(double) * n);
+    for (int r = 0; r < n; ++r) {
+        double d = 0;
+        for (int q = rp[r]; q < rp[r+1]; ++q) if (ci[q] == r) d += vv[q];
+        dinv[r] = d != 0 ? 1.0 / d : 1.0;
+    }
+    double *r = (double *)calloc(n, sizeof(double)), *r0 = (double *)calloc(n, sizeof(double));
+    double *p = (double *)calloc(n, sizeof(double)), *vq = (double *)calloc(n, sizeof(double));
+    double *y = (double *)calloc(n, sizeof(double)), *z = (double *)calloc(n, sizeof(double));
+    double *s = (double *)calloc(n, sizeof(double)), *t = (double *)calloc(n, sizeof(double));
+    double bn = 0;
+    for (int i = 0; i < n; ++i) { x[i] = dinv[i] * b[i]; bn += b[i] * b[i]; }
+    if (bn == 0) { for (int i = 0; i < n; ++i) x[i] = 0; goto done; }
+    sparse_matvec(n, rp, ci, vv, x, r);
+    for (int i = 0; i < n; ++i) { r[i] = b[i] - r[i]; r0[i] = r[i]; }
+    {
+        const double tol2 = DBL_EPSILON * DBL_EPSILON * bn;
+        double rho = 1, alpha = 1, w = 1;
+        for (int it = 0; it < 2 * n + 10; ++it) {
+            double rn = 0;
+            for (int i = 0; i < n; ++i) rn += r[i] * r[i];
+            if (rn <= tol2) break;
+            double rho_old = rho;
+            rho = 0;
+            for (int i = 0; i < n; ++i) rho += r0[i] * r[i];
+            if (fabs(rho) < 1e-300) {
+                /* restart */
+                for (int i = 0; i < n; ++i) r0[i] = r[i];
+                rho = rn; rho_old = 1; alpha = 1; w = 1;
+                for (int i = 0; i < n; ++i) { vq[i] = 0; p[i] = 0; }
+            }
+            const double beta = (rho / rho_old) * (alpha / w);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - w * vq[i]);
+            for (int i = 0; i < n; ++i) y[i] = dinv[i] * p[i];
+            sparse_matvec(n, rp, ci, vv, y, vq);
+            double r0v = 0;
+            for (int i = 0; i < n; ++i) r0v += r0[i] * vq[i];
+            alpha = rho / r0v;
+            for (int i = 0; i < n; ++i) s[i] = r[i] - alpha * vq[i];
+            for (int i = 0; i < n; ++i) z[i] = dinv[i] * s[i];
+            sparse_matvec(n, rp, ci, vv, z, t);
+            double tt = 0, ts = 0;
+            for (int i = 0; i < n; ++i) { tt += t[i] * t[i]; ts += t[i] * s[i]; }
+            w = tt > 0 ? ts / tt : 0;
+            for (int i = 0; i < n; ++i) { x[i] += alpha * y[i] + w * z[i]; r[i] = s[i] - w * t[i]; }
+        }
+    }
+done:
+    free(dinv); free(r); free(r0); free(p); free(vq); free(y); free(z); free(s); free(t);
+}
+
+static int cmp_trip(const void *a, const void *b)
+{
+    const trip_t *x = (const trip_t *)a, *y = (const trip_t *)b;
+    if (x->row != y->row) return x->row < y->row ? -1 : 1;
+    return x->col < y->col ? -1 : (x->col > y->col);
+}
+
+static void ghost_update(oracle *o, const double *p)
+{
+    const int G = o->ngh;
+    if (G == 0) return;
+    const double h = o->h, rho = o->rho, t = o->t;
+    const int coupled = o->sc.settings.boundary_handling == FDTD_BOUNDARY_FULLY_COUPLED;
+    double *rhs = (double *)malloc(sizeof(double) * G);
+    trip_t *trips = (trip_t *)malloc(sizeof(trip_t) * (size_t)G * 9);
+    int *ntr = (int *)calloc(G, sizeof(int));
+    int n_coupled = 0;
+#pragma omp parallel for schedule(static) reduction(+:n_coupled)
+    for (int g = 0; g < G; ++g) {
+        ghost_t *gc = &o->gh[g];
+        trip_t *tr = &trips[(size_t)g * 9];
+        const double p_neig = p[gc->neighbour];
+        const fdtd_object *obj = &o->objs[gc->object];
+        double image[3], bp[3], en[3], nrm_b[3];
+        const double distance = reflect_normal(obj, gc->position, image, bp, en, nrm_b);
+        const double an = object_an(o, gc->object, t);
+        double grad_p[3];
+        for (int i = 0; i < 3; ++i) { grad_p[i] = 0.0; grad_p[i] += an * nrm_b[i]; grad_p[i] *= (-rho); }
+        double cell_n[3] = {0, 0, 0};
+        cell_n[abs(gc->topology) - 1] = gc->topology > 0 ? 1.0 : -1.0;
+        const double grad_p_nr = dot3(grad_p, cell_n);
+        const double grad_p_ne = dot3(grad_p, en);
+        const double pg = p_neig - 0.75 * h * grad_p_nr;
+        double weights = (object_distance(obj, gc->position) < 0.0 ? -2.0 * fabs(distance) : -fabs(distance));
+        double weighted = grad_p_ne * weights;
+        long long nbrs[8];
+        const int nn = enclosing(o, image, nbrs);
+        int use_ii = (nn == 8) && coupled;
+        if (!use_ii) {
+            tr[0].row = g; tr[0].col = g; tr[0].val = 1.0; ntr[g] = 1;
+            rhs[g] = pg;
+            continue;
+        }
+        double origin[3] = {0, 0, 0};
+        for (int q = 0; q < 8; ++q) { int ijk[3]; double x[3]; pidx3(o, nbrs[q], ijk); ppos(o, ijk, x); for (int d = 0; d < 3; ++d) origin[d] += x[d]; }
+        for (int d = 0; d < 3; ++d) origin[d] /= 8.0;
+        double rows[24], r[8];
+        int s_i = -1, known[8], unk_g[8];
+        for (int q = 0; q < 8; ++q) { known[q] = 0; unk_g[q] = -1; r[q] = 0; }
+        for (int q = 0; q < 8; ++q) {
+            const long long ni = nbrs[q];
+            if (ni == gc->owner) {
+                rows[3*q] = bp[0]; rows[3*q+1] = bp[1]; rows[3*q+2] = bp[2];
+                r[q] = grad_p_ne * h;
+                s_i = q;
+            } else if (!o->ghostc[ni]) {
+                int ijk[3]; pidx3(o, ni, ijk); ppos(o, ijk, &rows[3*q]);
+                r[q] = p[ni];
+                known[q] = 1;
+                if (!o->bulk[ni]) { /* a solid non-ghost cell cannot be here */ }
+            } else {
+                int first; const int cnt = find_children(o, (int)ni, &first);
+                if (cnt == 0) { fprintf(stderr, "oracle: ghost without children\n"); abort(); }
+                double best = DBL_MAX; int bi = first;
+                for (int cidx = first; cidx < first + cnt; ++cidx) {
+                    const double *gp = o->gh[cidx].position;
+                    const double d2 = (gp[0]-image[0])*(gp[0]-image[0]) + (gp[1]-image[1])*(gp[1]-image[1]) + (gp[2]-image[2])*(gp[2]-image[2]);
+                    if (d2 < best) { best = d2; bi = cidx; }
+                }
+                rows[3*q] = o->gh[bi].position[0]; rows[3*q+1] = o->gh[bi].position[1]; rows[3*q+2] = o->gh[bi].position[2];
+                unk_g[q] = bi;
+            }
+        }
+        double w[8];
+        const double cond = oracle_vandermonde_weights(rows, s_i, bp, en, image, origin, h, w);
+        if (cond > CONDN_LIM && s_i != -1) {
+            for (int q = 0; q < 8; ++q) { w[q] = 0; r[q] = 0; known[q] = 0; unk_g[q] = -1; }
+            weighted = pg;
+        } else {
+            ++n_coupled;
+        }
+        double rr = -weighted;
+        int m = 0;
+        tr[m].row = g; tr[m].col = g; tr[m].val = -1.0; ++m;
+        for (int q = 0; q < 8; ++q) if (known[q]) rr -= w[q] * r[q];
+        if (s_i != -1) rr -= w[s_i] * r[s_i];
+        for (int q = 0; q < 8; ++q) if (unk_g[q] >= 0) { tr[m].row = g; tr[m].col = unk_g[q]; tr[m].val = w[q]; ++m; }
+        ntr[g] = m;
+        rhs[g] = rr;
+    }
+    o->n_coupled = n_coupled;
+    /* assemble CSR (duplicates summed, as Eigen::setFromTriplets does) */
+    int nt = 0;
+    for (int g = 0; g < G; ++g) nt += ntr[g];
+    trip_t *all = (trip_t *)malloc(sizeof(trip_t) * (nt ? nt : 1));
+    nt = 0;
+    for (int g = 0; g < G; ++g) for (int q = 0; q < ntr[g]; ++q) all[nt++] = trips[(size_t)g * 9 + q];
+    qsort(all, nt, sizeof(trip_t), cmp_trip);
+    int *rp = (int *)calloc(G + 1, sizeof(int)), *ci = (int *)malloc(sizeof(int) * (nt ? nt : 1));
+    double *vv = (double *)malloc(sizeof(double) * (nt ? nt : 1));
+    int m = 0;
+    for (int q = 0; q < nt; ++q) {
+        if (m > 0 && all[q].row == all[q-1].row && all[q].col == ci[m-1]) { vv[m-1] += all[q].val; continue; }
+        ci[m] = all[q].col; vv[m] = all[q].val; rp[all[q].row + 1] = m + 1; ++m;
+    }
+    for (int r = 1; r <= G; ++r) if (rp[r] < rp[r-1]) rp[r] = rp[r-1];
+    double *sol = (double *)malloc(sizeof(double) * G);
+    bicgstab(G, rp, ci, vv, rhs, sol);
+    for (int g = 0; g < G; ++g) o->gh[g].pressure = sol[g];
+    free(sol); free(rp); free(ci); free(vv); free(all); free(trips); free(ntr); free(rhs);
+}
+
+/* ======================================================================== */
+/* MAC_Grid::PML_velocityUpdateCollocated (MAC_Grid.cpp:461-479)             */
+/* ======================================================================== */
+static void pml_velocity(oracle *o, const double *pFull)
+{
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < o->npv; ++q) {
+        const pml_vcell *c = &o->pv[q];
+        double *v = o->v[c->dim];
+        v[c->index] *= c->upd;
+        v[c->index] += c->grad * pFull[c->pr];
+        v[c->index] -= c->grad * pFull[c->pl];
+    }
+}
+
+/* ======================================================================== */
+/* MAC_Grid::pressureFieldLaplacianGhostCell (MAC_Grid.cpp:187-299)          */
+/* ======================================================================== */
+static void laplacian_ghost(oracle *o, const double *val, double *lap)
+{
+    const double scale = 1.0 / pow(o->h, 2);
+    memset(lap, 0, sizeof(double) * o->N);
+#pragma omp parallel for schedule(static)
+    for (long long ci = 0; ci < o->N; ++ci) {
+        if (!o->bulk[ci]) continue;
+        int ijk[3]; pidx3(o, ci, ijk);
+        int surrounded = 0;
+        double acc = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            int pos[3] = {ijk[0], ijk[1], ijk[2]}, neg[3] = {ijk[0], ijk[1], ijk[2]};
+            if (ijk[d] != o->n[d] - 1) pos[d] += 1;
+            if (ijk[d] != 0) neg[d] -= 1;
+            const long long ip = pidx(o, pos[0], pos[1], pos[2]), in = pidx(o, neg[0], neg[1], neg[2]);
+            const int gp = o->ghostc[ip], gn = o->ghostc[in];
+            if (gp && gn) {
+                acc += (ghost_pressure(o, ci, ip) + ghost_pressure(o, ci, in) - 2.0 * val[ci]) / (9.0 / 16.0);
+                ++surrounded;
+            } else if (gp && !gn) {
+                acc += (ghost_pressure(o, ci, ip) + 0.75 * val[in] - (7.0 / 4.0) * val[ci]) / (21.0 / 32.0);
+            } else if (!gp && gn) {
+                acc += (0.75 * val[ip] + ghost_pressure(o, ci, in) - (7.0 / 4.0) * val[ci]) / (21.0 / 32.0);
+            } else {
+                acc += (val[ip] + val[in] - 2.0 * val[ci]);
+            }
+        }
+        o->cavity[ci] = (unsigned char)(surrounded > 1);
+        lap[ci] = acc * scale;
+    }
+}
+
+/* GaussianPressureSource::Evaluate (GaussianPressureSource.cpp:6-30) */
+static double pressure_sources(const oracle *o, const double *x, double t)
+{
+    double acc = 0;
+    for (int i = 0; i < o->sc.n_pressure_sources; ++i) {
+        const fdtd_pressure_source *s = &o->psrc[i];
+        const double *mn = o->src_bmin[i], *mx = o->src_bmax[i];
+        if (!(x[0] < mx[0] && x[0] > mn[0] && x[1] < mx[1] && x[1] > mn[1] && x[2] < mx[2] && x[2] > mn[2])) continue;
+        const double *sp = o->src_pos[i];
+        double value = (x[0]-sp[0])*(x[0]-sp[0]) + (x[1]-sp[1])*(x[1]-sp[1]) + (x[2]-sp[2])*(x[2]-sp[2]);
+        value = -value / (2.0 * s->width_space * s->width_space);
+        value = exp(value);
+        value *= exp(-pow((t - s->offset_time) / s->width_time, 2) / 2.0);
+        value *= 1. / 3.6836 * s->normalize_constant;
+        acc += value;
+    }
+    return acc;
+}
+
+/* ======================================================================== */
+/* MAC_Grid::PML_pressureUpdateCollocated (MAC_Grid.cpp:481-946), PML slab   */
+/* plus the non-PML bulk cells (the Engquist branch needs no PML, rejected)  */
+/* ======================================================================== */
+static void pressure_update(oracle *o, double *pLast, double *pCurr, double *pNext, const double *lapCurr, const double *lapLast)
+{
+    const fdtd_settings *s = &o->sc.settings;
+    const double dt = o->dt;
+    const double one_over_dx = 1.0 / o->h;
+    const double c2_k2 = pow(o->c * dt, 2);
+    const double kcalp = o->c * s->air_viscosity_alpha * dt;
+    const int visc = s->air_viscosity_alpha > 0.0;
+    for (long long q = 0; q < o->npp; ++q) {
+        const pml_pcell *c = &o->pp[q];
+        const int ci = c->index;
+        for (int d = 0; d < 3; ++d) {
+            o->pdir[d][ci] *= c->upd[d];
+            o->pdir[d][ci] += c->div[d] * o->v[d][c->vr[d]] * one_over_dx;
+            o->pdir[d][ci] -= c->div[d] * o->v[d][c->vl[d]] * one_over_dx;
+        }
+        pNext[ci] = o->pdir[0][ci] + o->pdir[1][ci] + o->pdir[2][ci];
+    }
+    const int has_src = o->sc.n_pressure_sources > 0;
+    const double tsrc = o->t + 0.5 * dt;
+#pragma omp parallel for schedule(static)
+    for (long long ci = 0; ci < o->N; ++ci) {
+        if (!o->bulk[ci] || o->pmlc[ci]) continue;
+        int ijk[3]; pidx3(o, ci, ijk);
+        if (ijk[0] == 0 || ijk[1] == 0 || ijk[2] == 0 || ijk[0] == o->n[0]-1 || ijk[1] == o->n[1]-1 || ijk[2] == o->n[2]-1)
+            continue;  /* layer 0: skipped (MAC_Grid.cpp:524-526) */
+        if (ijk[0] == 1 || ijk[1] == 1 || ijk[2] == 1 || ijk[0] == o->n[0]-2 || ijk[1] == o->n[1]-2 || ijk[2] == o->n[2]-2) {
+            fprintf(stderr, "oracle: Engquist ABC branch reached (no PML at layer 1)\n"); abort();
+        }
+        if (o->cavity[ci] && o->history[ci])
+            pNext[ci] = pLast[ci] = pCurr[ci];
+        else if (visc && o->history[ci])
+            pNext[ci] = 2.0 * pCurr[ci] - pLast[ci] - lapLast[ci] * kcalp + lapCurr[ci] * (c2_k2 + kcalp);
+        else
+            pNext[ci] = 2.0 * pCurr[ci] - pLast[ci] + lapCurr[ci] * c2_k2;
+        if (has_src) {
+            double x[3]; ppos(o, ijk, x);
+            pNext[ci] += pressure_sources(o, x, tsrc) * dt;
+        }
+    }
+}
+
+/* ======================================================================== */
+/* PML_WaveSolver::FetchScalarData (PML_WaveSolver.cpp:210-317)              */
+/* ======================================================================== */
+static void fetch_listeners(const oracle *o, const double *scalar, double *out)
+{
+    const int L = o->sc.n_listeners;
+#pragma omp parallel for schedule(static)
+    for (int li = 0; li < L; ++li) {
+        const double *p = &o->listeners[3*li];
+        long long nb[8];
+        const int nn = enclosing(o, p, nb);
+        double pts[3*256], vals[256];
+        int np = 0, kept = 0;
+        long long keptidx[8];
+        for (int q = 0; q < nn; ++q) {
+            const long long c = nb[q];
+            if (isnan(scalar[c])) continue;
+            keptidx[kept++] = c;
+            if (!o->ghostc[c]) {
+                int ijk[3]; pidx3(o, c, ijk); ppos(o, ijk, &pts[3*np]);
+                vals[np++] = scalar[c];
+            } else {
+                int first; const int cnt = find_children(o, (int)c, &first);
+                for (int g = first; g < first + cnt && np < 256; ++g) {
+                    pts[3*np] = o->gh[g].position[0]; pts[3*np+1] = o->gh[g].position[1]; pts[3*np+2] = o->gh[g].position[2];
+                    vals[np++] = o->gh[g].pressure;
+                }
+            }
+        }
+        if (kept < 1) { out[li] = NAN; continue; }
+        if (kept < 4) {
+            long long best = -1; double bd = DBL_MAX;
+            for (int q = 0; q < kept; ++q) {
+                int ijk[3]; double x[3]; pidx3(o, keptidx[q], ijk); ppos(o, ijk, x);
+                const double d2 = (x[0]-p[0])*(x[0]-p[0]) + (x[1]-p[1])*(x[1]-p[1]) + (x[2]-p[2])*(x[2]-p[2]);
+                if (d2 < bd) { bd = d2; best = keptidx[q]; }
+            }
+            out[li] = scalar[best];
+        } else {
+            out[li] = oracle_mls_lookup(p, np, pts, vals);
+        }
+    }
+}
+
+/* ======================================================================== */
+/* construction (PML_WaveSolver::Reinitialize_PML_WaveSolver, 102-146)       */
+/* ======================================================================== */
+oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
+{
+    const fdtd_settings *s = &scene->settings;
+    if (s->divisions[0] < 4 || s->divisions[1] < 4 || s->divisions[2] < 4) { set_err("grid too small"); return NULL; }
+    if (!s->use_mesh) { set_err("use_mesh=0 (Engquist boundary, no PML) is not restated"); return NULL; }
+    if (scene->n_pressure_sources > 64) { set_err("too many pressure sources"); return NULL; }
+    oracle *o = (oracle *)calloc(1, sizeof(oracle));
+    o->sc = *scene;
+    o->objs = (fdtd_object *)malloc(sizeof(fdtd_object) * (scene->n_objects + 1));
+    memcpy(o->objs, scene->objects, sizeof(fdtd_object) * scene->n_objects);
+    o->isrc = (fdtd_impulse_source *)malloc(sizeof(fdtd_impulse_source) * (scene->n_impulse_sources + 1));
+    memcpy(o->isrc, scene->impulse_sources, sizeof(fdtd_impulse_source) * scene->n_impulse_sources);
+    o->psrc = (fdtd_pressure_source *)malloc(sizeof(fdtd_pressure_source) * (scene->n_pressure_sources + 1));
+    memcpy(o->psrc, scene->pressure_sources, sizeof(fdtd_pressure_source) * scene->n_pressure_sources);
+    o->listeners = (double *)malloc(sizeof(double) * 3 * (scene->n_listeners + 1));
+    memcpy(o->listeners, scene->listeners, sizeof(double) * 3 * scene->n_listeners);
+    o->sc.objects = o->objs; o->sc.impulse_sources = o->isrc; o->sc.pressure_sources = o->psrc; o->sc.listeners = o->listeners;
+    for (int i = 0; i < scene->n_pressure_sources; ++i) {
+        /* GaussianPressureSource ctor: bbox = pos -/+ ws*2, position = bbox centroid */
+        for (int d = 0; d < 3; ++d) {
+            o->src_bmin[i][d] = o->psrc[i].position[d] - o->psrc[i].width_space * GAUSSIAN_CHECK_BOUND;
+            o->src_bmax[i][d] = o->psrc[i].position[d] + o->psrc[i].width_space * GAUSSIAN_CHECK_BOUND;
+            o->src_pos[i][d] = (o->src_bmax[i][d] + o->src_bmin[i][d]) / 2.0;
+        }
+    }
+    for (int d = 0; d < 3; ++d) o->n[d] = s->divisions[d];
+    o->N = (long long)o->n[0] * o->n[1] * o->n[2];
+    for (int d = 0; d < 3; ++d) o->Nv[d] = (long long)(o->n[0] + (d==0)) * (o->n[1] + (d==1)) * (o->n[2] + (d==2));
+    o->h = s->cell_size; o->dt = s->time_step; o->c = s->sound_speed; o->rho = s->density;
+    o->W = s->pml_width * s->cell_size;   /* MAC_Grid::setPMLBoundaryWidth */
+    pressure_bbox(s, o->bmin, o->bmax);
+    for (int r = 0; r < 3; ++r) o->pc[r] = (double *)calloc(o->N, sizeof(double));
+    for (int r = 0; r < 2; ++r) o->lap[r] = (double *)calloc(o->N, sizeof(double));
+    for (int d = 0; d < 3; ++d) o->pdir[d] = (double *)calloc(o->N, sizeof(double));
+    for (int d = 0; d < 3; ++d) o->v[d] = (double *)calloc(o->Nv[d], sizeof(double));
+    o->bulk = (unsigned char *)malloc(o->N); o->ghostc = (unsigned char *)calloc(o->N, 1);
+    o->pmlc = (unsigned char *)calloc(o->N, 1); o->cavity = (unsigned char *)calloc(o->N, 1);
+    o->history = (unsigned char *)malloc(o->N); o->classified = (unsigned char *)calloc(o->N, 1);
+    o->pp = (pml_pcell *)malloc(sizeof(pml_pcell) * o->N);
+    o->pv = (pml_vcell *)malloc(sizeof(pml_vcell) * (o->Nv[0] + o->Nv[1] + o->Nv[2]));
+    o->pc_ind = 1; o->lap_ind = 1;
+    o->t = s->start_time;
+    o->reclass = reclassify_each_step;
+    classify_static(o);
+    memset(o->history, 1, (size_t)o->N);
+    classify_fast(o);
+    memset(o->history, 1, (size_t)o->N);   /* ResetCellHistory(true) */
+    return o;
+}
+
+void oracle_free(oracle *o)
+{
+    if (!o) return;
+    for (int r = 0; r < 3; ++r) free(o->pc[r]);
+    for (int r = 0; r < 2; ++r) free(o->lap[r]);
+    for (int d = 0; d < 3; ++d) { free(o->pdir[d]); free(o->v[d]); }
+    free(o->bulk); free(o->ghostc); free(o->pmlc); free(o->cavity); free(o->history); free(o->classified);
+    free(o->pp); free(o->pv); free(o->gh); free(o->hkeys); free(o->hvals);
+    free(o->objs); free(o->isrc); free(o->psrc); free(o->listeners);
+    free(o);
+}
+
+/* PML_WaveSolver::stepCollocated (PML_WaveSolver.cpp:775-826) */
+int oracle_step(oracle *o, int nsteps, double *rows)
+{
+    for (int st = 0; st < nsteps; ++st) {
+        double *pLast = o->pc[(o->pc_ind + 2) % 3];
+        double *pCurr = o->pc[o->pc_ind];
+        double *pNext = o->pc[(o->pc_ind + 1) % 3];
+        double *lapLast = o->lap[(o->lap_ind + 1) % 2];
+        double *lapCurr = o->lap[o->lap_ind];
+        if (o->reclass) classify_fast(o);
+        else {
+            memcpy(o->history, o->bulk, (size_t)o->N);
+            memset(o->cavity, 0, (size_t)o->N);
+        }
+        ghost_update(o, pCurr);
+        pml_velocity(o, pCurr);
+        laplacian_ghost(o, pCurr, lapCurr);
+        pressure_update(o, pLast, pCurr, pNext, lapCurr, lapLast);
+        o->pc_ind = (o->pc_ind + 1) % 3;
+        o->lap_ind = (o->lap_ind + 1) % 2;
+        o->t += o->dt;
+        o->step += 1;
+        if (rows) fetch_listeners(o, o->pc[o->pc_ind], rows + (size_t)st * o->sc.n_listeners);
+    }
+    return 0;
+}
+
+void oracle_get_pressure(const oracle *o, double *out) { memcpy(out, o->pc[o->pc_ind], sizeof(double) * o->N); }
+
+void oracle_get_pml_state(const oracle *o, double *pdir, double *v)
+{
+    if (pdir) for (int d = 0; d < 3; ++d) memcpy(pdir + d * o->N, o->pdir[d], sizeof(double) * o->N);
+    if (v) {
+        long long off = 0;
+        for (int d = 0; d < 3; ++d) { memcpy(v + off, o->v[d], sizeof(double) * o->Nv[d]); off += o->Nv[d]; }
+    }
+}
+
+void oracle_counts(const oracle *o, long long *c)
+{
+    long long solid = 0, bulk = 0, cav = 0;
+    for (long long i = 0; i < o->N; ++i) { solid += !o->bulk[i]; bulk += o->bulk[i]; cav += o->cavity[i]; }
+    c[0] = o->N; c[1] = o->npp; c[2] = o->npv; c[3] = o->ngh; c[4] = solid; c[5] = bulk; c[6] = cav;
+    c[7] = o->n_coupled; c[8] = o->step;
+}
+
+double oracle_time(const oracle *o) { return o->t; }
