@@ -1,0 +1,126 @@
+// engine.h -- internal structures of the MI355X FDTD engine.
+//
+// Host side: SceneSetup is everything that is static for a scene (PML
+// coefficient tables, rasterised objects, ghost-cell stencils, listener
+// interpolation weights), precomputed once in FP64.  Device side: DeviceScene
+// holds the FP32 fields and the uploaded tables; StepParams is the POD handed
+// to the fused step kernel.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd_acoustics.h"
+
+namespace fdtd {
+
+// --------------------------------------------------------------------------
+// geometry (analytic rigid objects; stand-in for FDTD_RigidObject queries)
+// --------------------------------------------------------------------------
+struct Geom {
+    int type;
+    double c[3];
+    double a[3];
+    double R[9];  // world-from-object, row-major
+};
+Geom make_geom(const fdtd_object &o);
+// signed distance (negative inside), closest surface point, outward unit normal
+double geom_closest(const Geom &g, const double *x, double *xb, double *n);
+// strict inside test (distance < 0)
+bool geom_inside(const Geom &g, const double *x);
+void geom_aabb(const Geom &g, double *mn, double *mx);
+
+// --------------------------------------------------------------------------
+// host precompute
+// --------------------------------------------------------------------------
+enum : int { DIR_XM = 0, DIR_XP = 1, DIR_YM = 2, DIR_YP = 3, DIR_ZM = 4, DIR_ZP = 5 };
+
+struct GhostEntry {
+    int owner;        // solid cell (flat index)
+    int nbr;          // bulk neighbour (flat index)
+    int topology;     // +-1, +-2, +-3 (direction owner -> nbr)
+    int obj;          // closest object
+    double pos[3];    // owner centre + h/4 towards nbr
+    double k_raster;  // p_g = p[nbr] + k_raster * a_n(obj, t)
+    // fully-coupled row: x_g = sum_k w_k p[k] + c_acc * a_n(obj, t) + sum_u w_u x_u
+    int coupled;      // 1: coupled row, 0: rasterised value
+    double c_acc;
+};
+
+struct SpecialRec {      // bulk cell with at least one ghost neighbour
+    int32_t cell;
+    int32_t flags;       // bit0: one-cell cavity (p^{n+1} = p^n)
+    int32_t gid[6];      // ghost entry per direction (-1: regular neighbour)
+};
+
+struct SpecialBox {      // index box holding every special cell of one object
+    int32_t lo[3], hi[3];  // inclusive lower, exclusive upper
+    int32_t offset;        // into the concatenated class array
+    int32_t pad;
+};
+
+struct PSrcHost {
+    double t0, wt, amp;    // amp = K / 3.6836
+    int lo[3], hi[3];      // index range with a nonzero spatial factor
+    int goff[3];           // offsets of the 1D factor tables in gtab
+};
+
+struct ImpulseHost {
+    int obj;
+    double A, w, ton;
+};
+
+struct ListenerTerm {
+    int kind;      // 0: FP32 field cell, 1: ghost entry
+    int index;
+    double beta;
+};
+
+struct SceneSetup {
+    int n[3] = {0, 0, 0};
+    int64_t N = 0;
+    double h = 0, dt = 0, c = 0, rho = 0, t_start = 0, stop_time = 0;
+    double lam2 = 0;             // (c dt / h)^2
+    int boundary = 0;
+    // PML, separable per axis
+    std::vector<uint8_t> cpml[3];   // cell predicate, n_d entries
+    std::vector<uint8_t> fpml[3];   // face predicate, n_d + 1 entries (walls included)
+    std::vector<float> f1[3], f2[3];   // face tables (n_d + 1)
+    std::vector<float> f3[3], f5h[3];  // cell tables (n_d), f5h = f5 / h
+    int inner_lo[3], inner_hi[3];      // non-PML index box
+    int64_t n_pml = 0;
+    // sources
+    std::vector<PSrcHost> psrc;
+    std::vector<float> gtab;
+    std::vector<ImpulseHost> isrc;
+    // objects
+    std::vector<Geom> objs;
+    std::vector<uint8_t> bulk;          // full grid (host only)
+    std::vector<GhostEntry> ghosts;     // sorted by (owner, key)
+    std::vector<SpecialRec> special;
+    std::vector<SpecialBox> boxes;
+    std::vector<int32_t> cls;           // per box cell: -1 regular, -2 solid, >=0 special record
+    int64_t n_solid = 0;
+    int64_t n_cavity = 0;
+    // coupled ghost system (CSR over ghosts): known-cell terms and unknown-ghost terms
+    std::vector<int32_t> gk_ptr, gk_idx;  std::vector<double> gk_w;
+    std::vector<int32_t> gu_ptr, gu_idx;  std::vector<double> gu_w;
+    int coupled_iters = 0;
+    int64_t n_coupled = 0;
+    // listeners
+    int L = 0;
+    std::vector<double> listeners;
+    std::vector<int32_t> lst_ptr;
+    std::vector<ListenerTerm> lst_terms;
+};
+
+// Builds the full setup; returns FDTD_OK or an error with a message in err.
+fdtd_status build_setup(const fdtd_scene &scene, SceneSetup &out, std::string &err);
+// Algorithmic bytes per step: 12 N + 44 N_pml (FP32 fields).
+double algorithmic_bytes(const SceneSetup &s);
+
+// thread-local error text for fdtd_last_error
+void set_last_error(const std::string &msg);
+
+}  // namespace fdtd

@@ -1,0 +1,682 @@
+// setup.cpp -- FP64 host precompute of everything static in a scene.
+//
+// What the reference recomputes every step (classifyCells_FAST, the ghost
+// reflection and Vandermonde solves, the MLS listener fit) depends only on
+// the geometry when the objects are static, so it is done once here and the
+// per-step device work reduces to gathers and dot products:
+//   * PML: MAC_Grid::classifyCells + InsidePML + PML_absorptionCoefficient
+//     (src/wavesolver/MAC_Grid.cpp:2397-2693, 4027-4102; coefficient forms
+//     MAC_Grid.h:528-571).  sigma_d depends only on coordinate d, so the
+//     f1/f2 (faces) and f3/f5 (cells) coefficients are 1D tables per axis.
+//   * objects: classifyCells_FAST's 7-sample occupancy rule and MakeGhostCell
+//     (MAC_Grid.cpp:3455-3891, 4310-4340), LowestObjectDistance
+//     (FDTD_Objects.cpp:240-257).
+//   * ghost values: PML_pressureUpdateGhostCells (MAC_Grid.cpp:1131-1441):
+//     rasterize -> p_g = p[nbr] + k * a_n(t); fully_coupled -> a static
+//     sparse row x_g = sum w_k p_k + c * a_n(t) + sum w_u x_u.
+//   * listeners: FetchScalarData + MLS lookup (PML_WaveSolver.cpp:210-317,
+//     MLSModeInterpolator.hpp:282-429): the order-1 MLS value is linear in
+//     the sample values, c0 = sum beta_i v_i, with beta from the pseudo-inverse
+//     of the weighted design matrix.
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "engine.h"
+
+namespace fdtd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+const std::string &get_last_error() { return g_last_error; }
+
+namespace {
+
+constexpr double kPmlBuffer = 1e-12;      // MAC_Grid.cpp:4031
+constexpr double kCheckBound = 2.0;       // GAUSSIAN_CHECK_BOUND
+constexpr double kCondLimit = 25.0;       // CONDN_LIM
+constexpr double kMlsRadius = 0.018;      // MLS_FLEISHMAN_RAD
+constexpr double kRefEps = 1e-10;         // EPS, config.h
+
+// ---- small dense linear algebra (symmetric Jacobi eigen-decomposition) ----
+// A (n x n symmetric, row-major) -> eigenvalues lam, eigenvectors V (columns)
+void sym_eigen(int n, const double *A_in, double *lam, double *V)
+{
+    double A[64];
+    std::memcpy(A, A_in, sizeof(double) * n * n);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) V[i * n + j] = (i == j);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, diag = 0;
+        for (int i = 0; i < n; ++i) {
+            diag += A[i * n + i] * A[i * n + i];
+            for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
+        }
+        if (off <= 1e-34 * diag || off == 0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (apq == 0) continue;
+                const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = c * akp - s * akq;
+                    A[k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = c * apk - s * aqk;
+                    A[q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = c * vkp - s * vkq;
+                    V[k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; ++i) lam[i] = A[i * n + i];
+}
+
+// pseudo-inverse of a symmetric PSD Gram matrix G = B^T B with the rank rule
+// sigma > sigma_max * rank_scale * eps (sigma = sqrt(lambda)); returns cond.
+double gram_pinv(int n, const double *G, double rank_scale, double *Gp)
+{
+    double lam[8], V[64];
+    sym_eigen(n, G, lam, V);
+    double lmax = 0, lmin = INFINITY;
+    for (int i = 0; i < n; ++i) { lmax = std::max(lmax, lam[i]); lmin = std::min(lmin, lam[i]); }
+    const double smax = std::sqrt(std::max(lmax, 0.0));
+    double thr = smax * rank_scale * DBL_EPSILON;
+    thr = std::max(thr * thr, DBL_MIN);
+    for (int i = 0; i < n * n; ++i) Gp[i] = 0;
+    for (int k = 0; k < n; ++k) {
+        if (!(lam[k] > thr)) continue;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) Gp[i * n + j] += V[i * n + k] * V[j * n + k] / lam[k];
+    }
+    return lmin > 0 ? std::sqrt(lmax / lmin) : INFINITY;
+}
+
+// ScalarField::enclosingNeighbours (src/field/ScalarField.cpp:261-286)
+int enclosing_cells(const SceneSetup &S, const double *bmin, const double *p, int64_t *out)
+{
+    int a[3], b[3];
+    for (int d = 0; d < 3; ++d) {
+        double diff = p[d] - bmin[d];
+        diff -= S.h / 2.0;
+        int v0 = (int)(diff / S.h);
+        v0 = std::min(std::max(v0, 0), S.n[d] - 1);
+        a[d] = v0;
+        b[d] = diff < 0.0 ? 0 : std::min(v0 + 1, S.n[d] - 1);
+    }
+    int64_t tmp[8];
+    int m = 0;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int z = 0; z < 2; ++z)
+                tmp[m++] = ((int64_t)(x ? b[0] : a[0]) * S.n[1] + (y ? b[1] : a[1])) * S.n[2] + (z ? b[2] : a[2]);
+    std::sort(tmp, tmp + 8);
+    int u = 0;
+    for (int i = 0; i < 8; ++i)
+        if (u == 0 || tmp[i] != out[u - 1]) out[u++] = tmp[i];
+    return u;
+}
+
+inline uint64_t pair_key(int64_t a_, int64_t b_)
+{
+    const uint64_t a = (uint64_t)std::min(a_, b_), b = (uint64_t)std::max(a_, b_);
+    return a >= b ? a * a + a + b : a + b * b;
+}
+
+}  // namespace
+
+double algorithmic_bytes(const SceneSetup &s) { return 12.0 * (double)s.N + 44.0 * (double)s.n_pml; }
+
+fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
+{
+    const fdtd_settings &st = sc.settings;
+    for (int d = 0; d < 3; ++d) {
+        if (st.divisions[d] < 8) { err = "grid divisions must be >= 8 on every axis"; return FDTD_ERR_INVALID; }
+        S.n[d] = st.divisions[d];
+    }
+    if (!(st.cell_size > 0) || !(st.time_step > 0) || !(st.sound_speed > 0) || !(st.density > 0)) {
+        err = "cell_size, time_step, sound_speed and density must be positive";
+        return FDTD_ERR_INVALID;
+    }
+    if (!st.use_mesh) { err = "use_mesh=0 selects the Engquist boundary without a PML; this engine needs the PML"; return FDTD_ERR_UNSUPPORTED; }
+    if (st.air_viscosity_alpha != 0.0) { err = "air viscosity (air_viscosity_alpha > 0) is not implemented on the device path"; return FDTD_ERR_UNSUPPORTED; }
+    if (st.boundary_handling != FDTD_BOUNDARY_RASTERIZE && st.boundary_handling != FDTD_BOUNDARY_FULLY_COUPLED) {
+        err = "boundary_handling must be rasterize or fully_coupled";
+        return FDTD_ERR_INVALID;
+    }
+    if (sc.n_objects < 0 || sc.n_objects > 256 || sc.n_listeners < 0 || sc.n_pressure_sources < 0 || sc.n_impulse_sources < 0) {
+        err = "bad object/source/listener counts";
+        return FDTD_ERR_INVALID;
+    }
+    const int64_t N64 = (int64_t)S.n[0] * S.n[1] * S.n[2];
+    if (N64 > (int64_t)INT32_MAX - 1) { err = "grid too large for 32-bit cell indices"; return FDTD_ERR_INVALID; }
+    S.N = N64;
+    S.h = st.cell_size; S.dt = st.time_step; S.c = st.sound_speed; S.rho = st.density;
+    S.t_start = st.start_time; S.stop_time = st.stop_time;
+    S.lam2 = (S.c * S.dt / S.h) * (S.c * S.dt / S.h);
+    S.boundary = st.boundary_handling;
+    const double h = S.h, dt = S.dt, rho = S.rho, c = S.c;
+    const double W = st.pml_width * h;         // MAC_Grid::setPMLBoundaryWidth
+    const double Sg = st.pml_strength;
+    const int preset = st.boundary_condition_preset;
+
+    double bmin[3], bmax[3];
+    for (int d = 0; d < 3; ++d) { bmin[d] = st.min_bound[d]; bmax[d] = bmin[d]; bmax[d] += h * (double)S.n[d]; }
+
+    // ---- PML tables -------------------------------------------------------
+    auto inside_axis = [&](int d, double x) {
+        const double pw = W + kPmlBuffer, hMin = x - bmin[d], hMax = bmax[d] - x;
+        switch (preset) {
+        case 0: return hMin <= pw || hMax <= pw;
+        case 1: return hMin <= pw || (d == 1 && hMax <= pw);
+        case 2: return d == 2 && hMax <= pw;
+        default: return false;
+        }
+    };
+    auto sigma_axis = [&](int d, double x) {
+        const double hMin = x - bmin[d], hMax = bmax[d] - x, a2 = W * W;
+        const double dMin = W - hMin, dMax = W - hMax;
+        switch (preset) {
+        case 0:
+            if (hMin <= W) return Sg * std::pow(dMin, 2) / a2;
+            if (hMax <= W) return Sg * std::pow(dMax, 2) / a2;
+            return 0.0;
+        case 1:
+            if (hMin <= W) return Sg * std::pow(dMin, 2) / a2;
+            if (d == 1 && hMax <= W) return Sg * std::pow(dMax, 2) / a2;
+            return 0.0;
+        case 2:
+            if (d != 2 || hMin <= W) return 0.0;
+            if (hMax <= W) return Sg * std::pow(dMax, 2) / a2;
+            return 0.0;
+        default: return 0.0;
+        }
+    };
+    for (int d = 0; d < 3; ++d) {
+        const int n = S.n[d];
+        S.cpml[d].assign(n, 0); S.f3[d].assign(n, 0.f); S.f5h[d].assign(n, 0.f);
+        S.fpml[d].assign(n + 1, 0); S.f1[d].assign(n + 1, 0.f); S.f2[d].assign(n + 1, 0.f);
+        for (int i = 0; i < n; ++i) {
+            double x = bmin[d];
+            x += (0.5 + (double)i) * h;
+            S.cpml[d][i] = inside_axis(d, x);
+            const double sg = sigma_axis(d, x);
+            const double D = 1.0 / dt + sg / 2.0;
+            S.f3[d][i] = (float)((1.0 / dt - sg / 2.0) / D);
+            S.f5h[d][i] = (float)((-1.0 * rho * c * c / D) * (1.0 / h));
+        }
+        const double vmin = bmin[d] - h / 2.0;
+        for (int i = 0; i <= n; ++i) {
+            double x = vmin;
+            x += (0.5 + (double)i) * h;
+            S.fpml[d][i] = inside_axis(d, x);
+            const double sg = sigma_axis(d, x);
+            double f1 = 1.0 / dt - sg / 2.0;
+            f1 /= 1.0 / dt + sg / 2.0;
+            double f2 = 1.0 / dt + sg / 2.0;
+            f2 *= rho * h;
+            S.f1[d][i] = (float)f1;
+            S.f2[d][i] = (float)(-1.0 / f2);
+        }
+        // non-PML cells along d must form one interval
+        int lo = -1, hi = -1;
+        for (int i = 0; i < n; ++i)
+            if (!S.cpml[d][i]) { if (lo < 0) lo = i; hi = i + 1; }
+        if (lo < 0) { err = "PML covers the whole grid"; return FDTD_ERR_INVALID; }
+        for (int i = lo; i < hi; ++i)
+            if (S.cpml[d][i]) { err = "non-contiguous PML interior"; return FDTD_ERR_UNSUPPORTED; }
+        if (lo < 2 || hi > n - 2) {
+            err = "the PML must cover the two outermost cell layers on every face (boundary preset 0 with PML_width >= 2); "
+                  "the Engquist layer-1 boundary is not implemented";
+            return FDTD_ERR_UNSUPPORTED;
+        }
+        S.inner_lo[d] = lo; S.inner_hi[d] = hi;
+        // a PML face between two non-PML cells makes classifyCells throw
+        for (int i = 1; i < n; ++i)
+            if (S.fpml[d][i] && !S.cpml[d][i - 1] && !S.cpml[d][i]) {
+                err = "PML face between two interior cells (reference classifyCells throws)";
+                return FDTD_ERR_UNSUPPORTED;
+            }
+    }
+    S.n_pml = S.N - (int64_t)(S.inner_hi[0] - S.inner_lo[0]) * (S.inner_hi[1] - S.inner_lo[1]) * (S.inner_hi[2] - S.inner_lo[2]);
+
+    // ---- pressure sources (GaussianPressureSource) ------------------------
+    S.psrc.clear(); S.gtab.clear();
+    for (int q = 0; q < sc.n_pressure_sources; ++q) {
+        const fdtd_pressure_source &ps = sc.pressure_sources[q];
+        if (!(ps.width_time > 0) || !(ps.width_space > 0)) { err = "pressure source widths must be positive"; return FDTD_ERR_INVALID; }
+        PSrcHost P;
+        P.t0 = ps.offset_time; P.wt = ps.width_time; P.amp = 1. / 3.6836 * ps.normalize_constant;
+        for (int d = 0; d < 3; ++d) {
+            const double mn = ps.position[d] - ps.width_space * kCheckBound;
+            const double mx = ps.position[d] + ps.width_space * kCheckBound;
+            const double ctr = (mx + mn) / 2.0;
+            P.goff[d] = (int)S.gtab.size();
+            P.lo[d] = S.n[d]; P.hi[d] = 0;
+            for (int i = 0; i < S.n[d]; ++i) {
+                double x = bmin[d];
+                x += (0.5 + (double)i) * h;
+                double g = 0.0;
+                if (x < mx && x > mn) {
+                    g = std::exp(-((x - ctr) * (x - ctr)) / (2.0 * ps.width_space * ps.width_space));
+                    P.lo[d] = std::min(P.lo[d], i); P.hi[d] = std::max(P.hi[d], i + 1);
+                }
+                S.gtab.push_back((float)g);
+            }
+        }
+        S.psrc.push_back(P);
+    }
+    S.isrc.clear();
+    for (int q = 0; q < sc.n_impulse_sources; ++q) {
+        const fdtd_impulse_source &is = sc.impulse_sources[q];
+        if (is.object < 0 || is.object >= sc.n_objects) { err = "impulse source names a missing object"; return FDTD_ERR_INVALID; }
+        if (!(is.width > 0)) { err = "impulse width must be positive"; return FDTD_ERR_INVALID; }
+        S.isrc.push_back({is.object, is.amplitude, is.width, is.onset});
+    }
+
+    // ---- objects: rasterisation -------------------------------------------
+    S.objs.clear();
+    for (int q = 0; q < sc.n_objects; ++q) S.objs.push_back(make_geom(sc.objects[q]));
+    const int64_t N = S.N;
+    const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
+    auto flat = [&](int i, int j, int k) { return ((int64_t)i * ny + j) * nz + k; };
+    auto cpos = [&](int i, int j, int k, double *x) {
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) { x[d] = bmin[d]; x[d] += (0.5 + (double)ijk[d]) * h; }
+    };
+    S.bulk.assign(S.objs.empty() ? 0 : N, 1);
+    S.ghosts.clear(); S.special.clear(); S.boxes.clear(); S.cls.clear();
+    S.n_solid = 0; S.n_cavity = 0;
+    std::vector<int64_t> solid;
+    std::vector<std::array<int, 6>> itbox(S.objs.size());
+    if (!S.objs.empty()) {
+        std::vector<uint8_t> classified(N, 0);
+        auto occupied = [&](const double *x) {
+            for (const Geom &g : S.objs)
+                if (geom_inside(g, x)) return true;
+            return false;
+        };
+        for (size_t ob = 0; ob < S.objs.size(); ++ob) {
+            double mn[3], mx[3];
+            geom_aabb(S.objs[ob], mn, mx);
+            // ScalarField::GetIterationBox (src/field/ScalarField.cpp:288-314)
+            bool in_min = true, in_max = true;
+            for (int d = 0; d < 3; ++d) {
+                if (!(mn[d] >= bmin[d] && mn[d] <= bmax[d])) in_min = false;
+                if (!(mx[d] >= bmin[d] && mx[d] <= bmax[d])) in_max = false;
+            }
+            int st3[3], rg3[3];
+            for (int d = 0; d < 3; ++d) {
+                const double lo_ = in_min ? mn[d] : std::max(bmin[d], mn[d]);
+                const double hi_ = in_max ? mx[d] : std::min(bmax[d], mx[d]);
+                st3[d] = (int)std::floor((lo_ - bmin[d]) / h);
+                const int end = (int)std::ceil((hi_ - bmin[d]) / h);
+                rg3[d] = std::max(end - st3[d], 1);
+            }
+            for (int d = 0; d < 3; ++d) {
+                itbox[ob][d] = std::max(st3[d], 0);
+                itbox[ob][3 + d] = std::min(st3[d] + rg3[d], S.n[d]);
+            }
+            const double q4 = 0.25 * h;
+            for (int i = itbox[ob][0]; i < itbox[ob][3]; ++i)
+                for (int j = itbox[ob][1]; j < itbox[ob][4]; ++j)
+                    for (int k = itbox[ob][2]; k < itbox[ob][5]; ++k) {
+                        const int64_t ci = flat(i, j, k);
+                        if (classified[ci]) continue;
+                        classified[ci] = 1;
+                        double x[3];
+                        cpos(i, j, k, x);
+                        bool fluid = !occupied(x);
+                        for (int d = 0; d < 3 && fluid; ++d)
+                            for (int sg = -1; sg <= 1 && fluid; sg += 2) {
+                                double y[3] = {x[0], x[1], x[2]};
+                                y[d] += (double)sg * q4;
+                                if (occupied(y)) fluid = false;
+                            }
+                        if (!fluid) { S.bulk[ci] = 0; solid.push_back(ci); }
+                    }
+        }
+        std::sort(solid.begin(), solid.end());
+        S.n_solid = (int64_t)solid.size();
+        // a solid cell in the PML or on the outer layers is outside what the
+        // device path restates
+        for (int64_t ci : solid) {
+            const int i = (int)(ci / ((int64_t)ny * nz)), j = (int)((ci / nz) % ny), k = (int)(ci % nz);
+            if (S.cpml[0][i] || S.cpml[1][j] || S.cpml[2][k]) {
+                err = "an object reaches into the PML; objects must lie inside the non-PML interior";
+                return FDTD_ERR_UNSUPPORTED;
+            }
+        }
+        // ghost entries (MakeGhostCell), topology order -x,+x,-y,+y,-z,+z
+        std::vector<std::pair<uint64_t, GhostEntry>> tmp;
+        for (int64_t ci : solid) {
+            const int ijk[3] = {(int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz)};
+            for (int d = 0; d < 3; ++d)
+                for (int sg = -1; sg <= 1; sg += 2) {
+                    int t[3] = {ijk[0], ijk[1], ijk[2]};
+                    t[d] += sg;
+                    if (t[d] < 0 || t[d] >= S.n[d]) continue;
+                    const int64_t nb = flat(t[0], t[1], t[2]);
+                    if (!S.bulk[nb]) continue;
+                    GhostEntry g{};
+                    g.owner = (int)ci; g.nbr = (int)nb; g.topology = sg * (d + 1);
+                    double x[3];
+                    cpos(ijk[0], ijk[1], ijk[2], x);
+                    // LowestObjectDistance from the owner centre
+                    double best = DBL_MAX; int bo = 0;
+                    for (size_t ob = 0; ob < S.objs.size(); ++ob) {
+                        double xb[3], nn[3];
+                        const double dd = geom_closest(S.objs[ob], x, xb, nn);
+                        if (std::fabs(dd) < std::fabs(best)) { best = dd; bo = (int)ob; }
+                    }
+                    g.obj = bo;
+                    for (int e = 0; e < 3; ++e) g.pos[e] = x[e];
+                    g.pos[d] += (g.topology > 0 ? 1.0 : -1.0) * 0.25 * h;
+                    tmp.push_back({pair_key(ci, nb), g});
+                }
+        }
+        std::sort(tmp.begin(), tmp.end(), [](const auto &a, const auto &b) {
+            if (a.second.owner != b.second.owner) return a.second.owner < b.second.owner;
+            return a.first < b.first;
+        });
+        for (auto &t : tmp) S.ghosts.push_back(t.second);
+    }
+    const int G = (int)S.ghosts.size();
+    std::map<uint64_t, int> key2g;
+    std::vector<int> first_child;  // owner -> first ghost index
+    for (int g = 0; g < G; ++g) key2g[pair_key(S.ghosts[g].owner, S.ghosts[g].nbr)] = g;
+    auto children = [&](int64_t owner, int &first) {
+        auto it = std::lower_bound(S.ghosts.begin(), S.ghosts.end(), owner,
+                                   [](const GhostEntry &e, int64_t o) { return e.owner < o; });
+        first = (int)(it - S.ghosts.begin());
+        int cnt = 0;
+        while (first + cnt < G && S.ghosts[first + cnt].owner == owner) ++cnt;
+        return cnt;
+    };
+    std::vector<uint8_t> isghost(S.objs.empty() ? 0 : N, 0);
+    for (const GhostEntry &g : S.ghosts) isghost[g.owner] = 1;
+
+    // ---- ghost values (PML_pressureUpdateGhostCells) -----------------------
+    S.gk_ptr.assign(1, 0); S.gk_idx.clear(); S.gk_w.clear();
+    S.gu_ptr.assign(1, 0); S.gu_idx.clear(); S.gu_w.clear();
+    S.n_coupled = 0;
+    const bool coupled = S.boundary == FDTD_BOUNDARY_FULLY_COUPLED;
+    for (int g = 0; g < G; ++g) {
+        GhostEntry &gc = S.ghosts[g];
+        const Geom &ob = S.objs[gc.obj];
+        double bp[3], nb[3];
+        const double dist = geom_closest(ob, gc.pos, bp, nb);
+        // FDTD_RigidObject::ReflectAgainstBoundary
+        double en[3], image[3];
+        const double diff[3] = {bp[0] - gc.pos[0], bp[1] - gc.pos[1], bp[2] - gc.pos[2]};
+        if (std::sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]) > kRefEps) {
+            if (dist < 0) for (int e = 0; e < 3; ++e) { en[e] = bp[e] - gc.pos[e]; image[e] = bp[e] + en[e]; }
+            else          for (int e = 0; e < 3; ++e) { en[e] = gc.pos[e] - bp[e]; image[e] = gc.pos[e] + en[e]; }
+        } else {
+            for (int e = 0; e < 3; ++e) { en[e] = nb[e]; image[e] = gc.pos[e]; }
+        }
+        const double enn = std::sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+        for (int e = 0; e < 3; ++e) en[e] /= enn;
+        double cell_n[3] = {0, 0, 0};
+        cell_n[std::abs(gc.topology) - 1] = gc.topology > 0 ? 1.0 : -1.0;
+        const double ndc = nb[0] * cell_n[0] + nb[1] * cell_n[1] + nb[2] * cell_n[2];
+        const double nde = nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2];
+        gc.k_raster = 0.75 * h * rho * ndc;
+        gc.coupled = 0;
+        gc.c_acc = 0;
+        bool done = false;
+        if (coupled) {
+            int64_t cells[8];
+            const int ncell = enclosing_cells(S, bmin, image, cells);
+            if (ncell == 8) {
+                double origin[3] = {0, 0, 0};
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t ci = cells[q];
+                    double x[3];
+                    cpos((int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz), x);
+                    for (int e = 0; e < 3; ++e) origin[e] += x[e];
+                }
+                for (int e = 0; e < 3; ++e) origin[e] /= 8.0;
+                double V[64], v[8];
+                int s_i = -1, kcell[8], ughost[8];
+                auto scaled = [&](const double *p, double *np) {
+                    for (int e = 0; e < 3; ++e) { np[e] = p[e]; np[e] -= origin[e]; np[e] /= h; np[e] += 0.5; }
+                };
+                auto fill_reg = [&](const double *p, double *row) {
+                    double np[3]; scaled(p, np);
+                    row[0] = np[0] * np[1] * np[2]; row[1] = np[0] * np[1]; row[2] = np[0] * np[2]; row[3] = np[1] * np[2];
+                    row[4] = np[0]; row[5] = np[1]; row[6] = np[2]; row[7] = 1;
+                };
+                bool bad = false;
+                for (int q = 0; q < 8; ++q) {
+                    kcell[q] = -1; ughost[q] = -1;
+                    const int64_t ci = cells[q];
+                    if (ci == gc.owner) {
+                        double np[3]; scaled(bp, np);
+                        const double x = np[0], y = np[1], z = np[2];
+                        double *row = &V[q * 8];
+                        row[0] = en[0] * y * z + en[1] * x * z + en[2] * x * y; row[1] = en[0] * y + en[1] * x;
+                        row[2] = en[0] * z + en[2] * x; row[3] = en[1] * z + en[2] * y;
+                        row[4] = en[0]; row[5] = en[1]; row[6] = en[2]; row[7] = 0;
+                        s_i = q;
+                    } else if (!isghost[ci]) {
+                        double x[3];
+                        cpos((int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz), x);
+                        fill_reg(x, &V[q * 8]);
+                        kcell[q] = (int)ci;
+                    } else {
+                        int f0; const int cnt = children(ci, f0);
+                        if (cnt == 0) { bad = true; break; }
+                        double best = DBL_MAX; int bi = f0;
+                        for (int u = f0; u < f0 + cnt; ++u) {
+                            const double *gp = S.ghosts[u].pos;
+                            const double d2 = (gp[0] - image[0]) * (gp[0] - image[0]) + (gp[1] - image[1]) * (gp[1] - image[1]) + (gp[2] - image[2]) * (gp[2] - image[2]);
+                            if (d2 < best) { best = d2; bi = u; }
+                        }
+                        fill_reg(S.ghosts[bi].pos, &V[q * 8]);
+                        ughost[q] = bi;
+                    }
+                }
+                if (bad) { err = "ghost cell without children in a fully-coupled stencil"; return FDTD_ERR_INVALID; }
+                fill_reg(image, v);
+                // w solves V^T w = v: w = (V V^T)^+ V v; cond of V from V V^T
+                double M[64], Mp[64], Vv[8];
+                for (int a = 0; a < 8; ++a)
+                    for (int b = 0; b < 8; ++b) {
+                        double s = 0;
+                        for (int q = 0; q < 8; ++q) s += V[a * 8 + q] * V[b * 8 + q];
+                        M[a * 8 + b] = s;
+                    }
+                const double cond = gram_pinv(8, M, 8.0, Mp);
+                for (int a = 0; a < 8; ++a) {
+                    double s = 0;
+                    for (int q = 0; q < 8; ++q) s += V[a * 8 + q] * v[q];
+                    Vv[a] = s;
+                }
+                double w[8];
+                for (int a = 0; a < 8; ++a) {
+                    double s = 0;
+                    for (int b = 0; b < 8; ++b) s += Mp[a * 8 + b] * Vv[b];
+                    w[a] = s;
+                }
+                if (!(cond > kCondLimit && s_i != -1)) {
+                    const double weights = (dist < 0.0 ? -2.0 * std::fabs(dist) : -std::fabs(dist));
+                    gc.coupled = 1;
+                    gc.c_acc = -rho * nde * (weights + (s_i >= 0 ? w[s_i] * h : 0.0));
+                    for (int q = 0; q < 8; ++q) {
+                        if (kcell[q] >= 0 && S.bulk[kcell[q]]) { S.gk_idx.push_back(kcell[q]); S.gk_w.push_back(w[q]); }
+                        if (ughost[q] >= 0) { S.gu_idx.push_back(ughost[q]); S.gu_w.push_back(w[q]); }
+                    }
+                    ++S.n_coupled;
+                    done = true;
+                }
+            }
+        }
+        (void)done;
+        S.gk_ptr.push_back((int32_t)S.gk_idx.size());
+        S.gu_ptr.push_back((int32_t)S.gu_idx.size());
+    }
+    // Jacobi iteration count for the coupled rows: x <- b + W x converges when
+    // the spectral radius of W is below one; estimate it by power iteration.
+    S.coupled_iters = 0;
+    if (S.n_coupled > 0) {
+        std::vector<double> x(G, 1.0), y(G);
+        double rho_est = 0;
+        for (int it = 0; it < 200; ++it) {
+            double nrm = 0;
+            for (int g = 0; g < G; ++g) {
+                double s = 0;
+                for (int q = S.gu_ptr[g]; q < S.gu_ptr[g + 1]; ++q) s += std::fabs(S.gu_w[q]) * x[S.gu_idx[q]];
+                y[g] = s;
+                nrm = std::max(nrm, std::fabs(s));
+            }
+            if (nrm == 0) { rho_est = 0; break; }
+            rho_est = nrm / std::max(1e-300, *std::max_element(x.begin(), x.end()));
+            for (int g = 0; g < G; ++g) x[g] = y[g] / nrm;
+        }
+        if (rho_est >= 0.95) {
+            err = "fully_coupled ghost system: Jacobi iteration does not contract (spectral radius estimate " +
+                  std::to_string(rho_est) + ")";
+            return FDTD_ERR_UNSUPPORTED;
+        }
+        S.coupled_iters = rho_est <= 1e-12 ? 1 : std::min(200, (int)std::ceil(std::log(1e-17) / std::log(rho_est)) + 1);
+    }
+
+    // ---- special cells and their boxes ------------------------------------
+    if (G > 0) {
+        std::vector<int> rec_of(N, -1);  // sized to N only when objects exist
+        for (int g = 0; g < G; ++g) {
+            const GhostEntry &gc = S.ghosts[g];
+            const int c_ = gc.nbr;
+            if (rec_of[c_] < 0) {
+                SpecialRec r{};
+                r.cell = c_; r.flags = 0;
+                for (int q = 0; q < 6; ++q) r.gid[q] = -1;
+                rec_of[c_] = (int)S.special.size();
+                S.special.push_back(r);
+            }
+            // direction from the bulk cell towards the ghost owner
+            const int d = std::abs(gc.topology) - 1;
+            const int towards_owner_positive = gc.topology < 0;  // owner -> nbr negative => owner on + side
+            S.special[rec_of[c_]].gid[2 * d + (towards_owner_positive ? 1 : 0)] = g;
+        }
+        for (SpecialRec &r : S.special) {
+            int both = 0;
+            for (int d = 0; d < 3; ++d) both += (r.gid[2 * d] >= 0 && r.gid[2 * d + 1] >= 0);
+            if (both > 1) { r.flags |= 1; ++S.n_cavity; }
+        }
+        std::sort(S.special.begin(), S.special.end(), [](const SpecialRec &a, const SpecialRec &b) { return a.cell < b.cell; });
+        for (size_t r = 0; r < S.special.size(); ++r) rec_of[S.special[r].cell] = (int)r;
+        for (size_t ob = 0; ob < S.objs.size(); ++ob) {
+            SpecialBox b{};
+            for (int d = 0; d < 3; ++d) { b.lo[d] = std::max(itbox[ob][d] - 1, 0); b.hi[d] = std::min(itbox[ob][3 + d] + 1, S.n[d]); }
+            b.offset = (int32_t)S.cls.size();
+            for (int i = b.lo[0]; i < b.hi[0]; ++i)
+                for (int j = b.lo[1]; j < b.hi[1]; ++j)
+                    for (int k = b.lo[2]; k < b.hi[2]; ++k) {
+                        const int64_t ci = flat(i, j, k);
+                        S.cls.push_back(!S.bulk[ci] ? -2 : rec_of[ci]);
+                    }
+            S.boxes.push_back(b);
+        }
+        // every special cell and every solid cell must sit inside a box
+        for (int64_t ci : solid) {
+            const int ijk[3] = {(int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz)};
+            bool in = false;
+            for (const SpecialBox &b : S.boxes)
+                in |= ijk[0] >= b.lo[0] && ijk[0] < b.hi[0] && ijk[1] >= b.lo[1] && ijk[1] < b.hi[1] && ijk[2] >= b.lo[2] && ijk[2] < b.hi[2];
+            if (!in) { err = "internal: solid cell outside every object box"; return FDTD_ERR_INVALID; }
+        }
+    } else if (S.n_solid > 0) {
+        // solid cells with no bulk neighbour at all cannot happen: the
+        // object would fill the grid
+        err = "object covers the whole interior";
+        return FDTD_ERR_INVALID;
+    }
+
+    // ---- listeners: MLS weights --------------------------------------------
+    S.L = sc.n_listeners;
+    S.listeners.assign(sc.listeners, sc.listeners + 3 * (size_t)S.L);
+    S.lst_ptr.assign(1, 0); S.lst_terms.clear();
+    for (int l = 0; l < S.L; ++l) {
+        const double *p = &S.listeners[3 * l];
+        for (int d = 0; d < 3; ++d)
+            if (!(p[d] >= bmin[d] && p[d] <= bmax[d])) {
+                err = "listener " + std::to_string(l) + " lies outside the grid";
+                return FDTD_ERR_INVALID;
+            }
+        int64_t cells[8];
+        const int ne = enclosing_cells(S, bmin, p, cells);
+        struct Pt { double x[3]; int kind; int index; };
+        std::vector<Pt> pts;
+        for (int q = 0; q < ne; ++q) {
+            const int64_t ci = cells[q];
+            const bool gh = !S.objs.empty() && isghost[ci];
+            if (!gh) {
+                Pt P; cpos((int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz), P.x);
+                const bool isbulk = S.objs.empty() || S.bulk[ci];
+                P.kind = isbulk ? 0 : -1;  // solid non-ghost cells hold p = 0
+                P.index = (int)ci;
+                pts.push_back(P);
+            } else {
+                int f0; const int cnt = children(ci, f0);
+                for (int u = f0; u < f0 + cnt; ++u) {
+                    Pt P; for (int e = 0; e < 3; ++e) P.x[e] = S.ghosts[u].pos[e];
+                    P.kind = 1; P.index = u;
+                    pts.push_back(P);
+                }
+            }
+        }
+        if (ne < 4) {
+            int64_t best = -1; double bd = DBL_MAX;
+            for (int q = 0; q < ne; ++q) {
+                double x[3]; const int64_t ci = cells[q];
+                cpos((int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz), x);
+                const double d2 = (x[0] - p[0]) * (x[0] - p[0]) + (x[1] - p[1]) * (x[1] - p[1]) + (x[2] - p[2]) * (x[2] - p[2]);
+                if (d2 < bd) { bd = d2; best = ci; }
+            }
+            if (best >= 0 && (S.objs.empty() || S.bulk[best])) S.lst_terms.push_back({0, (int)best, 1.0});
+        } else {
+            const int m = (int)pts.size();
+            // weighted design matrix, columns 1..3 scaled by 1/h for conditioning
+            std::vector<double> A(4 * m), wgt(m);
+            const double r2 = kMlsRadius * kMlsRadius;
+            for (int i = 0; i < m; ++i) {
+                const double dx = pts[i].x[0] - p[0], dy = pts[i].x[1] - p[1], dz = pts[i].x[2] - p[2];
+                const double w = std::exp(-(dx * dx + dy * dy + dz * dz) / (2. * r2));
+                wgt[i] = w;
+                A[4 * i + 0] = w; A[4 * i + 1] = w * dx / h; A[4 * i + 2] = w * dy / h; A[4 * i + 3] = w * dz / h;
+            }
+            double Gm[16] = {0}, Gp[16];
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    double s = 0;
+                    for (int i = 0; i < m; ++i) s += A[4 * i + a] * A[4 * i + b];
+                    Gm[4 * a + b] = s;
+                }
+            gram_pinv(4, Gm, (double)std::min(m, 4), Gp);
+            for (int i = 0; i < m; ++i) {
+                double s = 0;
+                for (int b = 0; b < 4; ++b) s += Gp[b] * A[4 * i + b];  // row 0 of G^+ A^T
+                const double beta = s * wgt[i];
+                if (pts[i].kind >= 0) S.lst_terms.push_back({pts[i].kind, pts[i].index, beta});
+            }
+        }
+        S.lst_ptr.push_back((int32_t)S.lst_terms.size());
+    }
+    return FDTD_OK;
+}
+
+}  // namespace fdtd

@@ -1,0 +1,310 @@
+// xml.cpp -- the XML scene config of the wavesolver (ImpulseResponseParser,
+// src/parser/ImpulseResponseParser.cpp:492-751): <solver>,
+// <solver_control_policy type="static">, <pressure_source>/<gaussian_pressure_source>,
+// <listening_point_list>, plus analytic rigid objects in <scene>:
+//   <analytic_sphere center_x= center_y= center_z= radius=>
+//   <analytic_ellipsoid center_x=.. semi_axis_x=.. semi_axis_y=.. semi_axis_z=..
+//                       [quat_w= quat_x= quat_y= quat_z=]>
+// each with any number of <gaussian_impulse_source amplitude= width= onset=/>.
+// A small attribute-only XML reader replaces the vendored tinyxml.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace fdtd {
+namespace {
+
+struct Node {
+    std::string name;
+    std::map<std::string, std::string> attr;
+    std::vector<std::unique_ptr<Node>> kids;
+    std::vector<const Node *> children(const char *n) const
+    {
+        std::vector<const Node *> out;
+        for (auto &k : kids)
+            if (k->name == n) out.push_back(k.get());
+        return out;
+    }
+    const Node *child(const char *n) const
+    {
+        for (auto &k : kids)
+            if (k->name == n) return k.get();
+        return nullptr;
+    }
+};
+
+struct Reader {
+    const std::string &s;
+    size_t i = 0;
+    std::string err;
+    explicit Reader(const std::string &src) : s(src) {}
+    void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) ++i; }
+    bool starts(const char *p) const { return s.compare(i, std::strlen(p), p) == 0; }
+    void skip_misc()
+    {
+        for (;;) {
+            ws();
+            if (starts("<?")) { size_t e = s.find("?>", i); i = e == std::string::npos ? s.size() : e + 2; }
+            else if (starts("<!--")) { size_t e = s.find("-->", i); i = e == std::string::npos ? s.size() : e + 3; }
+            else if (starts("<!")) { size_t e = s.find('>', i); i = e == std::string::npos ? s.size() : e + 1; }
+            else break;
+        }
+    }
+    std::string name()
+    {
+        size_t b = i;
+        while (i < s.size() && (std::isalnum((unsigned char)s[i]) || s[i] == '_' || s[i] == '-' || s[i] == ':' || s[i] == '.')) ++i;
+        return s.substr(b, i - b);
+    }
+    std::unique_ptr<Node> element()
+    {
+        skip_misc();
+        if (i >= s.size() || s[i] != '<') { err = "expected '<'"; return nullptr; }
+        ++i;
+        auto n = std::make_unique<Node>();
+        n->name = name();
+        if (n->name.empty()) { err = "empty element name"; return nullptr; }
+        for (;;) {
+            ws();
+            if (starts("/>")) { i += 2; return n; }
+            if (starts(">")) { ++i; break; }
+            std::string a = name();
+            if (a.empty()) { err = "bad attribute in <" + n->name + ">"; return nullptr; }
+            ws();
+            if (i >= s.size() || s[i] != '=') { err = "missing '=' after " + a; return nullptr; }
+            ++i;
+            ws();
+            if (i >= s.size() || (s[i] != '"' && s[i] != '\'')) { err = "unquoted value of " + a; return nullptr; }
+            const char q = s[i++];
+            size_t e = s.find(q, i);
+            if (e == std::string::npos) { err = "unterminated value of " + a; return nullptr; }
+            n->attr[a] = s.substr(i, e - i);
+            i = e + 1;
+        }
+        for (;;) {
+            // text content is ignored
+            while (i < s.size() && s[i] != '<') ++i;
+            if (i >= s.size()) { err = "unterminated <" + n->name + ">"; return nullptr; }
+            if (starts("<!--")) { size_t e = s.find("-->", i); i = e == std::string::npos ? s.size() : e + 3; continue; }
+            if (starts("</")) {
+                i += 2;
+                std::string cn = name();
+                ws();
+                if (cn != n->name || i >= s.size() || s[i] != '>') { err = "mismatched </" + cn + "> for <" + n->name + ">"; return nullptr; }
+                ++i;
+                return n;
+            }
+            auto k = element();
+            if (!k) return nullptr;
+            n->kids.push_back(std::move(k));
+        }
+    }
+};
+
+struct ParseError {
+    std::string msg;
+};
+
+const std::string *find_attr(const Node *n, const char *a)
+{
+    auto it = n->attr.find(a);
+    return it == n->attr.end() ? nullptr : &it->second;
+}
+double to_real(const Node *n, const char *a, const std::string &v)
+{
+    char *end = nullptr;
+    const double x = std::strtod(v.c_str(), &end);
+    if (end == v.c_str()) throw ParseError{"<" + n->name + "> attribute " + a + " is not a number: " + v};
+    return x;
+}
+double req_real(const Node *n, const char *a)
+{
+    const std::string *v = find_attr(n, a);
+    if (!v) throw ParseError{"<" + n->name + "> is missing required attribute " + a};
+    return to_real(n, a, *v);
+}
+double opt_real(const Node *n, const char *a, double def)
+{
+    const std::string *v = find_attr(n, a);
+    return v ? to_real(n, a, *v) : def;
+}
+int req_int(const Node *n, const char *a) { return (int)std::lround(req_real(n, a)); }
+int opt_int(const Node *n, const char *a, int def) { return (int)std::lround(opt_real(n, a, def)); }
+std::string opt_str(const Node *n, const char *a, const char *def)
+{
+    const std::string *v = find_attr(n, a);
+    return v ? *v : std::string(def);
+}
+
+}  // namespace
+}  // namespace fdtd
+
+struct fdtd_parsed {
+    fdtd_scene scene;
+    std::vector<fdtd_object> objects;
+    std::vector<fdtd_impulse_source> impulses;
+    std::vector<fdtd_pressure_source> psrc;
+    std::vector<double> listeners;
+    int num_steps = 0;
+};
+
+extern "C" {
+
+fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
+{
+    using namespace fdtd;
+    if (!path || !out) return FDTD_ERR_INVALID;
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { set_last_error(std::string("cannot open ") + path); return FDTD_ERR_PARSE; }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    Reader rd(text);
+    std::unique_ptr<Node> root = rd.element();
+    if (!root) { set_last_error("XML: " + rd.err); return FDTD_ERR_PARSE; }
+    auto P = std::make_unique<fdtd_parsed>();
+    try {
+        if (root->name != "impulse_response") throw ParseError{"root element must be <impulse_response>"};
+        const Node *solver = root->child("solver");
+        if (!solver) throw ParseError{"missing <solver>"};
+        fdtd_settings &s = P->scene.settings;
+        fdtd_settings_default(&s);
+        s.sound_speed = opt_real(solver, "sound_speed", 343.0);
+        s.density = opt_real(solver, "density", 1.0);
+        s.cell_size = req_real(solver, "cellsize");
+        s.stop_time = req_real(solver, "stop_time");
+        P->num_steps = opt_int(solver, "number_of_timesteps", 0);
+        s.time_step = 1.0 / req_real(solver, "timestepfrequency");
+        s.pml_width = req_real(solver, "PML_width");
+        s.pml_strength = req_real(solver, "PML_strength");
+        s.air_viscosity_alpha = opt_real(solver, "air_viscosity_alpha", 0.0);
+        s.use_mesh = opt_int(solver, "use_mesh", 1) != 0;
+        s.boundary_condition_preset = opt_int(solver, "boundary_condition_preset", 0);
+        s.start_time = opt_real(solver, "fast_forward_to_event_time", 0.0);
+        const std::string bh = opt_str(solver, "boundary_handling", "rasterize");
+        if (bh == "rasterize") s.boundary_handling = FDTD_BOUNDARY_RASTERIZE;
+        else if (bh == "fully_coupled") s.boundary_handling = FDTD_BOUNDARY_FULLY_COUPLED;
+        else throw ParseError{"boundary handling type not understood: " + bh};
+        const Node *pol = root->child("solver_control_policy");
+        if (!pol) throw ParseError{"missing <solver_control_policy>"};
+        if (opt_str(pol, "type", "") != "static") throw ParseError{"only the static solver_control_policy is supported"};
+        const int res = req_int(pol, "gridresolution");
+        const int div[3] = {opt_int(pol, "gridresolution_x", res), opt_int(pol, "gridresolution_y", res),
+                            opt_int(pol, "gridresolution_z", res)};
+        const double ctr[3] = {opt_real(pol, "domain_center_x", 0.0), opt_real(pol, "domain_center_y", 0.0),
+                               opt_real(pol, "domain_center_z", 0.0)};
+        // BoundingBox(cellSize, cellDivisions, center) and the ScalarField
+        // division count ceil((length - EPS) / h) (ScalarField.cpp:37-46)
+        for (int d = 0; d < 3; ++d) {
+            const double mn = ctr[d] - s.cell_size * (double)div[d] / 2.0;
+            const double mx = ctr[d] + s.cell_size * (double)div[d] / 2.0;
+            s.min_bound[d] = mn;
+            s.divisions[d] = (int)std::ceil(((mx - mn) - 1e-10) / s.cell_size);
+        }
+        if (const Node *ps = root->child("pressure_source")) {
+            for (const Node *g : ps->children("gaussian_pressure_source")) {
+                fdtd_pressure_source q{};
+                q.width_time = req_real(g, "source_width_time");
+                q.width_space = opt_real(g, "source_width_space", s.sound_speed * q.width_time);
+                q.offset_time = opt_real(g, "source_offset_time", 0.0);
+                const bool flip = opt_real(g, "source_sign_flip", 0.0) > 1e-10;
+                q.normalize_constant = opt_real(g, "source_normalize_constant", 1.0 / std::pow(std::sqrt(2.0 * M_PI) * q.width_space, 3)) *
+                                       (flip ? -1.0 : 1.0);
+                q.position[0] = req_real(g, "source_position_x");
+                q.position[1] = req_real(g, "source_position_y");
+                q.position[2] = req_real(g, "source_position_z");
+                P->psrc.push_back(q);
+            }
+            if (!ps->children("speaker_pressure_source").empty())
+                throw ParseError{"speaker_pressure_source is not supported (needs an audio file reader)"};
+        }
+        if (const Node *ll = root->child("listening_point_list"))
+            for (const Node *l : ll->children("listening_point")) {
+                P->listeners.push_back(req_real(l, "x"));
+                P->listeners.push_back(req_real(l, "y"));
+                P->listeners.push_back(req_real(l, "z"));
+            }
+        if (const Node *scn = root->child("scene")) {
+            for (auto &k : scn->kids) {
+                const Node *o = k.get();
+                fdtd_object ob{};
+                if (o->name == "analytic_sphere") {
+                    ob.type = FDTD_OBJ_SPHERE;
+                    const double r = req_real(o, "radius");
+                    ob.semi_axes[0] = ob.semi_axes[1] = ob.semi_axes[2] = r;
+                    for (int q = 0; q < 9; ++q) ob.rotation[q] = (q % 4 == 0);
+                } else if (o->name == "analytic_ellipsoid") {
+                    ob.type = FDTD_OBJ_ELLIPSOID;
+                    ob.semi_axes[0] = req_real(o, "semi_axis_x");
+                    ob.semi_axes[1] = req_real(o, "semi_axis_y");
+                    ob.semi_axes[2] = req_real(o, "semi_axis_z");
+                    double w = opt_real(o, "quat_w", 1.0), x = opt_real(o, "quat_x", 0.0), y = opt_real(o, "quat_y", 0.0),
+                           z = opt_real(o, "quat_z", 0.0);
+                    const double nq = std::sqrt(w * w + x * x + y * y + z * z);
+                    w /= nq; x /= nq; y /= nq; z /= nq;
+                    const double R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                                         2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                                         2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)};
+                    std::memcpy(ob.rotation, R, sizeof R);
+                } else if (o->name == "rigid_object" || o->name == "rigid_sound_object") {
+                    throw ParseError{"<" + o->name + "> needs mesh and distance-field files; use analytic_sphere/analytic_ellipsoid"};
+                } else {
+                    continue;
+                }
+                ob.center[0] = req_real(o, "center_x");
+                ob.center[1] = req_real(o, "center_y");
+                ob.center[2] = req_real(o, "center_z");
+                const int oi = (int)P->objects.size();
+                P->objects.push_back(ob);
+                for (const Node *g : o->children("gaussian_impulse_source")) {
+                    fdtd_impulse_source is{};
+                    is.object = oi;
+                    is.amplitude = req_real(g, "amplitude");
+                    is.width = req_real(g, "width");
+                    is.onset = opt_real(g, "onset", 4.0 * is.width);
+                    P->impulses.push_back(is);
+                }
+            }
+        }
+    } catch (const ParseError &e) {
+        set_last_error(e.msg);
+        return FDTD_ERR_PARSE;
+    }
+    fdtd_scene &sc = P->scene;
+    sc.n_objects = (int)P->objects.size();
+    sc.objects = P->objects.data();
+    sc.n_impulse_sources = (int)P->impulses.size();
+    sc.impulse_sources = P->impulses.data();
+    sc.n_pressure_sources = (int)P->psrc.size();
+    sc.pressure_sources = P->psrc.data();
+    sc.n_listeners = (int)(P->listeners.size() / 3);
+    sc.listeners = P->listeners.data();
+    *out = P.release();
+    return FDTD_OK;
+}
+
+const fdtd_scene *fdtd_parsed_scene(const fdtd_parsed *p) { return p ? &p->scene : nullptr; }
+int fdtd_parsed_num_steps(const fdtd_parsed *p) { return p ? p->num_steps : 0; }
+void fdtd_parsed_free(fdtd_parsed *p) { delete p; }
+
+fdtd_status fdtd_create_from_xml(const char *path, const fdtd_options *opt, fdtd_handle **out)
+{
+    fdtd_parsed *p = nullptr;
+    fdtd_status st = fdtd_parse_xml(path, &p);
+    if (st != FDTD_OK) return st;
+    st = fdtd_create(&p->scene, opt, out);
+    fdtd_parsed_free(p);
+    return st;
+}
+
+}  // extern "C"
