@@ -1,5 +1,7 @@
-// device_scene.cpp -- arena allocation, table upload and per-step launch
-// sequence of one scene.
+// device_scene.cpp -- arena allocation, table upload, tile plans and the
+// per-step launch sequence.
+#include <algorithm>
+#include <climits>
 #include <cstring>
 
 #include "device_scene.h"
@@ -25,43 +27,35 @@ struct Carver {
 };
 
 struct Layout {
-    float *p[3], *pdir[3], *v[2][3];
-    uint8_t *cpml[3], *fpml[3];
-    float *f1[3], *f2[3], *f3[3], *f5h[3];
+    float *field;
+    float4 *coef[3];
+    uint8_t *flags[3];
+    uint8_t *skip;
     float *gtab;
     PSrcDev *psrc;
     ImpDev *isrc;
-    SpecialBox *boxes;
-    int32_t *cls;
     SpecialRec *recs;
     GhostDev *ghosts;
     int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
     double *gk_w, *gu_w, *gval0, *gval1, *gtmp;
     int32_t *lptr, *lkind, *lindex;
     double *lbeta, *series;
-    long long *clock;
+    unsigned long long *clock;
 };
+
+long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
 
 void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
 {
-    const size_t N = (size_t)S.N;
-    for (int r = 0; r < 3; ++r) L.p[r] = c.take<float>(N);
-    for (int d = 0; d < 3; ++d) L.pdir[d] = c.take<float>(N);
-    for (int b = 0; b < 2; ++b)
-        for (int d = 0; d < 3; ++d) L.v[b][d] = c.take<float>(N);
+    L.field = c.take<float>((size_t)NUM_SLOTS * (size_t)pad_cells(S.N));
     for (int d = 0; d < 3; ++d) {
-        L.cpml[d] = c.take<uint8_t>(S.n[d]);
-        L.fpml[d] = c.take<uint8_t>(S.n[d] + 1);
-        L.f1[d] = c.take<float>(S.n[d] + 1);
-        L.f2[d] = c.take<float>(S.n[d] + 1);
-        L.f3[d] = c.take<float>(S.n[d]);
-        L.f5h[d] = c.take<float>(S.n[d]);
+        L.coef[d] = c.take<float4>(S.n[d] + 1);
+        L.flags[d] = c.take<uint8_t>(S.n[d] + 1);
     }
+    L.skip = c.take<uint8_t>(S.objs.empty() ? 0 : (size_t)S.N);
     L.gtab = c.take<float>(S.gtab.size());
     L.psrc = c.take<PSrcDev>(S.psrc.size());
     L.isrc = c.take<ImpDev>(S.isrc.size());
-    L.boxes = c.take<SpecialBox>(S.boxes.size());
-    L.cls = c.take<int32_t>(S.cls.size());
     L.recs = c.take<SpecialRec>(S.special.size());
     L.ghosts = c.take<GhostDev>(S.ghosts.size());
     L.gk_ptr = c.take<int32_t>(S.gk_ptr.size());
@@ -78,10 +72,24 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.lindex = c.take<int32_t>(S.lst_terms.size());
     L.lbeta = c.take<double>(S.lst_terms.size());
     L.series = c.take<double>((size_t)cap * (size_t)(S.L ? S.L : 1));
-    L.clock = c.take<long long>(1);
+    L.clock = c.take<unsigned long long>(CLK_WORDS);
 }
 
 }  // namespace
+
+TilePlan tile_plan(int id)
+{
+    TilePlan p;
+    switch (id) {
+    case 1: p.interior_tk_log2 = 6; p.interior_ti = 8; p.pml_ti = 8; break;
+    case 2: p.interior_tk_log2 = 5; p.interior_ti = 16; p.pml_ti = 16; break;
+    case 3: p.interior_tk_log2 = 6; p.interior_ti = 32; p.pml_ti = 16; break;
+    case 4: p.interior_tk_log2 = 7; p.interior_ti = 16; p.pml_ti = 16; break;
+    case 5: p.interior_tk_log2 = 5; p.interior_ti = 32; p.pml_ti = 32; break;
+    default: break;  // 0: 64 x 4, 16 planes
+    }
+    return p;
+}
 
 size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)
 {
@@ -91,9 +99,10 @@ size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)
     return c.off;
 }
 
-fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap)
+fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long target)
 {
     series_cap = cap;
+    npad = pad_cells(S.N);
     Carver c;
     Layout L;
     carve(S, cap, c, L);
@@ -102,29 +111,34 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap)
     c.base = arena;
     c.off = 0;
     carve(S, cap, c, L);
-    for (int r = 0; r < 3; ++r) p[r] = L.p[r];
-    for (int d = 0; d < 3; ++d) pdir[d] = L.pdir[d];
-    for (int b = 0; b < 2; ++b)
-        for (int d = 0; d < 3; ++d) v[b][d] = L.v[b][d];
-    // fields start at rest: p, pDir and both face-velocity buffers
-    const size_t field_bytes = (size_t)((char *)L.cpml[0] - (char *)L.p[0]);
-    FDTD_HIP_TRY(hipMemsetAsync(L.p[0], 0, field_bytes, st));
+    // fields start at rest: p ring, pDir and both face-velocity buffers
+    FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * (size_t)NUM_SLOTS * (size_t)npad, st));
+    std::vector<char> staging;  // pageable copies are synchronised before return
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
+    std::vector<float4> coef[3];
+    std::vector<uint8_t> flags[3];
     for (int d = 0; d < 3; ++d) {
-        cpml[d] = L.cpml[d]; fpml[d] = L.fpml[d];
-        f1[d] = L.f1[d]; f2[d] = L.f2[d]; f3[d] = L.f3[d]; f5h[d] = L.f5h[d];
-        FDTD_HIP_TRY(up(cpml[d], S.cpml[d].data(), S.cpml[d].size()));
-        FDTD_HIP_TRY(up(fpml[d], S.fpml[d].data(), S.fpml[d].size()));
-        FDTD_HIP_TRY(up(f1[d], S.f1[d].data(), S.f1[d].size() * 4));
-        FDTD_HIP_TRY(up(f2[d], S.f2[d].data(), S.f2[d].size() * 4));
-        FDTD_HIP_TRY(up(f3[d], S.f3[d].data(), S.f3[d].size() * 4));
-        FDTD_HIP_TRY(up(f5h[d], S.f5h[d].data(), S.f5h[d].size() * 4));
+        const int n = S.n[d];
+        coef[d].resize(n + 1);
+        flags[d].resize(n + 1);
+        for (int i = 0; i <= n; ++i) {
+            coef[d][i] = make_float4(S.f1[d][i], S.f2[d][i], i < n ? S.f3[d][i] : 0.f, i < n ? S.f5h[d][i] : 0.f);
+            flags[d][i] = (uint8_t)((i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0));
+        }
+        FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
+        FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size()));
     }
-    gtab = L.gtab;
-    FDTD_HIP_TRY(up(gtab, S.gtab.data(), S.gtab.size() * 4));
+    std::vector<uint8_t> skip;
+    if (!S.objs.empty()) {
+        skip.assign((size_t)S.N, 0);
+        for (int64_t i = 0; i < S.N; ++i) skip[i] = S.bulk[i] ? 0 : 1;
+        for (const SpecialRec &r : S.special) skip[r.cell] = 1;
+        FDTD_HIP_TRY(up(L.skip, skip.data(), skip.size()));
+    }
+    FDTD_HIP_TRY(up(L.gtab, S.gtab.data(), S.gtab.size() * 4));
     std::vector<PSrcDev> ps;
     for (const PSrcHost &h : S.psrc) {
         PSrcDev d{};
@@ -132,47 +146,70 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap)
         for (int e = 0; e < 3; ++e) { d.lo[e] = h.lo[e]; d.hi[e] = h.hi[e]; d.goff[e] = h.goff[e]; }
         ps.push_back(d);
     }
-    psrc = L.psrc;
-    FDTD_HIP_TRY(up(psrc, ps.data(), ps.size() * sizeof(PSrcDev)));
+    FDTD_HIP_TRY(up(L.psrc, ps.data(), ps.size() * sizeof(PSrcDev)));
     std::vector<ImpDev> is;
     for (const ImpulseHost &h : S.isrc) is.push_back({h.obj, 0, h.A, h.w, h.ton});
-    isrc = L.isrc;
-    FDTD_HIP_TRY(up(isrc, is.data(), is.size() * sizeof(ImpDev)));
-    boxes = L.boxes; cls = L.cls; recs = L.recs;
-    FDTD_HIP_TRY(up(boxes, S.boxes.data(), S.boxes.size() * sizeof(SpecialBox)));
-    FDTD_HIP_TRY(up(cls, S.cls.data(), S.cls.size() * 4));
-    FDTD_HIP_TRY(up(recs, S.special.data(), S.special.size() * sizeof(SpecialRec)));
+    FDTD_HIP_TRY(up(L.isrc, is.data(), is.size() * sizeof(ImpDev)));
+    FDTD_HIP_TRY(up(L.recs, S.special.data(), S.special.size() * sizeof(SpecialRec)));
     std::vector<GhostDev> gd;
     for (const GhostEntry &g : S.ghosts) gd.push_back({g.nbr, g.obj, g.coupled, 0, g.k_raster, g.c_acc});
-    ghosts = L.ghosts;
-    FDTD_HIP_TRY(up(ghosts, gd.data(), gd.size() * sizeof(GhostDev)));
-    gk_ptr = L.gk_ptr; gk_idx = L.gk_idx; gk_w = L.gk_w;
-    gu_ptr = L.gu_ptr; gu_idx = L.gu_idx; gu_w = L.gu_w;
-    FDTD_HIP_TRY(up(gk_ptr, S.gk_ptr.data(), S.gk_ptr.size() * 4));
-    FDTD_HIP_TRY(up(gk_idx, S.gk_idx.data(), S.gk_idx.size() * 4));
-    FDTD_HIP_TRY(up(gk_w, S.gk_w.data(), S.gk_w.size() * 8));
-    FDTD_HIP_TRY(up(gu_ptr, S.gu_ptr.data(), S.gu_ptr.size() * 4));
-    FDTD_HIP_TRY(up(gu_idx, S.gu_idx.data(), S.gu_idx.size() * 4));
-    FDTD_HIP_TRY(up(gu_w, S.gu_w.data(), S.gu_w.size() * 8));
-    gval[0] = L.gval0; gval[1] = L.gval1; gtmp = L.gtmp;
+    FDTD_HIP_TRY(up(L.ghosts, gd.data(), gd.size() * sizeof(GhostDev)));
+    FDTD_HIP_TRY(up(L.gk_ptr, S.gk_ptr.data(), S.gk_ptr.size() * 4));
+    FDTD_HIP_TRY(up(L.gk_idx, S.gk_idx.data(), S.gk_idx.size() * 4));
+    FDTD_HIP_TRY(up(L.gk_w, S.gk_w.data(), S.gk_w.size() * 8));
+    FDTD_HIP_TRY(up(L.gu_ptr, S.gu_ptr.data(), S.gu_ptr.size() * 4));
+    FDTD_HIP_TRY(up(L.gu_idx, S.gu_idx.data(), S.gu_idx.size() * 4));
+    FDTD_HIP_TRY(up(L.gu_w, S.gu_w.data(), S.gu_w.size() * 8));
     if (!S.ghosts.empty()) {
-        FDTD_HIP_TRY(hipMemsetAsync(gval[0], 0, S.ghosts.size() * 8, st));
-        FDTD_HIP_TRY(hipMemsetAsync(gval[1], 0, S.ghosts.size() * 8, st));
+        FDTD_HIP_TRY(hipMemsetAsync(L.gval0, 0, S.ghosts.size() * 8, st));
+        FDTD_HIP_TRY(hipMemsetAsync(L.gval1, 0, S.ghosts.size() * 8, st));
     }
     std::vector<int32_t> kind, index;
     std::vector<double> beta;
     for (const ListenerTerm &t : S.lst_terms) { kind.push_back(t.kind); index.push_back(t.index); beta.push_back(t.beta); }
-    lptr = L.lptr; lkind = L.lkind; lindex = L.lindex; lbeta = L.lbeta;
-    FDTD_HIP_TRY(up(lptr, S.lst_ptr.data(), S.lst_ptr.size() * 4));
-    FDTD_HIP_TRY(up(lkind, kind.data(), kind.size() * 4));
-    FDTD_HIP_TRY(up(lindex, index.data(), index.size() * 4));
-    FDTD_HIP_TRY(up(lbeta, beta.data(), beta.size() * 8));
-    series = L.series;
-    clock = L.clock;
-    FDTD_HIP_TRY(hipMemsetAsync(clock, 0, sizeof(long long), st));
-    // the host vectors above are pageable: make the copies complete before
-    // they go out of scope
+    FDTD_HIP_TRY(up(L.lptr, S.lst_ptr.data(), S.lst_ptr.size() * 4));
+    FDTD_HIP_TRY(up(L.lkind, kind.data(), kind.size() * 4));
+    FDTD_HIP_TRY(up(L.lindex, index.data(), index.size() * 4));
+    FDTD_HIP_TRY(up(L.lbeta, beta.data(), beta.size() * 8));
+    const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
+    FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
+
+    SceneDev &D = dev;
+    std::memset(&D, 0, sizeof D);
+    for (int d = 0; d < 3; ++d) D.n[d] = S.n[d];
+    D.sy = S.n[2];
+    D.sx = S.n[1] * S.n[2];
+    D.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
+    D.npad = npad;
+    D.field = L.field;
+    for (int d = 0; d < 3; ++d) { D.coef[d] = L.coef[d]; D.flags[d] = L.flags[d]; }
+    D.skip = S.objs.empty() ? nullptr : L.skip;
+    D.lam2 = S.lam2;
+    D.lam2f = (float)S.lam2;
+    D.t_start = S.t_start;
+    D.dt = S.dt;
+    D.n_psrc = (int)S.psrc.size();
+    D.n_isrc = (int)S.isrc.size();
+    D.n_special = (int)S.special.size();
+    D.psrc = L.psrc;
+    D.gtab = L.gtab;
+    D.isrc = L.isrc;
+    D.recs = L.recs;
+    D.ghosts = L.ghosts;
+    D.gval[0] = L.gval0;
+    D.gval[1] = L.gval1;
+    D.gtmp = L.gtmp;
+    D.G = (int)S.ghosts.size();
+    D.coupled_iters = S.coupled_iters;
+    D.gk_ptr = L.gk_ptr; D.gk_idx = L.gk_idx; D.gk_w = L.gk_w;
+    D.gu_ptr = L.gu_ptr; D.gu_idx = L.gu_idx; D.gu_w = L.gu_w;
+    D.L = S.L;
+    D.cap = cap;
+    D.lptr = L.lptr; D.lkind = L.lkind; D.lindex = L.lindex; D.lbeta = L.lbeta;
+    D.series = L.series;
+    D.clock = L.clock;
+    D.n_listen_groups = S.L > 0 ? S.L : 1;
     return FDTD_OK;
 }
 
@@ -184,96 +221,146 @@ void DeviceScene::release()
     }
 }
 
-StepParams DeviceScene::step_params(long long s, const long long *clk, long long offset) const
+static int pow2_ceil_log2(int x)
 {
-    StepParams P{};
-    for (int d = 0; d < 3; ++d) P.n[d] = S.n[d];
-    P.sy = S.n[2];
-    P.sx = S.n[1] * S.n[2];
-    for (int d = 0; d < 3; ++d) { P.ilo[d] = S.inner_lo[d]; P.ihi[d] = S.inner_hi[d]; }
-    const int ind = (int)((1 + s) % 3);  // reference ring starts at _pCollocatedInd = 1
-    P.pcur = p[ind];
-    P.pprev = p[(ind + 2) % 3];
-    P.pnext = p[(ind + 1) % 3];
-    const int vb = (int)(s & 1);
-    for (int d = 0; d < 3; ++d) {
-        P.pdir[d] = pdir[d];
-        P.vin[d] = v[vb][d];
-        P.vout[d] = v[vb ^ 1][d];
-        P.f1[d] = f1[d]; P.f2[d] = f2[d]; P.f3[d] = f3[d]; P.f5h[d] = f5h[d];
-        P.cpml[d] = cpml[d]; P.fpml[d] = fpml[d];
+    int l = 0;
+    while ((1 << l) < x) ++l;
+    return l;
+}
+
+static bool box_overlap(const int *lo, const int *hi, const int *blo, const int *bhi)
+{
+    for (int d = 0; d < 3; ++d)
+        if (!(lo[d] < bhi[d] && hi[d] > blo[d])) return false;
+    return true;
+}
+
+void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out) const
+{
+    const int *n = S.n;
+    const int *a = S.inner_lo, *b = S.inner_hi;
+    auto src_hit = [&](const int *lo, const int *hi) {
+        for (const PSrcHost &p : S.psrc)
+            if (box_overlap(lo, hi, p.lo, p.hi)) return 1;
+        return 0;
+    };
+    auto special_hit = [&](const int *lo, const int *hi) {
+        for (const SpecialBox &bx : S.boxes)
+            if (box_overlap(lo, hi, bx.lo, bx.hi)) return true;
+        return false;
+    };
+    // PML slabs first (heaviest tiles start first)
+    const int slabs[6][6] = {
+        {0, a[0], 0, n[1], 0, n[2]},       {b[0], n[0], 0, n[1], 0, n[2]},
+        {a[0], b[0], 0, a[1], 0, n[2]},    {a[0], b[0], b[1], n[1], 0, n[2]},
+        {a[0], b[0], a[1], b[1], 0, a[2]}, {a[0], b[0], a[1], b[1], b[2], n[2]},
+    };
+    for (const auto &sl : slabs) {
+        const int ie = sl[1] - sl[0], je = sl[3] - sl[2], ke = sl[5] - sl[4];
+        if (ie <= 0 || je <= 0 || ke <= 0) continue;
+        const int tk = std::min(6, pow2_ceil_log2(ke));
+        const int TK = 1 << tk, TJ = 256 >> tk;
+        for (int i0 = sl[0]; i0 < sl[1]; i0 += plan.pml_ti)
+            for (int j0 = sl[2]; j0 < sl[3]; j0 += TJ)
+                for (int k0 = sl[4]; k0 < sl[5]; k0 += TK) {
+                    TileDesc t{};
+                    t.scene = scene_id;
+                    t.type = TILE_PML;
+                    t.i0 = i0; t.i1 = std::min(i0 + plan.pml_ti, sl[1]);
+                    t.j0 = j0; t.j1 = std::min(j0 + TJ, sl[3]);
+                    t.k0 = k0; t.k1 = std::min(k0 + TK, sl[5]);
+                    t.tk_log2 = tk;
+                    out.push_back(t);
+                }
     }
-    P.lam2f = (float)S.lam2;
-    P.lam2 = S.lam2;
-    P.clock = clk;
-    P.step_offset = offset;
-    P.t_start = S.t_start;
-    P.dt = S.dt;
-    P.n_psrc = (int)S.psrc.size();
-    P.psrc = psrc;
-    P.gtab = gtab;
-    P.n_isrc = (int)S.isrc.size();
-    P.isrc = isrc;
-    P.n_boxes = (int)S.boxes.size();
-    P.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
-    P.boxes = boxes;
-    P.cls = cls;
-    P.recs = recs;
-    P.ghosts = ghosts;
-    P.gval = gval[s & 1];
-    return P;
+    // object-adjacent cells
+    for (int r0 = 0; r0 < (int)S.special.size(); r0 += 256) {
+        TileDesc t{};
+        t.scene = scene_id;
+        t.type = TILE_SPECIAL;
+        t.i0 = r0;
+        t.i1 = std::min(r0 + 256, (int)S.special.size());
+        out.push_back(t);
+    }
+    // interior
+    const int tk = std::min(plan.interior_tk_log2, std::max(0, pow2_ceil_log2(b[2] - a[2])));
+    const int TK = 1 << tk, TJ = 256 >> tk;
+    for (int i0 = a[0]; i0 < b[0]; i0 += plan.interior_ti)
+        for (int j0 = a[1]; j0 < b[1]; j0 += TJ)
+            for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
+                TileDesc t{};
+                t.scene = scene_id;
+                t.i0 = i0; t.i1 = std::min(i0 + plan.interior_ti, b[0]);
+                t.j0 = j0; t.j1 = std::min(j0 + TJ, b[1]);
+                t.k0 = k0; t.k1 = std::min(k0 + TK, b[2]);
+                t.tk_log2 = tk;
+                const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+                t.type = special_hit(lo, hi) ? TILE_INTERIOR_SKIP : TILE_INTERIOR;
+                t.src = src_hit(lo, hi);
+                out.push_back(t);
+            }
 }
 
-GhostParams DeviceScene::ghost_params(long long s, const long long *clk, long long offset) const
+void DeviceScene::append_listen(int scene_id, std::vector<ListenDesc> &out) const
 {
-    GhostParams G{};
-    G.G = (int)S.ghosts.size();
-    G.iters = S.coupled_iters;
-    G.pcur = p[(int)((1 + s) % 3)];
-    G.gval = gval[s & 1];
-    G.gtmp = gtmp;
-    G.ghosts = ghosts;
-    G.gk_ptr = gk_ptr; G.gk_idx = gk_idx; G.gk_w = gk_w;
-    G.gu_ptr = gu_ptr; G.gu_idx = gu_idx; G.gu_w = gu_w;
-    G.isrc = isrc;
-    G.n_isrc = (int)S.isrc.size();
-    G.clock = clk;
-    G.step_offset = offset;
-    G.t_start = S.t_start;
-    G.dt = S.dt;
-    return G;
+    if (S.L == 0) out.push_back({scene_id, -1});
+    for (int l = 0; l < S.L; ++l) out.push_back({scene_id, l});
 }
 
-ListenParams DeviceScene::listen_params(long long s, const long long *clk, long long offset) const
+void xcd_permute(std::vector<TileDesc> &tiles, int nxcd)
 {
-    ListenParams P{};
-    P.L = S.L;
-    P.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
-    const int ind = (int)((1 + s) % 3);
-    P.pold = p[ind];
-    P.pnew = p[(ind + 1) % 3];
-    P.gval = gval[s & 1];
-    P.ghosts = ghosts;
-    P.ptr = lptr; P.kind = lkind; P.index = lindex; P.beta = lbeta;
-    P.isrc = isrc;
-    P.n_isrc = (int)S.isrc.size();
-    P.series = series;
-    P.cap = series_cap;
-    P.clock = clk;
-    P.step_offset = offset;
-    P.t_start = S.t_start;
-    P.dt = S.dt;
-    return P;
+    if (tiles.size() < (size_t)(2 * nxcd)) return;
+    TileDesc empty{};
+    empty.type = TILE_EMPTY;
+    while (tiles.size() % nxcd) tiles.push_back(empty);
+    const size_t per = tiles.size() / nxcd;
+    std::vector<TileDesc> out(tiles.size());
+    for (size_t w = 0; w < tiles.size(); ++w) out[w] = tiles[(w % nxcd) * per + w / nxcd];
+    tiles.swap(out);
 }
 
-fdtd_status DeviceScene::enqueue_step(long long s, const long long *clk, long long offset, int tile, hipStream_t st,
-                                      hipEvent_t ev_begin, hipEvent_t ev_end) const
+fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
+                                const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st)
 {
-    if (needs_ghost_kernel()) FDTD_HIP_TRY(launch_ghost(ghost_params(s, clk, offset), st));
+    release();
+    const size_t b0 = align_up(sizeof(SceneDev) * std::max<size_t>(1, scenes.size()));
+    const size_t b1 = align_up(sizeof(TileDesc) * std::max<size_t>(1, tiles.size()));
+    const size_t b2 = align_up(sizeof(ListenDesc) * std::max<size_t>(1, listen.size()));
+    const size_t b3 = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
+    char *base = nullptr;
+    FDTD_HIP_TRY(hipMalloc(&base, b0 + b1 + b2 + b3));
+    d_scenes = reinterpret_cast<SceneDev *>(base);
+    d_tiles = reinterpret_cast<TileDesc *>(base + b0);
+    d_listen = reinterpret_cast<ListenDesc *>(base + b0 + b1);
+    d_ghost = reinterpret_cast<GhostDesc *>(base + b0 + b1 + b2);
+    ntiles = (int)tiles.size();
+    nlisten = (int)listen.size();
+    nghost = (int)ghost.size();
+    if (!scenes.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice, st));
+    if (!tiles.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, st));
+    if (!listen.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_listen, listen.data(), sizeof(ListenDesc) * listen.size(), hipMemcpyHostToDevice, st));
+    if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
+    FDTD_HIP_TRY(hipStreamSynchronize(st));
+    return FDTD_OK;
+}
+
+void StepProgram::release()
+{
+    if (d_scenes) (void)hipFree(d_scenes);
+    d_scenes = nullptr;
+    d_tiles = nullptr;
+    d_listen = nullptr;
+    d_ghost = nullptr;
+    ntiles = nlisten = nghost = 0;
+}
+
+fdtd_status StepProgram::enqueue(hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) const
+{
+    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
-    FDTD_HIP_TRY(launch_step(step_params(s, clk, offset), tile, st));
+    FDTD_HIP_TRY(launch_step(d_scenes, d_tiles, ntiles, st));
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
-    FDTD_HIP_TRY(launch_listen(listen_params(s, clk, offset), st));
+    FDTD_HIP_TRY(launch_listen(d_scenes, d_listen, nlisten, st));
     return FDTD_OK;
 }
 

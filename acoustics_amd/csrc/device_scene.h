@@ -1,5 +1,6 @@
-// device_scene.h -- device-resident state of one scene (used by the single
-// scene handle, batches and the server).
+// device_scene.h -- device-resident state of one scene, the tile plan of its
+// step, and the launch sequence shared by single scenes, batches and the
+// server.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,50 +21,55 @@ namespace fdtd {
         }                                                                      \
     } while (0)
 
-// Device buffers and uploaded tables of one scene.  All memory comes from a
-// single hipMalloc arena owned by the object.
+// Tile shapes of one plan (chosen by the autotuner).
+struct TilePlan {
+    int interior_tk_log2 = 6;  // 64 threads along k, 4 rows along j
+    int interior_ti = 16;      // planes marched per interior tile
+    int pml_ti = 16;           // planes marched per PML tile
+};
+constexpr int kNumTilePlans = 6;
+TilePlan tile_plan(int id);
+
+// Device buffers and uploaded tables of one scene, all in one hipMalloc arena.
 struct DeviceScene {
     SceneSetup S;
     int device = 0;
     char *arena = nullptr;
     size_t arena_bytes = 0;
-    float *p[3] = {nullptr, nullptr, nullptr};
-    float *pdir[3] = {nullptr, nullptr, nullptr};
-    float *v[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    uint8_t *cpml[3], *fpml[3];
-    float *f1[3], *f2[3], *f3[3], *f5h[3];
-    float *gtab = nullptr;
-    PSrcDev *psrc = nullptr;
-    ImpDev *isrc = nullptr;
-    SpecialBox *boxes = nullptr;
-    int32_t *cls = nullptr;
-    SpecialRec *recs = nullptr;
-    GhostDev *ghosts = nullptr;
-    int32_t *gk_ptr = nullptr, *gk_idx = nullptr, *gu_ptr = nullptr, *gu_idx = nullptr;
-    double *gk_w = nullptr, *gu_w = nullptr;
-    double *gval[2] = {nullptr, nullptr};
-    double *gtmp = nullptr;
-    int32_t *lptr = nullptr, *lkind = nullptr, *lindex = nullptr;
-    double *lbeta = nullptr;
-    double *series = nullptr;
+    SceneDev dev{};            // host copy of the scene block (pointers into the arena)
+    long long npad = 0;
     int series_cap = 0;
-    long long *clock = nullptr;
 
-    // Sizes the arena for a setup; used by admission control.
+    // Device bytes a scene needs (admission control uses this).
     static size_t bytes_needed(const SceneSetup &S, int series_cap);
-    fdtd_status allocate_and_upload(hipStream_t stream, int series_cap);
+    // Allocates, zeroes the fields and uploads every table; target is the
+    // number of steps after which the kernels stop advancing this scene.
+    fdtd_status allocate_and_upload(hipStream_t stream, int series_cap, long long target);
     void release();
+    float *slot(int s) const { return dev.field + (size_t)s * (size_t)npad; }
 
-    // Kernel parameter blocks of global step s (0-based).  clock != nullptr
-    // makes the kernels read the base step from device memory (graph replay)
-    // and use `offset` relative to it; otherwise offset is the absolute step.
-    StepParams step_params(long long s, const long long *clk, long long offset) const;
-    GhostParams ghost_params(long long s, const long long *clk, long long offset) const;
-    ListenParams listen_params(long long s, const long long *clk, long long offset) const;
+    // Appends this scene's tiles / listener groups / ghost work, tagged with scene_id.
+    void append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out) const;
+    void append_listen(int scene_id, std::vector<ListenDesc> &out) const;
     bool needs_ghost_kernel() const { return S.boundary == FDTD_BOUNDARY_FULLY_COUPLED && !S.ghosts.empty(); }
-    // Enqueues the kernels of step s.
-    fdtd_status enqueue_step(long long s, const long long *clk, long long offset, int tile, hipStream_t st,
-                             hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
+};
+
+// Reorders a spatially ordered tile list so workgroup w (dispatched to XCD
+// w % 8) runs tile chunk (w % 8), keeping neighbouring tiles on one L2.
+void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
+
+// The per-step launch sequence of a set of scenes (ghost solve, fused step,
+// listeners + step advance), uploaded once.
+struct StepProgram {
+    SceneDev *d_scenes = nullptr;
+    TileDesc *d_tiles = nullptr;
+    ListenDesc *d_listen = nullptr;
+    GhostDesc *d_ghost = nullptr;
+    int ntiles = 0, nlisten = 0, nghost = 0;
+    fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
+                       const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st);
+    void release();
+    fdtd_status enqueue(hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
 };
 
 }  // namespace fdtd

@@ -1,4 +1,11 @@
-// kernels.h -- device-side parameter blocks and launch entry points.
+// kernels.h -- device-side scene blocks, work lists and launch entry points.
+//
+// Every launch works from a list of tiles; a tile belongs to one scene and is
+// homogeneous (interior leapfrog, PML slab, or a run of object-adjacent
+// cells), so a wavefront never mixes code paths.  Scene parameters live in
+// device memory (SceneDev) and are read with uniform loads; the step index of
+// each scene is a device counter, so a captured graph can be replayed at any
+// step and a batch can hold scenes at different steps.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,85 +33,64 @@ struct GhostDev {
     double k_raster, c_acc;
 };
 
-// Everything the fused collocated step needs for one scene.
-struct StepParams {
+// field slots inside SceneDev::field, each npad floats
+enum : int { SLOT_P0 = 0, SLOT_PDIR = 3, SLOT_V0 = 6, SLOT_V1 = 9, NUM_SLOTS = 12 };
+
+// clock words of a scene
+enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_WORDS = 4 };
+
+struct SceneDev {
     int n[3];
-    int sx, sy;                // strides of i and j (z fastest)
-    int ilo[3], ihi[3];        // non-PML box
-    const float *pprev;        // p^{n-1}
-    const float *pcur;         // p^n
-    float *pnext;              // p^{n+1}
-    float *pdir[3];            // PML split pressure, in place
-    const float *vin[3];       // PML face velocity, step n-1/2 (left face of each cell)
-    float *vout[3];            // PML face velocity, step n+1/2
-    const float *f1[3], *f2[3];       // face tables, n_d + 1
-    const float *f3[3], *f5h[3];      // cell tables, n_d
-    const uint8_t *cpml[3], *fpml[3];
-    float lam2f;               // (c dt / h)^2
-    double lam2;
-    // time: step = (clock ? *clock : 0) + step_offset, t_n = t_start + step dt
-    const long long *clock;
-    long long step_offset;
-    double t_start, dt;
-    int n_psrc, n_isrc;
+    int sx, sy;                 // strides of i and j (k fastest)
+    int raster;                 // 1: rasterized ghost values, 0: fully coupled
+    long long npad;             // floats per field slot
+    float *field;               // NUM_SLOTS slots: p ring (3), pDir (3), v ping-pong (2 x 3)
+    const float4 *coef[3];      // per axis, index 0..n: {f1(face), f2(face), f3(cell), f5/h(cell)}
+    const uint8_t *flags[3];    // per axis: bit0 cell in PML, bit1 face in PML
+    const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
+    double lam2, t_start, dt;
+    float lam2f;
+    int n_psrc, n_isrc, n_special;
     const PSrcDev *psrc;
     const float *gtab;
     const ImpDev *isrc;
-    // cells next to rigid objects
-    int n_boxes, raster;
-    const SpecialBox *boxes;
-    const int32_t *cls;
     const SpecialRec *recs;
     const GhostDev *ghosts;
-    const double *gval;        // ghost values of this step (fully-coupled mode)
+    double *gval[2];            // ghost values by step parity (fully-coupled mode)
+    double *gtmp;               // 2G scratch of the coupled iteration
+    int G, coupled_iters;
+    const int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
+    const double *gk_w, *gu_w;
+    int L, cap;                 // listeners, rows of the series ring
+    const int32_t *lptr, *lkind, *lindex;
+    const double *lbeta;
+    double *series;             // [cap][L]
+    unsigned long long *clock;  // CLK_WORDS words
+    int n_listen_groups;        // workgroups of the listener launch for this scene
+    int pad;
 };
 
-struct GhostParams {
-    int G, iters;
-    const float *pcur;
-    double *gval;              // output, G values
-    double *gtmp;              // scratch, G values
-    const GhostDev *ghosts;
-    const int32_t *gk_ptr, *gk_idx;
-    const double *gk_w;
-    const int32_t *gu_ptr, *gu_idx;
-    const double *gu_w;
-    const ImpDev *isrc;
-    int n_isrc;
-    const long long *clock;
-    long long step_offset;
-    double t_start, dt;
+enum : int { TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4 };
+
+struct TileDesc {
+    int scene;
+    int type;
+    int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records
+    int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK
+    int src;                     // tile intersects a pressure-source box
 };
 
-struct ListenParams {
-    int L, raster;
-    const float *pnew;         // p^{n+1}
-    const float *pold;         // p^n (rasterised ghost values derive from it)
-    const double *gval;        // ghost values of step n (fully-coupled mode)
-    const GhostDev *ghosts;
-    const int32_t *ptr;
-    const int32_t *kind;
-    const int32_t *index;
-    const double *beta;
-    const ImpDev *isrc;
-    int n_isrc;
-    double *series;            // [cap][L]
-    int cap;
-    const long long *clock;
-    long long step_offset;     // output row = step (t_n is the time before the step)
-    double t_start, dt;
+struct ListenDesc {
+    int scene;
+    int l;                       // listener index, or -1 for the step-advance-only group
 };
 
-struct TileConfig {
-    int tk, tj, ti;            // threads along k, j; planes marched per thread
+struct GhostDesc {
+    int scene;
 };
-constexpr int kNumTileConfigs = 6;
-TileConfig tile_config(int id);
 
-hipError_t launch_step(const StepParams &p, int tile_id, hipStream_t s);
-hipError_t launch_ghost(const GhostParams &p, hipStream_t s);
-hipError_t launch_listen(const ListenParams &p, hipStream_t s);
-hipError_t launch_advance_clock(long long *clock, long long by, hipStream_t s);
-hipError_t launch_set_clock(long long *clock, long long v, hipStream_t s);
+hipError_t launch_step(const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s);
+hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s);
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s);
 
 }  // namespace fdtd

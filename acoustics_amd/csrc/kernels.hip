@@ -1,20 +1,20 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the collocated FDTD step.
 //
-// One launch advances the whole grid by one step (PML_WaveSolver::stepCollocated,
-// src/wavesolver/PML_WaveSolver.cpp:775-826):
-//   * non-PML bulk cells: p^{n+1} = 2 p^n - p^{n-1} + (c dt/h)^2 L(p^n) + S dt
-//     (MAC_Grid.cpp:924-944), FP32;
-//   * PML cells: staggered face velocity v <- f1 v + f2 (p_R - p_L) on the
+// step_kernel advances every tile of the work list by one step
+// (PML_WaveSolver::stepCollocated, src/wavesolver/PML_WaveSolver.cpp:775-826):
+//   * TILE_INTERIOR: p^{n+1} = 2 p^n - p^{n-1} + (c dt/h)^2 L(p^n) + S dt
+//     (MAC_Grid.cpp:924-944), FP32.  Threads own a (j,k) column and march the
+//     tile's planes along i, keeping p^n(i-1), p^n(i), p^n(i+1) in registers;
+//     k is the fastest axis so a wavefront streams 64 consecutive floats.
+//   * TILE_PML: staggered face velocity v <- f1 v + f2 (p_R - p_L) on the
 //     cell's faces, then pDir_d <- f3 pDir_d + f5/h (v_R - v_L) and
-//     p^{n+1} = sum_d pDir_d (MAC_Grid.cpp:461-508), fused in the same thread;
-//   * bulk cells next to rigid objects: the non-uniform ghost-cell Laplacian
-//     (MAC_Grid.cpp:187-299) and the one-cell-cavity freeze, in FP64.
-// Threads own a (j,k) column and march TI planes along i (slowest axis),
-// keeping p^n(i-1), p^n(i), p^n(i+1) in registers; k is the fastest axis so
-// a wavefront reads 64 consecutive floats of a row.  Face velocities are
-// double-buffered (vin -> vout) so a thread may recompute the face it shares
-// with a neighbour without racing the neighbour's write.  No MFMA: a 7-point
-// stencil is not a contraction.
+//     p^{n+1} = sum_d pDir_d (MAC_Grid.cpp:461-508), fused per thread.  Face
+//     velocities are double-buffered (vin -> vout), so a thread may recompute
+//     the face it shares with a neighbour without racing the neighbour's write.
+//   * TILE_SPECIAL: one thread per bulk cell next to a rigid object: the
+//     non-uniform ghost-cell Laplacian (MAC_Grid.cpp:187-299) and the
+//     one-cell-cavity freeze, in FP64; ghost values are FP64.
+// No MFMA anywhere: a 7-point stencil is not a contraction.
 #include "kernels.h"
 
 namespace fdtd {
@@ -31,318 +31,313 @@ __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int o
     return a;
 }
 
-__device__ __forceinline__ long long current_step(const long long *clock, long long off)
+__device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double dt)
 {
-    return (clock ? *clock : 0LL) + off;
+    // S(x, t_n + dt/2) dt without the spatial factor (GaussianPressureSource.cpp:6-30)
+    const double u = (tn + 0.5 * dt - ps.t0) / ps.wt;
+    return (float)(exp(-(u * u) / 2.0) * ps.amp * dt);
 }
 
-template <int TK, int TJ, int TI>
-__global__ __launch_bounds__(TK * TJ) void step_kernel(const StepParams P)
+// ---------------------------------------------------------------------------
+// interior leapfrog
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc &t, long long step, bool skip_check)
 {
-    const int tx = threadIdx.x % TK, ty = threadIdx.x / TK;
-    const int nx = P.n[0], ny = P.n[1], nz = P.n[2];
-    const int k = blockIdx.x * TK + tx;
-    const int j = blockIdx.y * TJ + ty;
-    const int i0 = blockIdx.z * TI;
-    const int i1 = min(i0 + TI, nx);
-    const bool active = (k < nz) && (j < ny);
-    const int kk = min(k, nz - 1), jj = min(j, ny - 1);
-    const int sx = P.sx, sy = P.sy;
+    const int tk = t.tk_log2;
+    const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
+    const int k = t.k0 + tx, j = t.j0 + ty;
+    if (k >= t.k1 || j >= t.j1) return;
+    const int sx = S.sx, sy = S.sy;
+    const int ind = (int)((1 + step) % 3);
+    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
+    const float *__restrict__ pprev = S.field + (size_t)((ind + 2) % 3) * S.npad;
+    float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
+    const float lam2 = S.lam2f;
 
-    // does this tile touch a box of object-adjacent cells? (uniform branch)
-    bool tile_special = false;
-    for (int b = 0; b < P.n_boxes; ++b) {
-        const SpecialBox bx = P.boxes[b];
-        const int k0 = blockIdx.x * TK, j0 = blockIdx.y * TJ;
-        tile_special |= (i0 < bx.hi[0]) && (i1 > bx.lo[0]) && (j0 < bx.hi[1]) && (j0 + TJ > bx.lo[1]) &&
-                        (k0 < bx.hi[2]) && (k0 + TK > bx.lo[2]);
+    constexpr int kMaxSrc = 2;
+    float sjk[kMaxSrc] = {0.f, 0.f};
+    int soff[kMaxSrc] = {0, 0};
+    int nsrc = 0;
+    if (t.src) {
+        const double tn = S.t_start + (double)step * S.dt;
+        nsrc = min(S.n_psrc, kMaxSrc);
+        for (int s = 0; s < nsrc; ++s) {
+            const PSrcDev ps = S.psrc[s];
+            sjk[s] = source_amp(ps, tn, S.dt) * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
+            soff[s] = ps.goff[0];
+        }
     }
 
-    const long long step = current_step(P.clock, P.step_offset);
-    const double tn = P.t_start + (double)step * P.dt;
-
-    // pressure sources: S(x, t_n + dt/2) dt = T(t) gx[i] gy[j] gz[k]
-    constexpr int kMaxRegSrc = 4;
-    float src_jk[kMaxRegSrc];
-    int src_lo[kMaxRegSrc], src_hi[kMaxRegSrc], src_goff[kMaxRegSrc];
-    const int nsrc = min(P.n_psrc, kMaxRegSrc);
-    for (int s = 0; s < nsrc; ++s) {
-        const PSrcDev ps = P.psrc[s];
-        const double u = (tn + 0.5 * P.dt - ps.t0) / ps.wt;
-        const double T = exp(-(u * u) / 2.0) * ps.amp * P.dt;
-        src_jk[s] = (float)T * P.gtab[ps.goff[1] + jj] * P.gtab[ps.goff[2] + kk];
-        src_lo[s] = ps.lo[0]; src_hi[s] = ps.hi[0]; src_goff[s] = ps.goff[0];
-    }
-
-    const bool pml_j = P.cpml[1][jj], pml_k = P.cpml[2][kk];
-    const int base_jk = jj * sy + kk;
-    const int ym_off = (jj > 0) ? sy : 0, yp_off = (jj < ny - 1) ? sy : 0;
-    const int zm_off = (kk > 0) ? 1 : 0, zp_off = (kk < nz - 1) ? 1 : 0;
-
-    float pm = P.pcur[max(i0 - 1, 0) * sx + base_jk];
-    float pc = P.pcur[i0 * sx + base_jk];
-    for (int i = i0; i < i1; ++i) {
-        const int idx = i * sx + base_jk;
-        const float pp = P.pcur[min(i + 1, nx - 1) * sx + base_jk];
-        if (active) {
-            const float ym = P.pcur[idx - ym_off], yp = P.pcur[idx + yp_off];
-            const float zm = P.pcur[idx - zm_off], zp = P.pcur[idx + zp_off];
-            const bool pml_i = P.cpml[0][i];
-            if (pml_i | pml_j | pml_k) {
-                float pn = 0.f;
-                // ---- x faces ----
-                {
-                    const bool oth = pml_j | pml_k;
-                    const bool actL = (i >= 1) && (P.fpml[0][i] | oth);
-                    const float vl = P.vin[0][idx];
-                    const float vlN = actL ? P.f1[0][i] * vl + P.f2[0][i] * (pc - pm) : 0.f;
-                    float vrN = 0.f;
-                    if (i + 1 < nx) {
-                        const bool actR = P.fpml[0][i + 1] | oth;
-                        const float vr = P.vin[0][idx + sx];
-                        vrN = actR ? P.f1[0][i + 1] * vr + P.f2[0][i + 1] * (pp - pc) : 0.f;
-                        if (!(P.cpml[0][i + 1] | oth)) P.vout[0][idx + sx] = vrN;
-                    }
-                    P.vout[0][idx] = vlN;
-                    float pd = P.pdir[0][idx];
-                    pd = P.f3[0][i] * pd + P.f5h[0][i] * (vrN - vlN);
-                    P.pdir[0][idx] = pd;
-                    pn += pd;
-                }
-                // ---- y faces ----
-                {
-                    const bool oth = pml_i | pml_k;
-                    const bool actL = (jj >= 1) && (P.fpml[1][jj] | oth);
-                    const float vl = P.vin[1][idx];
-                    const float vlN = actL ? P.f1[1][jj] * vl + P.f2[1][jj] * (pc - ym) : 0.f;
-                    float vrN = 0.f;
-                    if (jj + 1 < ny) {
-                        const bool actR = P.fpml[1][jj + 1] | oth;
-                        const float vr = P.vin[1][idx + sy];
-                        vrN = actR ? P.f1[1][jj + 1] * vr + P.f2[1][jj + 1] * (yp - pc) : 0.f;
-                        if (!(P.cpml[1][jj + 1] | oth)) P.vout[1][idx + sy] = vrN;
-                    }
-                    P.vout[1][idx] = vlN;
-                    float pd = P.pdir[1][idx];
-                    pd = P.f3[1][jj] * pd + P.f5h[1][jj] * (vrN - vlN);
-                    P.pdir[1][idx] = pd;
-                    pn += pd;
-                }
-                // ---- z faces ----
-                {
-                    const bool oth = pml_i | pml_j;
-                    const bool actL = (kk >= 1) && (P.fpml[2][kk] | oth);
-                    const float vl = P.vin[2][idx];
-                    const float vlN = actL ? P.f1[2][kk] * vl + P.f2[2][kk] * (pc - zm) : 0.f;
-                    float vrN = 0.f;
-                    if (kk + 1 < nz) {
-                        const bool actR = P.fpml[2][kk + 1] | oth;
-                        const float vr = P.vin[2][idx + 1];
-                        vrN = actR ? P.f1[2][kk + 1] * vr + P.f2[2][kk + 1] * (zp - pc) : 0.f;
-                        if (!(P.cpml[2][kk + 1] | oth)) P.vout[2][idx + 1] = vrN;
-                    }
-                    P.vout[2][idx] = vlN;
-                    float pd = P.pdir[2][idx];
-                    pd = P.f3[2][kk] * pd + P.f5h[2][kk] * (vrN - vlN);
-                    P.pdir[2][idx] = pd;
-                    pn += pd;
-                }
-                P.pnext[idx] = pn;
-            } else {
-                int cls = -1;
-                if (tile_special) {
-                    for (int b = 0; b < P.n_boxes; ++b) {
-                        const SpecialBox bx = P.boxes[b];
-                        if (i >= bx.lo[0] && i < bx.hi[0] && jj >= bx.lo[1] && jj < bx.hi[1] && kk >= bx.lo[2] && kk < bx.hi[2]) {
-                            cls = P.cls[bx.offset + ((i - bx.lo[0]) * (bx.hi[1] - bx.lo[1]) + (jj - bx.lo[1])) * (bx.hi[2] - bx.lo[2]) + (kk - bx.lo[2])];
-                            break;
-                        }
-                    }
-                }
-                float src = 0.f;
-                for (int s = 0; s < nsrc; ++s)
-                    if (i >= src_lo[s] && i < src_hi[s]) src += src_jk[s] * P.gtab[src_goff[s] + i];
-                for (int s = kMaxRegSrc; s < P.n_psrc; ++s) {
-                    const PSrcDev ps = P.psrc[s];
-                    const double u = (tn + 0.5 * P.dt - ps.t0) / ps.wt;
-                    const double T = exp(-(u * u) / 2.0) * ps.amp * P.dt;
-                    src += (float)T * P.gtab[ps.goff[0] + i] * P.gtab[ps.goff[1] + jj] * P.gtab[ps.goff[2] + kk];
-                }
-                const float prev = P.pprev[idx];
-                if (cls == -1) {
-                    const float lap = ((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc;
-                    P.pnext[idx] = 2.f * pc - prev + P.lam2f * lap + src;
-                } else if (cls == -2) {
-                    P.pnext[idx] = 0.f;  // solid cell: pressure held at zero
-                } else {
-                    const SpecialRec r = P.recs[cls];
-                    const double c = pc;
-                    const double nb[6] = {pm, pp, ym, yp, zm, zp};
-                    double g[6];
-                    for (int q = 0; q < 6; ++q) {
-                        g[q] = 0.0;
-                        const int gid = r.gid[q];
-                        if (gid >= 0) {
-                            if (P.raster) {
-                                const GhostDev gd = P.ghosts[gid];
-                                g[q] = c + gd.k_raster * impulse_accel(P.isrc, P.n_isrc, gd.obj, tn);
-                            } else {
-                                g[q] = P.gval[gid];
-                            }
-                        }
-                    }
-                    double pnext;
-                    if (r.flags & 1) {
-                        pnext = c;
-                    } else {
-                        double lap = 0.0;
-                        for (int d = 0; d < 3; ++d) {
-                            const bool gn = r.gid[2 * d] >= 0, gp = r.gid[2 * d + 1] >= 0;
-                            if (gp && gn)      lap += (g[2 * d + 1] + g[2 * d] - 2.0 * c) / (9.0 / 16.0);
-                            else if (gp)       lap += (g[2 * d + 1] + 0.75 * nb[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-                            else if (gn)       lap += (0.75 * nb[2 * d + 1] + g[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-                            else               lap += (nb[2 * d + 1] + nb[2 * d] - 2.0 * c);
-                        }
-                        pnext = 2.0 * c - (double)prev + lap * P.lam2;
-                    }
-                    P.pnext[idx] = (float)(pnext + (double)src);
-                }
+    int idx = t.i0 * sx + j * sy + k;
+    float pm = pcur[idx - sx];
+    float pc = pcur[idx];
+    for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+        const float pp = pcur[idx + sx];
+        const float ym = pcur[idx - sy], yp = pcur[idx + sy];
+        const float zm = pcur[idx - 1], zp = pcur[idx + 1];
+        const float prev = pprev[idx];
+        float pn = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
+        for (int s = 0; s < nsrc; ++s) pn += sjk[s] * S.gtab[soff[s] + i];
+        if (t.src && S.n_psrc > kMaxSrc) {
+            const double tn = S.t_start + (double)step * S.dt;
+            for (int s = kMaxSrc; s < S.n_psrc; ++s) {
+                const PSrcDev ps = S.psrc[s];
+                pn += source_amp(ps, tn, S.dt) * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
             }
         }
+        if (!skip_check || !S.skip[idx]) pnext[idx] = pn;
         pm = pc;
         pc = pp;
     }
 }
 
-// Ghost values of the fully-coupled rows (MAC_Grid.cpp:1131-1441): Jacobi
-// iteration x <- b + W x on the static coupling matrix, one workgroup.
-__global__ __launch_bounds__(1024) void ghost_kernel(const GhostParams P)
+// ---------------------------------------------------------------------------
+// PML slab: fused staggered velocity + split pressure update
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pml_axis(const float4 *__restrict__ coef, const uint8_t *__restrict__ flg, int i, int n,
+                                          bool oth, float pc, float pminus, float pplus, const float *__restrict__ vin,
+                                          float *__restrict__ vout, float *__restrict__ pdir, int idx, int stride)
 {
-    const long long step = current_step(P.clock, P.step_offset);
-    const double tn = P.t_start + (double)step * P.dt;
-    double *b = P.gtmp, *x1 = P.gtmp + P.G;
-    for (int g = threadIdx.x; g < P.G; g += blockDim.x) {
-        const GhostDev gd = P.ghosts[g];
-        const double a = impulse_accel(P.isrc, P.n_isrc, gd.obj, tn);
+    const float4 cL = coef[i];
+    const uint8_t fL = flg[i];
+    const bool actL = (i >= 1) && ((fL & 2) || oth);
+    const float vlN = actL ? cL.x * vin[idx] + cL.y * (pc - pminus) : 0.f;
+    float vrN = 0.f;
+    if (i + 1 < n) {
+        const float4 cR = coef[i + 1];
+        const uint8_t fR = flg[i + 1];
+        if ((fR & 2) || oth) vrN = cR.x * vin[idx + stride] + cR.y * (pplus - pc);
+        if (!((fR & 1) || oth)) vout[idx + stride] = vrN;  // right neighbour is not PML: this cell owns the face
+    }
+    vout[idx] = vlN;
+    float pd = pdir[idx];
+    pd = cL.z * pd + cL.w * (vrN - vlN);
+    pdir[idx] = pd;
+    return pd;
+}
+
+__device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, long long step)
+{
+    const int tk = t.tk_log2;
+    const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
+    const int k = t.k0 + tx, j = t.j0 + ty;
+    if (k >= t.k1 || j >= t.j1) return;
+    const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
+    const int sx = S.sx, sy = S.sy;
+    const int ind = (int)((1 + step) % 3);
+    const int vb = (int)(step & 1);
+    float *__restrict__ field = S.field;
+    const size_t np = (size_t)S.npad;
+    const float *__restrict__ pcur = field + (size_t)ind * np;
+    float *__restrict__ pnext = field + (size_t)((ind + 1) % 3) * np;
+    float *__restrict__ pd0 = field + (size_t)(SLOT_PDIR + 0) * np;
+    float *__restrict__ pd1 = field + (size_t)(SLOT_PDIR + 1) * np;
+    float *__restrict__ pd2 = field + (size_t)(SLOT_PDIR + 2) * np;
+    const int vin_s = vb ? SLOT_V1 : SLOT_V0, vout_s = vb ? SLOT_V0 : SLOT_V1;
+    const float *__restrict__ vi0 = field + (size_t)(vin_s + 0) * np;
+    const float *__restrict__ vi1 = field + (size_t)(vin_s + 1) * np;
+    const float *__restrict__ vi2 = field + (size_t)(vin_s + 2) * np;
+    float *__restrict__ vo0 = field + (size_t)(vout_s + 0) * np;
+    float *__restrict__ vo1 = field + (size_t)(vout_s + 1) * np;
+    float *__restrict__ vo2 = field + (size_t)(vout_s + 2) * np;
+    const bool cj = S.flags[1][j] & 1, ck = S.flags[2][k] & 1;
+    const int ym_off = (j > 0) ? sy : 0, yp_off = (j < ny - 1) ? sy : 0;
+    const int zm_off = (k > 0) ? 1 : 0, zp_off = (k < nz - 1) ? 1 : 0;
+
+    int idx = t.i0 * sx + j * sy + k;
+    float pm = pcur[idx - (t.i0 > 0 ? sx : 0)];
+    float pc = pcur[idx];
+    for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+        const float pp = pcur[idx + (i < nx - 1 ? sx : 0)];
+        const float ym = pcur[idx - ym_off], yp = pcur[idx + yp_off];
+        const float zm = pcur[idx - zm_off], zp = pcur[idx + zp_off];
+        const bool ci = S.flags[0][i] & 1;
+        float pn = pml_axis(S.coef[0], S.flags[0], i, nx, cj || ck, pc, pm, pp, vi0, vo0, pd0, idx, sx);
+        pn += pml_axis(S.coef[1], S.flags[1], j, ny, ci || ck, pc, ym, yp, vi1, vo1, pd1, idx, sy);
+        pn += pml_axis(S.coef[2], S.flags[2], k, nz, ci || cj, pc, zm, zp, vi2, vo2, pd2, idx, 1);
+        pnext[idx] = pn;
+        pm = pc;
+        pc = pp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bulk cells next to rigid objects (FP64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &t, long long step)
+{
+    const int r = t.i0 + (int)threadIdx.x;
+    if (r >= t.i1) return;
+    const SpecialRec rec = S.recs[r];
+    const int sx = S.sx, sy = S.sy;
+    const int ind = (int)((1 + step) % 3);
+    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
+    const float *__restrict__ pprev = S.field + (size_t)((ind + 2) % 3) * S.npad;
+    float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
+    const double tn = S.t_start + (double)step * S.dt;
+    const int idx = rec.cell;
+    const double c = pcur[idx];
+    const double nb[6] = {pcur[idx - sx], pcur[idx + sx], pcur[idx - sy], pcur[idx + sy], pcur[idx - 1], pcur[idx + 1]};
+    double g[6];
+    for (int q = 0; q < 6; ++q) {
+        g[q] = 0.0;
+        const int gid = rec.gid[q];
+        if (gid >= 0) {
+            if (S.raster) {
+                const GhostDev gd = S.ghosts[gid];
+                g[q] = c + gd.k_raster * impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
+            } else {
+                g[q] = S.gval[step & 1][gid];
+            }
+        }
+    }
+    double pn;
+    if (rec.flags & 1) {
+        pn = c;  // one-cell cavity: p^{n+1} = p^n (MAC_Grid.cpp:924-925)
+    } else {
+        double lap = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const bool gn = rec.gid[2 * d] >= 0, gp = rec.gid[2 * d + 1] >= 0;
+            if (gp && gn)      lap += (g[2 * d + 1] + g[2 * d] - 2.0 * c) / (9.0 / 16.0);
+            else if (gp)       lap += (g[2 * d + 1] + 0.75 * nb[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
+            else if (gn)       lap += (0.75 * nb[2 * d + 1] + g[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
+            else               lap += (nb[2 * d + 1] + nb[2 * d] - 2.0 * c);
+        }
+        pn = 2.0 * c - (double)pprev[idx] + lap * S.lam2;
+    }
+    if (S.n_psrc > 0) {
+        const int i = idx / sx, j = (idx - i * sx) / sy, k = idx - i * sx - j * sy;
+        for (int s = 0; s < S.n_psrc; ++s) {
+            const PSrcDev ps = S.psrc[s];
+            pn += (double)(source_amp(ps, tn, S.dt) * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k]);
+        }
+    }
+    pnext[idx] = (float)pn;
+}
+
+__global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles)
+{
+    const TileDesc t = tiles[blockIdx.x];
+    if (t.type == TILE_EMPTY) return;
+    const SceneDev &S = scenes[t.scene];
+    const long long step = (long long)S.clock[CLK_STEP];
+    if (step >= (long long)S.clock[CLK_TARGET]) return;
+    switch (t.type) {
+    case TILE_INTERIOR: interior_tile(S, t, step, false); break;
+    case TILE_INTERIOR_SKIP: interior_tile(S, t, step, true); break;
+    case TILE_PML: pml_tile(S, t, step); break;
+    case TILE_SPECIAL: special_tile(S, t, step); break;
+    default: break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fully-coupled ghost values (MAC_Grid.cpp:1131-1441): Jacobi iteration
+// x <- b + W x on the static coupling matrix; one workgroup per scene.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list)
+{
+    const SceneDev &S = scenes[list[blockIdx.x].scene];
+    const long long step = (long long)S.clock[CLK_STEP];
+    if (step >= (long long)S.clock[CLK_TARGET]) return;
+    const double tn = S.t_start + (double)step * S.dt;
+    const int ind = (int)((1 + step) % 3);
+    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
+    double *gval = S.gval[step & 1];
+    double *b = S.gtmp, *x1 = S.gtmp + S.G;
+    for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
+        const GhostDev gd = S.ghosts[g];
+        const double a = impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
         double v;
         if (gd.coupled) {
             v = gd.c_acc * a;
-            for (int q = P.gk_ptr[g]; q < P.gk_ptr[g + 1]; ++q) v += P.gk_w[q] * (double)P.pcur[P.gk_idx[q]];
+            for (int q = S.gk_ptr[g]; q < S.gk_ptr[g + 1]; ++q) v += S.gk_w[q] * (double)pcur[S.gk_idx[q]];
         } else {
-            v = (double)P.pcur[gd.nbr] + gd.k_raster * a;
+            v = (double)pcur[gd.nbr] + gd.k_raster * a;
         }
         b[g] = v;
-        P.gval[g] = v;
+        gval[g] = v;
     }
     __syncthreads();
-    for (int it = 0; it < P.iters; ++it) {
-        const double *src = (it & 1) ? x1 : P.gval;
-        double *dst = (it & 1) ? P.gval : x1;
-        for (int g = threadIdx.x; g < P.G; g += blockDim.x) {
+    for (int it = 0; it < S.coupled_iters; ++it) {
+        const double *src = (it & 1) ? x1 : gval;
+        double *dst = (it & 1) ? gval : x1;
+        for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
             double v = b[g];
-            for (int q = P.gu_ptr[g]; q < P.gu_ptr[g + 1]; ++q) v += P.gu_w[q] * src[P.gu_idx[q]];
+            for (int q = S.gu_ptr[g]; q < S.gu_ptr[g + 1]; ++q) v += S.gu_w[q] * src[S.gu_idx[q]];
             dst[g] = v;
         }
         __syncthreads();
     }
-    if (P.iters & 1) {
-        for (int g = threadIdx.x; g < P.G; g += blockDim.x) P.gval[g] = x1[g];
-    }
+    if (S.coupled_iters & 1)
+        for (int g = threadIdx.x; g < S.G; g += blockDim.x) gval[g] = x1[g];
 }
 
-// Listener pressures (PML_WaveSolver::FetchScalarData): one wavefront per
-// listener, FP64 dot product of precomputed MLS weights with the samples,
-// reduced across the wavefront with DPP/permute shuffles.
-__global__ __launch_bounds__(64) void listen_kernel(const ListenParams P)
+// ---------------------------------------------------------------------------
+// listeners (PML_WaveSolver::FetchScalarData): one wavefront per listener,
+// FP64 dot product of precomputed MLS weights with the samples, reduced
+// across the wavefront.  The last workgroup of a scene to finish advances
+// the scene's step counter (vector atomics only).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
 {
-    const int l = blockIdx.x;
+    const ListenDesc gdsc = groups[blockIdx.x];
+    const SceneDev &S = scenes[gdsc.scene];
+    unsigned long long *clk = S.clock;
+    const long long step = (long long)clk[CLK_STEP];
+    if (step >= (long long)clk[CLK_TARGET]) return;
     const int lane = threadIdx.x;
-    const long long step = current_step(P.clock, P.step_offset);
-    const double tn = P.t_start + (double)step * P.dt;
-    double acc = 0.0;
-    for (int q = P.ptr[l] + lane; q < P.ptr[l + 1]; q += 64) {
-        const int kind = P.kind[q], idx = P.index[q];
-        double v;
-        if (kind == 0) {
-            v = (double)P.pnew[idx];
-        } else if (P.raster) {
-            const GhostDev gd = P.ghosts[idx];
-            v = (double)P.pold[gd.nbr] + gd.k_raster * impulse_accel(P.isrc, P.n_isrc, gd.obj, tn);
-        } else {
-            v = P.gval[idx];
+    if (gdsc.l >= 0) {
+        const int l = gdsc.l;
+        const double tn = S.t_start + (double)step * S.dt;
+        const int ind = (int)((1 + step) % 3);
+        const float *__restrict__ pold = S.field + (size_t)ind * S.npad;
+        const float *__restrict__ pnew = S.field + (size_t)((ind + 1) % 3) * S.npad;
+        double acc = 0.0;
+        for (int q = S.lptr[l] + lane; q < S.lptr[l + 1]; q += 64) {
+            const int kind = S.lkind[q], idx = S.lindex[q];
+            double v;
+            if (kind == 0) {
+                v = (double)pnew[idx];
+            } else if (S.raster) {
+                const GhostDev gd = S.ghosts[idx];
+                v = (double)pold[gd.nbr] + gd.k_raster * impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
+            } else {
+                v = S.gval[step & 1][idx];
+            }
+            acc += S.lbeta[q] * v;
         }
-        acc += P.beta[q] * v;
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) S.series[(size_t)(step % S.cap) * S.L + l] = acc;
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) P.series[(size_t)(step % P.cap) * P.L + l] = acc;
-}
-
-__global__ void advance_clock_kernel(long long *clock, long long by)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) clock[0] += by;
-}
-
-__global__ void set_clock_kernel(long long *clock, long long v)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) clock[0] = v;
-}
-
-TileConfig tile_config(int id)
-{
-    switch (id) {
-    case 0: return {64, 4, 16};
-    case 1: return {64, 4, 8};
-    case 2: return {64, 8, 8};
-    case 3: return {64, 2, 32};
-    case 4: return {128, 2, 16};
-    case 5: return {32, 8, 16};
-    default: return {64, 4, 16};
+    if (lane == 0) {
+        __threadfence();
+        const unsigned long long prev = atomicAdd(&clk[CLK_DONE], 1ULL);
+        if (prev + 1 == (unsigned long long)S.n_listen_groups) {
+            atomicExch(&clk[CLK_DONE], 0ULL);
+            atomicExch(&clk[CLK_STEP], (unsigned long long)(step + 1));
+        }
     }
 }
 
-template <int TK, int TJ, int TI>
-static hipError_t launch_t(const StepParams &p, hipStream_t s)
+hipError_t launch_step(const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s)
 {
-    dim3 grid((p.n[2] + TK - 1) / TK, (p.n[1] + TJ - 1) / TJ, (p.n[0] + TI - 1) / TI);
-    hipLaunchKernelGGL((step_kernel<TK, TJ, TI>), grid, dim3(TK * TJ), 0, s, p);
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(step_kernel, dim3(ntiles), dim3(256), 0, s, scenes, tiles);
     return hipGetLastError();
 }
 
-hipError_t launch_step(const StepParams &p, int tile_id, hipStream_t s)
+hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s)
 {
-    switch (tile_id) {
-    case 0: return launch_t<64, 4, 16>(p, s);
-    case 1: return launch_t<64, 4, 8>(p, s);
-    case 2: return launch_t<64, 8, 8>(p, s);
-    case 3: return launch_t<64, 2, 32>(p, s);
-    case 4: return launch_t<128, 2, 16>(p, s);
-    case 5: return launch_t<32, 8, 16>(p, s);
-    default: return launch_t<64, 4, 16>(p, s);
-    }
-}
-
-hipError_t launch_ghost(const GhostParams &p, hipStream_t s)
-{
-    hipLaunchKernelGGL(ghost_kernel, dim3(1), dim3(1024), 0, s, p);
+    if (ngroups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(64), 0, s, scenes, groups);
     return hipGetLastError();
 }
 
-hipError_t launch_listen(const ListenParams &p, hipStream_t s)
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s)
 {
-    if (p.L == 0) return hipSuccess;
-    hipLaunchKernelGGL(listen_kernel, dim3(p.L), dim3(64), 0, s, p);
-    return hipGetLastError();
-}
-
-hipError_t launch_advance_clock(long long *clock, long long by, hipStream_t s)
-{
-    hipLaunchKernelGGL(advance_clock_kernel, dim3(1), dim3(64), 0, s, clock, by);
-    return hipGetLastError();
-}
-
-hipError_t launch_set_clock(long long *clock, long long v, hipStream_t s)
-{
-    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(64), 0, s, clock, v);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ghost_kernel, dim3(n), dim3(1024), 0, s, scenes, list);
     return hipGetLastError();
 }
 

@@ -14,11 +14,11 @@ using namespace fdtd;
 
 namespace fdtd {
 const std::string &get_last_error();
-int graph_block_default() { return 48; }  // multiple of 6 = lcm(ring of 3, ping-pong of 2)
 }
 
 struct fdtd_handle {
     DeviceScene D;
+    StepProgram prog;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int tile = 0;
@@ -103,12 +103,23 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
     FDTD_HIP_TRY(hipGetDevice(&h->D.device));
     if (opt.stream) h->stream = (hipStream_t)opt.stream;
     else { FDTD_HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)); h->own_stream = true; }
-    h->tile = (opt.tile_config >= 0 && opt.tile_config < kNumTileConfigs) ? opt.tile_config : 0;
-    h->graph_steps = opt.steps_per_graph > 0 ? ((opt.steps_per_graph + 5) / 6) * 6 : 0;
+    h->tile = (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) ? opt.tile_config : 0;
+    h->graph_steps = std::max(0, opt.steps_per_graph);
     int cap = opt.series_capacity > 0 ? opt.series_capacity : 1024;
     cap = std::max(cap, h->graph_steps + 1);
-    st = h->D.allocate_and_upload(h->stream, cap);
+    st = h->D.allocate_and_upload(h->stream, cap, LLONG_MAX);
+    if (st == FDTD_OK) {
+        std::vector<TileDesc> tiles;
+        std::vector<ListenDesc> listen;
+        std::vector<GhostDesc> ghost;
+        h->D.append_tiles(0, tile_plan(h->tile), tiles);
+        xcd_permute(tiles);
+        h->D.append_listen(0, listen);
+        if (h->D.needs_ghost_kernel()) ghost.push_back({0});
+        st = h->prog.upload({h->D.dev}, tiles, listen, ghost, h->stream);
+    }
     if (st != FDTD_OK) {
+        h->prog.release();
         h->D.release();
         if (h->own_stream) (void)hipStreamDestroy(h->stream);
         return st;
@@ -131,6 +142,7 @@ fdtd_status fdtd_destroy(fdtd_handle *h)
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     if (h->call_begin) (void)hipEventDestroy(h->call_begin);
     if (h->call_end) (void)hipEventDestroy(h->call_end);
+    h->prog.release();
     h->D.release();
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -153,7 +165,7 @@ static fdtd_status flush_series(fdtd_handle *h)
         const long long ring = r % cap;
         const long long n = std::min<long long>(h->step - r, cap - ring);
         FDTD_HIP_TRY(hipMemcpy(h->host_series.data() + old + (size_t)(r - h->flushed) * L,
-                               h->D.series + (size_t)ring * L, (size_t)n * L * sizeof(double), hipMemcpyDeviceToHost));
+                               h->D.dev.series + (size_t)ring * L, (size_t)n * L * sizeof(double), hipMemcpyDeviceToHost));
         r += n;
     }
     h->flushed = h->step;
@@ -165,8 +177,7 @@ static fdtd_status build_graph(fdtd_handle *h)
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     fdtd_status st = FDTD_OK;
-    for (int o = 0; o < h->graph_steps && st == FDTD_OK; ++o)
-        st = h->D.enqueue_step((long long)o, h->D.clock, o, h->tile, h->stream);
+    for (int o = 0; o < h->graph_steps && st == FDTD_OK; ++o) st = h->prog.enqueue(h->stream);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (st != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return st; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
@@ -195,7 +206,7 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
     }
     int left = nsteps, done = 0;
     while (left > 0) {
-        const bool use_graph = !h->timing && h->graph_steps > 0 && left >= h->graph_steps && (h->step % 6) == 0;
+        const bool use_graph = !h->timing && h->graph_steps > 0 && left >= h->graph_steps;
         const int block = use_graph ? h->graph_steps : 1;
         if (h->step + block - h->flushed > cap) {
             fdtd_status st = flush_series(h);
@@ -206,12 +217,11 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
                 fdtd_status st = build_graph(h);
                 if (st != FDTD_OK) return st;
             }
-            FDTD_HIP_TRY(launch_set_clock(h->D.clock, h->step, h->stream));
             FDTD_HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
         } else {
             hipEvent_t b = nullptr, e = nullptr;
             if (h->timing) { b = h->ev[2 * done]; e = h->ev[2 * done + 1]; }
-            fdtd_status st = h->D.enqueue_step(h->step, nullptr, h->step, h->tile, h->stream, b, e);
+            fdtd_status st = h->prog.enqueue(h->stream, b, e);
             if (st != FDTD_OK) return st;
         }
         h->step += block;
@@ -287,7 +297,7 @@ fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out)
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const int ind = (int)((1 + h->step) % 3);  // p^{step}: the newest field
     std::vector<float> tmp((size_t)h->D.S.N);
-    FDTD_HIP_TRY(hipMemcpy(tmp.data(), h->D.p[ind], tmp.size() * 4, hipMemcpyDeviceToHost));
+    FDTD_HIP_TRY(hipMemcpy(tmp.data(), h->D.slot(SLOT_P0 + ind), tmp.size() * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < tmp.size(); ++i) out[i] = tmp[i];
     return FDTD_OK;
 }

@@ -1,0 +1,131 @@
+#!/usr/bin/env python3
+"""Benchmark of the FDTD time-stepping hot path on one MI355X.
+
+Workload (one "step" = one collocated time step of the whole grid):
+256^3 MAC grid, h = 2.5 mm, 16-cell PML, five rigid ellipsoids with seeded
+Gaussian-impulse boundary accelerations (rasterized ghost cells, FP64),
+128 listeners (FP64 MLS), FP32 fields -- the 256^3 scene of the engine's
+headline metric.  Inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line:
+  value        whole-job Gcell-updates/s = N * steps / wall time
+  roofline     dominant kernel (the fused step kernel): algorithmic bytes
+               (12 N + 44 N_pml per step) / mean kernel duration from HIP
+               events recorded on the kernel's stream during the timed region
+  cpu_baseline FP64 CPU oracle (OpenMP) on a bounded sample of the same scene
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "FDTD Gcell-updates/s (%HBM peak); served solves/s + p99 latency vs plain"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernel_256cube_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=256, help="grid edge (256 is the headline scene)")
+    ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    return ap.parse_args()
+
+
+def cpu_baseline(sb, steps):
+    """FP64 oracle on the same scene, timed over the stepping loop only."""
+    from oracle.oracle import Oracle, build
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if threads <= 0:
+        threads = min(16, os.cpu_count() or 1)
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    build()
+    o = Oracle(sb, reclassify_each_step=True)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    t = time.perf_counter() - t0
+    o.close()
+    n = sb.n_cells
+    return {"value": n * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{sb.divisions[0]}^3 headline scene, {steps} FP64 steps after 1 warmup step, "
+                      f"per-step reclassification as in the reference, {t:.2f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    from acoustics_amd import scenes
+    from acoustics_amd.solver import Solver
+
+    n = a.n
+    sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n)
+    s = Solver(sb, tile_config=a.tile, series_capacity=max(4096, a.steps + a.warmup + 8))
+    info = s.info()
+    N, Npml = info["n_cells"], info["n_pml_cells"]
+    bytes_step = info["bytes_per_step"]
+    # warmup (eager + timing path identical to the timed one)
+    s.set_timing(True)
+    s.step(a.warmup)
+    s.synchronize()
+    # timed region: K steps, events around every step kernel on its stream
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms, step_ms = s.last_kernel_ms()
+    series = s.series()
+    finite = bool((series == series).all())
+    s.close()
+
+    value = N * a.steps / wall / 1e9
+    achieved = bytes_step / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            tf = json.load(open(TRAFFIC_FILE))
+            if tf.get("n") == n:
+                traffic = tf.get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields; f64 ghost cells and listeners",
+        "data": "synthetic (seeded scene)",
+        "config": {"workload": f"{n}^3 MAC grid, h={0.64 / n * 1e3:.2f} mm, 16-cell PML, 5 rigid ellipsoids "
+                               f"(rasterized ghost cells), 128 listeners, collocated scheme",
+                   "cells": N, "pml_cells": Npml, "bytes_per_step": bytes_step, "listeners_finite": finite},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5), "step_ms_events": round(step_ms, 5)},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sb, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

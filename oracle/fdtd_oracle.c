@@ -1251,6 +1251,12 @@ int oracle_step(oracle *o, int nsteps, double *rows)
 
 void oracle_get_pressure(const oracle *o, double *out) { memcpy(out, o->pc[o->pc_ind], sizeof(double) * o->N); }
 
+void oracle_set_state(oracle *o, const double *p_last, const double *p_curr)
+{
+    memcpy(o->pc[(o->pc_ind + 2) % 3], p_last, sizeof(double) * o->N);
+    memcpy(o->pc[o->pc_ind], p_curr, sizeof(double) * o->N);
+}
+
 void oracle_get_pml_state(const oracle *o, double *pdir, double *v)
 {
     if (pdir) for (int d = 0; d < 3; ++d) memcpy(pdir + d * o->N, o->pdir[d], sizeof(double) * o->N);

@@ -42,6 +42,8 @@ void    oracle_free(oracle *o);
 /* Advances nsteps steps; rows (optional) receives [nsteps][L] listener values. */
 int     oracle_step(oracle *o, int nsteps, double *rows);
 void    oracle_get_pressure(const oracle *o, double *out);          /* newest p, N */
+/* Overwrites p^{n-1} and p^n (N values each) before the next step. */
+void    oracle_set_state(oracle *o, const double *p_last, const double *p_curr);
 void    oracle_get_pml_state(const oracle *o, double *pdir /*3N*/, double *v /*3 faces*/);
 /* counts: [0] N, [1] N_pml cells, [2] N_pml faces, [3] ghost entries,
  *         [4] solid cells, [5] bulk cells, [6] cavity cells,

@@ -39,6 +39,7 @@ def lib():
         L.oracle_step.argtypes = [C.c_void_p, C.c_int, D]
         L.oracle_get_pressure.argtypes = [C.c_void_p, D]
         L.oracle_get_pml_state.argtypes = [C.c_void_p, D, D]
+        L.oracle_set_state.argtypes = [C.c_void_p, D, D]
         L.oracle_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.oracle_time.restype = C.c_double
         L.oracle_time.argtypes = [C.c_void_p]
@@ -88,6 +89,12 @@ class Oracle:
         out = np.zeros(self.sb.n_cells, dtype=np.float64)
         self.L.oracle_get_pressure(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out.reshape(self.sb.divisions)
+
+    def set_state(self, p_last, p_curr):
+        a = np.ascontiguousarray(p_last, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(p_curr, dtype=np.float64).reshape(-1)
+        assert a.size == self.sb.n_cells and b.size == self.sb.n_cells
+        self.L.oracle_set_state(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)))
 
     def counts(self):
         c = (C.c_longlong * 9)()
