@@ -25,6 +25,7 @@ SOURCES = [
     ("device_scene.cpp", True),
     ("solver.cpp", True),
     ("batch.cpp", True),
+    ("server.cpp", True),
     ("setup.cpp", False),
     ("geometry.cpp", False),
     ("xml.cpp", False),

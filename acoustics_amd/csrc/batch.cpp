@@ -1,6 +1,154 @@
-// batch.cpp -- placeholder entry points for batches of scenes (replaced by
-// the persistent batched path).
-#include "device_scene.h"
+// batch.cpp -- many independent scenes advanced together: one work list of
+// tiles over every scene, one set of step launches per time step for the
+// whole batch, captured as hipGraph step blocks.  Scenes carry their own
+// device step counters and targets, so uneven grids and uneven step counts
+// share the launches; finished scenes are dropped from the work list at the
+// next block boundary.  The tile plan is chosen by an autotuner that times
+// every plan on the batch itself before the first step.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+
+#include "batch_internal.h"
+
+namespace fdtd {
+
+fdtd_status BatchRunner::rebuild(hipStream_t st)
+{
+    drop_graph();
+    std::vector<SceneDev> devs;
+    std::vector<TileDesc> tiles;
+    std::vector<ListenDesc> listen;
+    std::vector<GhostDesc> ghost;
+    members.clear();
+    for (size_t i = 0; i < scenes.size(); ++i) {
+        if (!scenes[i] || done[i]) continue;
+        const int id = (int)devs.size();
+        devs.push_back(scenes[i]->dev);
+        scenes[i]->append_tiles(id, tile_plan(plan), tiles);
+        scenes[i]->append_listen(id, listen);
+        if (scenes[i]->needs_ghost_kernel()) ghost.push_back({id});
+        members.push_back((int)i);
+    }
+    if (devs.empty()) { prog.release(); return FDTD_OK; }
+    return prog.upload(devs, tiles, listen, ghost, st);
+}
+
+void BatchRunner::drop_graph()
+{
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+}
+
+fdtd_status BatchRunner::build_graph(hipStream_t st)
+{
+    drop_graph();
+    hipGraph_t g = nullptr;
+    FDTD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    fdtd_status s = FDTD_OK;
+    for (int k = 0; k < graph_steps && s == FDTD_OK; ++k) s = prog.enqueue(st);
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
+    if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { set_last_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    return FDTD_OK;
+}
+
+// Times each tile plan's three step kernels on the batch at step 0 and keeps
+// the fastest; the fields are cleared afterwards (the clocks never moved).
+fdtd_status BatchRunner::autotune(hipStream_t st)
+{
+    float best = 1e30f;
+    int best_plan = 0;
+    hipEvent_t a, b;
+    FDTD_HIP_TRY(hipEventCreate(&a));
+    FDTD_HIP_TRY(hipEventCreate(&b));
+    for (int p = 0; p < kNumTilePlans; ++p) {
+        plan = p;
+        fdtd_status s = rebuild(st);
+        if (s != FDTD_OK) return s;
+        auto run = [&]() -> fdtd_status {
+            for (int k = 0; k < NUM_KINDS; ++k)
+                FDTD_HIP_TRY(launch_step(k, prog.d_scenes, prog.d_tiles[k], prog.ntiles[k], st));
+            return FDTD_OK;
+        };
+        if ((s = run()) != FDTD_OK) return s;
+        FDTD_HIP_TRY(hipEventRecord(a, st));
+        for (int r = 0; r < 3; ++r)
+            if ((s = run()) != FDTD_OK) return s;
+        FDTD_HIP_TRY(hipEventRecord(b, st));
+        FDTD_HIP_TRY(hipEventSynchronize(b));
+        float ms = 0;
+        FDTD_HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        timings[p] = ms / 3.0f;
+        if (ms < best) { best = ms; best_plan = p; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    plan = best_plan;
+    for (auto &sc : scenes)
+        if (sc) FDTD_HIP_TRY(hipMemsetAsync(sc->dev.field, 0, sizeof(float) * (size_t)NUM_SLOTS * (size_t)sc->npad, st));
+    FDTD_HIP_TRY(hipStreamSynchronize(st));
+    return rebuild(st);
+}
+
+fdtd_status BatchRunner::advance(hipStream_t st, int nsteps)
+{
+    int left = nsteps;
+    while (left > 0) {
+        bool changed = false;
+        long long remaining = 0;
+        for (size_t i = 0; i < scenes.size(); ++i) {
+            if (!scenes[i] || done[i]) continue;
+            if (taken >= targets[i]) { done[i] = 1; changed = true; continue; }
+            remaining = std::max(remaining, targets[i] - taken);
+        }
+        if (remaining == 0) break;
+        if (changed) {
+            fdtd_status s = rebuild(st);
+            if (s != FDTD_OK) return s;
+        }
+        const int want = (int)std::min<long long>(left, remaining);
+        if (graph_steps > 0 && want >= graph_steps) {
+            if (!exec) {
+                fdtd_status s = build_graph(st);
+                if (s != FDTD_OK) return s;
+            }
+            FDTD_HIP_TRY(hipGraphLaunch(exec, st));
+            taken += graph_steps;
+            left -= graph_steps;
+            ++blocks;
+        } else {
+            fdtd_status s = prog.enqueue(st);
+            if (s != FDTD_OK) return s;
+            taken += 1;
+            left -= 1;
+        }
+    }
+    return FDTD_OK;
+}
+
+BatchRunner::~BatchRunner()
+{
+    drop_graph();
+    prog.release();
+}
+
+}  // namespace fdtd
+
+using namespace fdtd;
+
+struct fdtd_batch {
+    BatchRunner R;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+};
 
 extern "C" {
 
@@ -8,42 +156,95 @@ void fdtd_batch_options_default(fdtd_batch_options *o)
 {
     if (!o) return;
     o->stream = nullptr;
-    o->steps_per_graph = 0;
+    o->steps_per_graph = 32;
     o->tile_config = -1;
     o->persistent_groups = 0;
 }
-fdtd_status fdtd_batch_create(const fdtd_scene *, const int *, int, const fdtd_batch_options *, fdtd_batch **out)
-{
-    if (out) *out = nullptr;
-    fdtd::set_last_error("batches are not built yet");
-    return FDTD_ERR_UNSUPPORTED;
-}
-fdtd_status fdtd_batch_run(fdtd_batch *) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_batch_step(fdtd_batch *, int) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_batch_steps_done(fdtd_batch *, int, int64_t *) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_batch_read_listener_series(fdtd_batch *, int, double *) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_batch_destroy(fdtd_batch *) { return FDTD_ERR_UNSUPPORTED; }
 
-void fdtd_server_options_default(fdtd_server_options *o)
+fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, int n, const fdtd_batch_options *opt_in,
+                              fdtd_batch **out)
 {
-    if (!o) return;
-    o->hbm_budget_bytes = 0;
-    o->max_active = 0;
-    o->steps_per_block = 0;
-    o->device = -1;
+    if (!scenes || !num_steps || n <= 0 || !out) { set_last_error("bad batch arguments"); return FDTD_ERR_INVALID; }
+    *out = nullptr;
+    fdtd_batch_options opt;
+    fdtd_batch_options_default(&opt);
+    if (opt_in) opt = *opt_in;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_last_error("no HIP device"); return FDTD_ERR_NO_DEVICE; }
+    std::unique_ptr<fdtd_batch> b(new (std::nothrow) fdtd_batch());
+    if (!b) return FDTD_ERR_OOM;
+    FDTD_HIP_TRY(hipGetDevice(&b->device));
+    if (opt.stream) b->stream = (hipStream_t)opt.stream;
+    else { FDTD_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); b->own_stream = true; }
+    BatchRunner &R = b->R;
+    R.graph_steps = std::max(0, opt.steps_per_graph);
+    for (int i = 0; i < n; ++i) {
+        if (num_steps[i] < 0) { set_last_error("negative step count"); return FDTD_ERR_INVALID; }
+        auto ds = std::make_unique<DeviceScene>();
+        std::string err;
+        fdtd_status st = build_setup(scenes[i], ds->S, err);
+        if (st != FDTD_OK) { set_last_error("scene " + std::to_string(i) + ": " + err); return st; }
+        st = ds->allocate_and_upload(b->stream, std::max(1, num_steps[i]), num_steps[i]);
+        if (st != FDTD_OK) { ds->release(); R.release_scenes(); return st; }
+        R.scenes.push_back(std::move(ds));
+        R.targets.push_back(num_steps[i]);
+        R.done.push_back(num_steps[i] == 0);
+    }
+    fdtd_status st;
+    if (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) { R.plan = opt.tile_config; st = R.rebuild(b->stream); }
+    else st = R.autotune(b->stream);
+    if (st != FDTD_OK) { R.release_scenes(); return st; }
+    *out = b.release();
+    return FDTD_OK;
 }
-fdtd_status fdtd_server_create(const fdtd_server_options *, fdtd_server **out)
+
+fdtd_status fdtd_batch_step(fdtd_batch *b, int nsteps)
 {
-    if (out) *out = nullptr;
-    return FDTD_ERR_UNSUPPORTED;
+    if (!b || nsteps < 0) return FDTD_ERR_INVALID;
+    FDTD_HIP_TRY(hipSetDevice(b->device));
+    return b->R.advance(b->stream, nsteps);
 }
-fdtd_status fdtd_server_submit(fdtd_server *, const fdtd_scene *, int, int64_t *) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_server_poll(fdtd_server *, int64_t) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_server_wait(fdtd_server *, int64_t, int) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_server_take_result(fdtd_server *, int64_t, double *, double *) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_server_cancel(fdtd_server *, int64_t) { return FDTD_ERR_UNSUPPORTED; }
-fdtd_status fdtd_server_get_stats(fdtd_server *, fdtd_server_stats *) { return FDTD_ERR_UNSUPPORTED; }
-int64_t fdtd_scene_device_bytes(const fdtd_scene *) { return -1; }
-fdtd_status fdtd_server_destroy(fdtd_server *) { return FDTD_ERR_UNSUPPORTED; }
+
+fdtd_status fdtd_batch_run(fdtd_batch *b)
+{
+    if (!b) return FDTD_ERR_INVALID;
+    long long mx = 0;
+    for (long long t : b->R.targets) mx = std::max(mx, t);
+    fdtd_status st = fdtd_batch_step(b, (int)std::max<long long>(0, mx - b->R.taken));
+    if (st != FDTD_OK) return st;
+    FDTD_HIP_TRY(hipStreamSynchronize(b->stream));
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_batch_steps_done(fdtd_batch *b, int scene, int64_t *steps)
+{
+    if (!b || !steps || scene < 0 || scene >= (int)b->R.scenes.size()) return FDTD_ERR_INVALID;
+    *steps = std::min<long long>(b->R.taken, b->R.targets[scene]);
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *out)
+{
+    if (!b || scene < 0 || scene >= (int)b->R.scenes.size()) return FDTD_ERR_INVALID;
+    FDTD_HIP_TRY(hipStreamSynchronize(b->stream));
+    const DeviceScene &D = *b->R.scenes[scene];
+    const long long rows = std::min<long long>(b->R.taken, b->R.targets[scene]);
+    if (rows > 0 && D.S.L > 0)
+        FDTD_HIP_TRY(hipMemcpy(out, D.dev.series, sizeof(double) * (size_t)rows * D.S.L, hipMemcpyDeviceToHost));
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_batch_destroy(fdtd_batch *b)
+{
+    if (!b) return FDTD_ERR_INVALID;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    b->R.drop_graph();
+    b->R.prog.release();
+    b->R.release_scenes();
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FDTD_OK;
+}
 
 }  // extern "C"

@@ -1,0 +1,39 @@
+// batch_internal.h -- the multi-scene step driver shared by batches and the
+// serving loop.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "device_scene.h"
+
+namespace fdtd {
+
+struct BatchRunner {
+    std::vector<std::unique_ptr<DeviceScene>> scenes;  // slot i (nullptr: freed)
+    std::vector<long long> targets;                   // steps each scene takes
+    std::vector<char> done;
+    std::vector<int> members;                         // program scene id -> slot
+    StepProgram prog;
+    int plan = 0;
+    int graph_steps = 32;
+    hipGraphExec_t exec = nullptr;
+    long long taken = 0;                              // batch steps launched
+    long long blocks = 0;
+    float timings[kNumTilePlans] = {0, 0, 0, 0, 0, 0};
+
+    fdtd_status rebuild(hipStream_t st);              // work list of unfinished scenes
+    fdtd_status build_graph(hipStream_t st);
+    void drop_graph();
+    fdtd_status autotune(hipStream_t st);
+    fdtd_status advance(hipStream_t st, int nsteps);
+    void release_scenes()
+    {
+        for (auto &s : scenes)
+            if (s) s->release();
+        scenes.clear();
+    }
+    ~BatchRunner();
+};
+
+}  // namespace fdtd

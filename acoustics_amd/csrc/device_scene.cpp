@@ -2,6 +2,7 @@
 // per-step launch sequence.
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 
 #include "device_scene.h"
@@ -41,6 +42,8 @@ struct Layout {
     int32_t *lptr, *lkind, *lindex;
     double *lbeta, *series;
     unsigned long long *clock;
+    double *acc_now;
+    float *src_now;
 };
 
 long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
@@ -73,9 +76,30 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.lbeta = c.take<double>(S.lst_terms.size());
     L.series = c.take<double>((size_t)cap * (size_t)(S.L ? S.L : 1));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
+    L.acc_now = c.take<double>(S.objs.size());
+    L.src_now = c.take<float>(S.psrc.size());
 }
 
 }  // namespace
+
+void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
+{
+    const double tn = S.t_start + (double)s * S.dt;
+    for (size_t o = 0; o < S.objs.size(); ++o) {
+        double a = 0.0;
+        for (const ImpulseHost &q : S.isrc) {
+            if (q.obj != (int)o) continue;
+            const double u = tn - q.ton;
+            a += q.A * std::exp(-(u * u) / (2.0 * q.w * q.w));
+        }
+        acc[o] = a;
+    }
+    for (size_t q = 0; q < S.psrc.size(); ++q) {
+        const PSrcHost &p = S.psrc[q];
+        const double u = (tn + 0.5 * S.dt - p.t0) / p.wt;
+        src[q] = (float)(std::exp(-(u * u) / 2.0) * p.amp * S.dt);
+    }
+}
 
 TilePlan tile_plan(int id)
 {
@@ -173,6 +197,11 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     FDTD_HIP_TRY(up(L.lbeta, beta.data(), beta.size() * 8));
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
+    std::vector<double> acc0(S.objs.size());
+    std::vector<float> src0(S.psrc.size());
+    step_amplitudes(S, 0, acc0.data(), src0.data());
+    FDTD_HIP_TRY(up(L.acc_now, acc0.data(), acc0.size() * sizeof(double)));
+    FDTD_HIP_TRY(up(L.src_now, src0.data(), src0.size() * sizeof(float)));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
 
     SceneDev &D = dev;
@@ -210,6 +239,9 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.series = L.series;
     D.clock = L.clock;
     D.n_listen_groups = S.L > 0 ? S.L : 1;
+    D.n_obj = (int)S.objs.size();
+    D.acc_now = L.acc_now;
+    D.src_now = L.src_now;
     return FDTD_OK;
 }
 
@@ -323,24 +355,47 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
                                 const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st)
 {
     release();
-    const size_t b0 = align_up(sizeof(SceneDev) * std::max<size_t>(1, scenes.size()));
-    const size_t b1 = align_up(sizeof(TileDesc) * std::max<size_t>(1, tiles.size()));
-    const size_t b2 = align_up(sizeof(ListenDesc) * std::max<size_t>(1, listen.size()));
-    const size_t b3 = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
+    std::vector<TileDesc> by_kind[NUM_KINDS];
+    for (const TileDesc &t : tiles) {
+        if (t.type == TILE_EMPTY) continue;
+        const int k = t.type == TILE_PML ? KIND_PML : t.type == TILE_SPECIAL ? KIND_SPECIAL : KIND_INTERIOR;
+        by_kind[k].push_back(t);
+    }
+    // neighbouring tiles of a scene share an XCD (and its L2)
+    xcd_permute(by_kind[KIND_INTERIOR]);
+    xcd_permute(by_kind[KIND_PML]);
+    size_t bytes[4 + NUM_KINDS];
+    bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, scenes.size()));
+    bytes[1] = align_up(sizeof(ListenDesc) * std::max<size_t>(1, listen.size()));
+    bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
+    bytes[3] = 0;
+    size_t total = bytes[0] + bytes[1] + bytes[2];
+    for (int k = 0; k < NUM_KINDS; ++k) {
+        bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
+        total += bytes[4 + k];
+    }
     char *base = nullptr;
-    FDTD_HIP_TRY(hipMalloc(&base, b0 + b1 + b2 + b3));
+    FDTD_HIP_TRY(hipMalloc(&base, total));
     d_scenes = reinterpret_cast<SceneDev *>(base);
-    d_tiles = reinterpret_cast<TileDesc *>(base + b0);
-    d_listen = reinterpret_cast<ListenDesc *>(base + b0 + b1);
-    d_ghost = reinterpret_cast<GhostDesc *>(base + b0 + b1 + b2);
-    ntiles = (int)tiles.size();
+    d_listen = reinterpret_cast<ListenDesc *>(base + bytes[0]);
+    d_ghost = reinterpret_cast<GhostDesc *>(base + bytes[0] + bytes[1]);
+    size_t off = bytes[0] + bytes[1] + bytes[2];
+    for (int k = 0; k < NUM_KINDS; ++k) {
+        d_tiles[k] = reinterpret_cast<TileDesc *>(base + off);
+        off += bytes[4 + k];
+        ntiles[k] = (int)by_kind[k].size();
+        if (ntiles[k])
+            FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
+    }
     nlisten = (int)listen.size();
     nghost = (int)ghost.size();
     if (!scenes.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice, st));
-    if (!tiles.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(TileDesc) * tiles.size(), hipMemcpyHostToDevice, st));
     if (!listen.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_listen, listen.data(), sizeof(ListenDesc) * listen.size(), hipMemcpyHostToDevice, st));
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
+    FDTD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    FDTD_HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    FDTD_HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     return FDTD_OK;
 }
 
@@ -348,17 +403,33 @@ void StepProgram::release()
 {
     if (d_scenes) (void)hipFree(d_scenes);
     d_scenes = nullptr;
-    d_tiles = nullptr;
+    for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; }
     d_listen = nullptr;
     d_ghost = nullptr;
-    ntiles = nlisten = nghost = 0;
+    nlisten = nghost = 0;
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
+    fork = join = nullptr;
+    side = nullptr;
 }
 
 fdtd_status StepProgram::enqueue(hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
-    FDTD_HIP_TRY(launch_step(d_scenes, d_tiles, ntiles, st));
+    const bool split = side && ntiles[KIND_INTERIOR] > 0 && (ntiles[KIND_PML] + ntiles[KIND_SPECIAL]) > 0;
+    if (split) {
+        FDTD_HIP_TRY(hipEventRecord(fork, st));
+        FDTD_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+        FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], side));
+        FDTD_HIP_TRY(hipEventRecord(join, side));
+    } else {
+        FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], st));
+    }
+    FDTD_HIP_TRY(launch_step(KIND_PML, d_scenes, d_tiles[KIND_PML], ntiles[KIND_PML], st));
+    FDTD_HIP_TRY(launch_step(KIND_SPECIAL, d_scenes, d_tiles[KIND_SPECIAL], ntiles[KIND_SPECIAL], st));
+    if (split) FDTD_HIP_TRY(hipStreamWaitEvent(st, join, 0));
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
     FDTD_HIP_TRY(launch_listen(d_scenes, d_listen, nlisten, st));
     return FDTD_OK;

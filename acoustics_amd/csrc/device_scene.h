@@ -58,17 +58,25 @@ struct DeviceScene {
 // w % 8) runs tile chunk (w % 8), keeping neighbouring tiles on one L2.
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 
-// The per-step launch sequence of a set of scenes (ghost solve, fused step,
-// listeners + step advance), uploaded once.
+// The per-step launch sequence of a set of scenes, uploaded once:
+//   ghost solve (fully-coupled scenes) -> {PML + object cells on the main
+//   stream || interior on the side stream} -> listeners + step advance.
+// The three step kernels write disjoint cells; the fork/join lets them
+// overlap (and is captured as parallel branches of a graph).
 struct StepProgram {
     SceneDev *d_scenes = nullptr;
-    TileDesc *d_tiles = nullptr;
+    TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
     ListenDesc *d_listen = nullptr;
     GhostDesc *d_ghost = nullptr;
-    int ntiles = 0, nlisten = 0, nghost = 0;
+    int ntiles[NUM_KINDS] = {0, 0, 0};
+    int nlisten = 0, nghost = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    // tiles of every kind in one list; split by type here
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                        const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st);
     void release();
+    // ev_begin/ev_end bracket the three step kernels on st
     fdtd_status enqueue(hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
 };
 

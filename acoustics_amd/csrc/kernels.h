@@ -67,8 +67,15 @@ struct SceneDev {
     double *series;             // [cap][L]
     unsigned long long *clock;  // CLK_WORDS words
     int n_listen_groups;        // workgroups of the listener launch for this scene
-    int pad;
+    int n_obj;
+    // time-dependent amplitudes of the current step, refreshed by the last
+    // listener workgroup when it advances the step counter:
+    double *acc_now;            // [n_obj] normal acceleration a_n(obj, t_n)
+    float *src_now;             // [n_psrc] S(t_n + dt/2) dt without the spatial factor
 };
+
+// Amplitudes of step s on the host (used to seed step 0).
+void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src);
 
 enum : int { TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4 };
 
@@ -89,7 +96,8 @@ struct GhostDesc {
     int scene;
 };
 
-hipError_t launch_step(const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s);
+enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3 };
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s);
 hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s);
 hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s);
 

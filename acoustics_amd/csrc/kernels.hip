@@ -54,37 +54,34 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
     const float lam2 = S.lam2f;
 
-    constexpr int kMaxSrc = 2;
-    float sjk[kMaxSrc] = {0.f, 0.f};
-    int soff[kMaxSrc] = {0, 0};
-    int nsrc = 0;
-    if (t.src) {
-        const double tn = S.t_start + (double)step * S.dt;
-        nsrc = min(S.n_psrc, kMaxSrc);
-        for (int s = 0; s < nsrc; ++s) {
-            const PSrcDev ps = S.psrc[s];
-            sjk[s] = source_amp(ps, tn, S.dt) * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
-            soff[s] = ps.goff[0];
-        }
-    }
-
     int idx = t.i0 * sx + j * sy + k;
     float pm = pcur[idx - sx];
     float pc = pcur[idx];
+    if (!t.src && !skip_check) {
+        // the streaming case: 12 bytes of HBM traffic per cell
+        for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+            const float pp = pcur[idx + sx];
+            const float ym = pcur[idx - sy], yp = pcur[idx + sy];
+            const float zm = pcur[idx - 1], zp = pcur[idx + 1];
+            const float prev = pprev[idx];
+            pnext[idx] = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
+            pm = pc;
+            pc = pp;
+        }
+        return;
+    }
+    // tiles with pressure sources and/or object cells to leave alone
     for (int i = t.i0; i < t.i1; ++i, idx += sx) {
         const float pp = pcur[idx + sx];
         const float ym = pcur[idx - sy], yp = pcur[idx + sy];
         const float zm = pcur[idx - 1], zp = pcur[idx + 1];
         const float prev = pprev[idx];
         float pn = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
-        for (int s = 0; s < nsrc; ++s) pn += sjk[s] * S.gtab[soff[s] + i];
-        if (t.src && S.n_psrc > kMaxSrc) {
-            const double tn = S.t_start + (double)step * S.dt;
-            for (int s = kMaxSrc; s < S.n_psrc; ++s) {
+        if (t.src)
+            for (int s = 0; s < S.n_psrc; ++s) {
                 const PSrcDev ps = S.psrc[s];
-                pn += source_amp(ps, tn, S.dt) * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
+                pn += S.src_now[s] * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
             }
-        }
         if (!skip_check || !S.skip[idx]) pnext[idx] = pn;
         pm = pc;
         pc = pp;
@@ -174,7 +171,6 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
     const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
     const float *__restrict__ pprev = S.field + (size_t)((ind + 2) % 3) * S.npad;
     float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
-    const double tn = S.t_start + (double)step * S.dt;
     const int idx = rec.cell;
     const double c = pcur[idx];
     const double nb[6] = {pcur[idx - sx], pcur[idx + sx], pcur[idx - sy], pcur[idx + sy], pcur[idx - 1], pcur[idx + 1]};
@@ -185,7 +181,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         if (gid >= 0) {
             if (S.raster) {
                 const GhostDev gd = S.ghosts[gid];
-                g[q] = c + gd.k_raster * impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
+                g[q] = c + gd.k_raster * S.acc_now[gd.obj];
             } else {
                 g[q] = S.gval[step & 1][gid];
             }
@@ -209,12 +205,16 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         const int i = idx / sx, j = (idx - i * sx) / sy, k = idx - i * sx - j * sy;
         for (int s = 0; s < S.n_psrc; ++s) {
             const PSrcDev ps = S.psrc[s];
-            pn += (double)(source_amp(ps, tn, S.dt) * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k]);
+            pn += (double)(S.src_now[s] * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k]);
         }
     }
     pnext[idx] = (float)pn;
 }
 
+// One kernel per tile class, so each class gets its own register allocation
+// (the FP64 object path would otherwise cap the occupancy of the streaming
+// paths).  The three classes write disjoint cells and can run concurrently.
+template <int KIND>
 __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles)
 {
     const TileDesc t = tiles[blockIdx.x];
@@ -222,13 +222,9 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
     const SceneDev &S = scenes[t.scene];
     const long long step = (long long)S.clock[CLK_STEP];
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    switch (t.type) {
-    case TILE_INTERIOR: interior_tile(S, t, step, false); break;
-    case TILE_INTERIOR_SKIP: interior_tile(S, t, step, true); break;
-    case TILE_PML: pml_tile(S, t, step); break;
-    case TILE_SPECIAL: special_tile(S, t, step); break;
-    default: break;
-    }
+    if constexpr (KIND == KIND_INTERIOR) interior_tile(S, t, step, t.type == TILE_INTERIOR_SKIP);
+    else if constexpr (KIND == KIND_PML) pml_tile(S, t, step);
+    else special_tile(S, t, step);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,14 +236,13 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
     const SceneDev &S = scenes[list[blockIdx.x].scene];
     const long long step = (long long)S.clock[CLK_STEP];
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const double tn = S.t_start + (double)step * S.dt;
     const int ind = (int)((1 + step) % 3);
     const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
     double *gval = S.gval[step & 1];
     double *b = S.gtmp, *x1 = S.gtmp + S.G;
     for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
         const GhostDev gd = S.ghosts[g];
-        const double a = impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
+        const double a = S.acc_now[gd.obj];
         double v;
         if (gd.coupled) {
             v = gd.c_acc * a;
@@ -289,7 +284,6 @@ __global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__
     const int lane = threadIdx.x;
     if (gdsc.l >= 0) {
         const int l = gdsc.l;
-        const double tn = S.t_start + (double)step * S.dt;
         const int ind = (int)((1 + step) % 3);
         const float *__restrict__ pold = S.field + (size_t)ind * S.npad;
         const float *__restrict__ pnew = S.field + (size_t)((ind + 1) % 3) * S.npad;
@@ -301,7 +295,7 @@ __global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__
                 v = (double)pnew[idx];
             } else if (S.raster) {
                 const GhostDev gd = S.ghosts[idx];
-                v = (double)pold[gd.nbr] + gd.k_raster * impulse_accel(S.isrc, S.n_isrc, gd.obj, tn);
+                v = (double)pold[gd.nbr] + gd.k_raster * S.acc_now[gd.obj];
             } else {
                 v = S.gval[step & 1][idx];
             }
@@ -310,20 +304,34 @@ __global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if (lane == 0) S.series[(size_t)(step % S.cap) * S.L + l] = acc;
     }
+    // every group of the scene has read this step's amplitudes before it
+    // counts itself done, so the last one may overwrite them
+    unsigned long long prev = 0;
     if (lane == 0) {
         __threadfence();
-        const unsigned long long prev = atomicAdd(&clk[CLK_DONE], 1ULL);
-        if (prev + 1 == (unsigned long long)S.n_listen_groups) {
+        prev = atomicAdd(&clk[CLK_DONE], 1ULL);
+    }
+    prev = __shfl(prev, 0, 64);
+    if (prev + 1 == (unsigned long long)S.n_listen_groups) {
+        const double tn1 = S.t_start + (double)(step + 1) * S.dt;
+        for (int o = lane; o < S.n_obj; o += 64) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
+        for (int s = lane; s < S.n_psrc; s += 64) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
+        __threadfence();
+        if (lane == 0) {
             atomicExch(&clk[CLK_DONE], 0ULL);
             atomicExch(&clk[CLK_STEP], (unsigned long long)(step + 1));
         }
     }
 }
 
-hipError_t launch_step(const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s)
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(step_kernel, dim3(ntiles), dim3(256), 0, s, scenes, tiles);
+    switch (kind) {
+    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
+    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
+    default: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
+    }
     return hipGetLastError();
 }
 

@@ -1,0 +1,341 @@
+// server.cpp -- continuous batching of impact-sound solve requests on one
+// GPU.  A worker thread owns the device: between step blocks it admits
+// queued requests while their device state fits the HBM budget (and a slot
+// cap), uploads them, rebuilds the batched work list, runs one step block
+// for every resident request (requests admitted at different times are at
+// different steps -- each carries its own device step counter), and retires
+// finished requests (copy the FP64 listener series, check it is finite, free
+// the memory).  Requests arriving while others run join at the next block.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+
+#include "device_scene.h"
+
+using namespace fdtd;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int64_t id = 0;
+    SceneSetup setup;             // built on the submitting thread
+    int steps = 0;
+    size_t bytes = 0;             // device bytes when admitted
+    Clock::time_point submitted, finished;
+    fdtd_status status = FDTD_ERR_BUSY;
+    bool cancel = false;
+    std::vector<double> result;   // [steps][L]
+    std::unique_ptr<DeviceScene> dev;
+    long long launched = 0;       // steps launched since admission
+};
+
+}  // namespace
+
+struct fdtd_server {
+    fdtd_server_options opt{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv, done_cv;
+    std::deque<int64_t> queue;
+    std::map<int64_t, std::unique_ptr<Request>> reqs;
+    std::vector<Request *> active;
+    int64_t next_id = 1;
+    bool stop = false;
+    fdtd_server_stats stats{};
+    int64_t resident = 0;
+    // step program of the resident set
+    StepProgram prog;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    int block = 64;
+
+    void run();
+    fdtd_status rebuild();
+    void finish(Request *r, fdtd_status st);
+};
+
+void fdtd_server::finish(Request *r, fdtd_status st)
+{
+    r->status = st;
+    r->finished = Clock::now();
+    if (r->dev) { r->dev->release(); r->dev.reset(); }
+    resident -= (int64_t)r->bytes;
+    if (st == FDTD_OK) ++stats.completed;
+    else if (st == FDTD_ERR_CANCELLED) ++stats.cancelled;
+    else ++stats.failed;
+}
+
+fdtd_status fdtd_server::rebuild()
+{
+    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    prog.release();
+    if (active.empty()) return FDTD_OK;
+    std::vector<SceneDev> devs;
+    std::vector<TileDesc> tiles;
+    std::vector<ListenDesc> listen;
+    std::vector<GhostDesc> ghost;
+    for (Request *r : active) {
+        const int id = (int)devs.size();
+        devs.push_back(r->dev->dev);
+        r->dev->append_tiles(id, tile_plan(0), tiles);
+        r->dev->append_listen(id, listen);
+        if (r->dev->needs_ghost_kernel()) ghost.push_back({id});
+    }
+    fdtd_status s = prog.upload(devs, tiles, listen, ghost, stream);
+    if (s != FDTD_OK) return s;
+    hipGraph_t g = nullptr;
+    FDTD_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < block && s == FDTD_OK; ++k) s = prog.enqueue(stream);
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
+    if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { set_last_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
+    return FDTD_OK;
+}
+
+void fdtd_server::run()
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        std::vector<Request *> admit;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (active.empty())
+                cv.wait(lk, [&] { return stop || !queue.empty(); });
+            if (stop) break;
+            // cancellations of resident requests take effect at the block boundary
+            for (Request *r : active)
+                if (r->cancel) { finish(r, FDTD_ERR_CANCELLED); dirty = true; }
+            active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->status != FDTD_ERR_BUSY; }),
+                         active.end());
+            // admission by HBM budget and slot cap, first come first served
+            while (!queue.empty()) {
+                Request *r = reqs.at(queue.front()).get();
+                if (r->cancel) {
+                    queue.pop_front();
+                    resident += (int64_t)r->bytes;  // finish() returns the (never used) reservation
+                    finish(r, FDTD_ERR_CANCELLED);
+                    continue;
+                }
+                if (opt.max_active > 0 && (int)(active.size() + admit.size()) >= opt.max_active) break;
+                if (resident + (int64_t)r->bytes > opt.hbm_budget_bytes) break;
+                queue.pop_front();
+                resident += (int64_t)r->bytes;
+                admit.push_back(r);
+            }
+            stats.queued = (int64_t)queue.size();
+            stats.peak_resident_bytes = std::max(stats.peak_resident_bytes, resident);
+        }
+        for (Request *r : admit) {
+            r->dev = std::make_unique<DeviceScene>();
+            r->dev->S = std::move(r->setup);
+            fdtd_status s = r->dev->allocate_and_upload(stream, std::max(1, r->steps), r->steps);
+            std::lock_guard<std::mutex> lk(mu);
+            if (s != FDTD_OK) { finish(r, s); done_cv.notify_all(); continue; }
+            active.push_back(r);
+            dirty = true;
+        }
+        if (active.empty()) continue;
+        if (dirty) {
+            fdtd_status s = rebuild();
+            dirty = false;
+            if (s != FDTD_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request *r : active) finish(r, s);
+                active.clear();
+                done_cv.notify_all();
+                continue;
+            }
+        }
+        if (hipGraphLaunch(exec, stream) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            set_last_error("step block failed on the device");
+            for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
+            active.clear();
+            done_cv.notify_all();
+            continue;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        ++stats.blocks_run;
+        for (Request *r : active) {
+            r->launched += block;
+            if (r->launched < r->steps) continue;
+            const int L = r->dev->S.L;
+            r->result.resize((size_t)r->steps * L);
+            fdtd_status s = FDTD_OK;
+            if (r->steps && L &&
+                hipMemcpy(r->result.data(), r->dev->dev.series, sizeof(double) * r->result.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                s = FDTD_ERR_DEVICE;
+            for (double v : r->result)
+                if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
+            finish(r, s);
+            dirty = true;
+        }
+        active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->status != FDTD_ERR_BUSY; }),
+                     active.end());
+        stats.active = (int64_t)active.size();
+        done_cv.notify_all();
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    prog.release();
+    std::lock_guard<std::mutex> lk(mu);
+    for (Request *r : active) finish(r, FDTD_ERR_CANCELLED);
+    active.clear();
+    done_cv.notify_all();
+}
+
+extern "C" {
+
+void fdtd_server_options_default(fdtd_server_options *o)
+{
+    if (!o) return;
+    o->hbm_budget_bytes = (int64_t)64 << 30;
+    o->max_active = 0;
+    o->steps_per_block = 64;
+    o->device = -1;
+}
+
+int64_t fdtd_scene_device_bytes(const fdtd_scene *scene)
+{
+    if (!scene) return -1;
+    SceneSetup S;
+    std::string err;
+    if (build_setup(*scene, S, err) != FDTD_OK) { set_last_error(err); return -1; }
+    return (int64_t)DeviceScene::bytes_needed(S, 1);
+}
+
+fdtd_status fdtd_server_create(const fdtd_server_options *opt_in, fdtd_server **out)
+{
+    if (!out) return FDTD_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_last_error("no HIP device"); return FDTD_ERR_NO_DEVICE; }
+    std::unique_ptr<fdtd_server> s(new (std::nothrow) fdtd_server());
+    if (!s) return FDTD_ERR_OOM;
+    fdtd_server_options_default(&s->opt);
+    if (opt_in) s->opt = *opt_in;
+    if (s->opt.hbm_budget_bytes <= 0) { set_last_error("hbm_budget_bytes must be positive"); return FDTD_ERR_INVALID; }
+    s->block = s->opt.steps_per_block > 0 ? s->opt.steps_per_block : 64;
+    if (s->opt.device >= 0) FDTD_HIP_TRY(hipSetDevice(s->opt.device));
+    FDTD_HIP_TRY(hipGetDevice(&s->device));
+    FDTD_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    fdtd_server *raw = s.release();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_server_submit(fdtd_server *s, const fdtd_scene *scene, int num_steps, int64_t *id)
+{
+    if (!s || !scene || num_steps <= 0 || !id) { set_last_error("bad submit arguments"); return FDTD_ERR_INVALID; }
+    auto r = std::make_unique<Request>();
+    std::string err;
+    fdtd_status st = build_setup(*scene, r->setup, err);
+    if (st != FDTD_OK) { set_last_error(err); return st; }
+    r->steps = num_steps;
+    r->bytes = DeviceScene::bytes_needed(r->setup, num_steps);
+    r->submitted = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if ((int64_t)r->bytes > s->opt.hbm_budget_bytes) {
+        set_last_error("request needs " + std::to_string(r->bytes) + " device bytes, over the whole HBM budget");
+        return FDTD_ERR_OOM;
+    }
+    r->id = s->next_id++;
+    *id = r->id;
+    s->queue.push_back(r->id);
+    s->reqs[r->id] = std::move(r);
+    ++s->stats.submitted;
+    s->cv.notify_all();
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_server_poll(fdtd_server *s, int64_t id)
+{
+    if (!s) return FDTD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return FDTD_ERR_NOT_FOUND;
+    return it->second->status;
+}
+
+fdtd_status fdtd_server_wait(fdtd_server *s, int64_t id, int timeout_ms)
+{
+    if (!s) return FDTD_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return FDTD_ERR_NOT_FOUND;
+    Request *r = it->second.get();
+    auto pred = [&] { return r->status != FDTD_ERR_BUSY; };
+    if (timeout_ms < 0) s->done_cv.wait(lk, pred);
+    else s->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred);
+    return r->status;
+}
+
+fdtd_status fdtd_server_take_result(fdtd_server *s, int64_t id, double *out, double *elapsed_ms)
+{
+    if (!s) return FDTD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return FDTD_ERR_NOT_FOUND;
+    Request *r = it->second.get();
+    if (r->status == FDTD_ERR_BUSY) return FDTD_ERR_BUSY;
+    const fdtd_status st = r->status;
+    if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(r->finished - r->submitted).count();
+    if (st == FDTD_OK && out && !r->result.empty()) std::memcpy(out, r->result.data(), sizeof(double) * r->result.size());
+    s->reqs.erase(it);
+    return st;
+}
+
+fdtd_status fdtd_server_cancel(fdtd_server *s, int64_t id)
+{
+    if (!s) return FDTD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return FDTD_ERR_NOT_FOUND;
+    it->second->cancel = true;
+    s->cv.notify_all();
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_server_get_stats(fdtd_server *s, fdtd_server_stats *out)
+{
+    if (!s || !out) return FDTD_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->active = (int64_t)s->active.size();
+    out->queued = (int64_t)s->queue.size();
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_server_destroy(fdtd_server *s)
+{
+    if (!s) return FDTD_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+    return FDTD_OK;
+}
+
+}  // extern "C"

@@ -113,7 +113,6 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
         std::vector<ListenDesc> listen;
         std::vector<GhostDesc> ghost;
         h->D.append_tiles(0, tile_plan(h->tile), tiles);
-        xcd_permute(tiles);
         h->D.append_listen(0, listen);
         if (h->D.needs_ghost_kernel()) ghost.push_back({0});
         st = h->prog.upload({h->D.dev}, tiles, listen, ghost, h->stream);

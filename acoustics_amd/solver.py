@@ -109,3 +109,127 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _scene_array(scene_buffers):
+    arr = (capi.Scene * len(scene_buffers))()
+    for i, sb in enumerate(scene_buffers):
+        arr[i] = sb.scene
+    return arr
+
+
+class Batch:
+    """Independent scenes advanced together (fdtd_batch_*)."""
+
+    def __init__(self, scene_buffers, num_steps, steps_per_graph=32, tile_config=-1):
+        self.lib = capi.load_library()
+        self.sbs = list(scene_buffers)
+        self.steps = [int(s) for s in num_steps]
+        self._arr = _scene_array(self.sbs)
+        ns = (C.c_int * len(self.steps))(*self.steps)
+        opt = capi.BatchOptions()
+        self.lib.fdtd_batch_options_default(C.byref(opt))
+        opt.steps_per_graph = steps_per_graph
+        opt.tile_config = tile_config
+        h = C.c_void_p()
+        capi.check(self.lib.fdtd_batch_create(self._arr, ns, len(self.sbs), C.byref(opt), C.byref(h)),
+                   "fdtd_batch_create")
+        self.h = h
+
+    def run(self):
+        capi.check(self.lib.fdtd_batch_run(self.h), "fdtd_batch_run")
+
+    def step(self, n):
+        capi.check(self.lib.fdtd_batch_step(self.h, int(n)), "fdtd_batch_step")
+
+    def series(self, i):
+        done = C.c_int64()
+        capi.check(self.lib.fdtd_batch_steps_done(self.h, i, C.byref(done)), "fdtd_batch_steps_done")
+        out = np.zeros((done.value, self.sbs[i].n_listeners))
+        capi.check(self.lib.fdtd_batch_read_listener_series(self.h, i, out.ctypes.data_as(C.POINTER(C.c_double))),
+                   "fdtd_batch_read_listener_series")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fdtd_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (fdtd_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_active=0, steps_per_block=64):
+        self.lib = capi.load_library()
+        opt = capi.ServerOptions()
+        self.lib.fdtd_server_options_default(C.byref(opt))
+        opt.hbm_budget_bytes = int(hbm_budget_bytes)
+        opt.max_active = int(max_active)
+        opt.steps_per_block = int(steps_per_block)
+        h = C.c_void_p()
+        capi.check(self.lib.fdtd_server_create(C.byref(opt), C.byref(h)), "fdtd_server_create")
+        self.h = h
+        self.meta = {}
+
+    def submit(self, scene_buffers, num_steps):
+        rid = C.c_int64()
+        capi.check(self.lib.fdtd_server_submit(self.h, scene_buffers.ref(), int(num_steps), C.byref(rid)),
+                   "fdtd_server_submit")
+        self.meta[rid.value] = (int(num_steps), scene_buffers.n_listeners)
+        return rid.value
+
+    def poll(self, rid):
+        return self.lib.fdtd_server_poll(self.h, rid)
+
+    def wait(self, rid, timeout_ms=-1):
+        return self.lib.fdtd_server_wait(self.h, rid, int(timeout_ms))
+
+    def cancel(self, rid):
+        capi.check(self.lib.fdtd_server_cancel(self.h, rid), "fdtd_server_cancel")
+
+    def take(self, rid):
+        """Returns (status, series or None, elapsed_ms)."""
+        steps, L = self.meta[rid]
+        out = np.zeros((steps, L))
+        el = C.c_double()
+        st = self.lib.fdtd_server_take_result(self.h, rid, out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(el))
+        return st, (out if st == 0 else None), el.value
+
+    def stats(self):
+        s = capi.ServerStats()
+        capi.check(self.lib.fdtd_server_get_stats(self.h, C.byref(s)), "fdtd_server_get_stats")
+        return {k: getattr(s, k) for k, _ in capi.ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fdtd_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_device_bytes(scene_buffers):
+    lib = capi.load_library()
+    return lib.fdtd_scene_device_bytes(scene_buffers.ref())

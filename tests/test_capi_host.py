@@ -1,0 +1,99 @@
+"""Host-side checks of the C-ABI library (no GPU compute calls): every
+symbol include/fdtd_acoustics.h declares is exported, the XML scene reader
+follows the ImpulseResponseParser keys and defaults, errors come back as
+status codes, and the C++ facade links and reports them."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from acoustics_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fdtd_acoustics.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(fdtd_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = capi.load_library()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(capi.SIGNATURES), set(names) ^ set(capi.SIGNATURES)
+
+
+def test_status_strings_and_defaults():
+    lib = capi.load_library()
+    assert lib.fdtd_status_string(0) == b"ok"
+    assert lib.fdtd_status_string(6) == b"unsupported"
+    s = capi.Settings()
+    lib.fdtd_settings_default(C.byref(s))
+    assert s.sound_speed == 343.0 and s.density == 1.0 and s.use_mesh == 1 and s.boundary_handling == 0
+
+
+def test_xml_scene_parse():
+    lib = capi.load_library()
+    p = C.c_void_p()
+    path = os.path.join(ROOT, "tests", "golden", "scene_sphere.xml").encode()
+    assert lib.fdtd_parse_xml(path, C.byref(p)) == 0
+    sc = lib.fdtd_parsed_scene(p).contents
+    st = sc.settings
+    assert lib.fdtd_parsed_num_steps(p) == 300
+    assert st.cell_size == 0.005 and st.density == 1.2 and st.pml_width == 8
+    assert abs(st.time_step - 1 / 237637.6) < 1e-18
+    assert list(st.divisions) == [40, 40, 40]
+    assert abs(st.min_bound[0] + 0.1) < 1e-15
+    assert sc.n_objects == 1 and sc.objects[0].type == capi.FDTD_OBJ_SPHERE and sc.objects[0].semi_axes[0] == 0.025
+    assert sc.n_impulse_sources == 1 and sc.impulse_sources[0].object == 0
+    assert sc.n_pressure_sources == 1
+    ps = sc.pressure_sources[0]
+    # parser defaults: width_space = c * width_time, K = 1/(sqrt(2 pi) ws)^3
+    assert abs(ps.width_space - 343 * 2e-4) < 1e-15
+    assert abs(ps.normalize_constant - 1 / (np.sqrt(2 * np.pi) * ps.width_space) ** 3) < 1e-9 * ps.normalize_constant
+    assert sc.n_listeners == 2 and sc.listeners[3] == -0.04
+    lib.fdtd_parsed_free(p)
+
+
+def test_xml_errors(tmp_path):
+    lib = capi.load_library()
+    p = C.c_void_p()
+    bad = tmp_path / "bad.xml"
+    bad.write_text('<impulse_response><solver cellsize="0.005"/></impulse_response>')
+    assert lib.fdtd_parse_xml(str(bad).encode(), C.byref(p)) == 5  # FDTD_ERR_PARSE
+    assert b"stop_time" in lib.fdtd_last_error()
+    bad.write_text('<impulse_response><solver')
+    assert lib.fdtd_parse_xml(str(bad).encode(), C.byref(p)) == 5
+    assert lib.fdtd_parse_xml(b"/nonexistent.xml", C.byref(p)) == 5
+
+
+def test_create_without_device_reports_status():
+    """No GPU in this container: creation must fail with a status, not abort."""
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present")
+    from acoustics_amd import scenes
+    lib = capi.load_library()
+    sb, _ = scenes.config1(n=16)
+    h = C.c_void_p()
+    st = lib.fdtd_create(sb.ref(), None, C.byref(h))
+    assert st == 2  # FDTD_ERR_NO_DEVICE
+
+
+def test_facade_compiles_and_links(tmp_path):
+    exe = tmp_path / "facade_probe"
+    src = os.path.join(ROOT, "tests", "facade", "facade_probe.cpp")
+    libdir = os.path.join(ROOT, "acoustics_amd", "lib")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", str(exe), src, "-L" + libdir, "-lfdtd_acoustics",
+                    "-Wl,-rpath," + libdir], check=True)
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present; the probe runs in the GPU tests")
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 3 and "no device" in r.stdout
