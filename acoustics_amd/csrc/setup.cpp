@@ -530,11 +530,16 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     }
     // Jacobi iteration count for the coupled rows: x <- b + W x converges when
     // the spectral radius of W is below one; estimate it by power iteration.
+    // Couplings usually form chains (a ghost's image point sees ghosts further
+    // along the surface, not back), making W nilpotent: then x_k = sum_{m<=k}
+    // W^m b is exact once k reaches the chain depth.  Otherwise bound the
+    // iteration count by the spectral radius.
     S.coupled_iters = 0;
     if (S.n_coupled > 0) {
         std::vector<double> x(G, 1.0), y(G);
         double rho_est = 0;
-        for (int it = 0; it < 200; ++it) {
+        int depth = 0;
+        for (int it = 0; it < 256; ++it) {
             double nrm = 0;
             for (int g = 0; g < G; ++g) {
                 double s = 0;
@@ -542,16 +547,19 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 y[g] = s;
                 nrm = std::max(nrm, std::fabs(s));
             }
-            if (nrm == 0) { rho_est = 0; break; }
+            if (nrm == 0) { depth = it + 1; break; }  // |W|^{it+1} 1 = 0
             rho_est = nrm / std::max(1e-300, *std::max_element(x.begin(), x.end()));
             for (int g = 0; g < G; ++g) x[g] = y[g] / nrm;
         }
-        if (rho_est >= 0.95) {
+        if (depth > 0) {
+            S.coupled_iters = depth;
+        } else if (rho_est < 0.95) {
+            S.coupled_iters = std::min(256, (int)std::ceil(std::log(1e-17) / std::log(rho_est)) + 1);
+        } else {
             err = "fully_coupled ghost system: Jacobi iteration does not contract (spectral radius estimate " +
                   std::to_string(rho_est) + ")";
             return FDTD_ERR_UNSUPPORTED;
         }
-        S.coupled_iters = rho_est <= 1e-12 ? 1 : std::min(200, (int)std::ceil(std::log(1e-17) / std::log(rho_est)) + 1);
     }
 
     // ---- special cells and their boxes ------------------------------------

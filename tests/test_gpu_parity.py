@@ -94,3 +94,35 @@ def test_tile_configs_agree(hip_lib, tile):
         b.step(60)
         eb = b.pressure()
     assert np.array_equal(ea, eb)
+
+
+def test_sphere_fully_coupled(hip_lib):
+    sb, _ = scenes.config2(seed=24, n=48, radius=0.03, shell=0.07, pml=8, n_listeners=16,
+                           boundary=FDTD_BOUNDARY_FULLY_COUPLED)
+    got, ref = _run_pair(sb, 300)
+    err = rel_l2(got, ref)
+    print("sphere fully_coupled rel L2", err)
+    assert err <= TOL
+
+
+def test_ellipsoids_fully_coupled(hip_lib):
+    sb, _ = scenes.config3(seed=32, n=64, h=0.004, pml=8, n_obj=3, n_listeners=16, axis_lo=0.01, axis_hi=0.025,
+                           boundary=FDTD_BOUNDARY_FULLY_COUPLED)
+    got, ref = _run_pair(sb, 250)
+    err = rel_l2(got, ref)
+    print("ellipsoids fully_coupled rel L2", err)
+    assert err <= TOL
+
+
+def test_listener_next_to_object(hip_lib):
+    """Listeners whose enclosing cells include ghost cells (MLS over ghost
+    children, PML_WaveSolver.cpp:246-262) and solid cells."""
+    sb0, _ = scenes.config2(seed=25, n=40, radius=0.025, shell=0.07, pml=6, n_listeners=4)
+    c = np.array(sb0.objects_list[0]["center"])
+    lst = np.array([c + [0.026, 0.0, 0.0], c + [0.0, -0.0255, 0.001], c + [0.018, 0.018, 0.0], c + [0.0, 0.0, 0.0262]])
+    from acoustics_amd.capi import SceneBuffers
+    sb = SceneBuffers(sb0.settings_dict, sb0.objects_list, sb0.impulse_list, [], lst)
+    got, ref = _run_pair(sb, 200)
+    err = rel_l2(got, ref)
+    print("near-object listeners rel L2", err)
+    assert err <= TOL
