@@ -30,7 +30,7 @@ struct Carver {
 struct Layout {
     float *field;
     float4 *coef[3];
-    uint8_t *flags[3];
+    int *flags[3];
     uint8_t *skip;
     float *gtab;
     PSrcDev *psrc;
@@ -53,7 +53,7 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.field = c.take<float>((size_t)NUM_SLOTS * (size_t)pad_cells(S.N));
     for (int d = 0; d < 3; ++d) {
         L.coef[d] = c.take<float4>(S.n[d] + 1);
-        L.flags[d] = c.take<uint8_t>(S.n[d] + 1);
+        L.flags[d] = c.take<int>(S.n[d] + 1);
     }
     L.skip = c.take<uint8_t>(S.objs.empty() ? 0 : (size_t)S.N);
     L.gtab = c.take<float>(S.gtab.size());
@@ -143,17 +143,17 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
     };
     std::vector<float4> coef[3];
-    std::vector<uint8_t> flags[3];
+    std::vector<int> flags[3];
     for (int d = 0; d < 3; ++d) {
         const int n = S.n[d];
         coef[d].resize(n + 1);
         flags[d].resize(n + 1);
         for (int i = 0; i <= n; ++i) {
             coef[d][i] = make_float4(S.f1[d][i], S.f2[d][i], i < n ? S.f3[d][i] : 0.f, i < n ? S.f5h[d][i] : 0.f);
-            flags[d][i] = (uint8_t)((i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0));
+            flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0);
         }
         FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
-        FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size()));
+        FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size() * sizeof(int)));
     }
     std::vector<uint8_t> skip;
     if (!S.objs.empty()) {

@@ -46,7 +46,7 @@ struct SceneDev {
     long long npad;             // floats per field slot
     float *field;               // NUM_SLOTS slots: p ring (3), pDir (3), v ping-pong (2 x 3)
     const float4 *coef[3];      // per axis, index 0..n: {f1(face), f2(face), f3(cell), f5/h(cell)}
-    const uint8_t *flags[3];    // per axis: bit0 cell in PML, bit1 face in PML
+    const int *flags[3];        // per axis, index 0..n: bit0 cell in PML, bit1 face in PML
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
     double lam2, t_start, dt;
     float lam2f;

@@ -91,26 +91,11 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
 // ---------------------------------------------------------------------------
 // PML slab: fused staggered velocity + split pressure update
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float pml_axis(const float4 *__restrict__ coef, const uint8_t *__restrict__ flg, int i, int n,
-                                          bool oth, float pc, float pminus, float pplus, const float *__restrict__ vin,
-                                          float *__restrict__ vout, float *__restrict__ pdir, int idx, int stride)
+// Face velocity update on one axis for a face whose old value is v, between
+// pressures pl (left) and pr (right); inactive faces stay at zero.
+__device__ __forceinline__ float face_update(bool active, float4 c, float v, float pl, float pr)
 {
-    const float4 cL = coef[i];
-    const uint8_t fL = flg[i];
-    const bool actL = (i >= 1) && ((fL & 2) || oth);
-    const float vlN = actL ? cL.x * vin[idx] + cL.y * (pc - pminus) : 0.f;
-    float vrN = 0.f;
-    if (i + 1 < n) {
-        const float4 cR = coef[i + 1];
-        const uint8_t fR = flg[i + 1];
-        if ((fR & 2) || oth) vrN = cR.x * vin[idx + stride] + cR.y * (pplus - pc);
-        if (!((fR & 1) || oth)) vout[idx + stride] = vrN;  // right neighbour is not PML: this cell owns the face
-    }
-    vout[idx] = vlN;
-    float pd = pdir[idx];
-    pd = cL.z * pd + cL.w * (vrN - vlN);
-    pdir[idx] = pd;
-    return pd;
+    return active ? c.x * v + c.y * (pr - pl) : 0.f;
 }
 
 __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, long long step)
@@ -137,22 +122,61 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     float *__restrict__ vo0 = field + (size_t)(vout_s + 0) * np;
     float *__restrict__ vo1 = field + (size_t)(vout_s + 1) * np;
     float *__restrict__ vo2 = field + (size_t)(vout_s + 2) * np;
-    const bool cj = S.flags[1][j] & 1, ck = S.flags[2][k] & 1;
+    const float4 *__restrict__ cx = S.coef[0];
+    const int *__restrict__ fx = S.flags[0];
+
+    // y and z coefficients are constant along the column
+    const int fyL = S.flags[1][j], fzL = S.flags[2][k];
+    const int fyR = (j + 1 < ny) ? S.flags[1][j + 1] : 0, fzR = (k + 1 < nz) ? S.flags[2][k + 1] : 0;
+    const float4 cyL = S.coef[1][j], czL = S.coef[2][k];
+    const float4 cyR = S.coef[1][min(j + 1, ny)], czR = S.coef[2][min(k + 1, nz)];
+    const bool cj = fyL & 1, ck = fzL & 1;
     const int ym_off = (j > 0) ? sy : 0, yp_off = (j < ny - 1) ? sy : 0;
     const int zm_off = (k > 0) ? 1 : 0, zp_off = (k < nz - 1) ? 1 : 0;
 
     int idx = t.i0 * sx + j * sy + k;
     float pm = pcur[idx - (t.i0 > 0 ? sx : 0)];
     float pc = pcur[idx];
+    // the x face on the left of the current cell (updated value), carried
+    // from one plane to the next: cell i's right face is cell i+1's left face
+    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || cj || ck), cx[t.i0], vi0[idx], pm, pc);
     for (int i = t.i0; i < t.i1; ++i, idx += sx) {
-        const float pp = pcur[idx + (i < nx - 1 ? sx : 0)];
+        const bool last = i + 1 >= nx;
+        const float pp = pcur[idx + (last ? 0 : sx)];
         const float ym = pcur[idx - ym_off], yp = pcur[idx + yp_off];
         const float zm = pcur[idx - zm_off], zp = pcur[idx + zp_off];
-        const bool ci = S.flags[0][i] & 1;
-        float pn = pml_axis(S.coef[0], S.flags[0], i, nx, cj || ck, pc, pm, pp, vi0, vo0, pd0, idx, sx);
-        pn += pml_axis(S.coef[1], S.flags[1], j, ny, ci || ck, pc, ym, yp, vi1, vo1, pd1, idx, sy);
-        pn += pml_axis(S.coef[2], S.flags[2], k, nz, ci || cj, pc, zm, zp, vi2, vo2, pd2, idx, 1);
-        pnext[idx] = pn;
+        const float vxr_old = last ? 0.f : vi0[idx + sx];
+        const float vyl_old = vi1[idx], vyr_old = (j + 1 < ny) ? vi1[idx + sy] : 0.f;
+        const float vzl_old = vi2[idx], vzr_old = (k + 1 < nz) ? vi2[idx + 1] : 0.f;
+        const float q0 = pd0[idx], q1 = pd1[idx], q2 = pd2[idx];
+        const int fxC = fx[i], fxR = last ? 0 : fx[i + 1];
+        const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
+        const bool ci = fxC & 1;
+        // x: right face (new), left face carried
+        const bool ox = cj || ck;
+        const float vxR = last ? 0.f : face_update((fxR & 2) || ox, cxR, vxr_old, pc, pp);
+        if (!last && !((fxR & 1) || ox)) vo0[idx + sx] = vxR;  // right cell is not PML: this cell owns the face
+        vo0[idx] = vxL;
+        const float n0 = cxC.z * q0 + cxC.w * (vxR - vxL);
+        // y
+        const bool oy = ci || ck;
+        const float vyL = face_update(j >= 1 && ((fyL & 2) || oy), cyL, vyl_old, ym, pc);
+        const float vyR = (j + 1 < ny) ? face_update((fyR & 2) || oy, cyR, vyr_old, pc, yp) : 0.f;
+        if (j + 1 < ny && !((fyR & 1) || oy)) vo1[idx + sy] = vyR;
+        vo1[idx] = vyL;
+        const float n1 = cyL.z * q1 + cyL.w * (vyR - vyL);
+        // z
+        const bool oz = ci || cj;
+        const float vzL = face_update(k >= 1 && ((fzL & 2) || oz), czL, vzl_old, zm, pc);
+        const float vzR = (k + 1 < nz) ? face_update((fzR & 2) || oz, czR, vzr_old, pc, zp) : 0.f;
+        if (k + 1 < nz && !((fzR & 1) || oz)) vo2[idx + 1] = vzR;
+        vo2[idx] = vzL;
+        const float n2 = czL.z * q2 + czL.w * (vzR - vzL);
+        pd0[idx] = n0;
+        pd1[idx] = n1;
+        pd2[idx] = n2;
+        pnext[idx] = n0 + n1 + n2;
+        vxL = vxR;
         pm = pc;
         pc = pp;
     }

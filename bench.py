@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--serve", type=int, default=24, help="requests in the serving comparison (0: skip)")
+    ap.add_argument("--block", type=int, default=64, help="server steps per block")
     return ap.parse_args()
 
 
@@ -60,6 +62,69 @@ def cpu_baseline(sb, steps):
     return {"value": n * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{sb.divisions[0]}^3 headline scene, {steps} FP64 steps after 1 warmup step, "
                       f"per-step reclassification as in the reference, {t:.2f} s"}
+
+
+def _percentile(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs), q)) if xs else None
+
+
+def serving(n_requests, block):
+    """Poisson stream of impact-sound requests (rate 200/s, edges 48-192,
+    1-3 ellipsoids, 32 listeners, 0.01-0.05 s of sound), served two ways:
+    the continuous-batching server, and the plain path that solves one
+    request at a time with the single-scene API.  Latency = finish - arrival."""
+    import threading
+    from acoustics_amd import scenes
+    from acoustics_amd.solver import Server, Solver
+    stream = scenes.config5_stream(seed=5, n_requests=n_requests)
+    cells_steps = sum(sb.n_cells * st for _, sb, st in stream)
+
+    # served
+    lat = {}
+    with Server(hbm_budget_bytes=160 << 30, steps_per_block=block) as srv:
+        ids = [None] * len(stream)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for q, (ta, sb, st) in enumerate(stream):
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[q] = (srv.submit(sb, st), time.perf_counter())
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for q in range(len(stream)):
+            rid, ts = ids[q]
+            st = srv.wait(rid, 600000)
+            status, series, el = srv.take(rid)
+            if status != 0:
+                raise RuntimeError(f"served request {q} failed with status {status}")
+            lat[q] = el
+        t_served = time.perf_counter() - t0
+    served_lat = [lat[q] for q in range(len(stream))]
+
+    # plain: one request at a time, in arrival order
+    plain_lat = []
+    t0 = time.perf_counter()
+    for ta, sb, st in stream:
+        d = t0 + ta - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        with Solver(sb) as s:
+            s.step(st)
+            s.series()
+        plain_lat.append((time.perf_counter() - (t0 + ta)) * 1e3)
+    t_plain = time.perf_counter() - t0
+    return {
+        "requests": len(stream), "cell_updates": cells_steps, "steps_per_block": block,
+        "served": {"solves_per_s": round(len(stream) / t_served, 3), "p50_ms": round(_percentile(served_lat, 50), 2),
+                   "p99_ms": round(_percentile(served_lat, 99), 2), "wall_s": round(t_served, 3)},
+        "plain": {"solves_per_s": round(len(stream) / t_plain, 3), "p50_ms": round(_percentile(plain_lat, 50), 2),
+                  "p99_ms": round(_percentile(plain_lat, 99), 2), "wall_s": round(t_plain, 3)},
+        "speedup_solves_per_s": round(t_plain / t_served, 3),
+    }
 
 
 def main():
@@ -122,6 +187,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5), "step_ms_events": round(step_ms, 5)},
     }
+    if a.serve > 0:
+        out["serving"] = serving(a.serve, a.block)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sb, a.cpu_steps)
     print(json.dumps(out), flush=True)
