@@ -92,7 +92,7 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
     (void)hipEventDestroy(b);
     plan = best_plan;
     for (auto &sc : scenes)
-        if (sc) FDTD_HIP_TRY(hipMemsetAsync(sc->dev.field, 0, sizeof(float) * (size_t)NUM_SLOTS * (size_t)sc->npad, st));
+        if (sc) FDTD_HIP_TRY(sc->zero_state(st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     return rebuild(st);
 }

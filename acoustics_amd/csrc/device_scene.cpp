@@ -28,7 +28,11 @@ struct Carver {
 };
 
 struct Layout {
-    float *field;
+    float *field;               // pressure slots, then the compact PML state
+    size_t state_floats;
+    PmlSlab *slabs;
+    float *pd[3];
+    float *vel[2][3];
     float4 *coef[3];
     int *flags[3];
     uint8_t *skip;
@@ -48,9 +52,29 @@ struct Layout {
 
 long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
 
+// Every piece of time-stepping state in one zero-initialised block: the two
+// pressure slots, the split pressures of the PML cells, and both buffers of
+// the PML face velocities.  Each array starts on a 256-byte boundary.
+size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6])
+{
+    size_t o = (size_t)NUM_P_SLOTS * (size_t)npad;
+    auto al = [](size_t x) { return (x + 63) / 64 * 64; };
+    for (int d = 0; d < 3; ++d) { off[1 + d] = o; o += al((size_t)S.n_pml); }
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d) { off[4 + 3 * b + d] = o; o += al((size_t)S.n_pml_faces[d]); }
+    off[0] = 0;
+    return o;
+}
+
 void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
 {
-    L.field = c.take<float>((size_t)NUM_SLOTS * (size_t)pad_cells(S.N));
+    size_t off[10];
+    L.state_floats = state_layout(S, pad_cells(S.N), off);
+    L.field = c.take<float>(L.state_floats);
+    for (int d = 0; d < 3; ++d) L.pd[d] = L.field ? L.field + off[1 + d] : nullptr;
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d) L.vel[b][d] = L.field ? L.field + off[4 + 3 * b + d] : nullptr;
+    L.slabs = c.take<PmlSlab>(S.slabs.size());
     for (int d = 0; d < 3; ++d) {
         L.coef[d] = c.take<float4>(S.n[d] + 1);
         L.flags[d] = c.take<int>(S.n[d] + 1);
@@ -135,8 +159,9 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     c.base = arena;
     c.off = 0;
     carve(S, cap, c, L);
-    // fields start at rest: p ring, pDir and both face-velocity buffers
-    FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * (size_t)NUM_SLOTS * (size_t)npad, st));
+    state_floats = L.state_floats;
+    // fields start at rest: pressures, split pressures and both face-velocity buffers
+    FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * state_floats, st));
     std::vector<char> staging;  // pageable copies are synchronised before return
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
@@ -163,6 +188,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         FDTD_HIP_TRY(up(L.skip, skip.data(), skip.size()));
     }
     FDTD_HIP_TRY(up(L.gtab, S.gtab.data(), S.gtab.size() * 4));
+    FDTD_HIP_TRY(up(L.slabs, S.slabs.data(), S.slabs.size() * sizeof(PmlSlab)));
     std::vector<PSrcDev> ps;
     for (const PSrcHost &h : S.psrc) {
         PSrcDev d{};
@@ -212,6 +238,12 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
     D.npad = npad;
     D.field = L.field;
+    D.slabs = L.slabs;
+    for (int d = 0; d < 3; ++d) {
+        D.pd[d] = L.pd[d];
+        D.vel[0][d] = L.vel[0][d];
+        D.vel[1][d] = L.vel[1][d];
+    }
     for (int d = 0; d < 3; ++d) { D.coef[d] = L.coef[d]; D.flags[d] = L.flags[d]; }
     D.skip = S.objs.empty() ? nullptr : L.skip;
     D.lam2 = S.lam2;
@@ -238,7 +270,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.lptr = L.lptr; D.lkind = L.lkind; D.lindex = L.lindex; D.lbeta = L.lbeta;
     D.series = L.series;
     D.clock = L.clock;
-    D.n_listen_groups = S.L > 0 ? S.L : 1;
+    D.n_listen_groups = S.L > 0 ? (S.L + kListenersPerGroup - 1) / kListenersPerGroup : 1;
     D.n_obj = (int)S.objs.size();
     D.acc_now = L.acc_now;
     D.src_now = L.src_now;
@@ -282,26 +314,21 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         return false;
     };
     // PML slabs first (heaviest tiles start first)
-    const int slabs[6][6] = {
-        {0, a[0], 0, n[1], 0, n[2]},       {b[0], n[0], 0, n[1], 0, n[2]},
-        {a[0], b[0], 0, a[1], 0, n[2]},    {a[0], b[0], b[1], n[1], 0, n[2]},
-        {a[0], b[0], a[1], b[1], 0, a[2]}, {a[0], b[0], a[1], b[1], b[2], n[2]},
-    };
-    for (const auto &sl : slabs) {
-        const int ie = sl[1] - sl[0], je = sl[3] - sl[2], ke = sl[5] - sl[4];
-        if (ie <= 0 || je <= 0 || ke <= 0) continue;
-        const int tk = std::min(6, pow2_ceil_log2(ke));
+    for (int q = 0; q < (int)S.slabs.size(); ++q) {
+        const PmlSlab &sl = S.slabs[q];
+        const int tk = std::min(6, pow2_ceil_log2(sl.e[2]));
         const int TK = 1 << tk, TJ = 256 >> tk;
-        for (int i0 = sl[0]; i0 < sl[1]; i0 += plan.pml_ti)
-            for (int j0 = sl[2]; j0 < sl[3]; j0 += TJ)
-                for (int k0 = sl[4]; k0 < sl[5]; k0 += TK) {
+        for (int i0 = sl.lo[0]; i0 < sl.lo[0] + sl.e[0]; i0 += plan.pml_ti)
+            for (int j0 = sl.lo[1]; j0 < sl.lo[1] + sl.e[1]; j0 += TJ)
+                for (int k0 = sl.lo[2]; k0 < sl.lo[2] + sl.e[2]; k0 += TK) {
                     TileDesc t{};
                     t.scene = scene_id;
                     t.type = TILE_PML;
-                    t.i0 = i0; t.i1 = std::min(i0 + plan.pml_ti, sl[1]);
-                    t.j0 = j0; t.j1 = std::min(j0 + TJ, sl[3]);
-                    t.k0 = k0; t.k1 = std::min(k0 + TK, sl[5]);
+                    t.i0 = i0; t.i1 = std::min(i0 + plan.pml_ti, sl.lo[0] + sl.e[0]);
+                    t.j0 = j0; t.j1 = std::min(j0 + TJ, sl.lo[1] + sl.e[1]);
+                    t.k0 = k0; t.k1 = std::min(k0 + TK, sl.lo[2] + sl.e[2]);
                     t.tk_log2 = tk;
+                    t.slab = q;
                     out.push_back(t);
                 }
     }
@@ -335,8 +362,8 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
 
 void DeviceScene::append_listen(int scene_id, std::vector<ListenDesc> &out) const
 {
-    if (S.L == 0) out.push_back({scene_id, -1});
-    for (int l = 0; l < S.L; ++l) out.push_back({scene_id, l});
+    if (S.L == 0) out.push_back({scene_id, 0, 0});
+    for (int l0 = 0; l0 < S.L; l0 += kListenersPerGroup) out.push_back({scene_id, l0, std::min(S.L, l0 + kListenersPerGroup)});
 }
 
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd)
@@ -418,17 +445,18 @@ fdtd_status StepProgram::enqueue(hipStream_t st, hipEvent_t ev_begin, hipEvent_t
 {
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
-    const bool split = side && ntiles[KIND_INTERIOR] > 0 && (ntiles[KIND_PML] + ntiles[KIND_SPECIAL]) > 0;
+    // the PML slabs carry most of the bytes: they run alone on the main
+    // stream while the object cells and then the interior run beside them
+    const bool split = side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
+    hipStream_t aux = split ? side : st;
     if (split) {
         FDTD_HIP_TRY(hipEventRecord(fork, st));
         FDTD_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
-        FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], side));
-        FDTD_HIP_TRY(hipEventRecord(join, side));
-    } else {
-        FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], st));
     }
+    FDTD_HIP_TRY(launch_step(KIND_SPECIAL, d_scenes, d_tiles[KIND_SPECIAL], ntiles[KIND_SPECIAL], aux));
+    FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], aux));
+    if (split) FDTD_HIP_TRY(hipEventRecord(join, side));
     FDTD_HIP_TRY(launch_step(KIND_PML, d_scenes, d_tiles[KIND_PML], ntiles[KIND_PML], st));
-    FDTD_HIP_TRY(launch_step(KIND_SPECIAL, d_scenes, d_tiles[KIND_SPECIAL], ntiles[KIND_SPECIAL], st));
     if (split) FDTD_HIP_TRY(hipStreamWaitEvent(st, join, 0));
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
     FDTD_HIP_TRY(launch_listen(d_scenes, d_listen, nlisten, st));

@@ -38,6 +38,7 @@ struct DeviceScene {
     size_t arena_bytes = 0;
     SceneDev dev{};            // host copy of the scene block (pointers into the arena)
     long long npad = 0;
+    size_t state_floats = 0;   // floats of time-stepping state from dev.field on
     int series_cap = 0;
 
     // Device bytes a scene needs (admission control uses this).
@@ -47,6 +48,8 @@ struct DeviceScene {
     fdtd_status allocate_and_upload(hipStream_t stream, int series_cap, long long target);
     void release();
     float *slot(int s) const { return dev.field + (size_t)s * (size_t)npad; }
+    // zeroes every time-stepping array (pressures, split pressures, velocities)
+    hipError_t zero_state(hipStream_t st) const { return hipMemsetAsync(dev.field, 0, sizeof(float) * state_floats, st); }
 
     // Appends this scene's tiles / listener groups / ghost work, tagged with scene_id.
     void append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out) const;
@@ -59,8 +62,8 @@ struct DeviceScene {
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 
 // The per-step launch sequence of a set of scenes, uploaded once:
-//   ghost solve (fully-coupled scenes) -> {PML + object cells on the main
-//   stream || interior on the side stream} -> listeners + step advance.
+//   ghost solve (fully-coupled scenes) -> {PML slabs on the main stream ||
+//   object cells then interior on the side stream} -> listeners + step advance.
 // The three step kernels write disjoint cells; the fork/join lets them
 // overlap (and is captured as parallel branches of a graph).
 struct StepProgram {

@@ -77,6 +77,19 @@ struct ListenerTerm {
     double beta;
 };
 
+// One box of PML cells with its own compact storage: split pressures at
+// cell_base + (li * e1 + lj) * e2 + lk (local indices), and for each axis d
+// face velocities in a box one layer longer along d, at v_base[d] + the same
+// formula with e[d] + 1 in place of e[d].  Face li along d is the face on the
+// low side of local cell li; the extra layer holds the high faces of the last
+// layer of cells.  A face shared by two boxes is stored in both and given the
+// same update by each.
+struct PmlSlab {
+    int32_t lo[3], e[3];
+    int64_t cell_base;
+    int64_t v_base[3];
+};
+
 struct SceneSetup {
     int n[3] = {0, 0, 0};
     int64_t N = 0;
@@ -90,6 +103,8 @@ struct SceneSetup {
     std::vector<float> f3[3], f5h[3];  // cell tables (n_d), f5h = f5 / h
     int inner_lo[3], inner_hi[3];      // non-PML index box
     int64_t n_pml = 0;
+    std::vector<PmlSlab> slabs;        // the PML as up to six disjoint boxes
+    int64_t n_pml_faces[3] = {0, 0, 0};  // compact face-velocity entries per axis
     // sources
     std::vector<PSrcHost> psrc;
     std::vector<float> gtab;

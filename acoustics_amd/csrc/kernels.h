@@ -33,8 +33,14 @@ struct GhostDev {
     double k_raster, c_acc;
 };
 
-// field slots inside SceneDev::field, each npad floats
-enum : int { SLOT_P0 = 0, SLOT_PDIR = 3, SLOT_V0 = 6, SLOT_V1 = 9, NUM_SLOTS = 12 };
+// pressure slots inside SceneDev::field, each npad floats
+enum : int { NUM_P_SLOTS = 2 };
+
+// Pressure slots of step s: p^s is read from slot_cur, and p^{s+1} is written
+// over p^{s-1} in slot_next (the leapfrog reads p^{s-1} only at the cell it
+// writes, so two slots suffice).
+__host__ __device__ inline int slot_cur(long long s) { return (int)((s + 1) & 1); }
+__host__ __device__ inline int slot_next(long long s) { return (int)(s & 1); }
 
 // clock words of a scene
 enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_WORDS = 4 };
@@ -43,8 +49,13 @@ struct SceneDev {
     int n[3];
     int sx, sy;                 // strides of i and j (k fastest)
     int raster;                 // 1: rasterized ghost values, 0: fully coupled
-    long long npad;             // floats per field slot
-    float *field;               // NUM_SLOTS slots: p ring (3), pDir (3), v ping-pong (2 x 3)
+    long long npad;             // floats per pressure slot
+    float *field;               // NUM_P_SLOTS pressure slots, full grid, k fastest
+    // PML state in compact per-slab storage (PmlSlab): split pressures per
+    // PML cell, face velocities per axis, double-buffered by step parity
+    const PmlSlab *slabs;
+    float *pd[3];
+    float *vel[2][3];
     const float4 *coef[3];      // per axis, index 0..n: {f1(face), f2(face), f3(cell), f5/h(cell)}
     const int *flags[3];        // per axis, index 0..n: bit0 cell in PML, bit1 face in PML
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
@@ -85,11 +96,17 @@ struct TileDesc {
     int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records
     int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK
     int src;                     // tile intersects a pressure-source box
+    int slab;                    // TILE_PML: index into SceneDev::slabs
+    int pad;
 };
 
+// One listener workgroup: listeners [l0, l1) of a scene, one thread each
+// (an order-1 MLS listener has about eight terms).  A scene without
+// listeners gets one group with l0 == l1 that only advances the step.
+constexpr int kListenersPerGroup = 64;
 struct ListenDesc {
     int scene;
-    int l;                       // listener index, or -1 for the step-advance-only group
+    int l0, l1;
 };
 
 struct GhostDesc {

@@ -8,9 +8,10 @@
 //     k is the fastest axis so a wavefront streams 64 consecutive floats.
 //   * TILE_PML: staggered face velocity v <- f1 v + f2 (p_R - p_L) on the
 //     cell's faces, then pDir_d <- f3 pDir_d + f5/h (v_R - v_L) and
-//     p^{n+1} = sum_d pDir_d (MAC_Grid.cpp:461-508), fused per thread.  Face
-//     velocities are double-buffered (vin -> vout), so a thread may recompute
-//     the face it shares with a neighbour without racing the neighbour's write.
+//     p^{n+1} = sum_d pDir_d (MAC_Grid.cpp:461-508), fused per thread.  The
+//     PML state lives in compact per-slab arrays (PmlSlab).  Face velocities
+//     are double-buffered (vin -> vout), so a thread may recompute the face it
+//     shares with a neighbour without racing the neighbour's write.
 //   * TILE_SPECIAL: one thread per bulk cell next to a rigid object: the
 //     non-uniform ghost-cell Laplacian (MAC_Grid.cpp:187-299) and the
 //     one-cell-cavity freeze, in FP64; ghost values are FP64.
@@ -48,10 +49,10 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     const int k = t.k0 + tx, j = t.j0 + ty;
     if (k >= t.k1 || j >= t.j1) return;
     const int sx = S.sx, sy = S.sy;
-    const int ind = (int)((1 + step) % 3);
-    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
-    const float *__restrict__ pprev = S.field + (size_t)((ind + 2) % 3) * S.npad;
-    float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
+    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    // p^{n+1} overwrites p^{n-1}: each thread reads and writes only its own cells there
+    const float *pprev = S.field + (size_t)slot_next(step) * S.npad;
+    float *pnext = S.field + (size_t)slot_next(step) * S.npad;
     const float lam2 = S.lam2f;
 
     int idx = t.i0 * sx + j * sy + k;
@@ -106,22 +107,24 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     if (k >= t.k1 || j >= t.j1) return;
     const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
     const int sx = S.sx, sy = S.sy;
-    const int ind = (int)((1 + step) % 3);
+    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    float *__restrict__ pnext = S.field + (size_t)slot_next(step) * S.npad;
+    // compact storage of this slab: cells (and x faces) have plane stride
+    // e1*e2; y faces (e1+1)*e2; z faces e1*(e2+1)
+    const PmlSlab &sl = S.slabs[t.slab];
+    const int e0 = sl.e[0], e1 = sl.e[1], e2 = sl.e[2];
+    const int li0 = t.i0 - sl.lo[0], lj = j - sl.lo[1], lk = k - sl.lo[2];
+    const int cs = e1 * e2, ys = (e1 + 1) * e2, zs = e1 * (e2 + 1);
     const int vb = (int)(step & 1);
-    float *__restrict__ field = S.field;
-    const size_t np = (size_t)S.npad;
-    const float *__restrict__ pcur = field + (size_t)ind * np;
-    float *__restrict__ pnext = field + (size_t)((ind + 1) % 3) * np;
-    float *__restrict__ pd0 = field + (size_t)(SLOT_PDIR + 0) * np;
-    float *__restrict__ pd1 = field + (size_t)(SLOT_PDIR + 1) * np;
-    float *__restrict__ pd2 = field + (size_t)(SLOT_PDIR + 2) * np;
-    const int vin_s = vb ? SLOT_V1 : SLOT_V0, vout_s = vb ? SLOT_V0 : SLOT_V1;
-    const float *__restrict__ vi0 = field + (size_t)(vin_s + 0) * np;
-    const float *__restrict__ vi1 = field + (size_t)(vin_s + 1) * np;
-    const float *__restrict__ vi2 = field + (size_t)(vin_s + 2) * np;
-    float *__restrict__ vo0 = field + (size_t)(vout_s + 0) * np;
-    float *__restrict__ vo1 = field + (size_t)(vout_s + 1) * np;
-    float *__restrict__ vo2 = field + (size_t)(vout_s + 2) * np;
+    float *__restrict__ pd0 = S.pd[0] + sl.cell_base;
+    float *__restrict__ pd1 = S.pd[1] + sl.cell_base;
+    float *__restrict__ pd2 = S.pd[2] + sl.cell_base;
+    const float *__restrict__ vi0 = S.vel[vb][0] + sl.v_base[0];
+    const float *__restrict__ vi1 = S.vel[vb][1] + sl.v_base[1];
+    const float *__restrict__ vi2 = S.vel[vb][2] + sl.v_base[2];
+    float *__restrict__ vo0 = S.vel[vb ^ 1][0] + sl.v_base[0];
+    float *__restrict__ vo1 = S.vel[vb ^ 1][1] + sl.v_base[1];
+    float *__restrict__ vo2 = S.vel[vb ^ 1][2] + sl.v_base[2];
     const float4 *__restrict__ cx = S.coef[0];
     const int *__restrict__ fx = S.flags[0];
 
@@ -133,48 +136,53 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     const bool cj = fyL & 1, ck = fzL & 1;
     const int ym_off = (j > 0) ? sy : 0, yp_off = (j < ny - 1) ? sy : 0;
     const int zm_off = (k > 0) ? 1 : 0, zp_off = (k < nz - 1) ? 1 : 0;
+    // the high y / z face of this column is stored here when it is the slab's last layer
+    const bool own_yR = lj == e1 - 1, own_zR = lk == e2 - 1;
 
     int idx = t.i0 * sx + j * sy + k;
+    int c = (li0 * e1 + lj) * e2 + lk;              // cell and x-face offset
+    int yo = (li0 * (e1 + 1) + lj) * e2 + lk;       // y-face offset
+    int zo = (li0 * e1 + lj) * (e2 + 1) + lk;       // z-face offset
     float pm = pcur[idx - (t.i0 > 0 ? sx : 0)];
     float pc = pcur[idx];
-    // the x face on the left of the current cell (updated value), carried
-    // from one plane to the next: cell i's right face is cell i+1's left face
-    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || cj || ck), cx[t.i0], vi0[idx], pm, pc);
-    for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+    // the x face on the low side of the current cell (updated value), carried
+    // from one plane to the next: cell i's high face is cell i+1's low face
+    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || cj || ck), cx[t.i0], vi0[c], pm, pc);
+    for (int i = t.i0; i < t.i1; ++i, idx += sx, c += cs, yo += ys, zo += zs) {
         const bool last = i + 1 >= nx;
         const float pp = pcur[idx + (last ? 0 : sx)];
         const float ym = pcur[idx - ym_off], yp = pcur[idx + yp_off];
         const float zm = pcur[idx - zm_off], zp = pcur[idx + zp_off];
-        const float vxr_old = last ? 0.f : vi0[idx + sx];
-        const float vyl_old = vi1[idx], vyr_old = (j + 1 < ny) ? vi1[idx + sy] : 0.f;
-        const float vzl_old = vi2[idx], vzr_old = (k + 1 < nz) ? vi2[idx + 1] : 0.f;
-        const float q0 = pd0[idx], q1 = pd1[idx], q2 = pd2[idx];
+        const float vxr_old = vi0[c + cs];
+        const float vyl_old = vi1[yo], vyr_old = vi1[yo + e2];
+        const float vzl_old = vi2[zo], vzr_old = vi2[zo + 1];
+        const float q0 = pd0[c], q1 = pd1[c], q2 = pd2[c];
         const int fxC = fx[i], fxR = last ? 0 : fx[i + 1];
         const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
         const bool ci = fxC & 1;
-        // x: right face (new), left face carried
+        // x: high face (new), low face carried
         const bool ox = cj || ck;
         const float vxR = last ? 0.f : face_update((fxR & 2) || ox, cxR, vxr_old, pc, pp);
-        if (!last && !((fxR & 1) || ox)) vo0[idx + sx] = vxR;  // right cell is not PML: this cell owns the face
-        vo0[idx] = vxL;
+        vo0[c] = vxL;
+        if (i - sl.lo[0] == e0 - 1) vo0[c + cs] = vxR;
         const float n0 = cxC.z * q0 + cxC.w * (vxR - vxL);
         // y
         const bool oy = ci || ck;
         const float vyL = face_update(j >= 1 && ((fyL & 2) || oy), cyL, vyl_old, ym, pc);
         const float vyR = (j + 1 < ny) ? face_update((fyR & 2) || oy, cyR, vyr_old, pc, yp) : 0.f;
-        if (j + 1 < ny && !((fyR & 1) || oy)) vo1[idx + sy] = vyR;
-        vo1[idx] = vyL;
+        vo1[yo] = vyL;
+        if (own_yR) vo1[yo + e2] = vyR;
         const float n1 = cyL.z * q1 + cyL.w * (vyR - vyL);
         // z
         const bool oz = ci || cj;
         const float vzL = face_update(k >= 1 && ((fzL & 2) || oz), czL, vzl_old, zm, pc);
         const float vzR = (k + 1 < nz) ? face_update((fzR & 2) || oz, czR, vzr_old, pc, zp) : 0.f;
-        if (k + 1 < nz && !((fzR & 1) || oz)) vo2[idx + 1] = vzR;
-        vo2[idx] = vzL;
+        vo2[zo] = vzL;
+        if (own_zR) vo2[zo + 1] = vzR;
         const float n2 = czL.z * q2 + czL.w * (vzR - vzL);
-        pd0[idx] = n0;
-        pd1[idx] = n1;
-        pd2[idx] = n2;
+        pd0[c] = n0;
+        pd1[c] = n1;
+        pd2[c] = n2;
         pnext[idx] = n0 + n1 + n2;
         vxL = vxR;
         pm = pc;
@@ -191,10 +199,9 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
     if (r >= t.i1) return;
     const SpecialRec rec = S.recs[r];
     const int sx = S.sx, sy = S.sy;
-    const int ind = (int)((1 + step) % 3);
-    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
-    const float *__restrict__ pprev = S.field + (size_t)((ind + 2) % 3) * S.npad;
-    float *__restrict__ pnext = S.field + (size_t)((ind + 1) % 3) * S.npad;
+    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    const float *pprev = S.field + (size_t)slot_next(step) * S.npad;
+    float *pnext = S.field + (size_t)slot_next(step) * S.npad;
     const int idx = rec.cell;
     const double c = pcur[idx];
     const double nb[6] = {pcur[idx - sx], pcur[idx + sx], pcur[idx - sy], pcur[idx + sy], pcur[idx - 1], pcur[idx + 1]};
@@ -260,8 +267,7 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
     const SceneDev &S = scenes[list[blockIdx.x].scene];
     const long long step = (long long)S.clock[CLK_STEP];
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const int ind = (int)((1 + step) % 3);
-    const float *__restrict__ pcur = S.field + (size_t)ind * S.npad;
+    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
     double *gval = S.gval[step & 1];
     double *b = S.gtmp, *x1 = S.gtmp + S.G;
     for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
@@ -293,26 +299,27 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// listeners (PML_WaveSolver::FetchScalarData): one wavefront per listener,
-// FP64 dot product of precomputed MLS weights with the samples, reduced
-// across the wavefront.  The last workgroup of a scene to finish advances
-// the scene's step counter (vector atomics only).
+// listeners (PML_WaveSolver::FetchScalarData): one thread per listener,
+// an FP64 dot product of the precomputed MLS weights with the samples of
+// the eight enclosing cells (or their ghost values).  The last workgroup of
+// a scene to finish advances the scene's step counter and refreshes the
+// per-step amplitudes (vector atomics only).  A workgroup is one wavefront,
+// so the fence before the done count covers all of its stores.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
+__global__ __launch_bounds__(kListenersPerGroup) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
 {
     const ListenDesc gdsc = groups[blockIdx.x];
     const SceneDev &S = scenes[gdsc.scene];
     unsigned long long *clk = S.clock;
     const long long step = (long long)clk[CLK_STEP];
     if (step >= (long long)clk[CLK_TARGET]) return;
-    const int lane = threadIdx.x;
-    if (gdsc.l >= 0) {
-        const int l = gdsc.l;
-        const int ind = (int)((1 + step) % 3);
-        const float *__restrict__ pold = S.field + (size_t)ind * S.npad;
-        const float *__restrict__ pnew = S.field + (size_t)((ind + 1) % 3) * S.npad;
+    const int tid = threadIdx.x;
+    const int l = gdsc.l0 + tid;
+    if (l < gdsc.l1) {
+        const float *__restrict__ pold = S.field + (size_t)slot_cur(step) * S.npad;
+        const float *__restrict__ pnew = S.field + (size_t)slot_next(step) * S.npad;
         double acc = 0.0;
-        for (int q = S.lptr[l] + lane; q < S.lptr[l + 1]; q += 64) {
+        for (int q = S.lptr[l]; q < S.lptr[l + 1]; ++q) {
             const int kind = S.lkind[q], idx = S.lindex[q];
             double v;
             if (kind == 0) {
@@ -325,23 +332,25 @@ __global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__
             }
             acc += S.lbeta[q] * v;
         }
-        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-        if (lane == 0) S.series[(size_t)(step % S.cap) * S.L + l] = acc;
+        S.series[(size_t)(step % S.cap) * S.L + l] = acc;
     }
     // every group of the scene has read this step's amplitudes before it
     // counts itself done, so the last one may overwrite them
-    unsigned long long prev = 0;
-    if (lane == 0) {
+    __shared__ int last;
+    __syncthreads();
+    if (tid == 0) {
         __threadfence();
-        prev = atomicAdd(&clk[CLK_DONE], 1ULL);
+        const unsigned long long prev = atomicAdd(&clk[CLK_DONE], 1ULL);
+        last = (prev + 1 == (unsigned long long)S.n_listen_groups);
     }
-    prev = __shfl(prev, 0, 64);
-    if (prev + 1 == (unsigned long long)S.n_listen_groups) {
+    __syncthreads();
+    if (last) {
         const double tn1 = S.t_start + (double)(step + 1) * S.dt;
-        for (int o = lane; o < S.n_obj; o += 64) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
-        for (int s = lane; s < S.n_psrc; s += 64) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
+        for (int o = tid; o < S.n_obj; o += kListenersPerGroup) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
+        for (int s = tid; s < S.n_psrc; s += kListenersPerGroup) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
         __threadfence();
-        if (lane == 0) {
+        __syncthreads();
+        if (tid == 0) {
             atomicExch(&clk[CLK_DONE], 0ULL);
             atomicExch(&clk[CLK_STEP], (unsigned long long)(step + 1));
         }
@@ -362,7 +371,7 @@ hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, 
 hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s)
 {
     if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(64), 0, s, scenes, groups);
+    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(kListenersPerGroup), 0, s, scenes, groups);
     return hipGetLastError();
 }
 

@@ -251,6 +251,31 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             }
     }
     S.n_pml = S.N - (int64_t)(S.inner_hi[0] - S.inner_lo[0]) * (S.inner_hi[1] - S.inner_lo[1]) * (S.inner_hi[2] - S.inner_lo[2]);
+    {
+        // x slabs span the grid, y slabs the inner x range, z slabs the inner x and y ranges
+        const int *n = S.n, *a = S.inner_lo, *b = S.inner_hi;
+        const int boxes[6][6] = {
+            {0, a[0], 0, n[1], 0, n[2]},    {b[0], n[0], 0, n[1], 0, n[2]},
+            {a[0], b[0], 0, a[1], 0, n[2]}, {a[0], b[0], b[1], n[1], 0, n[2]},
+            {a[0], b[0], a[1], b[1], 0, a[2]}, {a[0], b[0], a[1], b[1], b[2], n[2]},
+        };
+        S.slabs.clear();
+        int64_t cells = 0, faces[3] = {0, 0, 0};
+        for (const auto &bx : boxes) {
+            PmlSlab sl{};
+            for (int d = 0; d < 3; ++d) { sl.lo[d] = bx[2 * d]; sl.e[d] = bx[2 * d + 1] - bx[2 * d]; }
+            if (sl.e[0] <= 0 || sl.e[1] <= 0 || sl.e[2] <= 0) continue;
+            sl.cell_base = cells;
+            cells += (int64_t)sl.e[0] * sl.e[1] * sl.e[2];
+            for (int d = 0; d < 3; ++d) {
+                sl.v_base[d] = faces[d];
+                faces[d] += (int64_t)(sl.e[0] + (d == 0)) * (sl.e[1] + (d == 1)) * (sl.e[2] + (d == 2));
+            }
+            S.slabs.push_back(sl);
+        }
+        if (cells != S.n_pml) { err = "internal: PML slab decomposition does not cover the PML"; return FDTD_ERR_INVALID; }
+        for (int d = 0; d < 3; ++d) S.n_pml_faces[d] = faces[d];
+    }
 
     // ---- pressure sources (GaussianPressureSource) ------------------------
     S.psrc.clear(); S.gtab.clear();

@@ -294,9 +294,9 @@ fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out)
 {
     if (!h || !out) return FDTD_ERR_INVALID;
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
-    const int ind = (int)((1 + h->step) % 3);  // p^{step}: the newest field
+    const int slot = slot_cur(h->step);  // p^{step}: the newest field
     std::vector<float> tmp((size_t)h->D.S.N);
-    FDTD_HIP_TRY(hipMemcpy(tmp.data(), h->D.slot(SLOT_P0 + ind), tmp.size() * 4, hipMemcpyDeviceToHost));
+    FDTD_HIP_TRY(hipMemcpy(tmp.data(), h->D.slot(slot), tmp.size() * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < tmp.size(); ++i) out[i] = tmp[i];
     return FDTD_OK;
 }
