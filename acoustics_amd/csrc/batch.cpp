@@ -49,7 +49,7 @@ fdtd_status BatchRunner::build_graph(hipStream_t st)
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     fdtd_status s = FDTD_OK;
-    for (int k = 0; k < graph_steps && s == FDTD_OK; ++k) s = prog.enqueue(st);
+    for (int k = 0; k < graph_steps && s == FDTD_OK; ++k) s = prog.enqueue(st, k);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
@@ -74,7 +74,7 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
         if (s != FDTD_OK) return s;
         auto run = [&]() -> fdtd_status {
             for (int k = 0; k < NUM_KINDS; ++k)
-                FDTD_HIP_TRY(launch_step(k, prog.d_scenes, prog.d_tiles[k], prog.ntiles[k], st));
+                FDTD_HIP_TRY(launch_step(k, prog.d_scenes, prog.d_tiles[k], prog.ntiles[k], prog.order_clock, st));
             return FDTD_OK;
         };
         if ((s = run()) != FDTD_OK) return s;
@@ -124,7 +124,7 @@ fdtd_status BatchRunner::advance(hipStream_t st, int nsteps)
             left -= graph_steps;
             ++blocks;
         } else {
-            fdtd_status s = prog.enqueue(st);
+            fdtd_status s = prog.enqueue(st, taken);
             if (s != FDTD_OK) return s;
             taken += 1;
             left -= 1;

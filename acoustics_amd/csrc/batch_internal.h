@@ -20,7 +20,7 @@ struct BatchRunner {
     hipGraphExec_t exec = nullptr;
     long long taken = 0;                              // batch steps launched
     long long blocks = 0;
-    float timings[kNumTilePlans] = {0, 0, 0, 0, 0, 0};
+    float timings[kNumTilePlans] = {};
 
     fdtd_status rebuild(hipStream_t st);              // work list of unfinished scenes
     fdtd_status build_graph(hipStream_t st);

@@ -134,6 +134,10 @@ TilePlan tile_plan(int id)
     case 3: p.interior_tk_log2 = 6; p.interior_ti = 32; p.pml_ti = 16; break;
     case 4: p.interior_tk_log2 = 7; p.interior_ti = 16; p.pml_ti = 16; break;
     case 5: p.interior_tk_log2 = 5; p.interior_ti = 32; p.pml_ti = 32; break;
+    case 6: p.interior_tk_log2 = 6; p.interior_ti = 4; p.pml_ti = 4; break;
+    case 7: p.interior_tk_log2 = 6; p.interior_ti = 2; p.pml_ti = 2; break;
+    case 8: p.interior_tk_log2 = 6; p.interior_ti = 1; p.pml_ti = 1; break;
+    case 9: p.interior_tk_log2 = 6; p.interior_ti = 8; p.pml_ti = 2; break;
     default: break;  // 0: 64 x 4, 16 planes
     }
     return p;
@@ -420,6 +424,7 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     if (!listen.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_listen, listen.data(), sizeof(ListenDesc) * listen.size(), hipMemcpyHostToDevice, st));
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
+    order_clock = scenes.empty() ? nullptr : scenes[0].clock + CLK_STEP;
     FDTD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
     FDTD_HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     FDTD_HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
@@ -439,25 +444,34 @@ void StepProgram::release()
     if (side) (void)hipStreamDestroy(side);
     fork = join = nullptr;
     side = nullptr;
+    order_clock = nullptr;
 }
 
-fdtd_status StepProgram::enqueue(hipStream_t st, hipEvent_t ev_begin, hipEvent_t ev_end) const
+fdtd_status StepProgram::enqueue(hipStream_t st, long long step, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
-    // the PML slabs carry most of the bytes: they run alone on the main
-    // stream while the object cells and then the interior run beside them
-    const bool split = side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
-    hipStream_t aux = split ? side : st;
+    auto launch = [&](int kind, hipStream_t s) { return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], order_clock, s); };
+    const bool split = concurrent && side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
     if (split) {
+        // the PML slabs carry most of the bytes: they run alone on the main
+        // stream while the object cells and then the interior run beside them
         FDTD_HIP_TRY(hipEventRecord(fork, st));
         FDTD_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+        FDTD_HIP_TRY(launch(KIND_SPECIAL, side));
+        FDTD_HIP_TRY(launch(KIND_INTERIOR, side));
+        FDTD_HIP_TRY(hipEventRecord(join, side));
+        FDTD_HIP_TRY(launch(KIND_PML, st));
+        FDTD_HIP_TRY(hipStreamWaitEvent(st, join, 0));
+    } else if ((step & 1) == 0) {
+        FDTD_HIP_TRY(launch(KIND_SPECIAL, st));
+        FDTD_HIP_TRY(launch(KIND_INTERIOR, st));
+        FDTD_HIP_TRY(launch(KIND_PML, st));
+    } else {
+        FDTD_HIP_TRY(launch(KIND_PML, st));
+        FDTD_HIP_TRY(launch(KIND_INTERIOR, st));
+        FDTD_HIP_TRY(launch(KIND_SPECIAL, st));
     }
-    FDTD_HIP_TRY(launch_step(KIND_SPECIAL, d_scenes, d_tiles[KIND_SPECIAL], ntiles[KIND_SPECIAL], aux));
-    FDTD_HIP_TRY(launch_step(KIND_INTERIOR, d_scenes, d_tiles[KIND_INTERIOR], ntiles[KIND_INTERIOR], aux));
-    if (split) FDTD_HIP_TRY(hipEventRecord(join, side));
-    FDTD_HIP_TRY(launch_step(KIND_PML, d_scenes, d_tiles[KIND_PML], ntiles[KIND_PML], st));
-    if (split) FDTD_HIP_TRY(hipStreamWaitEvent(st, join, 0));
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
     FDTD_HIP_TRY(launch_listen(d_scenes, d_listen, nlisten, st));
     return FDTD_OK;

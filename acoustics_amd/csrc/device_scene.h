@@ -27,7 +27,7 @@ struct TilePlan {
     int interior_ti = 16;      // planes marched per interior tile
     int pml_ti = 16;           // planes marched per PML tile
 };
-constexpr int kNumTilePlans = 6;
+constexpr int kNumTilePlans = 10;
 TilePlan tile_plan(int id);
 
 // Device buffers and uploaded tables of one scene, all in one hipMalloc arena.
@@ -62,10 +62,16 @@ struct DeviceScene {
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 
 // The per-step launch sequence of a set of scenes, uploaded once:
-//   ghost solve (fully-coupled scenes) -> {PML slabs on the main stream ||
-//   object cells then interior on the side stream} -> listeners + step advance.
-// The three step kernels write disjoint cells; the fork/join lets them
-// overlap (and is captured as parallel branches of a graph).
+//   ghost solve (fully-coupled scenes) -> the three step kernels ->
+//   listeners + step advance.
+// The step kernels write disjoint cells.  By default they run back to back
+// on one stream in an order that alternates with the step parity (object
+// cells, interior, PML on even steps; the reverse on odd steps), and the
+// tiles inside each kernel alternate direction too, so each step begins on
+// the data the previous step touched last.  With `concurrent` set, the PML
+// slabs run on the main stream while the object cells and the interior run
+// on a side stream (a fork/join that graph capture keeps as parallel
+// branches).
 struct StepProgram {
     SceneDev *d_scenes = nullptr;
     TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
@@ -73,14 +79,18 @@ struct StepProgram {
     GhostDesc *d_ghost = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
     int nlisten = 0, nghost = 0;
+    bool concurrent = false;
+    const unsigned long long *order_clock = nullptr;  // step word of the first scene
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     // tiles of every kind in one list; split by type here
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                        const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st);
     void release();
-    // ev_begin/ev_end bracket the three step kernels on st
-    fdtd_status enqueue(hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
+    // enqueues one step; `step` (the step it will run, or any value of the
+    // same parity) picks the kernel order.  ev_begin/ev_end bracket the
+    // three step kernels on st.
+    fdtd_status enqueue(hipStream_t st, long long step, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
 };
 
 }  // namespace fdtd

@@ -100,10 +100,12 @@ struct TileDesc {
     int pad;
 };
 
-// One listener workgroup: listeners [l0, l1) of a scene, one thread each
-// (an order-1 MLS listener has about eight terms).  A scene without
-// listeners gets one group with l0 == l1 that only advances the step.
-constexpr int kListenersPerGroup = 64;
+// One listener workgroup (one wavefront): listeners [l0, l1) of a scene,
+// eight lanes each (an order-1 MLS listener has eight enclosing-cell terms;
+// more are looped over).  A scene without listeners gets one group with
+// l0 == l1 that only advances the step.
+constexpr int kLanesPerListener = 8;
+constexpr int kListenersPerGroup = 64 / kLanesPerListener;
 struct ListenDesc {
     int scene;
     int l0, l1;
@@ -114,7 +116,12 @@ struct GhostDesc {
 };
 
 enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3 };
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s);
+// order_clock: when non-null, the step word of a scene clock whose parity
+// selects the tile order -- odd steps walk each XCD's chunk of the tile list
+// backwards, so a step starts on the data the previous step touched last and
+// the Infinity Cache still holds.
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles,
+                       const unsigned long long *order_clock, hipStream_t s);
 hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s);
 hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s);
 

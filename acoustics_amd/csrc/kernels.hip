@@ -20,6 +20,27 @@
 
 namespace fdtd {
 
+// Pointers read out of a SceneDev are generic to the compiler, which then
+// emits flat memory instructions; those count against both the vector-memory
+// and the LDS counters, so every wait becomes a full wait.  Every scene array
+// lives in device global memory: casting to the global address space gives
+// global_load/global_store with vmcnt-only waits, and read-only tables cast to
+// the constant address space are fetched with scalar loads when the index is
+// uniform across the wavefront.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define AS_GLOBAL __attribute__((address_space(1)))
+#define AS_CONST __attribute__((address_space(4)))
+#else  // the host pass only parses device code
+#define AS_GLOBAL
+#define AS_CONST
+#endif
+template <class T>
+__device__ __forceinline__ AS_GLOBAL T *gmem(T *p) { return (AS_GLOBAL T *)p; }
+template <class T>
+__device__ __forceinline__ const AS_GLOBAL T *gmem(const T *p) { return (const AS_GLOBAL T *)p; }
+template <class T>
+__device__ __forceinline__ const AS_CONST T *cmem(const T *p) { return (const AS_CONST T *)p; }
+
 __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int obj, double t)
 {
     double a = 0.0;
@@ -49,10 +70,10 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     const int k = t.k0 + tx, j = t.j0 + ty;
     if (k >= t.k1 || j >= t.j1) return;
     const int sx = S.sx, sy = S.sy;
-    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
     // p^{n+1} overwrites p^{n-1}: each thread reads and writes only its own cells there
-    const float *pprev = S.field + (size_t)slot_next(step) * S.npad;
-    float *pnext = S.field + (size_t)slot_next(step) * S.npad;
+    const AS_GLOBAL float *pprev = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     const float lam2 = S.lam2f;
 
     int idx = t.i0 * sx + j * sy + k;
@@ -80,10 +101,11 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
         float pn = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
         if (t.src)
             for (int s = 0; s < S.n_psrc; ++s) {
-                const PSrcDev ps = S.psrc[s];
-                pn += S.src_now[s] * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k];
+                const PSrcDev ps = cmem(S.psrc)[s];
+                const AS_CONST float *g = cmem(S.gtab);
+                pn += gmem(S.src_now)[s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
             }
-        if (!skip_check || !S.skip[idx]) pnext[idx] = pn;
+        if (!skip_check || !gmem(S.skip)[idx]) pnext[idx] = pn;
         pm = pc;
         pc = pp;
     }
@@ -107,32 +129,34 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     if (k >= t.k1 || j >= t.j1) return;
     const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
     const int sx = S.sx, sy = S.sy;
-    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
-    float *__restrict__ pnext = S.field + (size_t)slot_next(step) * S.npad;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *__restrict__ pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     // compact storage of this slab: cells (and x faces) have plane stride
     // e1*e2; y faces (e1+1)*e2; z faces e1*(e2+1)
-    const PmlSlab &sl = S.slabs[t.slab];
+    const PmlSlab sl = cmem(S.slabs)[t.slab];
     const int e0 = sl.e[0], e1 = sl.e[1], e2 = sl.e[2];
     const int li0 = t.i0 - sl.lo[0], lj = j - sl.lo[1], lk = k - sl.lo[2];
     const int cs = e1 * e2, ys = (e1 + 1) * e2, zs = e1 * (e2 + 1);
     const int vb = (int)(step & 1);
-    float *__restrict__ pd0 = S.pd[0] + sl.cell_base;
-    float *__restrict__ pd1 = S.pd[1] + sl.cell_base;
-    float *__restrict__ pd2 = S.pd[2] + sl.cell_base;
-    const float *__restrict__ vi0 = S.vel[vb][0] + sl.v_base[0];
-    const float *__restrict__ vi1 = S.vel[vb][1] + sl.v_base[1];
-    const float *__restrict__ vi2 = S.vel[vb][2] + sl.v_base[2];
-    float *__restrict__ vo0 = S.vel[vb ^ 1][0] + sl.v_base[0];
-    float *__restrict__ vo1 = S.vel[vb ^ 1][1] + sl.v_base[1];
-    float *__restrict__ vo2 = S.vel[vb ^ 1][2] + sl.v_base[2];
-    const float4 *__restrict__ cx = S.coef[0];
-    const int *__restrict__ fx = S.flags[0];
+    AS_GLOBAL float *__restrict__ pd0 = gmem(S.pd[0] + sl.cell_base);
+    AS_GLOBAL float *__restrict__ pd1 = gmem(S.pd[1] + sl.cell_base);
+    AS_GLOBAL float *__restrict__ pd2 = gmem(S.pd[2] + sl.cell_base);
+    const AS_GLOBAL float *__restrict__ vi0 = gmem(S.vel[vb][0] + sl.v_base[0]);
+    const AS_GLOBAL float *__restrict__ vi1 = gmem(S.vel[vb][1] + sl.v_base[1]);
+    const AS_GLOBAL float *__restrict__ vi2 = gmem(S.vel[vb][2] + sl.v_base[2]);
+    AS_GLOBAL float *__restrict__ vo0 = gmem(S.vel[vb ^ 1][0] + sl.v_base[0]);
+    AS_GLOBAL float *__restrict__ vo1 = gmem(S.vel[vb ^ 1][1] + sl.v_base[1]);
+    AS_GLOBAL float *__restrict__ vo2 = gmem(S.vel[vb ^ 1][2] + sl.v_base[2]);
+    const AS_CONST float4 *__restrict__ cx = cmem(S.coef[0]);
+    const AS_CONST int *__restrict__ fx = cmem(S.flags[0]);
 
     // y and z coefficients are constant along the column
-    const int fyL = S.flags[1][j], fzL = S.flags[2][k];
-    const int fyR = (j + 1 < ny) ? S.flags[1][j + 1] : 0, fzR = (k + 1 < nz) ? S.flags[2][k + 1] : 0;
-    const float4 cyL = S.coef[1][j], czL = S.coef[2][k];
-    const float4 cyR = S.coef[1][min(j + 1, ny)], czR = S.coef[2][min(k + 1, nz)];
+    const AS_CONST int *fy = cmem(S.flags[1]), *fz = cmem(S.flags[2]);
+    const AS_CONST float4 *cy = cmem(S.coef[1]), *cz = cmem(S.coef[2]);
+    const int fyL = fy[j], fzL = fz[k];
+    const int fyR = (j + 1 < ny) ? fy[j + 1] : 0, fzR = (k + 1 < nz) ? fz[k + 1] : 0;
+    const float4 cyL = cy[j], czL = cz[k];
+    const float4 cyR = cy[min(j + 1, ny)], czR = cz[min(k + 1, nz)];
     const bool cj = fyL & 1, ck = fzL & 1;
     const int ym_off = (j > 0) ? sy : 0, yp_off = (j < ny - 1) ? sy : 0;
     const int zm_off = (k > 0) ? 1 : 0, zp_off = (k < nz - 1) ? 1 : 0;
@@ -197,11 +221,11 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
 {
     const int r = t.i0 + (int)threadIdx.x;
     if (r >= t.i1) return;
-    const SpecialRec rec = S.recs[r];
+    const SpecialRec rec = gmem(S.recs)[r];
     const int sx = S.sx, sy = S.sy;
-    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
-    const float *pprev = S.field + (size_t)slot_next(step) * S.npad;
-    float *pnext = S.field + (size_t)slot_next(step) * S.npad;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    const AS_GLOBAL float *pprev = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     const int idx = rec.cell;
     const double c = pcur[idx];
     const double nb[6] = {pcur[idx - sx], pcur[idx + sx], pcur[idx - sy], pcur[idx + sy], pcur[idx - 1], pcur[idx + 1]};
@@ -211,10 +235,10 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         const int gid = rec.gid[q];
         if (gid >= 0) {
             if (S.raster) {
-                const GhostDev gd = S.ghosts[gid];
-                g[q] = c + gd.k_raster * S.acc_now[gd.obj];
+                const GhostDev gd = gmem(S.ghosts)[gid];
+                g[q] = c + gd.k_raster * gmem(S.acc_now)[gd.obj];
             } else {
-                g[q] = S.gval[step & 1][gid];
+                g[q] = gmem(S.gval[step & 1])[gid];
             }
         }
     }
@@ -235,20 +259,32 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
     if (S.n_psrc > 0) {
         const int i = idx / sx, j = (idx - i * sx) / sy, k = idx - i * sx - j * sy;
         for (int s = 0; s < S.n_psrc; ++s) {
-            const PSrcDev ps = S.psrc[s];
-            pn += (double)(S.src_now[s] * S.gtab[ps.goff[0] + i] * S.gtab[ps.goff[1] + j] * S.gtab[ps.goff[2] + k]);
+            const PSrcDev ps = cmem(S.psrc)[s];
+            const AS_CONST float *g3 = cmem(S.gtab);
+            pn += (double)(gmem(S.src_now)[s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
         }
     }
     pnext[idx] = (float)pn;
 }
 
+// Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
+// eight and arranged so that workgroup w (dispatched to XCD w % 8) runs chunk
+// w % 8; the reversed order keeps that assignment and reverses each chunk.
+__device__ __forceinline__ int tile_of(int w, int ntiles, const unsigned long long *order_clock)
+{
+    if (!order_clock || !(*order_clock & 1ULL)) return w;
+    if (ntiles >= 16 && (ntiles & 7) == 0) return (ntiles / 8 - 1 - w / 8) * 8 + (w & 7);
+    return ntiles - 1 - w;
+}
+
 // One kernel per tile class, so each class gets its own register allocation
 // (the FP64 object path would otherwise cap the occupancy of the streaming
-// paths).  The three classes write disjoint cells and can run concurrently.
+// paths).  The three classes write disjoint cells and can run in any order.
 template <int KIND>
-__global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles)
+__global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles,
+                                                   int ntiles, const unsigned long long *__restrict__ order_clock)
 {
-    const TileDesc t = tiles[blockIdx.x];
+    const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, order_clock)];
     if (t.type == TILE_EMPTY) return;
     const SceneDev &S = scenes[t.scene];
     const long long step = (long long)S.clock[CLK_STEP];
@@ -299,41 +335,44 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// listeners (PML_WaveSolver::FetchScalarData): one thread per listener,
+// listeners (PML_WaveSolver::FetchScalarData): eight lanes per listener,
 // an FP64 dot product of the precomputed MLS weights with the samples of
-// the eight enclosing cells (or their ghost values).  The last workgroup of
-// a scene to finish advances the scene's step counter and refreshes the
-// per-step amplitudes (vector atomics only).  A workgroup is one wavefront,
-// so the fence before the done count covers all of its stores.
+// the eight enclosing cells (or their ghost values), reduced across the
+// eight lanes.  The last workgroup of a scene to finish advances the
+// scene's step counter and refreshes the per-step amplitudes (vector atomics
+// only).  A workgroup is one wavefront, so the fence before the done count
+// covers all of its stores.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kListenersPerGroup) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
+__global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
 {
-    const ListenDesc gdsc = groups[blockIdx.x];
+    const ListenDesc gdsc = gmem(groups)[blockIdx.x];
     const SceneDev &S = scenes[gdsc.scene];
     unsigned long long *clk = S.clock;
     const long long step = (long long)clk[CLK_STEP];
     if (step >= (long long)clk[CLK_TARGET]) return;
     const int tid = threadIdx.x;
-    const int l = gdsc.l0 + tid;
+    const int l = gdsc.l0 + tid / kLanesPerListener, sub = tid % kLanesPerListener;
+    double acc = 0.0;
     if (l < gdsc.l1) {
-        const float *__restrict__ pold = S.field + (size_t)slot_cur(step) * S.npad;
-        const float *__restrict__ pnew = S.field + (size_t)slot_next(step) * S.npad;
-        double acc = 0.0;
-        for (int q = S.lptr[l]; q < S.lptr[l + 1]; ++q) {
-            const int kind = S.lkind[q], idx = S.lindex[q];
+        const AS_GLOBAL float *__restrict__ pold = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+        const AS_GLOBAL float *__restrict__ pnew = gmem(S.field + (size_t)slot_next(step) * S.npad);
+        const int q1 = gmem(S.lptr)[l + 1];
+        for (int q = gmem(S.lptr)[l] + sub; q < q1; q += kLanesPerListener) {
+            const int kind = gmem(S.lkind)[q], idx = gmem(S.lindex)[q];
             double v;
             if (kind == 0) {
                 v = (double)pnew[idx];
             } else if (S.raster) {
-                const GhostDev gd = S.ghosts[idx];
-                v = (double)pold[gd.nbr] + gd.k_raster * S.acc_now[gd.obj];
+                const GhostDev gd = gmem(S.ghosts)[idx];
+                v = (double)pold[gd.nbr] + gd.k_raster * gmem(S.acc_now)[gd.obj];
             } else {
-                v = S.gval[step & 1][idx];
+                v = gmem(S.gval[step & 1])[idx];
             }
-            acc += S.lbeta[q] * v;
+            acc += gmem(S.lbeta)[q] * v;
         }
-        S.series[(size_t)(step % S.cap) * S.L + l] = acc;
     }
+    for (int off = kLanesPerListener / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kLanesPerListener);
+    if (l < gdsc.l1 && sub == 0) gmem(S.series)[(size_t)(step % S.cap) * S.L + l] = acc;
     // every group of the scene has read this step's amplitudes before it
     // counts itself done, so the last one may overwrite them
     __shared__ int last;
@@ -346,8 +385,8 @@ __global__ __launch_bounds__(kListenersPerGroup) void listen_kernel(const SceneD
     __syncthreads();
     if (last) {
         const double tn1 = S.t_start + (double)(step + 1) * S.dt;
-        for (int o = tid; o < S.n_obj; o += kListenersPerGroup) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
-        for (int s = tid; s < S.n_psrc; s += kListenersPerGroup) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
+        for (int o = tid; o < S.n_obj; o += 64) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
+        for (int s = tid; s < S.n_psrc; s += 64) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
         __threadfence();
         __syncthreads();
         if (tid == 0) {
@@ -357,13 +396,14 @@ __global__ __launch_bounds__(kListenersPerGroup) void listen_kernel(const SceneD
     }
 }
 
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, hipStream_t s)
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles,
+                       const unsigned long long *order_clock, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     switch (kind) {
-    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
-    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
-    default: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles); break;
+    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
+    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
+    default: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
     }
     return hipGetLastError();
 }
@@ -371,7 +411,7 @@ hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, 
 hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s)
 {
     if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(kListenersPerGroup), 0, s, scenes, groups);
+    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(64), 0, s, scenes, groups);
     return hipGetLastError();
 }
 

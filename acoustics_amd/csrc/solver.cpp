@@ -176,7 +176,7 @@ static fdtd_status build_graph(fdtd_handle *h)
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     fdtd_status st = FDTD_OK;
-    for (int o = 0; o < h->graph_steps && st == FDTD_OK; ++o) st = h->prog.enqueue(h->stream);
+    for (int o = 0; o < h->graph_steps && st == FDTD_OK; ++o) st = h->prog.enqueue(h->stream, o);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (st != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return st; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
@@ -205,7 +205,9 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
     }
     int left = nsteps, done = 0;
     while (left > 0) {
-        const bool use_graph = !h->timing && h->graph_steps > 0 && left >= h->graph_steps;
+        // graphs are captured for a block that starts on an even step (the
+        // kernel order alternates with the step parity)
+        const bool use_graph = !h->timing && h->graph_steps > 0 && left >= h->graph_steps && (h->step & 1) == 0;
         const int block = use_graph ? h->graph_steps : 1;
         if (h->step + block - h->flushed > cap) {
             fdtd_status st = flush_series(h);
@@ -220,7 +222,7 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
         } else {
             hipEvent_t b = nullptr, e = nullptr;
             if (h->timing) { b = h->ev[2 * done]; e = h->ev[2 * done + 1]; }
-            fdtd_status st = h->prog.enqueue(h->stream, b, e);
+            fdtd_status st = h->prog.enqueue(h->stream, h->step, b, e);
             if (st != FDTD_OK) return st;
         }
         h->step += block;
