@@ -127,20 +127,12 @@ void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
 
 TilePlan tile_plan(int id)
 {
-    TilePlan p;
-    switch (id) {
-    case 1: p.interior_tk_log2 = 6; p.interior_ti = 8; p.pml_ti = 8; break;
-    case 2: p.interior_tk_log2 = 5; p.interior_ti = 16; p.pml_ti = 16; break;
-    case 3: p.interior_tk_log2 = 6; p.interior_ti = 32; p.pml_ti = 16; break;
-    case 4: p.interior_tk_log2 = 7; p.interior_ti = 16; p.pml_ti = 16; break;
-    case 5: p.interior_tk_log2 = 5; p.interior_ti = 32; p.pml_ti = 32; break;
-    case 6: p.interior_tk_log2 = 6; p.interior_ti = 4; p.pml_ti = 4; break;
-    case 7: p.interior_tk_log2 = 6; p.interior_ti = 2; p.pml_ti = 2; break;
-    case 8: p.interior_tk_log2 = 6; p.interior_ti = 1; p.pml_ti = 1; break;
-    case 9: p.interior_tk_log2 = 6; p.interior_ti = 8; p.pml_ti = 2; break;
-    default: break;  // 0: 64 x 4, 16 planes
-    }
-    return p;
+    // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior}
+    static const TilePlan plans[kNumTilePlans] = {
+        {6, 4, 8, true},  {6, 8, 8, false}, {6, 8, 8, true},   {6, 4, 4, true},  {6, 2, 8, true},
+        {5, 16, 16, false}, {6, 4, 4, false}, {6, 8, 4, true}, {6, 16, 16, false}, {6, 16, 16, true},
+    };
+    return plans[(id >= 0 && id < kNumTilePlans) ? id : 0];
 }
 
 size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)
@@ -345,16 +337,19 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         t.i1 = std::min(r0 + 256, (int)S.special.size());
         out.push_back(t);
     }
-    // interior
+    // interior: whole-row float4 tiles of four rows where the layout allows
+    // and the rows hold no source and no object cell; 2D chunks otherwise
     const int tk = std::min(plan.interior_tk_log2, std::max(0, pow2_ceil_log2(b[2] - a[2])));
     const int TK = 1 << tk, TJ = 256 >> tk;
-    for (int i0 = a[0]; i0 < b[0]; i0 += plan.interior_ti)
-        for (int j0 = a[1]; j0 < b[1]; j0 += TJ)
+    const int kw = b[2] - a[2];
+    const bool vec = plan.interior_vec && n[2] % 4 == 0 && a[2] % 4 == 0 && kw % 4 == 0 && kw <= 256;
+    auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
+        for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
                 TileDesc t{};
                 t.scene = scene_id;
-                t.i0 = i0; t.i1 = std::min(i0 + plan.interior_ti, b[0]);
-                t.j0 = j0; t.j1 = std::min(j0 + TJ, b[1]);
+                t.i0 = i0; t.i1 = i1;
+                t.j0 = j0; t.j1 = std::min(j0 + TJ, jhi);
                 t.k0 = k0; t.k1 = std::min(k0 + TK, b[2]);
                 t.tk_log2 = tk;
                 const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
@@ -362,6 +357,22 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                 t.src = src_hit(lo, hi);
                 out.push_back(t);
             }
+    };
+    for (int i0 = a[0]; i0 < b[0]; i0 += plan.interior_ti) {
+        const int i1 = std::min(i0 + plan.interior_ti, b[0]);
+        if (!vec) { scalar_tiles(i0, i1, a[1], b[1]); continue; }
+        for (int j0 = a[1]; j0 < b[1]; j0 += 4) {
+            const int j1 = std::min(j0 + 4, b[1]);
+            const int lo[3] = {i0, j0, a[2]}, hi[3] = {i1, j1, b[2]};
+            if (special_hit(lo, hi) || src_hit(lo, hi)) { scalar_tiles(i0, i1, j0, j1); continue; }
+            TileDesc t{};
+            t.scene = scene_id;
+            t.type = TILE_INTERIOR_VEC;
+            t.i0 = i0; t.i1 = i1; t.j0 = j0; t.j1 = j1; t.k0 = a[2]; t.k1 = b[2];
+            t.tk_log2 = 6;
+            out.push_back(t);
+        }
+    }
 }
 
 void DeviceScene::append_listen(int scene_id, std::vector<ListenDesc> &out) const
@@ -387,14 +398,24 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
 {
     release();
     std::vector<TileDesc> by_kind[NUM_KINDS];
+    std::vector<TileDesc> special;
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
-        const int k = t.type == TILE_PML ? KIND_PML : t.type == TILE_SPECIAL ? KIND_SPECIAL : KIND_INTERIOR;
-        by_kind[k].push_back(t);
+        if (t.type == TILE_SPECIAL) special.push_back(t);
+        else by_kind[t.type == TILE_PML ? KIND_PML : KIND_INTERIOR].push_back(t);
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
     xcd_permute(by_kind[KIND_INTERIOR]);
     xcd_permute(by_kind[KIND_PML]);
+    // the object-adjacent cells ride at the head of the interior launch, one
+    // tile per XCD in turn; the prefix is padded to a multiple of eight so
+    // each interior chunk still lands on a single XCD
+    if (!special.empty()) {
+        TileDesc empty{};
+        empty.type = TILE_EMPTY;
+        while (!by_kind[KIND_INTERIOR].empty() && special.size() % 8) special.push_back(empty);
+        by_kind[KIND_INTERIOR].insert(by_kind[KIND_INTERIOR].begin(), special.begin(), special.end());
+    }
     size_t bytes[4 + NUM_KINDS];
     bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, scenes.size()));
     bytes[1] = align_up(sizeof(ListenDesc) * std::max<size_t>(1, listen.size()));

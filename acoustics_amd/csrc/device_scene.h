@@ -26,6 +26,7 @@ struct TilePlan {
     int interior_tk_log2 = 6;  // 64 threads along k, 4 rows along j
     int interior_ti = 16;      // planes marched per interior tile
     int pml_ti = 16;           // planes marched per PML tile
+    bool interior_vec = false; // whole-row float4 interior tiles where the layout allows
 };
 constexpr int kNumTilePlans = 10;
 TilePlan tile_plan(int id);

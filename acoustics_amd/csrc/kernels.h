@@ -88,13 +88,13 @@ struct SceneDev {
 // Amplitudes of step s on the host (used to seed step 0).
 void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src);
 
-enum : int { TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4 };
+enum : int { TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4, TILE_INTERIOR_VEC = 5 };
 
 struct TileDesc {
     int scene;
     int type;
     int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records
-    int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK
+    int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK (TILE_INTERIOR_VEC: 4 whole rows)
     int src;                     // tile intersects a pressure-source box
     int slab;                    // TILE_PML: index into SceneDev::slabs
     int pad;

@@ -111,6 +111,45 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     }
 }
 
+// Whole rows, four floats per lane: wavefront w of the workgroup owns row
+// j0 + w, lane l the cells k0 + 4l .. k0 + 4l + 3, so one load moves up to
+// 1 KiB of a row.  The k neighbours come from the adjacent lanes; the two row
+// ends read theirs from memory.  Plain cells only (no source, no object
+// cell); k0 and the row stride are multiples of four.
+__device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileDesc &t, long long step)
+{
+    const int lane = threadIdx.x & 63, j = t.j0 + (int)(threadIdx.x >> 6);
+    if (j >= t.j1) return;
+    const int nk4 = (t.k1 - t.k0) >> 2;
+    const bool act = lane < nk4;
+    const int k = t.k0 + 4 * (act ? lane : nk4 - 1);
+    const int sx = S.sx, sy = S.sy;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    const float lam2 = S.lam2f;
+    int idx = t.i0 * sx + j * sy + k;
+    float4 pm = *(const AS_GLOBAL float4 *)(pcur + idx - sx);
+    float4 pc = *(const AS_GLOBAL float4 *)(pcur + idx);
+    for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+        const float4 pp = *(const AS_GLOBAL float4 *)(pcur + idx + sx);
+        const float4 ym = *(const AS_GLOBAL float4 *)(pcur + idx - sy);
+        const float4 yp = *(const AS_GLOBAL float4 *)(pcur + idx + sy);
+        const float4 prev = *(const AS_GLOBAL float4 *)(pnext + idx);
+        const float zl_row = pcur[idx - 1], zr_row = pcur[idx + 4];
+        float zl = __shfl_up(pc.w, 1), zr = __shfl_down(pc.x, 1);
+        if (lane == 0) zl = zl_row;
+        if (lane == nk4 - 1) zr = zr_row;
+        float4 o;
+        o.x = 2.f * pc.x - prev.x + lam2 * (((pm.x + pp.x) + (ym.x + yp.x) + (zl + pc.y)) - 6.f * pc.x);
+        o.y = 2.f * pc.y - prev.y + lam2 * (((pm.y + pp.y) + (ym.y + yp.y) + (pc.x + pc.z)) - 6.f * pc.y);
+        o.z = 2.f * pc.z - prev.z + lam2 * (((pm.z + pp.z) + (ym.z + yp.z) + (pc.y + pc.w)) - 6.f * pc.z);
+        o.w = 2.f * pc.w - prev.w + lam2 * (((pm.w + pp.w) + (ym.w + yp.w) + (pc.z + zr)) - 6.f * pc.w);
+        if (act) *(AS_GLOBAL float4 *)(pnext + idx) = o;
+        pm = pc;
+        pc = pp;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // PML slab: fused staggered velocity + split pressure update
 // ---------------------------------------------------------------------------
@@ -289,9 +328,17 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
     const SceneDev &S = scenes[t.scene];
     const long long step = (long long)S.clock[CLK_STEP];
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    if constexpr (KIND == KIND_INTERIOR) interior_tile(S, t, step, t.type == TILE_INTERIOR_SKIP);
-    else if constexpr (KIND == KIND_PML) pml_tile(S, t, step);
-    else special_tile(S, t, step);
+    if constexpr (KIND == KIND_INTERIOR) {
+        // the interior launch also carries the object-adjacent cells, whose
+        // FP64 gathers then hide under the interior's streaming
+        if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, step);
+        else if (t.type == TILE_SPECIAL) special_tile(S, t, step);
+        else interior_tile(S, t, step, t.type == TILE_INTERIOR_SKIP);
+    } else if constexpr (KIND == KIND_PML) {
+        pml_tile(S, t, step);
+    } else {
+        special_tile(S, t, step);
+    }
 }
 
 // ---------------------------------------------------------------------------
