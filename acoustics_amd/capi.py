@@ -132,6 +132,7 @@ SIGNATURES = {
     "fdtd_server_cancel": (C.c_int, [_P, C.c_int64]),
     "fdtd_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "fdtd_scene_device_bytes": (C.c_int64, [C.POINTER(Scene)]),
+    "fdtd_plan_check": (C.c_int, [C.POINTER(Scene), C.c_int, C.POINTER(C.c_int64)]),
     "fdtd_server_destroy": (C.c_int, [_P]),
 }
 

@@ -22,20 +22,19 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
     drop_graph();
     std::vector<SceneDev> devs;
     std::vector<TileDesc> tiles;
-    std::vector<ListenDesc> listen;
+    std::vector<TermRef> terms;
     std::vector<GhostDesc> ghost;
     members.clear();
     for (size_t i = 0; i < scenes.size(); ++i) {
         if (!scenes[i] || done[i]) continue;
         const int id = (int)devs.size();
         devs.push_back(scenes[i]->dev);
-        scenes[i]->append_tiles(id, tile_plan(plan), tiles);
-        scenes[i]->append_listen(id, listen);
+        scenes[i]->append_tiles(id, tile_plan(plan), tiles, terms);
         if (scenes[i]->needs_ghost_kernel()) ghost.push_back({id});
         members.push_back((int)i);
     }
     if (devs.empty()) { prog.release(); return FDTD_OK; }
-    return prog.upload(devs, tiles, listen, ghost, st);
+    return prog.upload(devs, tiles, terms, ghost, st, tile_plan(plan).fused);
 }
 
 void BatchRunner::drop_graph()
@@ -59,8 +58,9 @@ fdtd_status BatchRunner::build_graph(hipStream_t st)
     return FDTD_OK;
 }
 
-// Times each tile plan's three step kernels on the batch at step 0 and keeps
-// the fastest; the fields are cleared afterwards (the clocks never moved).
+// Times each tile plan's step program (three steps after one untimed step)
+// on the batch and keeps the fastest.  The timed steps advance the scenes,
+// so every scene is put back at step 0 afterwards.
 fdtd_status BatchRunner::autotune(hipStream_t st)
 {
     float best = 1e30f;
@@ -72,28 +72,22 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
         plan = p;
         fdtd_status s = rebuild(st);
         if (s != FDTD_OK) return s;
-        auto run = [&]() -> fdtd_status {
-            for (int k = 0; k < NUM_KINDS; ++k)
-                FDTD_HIP_TRY(launch_step(k, prog.d_scenes, prog.d_tiles[k], prog.ntiles[k], prog.order_clock, st));
-            return FDTD_OK;
-        };
-        if ((s = run()) != FDTD_OK) return s;
+        if ((s = prog.enqueue(st, 0)) != FDTD_OK) return s;
         FDTD_HIP_TRY(hipEventRecord(a, st));
-        for (int r = 0; r < 3; ++r)
-            if ((s = run()) != FDTD_OK) return s;
+        for (int r = 1; r <= 3; ++r)
+            if ((s = prog.enqueue(st, r)) != FDTD_OK) return s;
         FDTD_HIP_TRY(hipEventRecord(b, st));
         FDTD_HIP_TRY(hipEventSynchronize(b));
         float ms = 0;
         FDTD_HIP_TRY(hipEventElapsedTime(&ms, a, b));
         timings[p] = ms / 3.0f;
         if (ms < best) { best = ms; best_plan = p; }
+        for (auto &sc : scenes)
+            if (sc) { fdtd_status z = sc->reset_to_start(st); if (z != FDTD_OK) return z; }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     plan = best_plan;
-    for (auto &sc : scenes)
-        if (sc) FDTD_HIP_TRY(sc->zero_state(st));
-    FDTD_HIP_TRY(hipStreamSynchronize(st));
     return rebuild(st);
 }
 
@@ -229,9 +223,7 @@ fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *ou
     FDTD_HIP_TRY(hipStreamSynchronize(b->stream));
     const DeviceScene &D = *b->R.scenes[scene];
     const long long rows = std::min<long long>(b->R.taken, b->R.targets[scene]);
-    if (rows > 0 && D.S.L > 0)
-        FDTD_HIP_TRY(hipMemcpy(out, D.dev.series, sizeof(double) * (size_t)rows * D.S.L, hipMemcpyDeviceToHost));
-    return FDTD_OK;
+    return D.read_series(0, rows, out);
 }
 
 fdtd_status fdtd_batch_destroy(fdtd_batch *b)

@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "device_scene.h"
 
@@ -43,8 +44,7 @@ struct Layout {
     GhostDev *ghosts;
     int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
     double *gk_w, *gu_w, *gval0, *gval1, *gtmp;
-    int32_t *lptr, *lkind, *lindex;
-    double *lbeta, *series;
+    double *terms;
     unsigned long long *clock;
     double *acc_now;
     float *src_now;
@@ -94,11 +94,7 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.gval0 = c.take<double>(S.ghosts.size());
     L.gval1 = c.take<double>(S.ghosts.size());
     L.gtmp = c.take<double>(2 * S.ghosts.size());
-    L.lptr = c.take<int32_t>(S.lst_ptr.size());
-    L.lkind = c.take<int32_t>(S.lst_terms.size());
-    L.lindex = c.take<int32_t>(S.lst_terms.size());
-    L.lbeta = c.take<double>(S.lst_terms.size());
-    L.series = c.take<double>((size_t)cap * (size_t)(S.L ? S.L : 1));
+    L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
     L.acc_now = c.take<double>(S.objs.size());
     L.src_now = c.take<float>(S.psrc.size());
@@ -127,10 +123,11 @@ void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
 
 TilePlan tile_plan(int id)
 {
-    // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior}
+    // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior, one launch}
     static const TilePlan plans[kNumTilePlans] = {
-        {6, 4, 8, true},  {6, 8, 8, false}, {6, 8, 8, true},   {6, 4, 4, true},  {6, 2, 8, true},
-        {5, 16, 16, false}, {6, 4, 4, false}, {6, 8, 4, true}, {6, 16, 16, false}, {6, 16, 16, true},
+        {6, 4, 8, true, true},   {6, 4, 4, true, true},   {6, 8, 8, true, true},   {6, 4, 8, true, false},
+        {6, 8, 8, false, false}, {6, 8, 8, true, false},  {6, 4, 4, true, false},  {6, 2, 8, true, false},
+        {6, 4, 4, false, false}, {6, 16, 16, false, false},
     };
     return plans[(id >= 0 && id < kNumTilePlans) ? id : 0];
 }
@@ -143,8 +140,29 @@ size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)
     return c.off;
 }
 
-fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long target)
+fdtd_status DeviceScene::reset_to_start(hipStream_t st)
 {
+    FDTD_HIP_TRY(zero_state(st));
+    FDTD_HIP_TRY(hipMemsetAsync(dev.terms, 0, sizeof(double) * (size_t)series_cap * std::max<size_t>(1, S.lst_terms.size()), st));
+    const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
+    std::vector<double> acc0(S.objs.size());
+    std::vector<float> src0(S.psrc.size());
+    step_amplitudes(S, 0, acc0.data(), src0.data());
+    FDTD_HIP_TRY(hipMemcpyAsync(dev.clock, clk, sizeof clk, hipMemcpyHostToDevice, st));
+    if (!acc0.empty()) FDTD_HIP_TRY(hipMemcpyAsync(dev.acc_now, acc0.data(), acc0.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    if (!src0.empty()) FDTD_HIP_TRY(hipMemcpyAsync(dev.src_now, src0.data(), src0.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    if (!S.ghosts.empty()) {
+        FDTD_HIP_TRY(hipMemsetAsync(dev.gval[0], 0, S.ghosts.size() * 8, st));
+        FDTD_HIP_TRY(hipMemsetAsync(dev.gval[1], 0, S.ghosts.size() * 8, st));
+    }
+    FDTD_HIP_TRY(hipStreamSynchronize(st));
+    return FDTD_OK;
+}
+
+fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long target_steps)
+{
+    const long long target = target_steps;
+    this->target = target_steps;
     series_cap = cap;
     npad = pad_cells(S.N);
     Carver c;
@@ -158,6 +176,8 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     state_floats = L.state_floats;
     // fields start at rest: pressures, split pressures and both face-velocity buffers
     FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * state_floats, st));
+    // terms no tile writes (samples of cells that never change) read as zero
+    FDTD_HIP_TRY(hipMemsetAsync(L.terms, 0, sizeof(double) * (size_t)cap * std::max<size_t>(1, S.lst_terms.size()), st));
     std::vector<char> staging;  // pageable copies are synchronised before return
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
@@ -210,13 +230,6 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         FDTD_HIP_TRY(hipMemsetAsync(L.gval0, 0, S.ghosts.size() * 8, st));
         FDTD_HIP_TRY(hipMemsetAsync(L.gval1, 0, S.ghosts.size() * 8, st));
     }
-    std::vector<int32_t> kind, index;
-    std::vector<double> beta;
-    for (const ListenerTerm &t : S.lst_terms) { kind.push_back(t.kind); index.push_back(t.index); beta.push_back(t.beta); }
-    FDTD_HIP_TRY(up(L.lptr, S.lst_ptr.data(), S.lst_ptr.size() * 4));
-    FDTD_HIP_TRY(up(L.lkind, kind.data(), kind.size() * 4));
-    FDTD_HIP_TRY(up(L.lindex, index.data(), index.size() * 4));
-    FDTD_HIP_TRY(up(L.lbeta, beta.data(), beta.size() * 8));
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
     std::vector<double> acc0(S.objs.size());
@@ -261,15 +274,28 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.coupled_iters = S.coupled_iters;
     D.gk_ptr = L.gk_ptr; D.gk_idx = L.gk_idx; D.gk_w = L.gk_w;
     D.gu_ptr = L.gu_ptr; D.gu_idx = L.gu_idx; D.gu_w = L.gu_w;
-    D.L = S.L;
+    D.T = (int)S.lst_terms.size();
     D.cap = cap;
-    D.lptr = L.lptr; D.lkind = L.lkind; D.lindex = L.lindex; D.lbeta = L.lbeta;
-    D.series = L.series;
+    D.terms = L.terms;
     D.clock = L.clock;
-    D.n_listen_groups = S.L > 0 ? (S.L + kListenersPerGroup - 1) / kListenersPerGroup : 1;
     D.n_obj = (int)S.objs.size();
     D.acc_now = L.acc_now;
     D.src_now = L.src_now;
+    return FDTD_OK;
+}
+
+fdtd_status DeviceScene::read_series(long long ring_row, long long nrows, double *out) const
+{
+    const size_t T = S.lst_terms.size();
+    if (nrows <= 0 || S.L == 0) return FDTD_OK;
+    std::vector<double> rows((size_t)nrows * T);
+    if (T) FDTD_HIP_TRY(hipMemcpy(rows.data(), dev.terms + (size_t)ring_row * T, rows.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long r = 0; r < nrows; ++r)
+        for (int l = 0; l < S.L; ++l) {
+            double v = 0.0;
+            for (int q = S.lst_ptr[l]; q < S.lst_ptr[l + 1]; ++q) v += rows[(size_t)r * T + q];
+            out[(size_t)r * S.L + l] = v;
+        }
     return FDTD_OK;
 }
 
@@ -295,8 +321,9 @@ static bool box_overlap(const int *lo, const int *hi, const int *blo, const int 
     return true;
 }
 
-void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out) const
+void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out, std::vector<TermRef> &terms) const
 {
+    const size_t base = out.size();
     const int *n = S.n;
     const int *a = S.inner_lo, *b = S.inner_hi;
     auto src_hit = [&](const int *lo, const int *hi) {
@@ -373,12 +400,50 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
             out.push_back(t);
         }
     }
+    assign_terms(scene_id, base, out, terms);
 }
 
-void DeviceScene::append_listen(int scene_id, std::vector<ListenDesc> &out) const
+// Gives every listener term to the tile that produces its sample: a cell's
+// new pressure comes from the object-adjacent tile holding its record, else
+// from the PML or interior tile whose box holds it; ghost-entry terms go to
+// the tile of the entry's fluid neighbour.
+void DeviceScene::assign_terms(int scene_id, size_t base, std::vector<TileDesc> &out, std::vector<TermRef> &terms) const
 {
-    if (S.L == 0) out.push_back({scene_id, 0, 0});
-    for (int l0 = 0; l0 < S.L; l0 += kListenersPerGroup) out.push_back({scene_id, l0, std::min(S.L, l0 + kListenersPerGroup)});
+    const int nterm = (int)S.lst_terms.size();
+    std::vector<int> owner(nterm, -1);
+    std::unordered_map<int, int> record_of;
+    for (int r = 0; r < (int)S.special.size(); ++r) record_of[S.special[r].cell] = r;
+    const int sy = S.n[2], sx = S.n[1] * S.n[2];
+    auto tile_of_cell = [&](int cell) -> int {
+        auto it = record_of.find(cell);
+        for (size_t q = base; q < out.size(); ++q) {
+            const TileDesc &t = out[q];
+            if (t.type == TILE_SPECIAL) {
+                if (it != record_of.end() && it->second >= t.i0 && it->second < t.i1) return (int)q;
+                continue;
+            }
+            if (it != record_of.end()) continue;
+            const int i = cell / sx, j = (cell - i * sx) / sy, k = cell - i * sx - j * sy;
+            if (i >= t.i0 && i < t.i1 && j >= t.j0 && j < t.j1 && k >= t.k0 && k < t.k1) return (int)q;
+        }
+        return -1;
+    };
+    for (int q = 0; q < nterm; ++q) {
+        const ListenerTerm &lt = S.lst_terms[q];
+        owner[q] = tile_of_cell(lt.kind == 0 ? lt.index : S.ghosts[lt.index].nbr);
+    }
+    std::vector<std::vector<int>> by_tile(out.size() - base);
+    for (int q = 0; q < nterm; ++q)
+        if (owner[q] >= 0) by_tile[owner[q] - base].push_back(q);
+    for (size_t q = base; q < out.size(); ++q) {
+        TileDesc &t = out[q];
+        t.tb = (int)terms.size();
+        for (int qq : by_tile[q - base]) {
+            const ListenerTerm &lt = S.lst_terms[qq];
+            terms.push_back({scene_id, qq, lt.kind, lt.index, lt.beta});
+        }
+        t.te = (int)terms.size();
+    }
 }
 
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd)
@@ -393,12 +458,51 @@ void xcd_permute(std::vector<TileDesc> &tiles, int nxcd)
     tiles.swap(out);
 }
 
-fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
-                                const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st)
+// Host check of a launch list before anything runs on the device: every tile
+// belongs to a class the list's kernel handles, and its box keeps every
+// neighbour read of that class inside the grid (interior tiles read one cell
+// beyond the box on each side; whole-row float4 tiles also need aligned,
+// full-width rows).  Returns an error text, or empty.
+static std::string check_tiles(const std::vector<TileDesc> &list, int kind, const std::vector<SceneDev> &scenes,
+                               size_t nterms)
 {
-    release();
-    std::vector<TileDesc> by_kind[NUM_KINDS];
+    for (const TileDesc &t : list) {
+        if (t.type == TILE_EMPTY) continue;
+        if (t.scene < 0 || t.scene >= (int)scenes.size()) return "tile scene out of range";
+        const SceneDev &d = scenes[t.scene];
+        if (t.tb < 0 || t.te < t.tb || (size_t)t.te > nterms) return "tile term range out of range";
+        const bool ok_kind = kind == KIND_ALL || (kind == KIND_PML && t.type == TILE_PML) ||
+                             (kind == KIND_INTERIOR && t.type != TILE_PML) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL);
+        if (!ok_kind) return "tile class does not match its launch";
+        if (t.type == TILE_SPECIAL) {
+            if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";
+            continue;
+        }
+        const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] >= hi[a]) return "empty tile box";
+            if (t.type == TILE_PML) {
+                if (lo[a] < 0 || hi[a] > d.n[a]) return "PML tile outside the grid";
+            } else if (lo[a] < 1 || hi[a] > d.n[a] - 1) {
+                return "interior tile reads outside the grid";
+            }
+        }
+        if (t.type == TILE_PML) {
+            if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) return "PML tile too large";
+        } else if (t.type == TILE_INTERIOR_VEC) {
+            if (t.j1 - t.j0 > 4 || (t.k0 & 3) || ((t.k1 - t.k0) & 3) || t.k1 - t.k0 > 256 || (d.sy & 3))
+                return "float4 row tile misaligned or too wide";
+        } else if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) {
+            return "interior tile too large";
+        }
+    }
+    return std::string();
+}
+
+void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS])
+{
     std::vector<TileDesc> special;
+    for (int k = 0; k < NUM_KINDS; ++k) by_kind[k].clear();
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
@@ -416,9 +520,42 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
         while (!by_kind[KIND_INTERIOR].empty() && special.size() % 8) special.push_back(empty);
         by_kind[KIND_INTERIOR].insert(by_kind[KIND_INTERIOR].begin(), special.begin(), special.end());
     }
+    if (fused_launch) {
+        // PML tiles (the longest) start first; the interior fills in behind them
+        std::vector<TileDesc> all = by_kind[KIND_PML];
+        all.insert(all.end(), by_kind[KIND_INTERIOR].begin(), by_kind[KIND_INTERIOR].end());
+        by_kind[KIND_INTERIOR].swap(all);
+        by_kind[KIND_PML].clear();
+    }
+}
+
+std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
+                               const std::vector<SceneDev> &scenes, size_t nterms)
+{
+    for (int k = 0; k < NUM_KINDS; ++k) {
+        const std::string e = check_tiles(by_kind[k], fused_launch ? KIND_ALL : k, scenes, nterms);
+        if (!e.empty()) return e;
+    }
+    return std::string();
+}
+
+fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
+                                const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
+                                bool fused_launch)
+{
+    release();
+    std::vector<TileDesc> by_kind[NUM_KINDS];
+    build_launch_lists(tiles, fused_launch, by_kind);
+    fused = fused_launch;
+    {
+        const std::string e = check_launch_lists(by_kind, fused, scenes, terms.size());
+        if (!e.empty()) { set_last_error("step program: " + e); return FDTD_ERR_INVALID; }
+    }
+    const std::vector<SceneDev> &devs = scenes;
+    nscenes = (int)devs.size();
     size_t bytes[4 + NUM_KINDS];
-    bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, scenes.size()));
-    bytes[1] = align_up(sizeof(ListenDesc) * std::max<size_t>(1, listen.size()));
+    bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, devs.size()));
+    bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, terms.size()));
     bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
     bytes[3] = 0;
     size_t total = bytes[0] + bytes[1] + bytes[2];
@@ -429,7 +566,7 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     char *base = nullptr;
     FDTD_HIP_TRY(hipMalloc(&base, total));
     d_scenes = reinterpret_cast<SceneDev *>(base);
-    d_listen = reinterpret_cast<ListenDesc *>(base + bytes[0]);
+    d_terms = reinterpret_cast<TermRef *>(base + bytes[0]);
     d_ghost = reinterpret_cast<GhostDesc *>(base + bytes[0] + bytes[1]);
     size_t off = bytes[0] + bytes[1] + bytes[2];
     for (int k = 0; k < NUM_KINDS; ++k) {
@@ -439,10 +576,9 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
         if (ntiles[k])
             FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
     }
-    nlisten = (int)listen.size();
     nghost = (int)ghost.size();
-    if (!scenes.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, scenes.data(), sizeof(SceneDev) * scenes.size(), hipMemcpyHostToDevice, st));
-    if (!listen.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_listen, listen.data(), sizeof(ListenDesc) * listen.size(), hipMemcpyHostToDevice, st));
+    if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
+    if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     order_clock = scenes.empty() ? nullptr : scenes[0].clock + CLK_STEP;
@@ -457,22 +593,27 @@ void StepProgram::release()
     if (d_scenes) (void)hipFree(d_scenes);
     d_scenes = nullptr;
     for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; }
-    d_listen = nullptr;
+    d_terms = nullptr;
     d_ghost = nullptr;
-    nlisten = nghost = 0;
+    nghost = 0;
+    nscenes = 0;
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (side) (void)hipStreamDestroy(side);
     fork = join = nullptr;
     side = nullptr;
     order_clock = nullptr;
+    fused = false;
 }
 
 fdtd_status StepProgram::enqueue(hipStream_t st, long long step, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
-    auto launch = [&](int kind, hipStream_t s) { return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], order_clock, s); };
+    auto launch = [&](int kind, hipStream_t s) {
+        if (fused) return kind == KIND_INTERIOR ? launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, s) : hipSuccess;
+        return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, s);
+    };
     const bool split = concurrent && side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
     if (split) {
         // the PML slabs carry most of the bytes: they run alone on the main
@@ -494,7 +635,7 @@ fdtd_status StepProgram::enqueue(hipStream_t st, long long step, hipEvent_t ev_b
         FDTD_HIP_TRY(launch(KIND_SPECIAL, st));
     }
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
-    FDTD_HIP_TRY(launch_listen(d_scenes, d_listen, nlisten, st));
+    FDTD_HIP_TRY(launch_advance(d_scenes, nscenes, st));
     return FDTD_OK;
 }
 

@@ -27,6 +27,7 @@ struct TilePlan {
     int interior_ti = 16;      // planes marched per interior tile
     int pml_ti = 16;           // planes marched per PML tile
     bool interior_vec = false; // whole-row float4 interior tiles where the layout allows
+    bool fused = false;        // one launch for all tile classes (PML tiles first)
 };
 constexpr int kNumTilePlans = 10;
 TilePlan tile_plan(int id);
@@ -49,12 +50,20 @@ struct DeviceScene {
     fdtd_status allocate_and_upload(hipStream_t stream, int series_cap, long long target);
     void release();
     float *slot(int s) const { return dev.field + (size_t)s * (size_t)npad; }
+    long long target = 0;      // steps after which the kernels stop advancing this scene
     // zeroes every time-stepping array (pressures, split pressures, velocities)
     hipError_t zero_state(hipStream_t st) const { return hipMemsetAsync(dev.field, 0, sizeof(float) * state_floats, st); }
+    // puts the scene back at step 0: fields at rest, empty series ring, clock
+    // and per-step amplitudes of step 0 (synchronises st)
+    fdtd_status reset_to_start(hipStream_t st);
 
-    // Appends this scene's tiles / listener groups / ghost work, tagged with scene_id.
-    void append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out) const;
-    void append_listen(int scene_id, std::vector<ListenDesc> &out) const;
+    // Appends this scene's tiles, tagged with scene_id, and the listener
+    // terms they carry (TileDesc::tb/te index `terms`).
+    void append_tiles(int scene_id, const TilePlan &plan, std::vector<TileDesc> &out, std::vector<TermRef> &terms) const;
+    void assign_terms(int scene_id, size_t base, std::vector<TileDesc> &out, std::vector<TermRef> &terms) const;
+    // Listener values of ring rows [ring_row, ring_row + nrows): FP64 [nrows][L],
+    // each the sum of the listener's terms in term order.
+    fdtd_status read_series(long long ring_row, long long nrows, double *out) const;
     bool needs_ghost_kernel() const { return S.boundary == FDTD_BOUNDARY_FULLY_COUPLED && !S.ghosts.empty(); }
 };
 
@@ -62,9 +71,16 @@ struct DeviceScene {
 // w % 8) runs tile chunk (w % 8), keeping neighbouring tiles on one L2.
 void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 
+// Splits the tiles of a step into the lists of its launches (per class, or
+// one list when fused) in launch order, and checks them on the host: each
+// tile's class matches its launch and its reads stay inside the grid.
+void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS]);
+std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
+                               const std::vector<SceneDev> &scenes, size_t nterms);
+
 // The per-step launch sequence of a set of scenes, uploaded once:
-//   ghost solve (fully-coupled scenes) -> the three step kernels ->
-//   listeners + step advance.
+//   ghost solve (fully-coupled scenes) -> the step kernels, which also write
+//   the listener terms -> step advance (one workgroup per scene).
 // The step kernels write disjoint cells.  By default they run back to back
 // on one stream in an order that alternates with the step parity (object
 // cells, interior, PML on even steps; the reverse on odd steps), and the
@@ -76,17 +92,20 @@ void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 struct StepProgram {
     SceneDev *d_scenes = nullptr;
     TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
-    ListenDesc *d_listen = nullptr;
     GhostDesc *d_ghost = nullptr;
+    TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
-    int nlisten = 0, nghost = 0;
+    int nghost = 0, nscenes = 0;
     bool concurrent = false;
+    bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // tiles of every kind in one list; split by type here
+    // tiles of every kind in one list; split by type here (or kept in one
+    // launch when `fused_launch` is set)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
-                       const std::vector<ListenDesc> &listen, const std::vector<GhostDesc> &ghost, hipStream_t st);
+                       const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
+                       bool fused_launch = false);
     void release();
     // enqueues one step; `step` (the step it will run, or any value of the
     // same parity) picks the kernel order.  ev_begin/ev_end bracket the

@@ -72,17 +72,27 @@ struct SceneDev {
     int G, coupled_iters;
     const int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
     const double *gk_w, *gu_w;
-    int L, cap;                 // listeners, rows of the series ring
-    const int32_t *lptr, *lkind, *lindex;
-    const double *lbeta;
-    double *series;             // [cap][L]
+    // listener series: row s of the ring holds beta_q * sample_q of every MLS
+    // term q of every listener after step s; a listener's value is the sum
+    // of its terms, formed when the rows are read out
+    int T, cap;                 // terms per row, rows of the ring
+    double *terms;              // [cap][T]
     unsigned long long *clock;  // CLK_WORDS words
-    int n_listen_groups;        // workgroups of the listener launch for this scene
     int n_obj;
-    // time-dependent amplitudes of the current step, refreshed by the last
-    // listener workgroup when it advances the step counter:
+    // time-dependent amplitudes of the current step, refreshed by the
+    // advance kernel when it moves the step counter on:
     double *acc_now;            // [n_obj] normal acceleration a_n(obj, t_n)
     float *src_now;             // [n_psrc] S(t_n + dt/2) dt without the spatial factor
+};
+
+// A listener term handled by the tile that produces its sample: kind 0 is
+// the new pressure of flat cell `cell`; kind 1 the value of ghost entry
+// `cell` (raster: p^n of its fluid neighbour plus the acceleration term;
+// fully coupled: the solved ghost value).  `slot` is the term's column in
+// the scene's series ring.
+struct TermRef {
+    int scene, slot, kind, cell;
+    double beta;
 };
 
 // Amplitudes of step s on the host (used to seed step 0).
@@ -97,32 +107,25 @@ struct TileDesc {
     int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK (TILE_INTERIOR_VEC: 4 whole rows)
     int src;                     // tile intersects a pressure-source box
     int slab;                    // TILE_PML: index into SceneDev::slabs
+    int tb, te;                  // listener terms [tb, te) of the program's TermRef table
     int pad;
-};
-
-// One listener workgroup (one wavefront): listeners [l0, l1) of a scene,
-// eight lanes each (an order-1 MLS listener has eight enclosing-cell terms;
-// more are looped over).  A scene without listeners gets one group with
-// l0 == l1 that only advances the step.
-constexpr int kLanesPerListener = 8;
-constexpr int kListenersPerGroup = 64 / kLanesPerListener;
-struct ListenDesc {
-    int scene;
-    int l0, l1;
 };
 
 struct GhostDesc {
     int scene;
 };
 
-enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3 };
+// KIND_ALL: one launch carrying every tile class (dispatch per workgroup on
+// the tile type); the other kinds carry one class each.
+enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3, KIND_ALL = 3 };
 // order_clock: when non-null, the step word of a scene clock whose parity
 // selects the tile order -- odd steps walk each XCD's chunk of the tile list
 // backwards, so a step starts on the data the previous step touched last and
 // the Infinity Cache still holds.
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles,
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
                        const unsigned long long *order_clock, hipStream_t s);
-hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s);
+// Moves every scene of the launch to its next step (amplitudes, then counter).
+hipError_t launch_advance(const SceneDev *scenes, int nscenes, hipStream_t s);
 hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s);
 
 }  // namespace fdtd

@@ -61,9 +61,41 @@ __device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double
 }
 
 // ---------------------------------------------------------------------------
+// listener terms (PML_WaveSolver::FetchScalarData, MLSModeInterpolator::lookup
+// with precomputed weights): the thread that produces a term's sample writes
+// beta * sample into the scene's series ring; the terms of a listener are
+// summed when the ring is read out.  A tile lists its terms in [tb, te).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void emit_cell_terms(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step,
+                                                int idx, float pn)
+{
+    for (int q = t.tb; q < t.te; ++q) {
+        const TermRef r = cmem(terms)[q];
+        if (r.kind == 0 && r.cell == idx) gmem(S.terms)[(size_t)(step % S.cap) * S.T + r.slot] = r.beta * (double)pn;
+    }
+}
+
+__device__ __forceinline__ void emit_ghost_terms(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
+{
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    for (int q = t.tb + (int)threadIdx.x; q < t.te; q += (int)blockDim.x) {
+        const TermRef r = cmem(terms)[q];
+        if (r.kind == 0) continue;
+        double v;
+        if (S.raster) {
+            const GhostDev gd = gmem(S.ghosts)[r.cell];
+            v = (double)pcur[gd.nbr] + gd.k_raster * gmem(S.acc_now)[gd.obj];
+        } else {
+            v = gmem(S.gval[step & 1])[r.cell];
+        }
+        gmem(S.terms)[(size_t)(step % S.cap) * S.T + r.slot] = r.beta * v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // interior leapfrog
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc &t, long long step, bool skip_check)
+__device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step, bool skip_check)
 {
     const int tk = t.tk_log2;
     const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
@@ -79,7 +111,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     int idx = t.i0 * sx + j * sy + k;
     float pm = pcur[idx - sx];
     float pc = pcur[idx];
-    if (!t.src && !skip_check) {
+    if (!t.src && !skip_check && t.te == t.tb) {
         // the streaming case: 12 bytes of HBM traffic per cell
         for (int i = t.i0; i < t.i1; ++i, idx += sx) {
             const float pp = pcur[idx + sx];
@@ -105,7 +137,10 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
                 const AS_CONST float *g = cmem(S.gtab);
                 pn += gmem(S.src_now)[s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
             }
-        if (!skip_check || !gmem(S.skip)[idx]) pnext[idx] = pn;
+        if (!skip_check || !gmem(S.skip)[idx]) {
+            pnext[idx] = pn;
+            if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, pn);
+        }
         pm = pc;
         pc = pp;
     }
@@ -116,7 +151,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
 // 1 KiB of a row.  The k neighbours come from the adjacent lanes; the two row
 // ends read theirs from memory.  Plain cells only (no source, no object
 // cell); k0 and the row stride are multiples of four.
-__device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileDesc &t, long long step)
+__device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     const int lane = threadIdx.x & 63, j = t.j0 + (int)(threadIdx.x >> 6);
     if (j >= t.j1) return;
@@ -144,7 +179,15 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         o.y = 2.f * pc.y - prev.y + lam2 * (((pm.y + pp.y) + (ym.y + yp.y) + (pc.x + pc.z)) - 6.f * pc.y);
         o.z = 2.f * pc.z - prev.z + lam2 * (((pm.z + pp.z) + (ym.z + yp.z) + (pc.y + pc.w)) - 6.f * pc.z);
         o.w = 2.f * pc.w - prev.w + lam2 * (((pm.w + pp.w) + (ym.w + yp.w) + (pc.z + zr)) - 6.f * pc.w);
-        if (act) *(AS_GLOBAL float4 *)(pnext + idx) = o;
+        if (act) {
+            *(AS_GLOBAL float4 *)(pnext + idx) = o;
+            if (t.te > t.tb) {
+                emit_cell_terms(S, t, terms, step, idx, o.x);
+                emit_cell_terms(S, t, terms, step, idx + 1, o.y);
+                emit_cell_terms(S, t, terms, step, idx + 2, o.z);
+                emit_cell_terms(S, t, terms, step, idx + 3, o.w);
+            }
+        }
         pm = pc;
         pc = pp;
     }
@@ -160,14 +203,28 @@ __device__ __forceinline__ float face_update(bool active, float4 c, float v, flo
     return active ? c.x * v + c.y * (pr - pl) : 0.f;
 }
 
-__device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, long long step)
+// Loads and stores at a 32-bit byte offset from a wavefront-uniform base, so
+// the compiler can use the scalar-base form of global memory instructions
+// (one VGPR per address instead of a 64-bit pair).  Offsets stay below 4 GiB:
+// the host caps a pressure slot at 2^30 cells.
+template <class T>
+__device__ __forceinline__ T ldb(const AS_GLOBAL T *base, uint32_t off)
+{
+    return *(const AS_GLOBAL T *)((const AS_GLOBAL char *)base + off);
+}
+template <class T>
+__device__ __forceinline__ void stb(AS_GLOBAL T *base, uint32_t off, T v)
+{
+    *(AS_GLOBAL T *)((AS_GLOBAL char *)base + off) = v;
+}
+
+__device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     const int tk = t.tk_log2;
     const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
     const int k = t.k0 + tx, j = t.j0 + ty;
     if (k >= t.k1 || j >= t.j1) return;
     const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
-    const int sx = S.sx, sy = S.sy;
     const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
     AS_GLOBAL float *__restrict__ pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     // compact storage of this slab: cells (and x faces) have plane stride
@@ -175,7 +232,6 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     const PmlSlab sl = cmem(S.slabs)[t.slab];
     const int e0 = sl.e[0], e1 = sl.e[1], e2 = sl.e[2];
     const int li0 = t.i0 - sl.lo[0], lj = j - sl.lo[1], lk = k - sl.lo[2];
-    const int cs = e1 * e2, ys = (e1 + 1) * e2, zs = e1 * (e2 + 1);
     const int vb = (int)(step & 1);
     AS_GLOBAL float *__restrict__ pd0 = gmem(S.pd[0] + sl.cell_base);
     AS_GLOBAL float *__restrict__ pd1 = gmem(S.pd[1] + sl.cell_base);
@@ -189,64 +245,77 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
     const AS_CONST float4 *__restrict__ cx = cmem(S.coef[0]);
     const AS_CONST int *__restrict__ fx = cmem(S.flags[0]);
 
-    // y and z coefficients are constant along the column
+    // y and z coefficients and face predicates are constant along the column
     const AS_CONST int *fy = cmem(S.flags[1]), *fz = cmem(S.flags[2]);
     const AS_CONST float4 *cy = cmem(S.coef[1]), *cz = cmem(S.coef[2]);
     const int fyL = fy[j], fzL = fz[k];
-    const int fyR = (j + 1 < ny) ? fy[j + 1] : 0, fzR = (k + 1 < nz) ? fz[k + 1] : 0;
+    const bool has_yR = j + 1 < ny, has_zR = k + 1 < nz;
+    const int fyR = has_yR ? fy[j + 1] : 0, fzR = has_zR ? fz[k + 1] : 0;
     const float4 cyL = cy[j], czL = cz[k];
     const float4 cyR = cy[min(j + 1, ny)], czR = cz[min(k + 1, nz)];
     const bool cj = fyL & 1, ck = fzL & 1;
-    const int ym_off = (j > 0) ? sy : 0, yp_off = (j < ny - 1) ? sy : 0;
-    const int zm_off = (k > 0) ? 1 : 0, zp_off = (k < nz - 1) ? 1 : 0;
+    const bool ox = cj || ck;
+    // a face is updated when it lies in the PML or its cell is in the PML on
+    // another axis (ci, the x flag of the plane, joins per plane); wall faces stay zero
+    const bool yl_on = j >= 1 && ((fyL & 2) || ck), yl_ci = j >= 1;
+    const bool yr_on = has_yR && ((fyR & 2) || ck), yr_ci = has_yR;
+    const bool zl_on = k >= 1 && ((fzL & 2) || cj), zl_ci = k >= 1;
+    const bool zr_on = has_zR && ((fzR & 2) || cj), zr_ci = has_zR;
     // the high y / z face of this column is stored here when it is the slab's last layer
     const bool own_yR = lj == e1 - 1, own_zR = lk == e2 - 1;
 
-    int idx = t.i0 * sx + j * sy + k;
-    int c = (li0 * e1 + lj) * e2 + lk;              // cell and x-face offset
-    int yo = (li0 * (e1 + 1) + lj) * e2 + lk;       // y-face offset
-    int zo = (li0 * e1 + lj) * (e2 + 1) + lk;       // z-face offset
-    float pm = pcur[idx - (t.i0 > 0 ? sx : 0)];
-    float pc = pcur[idx];
+    // byte offsets
+    const uint32_t sx4 = 4u * (uint32_t)S.sx, sy4 = 4u * (uint32_t)S.sy;
+    const uint32_t cs4 = 4u * (uint32_t)(e1 * e2), ys4 = 4u * (uint32_t)((e1 + 1) * e2), zs4 = 4u * (uint32_t)(e1 * (e2 + 1));
+    const uint32_t e24 = 4u * (uint32_t)e2;
+    const uint32_t ym4 = j > 0 ? sy4 : 0u, yp4 = j < ny - 1 ? sy4 : 0u;
+    const uint32_t zm4 = k > 0 ? 4u : 0u, zp4 = k < nz - 1 ? 4u : 0u;
+    uint32_t po = 4u * (uint32_t)(t.i0 * S.sx + j * S.sy + k);        // pressure cell
+    uint32_t co = 4u * (uint32_t)((li0 * e1 + lj) * e2 + lk);         // PML cell and x face
+    uint32_t yo = 4u * (uint32_t)((li0 * (e1 + 1) + lj) * e2 + lk);   // y face
+    uint32_t zo = 4u * (uint32_t)((li0 * e1 + lj) * (e2 + 1) + lk);   // z face
+
+    float pm = ldb(pcur, po - (t.i0 > 0 ? sx4 : 0u));
+    float pc = ldb(pcur, po);
     // the x face on the low side of the current cell (updated value), carried
     // from one plane to the next: cell i's high face is cell i+1's low face
-    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || cj || ck), cx[t.i0], vi0[c], pm, pc);
-    for (int i = t.i0; i < t.i1; ++i, idx += sx, c += cs, yo += ys, zo += zs) {
+    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || ox), cx[t.i0], ldb(vi0, co), pm, pc);
+    const int ilast = sl.lo[0] + e0 - 1;
+    for (int i = t.i0; i < t.i1; ++i, po += sx4, co += cs4, yo += ys4, zo += zs4) {
         const bool last = i + 1 >= nx;
-        const float pp = pcur[idx + (last ? 0 : sx)];
-        const float ym = pcur[idx - ym_off], yp = pcur[idx + yp_off];
-        const float zm = pcur[idx - zm_off], zp = pcur[idx + zp_off];
-        const float vxr_old = vi0[c + cs];
-        const float vyl_old = vi1[yo], vyr_old = vi1[yo + e2];
-        const float vzl_old = vi2[zo], vzr_old = vi2[zo + 1];
-        const float q0 = pd0[c], q1 = pd1[c], q2 = pd2[c];
+        const float pp = ldb(pcur, po + (last ? 0u : sx4));
+        const float ym = ldb(pcur, po - ym4), yp = ldb(pcur, po + yp4);
+        const float zm = ldb(pcur, po - zm4), zp = ldb(pcur, po + zp4);
+        const float vxr_old = ldb(vi0, co + cs4);
+        const float vyl_old = ldb(vi1, yo), vyr_old = ldb(vi1, yo + e24);
+        const float vzl_old = ldb(vi2, zo), vzr_old = ldb(vi2, zo + 4u);
+        const float q0 = ldb(pd0, co), q1 = ldb(pd1, co), q2 = ldb(pd2, co);
         const int fxC = fx[i], fxR = last ? 0 : fx[i + 1];
         const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
         const bool ci = fxC & 1;
         // x: high face (new), low face carried
-        const bool ox = cj || ck;
         const float vxR = last ? 0.f : face_update((fxR & 2) || ox, cxR, vxr_old, pc, pp);
-        vo0[c] = vxL;
-        if (i - sl.lo[0] == e0 - 1) vo0[c + cs] = vxR;
+        stb(vo0, co, vxL);
+        if (i == ilast) stb(vo0, co + cs4, vxR);
         const float n0 = cxC.z * q0 + cxC.w * (vxR - vxL);
         // y
-        const bool oy = ci || ck;
-        const float vyL = face_update(j >= 1 && ((fyL & 2) || oy), cyL, vyl_old, ym, pc);
-        const float vyR = (j + 1 < ny) ? face_update((fyR & 2) || oy, cyR, vyr_old, pc, yp) : 0.f;
-        vo1[yo] = vyL;
-        if (own_yR) vo1[yo + e2] = vyR;
+        const float vyL = face_update(yl_on || (ci && yl_ci), cyL, vyl_old, ym, pc);
+        const float vyR = face_update(yr_on || (ci && yr_ci), cyR, vyr_old, pc, yp);
+        stb(vo1, yo, vyL);
+        if (own_yR) stb(vo1, yo + e24, vyR);
         const float n1 = cyL.z * q1 + cyL.w * (vyR - vyL);
         // z
-        const bool oz = ci || cj;
-        const float vzL = face_update(k >= 1 && ((fzL & 2) || oz), czL, vzl_old, zm, pc);
-        const float vzR = (k + 1 < nz) ? face_update((fzR & 2) || oz, czR, vzr_old, pc, zp) : 0.f;
-        vo2[zo] = vzL;
-        if (own_zR) vo2[zo + 1] = vzR;
+        const float vzL = face_update(zl_on || (ci && zl_ci), czL, vzl_old, zm, pc);
+        const float vzR = face_update(zr_on || (ci && zr_ci), czR, vzr_old, pc, zp);
+        stb(vo2, zo, vzL);
+        if (own_zR) stb(vo2, zo + 4u, vzR);
         const float n2 = czL.z * q2 + czL.w * (vzR - vzL);
-        pd0[c] = n0;
-        pd1[c] = n1;
-        pd2[c] = n2;
-        pnext[idx] = n0 + n1 + n2;
+        stb(pd0, co, n0);
+        stb(pd1, co, n1);
+        stb(pd2, co, n2);
+        const float pn = n0 + n1 + n2;
+        stb(pnext, po, pn);
+        if (t.te > t.tb) emit_cell_terms(S, t, terms, step, (int)(po >> 2), pn);
         vxL = vxR;
         pm = pc;
         pc = pp;
@@ -256,7 +325,7 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, l
 // ---------------------------------------------------------------------------
 // bulk cells next to rigid objects (FP64)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &t, long long step)
+__device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     const int r = t.i0 + (int)threadIdx.x;
     if (r >= t.i1) return;
@@ -304,6 +373,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         }
     }
     pnext[idx] = (float)pn;
+    if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
 }
 
 // Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
@@ -316,12 +386,30 @@ __device__ __forceinline__ int tile_of(int w, int ntiles, const unsigned long lo
     return ntiles - 1 - w;
 }
 
+// Step advance (PML_WaveSolver::stepSystem's time update), one workgroup per
+// scene after the step kernels: the amplitudes of step + 1, then the step
+// counter.  The kernel boundary orders it after every read of this step's
+// amplitudes and before the next step reads the counter.
+__global__ __launch_bounds__(64) void advance_kernel(const SceneDev *__restrict__ scenes)
+{
+    const SceneDev &S = scenes[blockIdx.x];
+    AS_GLOBAL unsigned long long *clk = gmem(S.clock);
+    const long long step = (long long)clk[CLK_STEP];
+    if (step >= (long long)clk[CLK_TARGET]) return;
+    const double tn1 = S.t_start + (double)(step + 1) * S.dt;
+    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) gmem(S.acc_now)[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
+    for (int s = threadIdx.x; s < S.n_psrc; s += blockDim.x) gmem(S.src_now)[s] = source_amp(S.psrc[s], tn1, S.dt);
+    if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)(step + 1);
+}
+
 // One kernel per tile class, so each class gets its own register allocation
 // (the FP64 object path would otherwise cap the occupancy of the streaming
-// paths).  The three classes write disjoint cells and can run in any order.
+// paths), or one launch for all classes (KIND_ALL).  The classes write
+// disjoint cells and can run in any order.
 template <int KIND>
 __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles,
-                                                   int ntiles, const unsigned long long *__restrict__ order_clock)
+                                                   int ntiles, const TermRef *__restrict__ terms,
+                                                   const unsigned long long *__restrict__ order_clock)
 {
     const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, order_clock)];
     if (t.type == TILE_EMPTY) return;
@@ -331,14 +419,20 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
     if constexpr (KIND == KIND_INTERIOR) {
         // the interior launch also carries the object-adjacent cells, whose
         // FP64 gathers then hide under the interior's streaming
-        if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, step);
-        else if (t.type == TILE_SPECIAL) special_tile(S, t, step);
-        else interior_tile(S, t, step, t.type == TILE_INTERIOR_SKIP);
+        if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
+        else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
+        else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     } else if constexpr (KIND == KIND_PML) {
-        pml_tile(S, t, step);
+        pml_tile(S, t, terms, step);
+    } else if constexpr (KIND == KIND_SPECIAL) {
+        special_tile(S, t, terms, step);
     } else {
-        special_tile(S, t, step);
+        if (t.type == TILE_PML) pml_tile(S, t, terms, step);
+        else if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
+        else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
+        else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     }
+    if (t.te > t.tb) emit_ghost_terms(S, t, terms, step);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,84 +475,23 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
         for (int g = threadIdx.x; g < S.G; g += blockDim.x) gval[g] = x1[g];
 }
 
-// ---------------------------------------------------------------------------
-// listeners (PML_WaveSolver::FetchScalarData): eight lanes per listener,
-// an FP64 dot product of the precomputed MLS weights with the samples of
-// the eight enclosing cells (or their ghost values), reduced across the
-// eight lanes.  The last workgroup of a scene to finish advances the
-// scene's step counter and refreshes the per-step amplitudes (vector atomics
-// only).  A workgroup is one wavefront, so the fence before the done count
-// covers all of its stores.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void listen_kernel(const SceneDev *__restrict__ scenes, const ListenDesc *__restrict__ groups)
-{
-    const ListenDesc gdsc = gmem(groups)[blockIdx.x];
-    const SceneDev &S = scenes[gdsc.scene];
-    unsigned long long *clk = S.clock;
-    const long long step = (long long)clk[CLK_STEP];
-    if (step >= (long long)clk[CLK_TARGET]) return;
-    const int tid = threadIdx.x;
-    const int l = gdsc.l0 + tid / kLanesPerListener, sub = tid % kLanesPerListener;
-    double acc = 0.0;
-    if (l < gdsc.l1) {
-        const AS_GLOBAL float *__restrict__ pold = gmem(S.field + (size_t)slot_cur(step) * S.npad);
-        const AS_GLOBAL float *__restrict__ pnew = gmem(S.field + (size_t)slot_next(step) * S.npad);
-        const int q1 = gmem(S.lptr)[l + 1];
-        for (int q = gmem(S.lptr)[l] + sub; q < q1; q += kLanesPerListener) {
-            const int kind = gmem(S.lkind)[q], idx = gmem(S.lindex)[q];
-            double v;
-            if (kind == 0) {
-                v = (double)pnew[idx];
-            } else if (S.raster) {
-                const GhostDev gd = gmem(S.ghosts)[idx];
-                v = (double)pold[gd.nbr] + gd.k_raster * gmem(S.acc_now)[gd.obj];
-            } else {
-                v = gmem(S.gval[step & 1])[idx];
-            }
-            acc += gmem(S.lbeta)[q] * v;
-        }
-    }
-    for (int off = kLanesPerListener / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kLanesPerListener);
-    if (l < gdsc.l1 && sub == 0) gmem(S.series)[(size_t)(step % S.cap) * S.L + l] = acc;
-    // every group of the scene has read this step's amplitudes before it
-    // counts itself done, so the last one may overwrite them
-    __shared__ int last;
-    __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        const unsigned long long prev = atomicAdd(&clk[CLK_DONE], 1ULL);
-        last = (prev + 1 == (unsigned long long)S.n_listen_groups);
-    }
-    __syncthreads();
-    if (last) {
-        const double tn1 = S.t_start + (double)(step + 1) * S.dt;
-        for (int o = tid; o < S.n_obj; o += 64) S.acc_now[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
-        for (int s = tid; s < S.n_psrc; s += 64) S.src_now[s] = source_amp(S.psrc[s], tn1, S.dt);
-        __threadfence();
-        __syncthreads();
-        if (tid == 0) {
-            atomicExch(&clk[CLK_DONE], 0ULL);
-            atomicExch(&clk[CLK_STEP], (unsigned long long)(step + 1));
-        }
-    }
-}
-
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles,
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
                        const unsigned long long *order_clock, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     switch (kind) {
-    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
-    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
-    default: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, order_clock); break;
+    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
+    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
+    case KIND_SPECIAL: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
+    default: hipLaunchKernelGGL(step_kernel<KIND_ALL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_listen(const SceneDev *scenes, const ListenDesc *groups, int ngroups, hipStream_t s)
+hipError_t launch_advance(const SceneDev *scenes, int nscenes, hipStream_t s)
 {
-    if (ngroups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(listen_kernel, dim3(ngroups), dim3(64), 0, s, scenes, groups);
+    if (nscenes <= 0) return hipSuccess;
+    hipLaunchKernelGGL(advance_kernel, dim3(nscenes), dim3(64), 0, s, scenes);
     return hipGetLastError();
 }
 

@@ -86,16 +86,15 @@ fdtd_status fdtd_server::rebuild()
     if (active.empty()) return FDTD_OK;
     std::vector<SceneDev> devs;
     std::vector<TileDesc> tiles;
-    std::vector<ListenDesc> listen;
+    std::vector<TermRef> terms;
     std::vector<GhostDesc> ghost;
     for (Request *r : active) {
         const int id = (int)devs.size();
         devs.push_back(r->dev->dev);
-        r->dev->append_tiles(id, tile_plan(0), tiles);
-        r->dev->append_listen(id, listen);
+        r->dev->append_tiles(id, tile_plan(0), tiles, terms);
         if (r->dev->needs_ghost_kernel()) ghost.push_back({id});
     }
-    fdtd_status s = prog.upload(devs, tiles, listen, ghost, stream);
+    fdtd_status s = prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused);
     if (s != FDTD_OK) return s;
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -179,9 +178,7 @@ void fdtd_server::run()
             const int L = r->dev->S.L;
             r->result.resize((size_t)r->steps * L);
             fdtd_status s = FDTD_OK;
-            if (r->steps && L &&
-                hipMemcpy(r->result.data(), r->dev->dev.series, sizeof(double) * r->result.size(), hipMemcpyDeviceToHost) != hipSuccess)
-                s = FDTD_ERR_DEVICE;
+            if (r->steps && L) s = r->dev->read_series(0, r->steps, r->result.data());
             for (double v : r->result)
                 if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
             finish(r, s);
@@ -218,6 +215,68 @@ int64_t fdtd_scene_device_bytes(const fdtd_scene *scene)
     std::string err;
     if (build_setup(*scene, S, err) != FDTD_OK) { set_last_error(err); return -1; }
     return (int64_t)DeviceScene::bytes_needed(S, 1);
+}
+
+fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n_tiles)
+{
+    if (!scene) return FDTD_ERR_INVALID;
+    DeviceScene D;
+    std::string err;
+    fdtd_status st = build_setup(*scene, D.S, err);
+    if (st != FDTD_OK) { set_last_error(err); return st; }
+    const int plan = (tile_config >= 0 && tile_config < kNumTilePlans) ? tile_config : 0;
+    const TilePlan tp = tile_plan(plan);
+    // the scene block as the kernels see it, without device arrays
+    SceneDev d{};
+    for (int a = 0; a < 3; ++a) d.n[a] = D.S.n[a];
+    d.sy = D.S.n[2];
+    d.sx = D.S.n[1] * D.S.n[2];
+    d.n_special = (int)D.S.special.size();
+    std::vector<TileDesc> tiles;
+    std::vector<TermRef> terms;
+    D.append_tiles(0, tp, tiles, terms);
+    std::vector<TileDesc> lists[NUM_KINDS];
+    build_launch_lists(tiles, tp.fused, lists);
+    const std::string e = check_launch_lists(lists, tp.fused, {d}, terms.size());
+    if (!e.empty()) { set_last_error("plan " + std::to_string(plan) + ": " + e); return FDTD_ERR_INVALID; }
+    // every cell lies in exactly one box tile; object cells (solid or next to
+    // an object) only in boxes that skip them, and each record in one
+    // object-cell tile
+    std::vector<uint8_t> hit((size_t)D.S.N, 0), skips((size_t)D.S.N, 0);
+    std::vector<int> rec_hits(D.S.special.size(), 0);
+    for (const TileDesc &t : tiles) {
+        if (t.type == TILE_SPECIAL) {
+            for (int r = t.i0; r < t.i1; ++r) ++rec_hits[r];
+            continue;
+        }
+        for (int i = t.i0; i < t.i1; ++i)
+            for (int j = t.j0; j < t.j1; ++j)
+                for (int k = t.k0; k < t.k1; ++k) {
+                    const int64_t c = ((int64_t)i * D.S.n[1] + j) * D.S.n[2] + k;
+                    ++hit[c];
+                    skips[c] = t.type == TILE_INTERIOR_SKIP;
+                }
+    }
+    std::vector<uint8_t> object_cell((size_t)D.S.N, 0);
+    if (!D.S.objs.empty())
+        for (int64_t c = 0; c < D.S.N; ++c) object_cell[c] = D.S.bulk[c] ? 0 : 1;
+    for (const SpecialRec &r : D.S.special) object_cell[r.cell] = 1;
+    auto fail = [&](const std::string &why) {
+        set_last_error("plan " + std::to_string(plan) + ": " + why);
+        return FDTD_ERR_INVALID;
+    };
+    for (int64_t c = 0; c < D.S.N; ++c) {
+        if (hit[c] != 1) return fail("a cell is covered " + std::to_string(hit[c]) + " times");
+        if (object_cell[c] && !skips[c]) return fail("an object cell lies in a tile that would overwrite it");
+    }
+    for (int h : rec_hits)
+        if (h != 1) return fail("an object-adjacent record is covered " + std::to_string(h) + " times");
+    if (n_tiles) {
+        int64_t n = 0;
+        for (int k = 0; k < NUM_KINDS; ++k) n += (int64_t)lists[k].size();
+        *n_tiles = n;
+    }
+    return FDTD_OK;
 }
 
 fdtd_status fdtd_server_create(const fdtd_server_options *opt_in, fdtd_server **out)

@@ -110,12 +110,11 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
     st = h->D.allocate_and_upload(h->stream, cap, LLONG_MAX);
     if (st == FDTD_OK) {
         std::vector<TileDesc> tiles;
-        std::vector<ListenDesc> listen;
+        std::vector<TermRef> terms;
         std::vector<GhostDesc> ghost;
-        h->D.append_tiles(0, tile_plan(h->tile), tiles);
-        h->D.append_listen(0, listen);
+        h->D.append_tiles(0, tile_plan(h->tile), tiles, terms);
         if (h->D.needs_ghost_kernel()) ghost.push_back({0});
-        st = h->prog.upload({h->D.dev}, tiles, listen, ghost, h->stream);
+        st = h->prog.upload({h->D.dev}, tiles, terms, ghost, h->stream, tile_plan(h->tile).fused);
     }
     if (st != FDTD_OK) {
         h->prog.release();
@@ -163,8 +162,8 @@ static fdtd_status flush_series(fdtd_handle *h)
     while (r < h->step) {
         const long long ring = r % cap;
         const long long n = std::min<long long>(h->step - r, cap - ring);
-        FDTD_HIP_TRY(hipMemcpy(h->host_series.data() + old + (size_t)(r - h->flushed) * L,
-                               h->D.dev.series + (size_t)ring * L, (size_t)n * L * sizeof(double), hipMemcpyDeviceToHost));
+        fdtd_status st = h->D.read_series(ring, n, h->host_series.data() + old + (size_t)(r - h->flushed) * L);
+        if (st != FDTD_OK) return st;
         r += n;
     }
     h->flushed = h->step;

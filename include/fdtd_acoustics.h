@@ -258,6 +258,12 @@ fdtd_status fdtd_server_cancel(fdtd_server *s, int64_t request_id);
 fdtd_status fdtd_server_get_stats(fdtd_server *s, fdtd_server_stats *out);
 /* Bytes of device state a scene needs when admitted (used by admission). */
 int64_t     fdtd_scene_device_bytes(const fdtd_scene *scene);
+/* Host-only dry run of the device planner (no counterpart in the reference,
+ * which has no tiling): builds the scene's tiles for tile_config (-1: default)
+ * and the launch lists of its step, and checks on the host that every tile's
+ * class matches its launch and every read stays inside the grid.  No device
+ * is touched.  n_tiles (may be NULL) gets the number of workgroups per step. */
+fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n_tiles);
 fdtd_status fdtd_server_destroy(fdtd_server *s);
 
 #ifdef __cplusplus
