@@ -26,9 +26,11 @@ import json
 import os
 import sys
 
-STEP_KERNELS = ("step_kernel<0>", "step_kernel<1>", "step_kernel<2>")
-KERNEL_ROLE = {"step_kernel<0>": "interior", "step_kernel<1>": "PML slabs", "step_kernel<2>": "object-adjacent cells",
-               "listen_kernel": "listeners + step advance", "ghost_kernel": "coupled ghost solve"}
+STEP_KERNELS = ("step_kernel<0>", "step_kernel<1>", "step_kernel<2>", "step_kernel<3>")
+KERNEL_ROLE = {"step_kernel<0>": "interior + object-adjacent cells", "step_kernel<1>": "PML slabs",
+               "step_kernel<2>": "object-adjacent cells",
+               "step_kernel<3>": "one step, all tile classes (PML, interior, object-adjacent) + listener terms",
+               "advance_kernel": "step advance (per-step amplitudes, step counter)", "ghost_kernel": "coupled ghost solve"}
 
 
 def copy(mib, iters):
@@ -56,7 +58,7 @@ def step(n, steps, tile):
 
 
 def _short(name):
-    for k in list(STEP_KERNELS) + ["listen_kernel", "ghost_kernel", "copyBuffer", "vectorized_elementwise"]:
+    for k in list(STEP_KERNELS) + ["advance_kernel", "ghost_kernel", "copyBuffer", "fillBuffer", "vectorized_elementwise"]:
         if k in name:
             return k
     return name[:40]
@@ -89,7 +91,7 @@ def traffic(a):
         per[k] = {"role": KERNEL_ROLE[k], "read_bytes": round(rd), "write_bytes": round(wr), "dispatches": sf[k][1]}
         total += rd + wr
     out = {"n": a.n, "hbm_bytes_per_launch": round(total),
-           "launch": "one step: the three step kernels (interior, PML slabs, object-adjacent cells)",
+           "launch": "one step: the step kernel launches (the fused all-class kernel of the default plan)",
            "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
            "calibration": f"rocclr copyBuffer of {a.mib} MiB: FETCH_SIZE and WRITE_SIZE scaled so the copy reads "
                           f"and writes exactly {a.mib} MiB",

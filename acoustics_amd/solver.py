@@ -54,8 +54,9 @@ class Solver:
         capi.check(self.lib.fdtd_synchronize(self.h), "fdtd_synchronize")
 
     def listen(self):
-        L = self.info()["n_listeners"]
-        out = np.zeros(L)
+        if getattr(self, "_n_listeners", None) is None:
+            self._n_listeners = self.info()["n_listeners"]
+        out = np.zeros(self._n_listeners)
         capi.check(self.lib.fdtd_listen(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "fdtd_listen")
         return out
 
