@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libfdtd_acoustics.so")
 
 FDTD_BOUNDARY_RASTERIZE = 0
 FDTD_BOUNDARY_FULLY_COUPLED = 1
+FDTD_SCHEME_COLLOCATED = 0
+FDTD_SCHEME_STAGGERED = 1
 FDTD_OBJ_SPHERE = 0
 FDTD_OBJ_ELLIPSOID = 1
 
@@ -34,6 +36,7 @@ class Settings(C.Structure):
         ("pml_strength", C.c_double), ("air_viscosity_alpha", C.c_double), ("start_time", C.c_double),
         ("boundary_condition_preset", C.c_int), ("boundary_handling", C.c_int), ("use_mesh", C.c_int),
         ("divisions", C.c_int * 3), ("min_bound", C.c_double * 3),
+        ("scheme", C.c_int), ("reserved", C.c_int),
     ]
 
 

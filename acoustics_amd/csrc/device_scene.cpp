@@ -60,8 +60,12 @@ size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6])
     size_t o = (size_t)NUM_P_SLOTS * (size_t)npad;
     auto al = [](size_t x) { return (x + 63) / 64 * 64; };
     for (int d = 0; d < 3; ++d) { off[1 + d] = o; o += al((size_t)S.n_pml); }
+    // face velocities: the PML's compact faces (collocated) or every face of the grid (staggered)
     for (int b = 0; b < 2; ++b)
-        for (int d = 0; d < 3; ++d) { off[4 + 3 * b + d] = o; o += al((size_t)S.n_pml_faces[d]); }
+        for (int d = 0; d < 3; ++d) {
+            off[4 + 3 * b + d] = o;
+            o += al(S.scheme == FDTD_SCHEME_STAGGERED ? (size_t)npad : (size_t)S.n_pml_faces[d]);
+        }
     off[0] = 0;
     return o;
 }
@@ -116,8 +120,14 @@ void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
     }
     for (size_t q = 0; q < S.psrc.size(); ++q) {
         const PSrcHost &p = S.psrc[q];
-        const double u = (tn + 0.5 * S.dt - p.t0) / p.wt;
-        src[q] = (float)(std::exp(-(u * u) / 2.0) * p.amp * S.dt);
+        const double t = tn + 0.5 * S.dt;
+        if (S.scheme == FDTD_SCHEME_STAGGERED) {
+            const double s2 = std::sqrt(2.0) * p.wt;
+            src[q] = (float)((std::erf((t - p.t0) / s2) - std::erf(-p.t0 / s2)) * p.amp * S.dt);
+        } else {
+            const double u = (t - p.t0) / p.wt;
+            src[q] = (float)(std::exp(-(u * u) / 2.0) * p.amp * S.dt);
+        }
     }
 }
 
@@ -191,7 +201,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         flags[d].resize(n + 1);
         for (int i = 0; i <= n; ++i) {
             coef[d][i] = make_float4(S.f1[d][i], S.f2[d][i], i < n ? S.f3[d][i] : 0.f, i < n ? S.f5h[d][i] : 0.f);
-            flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0);
+            flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0) | (i < n && S.sig0[d][i] ? 4 : 0);
         }
         FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
         FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size() * sizeof(int)));
@@ -245,6 +255,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.sy = S.n[2];
     D.sx = S.n[1] * S.n[2];
     D.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
+    D.scheme = S.scheme;
     D.npad = npad;
     D.field = L.field;
     D.slabs = L.slabs;
@@ -346,12 +357,16 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                 for (int k0 = sl.lo[2]; k0 < sl.lo[2] + sl.e[2]; k0 += TK) {
                     TileDesc t{};
                     t.scene = scene_id;
-                    t.type = TILE_PML;
+                    t.type = S.scheme == FDTD_SCHEME_STAGGERED ? TILE_MAC_PML : TILE_PML;
                     t.i0 = i0; t.i1 = std::min(i0 + plan.pml_ti, sl.lo[0] + sl.e[0]);
                     t.j0 = j0; t.j1 = std::min(j0 + TJ, sl.lo[1] + sl.e[1]);
                     t.k0 = k0; t.k1 = std::min(k0 + TK, sl.lo[2] + sl.e[2]);
                     t.tk_log2 = tk;
                     t.slab = q;
+                    if (S.scheme == FDTD_SCHEME_STAGGERED) {
+                        const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+                        t.src = src_hit(lo, hi);
+                    }
                     out.push_back(t);
                 }
     }
@@ -369,7 +384,8 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     const int tk = std::min(plan.interior_tk_log2, std::max(0, pow2_ceil_log2(b[2] - a[2])));
     const int TK = 1 << tk, TJ = 256 >> tk;
     const int kw = b[2] - a[2];
-    const bool vec = plan.interior_vec && n[2] % 4 == 0 && a[2] % 4 == 0 && kw % 4 == 0 && kw <= 256;
+    const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
+    const bool vec = !mac && plan.interior_vec && n[2] % 4 == 0 && a[2] % 4 == 0 && kw % 4 == 0 && kw <= 256;
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
@@ -380,7 +396,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                 t.k0 = k0; t.k1 = std::min(k0 + TK, b[2]);
                 t.tk_log2 = tk;
                 const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
-                t.type = special_hit(lo, hi) ? TILE_INTERIOR_SKIP : TILE_INTERIOR;
+                t.type = mac ? TILE_MAC : special_hit(lo, hi) ? TILE_INTERIOR_SKIP : TILE_INTERIOR;
                 t.src = src_hit(lo, hi);
                 out.push_back(t);
             }
@@ -471,8 +487,11 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         if (t.scene < 0 || t.scene >= (int)scenes.size()) return "tile scene out of range";
         const SceneDev &d = scenes[t.scene];
         if (t.tb < 0 || t.te < t.tb || (size_t)t.te > nterms) return "tile term range out of range";
-        const bool ok_kind = kind == KIND_ALL || (kind == KIND_PML && t.type == TILE_PML) ||
-                             (kind == KIND_INTERIOR && t.type != TILE_PML) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL);
+        const bool pml_class = t.type == TILE_PML || t.type == TILE_MAC_PML;
+        const bool ok_kind = kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
+                             (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL);
+        if ((t.type == TILE_MAC || t.type == TILE_MAC_PML) != (d.scheme == FDTD_SCHEME_STAGGERED))
+            return "tile class does not match the scene's scheme";
         if (!ok_kind) return "tile class does not match its launch";
         if (t.type == TILE_SPECIAL) {
             if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";
@@ -481,13 +500,13 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
         for (int a = 0; a < 3; ++a) {
             if (lo[a] >= hi[a]) return "empty tile box";
-            if (t.type == TILE_PML) {
+            if (pml_class) {
                 if (lo[a] < 0 || hi[a] > d.n[a]) return "PML tile outside the grid";
             } else if (lo[a] < 1 || hi[a] > d.n[a] - 1) {
                 return "interior tile reads outside the grid";
             }
         }
-        if (t.type == TILE_PML) {
+        if (pml_class) {
             if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) return "PML tile too large";
         } else if (t.type == TILE_INTERIOR_VEC) {
             if (t.j1 - t.j0 > 4 || (t.k0 & 3) || ((t.k1 - t.k0) & 3) || t.k1 - t.k0 > 256 || (d.sy & 3))
@@ -506,7 +525,7 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
-        else by_kind[t.type == TILE_PML ? KIND_PML : KIND_INTERIOR].push_back(t);
+        else by_kind[(t.type == TILE_PML || t.type == TILE_MAC_PML) ? KIND_PML : KIND_INTERIOR].push_back(t);
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
     xcd_permute(by_kind[KIND_INTERIOR]);

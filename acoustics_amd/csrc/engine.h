@@ -61,7 +61,7 @@ struct SpecialBox {      // index box holding every special cell of one object
 };
 
 struct PSrcHost {
-    double t0, wt, amp;    // amp = K / 3.6836
+    double t0, wt, amp;    // amp = K / 3.6836 (collocated), K (staggered)
     int lo[3], hi[3];      // index range with a nonzero spatial factor
     int goff[3];           // offsets of the 1D factor tables in gtab
 };
@@ -96,11 +96,13 @@ struct SceneSetup {
     double h = 0, dt = 0, c = 0, rho = 0, t_start = 0, stop_time = 0;
     double lam2 = 0;             // (c dt / h)^2
     int boundary = 0;
+    int scheme = 0;                 // FDTD_SCHEME_*
     // PML, separable per axis
     std::vector<uint8_t> cpml[3];   // cell predicate, n_d entries
     std::vector<uint8_t> fpml[3];   // face predicate, n_d + 1 entries (walls included)
     std::vector<float> f1[3], f2[3];   // face tables (n_d + 1)
     std::vector<float> f3[3], f5h[3];  // cell tables (n_d), f5h = f5 / h
+    std::vector<uint8_t> sig0[3];      // cell: sigma_d <= 1e-12 (staggered source goes into pDir_d)
     int inner_lo[3], inner_hi[3];      // non-PML index box
     int64_t n_pml = 0;
     std::vector<PmlSlab> slabs;        // the PML as up to six disjoint boxes
@@ -132,7 +134,8 @@ struct SceneSetup {
 
 // Builds the full setup; returns FDTD_OK or an error with a message in err.
 fdtd_status build_setup(const fdtd_scene &scene, SceneSetup &out, std::string &err);
-// Algorithmic bytes per step: 12 N + 44 N_pml (FP32 fields).
+// Algorithmic bytes per step (FP32 fields): collocated 12 N + 44 N_pml;
+// staggered 32 N + 24 N_pml.
 double algorithmic_bytes(const SceneSetup &s);
 
 // thread-local error text for fdtd_last_error

@@ -49,13 +49,15 @@ struct SceneDev {
     int n[3];
     int sx, sy;                 // strides of i and j (k fastest)
     int raster;                 // 1: rasterized ghost values, 0: fully coupled
+    int scheme;                 // FDTD_SCHEME_*
     long long npad;             // floats per pressure slot
     float *field;               // NUM_P_SLOTS pressure slots, full grid, k fastest
     // PML state in compact per-slab storage (PmlSlab): split pressures per
     // PML cell, face velocities per axis, double-buffered by step parity
     const PmlSlab *slabs;
     float *pd[3];
-    float *vel[2][3];
+    float *vel[2][3];           // collocated: compact PML faces; staggered: every face, full grid
+                                // (face d of cell c, on its low side, at c's index)
     const float4 *coef[3];      // per axis, index 0..n: {f1(face), f2(face), f3(cell), f5/h(cell)}
     const int *flags[3];        // per axis, index 0..n: bit0 cell in PML, bit1 face in PML
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
@@ -98,7 +100,11 @@ struct TermRef {
 // Amplitudes of step s on the host (used to seed step 0).
 void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src);
 
-enum : int { TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4, TILE_INTERIOR_VEC = 5 };
+enum : int {
+    TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4, TILE_INTERIOR_VEC = 5,
+    TILE_MAC = 6,      // staggered scheme, cells outside the PML
+    TILE_MAC_PML = 7,  // staggered scheme, a PML slab (split pressures in the slab's compact storage)
+};
 
 struct TileDesc {
     int scene;

@@ -53,10 +53,16 @@ __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int o
     return a;
 }
 
-__device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double dt)
+__device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double dt, int scheme)
 {
-    // S(x, t_n + dt/2) dt without the spatial factor (GaussianPressureSource.cpp:6-30)
-    const double u = (tn + 0.5 * dt - ps.t0) / ps.wt;
+    // S(x, t_n + dt/2) dt without the spatial factor (GaussianPressureSource.cpp:6-30):
+    // the pulse itself (collocated) or its time integral (staggered)
+    const double t = tn + 0.5 * dt;
+    if (scheme == FDTD_SCHEME_STAGGERED) {
+        const double s2 = 1.4142135623730951 * ps.wt;
+        return (float)((erf((t - ps.t0) / s2) - erf(-ps.t0 / s2)) * ps.amp * dt);
+    }
+    const double u = (t - ps.t0) / ps.wt;
     return (float)(exp(-(u * u) / 2.0) * ps.amp * dt);
 }
 
@@ -323,6 +329,109 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, c
 }
 
 // ---------------------------------------------------------------------------
+// staggered (MAC) leapfrog, PML_WaveSolver::stepLeapfrog (PML_WaveSolver.cpp:
+// 710-773): face velocities v <- f1 v + f2 (p_R - p_L) with zero wall faces
+// (MAC_Grid.cpp:301-370), split pressures pDir_d <- f3 pDir_d + f5/h
+// (v_R - v_L) plus the source on every axis whose sigma is zero at the cell
+// (948-1020), p = sum_d pDir_d (1102-1129).  Outside the PML f1 = f3 = 1 and
+// the split pressures are not stored: p^{n+1} = p^n + sum_d increments.  Face
+// velocities are full-grid and double-buffered; a thread writes the low
+// faces of its cells and recomputes the high ones from the old values.
+// ---------------------------------------------------------------------------
+// One staggered face update, v <- f1 v + f2 (p_R - p_L), with its rounding
+// fixed by an explicit fma: a face is computed by two threads (as the high
+// face of one cell and the low face of the next) and must come out the same.
+__device__ __forceinline__ float mac_face(float4 c, float v, float pl, float pr)
+{
+    return fmaf(c.y, pr - pl, c.x * v);
+}
+
+template <bool PML>
+__device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
+{
+    const int tk = t.tk_log2;
+    const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
+    const int k = t.k0 + tx, j = t.j0 + ty;
+    if (k >= t.k1 || j >= t.j1) return;
+    const int nx = S.n[0], ny = S.n[1], nz = S.n[2];
+    const int sx = S.sx, sy = S.sy;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *__restrict__ pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    const int vb = (int)(step & 1);
+    const AS_GLOBAL float *__restrict__ vi0 = gmem(S.vel[vb][0]);
+    const AS_GLOBAL float *__restrict__ vi1 = gmem(S.vel[vb][1]);
+    const AS_GLOBAL float *__restrict__ vi2 = gmem(S.vel[vb][2]);
+    AS_GLOBAL float *__restrict__ vo0 = gmem(S.vel[vb ^ 1][0]);
+    AS_GLOBAL float *__restrict__ vo1 = gmem(S.vel[vb ^ 1][1]);
+    AS_GLOBAL float *__restrict__ vo2 = gmem(S.vel[vb ^ 1][2]);
+    const AS_CONST float4 *__restrict__ cx = cmem(S.coef[0]);
+    const AS_CONST int *__restrict__ fx = cmem(S.flags[0]);
+    const AS_CONST float4 *cy = cmem(S.coef[1]), *cz = cmem(S.coef[2]);
+    const AS_CONST int *fy = cmem(S.flags[1]), *fz = cmem(S.flags[2]);
+    const float4 cyL = cy[j], cyR = cy[min(j + 1, ny)], czL = cz[k], czR = cz[min(k + 1, nz)];
+    const bool has_yL = j > 0, has_yR = j + 1 < ny, has_zL = k > 0, has_zR = k + 1 < nz;
+    const bool src_y = fy[j] & 4, src_z = fz[k] & 4;   // sigma_d == 0 at the cell
+    const int ym_off = has_yL ? sy : 0, yp_off = has_yR ? sy : 0;
+    const int zm_off = has_zL ? 1 : 0, zp_off = has_zR ? 1 : 0;
+
+    AS_GLOBAL float *pd0 = nullptr, *pd1 = nullptr, *pd2 = nullptr;
+    int c = 0, cs = 0;
+    if constexpr (PML) {
+        const PmlSlab sl = cmem(S.slabs)[t.slab];
+        pd0 = gmem(S.pd[0] + sl.cell_base);
+        pd1 = gmem(S.pd[1] + sl.cell_base);
+        pd2 = gmem(S.pd[2] + sl.cell_base);
+        cs = sl.e[1] * sl.e[2];
+        c = ((t.i0 - sl.lo[0]) * sl.e[1] + (j - sl.lo[1])) * sl.e[2] + (k - sl.lo[2]);
+    }
+    int idx = t.i0 * sx + j * sy + k;
+    float pm = pcur[idx - (t.i0 > 0 ? sx : 0)];
+    float pc = pcur[idx];
+    float vxL = t.i0 > 0 ? mac_face(cx[t.i0], vi0[idx], pm, pc) : 0.f;
+    for (int i = t.i0; i < t.i1; ++i, idx += sx, c += cs) {
+        const bool has_xR = i + 1 < nx;
+        const float pp = pcur[idx + (has_xR ? sx : 0)];
+        const float ym = pcur[idx - ym_off], yp = pcur[idx + yp_off];
+        const float zm = pcur[idx - zm_off], zp = pcur[idx + zp_off];
+        const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
+        const float vxR = has_xR ? mac_face(cxR, vi0[idx + sx], pc, pp) : 0.f;
+        const float vyL = has_yL ? mac_face(cyL, vi1[idx], ym, pc) : 0.f;
+        const float vyR = has_yR ? mac_face(cyR, vi1[idx + sy], pc, yp) : 0.f;
+        const float vzL = has_zL ? mac_face(czL, vi2[idx], zm, pc) : 0.f;
+        const float vzR = has_zR ? mac_face(czR, vi2[idx + 1], pc, zp) : 0.f;
+        vo0[idx] = vxL;
+        vo1[idx] = vyL;
+        vo2[idx] = vzL;
+        const float ix = cxC.w * (vxR - vxL), iy = cyL.w * (vyR - vyL), iz = czL.w * (vzR - vzL);
+        float src = 0.f;
+        if (t.src)
+            for (int s = 0; s < S.n_psrc; ++s) {
+                const PSrcDev ps = cmem(S.psrc)[s];
+                const AS_CONST float *g = cmem(S.gtab);
+                src += gmem(S.src_now)[s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
+            }
+        float pn;
+        if constexpr (PML) {
+            const bool src_x = fx[i] & 4;
+            const float n0 = fmaf(cxC.z, pd0[c], ix) + (src_x ? src : 0.f);
+            const float n1 = fmaf(cyL.z, pd1[c], iy) + (src_y ? src : 0.f);
+            const float n2 = fmaf(czL.z, pd2[c], iz) + (src_z ? src : 0.f);
+            pd0[c] = n0;
+            pd1[c] = n1;
+            pd2[c] = n2;
+            pn = n0 + n1 + n2;
+        } else {
+            pn = pc + fmaf(3.f, src, (ix + iy) + iz);
+        }
+        pnext[idx] = pn;
+        if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, pn);
+        vxL = vxR;
+        pm = pc;
+        pc = pp;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // bulk cells next to rigid objects (FP64)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
@@ -398,7 +507,7 @@ __global__ __launch_bounds__(64) void advance_kernel(const SceneDev *__restrict_
     if (step >= (long long)clk[CLK_TARGET]) return;
     const double tn1 = S.t_start + (double)(step + 1) * S.dt;
     for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) gmem(S.acc_now)[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
-    for (int s = threadIdx.x; s < S.n_psrc; s += blockDim.x) gmem(S.src_now)[s] = source_amp(S.psrc[s], tn1, S.dt);
+    for (int s = threadIdx.x; s < S.n_psrc; s += blockDim.x) gmem(S.src_now)[s] = source_amp(S.psrc[s], tn1, S.dt, S.scheme);
     if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)(step + 1);
 }
 
@@ -421,15 +530,19 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         // FP64 gathers then hide under the interior's streaming
         if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
+        else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     } else if constexpr (KIND == KIND_PML) {
-        pml_tile(S, t, terms, step);
+        if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
+        else pml_tile(S, t, terms, step);
     } else if constexpr (KIND == KIND_SPECIAL) {
         special_tile(S, t, terms, step);
     } else {
         if (t.type == TILE_PML) pml_tile(S, t, terms, step);
         else if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
+        else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
+        else if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     }
     if (t.te > t.tb) emit_ghost_terms(S, t, terms, step);

@@ -232,6 +232,7 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     d.sy = D.S.n[2];
     d.sx = D.S.n[1] * D.S.n[2];
     d.n_special = (int)D.S.special.size();
+    d.scheme = D.S.scheme;
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
     D.append_tiles(0, tp, tiles, terms);

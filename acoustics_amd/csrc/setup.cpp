@@ -137,7 +137,14 @@ inline uint64_t pair_key(int64_t a_, int64_t b_)
 
 }  // namespace
 
-double algorithmic_bytes(const SceneSetup &s) { return 12.0 * (double)s.N + 44.0 * (double)s.n_pml; }
+double algorithmic_bytes(const SceneSetup &s)
+{
+    // collocated: interior cell reads p^n, p^{n-1}, writes p^{n+1}; PML cell adds
+    // v[3] and pDir[3] read and written.  staggered: every cell reads and
+    // writes v[3] and reads p^n, writes p^{n+1}; PML cells add pDir[3].
+    if (s.scheme == FDTD_SCHEME_STAGGERED) return 32.0 * (double)s.N + 24.0 * (double)s.n_pml;
+    return 12.0 * (double)s.N + 44.0 * (double)s.n_pml;
+}
 
 fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
 {
@@ -167,6 +174,13 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     S.t_start = st.start_time; S.stop_time = st.stop_time;
     S.lam2 = (S.c * S.dt / S.h) * (S.c * S.dt / S.h);
     S.boundary = st.boundary_handling;
+    S.scheme = st.scheme;
+    if (st.scheme != FDTD_SCHEME_COLLOCATED && st.scheme != FDTD_SCHEME_STAGGERED) { err = "unknown scheme"; return FDTD_ERR_INVALID; }
+    if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0)) {
+        err = "the staggered scheme runs free-field scenes only (objects need the reference's mesh distance-field "
+              "classification of the MAC mode); use the collocated scheme";
+        return FDTD_ERR_UNSUPPORTED;
+    }
     const double h = S.h, dt = S.dt, rho = S.rho, c = S.c;
     const double W = st.pml_width * h;         // MAC_Grid::setPMLBoundaryWidth
     const double Sg = st.pml_strength;
@@ -206,13 +220,14 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     };
     for (int d = 0; d < 3; ++d) {
         const int n = S.n[d];
-        S.cpml[d].assign(n, 0); S.f3[d].assign(n, 0.f); S.f5h[d].assign(n, 0.f);
+        S.cpml[d].assign(n, 0); S.f3[d].assign(n, 0.f); S.f5h[d].assign(n, 0.f); S.sig0[d].assign(n, 1);
         S.fpml[d].assign(n + 1, 0); S.f1[d].assign(n + 1, 0.f); S.f2[d].assign(n + 1, 0.f);
         for (int i = 0; i < n; ++i) {
             double x = bmin[d];
             x += (0.5 + (double)i) * h;
             S.cpml[d][i] = inside_axis(d, x);
             const double sg = sigma_axis(d, x);
+            S.sig0[d][i] = sg > 1e-12 ? 0 : 1;   // MAC_Grid.cpp:966-967 inPML test
             const double D = 1.0 / dt + sg / 2.0;
             S.f3[d][i] = (float)((1.0 / dt - sg / 2.0) / D);
             S.f5h[d][i] = (float)((-1.0 * rho * c * c / D) * (1.0 / h));
@@ -283,7 +298,10 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         const fdtd_pressure_source &ps = sc.pressure_sources[q];
         if (!(ps.width_time > 0) || !(ps.width_space > 0)) { err = "pressure source widths must be positive"; return FDTD_ERR_INVALID; }
         PSrcHost P;
-        P.t0 = ps.offset_time; P.wt = ps.width_time; P.amp = 1. / 3.6836 * ps.normalize_constant;
+        P.t0 = ps.offset_time; P.wt = ps.width_time;
+        // GaussianPressureSource.cpp:21-27: the collocated build scales the pulse
+        // by K / 3.6836, the staggered build integrates it in time and scales by K
+        P.amp = st.scheme == FDTD_SCHEME_STAGGERED ? ps.normalize_constant : 1. / 3.6836 * ps.normalize_constant;
         for (int d = 0; d < 3; ++d) {
             const double mn = ps.position[d] - ps.width_space * kCheckBound;
             const double mx = ps.position[d] + ps.width_space * kCheckBound;

@@ -12,7 +12,7 @@ import math
 
 import numpy as np
 
-from .capi import (FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_ELLIPSOID,
+from .capi import (FDTD_SCHEME_STAGGERED, FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_ELLIPSOID,
                    FDTD_OBJ_SPHERE, SceneBuffers)
 
 C_SOUND = 343.0
@@ -121,6 +121,19 @@ def config1(seed=1, n=64, steps=500):
     lo, hi = interior_bounds(s, 2)
     src = gaussian_pressure_source(rng.uniform(lo, hi), 2e-4)
     listeners = rng.uniform(lo, hi, size=(8, 3))
+    return SceneBuffers(s, pressure_sources=[src], listeners=listeners), steps
+
+
+def staggered_free_field(seed=7, div=(40, 40, 40), h=0.005, pml=6, steps=300, n_listeners=8):
+    """Free field in the reference's staggered (MAC) scheme: one Gaussian
+    pressure pulse (width 2e-4 s, time-integrated by that scheme) at a seeded
+    interior point, seeded listeners."""
+    rng = np.random.default_rng(seed)
+    s = base_settings(div, h, pml)
+    s["scheme"] = FDTD_SCHEME_STAGGERED
+    lo, hi = interior_bounds(s, 2)
+    src = gaussian_pressure_source(rng.uniform(lo, hi), 2e-4)
+    listeners = rng.uniform(lo, hi, size=(n_listeners, 3))
     return SceneBuffers(s, pressure_sources=[src], listeners=listeners), steps
 
 

@@ -81,6 +81,18 @@ enum { FDTD_OBJ_SPHERE = 0, FDTD_OBJ_ELLIPSOID = 1 };
 /* Solver settings, field by field from PML_WaveSolver_Settings and the XML
  * <solver> / <solver_control_policy type="static"> attributes
  * (src/parser/ImpulseResponseParser.cpp:492-641). */
+/* Time-stepping scheme.  The reference compiles one of two: the collocated
+ * pressure scheme (PML_WaveSolver::stepCollocated, its default because
+ * USE_COLLOCATED is defined in Wavesolver_ConstantsAndTypes.h:37) or the fully
+ * staggered MAC leapfrog (PML_WaveSolver::stepLeapfrog, PML_WaveSolver.cpp:710-773).
+ * The staggered scheme runs free-field scenes (pressure sources, PML, no
+ * objects); its Gaussian pressure source is the time integral of the pulse
+ * (GaussianPressureSource.cpp:24-27). */
+enum {
+    FDTD_SCHEME_COLLOCATED = 0,
+    FDTD_SCHEME_STAGGERED  = 1
+};
+
 typedef struct fdtd_settings {
     double sound_speed;               /* sound_speed (m/s), default 343        */
     double density;                   /* density (kg/m^3)                      */
@@ -96,6 +108,8 @@ typedef struct fdtd_settings {
     int    use_mesh;                  /* must be 1: builds the PML lists       */
     int    divisions[3];              /* pressure cells per axis               */
     double min_bound[3];              /* minimum corner of the pressure grid   */
+    int    scheme;                    /* FDTD_SCHEME_*                         */
+    int    reserved;
 } fdtd_settings;
 
 /* Rigid object with an analytic signed distance.  The rotation maps object

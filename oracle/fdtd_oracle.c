@@ -521,6 +521,9 @@ struct oracle {
 
     double t;
     long long step;
+    /* staggered (MAC) scheme: pFull; pdir[3] and v[3] are the full-grid fields */
+    int scheme;
+    double *pfull;
 };
 
 static inline long long pidx(const oracle *o, int i, int j, int k) { return ((long long)i * o->n[1] + j) * o->n[2] + k; }
@@ -1158,6 +1161,109 @@ static void fetch_listeners(const oracle *o, const double *scalar, double *out)
 }
 
 /* ======================================================================== */
+/* staggered MAC leapfrog (PML_WaveSolver::stepLeapfrog, 710-773), free      */
+/* field: every cell and every velocity face is bulk                         */
+/* ======================================================================== */
+
+/* GaussianPressureSource::Evaluate with USE_COLLOCATED undefined
+ * (GaussianPressureSource.cpp:6-30): the time factor is the pulse integrated
+ * from t = 0, erf((t - t0)/(sqrt2 wt)) - erf(-t0/(sqrt2 wt)), and the
+ * amplitude is the normalisation constant itself. */
+static double pressure_sources_mac(const oracle *o, const double *x, double t)
+{
+    const double sqrt_2 = sqrt(2.0);   /* Wavesolver_ConstantsAndTypes.h:75 */
+    double acc = 0;
+    for (int i = 0; i < o->sc.n_pressure_sources; ++i) {
+        const fdtd_pressure_source *s = &o->psrc[i];
+        const double *mn = o->src_bmin[i], *mx = o->src_bmax[i];
+        if (!(x[0] < mx[0] && x[0] > mn[0] && x[1] < mx[1] && x[1] > mn[1] && x[2] < mx[2] && x[2] > mn[2])) continue;
+        const double *sp = o->src_pos[i];
+        double value = (x[0]-sp[0])*(x[0]-sp[0]) + (x[1]-sp[1])*(x[1]-sp[1]) + (x[2]-sp[2])*(x[2]-sp[2]);
+        value = -value / (2.0 * s->width_space * s->width_space);
+        value = exp(value);
+        value *= erf((t - s->offset_time) / (sqrt_2 * s->width_time)) - erf(-s->offset_time / (sqrt_2 * s->width_time));
+        value *= s->normalize_constant;
+        acc += value;
+    }
+    return acc;
+}
+
+/* MAC_Grid::PML_velocityUpdate (MAC_Grid.cpp:301-370), bulk faces: zero on
+ * the two walls of each axis; in the PML (sigma_d > 1e-12 at the face)
+ * v = f1 v + f2 (p_R - p_L), elsewhere v += -dt/(h rho) (p_R - p_L). */
+static void mac_velocity(oracle *o)
+{
+    const fdtd_settings *s = &o->sc.settings;
+    const double *p = o->pfull;
+    const double ndt = -o->dt / (o->h * o->rho);
+    for (int d = 0; d < 3; ++d) {
+        const int nv[3] = {o->n[0] + (d == 0), o->n[1] + (d == 1), o->n[2] + (d == 2)};
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < nv[0]; ++i)
+            for (int j = 0; j < nv[1]; ++j)
+                for (int k = 0; k < nv[2]; ++k) {
+                    int ijk[3] = {i, j, k};
+                    double *v = &o->v[d][vidx(o, d, i, j, k)];
+                    if (ijk[d] == 0 || ijk[d] == nv[d] - 1) { *v = 0.0; continue; }
+                    double x[3];
+                    vpos(o, d, ijk, x);
+                    const double sg = pml_sigma(s, x, o->W, d);
+                    const long long right = pidx(o, i, j, k);
+                    ijk[d] -= 1;
+                    const long long left = pidx(o, ijk[0], ijk[1], ijk[2]);
+                    if (sg > 1e-12) {
+                        const double upd = f_vel_update(sg, o->dt), grad = f_grad(sg, o->dt, o->h, o->rho);
+                        *v *= upd;
+                        *v += grad * p[right];
+                        *v -= grad * p[left];
+                    } else {
+                        *v += ndt * p[right];
+                        *v -= ndt * p[left];
+                    }
+                }
+    }
+}
+
+/* MAC_Grid::PML_pressureUpdate (MAC_Grid.cpp:948-1020) for each axis, then
+ * MAC_Grid::UpdatePMLPressure (1102-1129): pFull = pDir0 + pDir1 + pDir2.
+ * The pressure source is added to pDir_d of every axis d whose sigma_d is
+ * zero at the cell. */
+static void mac_pressure(oracle *o)
+{
+    const fdtd_settings *s = &o->sc.settings;
+    const double nrc = -o->rho * o->c * o->c * o->dt;
+    const double hinv = 1.0 / o->h;
+    const int has_src = o->sc.n_pressure_sources > 0;
+    for (int d = 0; d < 3; ++d) {
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < o->n[0]; ++i)
+            for (int j = 0; j < o->n[1]; ++j)
+                for (int k = 0; k < o->n[2]; ++k) {
+                    int ijk[3] = {i, j, k};
+                    double x[3];
+                    ppos(o, ijk, x);
+                    const double sg = pml_sigma(s, x, o->W, d);
+                    double *pd = &o->pdir[d][pidx(o, i, j, k)];
+                    const long long left = vidx(o, d, i, j, k);
+                    ijk[d] += 1;
+                    const long long right = vidx(o, d, ijk[0], ijk[1], ijk[2]);
+                    if (sg > 1e-12) {
+                        const double D = f_dir(sg, o->dt), upd = f_pupd(sg, o->dt, D), div = f_div(o->rho, o->c, D);
+                        *pd *= upd;
+                        *pd += div * o->v[d][right] / o->h;
+                        *pd -= div * o->v[d][left] / o->h;
+                    } else {
+                        *pd += nrc * o->v[d][right] * hinv;
+                        *pd -= nrc * o->v[d][left] * hinv;
+                        if (has_src) *pd += pressure_sources_mac(o, x, o->t + 0.5 * o->dt) * o->dt;
+                    }
+                }
+    }
+    #pragma omp parallel for schedule(static)
+    for (long long c = 0; c < o->N; ++c) o->pfull[c] = o->pdir[0][c] + o->pdir[1][c] + o->pdir[2][c];
+}
+
+/* ======================================================================== */
 /* construction (PML_WaveSolver::Reinitialize_PML_WaveSolver, 102-146)       */
 /* ======================================================================== */
 oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
@@ -1203,6 +1309,15 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     o->pc_ind = 1; o->lap_ind = 1;
     o->t = s->start_time;
     o->reclass = reclassify_each_step;
+    o->scheme = s->scheme;
+    if (o->scheme == FDTD_SCHEME_STAGGERED) {
+        if (scene->n_objects > 0) {
+            set_err("the staggered scheme is restated for free-field scenes only");
+            oracle_free(o);
+            return NULL;
+        }
+        o->pfull = (double *)calloc(o->N, sizeof(double));
+    }
     classify_static(o);
     memset(o->history, 1, (size_t)o->N);
     classify_fast(o);
@@ -1219,12 +1334,23 @@ void oracle_free(oracle *o)
     free(o->bulk); free(o->ghostc); free(o->pmlc); free(o->cavity); free(o->history); free(o->classified);
     free(o->pp); free(o->pv); free(o->gh); free(o->hkeys); free(o->hvals);
     free(o->objs); free(o->isrc); free(o->psrc); free(o->listeners);
+    free(o->pfull);
     free(o);
 }
 
 /* PML_WaveSolver::stepCollocated (PML_WaveSolver.cpp:775-826) */
 int oracle_step(oracle *o, int nsteps, double *rows)
 {
+    if (o->scheme == FDTD_SCHEME_STAGGERED) {
+        for (int st = 0; st < nsteps; ++st) {
+            mac_velocity(o);
+            mac_pressure(o);
+            o->t += o->dt;
+            o->step += 1;
+            if (rows) fetch_listeners(o, o->pfull, rows + (size_t)st * o->sc.n_listeners);
+        }
+        return 0;
+    }
     for (int st = 0; st < nsteps; ++st) {
         double *pLast = o->pc[(o->pc_ind + 2) % 3];
         double *pCurr = o->pc[o->pc_ind];
@@ -1249,7 +1375,10 @@ int oracle_step(oracle *o, int nsteps, double *rows)
     return 0;
 }
 
-void oracle_get_pressure(const oracle *o, double *out) { memcpy(out, o->pc[o->pc_ind], sizeof(double) * o->N); }
+void oracle_get_pressure(const oracle *o, double *out)
+{
+    memcpy(out, o->scheme == FDTD_SCHEME_STAGGERED ? o->pfull : o->pc[o->pc_ind], sizeof(double) * o->N);
+}
 
 void oracle_set_state(oracle *o, const double *p_last, const double *p_curr)
 {

@@ -1,6 +1,7 @@
 /*
  * fdtd_oracle.h -- FP64 CPU restatement of the reference wavesolver's default
- * (USE_COLLOCATED) time-stepping path.
+ * (USE_COLLOCATED) time-stepping path, and of its fully staggered MAC
+ * leapfrog for free-field scenes (settings.scheme = FDTD_SCHEME_STAGGERED).
  *
  * TEST INFRASTRUCTURE.  This library is the checker: only tests/, the smoke
  * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load
@@ -18,6 +19,11 @@
  *   src/wavesolver/FDTD_RigidObject.cpp  256-285, 330-368, 429-536 (semantics;
  *                                        analytic sphere/ellipsoid geometry)
  *   src/math/MLSModeInterpolator.hpp     282-429, 588-597, 624-660, 712-738
+ * and, for the staggered scheme:
+ *   src/wavesolver/PML_WaveSolver.cpp    710-773 (stepLeapfrog)
+ *   src/wavesolver/MAC_Grid.cpp          301-370, 948-1020, 1102-1129
+ *   src/wavesolver/GaussianPressureSource.cpp 24-27 (erf time factor)
+ *   src/linearalgebra/MATRIX_FAST.cpp    555-569 (pFull = pDir0 + pDir1 + pDir2)
  * Third-party arithmetic the reference takes from Eigen 3.3.7 (JacobiSVD,
  * BiCGSTAB) is restated here from the published algorithms (one-sided
  * Jacobi SVD with Eigen's rank threshold; BiCGSTAB with a diagonal

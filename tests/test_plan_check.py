@@ -25,6 +25,7 @@ def _scenes():
     out += [(f"batch{i}", b) for i, b in enumerate(bs)]
     stream = scenes.config5_stream(seed=5, n_requests=2, edge_lo=48, edge_hi=64)
     out += [(f"stream{i}", sb) for i, (_, sb, _) in enumerate(stream)]
+    out.append(("staggered_36x40x44", scenes.staggered_free_field(seed=7, div=(36, 40, 44))[0]))
     return out
 
 
@@ -44,3 +45,13 @@ def test_every_plan_is_valid(name, sb):
     # the default plan is plan 0
     n = C.c_int64(0)
     assert lib.fdtd_plan_check(C.byref(sb.scene), -1, C.byref(n)) == 0 and n.value == counts[0]
+
+
+def test_staggered_scheme_rejects_objects():
+    """The staggered scheme runs free-field scenes only; a scene with an object
+    is refused with FDTD_ERR_UNSUPPORTED and a message."""
+    lib = capi.load_library()
+    sb, _ = scenes.config2(seed=2, n=48, pml=8, radius=0.03, shell=0.07, n_listeners=4)
+    sb.scene.settings.scheme = capi.FDTD_SCHEME_STAGGERED
+    assert lib.fdtd_plan_check(C.byref(sb.scene), 0, None) == 6
+    assert b"staggered" in lib.fdtd_last_error()
