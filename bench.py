@@ -9,9 +9,14 @@ headline metric.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line:
   value        whole-job Gcell-updates/s = N * steps / wall time
-  roofline     dominant kernel (the fused step kernel): algorithmic bytes
-               (12 N + 44 N_pml per step) / mean kernel duration from HIP
-               events recorded on the kernel's stream during the timed region
+  roofline     the step kernel: algorithmic bytes (12 N + 44 N_pml per step)
+               / mean kernel duration from HIP events recorded on the kernel's
+               stream during the timed region; traffic from the committed
+               counter summary (profiles/step_kernel_256cube_traffic.json)
+  staggered    the reference's staggered (MAC) scheme on a 256^3 free field
+  batch        64 uneven scenes advanced together vs one at a time
+  serving      a Poisson request stream: continuous batching vs the plain
+               one-request-at-a-time path
   cpu_baseline FP64 CPU oracle (OpenMP) on a bounded sample of the same scene
 """
 from __future__ import annotations
@@ -21,6 +26,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -41,6 +48,9 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--serve", type=int, default=24, help="requests in the serving comparison (0: skip)")
     ap.add_argument("--block", type=int, default=64, help="server steps per block")
+    ap.add_argument("--staggered-steps", type=int, default=200,
+                    help="steps of the staggered-scheme measurement (0: skip)")
+    ap.add_argument("--batch", type=int, default=64, help="scenes in the batched measurement (0: skip)")
     return ap.parse_args()
 
 
@@ -62,6 +72,62 @@ def cpu_baseline(sb, steps):
     return {"value": n * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"{sb.divisions[0]}^3 headline scene, {steps} FP64 steps after 1 warmup step, "
                       f"per-step reclassification as in the reference, {t:.2f} s"}
+
+
+def staggered(steps, n=256):
+    """The reference's staggered (MAC) scheme on an n^3 free-field scene (h =
+    0.64/n m, 16-cell PML, one Gaussian pressure pulse, 8 listeners): whole-job
+    Gcell-updates/s and the step kernel against HBM peak.  Algorithmic bytes
+    per step for that scheme: 32 N + 24 N_pml."""
+    from acoustics_amd import scenes
+    from acoustics_amd.solver import Solver
+    sb, _ = scenes.staggered_free_field(seed=7, div=(n, n, n), h=0.64 / n, pml=16, n_listeners=8)
+    with Solver(sb, series_capacity=max(1024, steps + 64)) as s:
+        info = s.info()
+        s.set_timing(True)
+        s.step(20)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        kern_ms, step_ms = s.last_kernel_ms()
+        finite = bool(np.isfinite(s.series()).all())
+    ach = info["bytes_per_step"] / (kern_ms * 1e-3) / 1e9
+    return {"workload": f"{n}^3 free field, staggered MAC scheme, 16-cell PML, 1 pressure source, 8 listeners",
+            "value": round(info["n_cells"] * steps / wall / 1e9, 3), "unit": "Gcell-updates/s", "steps": steps,
+            "bytes_per_step": info["bytes_per_step"], "kernel_ms": round(kern_ms, 5), "step_ms": round(step_ms, 5),
+            "achieved_GBps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4), "listeners_finite": finite}
+
+
+def batched(n_scenes):
+    """Uneven batch (edges U{32..128} per axis, 8-cell PML, one sphere or
+    ellipsoid with an impulse, 16 listeners, 1000-3000 steps each): all scenes
+    advanced together (one launch per step, 32-step graphs, autotuned tile
+    plan) against the same scenes solved one after another with the
+    single-scene API (all steps in one call each)."""
+    from acoustics_amd import scenes
+    from acoustics_amd.solver import Batch, Solver
+    sbs, steps = scenes.config4(seed=4, n_scenes=n_scenes)
+    cells = sum(sb.n_cells * st for sb, st in zip(sbs, steps))
+    t0 = time.perf_counter()
+    with Batch(sbs, steps) as b:
+        t1 = time.perf_counter()
+        b.run()
+        ok = all(bool(np.isfinite(b.series(i)).all()) for i in range(len(sbs)))
+    t_batch = time.perf_counter() - t1
+    t_batch_all = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for sb, st in zip(sbs, steps):
+        with Solver(sb) as s:
+            s.step(st)
+            s.series()
+    t_plain = time.perf_counter() - t0
+    return {"scenes": len(sbs), "cell_updates": cells,
+            "batched_Gcell_per_s": round(cells / t_batch / 1e9, 3),
+            "batched_wall_s": round(t_batch, 3), "batched_wall_with_setup_s": round(t_batch_all, 3),
+            "one_at_a_time_Gcell_per_s": round(cells / t_plain / 1e9, 3), "one_at_a_time_wall_s": round(t_plain, 3),
+            "speedup": round(t_plain / t_batch_all, 3), "listeners_finite": ok}
 
 
 def _percentile(xs, q):
@@ -206,6 +272,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5), "step_ms_events": round(step_ms, 5)},
     }
+    if a.staggered_steps > 0:
+        out["staggered"] = staggered(a.staggered_steps, n)
+    if a.batch > 0:
+        out["batch"] = batched(a.batch)
     if a.serve > 0:
         out["serving"] = serving(a.serve, a.block)
     if not a.no_cpu_baseline:
