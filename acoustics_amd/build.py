@@ -29,6 +29,7 @@ SOURCES = [
     ("setup.cpp", False),
     ("geometry.cpp", False),
     ("xml.cpp", False),
+    ("io.cpp", False),
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -136,6 +136,12 @@ SIGNATURES = {
     "fdtd_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "fdtd_scene_device_bytes": (C.c_int64, [C.POINTER(Scene)]),
     "fdtd_plan_check": (C.c_int, [C.POINTER(Scene), C.c_int, C.POINTER(C.c_int64)]),
+    "fdtd_parsed_output_pattern": (C.c_char_p, [C.c_void_p]),
+    "fdtd_composite_filename": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "fdtd_write_matrix": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.c_int, C.c_int]),
+    "fdtd_run_xml": (C.c_int, [C.c_char_p, C.c_char_p, C.c_void_p, C.POINTER(C.c_int64)]),
+    "fdtd_save_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "fdtd_load_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
     "fdtd_server_destroy": (C.c_int, [_P]),
 }
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <new>
 
@@ -315,6 +316,213 @@ fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *o)
     o->time = S.t_start + (double)h->step * S.dt;
     o->bytes_per_step = algorithmic_bytes(S);
     o->device_bytes = (int64_t)h->D.arena_bytes;
+    return FDTD_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// snapshots (FDTD_AcousticSimulator::_SaveSimulationSnapshot layout)
+// ---------------------------------------------------------------------------
+namespace {
+
+std::string snap_name(const char *pattern, const char *id, int numbering, const std::string &what)
+{
+    const std::string name = "snapshot_" + std::to_string(numbering) + "_" + what;
+    std::vector<char> buf(name.size() + (pattern ? std::strlen(pattern) : 2) + (id ? std::strlen(id) : 0) + 16);
+    fdtd_composite_filename(pattern, id, name.c_str(), buf.data(), (int)buf.size());
+    return std::string(buf.data());
+}
+
+fdtd_status read_matrix(const std::string &path, long long rows, int cols, std::vector<double> &out)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) { set_last_error("cannot open " + path); return FDTD_ERR_INVALID; }
+    int r = 0, c = 0;
+    in.read(reinterpret_cast<char *>(&r), sizeof(int));
+    in.read(reinterpret_cast<char *>(&c), sizeof(int));
+    if (!in || r != rows || c != cols) { set_last_error(path + ": matrix shape does not match the scene"); return FDTD_ERR_INVALID; }
+    out.resize((size_t)rows * cols);
+    in.read(reinterpret_cast<char *>(out.data()), sizeof(double) * out.size());
+    if (!in) { set_last_error(path + ": short file"); return FDTD_ERR_INVALID; }
+    return FDTD_OK;
+}
+
+// index of face (i, j, k) of axis d in the full face grid, n_d + 1 faces along d
+inline long long face_index(const int *n, int d, long long i, long long j, long long k)
+{
+    const long long ny = n[1] + (d == 1), nz = n[2] + (d == 2);
+    return (i * ny + j) * nz + k;
+}
+
+// Walks every compact PML cell (fn(cell_offset, flat_cell)) or face of axis d
+// (fn(face_offset, full_face_index)).
+template <class F>
+void for_pml_cells(const SceneSetup &S, F fn)
+{
+    for (const PmlSlab &sl : S.slabs)
+        for (int li = 0; li < sl.e[0]; ++li)
+            for (int lj = 0; lj < sl.e[1]; ++lj)
+                for (int lk = 0; lk < sl.e[2]; ++lk) {
+                    const long long c = sl.cell_base + ((long long)li * sl.e[1] + lj) * sl.e[2] + lk;
+                    const long long g = ((long long)(sl.lo[0] + li) * S.n[1] + sl.lo[1] + lj) * S.n[2] + sl.lo[2] + lk;
+                    fn(c, g);
+                }
+}
+template <class F>
+void for_pml_faces(const SceneSetup &S, int d, F fn)
+{
+    for (const PmlSlab &sl : S.slabs) {
+        const int E[3] = {sl.e[0] + (d == 0), sl.e[1] + (d == 1), sl.e[2] + (d == 2)};
+        for (int li = 0; li < E[0]; ++li)
+            for (int lj = 0; lj < E[1]; ++lj)
+                for (int lk = 0; lk < E[2]; ++lk) {
+                    const long long c = sl.v_base[d] + ((long long)li * E[1] + lj) * E[2] + lk;
+                    fn(c, face_index(S.n, d, sl.lo[0] + li, sl.lo[1] + lj, sl.lo[2] + lk));
+                }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
+{
+    if (!h) return FDTD_ERR_INVALID;
+    FDTD_HIP_TRY(hipSetDevice(h->D.device));
+    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
+    const SceneSetup &S = h->D.S;
+    const long long N = S.N, s = h->step;
+    const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
+    std::vector<float> f(N);
+    std::vector<double> cur(N), prev(N), zero(N, 0.0);
+    FDTD_HIP_TRY(hipMemcpy(f.data(), h->D.slot(slot_cur(s)), N * 4, hipMemcpyDeviceToHost));
+    for (long long c = 0; c < N; ++c) cur[c] = f[c];
+    FDTD_HIP_TRY(hipMemcpy(f.data(), h->D.slot(slot_next(s)), N * 4, hipMemcpyDeviceToHost));
+    for (long long c = 0; c < N; ++c) prev[c] = f[c];
+    auto put = [&](const std::string &what, const std::vector<double> &v, long long rows) {
+        return fdtd_write_matrix(snap_name(pattern, simulator_id, numbering, what).c_str(), v.data(), (int)rows, 1);
+    };
+    fdtd_status st;
+    // the reference's ring: p_collocated[ind] is p^n, p_collocated[(ind + 2) % 3] is p^{n-1}
+    if ((st = put("p_collocated_0.dat", mac ? zero : prev, N)) != FDTD_OK) return st;
+    if ((st = put("p_collocated_1.dat", mac ? zero : cur, N)) != FDTD_OK) return st;
+    if ((st = put("p_collocated_2.dat", zero, N)) != FDTD_OK) return st;
+    if ((st = put("p_pml_full.dat", mac ? cur : zero, N)) != FDTD_OK) return st;
+    {
+        std::ofstream of(snap_name(pattern, simulator_id, numbering, "p_collocated_ind.dat"));
+        of << 1;
+        std::ofstream os(snap_name(pattern, simulator_id, numbering, "step.txt"));
+        os << s << "\n";
+        if (!of || !os) { set_last_error("cannot write snapshot text files"); return FDTD_ERR_INVALID; }
+    }
+    std::vector<float> comp;
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double> full(N, 0.0);
+        comp.resize(std::max<int64_t>(1, S.n_pml));
+        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.pd[d], S.n_pml * 4, hipMemcpyDeviceToHost));
+        for_pml_cells(S, [&](long long c, long long g) { full[g] = comp[c]; });
+        if ((st = put("p_pml_" + std::to_string(d) + ".dat", full, N)) != FDTD_OK) return st;
+    }
+    const int vb = (int)(s & 1);  // the buffer the next step reads
+    for (int d = 0; d < 3; ++d) {
+        const long long NV = (long long)(S.n[0] + (d == 0)) * (S.n[1] + (d == 1)) * (S.n[2] + (d == 2));
+        std::vector<double> full(NV, 0.0);
+        if (mac) {
+            comp.resize(N);
+            FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.vel[vb][d], N * 4, hipMemcpyDeviceToHost));
+            for (long long i = 0; i < S.n[0]; ++i)
+                for (long long j = 0; j < S.n[1]; ++j)
+                    for (long long k = 0; k < S.n[2]; ++k)
+                        full[face_index(S.n, d, i, j, k)] = comp[(i * S.n[1] + j) * S.n[2] + k];
+        } else {
+            comp.resize(std::max<int64_t>(1, S.n_pml_faces[d]));
+            if (S.n_pml_faces[d])
+                FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.vel[vb][d], S.n_pml_faces[d] * 4, hipMemcpyDeviceToHost));
+            for_pml_faces(S, d, [&](long long c, long long g) { full[g] = comp[c]; });
+        }
+        if ((st = put("v_pml_" + std::to_string(d) + ".dat", full, NV)) != FDTD_OK) return st;
+    }
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
+{
+    if (!h) return FDTD_ERR_INVALID;
+    FDTD_HIP_TRY(hipSetDevice(h->D.device));
+    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
+    const SceneSetup &S = h->D.S;
+    const long long N = S.N;
+    const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
+    long long s = -1;
+    int ind = -1;
+    {
+        std::ifstream is(snap_name(pattern, simulator_id, numbering, "step.txt"));
+        std::ifstream ii(snap_name(pattern, simulator_id, numbering, "p_collocated_ind.dat"));
+        if (!(is >> s) || s < 0 || !(ii >> ind) || ind < 0 || ind > 2) {
+            set_last_error("snapshot step or ring index missing");
+            return FDTD_ERR_INVALID;
+        }
+    }
+    std::vector<double> cur, prev;
+    fdtd_status st;
+    if (mac) {
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_pml_full.dat"), N, 1, cur)) != FDTD_OK) return st;
+        prev.assign(N, 0.0);
+    } else {
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_collocated_" + std::to_string(ind) + ".dat"), N, 1, cur)) != FDTD_OK)
+            return st;
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_collocated_" + std::to_string((ind + 2) % 3) + ".dat"), N, 1, prev)) != FDTD_OK)
+            return st;
+    }
+    // the whole state starts from zero: faces and cells the snapshot does not hold stay at rest
+    FDTD_HIP_TRY(h->D.zero_state(h->stream));
+    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<float> f(N);
+    for (long long c = 0; c < N; ++c) f[c] = (float)cur[c];
+    FDTD_HIP_TRY(hipMemcpy(h->D.slot(slot_cur(s)), f.data(), N * 4, hipMemcpyHostToDevice));
+    for (long long c = 0; c < N; ++c) f[c] = (float)prev[c];
+    FDTD_HIP_TRY(hipMemcpy(h->D.slot(slot_next(s)), f.data(), N * 4, hipMemcpyHostToDevice));
+    std::vector<double> full;
+    std::vector<float> comp;
+    for (int d = 0; d < 3; ++d) {
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_pml_" + std::to_string(d) + ".dat"), N, 1, full)) != FDTD_OK)
+            return st;
+        comp.assign(std::max<int64_t>(1, S.n_pml), 0.f);
+        for_pml_cells(S, [&](long long c, long long g) { comp[c] = (float)full[g]; });
+        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(h->D.dev.pd[d], comp.data(), S.n_pml * 4, hipMemcpyHostToDevice));
+    }
+    const int vb = (int)(s & 1);
+    for (int d = 0; d < 3; ++d) {
+        const long long NV = (long long)(S.n[0] + (d == 0)) * (S.n[1] + (d == 1)) * (S.n[2] + (d == 2));
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "v_pml_" + std::to_string(d) + ".dat"), NV, 1, full)) != FDTD_OK)
+            return st;
+        if (mac) {
+            comp.assign(N, 0.f);
+            for (long long i = 0; i < S.n[0]; ++i)
+                for (long long j = 0; j < S.n[1]; ++j)
+                    for (long long k = 0; k < S.n[2]; ++k)
+                        comp[(i * S.n[1] + j) * S.n[2] + k] = (float)full[face_index(S.n, d, i, j, k)];
+            FDTD_HIP_TRY(hipMemcpy(h->D.dev.vel[vb][d], comp.data(), N * 4, hipMemcpyHostToDevice));
+        } else {
+            comp.assign(std::max<int64_t>(1, S.n_pml_faces[d]), 0.f);
+            for_pml_faces(S, d, [&](long long c, long long g) { comp[c] = (float)full[g]; });
+            if (S.n_pml_faces[d])
+                FDTD_HIP_TRY(hipMemcpy(h->D.dev.vel[vb][d], comp.data(), S.n_pml_faces[d] * 4, hipMemcpyHostToDevice));
+        }
+    }
+    // clock and amplitudes of step s; the series ring starts empty at s
+    const unsigned long long clk[CLK_WORDS] = {(unsigned long long)s, (unsigned long long)LLONG_MAX, 0ULL, 0ULL};
+    FDTD_HIP_TRY(hipMemcpy(h->D.dev.clock, clk, sizeof clk, hipMemcpyHostToDevice));
+    std::vector<double> acc(S.objs.size());
+    std::vector<float> src(S.psrc.size());
+    step_amplitudes(S, s, acc.data(), src.data());
+    if (!acc.empty()) FDTD_HIP_TRY(hipMemcpy(h->D.dev.acc_now, acc.data(), acc.size() * 8, hipMemcpyHostToDevice));
+    if (!src.empty()) FDTD_HIP_TRY(hipMemcpy(h->D.dev.src_now, src.data(), src.size() * 4, hipMemcpyHostToDevice));
+    h->step = s;
+    h->flushed = s;
+    h->host_series.assign((size_t)s * S.L, 0.0);
     return FDTD_OK;
 }
 

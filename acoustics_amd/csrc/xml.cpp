@@ -155,6 +155,7 @@ struct fdtd_parsed {
     std::vector<fdtd_pressure_source> psrc;
     std::vector<double> listeners;
     int num_steps = 0;
+    std::string output_pattern = "%s";
 };
 
 extern "C" {
@@ -184,6 +185,14 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
         s.cell_size = req_real(solver, "cellsize");
         s.stop_time = req_real(solver, "stop_time");
         P->num_steps = opt_int(solver, "number_of_timesteps", 0);
+        // output_pattern is required by the reference (ImpulseResponseParser.cpp:521);
+        // here a missing one means "files in the working directory"
+        P->output_pattern = opt_str(solver, "output_pattern", "%s");
+        // not a reference key: the reference picks its scheme at compile time
+        const std::string scheme = opt_str(solver, "scheme", "collocated");
+        if (scheme == "collocated") s.scheme = FDTD_SCHEME_COLLOCATED;
+        else if (scheme == "staggered") s.scheme = FDTD_SCHEME_STAGGERED;
+        else throw ParseError{"scheme must be collocated or staggered: " + scheme};
         s.time_step = 1.0 / req_real(solver, "timestepfrequency");
         s.pml_width = req_real(solver, "PML_width");
         s.pml_strength = req_real(solver, "PML_strength");
@@ -295,6 +304,7 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
 
 const fdtd_scene *fdtd_parsed_scene(const fdtd_parsed *p) { return p ? &p->scene : nullptr; }
 int fdtd_parsed_num_steps(const fdtd_parsed *p) { return p ? p->num_steps : 0; }
+const char *fdtd_parsed_output_pattern(const fdtd_parsed *p) { return p ? p->output_pattern.c_str() : "%s"; }
 void fdtd_parsed_free(fdtd_parsed *p) { delete p; }
 
 fdtd_status fdtd_create_from_xml(const char *path, const fdtd_options *opt, fdtd_handle **out)

@@ -80,6 +80,14 @@ class Solver:
             return out.reshape(self.sb.divisions)
         return out
 
+    def save_snapshot(self, pattern, simulator_id=None, numbering=0):
+        capi.check(self.lib.fdtd_save_snapshot(self.h, pattern.encode(), simulator_id.encode() if simulator_id else None,
+                                               int(numbering)), "fdtd_save_snapshot")
+
+    def load_snapshot(self, pattern, simulator_id=None, numbering=0):
+        capi.check(self.lib.fdtd_load_snapshot(self.h, pattern.encode(), simulator_id.encode() if simulator_id else None,
+                                               int(numbering)), "fdtd_load_snapshot")
+
     def set_timing(self, on=True):
         capi.check(self.lib.fdtd_set_timing(self.h, 1 if on else 0), "fdtd_set_timing")
 

@@ -164,6 +164,8 @@ typedef struct fdtd_parsed fdtd_parsed;
 fdtd_status        fdtd_parse_xml(const char *path, fdtd_parsed **out);
 const fdtd_scene  *fdtd_parsed_scene(const fdtd_parsed *p);
 int                fdtd_parsed_num_steps(const fdtd_parsed *p); /* number_of_timesteps */
+/* solver output_pattern (ImpulseResponseParser.cpp:521), "%s" when absent. */
+const char        *fdtd_parsed_output_pattern(const fdtd_parsed *p);
 void               fdtd_parsed_free(fdtd_parsed *p);
 /* Error text of the last failed call on this thread (parse errors name the
  * element and attribute). */
@@ -272,6 +274,40 @@ fdtd_status fdtd_server_cancel(fdtd_server *s, int64_t request_id);
 fdtd_status fdtd_server_get_stats(fdtd_server *s, fdtd_server_stats *out);
 /* Bytes of device state a scene needs when admitted (used by admission). */
 int64_t     fdtd_scene_device_bytes(const fdtd_scene *scene);
+/* ------------------------------------------------------------------------- */
+/* the reference tool's files                                                */
+/* ------------------------------------------------------------------------- */
+/* Output name as FDTD_AcousticSimulator::CompositeFilename
+ * (FDTD_AcousticSimulator.cpp:110-124): pattern's %s replaced by
+ * "<simulator_id>_<name>" (just <name> when simulator_id is NULL or empty). */
+fdtd_status fdtd_composite_filename(const char *pattern, const char *simulator_id, const char *name,
+                                    char *out, int capacity);
+/* Binary matrix as IO::writeMatrixX(..., BINARY) (src/utils/IO/IO.h:107-139):
+ * int32 rows, int32 cols, then rows*cols doubles row by row; the reader is
+ * scripts/binary_io.py:readMatrixXdBinary. */
+fdtd_status fdtd_write_matrix(const char *filename, const double *data, int rows, int cols);
+/* Runs an XML scene the way the reference's tool does
+ * (FDTD_AcousticSimulator::InitializeSolver / SaveSolverConfig / Run and
+ * SimWorld's audio output): writes solver_settings.txt, a copy of the control
+ * file (input_control_file.xml) and listening_position.dat under the scene's
+ * output_pattern, steps until stop_time (or number_of_timesteps), and writes
+ * all_audio.dat as one row of L doubles per step (AudioOutput.cpp:55-64,
+ * read by scripts/wavesolver_results.py). */
+fdtd_status fdtd_run_xml(const char *xml_path, const char *simulator_id, const fdtd_options *opt, int64_t *steps_run);
+/* Snapshot of the time-stepping state in the file set of
+ * FDTD_AcousticSimulator::_SaveSimulationSnapshot (FDTD_AcousticSimulator.cpp:
+ * 381-415), each file a MATRIX::write matrix (int32 rows, int32 cols, doubles):
+ * snapshot_<n>_p_collocated_{0,1,2}.dat, _p_collocated_ind.dat (text),
+ * _p_pml_{0,1,2}.dat (split pressures, full grid), _v_pml_{0,1,2}.dat (face
+ * velocities, (n_d+1) along axis d), _p_pml_full.dat, plus _step.txt with the
+ * step index.  Names follow fdtd_composite_filename. */
+fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering);
+/* Restores a snapshot written by fdtd_save_snapshot into a handle created
+ * for the same scene (the reference declares LoadSolverResult,
+ * FDTD_AcousticSimulator.h:108, without a definition).  Stepping continues
+ * from the saved step bit for bit; listener rows before it read as zero. */
+fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering);
+
 /* Host-only dry run of the device planner (no counterpart in the reference,
  * which has no tiling): builds the scene's tiles for tile_config (-1: default)
  * and the launch lists of its step, and checks on the host that every tile's

@@ -1,0 +1,81 @@
+"""GPU checks of the reference tool's files and of checkpoint/resume.
+
+fdtd_run_xml runs an XML scene as the reference's tool does
+(FDTD_AcousticSimulator::SaveSolverConfig / Run, SimWorld's audio output):
+listening_position.dat must read back through scripts/binary_io.py's format,
+all_audio.dat must hold one row of L doubles per step equal (bit for bit) to
+the series of the same scene stepped through the single-scene API, and the
+control file must be copied unchanged.  Snapshots in the layout of
+FDTD_AcousticSimulator::_SaveSimulationSnapshot must resume a run bit for bit,
+in both schemes."""
+import os
+
+import numpy as np
+import pytest
+
+from acoustics_amd import capi, scenes
+from acoustics_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def read_matrix_xd_binary(path):
+    """scripts/binary_io.py:readMatrixXdBinary: int32 rows, int32 cols, doubles in row order."""
+    with open(path, "rb") as f:
+        n1 = int(np.fromfile(f, dtype=np.int32, count=1)[0])
+        n2 = int(np.fromfile(f, dtype=np.int32, count=1)[0])
+        data = np.fromfile(f, dtype=np.float64, count=n1 * n2)
+    return data.reshape(n1, n2)
+
+
+def test_run_xml_writes_reference_files(hip_lib, tmp_path):
+    import ctypes as C
+    src = os.path.join(ROOT, "tests", "golden", "scene_sphere.xml")
+    text = open(src).read().replace('output_pattern="out_%s"', f'output_pattern="{tmp_path}/run_%s"')
+    xml = tmp_path / "scene.xml"
+    xml.write_text(text)
+    print(f"case: run_xml {xml}", flush=True)
+    n = C.c_int64(0)
+    assert hip_lib.fdtd_run_xml(str(xml).encode(), b"7", None, C.byref(n)) == 0, hip_lib.fdtd_last_error()
+    assert n.value == 300
+    lp = read_matrix_xd_binary(str(tmp_path / "run_7_listening_position.dat"))
+    assert np.allclose(lp, [[0.05, 0.01, -0.02], [-0.04, 0.03, 0.01]])
+    audio = np.fromfile(str(tmp_path / "run_all_audio.dat"), dtype=np.float64).reshape(-1, 2)
+    assert audio.shape == (300, 2)
+    with Solver.from_xml(str(xml)) as s:
+        s.step(300)
+        ref = s.series()
+    assert np.array_equal(audio, ref)
+    assert open(tmp_path / "run_7_input_control_file.xml").read() == text
+    assert "steps 300" in open(tmp_path / "run_7_solver_settings.txt").read()
+
+
+@pytest.mark.parametrize("kind", ["collocated_sphere", "staggered_free_field"])
+def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
+    if kind == "collocated_sphere":
+        sb, _ = scenes.config2(seed=21, n=40, radius=0.025, shell=0.06, pml=8, n_listeners=8)
+    else:
+        sb, _ = scenes.staggered_free_field(seed=9, div=(36, 32, 40), pml=6)
+    print(f"case: snapshot {kind} divisions={sb.divisions}", flush=True)
+    pat = str(tmp_path / "snap_%s")
+    with Solver(sb) as a:
+        a.step(70)
+        a.save_snapshot(pat, "0", 2)
+        a.step(50)
+        ser_a = a.series()
+        p_a = a.pressure()
+    # the files follow the reference's MATRIX::write layout
+    N = int(np.prod(sb.divisions))
+    assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_p_pml_0.dat")).shape == (N, 1)
+    nx, ny, nz = sb.divisions
+    assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_v_pml_1.dat")).shape == (nx * (ny + 1) * nz, 1)
+    with Solver(sb) as b:
+        b.load_snapshot(pat, "0", 2)
+        b.step(50)
+        ser_b = b.series()
+        p_b = b.pressure()
+    assert np.array_equal(ser_b[70:], ser_a[70:])
+    assert np.array_equal(p_b, p_a)
+    assert np.all(ser_b[:70] == 0)
