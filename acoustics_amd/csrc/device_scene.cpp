@@ -133,11 +133,12 @@ void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
 
 TilePlan tile_plan(int id)
 {
-    // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior, one launch}
+    // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior, one launch,
+    //  PML k threads (log2)}
     static const TilePlan plans[kNumTilePlans] = {
-        {6, 4, 8, true, true},   {6, 4, 4, true, true},   {6, 8, 8, true, true},   {6, 4, 8, true, false},
-        {6, 8, 8, false, false}, {6, 8, 8, true, false},  {6, 4, 4, true, false},  {6, 2, 8, true, false},
-        {6, 4, 4, false, false}, {6, 16, 16, false, false},
+        {6, 2, 8, true, true, 6},   {6, 4, 4, true, true, 6},   {6, 8, 8, true, true, 6},   {6, 4, 8, true, false, 6},
+        {6, 8, 8, false, false, 6}, {6, 4, 8, true, true, 5},   {6, 4, 16, true, true, 6},  {6, 4, 8, true, true, 6},
+        {6, 4, 4, false, false, 6}, {6, 4, 8, true, true, 4},
     };
     return plans[(id >= 0 && id < kNumTilePlans) ? id : 0];
 }
@@ -350,7 +351,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     // PML slabs first (heaviest tiles start first)
     for (int q = 0; q < (int)S.slabs.size(); ++q) {
         const PmlSlab &sl = S.slabs[q];
-        const int tk = std::min(6, pow2_ceil_log2(sl.e[2]));
+        const int tk = std::min(plan.pml_tk_log2, pow2_ceil_log2(sl.e[2]));
         const int TK = 1 << tk, TJ = 256 >> tk;
         for (int i0 = sl.lo[0]; i0 < sl.lo[0] + sl.e[0]; i0 += plan.pml_ti)
             for (int j0 = sl.lo[1]; j0 < sl.lo[1] + sl.e[1]; j0 += TJ)

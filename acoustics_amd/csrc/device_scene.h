@@ -28,6 +28,7 @@ struct TilePlan {
     int pml_ti = 16;           // planes marched per PML tile
     bool interior_vec = false; // whole-row float4 interior tiles where the layout allows
     bool fused = false;        // one launch for all tile classes (PML tiles first)
+    int pml_tk_log2 = 6;       // PML slab columns: up to 1 << pml_tk_log2 along k, 256 >> it along j
 };
 constexpr int kNumTilePlans = 10;
 TilePlan tile_plan(int id);
