@@ -19,6 +19,8 @@ FDTD_BOUNDARY_RASTERIZE = 0
 FDTD_BOUNDARY_FULLY_COUPLED = 1
 FDTD_SCHEME_COLLOCATED = 0
 FDTD_SCHEME_STAGGERED = 1
+FDTD_PML_STATE_MERGED = 0
+FDTD_PML_STATE_SPLIT = 1
 FDTD_OBJ_SPHERE = 0
 FDTD_OBJ_ELLIPSOID = 1
 
@@ -69,13 +71,14 @@ class Scene(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("stream", C.c_void_p), ("device", C.c_int), ("steps_per_graph", C.c_int),
-                ("series_capacity", C.c_int), ("tile_config", C.c_int)]
+                ("series_capacity", C.c_int), ("tile_config", C.c_int), ("pml_state", C.c_int)]
 
 
 class Info(C.Structure):
     _fields_ = [("n_cells", C.c_int64), ("n_pml_cells", C.c_int64), ("n_ghost_cells", C.c_int64),
                 ("n_solid_cells", C.c_int64), ("n_listeners", C.c_int64), ("step_index", C.c_int64),
-                ("time", C.c_double), ("bytes_per_step", C.c_double), ("device_bytes", C.c_int64)]
+                ("time", C.c_double), ("bytes_per_step", C.c_double), ("device_bytes", C.c_int64),
+                ("n_pml_merged_cells", C.c_int64), ("state_bytes_per_step", C.c_double)]
 
 
 class BatchOptions(C.Structure):

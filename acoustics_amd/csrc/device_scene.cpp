@@ -202,7 +202,8 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         flags[d].resize(n + 1);
         for (int i = 0; i <= n; ++i) {
             coef[d][i] = make_float4(S.f1[d][i], S.f2[d][i], i < n ? S.f3[d][i] : 0.f, i < n ? S.f5h[d][i] : 0.f);
-            flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0) | (i < n && S.sig0[d][i] ? 4 : 0);
+            flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0) | (i < n && S.sig0[d][i] ? 4 : 0) |
+                          (axis_plain(S, d, i) ? 8 : 0);
         }
         FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
         FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size() * sizeof(int)));

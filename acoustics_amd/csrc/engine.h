@@ -84,8 +84,26 @@ struct ListenerTerm {
 // low side of local cell li; the extra layer holds the high faces of the last
 // layer of cells.  A face shared by two boxes is stored in both and given the
 // same update by each.
+//
+// Merged transverse state (merge = 1, collocated scheme only): a cell of the
+// box that is plain along both axes other than the box axis (no damping on
+// those axes: f1 = f3 = 1 on its faces and at the cell, no wall faces) keeps
+// the sum q of its two transverse split pressures and its last increment
+// d^n = q^n - q^{n-1} instead of the two splits and their four face
+// velocities.  With f1 = f3 = 1 the split updates give exactly
+//   d^{n+1} = d^n + sum_{b transverse} f5h_b (f2_b(R) (p_R - p) - f2_b(L) (p - p_L)),
+//   q^{n+1} = q^n + d^{n+1}
+// (d is f5h times the transverse velocity divergence).  Keeping d rather
+// than q^{n-1} integrates rounding errors once, as the split form does; the
+// form 2 q^n - q^{n-1} would integrate them twice and let them drift.  q
+// lives in the split array of the lower transverse axis, d in the higher
+// one's.  The box-axis split and its faces are kept as before.  Transverse
+// faces of a merged cell are updated only where a non-merged cell of the same
+// box reads them.
 struct PmlSlab {
     int32_t lo[3], e[3];
+    int32_t axis;                // the box's damped axis (x slabs 0, y slabs 1, z slabs 2)
+    int32_t merge;               // 1: merged transverse state for eligible cells
     int64_t cell_base;
     int64_t v_base[3];
 };
@@ -107,6 +125,7 @@ struct SceneSetup {
     int64_t n_pml = 0;
     std::vector<PmlSlab> slabs;        // the PML as up to six disjoint boxes
     int64_t n_pml_faces[3] = {0, 0, 0};  // compact face-velocity entries per axis
+    int64_t n_pml_merged = 0;           // PML cells holding merged transverse state
     // sources
     std::vector<PSrcHost> psrc;
     std::vector<float> gtab;
@@ -134,9 +153,20 @@ struct SceneSetup {
 
 // Builds the full setup; returns FDTD_OK or an error with a message in err.
 fdtd_status build_setup(const fdtd_scene &scene, SceneSetup &out, std::string &err);
+// Axis d is plain at cell i: no damping on the cell or its two faces
+// (f3 = f1 = 1 in the FP32 tables) and neither face is a wall.
+bool axis_plain(const SceneSetup &s, int d, int i);
+// Chooses the PML state representation: merged (the default of build_setup
+// for the collocated scheme) or the reference's per-axis split state, which
+// snapshots need.  Sets PmlSlab::merge and n_pml_merged.
+void set_pml_state(SceneSetup &s, bool merged);
 // Algorithmic bytes per step (FP32 fields): collocated 12 N + 44 N_pml;
 // staggered 32 N + 24 N_pml.
 double algorithmic_bytes(const SceneSetup &s);
+// Bytes per step of this engine's state representation: a merged PML cell
+// moves 40 B instead of 56 (it drops two splits and four face velocities,
+// read and written, and reads and writes q and d).
+double state_bytes(const SceneSetup &s);
 
 // thread-local error text for fdtd_last_error
 void set_last_error(const std::string &msg);

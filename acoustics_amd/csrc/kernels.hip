@@ -201,12 +201,30 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
 
 // ---------------------------------------------------------------------------
 // PML slab: fused staggered velocity + split pressure update
+// (MAC_Grid::PML_velocityUpdateCollocated, MAC_Grid.cpp:383-459, and the PML
+// branch of PML_pressureUpdateCollocated, 496-508), with the merged
+// transverse state of PmlSlab for cells damped along the slab axis only.
 // ---------------------------------------------------------------------------
 // Face velocity update on one axis for a face whose old value is v, between
-// pressures pl (left) and pr (right); inactive faces stay at zero.
+// pressures pl (left) and pr (right); inactive faces stay at zero.  The fma
+// is explicit: a face is computed by the two cells that share it and both
+// must round it the same way.
 __device__ __forceinline__ float face_update(bool active, float4 c, float v, float pl, float pr)
 {
-    return active ? c.x * v + c.y * (pr - pl) : 0.f;
+    return active ? fmaf(c.y, pr - pl, c.x * v) : 0.f;
+}
+
+// Split pressure update pDir <- f3 pDir + f5/h (v_R - v_L).
+__device__ __forceinline__ float split_update(float4 c, float q, float dv)
+{
+    return fmaf(c.w, dv, c.z * q);
+}
+
+// One transverse axis of the merged update: f5h (f2_R (p_R - p) - f2_L (p - p_L)),
+// cL the cell's table entry (its low face and itself), cR the high face's.
+__device__ __forceinline__ float transverse_term(float4 cL, float4 cR, float pl, float pc, float pr)
+{
+    return cL.w * fmaf(cR.y, pr - pc, -(cL.y * (pc - pl)));
 }
 
 // Loads and stores at a 32-bit byte offset from a wavefront-uniform base, so
@@ -224,7 +242,13 @@ __device__ __forceinline__ void stb(AS_GLOBAL T *base, uint32_t off, T v)
     *(AS_GLOBAL T *)((AS_GLOBAL char *)base + off) = v;
 }
 
-__device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
+// A = the slab's damped axis.  Threads run along z (lanes) and y (rows) and
+// march along x, carrying the x face between planes.  A cell is merged when
+// the slab merges and both transverse axes are plain at it; in y and z slabs
+// x is plain over the whole slab, so merging is constant along a column.
+template <int A>
+__device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc &t, const PmlSlab &sl, const TermRef *terms,
+                                              long long step)
 {
     const int tk = t.tk_log2;
     const int tx = threadIdx.x & ((1 << tk) - 1), ty = threadIdx.x >> tk;
@@ -235,7 +259,6 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, c
     AS_GLOBAL float *__restrict__ pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     // compact storage of this slab: cells (and x faces) have plane stride
     // e1*e2; y faces (e1+1)*e2; z faces e1*(e2+1)
-    const PmlSlab sl = cmem(S.slabs)[t.slab];
     const int e0 = sl.e[0], e1 = sl.e[1], e2 = sl.e[2];
     const int li0 = t.i0 - sl.lo[0], lj = j - sl.lo[1], lk = k - sl.lo[2];
     const int vb = (int)(step & 1);
@@ -270,6 +293,20 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, c
     // the high y / z face of this column is stored here when it is the slab's last layer
     const bool own_yR = lj == e1 - 1, own_zR = lk == e2 - 1;
 
+    // merged or split; a split axis keeps its pDir and faces.  The low face of
+    // a merged cell is still updated when the cell below it in the slab is
+    // split on that axis (that cell reads it as its high face).
+    const bool mg = sl.merge != 0;
+    const bool py = A == 1 || (fyL & 8), pz = A == 2 || (fzL & 8);
+    const bool m = mg && py && pz;
+    const bool sx = A == 0 || !m, sy = A == 1 || !m, sz = A == 2 || !m;
+    const int fyM = lj >= 1 ? fy[j - 1] : 0, fzM = lk >= 1 ? fz[k - 1] : 0;
+    const bool need_yL = sy || (lj >= 1 && !(mg && (A == 1 || (fyM & 8)) && pz));
+    const bool need_zL = sz || (lk >= 1 && !(mg && py && (A == 2 || (fzM & 8))));
+    // merged state: q in the lower transverse axis's array, its increment d in the higher one's
+    AS_GLOBAL float *__restrict__ qarr = A == 0 ? pd1 : pd0;
+    AS_GLOBAL float *__restrict__ darr = A == 2 ? pd1 : pd2;
+
     // byte offsets
     const uint32_t sx4 = 4u * (uint32_t)S.sx, sy4 = 4u * (uint32_t)S.sy;
     const uint32_t cs4 = 4u * (uint32_t)(e1 * e2), ys4 = 4u * (uint32_t)((e1 + 1) * e2), zs4 = 4u * (uint32_t)(e1 * (e2 + 1));
@@ -285,47 +322,86 @@ __device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, c
     float pc = ldb(pcur, po);
     // the x face on the low side of the current cell (updated value), carried
     // from one plane to the next: cell i's high face is cell i+1's low face
-    float vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || ox), cx[t.i0], ldb(vi0, co), pm, pc);
+    float vxL = 0.f;
+    if (sx) vxL = face_update(t.i0 >= 1 && ((fx[t.i0] & 2) || ox), cx[t.i0], ldb(vi0, co), pm, pc);
     const int ilast = sl.lo[0] + e0 - 1;
     for (int i = t.i0; i < t.i1; ++i, po += sx4, co += cs4, yo += ys4, zo += zs4) {
         const bool last = i + 1 >= nx;
         const float pp = ldb(pcur, po + (last ? 0u : sx4));
         const float ym = ldb(pcur, po - ym4), yp = ldb(pcur, po + yp4);
         const float zm = ldb(pcur, po - zm4), zp = ldb(pcur, po + zp4);
-        const float vxr_old = ldb(vi0, co + cs4);
-        const float vyl_old = ldb(vi1, yo), vyr_old = ldb(vi1, yo + e24);
-        const float vzl_old = ldb(vi2, zo), vzr_old = ldb(vi2, zo + 4u);
-        const float q0 = ldb(pd0, co), q1 = ldb(pd1, co), q2 = ldb(pd2, co);
+        float vxr_old = 0.f, vyl_old = 0.f, vyr_old = 0.f, vzl_old = 0.f, vzr_old = 0.f;
+        float q0 = 0.f, q1 = 0.f, q2 = 0.f, qc = 0.f, qd = 0.f;
+        if (sx) { vxr_old = ldb(vi0, co + cs4); q0 = ldb(pd0, co); }
+        if (need_yL) vyl_old = ldb(vi1, yo);
+        if (sy) { vyr_old = ldb(vi1, yo + e24); q1 = ldb(pd1, co); }
+        if (need_zL) vzl_old = ldb(vi2, zo);
+        if (sz) { vzr_old = ldb(vi2, zo + 4u); q2 = ldb(pd2, co); }
+        if (m) { qc = ldb(qarr, co); qd = ldb(darr, co); }
         const int fxC = fx[i], fxR = last ? 0 : fx[i + 1];
         const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
         const bool ci = fxC & 1;
+        float n0 = 0.f, n1 = 0.f, n2 = 0.f;
         // x: high face (new), low face carried
-        const float vxR = last ? 0.f : face_update((fxR & 2) || ox, cxR, vxr_old, pc, pp);
-        stb(vo0, co, vxL);
-        if (i == ilast) stb(vo0, co + cs4, vxR);
-        const float n0 = cxC.z * q0 + cxC.w * (vxR - vxL);
+        if (sx) {
+            const float vxR = last ? 0.f : face_update((fxR & 2) || ox, cxR, vxr_old, pc, pp);
+            stb(vo0, co, vxL);
+            if (i == ilast) stb(vo0, co + cs4, vxR);
+            n0 = split_update(cxC, q0, vxR - vxL);
+            stb(pd0, co, n0);
+            vxL = vxR;
+        }
         // y
-        const float vyL = face_update(yl_on || (ci && yl_ci), cyL, vyl_old, ym, pc);
-        const float vyR = face_update(yr_on || (ci && yr_ci), cyR, vyr_old, pc, yp);
-        stb(vo1, yo, vyL);
-        if (own_yR) stb(vo1, yo + e24, vyR);
-        const float n1 = cyL.z * q1 + cyL.w * (vyR - vyL);
+        float vyL = 0.f;
+        if (need_yL) {
+            vyL = face_update(yl_on || (ci && yl_ci), cyL, vyl_old, ym, pc);
+            stb(vo1, yo, vyL);
+        }
+        if (sy) {
+            const float vyR = face_update(yr_on || (ci && yr_ci), cyR, vyr_old, pc, yp);
+            if (own_yR) stb(vo1, yo + e24, vyR);
+            n1 = split_update(cyL, q1, vyR - vyL);
+            stb(pd1, co, n1);
+        }
         // z
-        const float vzL = face_update(zl_on || (ci && zl_ci), czL, vzl_old, zm, pc);
-        const float vzR = face_update(zr_on || (ci && zr_ci), czR, vzr_old, pc, zp);
-        stb(vo2, zo, vzL);
-        if (own_zR) stb(vo2, zo + 4u, vzR);
-        const float n2 = czL.z * q2 + czL.w * (vzR - vzL);
-        stb(pd0, co, n0);
-        stb(pd1, co, n1);
-        stb(pd2, co, n2);
-        const float pn = n0 + n1 + n2;
+        float vzL = 0.f;
+        if (need_zL) {
+            vzL = face_update(zl_on || (ci && zl_ci), czL, vzl_old, zm, pc);
+            stb(vo2, zo, vzL);
+        }
+        if (sz) {
+            const float vzR = face_update(zr_on || (ci && zr_ci), czR, vzr_old, pc, zp);
+            if (own_zR) stb(vo2, zo + 4u, vzR);
+            n2 = split_update(czL, q2, vzR - vzL);
+            stb(pd2, co, n2);
+        }
+        float pn;
+        if (m) {
+            float lap = 0.f;
+            if constexpr (A != 0) lap += transverse_term(cxC, cxR, pm, pc, pp);
+            if constexpr (A != 1) lap += transverse_term(cyL, cyR, ym, pc, yp);
+            if constexpr (A != 2) lap += transverse_term(czL, czR, zm, pc, zp);
+            const float dn = qd + lap;
+            const float qn = qc + dn;
+            stb(darr, co, dn);
+            stb(qarr, co, qn);
+            pn = (A == 0 ? n0 : A == 1 ? n1 : n2) + qn;
+        } else {
+            pn = n0 + n1 + n2;
+        }
         stb(pnext, po, pn);
         if (t.te > t.tb) emit_cell_terms(S, t, terms, step, (int)(po >> 2), pn);
-        vxL = vxR;
         pm = pc;
         pc = pp;
     }
+}
+
+__device__ __forceinline__ void pml_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
+{
+    const PmlSlab sl = cmem(S.slabs)[t.slab];
+    if (sl.axis == 0) pml_tile_axis<0>(S, t, sl, terms, step);
+    else if (sl.axis == 1) pml_tile_axis<1>(S, t, sl, terms, step);
+    else pml_tile_axis<2>(S, t, sl, terms, step);
 }
 
 // ---------------------------------------------------------------------------

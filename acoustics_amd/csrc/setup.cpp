@@ -146,6 +146,38 @@ double algorithmic_bytes(const SceneSetup &s)
     return 12.0 * (double)s.N + 44.0 * (double)s.n_pml;
 }
 
+double state_bytes(const SceneSetup &s)
+{
+    return algorithmic_bytes(s) - 16.0 * (double)s.n_pml_merged;
+}
+
+bool axis_plain(const SceneSetup &s, int d, int i)
+{
+    const int n = s.n[d];
+    if (i < 1 || i > n - 2) return false;
+    return s.f3[d][i] == 1.0f && s.f1[d][i] == 1.0f && s.f1[d][i + 1] == 1.0f;
+}
+
+void set_pml_state(SceneSetup &s, bool merged)
+{
+    s.n_pml_merged = 0;
+    for (PmlSlab &sl : s.slabs) {
+        sl.merge = 0;
+        if (!merged || s.scheme != FDTD_SCHEME_COLLOCATED) continue;
+        // the march runs along x: y and z slabs merge only when x is plain
+        // over their whole x range, so a column is merged or split throughout
+        bool ok = true;
+        if (sl.axis != 0)
+            for (int i = sl.lo[0]; i < sl.lo[0] + sl.e[0] && ok; ++i) ok = axis_plain(s, 0, i);
+        if (!ok) continue;
+        sl.merge = 1;
+        int64_t ny = 0, nz = 0;
+        for (int j = sl.lo[1]; j < sl.lo[1] + sl.e[1]; ++j) ny += (sl.axis == 1 || axis_plain(s, 1, j));
+        for (int k = sl.lo[2]; k < sl.lo[2] + sl.e[2]; ++k) nz += (sl.axis == 2 || axis_plain(s, 2, k));
+        s.n_pml_merged += (int64_t)sl.e[0] * ny * nz;
+    }
+}
+
 fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
 {
     const fdtd_settings &st = sc.settings;
@@ -279,6 +311,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         for (const auto &bx : boxes) {
             PmlSlab sl{};
             for (int d = 0; d < 3; ++d) { sl.lo[d] = bx[2 * d]; sl.e[d] = bx[2 * d + 1] - bx[2 * d]; }
+            sl.axis = (int)(&bx - &boxes[0]) / 2;
             if (sl.e[0] <= 0 || sl.e[1] <= 0 || sl.e[2] <= 0) continue;
             sl.cell_base = cells;
             cells += (int64_t)sl.e[0] * sl.e[1] * sl.e[2];
@@ -290,6 +323,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         }
         if (cells != S.n_pml) { err = "internal: PML slab decomposition does not cover the PML"; return FDTD_ERR_INVALID; }
         for (int d = 0; d < 3; ++d) S.n_pml_faces[d] = faces[d];
+        set_pml_state(S, true);
     }
 
     // ---- pressure sources (GaussianPressureSource) ------------------------

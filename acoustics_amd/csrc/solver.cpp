@@ -81,6 +81,7 @@ void fdtd_options_default(fdtd_options *o)
     o->steps_per_graph = 0;
     o->series_capacity = 0;
     o->tile_config = -1;
+    o->pml_state = FDTD_PML_STATE_MERGED;
 }
 
 fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdtd_handle **out)
@@ -97,6 +98,11 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
     std::string err;
     fdtd_status st = build_setup(*scene, h->D.S, err);
     if (st != FDTD_OK) { set_last_error(err); return st; }
+    if (opt.pml_state != FDTD_PML_STATE_MERGED && opt.pml_state != FDTD_PML_STATE_SPLIT) {
+        set_last_error("pml_state must be FDTD_PML_STATE_MERGED or FDTD_PML_STATE_SPLIT");
+        return FDTD_ERR_INVALID;
+    }
+    set_pml_state(h->D.S, opt.pml_state == FDTD_PML_STATE_MERGED);
     if (opt.device >= 0) {
         if (opt.device >= ndev) { set_last_error("device ordinal out of range"); return FDTD_ERR_INVALID; }
         FDTD_HIP_TRY(hipSetDevice(opt.device));
@@ -316,6 +322,8 @@ fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *o)
     o->time = S.t_start + (double)h->step * S.dt;
     o->bytes_per_step = algorithmic_bytes(S);
     o->device_bytes = (int64_t)h->D.arena_bytes;
+    o->n_pml_merged_cells = S.n_pml_merged;
+    o->state_bytes_per_step = state_bytes(S);
     return FDTD_OK;
 }
 
@@ -390,6 +398,11 @@ extern "C" {
 fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.n_pml_merged > 0) {
+        set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
+                       "FDTD_PML_STATE_SPLIT");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const SceneSetup &S = h->D.S;
@@ -450,6 +463,11 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
 fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.n_pml_merged > 0) {
+        set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
+                       "FDTD_PML_STATE_SPLIT");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const SceneSetup &S = h->D.S;

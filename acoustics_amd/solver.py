@@ -15,7 +15,8 @@ from . import capi
 class Solver:
     """One scene on the device (fdtd_create ... fdtd_destroy)."""
 
-    def __init__(self, scene_buffers, steps_per_graph=0, tile_config=-1, series_capacity=0, stream=None):
+    def __init__(self, scene_buffers, steps_per_graph=0, tile_config=-1, series_capacity=0, stream=None,
+                 pml_state=capi.FDTD_PML_STATE_MERGED):
         self.lib = capi.load_library()
         self.sb = scene_buffers
         opt = capi.Options()
@@ -23,6 +24,7 @@ class Solver:
         opt.steps_per_graph = steps_per_graph
         opt.tile_config = tile_config
         opt.series_capacity = series_capacity
+        opt.pml_state = pml_state
         if stream is not None:
             opt.stream = stream
         h = C.c_void_p()
@@ -30,13 +32,14 @@ class Solver:
         self.h = h
 
     @classmethod
-    def from_xml(cls, path, steps_per_graph=0):
+    def from_xml(cls, path, steps_per_graph=0, pml_state=capi.FDTD_PML_STATE_MERGED):
         self = cls.__new__(cls)
         self.lib = capi.load_library()
         self.sb = None
         opt = capi.Options()
         self.lib.fdtd_options_default(C.byref(opt))
         opt.steps_per_graph = steps_per_graph
+        opt.pml_state = pml_state
         h = C.c_void_p()
         capi.check(self.lib.fdtd_create_from_xml(path.encode(), C.byref(opt), C.byref(h)), "fdtd_create_from_xml")
         self.h = h

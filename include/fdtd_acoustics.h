@@ -176,12 +176,29 @@ const char        *fdtd_last_error(void);
 /* ------------------------------------------------------------------------- */
 typedef struct fdtd_handle fdtd_handle;
 
+/* How a PML cell of the collocated scheme keeps its state.  SPLIT is the
+ * reference's state (MAC_Grid.cpp:383-508): three split pressures pDir_d and
+ * the face velocities of every PML cell.  MERGED (default) gives the same
+ * pressures up to FP32 rounding with fewer bytes: a cell damped along one
+ * axis only keeps that axis's split and faces plus the sum q of the two
+ * undamped splits and its increment d, since with f1 = f3 = 1 the split
+ * updates of those axes reduce exactly to
+ *   d^{n+1} = d^n + sum_b f5/h_b (f2_b(R) dp_R - f2_b(L) dp_L),
+ *   q^{n+1} = q^n + d^{n+1}
+ * (40 B per such cell and step instead of 56).  Snapshots need SPLIT (they
+ * write every pDir_d and velocity). */
+enum {
+    FDTD_PML_STATE_MERGED = 0,
+    FDTD_PML_STATE_SPLIT  = 1
+};
+
 typedef struct fdtd_options {
     void *stream;            /* hipStream_t to run on; NULL: handle-owned stream */
     int   device;            /* HIP device ordinal (-1: current)               */
     int   steps_per_graph;   /* >0: capture blocks of this many steps in a hipGraph */
     int   series_capacity;   /* device rows of the listener series (0: 1024)   */
     int   tile_config;       /* kernel tile id from the autotuner (-1: default) */
+    int   pml_state;         /* FDTD_PML_STATE_*: how PML cells keep their state   */
 } fdtd_options;
 
 void fdtd_options_default(fdtd_options *o);
@@ -196,6 +213,9 @@ typedef struct fdtd_info {
     double  time;              /* simulation time (s)                          */
     double  bytes_per_step;    /* algorithmic HBM bytes: 12 N + 44 N_pml      */
     int64_t device_bytes;      /* device memory held by the handle             */
+    int64_t n_pml_merged_cells;   /* PML cells in the merged state (pml_state)  */
+    double  state_bytes_per_step; /* bytes the state representation moves per
+                                     step: bytes_per_step - 16 per merged cell */
 } fdtd_info;
 
 fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt, fdtd_handle **out);

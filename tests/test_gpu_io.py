@@ -60,7 +60,8 @@ def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
         sb, _ = scenes.staggered_free_field(seed=9, div=(36, 32, 40), pml=6)
     print(f"case: snapshot {kind} divisions={sb.divisions}", flush=True)
     pat = str(tmp_path / "snap_%s")
-    with Solver(sb) as a:
+    split = capi.FDTD_PML_STATE_SPLIT
+    with Solver(sb, pml_state=split) as a:
         a.step(70)
         a.save_snapshot(pat, "0", 2)
         a.step(50)
@@ -71,7 +72,7 @@ def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
     assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_p_pml_0.dat")).shape == (N, 1)
     nx, ny, nz = sb.divisions
     assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_v_pml_1.dat")).shape == (nx * (ny + 1) * nz, 1)
-    with Solver(sb) as b:
+    with Solver(sb, pml_state=split) as b:
         b.load_snapshot(pat, "0", 2)
         b.step(50)
         ser_b = b.series()
@@ -79,3 +80,17 @@ def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
     assert np.array_equal(ser_b[70:], ser_a[70:])
     assert np.array_equal(p_b, p_a)
     assert np.all(ser_b[:70] == 0)
+
+
+def test_snapshot_needs_split_state(hip_lib, tmp_path):
+    """The merged PML state cannot produce the per-axis pDir_d and velocity
+    files, so a merged handle refuses to save or load a snapshot."""
+    sb, _ = scenes.config1(seed=5, n=32)
+    print(f"case: snapshot refused divisions={sb.divisions}", flush=True)
+    with Solver(sb) as a:
+        assert a.info()["n_pml_merged_cells"] > 0
+        a.step(3)
+        with pytest.raises(RuntimeError, match="FDTD_PML_STATE_SPLIT"):
+            a.save_snapshot(str(tmp_path / "s_%s"), "0", 0)
+        with pytest.raises(RuntimeError, match="FDTD_PML_STATE_SPLIT"):
+            a.load_snapshot(str(tmp_path / "s_%s"), "0", 0)
