@@ -21,6 +21,9 @@ FDTD_SCHEME_COLLOCATED = 0
 FDTD_SCHEME_STAGGERED = 1
 FDTD_PML_STATE_MERGED = 0
 FDTD_PML_STATE_SPLIT = 1
+FDTD_TIMING_OFF = 0
+FDTD_TIMING_PER_STEP = 1
+FDTD_TIMING_SPAN = 2
 FDTD_OBJ_SPHERE = 0
 FDTD_OBJ_ELLIPSOID = 1
 

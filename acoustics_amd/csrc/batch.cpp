@@ -48,7 +48,7 @@ fdtd_status BatchRunner::build_graph(hipStream_t st)
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     fdtd_status s = FDTD_OK;
-    for (int k = 0; k < graph_steps && s == FDTD_OK; ++k) s = prog.enqueue(st, k);
+    s = prog.enqueue(st, 0, graph_steps);
     hipError_t e = hipStreamEndCapture(st, &g);
     if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
@@ -74,8 +74,7 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
         if (s != FDTD_OK) return s;
         if ((s = prog.enqueue(st, 0)) != FDTD_OK) return s;
         FDTD_HIP_TRY(hipEventRecord(a, st));
-        for (int r = 1; r <= 3; ++r)
-            if ((s = prog.enqueue(st, r)) != FDTD_OK) return s;
+        if ((s = prog.enqueue(st, 1, 3)) != FDTD_OK) return s;
         FDTD_HIP_TRY(hipEventRecord(b, st));
         FDTD_HIP_TRY(hipEventSynchronize(b));
         float ms = 0;

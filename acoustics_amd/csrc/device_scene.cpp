@@ -46,8 +46,9 @@ struct Layout {
     double *gk_w, *gu_w, *gval0, *gval1, *gtmp;
     double *terms;
     unsigned long long *clock;
-    double *acc_now;
-    float *src_now;
+    double *acc_tab;
+    float *src_tab;
+    SceneDev *self;
 };
 
 long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
@@ -100,36 +101,12 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.gtmp = c.take<double>(2 * S.ghosts.size());
     L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
-    L.acc_now = c.take<double>(S.objs.size());
-    L.src_now = c.take<float>(S.psrc.size());
+    L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
+    L.src_tab = c.take<float>((size_t)AMP_ROWS * S.psrc.size());
+    L.self = c.take<SceneDev>(1);
 }
 
 }  // namespace
-
-void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src)
-{
-    const double tn = S.t_start + (double)s * S.dt;
-    for (size_t o = 0; o < S.objs.size(); ++o) {
-        double a = 0.0;
-        for (const ImpulseHost &q : S.isrc) {
-            if (q.obj != (int)o) continue;
-            const double u = tn - q.ton;
-            a += q.A * std::exp(-(u * u) / (2.0 * q.w * q.w));
-        }
-        acc[o] = a;
-    }
-    for (size_t q = 0; q < S.psrc.size(); ++q) {
-        const PSrcHost &p = S.psrc[q];
-        const double t = tn + 0.5 * S.dt;
-        if (S.scheme == FDTD_SCHEME_STAGGERED) {
-            const double s2 = std::sqrt(2.0) * p.wt;
-            src[q] = (float)((std::erf((t - p.t0) / s2) - std::erf(-p.t0 / s2)) * p.amp * S.dt);
-        } else {
-            const double u = (t - p.t0) / p.wt;
-            src[q] = (float)(std::exp(-(u * u) / 2.0) * p.amp * S.dt);
-        }
-    }
-}
 
 TilePlan tile_plan(int id)
 {
@@ -156,12 +133,8 @@ fdtd_status DeviceScene::reset_to_start(hipStream_t st)
     FDTD_HIP_TRY(zero_state(st));
     FDTD_HIP_TRY(hipMemsetAsync(dev.terms, 0, sizeof(double) * (size_t)series_cap * std::max<size_t>(1, S.lst_terms.size()), st));
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
-    std::vector<double> acc0(S.objs.size());
-    std::vector<float> src0(S.psrc.size());
-    step_amplitudes(S, 0, acc0.data(), src0.data());
     FDTD_HIP_TRY(hipMemcpyAsync(dev.clock, clk, sizeof clk, hipMemcpyHostToDevice, st));
-    if (!acc0.empty()) FDTD_HIP_TRY(hipMemcpyAsync(dev.acc_now, acc0.data(), acc0.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    if (!src0.empty()) FDTD_HIP_TRY(hipMemcpyAsync(dev.src_now, src0.data(), src0.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    FDTD_HIP_TRY(fill_amplitudes(st));
     if (!S.ghosts.empty()) {
         FDTD_HIP_TRY(hipMemsetAsync(dev.gval[0], 0, S.ghosts.size() * 8, st));
         FDTD_HIP_TRY(hipMemsetAsync(dev.gval[1], 0, S.ghosts.size() * 8, st));
@@ -244,11 +217,6 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     }
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
-    std::vector<double> acc0(S.objs.size());
-    std::vector<float> src0(S.psrc.size());
-    step_amplitudes(S, 0, acc0.data(), src0.data());
-    FDTD_HIP_TRY(up(L.acc_now, acc0.data(), acc0.size() * sizeof(double)));
-    FDTD_HIP_TRY(up(L.src_now, src0.data(), src0.size() * sizeof(float)));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
 
     SceneDev &D = dev;
@@ -292,8 +260,12 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.terms = L.terms;
     D.clock = L.clock;
     D.n_obj = (int)S.objs.size();
-    D.acc_now = L.acc_now;
-    D.src_now = L.src_now;
+    D.acc_tab = L.acc_tab;
+    D.src_tab = L.src_tab;
+    self = L.self;
+    FDTD_HIP_TRY(hipMemcpyAsync(self, &D, sizeof D, hipMemcpyHostToDevice, st));
+    FDTD_HIP_TRY(fill_amplitudes(st));
+    FDTD_HIP_TRY(hipStreamSynchronize(st));
     return FDTD_OK;
 }
 
@@ -627,13 +599,13 @@ void StepProgram::release()
     fused = false;
 }
 
-fdtd_status StepProgram::enqueue(hipStream_t st, long long step, hipEvent_t ev_begin, hipEvent_t ev_end) const
+fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
-    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, st));
+    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {
-        if (fused) return kind == KIND_INTERIOR ? launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, s) : hipSuccess;
-        return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, s);
+        if (fused) return kind == KIND_INTERIOR ? launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s) : hipSuccess;
+        return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
     };
     const bool split = concurrent && side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
     if (split) {
@@ -656,7 +628,21 @@ fdtd_status StepProgram::enqueue(hipStream_t st, long long step, hipEvent_t ev_b
         FDTD_HIP_TRY(launch(KIND_SPECIAL, st));
     }
     if (ev_end) FDTD_HIP_TRY(hipEventRecord(ev_end, st));
-    FDTD_HIP_TRY(launch_advance(d_scenes, nscenes, st));
+    return FDTD_OK;
+}
+
+fdtd_status StepProgram::enqueue(hipStream_t st, long long step, int n, const hipEvent_t *ev) const
+{
+    for (int done = 0; done < n;) {
+        const int blk = std::min(n - done, (int)AMP_ROWS);
+        for (int k = 0; k < blk; ++k) {
+            const int q = done + k;
+            fdtd_status s = enqueue_step(st, step + q, k, ev ? ev[2 * q] : nullptr, ev ? ev[2 * q + 1] : nullptr);
+            if (s != FDTD_OK) return s;
+        }
+        FDTD_HIP_TRY(launch_advance(d_scenes, nscenes, blk, st));
+        done += blk;
+    }
     return FDTD_OK;
 }
 

@@ -40,6 +40,7 @@ struct DeviceScene {
     char *arena = nullptr;
     size_t arena_bytes = 0;
     SceneDev dev{};            // host copy of the scene block (pointers into the arena)
+    SceneDev *self = nullptr;  // device copy of dev (in the arena), for the scene's own launches
     long long npad = 0;
     size_t state_floats = 0;   // floats of time-stepping state from dev.field on
     int series_cap = 0;
@@ -57,6 +58,8 @@ struct DeviceScene {
     // puts the scene back at step 0: fields at rest, empty series ring, clock
     // and per-step amplitudes of step 0 (synchronises st)
     fdtd_status reset_to_start(hipStream_t st);
+    // refills the amplitude ring for the AMP_ROWS steps from the device clock
+    hipError_t fill_amplitudes(hipStream_t st) const { return launch_advance(self, 1, 0, st); }
 
     // Appends this scene's tiles, tagged with scene_id, and the listener
     // terms they carry (TileDesc::tb/te index `terms`).
@@ -79,9 +82,10 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
 std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
                                const std::vector<SceneDev> &scenes, size_t nterms);
 
-// The per-step launch sequence of a set of scenes, uploaded once:
-//   ghost solve (fully-coupled scenes) -> the step kernels, which also write
-//   the listener terms -> step advance (one workgroup per scene).
+// The launch sequence of a set of scenes, uploaded once.  Per step: ghost
+// solve (fully-coupled scenes) -> the step kernels, which also write the
+// listener terms; per block of up to AMP_ROWS steps, one step advance (one
+// workgroup per scene).
 // The step kernels write disjoint cells.  By default they run back to back
 // on one stream in an order that alternates with the step parity (object
 // cells, interior, PML on even steps; the reverse on odd steps), and the
@@ -108,10 +112,15 @@ struct StepProgram {
                        const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
                        bool fused_launch = false);
     void release();
-    // enqueues one step; `step` (the step it will run, or any value of the
-    // same parity) picks the kernel order.  ev_begin/ev_end bracket the
-    // three step kernels on st.
-    fdtd_status enqueue(hipStream_t st, long long step, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) const;
+    // Enqueues n steps from `step` (the host's count of the step the first
+    // runs; its parity picks the kernel order of each step): the step
+    // launches at offsets k = 0, 1, ... from the scenes' device counters, and
+    // one advance after every AMP_ROWS steps and after the last.  ev, when
+    // given, holds 2n events; ev[2q], ev[2q+1] bracket step q's kernels.
+    fdtd_status enqueue(hipStream_t st, long long step, int n = 1, const hipEvent_t *ev = nullptr) const;
+    // one step's launches at block offset k, without the advance
+    fdtd_status enqueue_step(hipStream_t st, long long step, int k, hipEvent_t ev_begin = nullptr,
+                             hipEvent_t ev_end = nullptr) const;
 };
 
 }  // namespace fdtd

@@ -45,6 +45,14 @@ __host__ __device__ inline int slot_next(long long s) { return (int)(s & 1); }
 // clock words of a scene
 enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_WORDS = 4 };
 
+// Steps are launched in blocks: the kernels of the k-th step of a block run
+// step clock[CLK_STEP] + k, and one advance kernel after the block moves the
+// counter on.  The per-step amplitudes live in a ring of AMP_ROWS rows (step
+// s in row s & (AMP_ROWS - 1)) that the advance kernel refills for the next
+// AMP_ROWS steps, so a block is at most AMP_ROWS steps long.
+enum : int { AMP_ROWS = 64 };
+__host__ __device__ inline int amp_row(long long s) { return (int)(s & (AMP_ROWS - 1)); }
+
 struct SceneDev {
     int n[3];
     int sx, sy;                 // strides of i and j (k fastest)
@@ -81,10 +89,9 @@ struct SceneDev {
     double *terms;              // [cap][T]
     unsigned long long *clock;  // CLK_WORDS words
     int n_obj;
-    // time-dependent amplitudes of the current step, refreshed by the
-    // advance kernel when it moves the step counter on:
-    double *acc_now;            // [n_obj] normal acceleration a_n(obj, t_n)
-    float *src_now;             // [n_psrc] S(t_n + dt/2) dt without the spatial factor
+    // time-dependent amplitudes, one ring row per step (amp_row):
+    double *acc_tab;            // [AMP_ROWS][n_obj] normal acceleration a_n(obj, t_s)
+    float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
 };
 
 // A listener term handled by the tile that produces its sample: kind 0 is
@@ -96,9 +103,6 @@ struct TermRef {
     int scene, slot, kind, cell;
     double beta;
 };
-
-// Amplitudes of step s on the host (used to seed step 0).
-void step_amplitudes(const SceneSetup &S, long long s, double *acc, float *src);
 
 enum : int {
     TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4, TILE_INTERIOR_VEC = 5,
@@ -125,13 +129,15 @@ struct GhostDesc {
 // the tile type); the other kinds carry one class each.
 enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3, KIND_ALL = 3 };
 // order_clock: when non-null, the step word of a scene clock whose parity
-// selects the tile order -- odd steps walk each XCD's chunk of the tile list
-// backwards, so a step starts on the data the previous step touched last and
-// the Infinity Cache still holds.
+// (plus k) selects the tile order -- odd steps walk each XCD's chunk of the
+// tile list backwards, so a step starts on the data the previous step touched
+// last and the Infinity Cache still holds.  k: the step's offset in its block.
 hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
-                       const unsigned long long *order_clock, hipStream_t s);
-// Moves every scene of the launch to its next step (amplitudes, then counter).
-hipError_t launch_advance(const SceneDev *scenes, int nscenes, hipStream_t s);
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s);
+                       const unsigned long long *order_clock, int k, hipStream_t s);
+// Moves every scene of the launch n steps on (clamped to its target) and
+// refills its amplitude ring for the AMP_ROWS steps from there; n = 0 only
+// fills the ring.
+hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s);
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s);
 
 }  // namespace fdtd

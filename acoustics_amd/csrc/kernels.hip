@@ -90,7 +90,7 @@ __device__ __forceinline__ void emit_ghost_terms(const SceneDev &S, const TileDe
         double v;
         if (S.raster) {
             const GhostDev gd = gmem(S.ghosts)[r.cell];
-            v = (double)pcur[gd.nbr] + gd.k_raster * gmem(S.acc_now)[gd.obj];
+            v = (double)pcur[gd.nbr] + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
         } else {
             v = gmem(S.gval[step & 1])[r.cell];
         }
@@ -141,7 +141,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
             for (int s = 0; s < S.n_psrc; ++s) {
                 const PSrcDev ps = cmem(S.psrc)[s];
                 const AS_CONST float *g = cmem(S.gtab);
-                pn += gmem(S.src_now)[s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
+                pn += gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
             }
         if (!skip_check || !gmem(S.skip)[idx]) {
             pnext[idx] = pn;
@@ -484,7 +484,7 @@ __device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, c
             for (int s = 0; s < S.n_psrc; ++s) {
                 const PSrcDev ps = cmem(S.psrc)[s];
                 const AS_CONST float *g = cmem(S.gtab);
-                src += gmem(S.src_now)[s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
+                src += gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
             }
         float pn;
         if constexpr (PML) {
@@ -529,7 +529,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         if (gid >= 0) {
             if (S.raster) {
                 const GhostDev gd = gmem(S.ghosts)[gid];
-                g[q] = c + gd.k_raster * gmem(S.acc_now)[gd.obj];
+                g[q] = c + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
             } else {
                 g[q] = gmem(S.gval[step & 1])[gid];
             }
@@ -554,7 +554,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         for (int s = 0; s < S.n_psrc; ++s) {
             const PSrcDev ps = cmem(S.psrc)[s];
             const AS_CONST float *g3 = cmem(S.gtab);
-            pn += (double)(gmem(S.src_now)[s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
+            pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
         }
     }
     pnext[idx] = (float)pn;
@@ -564,27 +564,37 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
 // Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
 // eight and arranged so that workgroup w (dispatched to XCD w % 8) runs chunk
 // w % 8; the reversed order keeps that assignment and reverses each chunk.
-__device__ __forceinline__ int tile_of(int w, int ntiles, const unsigned long long *order_clock)
+__device__ __forceinline__ int tile_of(int w, int ntiles, const unsigned long long *order_clock, int k)
 {
-    if (!order_clock || !(*order_clock & 1ULL)) return w;
+    if (!order_clock || !((*order_clock + (unsigned long long)k) & 1ULL)) return w;
     if (ntiles >= 16 && (ntiles & 7) == 0) return (ntiles / 8 - 1 - w / 8) * 8 + (w & 7);
     return ntiles - 1 - w;
 }
 
 // Step advance (PML_WaveSolver::stepSystem's time update), one workgroup per
-// scene after the step kernels: the amplitudes of step + 1, then the step
-// counter.  The kernel boundary orders it after every read of this step's
-// amplitudes and before the next step reads the counter.
-__global__ __launch_bounds__(64) void advance_kernel(const SceneDev *__restrict__ scenes)
+// scene after a block of steps: the counter moves n steps on (not past the
+// target), and the amplitude ring is refilled for the AMP_ROWS steps from
+// there.  The kernel boundaries order it after every read of the block's
+// amplitudes and counter and before the next block reads them.
+__global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict__ scenes, int n)
 {
     const SceneDev &S = scenes[blockIdx.x];
     AS_GLOBAL unsigned long long *clk = gmem(S.clock);
-    const long long step = (long long)clk[CLK_STEP];
-    if (step >= (long long)clk[CLK_TARGET]) return;
-    const double tn1 = S.t_start + (double)(step + 1) * S.dt;
-    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) gmem(S.acc_now)[o] = impulse_accel(S.isrc, S.n_isrc, o, tn1);
-    for (int s = threadIdx.x; s < S.n_psrc; s += blockDim.x) gmem(S.src_now)[s] = source_amp(S.psrc[s], tn1, S.dt, S.scheme);
-    if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)(step + 1);
+    const long long step = (long long)clk[CLK_STEP], target = (long long)clk[CLK_TARGET];
+    if (step >= target) return;
+    const long long base = min(step + (long long)n, target);
+    for (int q = threadIdx.x; q < AMP_ROWS * S.n_obj; q += blockDim.x) {
+        const long long s = base + q / S.n_obj;
+        const int o = q % S.n_obj;
+        gmem(S.acc_tab)[amp_row(s) * S.n_obj + o] = impulse_accel(S.isrc, S.n_isrc, o, S.t_start + (double)s * S.dt);
+    }
+    for (int q = threadIdx.x; q < AMP_ROWS * S.n_psrc; q += blockDim.x) {
+        const long long s = base + q / S.n_psrc;
+        const int p = q % S.n_psrc;
+        gmem(S.src_tab)[amp_row(s) * S.n_psrc + p] = source_amp(S.psrc[p], S.t_start + (double)s * S.dt, S.dt, S.scheme);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)base;
 }
 
 // One kernel per tile class, so each class gets its own register allocation
@@ -594,12 +604,12 @@ __global__ __launch_bounds__(64) void advance_kernel(const SceneDev *__restrict_
 template <int KIND>
 __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles,
                                                    int ntiles, const TermRef *__restrict__ terms,
-                                                   const unsigned long long *__restrict__ order_clock)
+                                                   const unsigned long long *__restrict__ order_clock, int k)
 {
-    const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, order_clock)];
+    const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, order_clock, k)];
     if (t.type == TILE_EMPTY) return;
     const SceneDev &S = scenes[t.scene];
-    const long long step = (long long)S.clock[CLK_STEP];
+    const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
     if constexpr (KIND == KIND_INTERIOR) {
         // the interior launch also carries the object-adjacent cells, whose
@@ -628,17 +638,17 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
 // fully-coupled ghost values (MAC_Grid.cpp:1131-1441): Jacobi iteration
 // x <- b + W x on the static coupling matrix; one workgroup per scene.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list)
+__global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list, int k)
 {
     const SceneDev &S = scenes[list[blockIdx.x].scene];
-    const long long step = (long long)S.clock[CLK_STEP];
+    const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
     const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
     double *gval = S.gval[step & 1];
     double *b = S.gtmp, *x1 = S.gtmp + S.G;
     for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
         const GhostDev gd = S.ghosts[g];
-        const double a = S.acc_now[gd.obj];
+        const double a = S.acc_tab[amp_row(step) * S.n_obj + gd.obj];
         double v;
         if (gd.coupled) {
             v = gd.c_acc * a;
@@ -665,29 +675,29 @@ __global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict_
 }
 
 hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
-                       const unsigned long long *order_clock, hipStream_t s)
+                       const unsigned long long *order_clock, int k, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     switch (kind) {
-    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
-    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
-    case KIND_SPECIAL: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
-    default: hipLaunchKernelGGL(step_kernel<KIND_ALL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock); break;
+    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
+    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
+    case KIND_SPECIAL: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
+    default: hipLaunchKernelGGL(step_kernel<KIND_ALL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_advance(const SceneDev *scenes, int nscenes, hipStream_t s)
+hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s)
 {
     if (nscenes <= 0) return hipSuccess;
-    hipLaunchKernelGGL(advance_kernel, dim3(nscenes), dim3(64), 0, s, scenes);
+    hipLaunchKernelGGL(advance_kernel, dim3(nscenes), dim3(256), 0, s, scenes, n);
     return hipGetLastError();
 }
 
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, hipStream_t s)
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ghost_kernel, dim3(n), dim3(1024), 0, s, scenes, list);
+    hipLaunchKernelGGL(ghost_kernel, dim3(n), dim3(1024), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

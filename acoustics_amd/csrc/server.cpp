@@ -98,7 +98,7 @@ fdtd_status fdtd_server::rebuild()
     if (s != FDTD_OK) return s;
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < block && s == FDTD_OK; ++k) s = prog.enqueue(stream, k);
+    s = prog.enqueue(stream, 0, block);
     hipError_t e = hipStreamEndCapture(stream, &g);
     if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }

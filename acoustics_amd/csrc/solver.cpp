@@ -28,7 +28,7 @@ struct fdtd_handle {
     long long step = 0;          // steps taken
     long long flushed = 0;       // listener rows copied to host_series
     std::vector<double> host_series;
-    bool timing = false;
+    int timing = 0;             // FDTD_TIMING_*
     std::vector<hipEvent_t> ev;  // [2*i] begin, [2*i+1] end of step kernel i
     hipEvent_t call_begin = nullptr, call_end = nullptr;
     int timed_steps = 0;
@@ -182,7 +182,7 @@ static fdtd_status build_graph(fdtd_handle *h)
     hipGraph_t g = nullptr;
     FDTD_HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     fdtd_status st = FDTD_OK;
-    for (int o = 0; o < h->graph_steps && st == FDTD_OK; ++o) st = h->prog.enqueue(h->stream, o);
+    st = h->prog.enqueue(h->stream, 0, h->graph_steps);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (st != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return st; }
     if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
@@ -197,15 +197,22 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
     if (!h || nsteps < 0) return FDTD_ERR_INVALID;
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     const int cap = h->D.series_cap;
+    const bool per_step = h->timing == FDTD_TIMING_PER_STEP;
     if (h->timing) {
-        if ((int)h->ev.size() < 2 * nsteps) {
+        if (per_step && (int)h->ev.size() < 2 * nsteps) {
+            // timing-only events: no system-scope fence, so recording one does
+            // not write back and invalidate the caches under the next kernel
+            // (read only after a stream synchronisation)
             while ((int)h->ev.size() < 2 * nsteps) {
                 hipEvent_t e;
-                FDTD_HIP_TRY(hipEventCreate(&e));
+                FDTD_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
                 h->ev.push_back(e);
             }
         }
-        if (!h->call_begin) { FDTD_HIP_TRY(hipEventCreate(&h->call_begin)); FDTD_HIP_TRY(hipEventCreate(&h->call_end)); }
+        if (!h->call_begin) {
+            FDTD_HIP_TRY(hipEventCreateWithFlags(&h->call_begin, hipEventDisableSystemFence));
+            FDTD_HIP_TRY(hipEventCreateWithFlags(&h->call_end, hipEventDisableSystemFence));
+        }
         FDTD_HIP_TRY(hipEventRecord(h->call_begin, h->stream));
         h->timed_steps = nsteps;
     }
@@ -213,8 +220,10 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
     while (left > 0) {
         // graphs are captured for a block that starts on an even step (the
         // kernel order alternates with the step parity)
-        const bool use_graph = !h->timing && h->graph_steps > 0 && left >= h->graph_steps && (h->step & 1) == 0;
-        const int block = use_graph ? h->graph_steps : 1;
+        const bool use_graph = !per_step && h->graph_steps > 0 && left >= h->graph_steps && (h->step & 1) == 0;
+        // eager blocks share one advance launch (at most AMP_ROWS steps, and
+        // no more than the series ring holds)
+        const int block = use_graph ? h->graph_steps : std::min({left, (int)AMP_ROWS, cap});
         if (h->step + block - h->flushed > cap) {
             fdtd_status st = flush_series(h);
             if (st != FDTD_OK) return st;
@@ -226,9 +235,7 @@ fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
             }
             FDTD_HIP_TRY(hipGraphLaunch(h->graph_exec, h->stream));
         } else {
-            hipEvent_t b = nullptr, e = nullptr;
-            if (h->timing) { b = h->ev[2 * done]; e = h->ev[2 * done + 1]; }
-            fdtd_status st = h->prog.enqueue(h->stream, h->step, b, e);
+            fdtd_status st = h->prog.enqueue(h->stream, h->step, block, per_step ? &h->ev[2 * done] : nullptr);
             if (st != FDTD_OK) return st;
         }
         h->step += block;
@@ -249,7 +256,8 @@ fdtd_status fdtd_synchronize(fdtd_handle *h)
 fdtd_status fdtd_set_timing(fdtd_handle *h, int enable)
 {
     if (!h) return FDTD_ERR_INVALID;
-    h->timing = enable != 0;
+    if (enable < FDTD_TIMING_OFF || enable > FDTD_TIMING_SPAN) { set_last_error("unknown timing mode"); return FDTD_ERR_INVALID; }
+    h->timing = enable;
     return FDTD_OK;
 }
 
@@ -259,7 +267,7 @@ fdtd_status fdtd_last_kernel_ms(fdtd_handle *h, double *stencil_ms, double *step
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     double ks = 0, cs = 0;
     if (h->timed_steps > 0) {
-        for (int i = 0; i < h->timed_steps; ++i) {
+        for (int i = 0; h->timing == FDTD_TIMING_PER_STEP && i < h->timed_steps; ++i) {
             float ms = 0;
             FDTD_HIP_TRY(hipEventElapsedTime(&ms, h->ev[2 * i], h->ev[2 * i + 1]));
             ks += ms;
@@ -533,11 +541,8 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
     // clock and amplitudes of step s; the series ring starts empty at s
     const unsigned long long clk[CLK_WORDS] = {(unsigned long long)s, (unsigned long long)LLONG_MAX, 0ULL, 0ULL};
     FDTD_HIP_TRY(hipMemcpy(h->D.dev.clock, clk, sizeof clk, hipMemcpyHostToDevice));
-    std::vector<double> acc(S.objs.size());
-    std::vector<float> src(S.psrc.size());
-    step_amplitudes(S, s, acc.data(), src.data());
-    if (!acc.empty()) FDTD_HIP_TRY(hipMemcpy(h->D.dev.acc_now, acc.data(), acc.size() * 8, hipMemcpyHostToDevice));
-    if (!src.empty()) FDTD_HIP_TRY(hipMemcpy(h->D.dev.src_now, src.data(), src.size() * 4, hipMemcpyHostToDevice));
+    FDTD_HIP_TRY(h->D.fill_amplitudes(h->stream));
+    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     h->step = s;
     h->flushed = s;
     h->host_series.assign((size_t)s * S.L, 0.0);

@@ -30,7 +30,7 @@ STEP_KERNELS = ("step_kernel<0>", "step_kernel<1>", "step_kernel<2>", "step_kern
 KERNEL_ROLE = {"step_kernel<0>": "interior + object-adjacent cells", "step_kernel<1>": "PML slabs",
                "step_kernel<2>": "object-adjacent cells",
                "step_kernel<3>": "one step, all tile classes (PML, interior, object-adjacent) + listener terms",
-               "advance_kernel": "step advance (per-step amplitudes, step counter)", "ghost_kernel": "coupled ghost solve"}
+               "advance_kernel": "block advance (step counter, amplitude ring for the next 64 steps)", "ghost_kernel": "coupled ghost solve"}
 
 
 def copy(mib, iters):
@@ -49,7 +49,8 @@ def step(n, steps, tile):
     from .solver import Solver
     sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n)
     with Solver(sb, tile_config=tile) as s:
-        s.set_timing(True)  # eager: one dispatch per kernel per step
+        # eager (one dispatch per kernel per step, one advance per 64 steps),
+        # no timing events between the kernels
         s.step(steps)
         s.synchronize()
         inf = s.info()

@@ -92,7 +92,10 @@ class Solver:
                                                int(numbering)), "fdtd_load_snapshot")
 
     def set_timing(self, on=True):
-        capi.check(self.lib.fdtd_set_timing(self.h, 1 if on else 0), "fdtd_set_timing")
+        """on: True / FDTD_TIMING_PER_STEP (events around each step's kernels),
+        FDTD_TIMING_SPAN (events around each fdtd_step call only), or False."""
+        mode = capi.FDTD_TIMING_PER_STEP if on is True else (capi.FDTD_TIMING_OFF if on is False else int(on))
+        capi.check(self.lib.fdtd_set_timing(self.h, mode), "fdtd_set_timing")
 
     def last_kernel_ms(self):
         a, b = C.c_double(), C.c_double()

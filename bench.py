@@ -10,9 +10,13 @@ headline metric.  Inputs are resident in HBM before timing starts.
 Prints ONE JSON line:
   value        whole-job Gcell-updates/s = N * steps / wall time
   roofline     the step kernel: algorithmic bytes (12 N + 44 N_pml per step)
-               / mean kernel duration from HIP events recorded on the kernel's
-               stream during the timed region; traffic from the committed
-               counter summary (profiles/step_kernel_256cube_traffic.json)
+               / device time per step from two HIP events recorded on the
+               kernel's stream around the timed region (the step kernels run
+               back to back, with one block advance per 64 steps: an upper
+               bound on the kernel's mean duration); traffic from the
+               committed counter summary (profiles/step_kernel_256cube_traffic.json);
+               state_bytes / state_frac: the same for the bytes the engine's
+               merged PML state moves (12 N + 44 N_pml - 16 N_merged)
   staggered    the reference's staggered (MAC) scheme on a 256^3 free field
   batch        64 uneven scenes advanced together vs one at a time
   serving      a Poisson request stream: continuous batching vs the plain
@@ -82,16 +86,18 @@ def staggered(steps, n=256):
     from acoustics_amd import scenes
     from acoustics_amd.solver import Solver
     sb, _ = scenes.staggered_free_field(seed=7, div=(n, n, n), h=0.64 / n, pml=16, n_listeners=8)
+    from acoustics_amd import capi
     with Solver(sb, series_capacity=max(1024, steps + 64)) as s:
         info = s.info()
-        s.set_timing(True)
+        s.set_timing(capi.FDTD_TIMING_SPAN)
         s.step(20)
         s.synchronize()
         t0 = time.perf_counter()
         s.step(steps)
         s.synchronize()
         wall = time.perf_counter() - t0
-        kern_ms, step_ms = s.last_kernel_ms()
+        _, step_ms = s.last_kernel_ms()
+        kern_ms = step_ms
         finite = bool(np.isfinite(s.series()).all())
     ach = info["bytes_per_step"] / (kern_ms * 1e-3) / 1e9
     return {"workload": f"{n}^3 free field, staggered MAC scheme, 16-cell PML, 1 pressure source, 8 listeners",
@@ -218,7 +224,7 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(multi-GPU slab decomposition is a design note only)", file=sys.stderr)
         sys.exit(2)
-    from acoustics_amd import scenes
+    from acoustics_amd import capi, scenes
     from acoustics_amd.solver import Solver
 
     n = a.n
@@ -227,17 +233,20 @@ def main():
     info = s.info()
     N, Npml = info["n_cells"], info["n_pml_cells"]
     bytes_step = info["bytes_per_step"]
-    # warmup (eager + timing path identical to the timed one)
-    s.set_timing(True)
+    state_bytes = info["state_bytes_per_step"]
+    # timed region: K steps back to back on the solver's stream, bracketed by
+    # two HIP events on that stream (events between the kernels would add a
+    # few microseconds to every step).  The device time per step is the step
+    # kernel's duration plus one block advance per 64 steps and the launch
+    # gaps, an upper bound on the kernel's average duration.
+    s.set_timing(capi.FDTD_TIMING_SPAN)
     s.step(a.warmup)
-    s.synchronize()
-    # timed region: K steps, events around every step kernel on its stream
     s.synchronize()
     t0 = time.perf_counter()
     s.step(a.steps)
     s.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms, step_ms = s.last_kernel_ms()
+    _, kern_ms = s.last_kernel_ms()
     series = s.series()
     finite = bool((series == series).all())
     s.close()
@@ -270,7 +279,10 @@ def main():
                    "cells": N, "pml_cells": Npml, "bytes_per_step": bytes_step, "listeners_finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5), "step_ms_events": round(step_ms, 5)},
+                     "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5),
+                     "kernel_ms_source": "HIP events around the timed steps on the solver stream, per step",
+                     "state_bytes": state_bytes,
+                     "state_frac": round(state_bytes / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4) if kern_ms > 0 else 0.0},
     }
     if a.staggered_steps > 0:
         out["staggered"] = staggered(a.staggered_steps, n)

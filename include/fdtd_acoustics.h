@@ -232,9 +232,18 @@ fdtd_status fdtd_read_listener_series(fdtd_handle *h, int64_t first_step, int64_
 /* Current pressure field (FP64 copy of the FP32 field, z fastest). */
 fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out);
 fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *out);
-/* Mean device duration (ms) of the stencil kernel over the last fdtd_step
- * call, measured with HIP events on the handle's stream (0 if unmeasured). */
-fdtd_status fdtd_set_timing(fdtd_handle *h, int enable);
+/* Device timing of the next fdtd_step calls with HIP events on the handle's
+ * stream (timing-only events without a system-scope fence).  PER_STEP
+ * brackets each step's kernels (stencil_ms: their mean duration; the event
+ * records add a few microseconds between kernels); SPAN brackets the call
+ * only (step_ms: device time per step, nothing recorded between kernels;
+ * stencil_ms 0).  Both set step_ms.  1 = PER_STEP for older callers. */
+enum {
+    FDTD_TIMING_OFF      = 0,
+    FDTD_TIMING_PER_STEP = 1,
+    FDTD_TIMING_SPAN     = 2
+};
+fdtd_status fdtd_set_timing(fdtd_handle *h, int mode);
 fdtd_status fdtd_last_kernel_ms(fdtd_handle *h, double *stencil_ms, double *step_ms);
 
 /* ------------------------------------------------------------------------- */
