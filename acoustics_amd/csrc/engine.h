@@ -86,20 +86,22 @@ struct ListenerTerm {
 // same update by each.
 //
 // Merged transverse state (merge = 1, collocated scheme only): a cell of the
-// box that is plain along both axes other than the box axis (no damping on
+// box that is plain along both axes other than the box axis A (no damping on
 // those axes: f1 = f3 = 1 on its faces and at the cell, no wall faces) keeps
-// the sum q of its two transverse split pressures and its last increment
-// d^n = q^n - q^{n-1} instead of the two splits and their four face
+// only the last increment d^n = q^n - q^{n-1} of the sum q of its two
+// transverse split pressures, instead of the two splits and their four face
 // velocities.  With f1 = f3 = 1 the split updates give exactly
 //   d^{n+1} = d^n + sum_{b transverse} f5h_b (f2_b(R) (p_R - p) - f2_b(L) (p - p_L)),
-//   q^{n+1} = q^n + d^{n+1}
-// (d is f5h times the transverse velocity divergence).  Keeping d rather
-// than q^{n-1} integrates rounding errors once, as the split form does; the
-// form 2 q^n - q^{n-1} would integrate them twice and let them drift.  q
-// lives in the split array of the lower transverse axis, d in the higher
-// one's.  The box-axis split and its faces are kept as before.  Transverse
-// faces of a merged cell are updated only where a non-merged cell of the same
-// box reads them.
+//   q^{n+1} = q^n + d^{n+1},   p^{n+1} = pDir_A^{n+1} + q^{n+1},
+// (d is f5h times the transverse velocity divergence), so q need not be kept:
+//   p^{n+1} = p^n + (pDir_A^{n+1} - pDir_A^n) + d^{n+1}.
+// Only increments are rounded before the final addition, so rounding stays of
+// order eps |p| per step, integrated once.  (Forming q^n = p^n - pDir_A^n
+// rounds at the scale of pDir_A, which deep in the layer is much larger than
+// p; keeping q^{n-1} and forming 2 q^n - q^{n-1} integrates rounding twice and
+// lets it drift.)  d lives in the split array of the lower transverse axis.  The box-axis split and its faces are kept as
+// before.  Transverse faces of a merged cell are updated only where a
+// non-merged cell of the same box reads them.
 struct PmlSlab {
     int32_t lo[3], e[3];
     int32_t axis;                // the box's damped axis (x slabs 0, y slabs 1, z slabs 2)
@@ -164,8 +166,8 @@ void set_pml_state(SceneSetup &s, bool merged);
 // staggered 32 N + 24 N_pml.
 double algorithmic_bytes(const SceneSetup &s);
 // Bytes per step of this engine's state representation: a merged PML cell
-// moves 40 B instead of 56 (it drops two splits and four face velocities,
-// read and written, and reads and writes q and d).
+// moves 32 B instead of 56 (it drops two splits and four face velocities,
+// read and written, and reads and writes d).
 double state_bytes(const SceneSetup &s);
 
 // thread-local error text for fdtd_last_error

@@ -152,6 +152,20 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     }
 }
 
+// Leapfrog update of four cells along k: centre c, previous level prev, x
+// neighbours xm/xp, y neighbours ym/yp, and the k neighbours beyond the four
+// (zl below c.x, zr above c.w).
+__device__ __forceinline__ float4 leapfrog4(float4 c, float4 prev, float4 xm, float4 xp, float4 ym, float4 yp, float zl,
+                                            float zr, float lam2)
+{
+    float4 o;
+    o.x = 2.f * c.x - prev.x + lam2 * (((xm.x + xp.x) + (ym.x + yp.x) + (zl + c.y)) - 6.f * c.x);
+    o.y = 2.f * c.y - prev.y + lam2 * (((xm.y + xp.y) + (ym.y + yp.y) + (c.x + c.z)) - 6.f * c.y);
+    o.z = 2.f * c.z - prev.z + lam2 * (((xm.z + xp.z) + (ym.z + yp.z) + (c.y + c.w)) - 6.f * c.z);
+    o.w = 2.f * c.w - prev.w + lam2 * (((xm.w + xp.w) + (ym.w + yp.w) + (c.z + zr)) - 6.f * c.w);
+    return o;
+}
+
 // Whole rows, four floats per lane: wavefront w of the workgroup owns row
 // j0 + w, lane l the cells k0 + 4l .. k0 + 4l + 3, so one load moves up to
 // 1 KiB of a row.  The k neighbours come from the adjacent lanes; the two row
@@ -169,6 +183,29 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
     AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
     const float lam2 = S.lam2f;
     int idx = t.i0 * sx + j * sy + k;
+    if (t.i1 - t.i0 == 2 && t.te == t.tb) {
+        // two planes with every load issued before any arithmetic (twice the
+        // bytes in flight per wave of the marching form below)
+        const float4 a0 = *(const AS_GLOBAL float4 *)(pcur + idx - sx);
+        const float4 a1 = *(const AS_GLOBAL float4 *)(pcur + idx);
+        const float4 a2 = *(const AS_GLOBAL float4 *)(pcur + idx + sx);
+        const float4 a3 = *(const AS_GLOBAL float4 *)(pcur + idx + 2 * sx);
+        const float4 ym1 = *(const AS_GLOBAL float4 *)(pcur + idx - sy);
+        const float4 yp1 = *(const AS_GLOBAL float4 *)(pcur + idx + sy);
+        const float4 ym2 = *(const AS_GLOBAL float4 *)(pcur + idx + sx - sy);
+        const float4 yp2 = *(const AS_GLOBAL float4 *)(pcur + idx + sx + sy);
+        const float4 pv1 = *(const AS_GLOBAL float4 *)(pnext + idx);
+        const float4 pv2 = *(const AS_GLOBAL float4 *)(pnext + idx + sx);
+        float zl1 = __shfl_up(a1.w, 1), zr1 = __shfl_down(a1.x, 1);
+        float zl2 = __shfl_up(a2.w, 1), zr2 = __shfl_down(a2.x, 1);
+        if (lane == 0) { zl1 = pcur[idx - 1]; zl2 = pcur[idx + sx - 1]; }
+        if (lane == nk4 - 1) { zr1 = pcur[idx + 4]; zr2 = pcur[idx + sx + 4]; }
+        if (act) {
+            *(AS_GLOBAL float4 *)(pnext + idx) = leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2);
+            *(AS_GLOBAL float4 *)(pnext + idx + sx) = leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2);
+        }
+        return;
+    }
     float4 pm = *(const AS_GLOBAL float4 *)(pcur + idx - sx);
     float4 pc = *(const AS_GLOBAL float4 *)(pcur + idx);
     for (int i = t.i0; i < t.i1; ++i, idx += sx) {
@@ -176,15 +213,10 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         const float4 ym = *(const AS_GLOBAL float4 *)(pcur + idx - sy);
         const float4 yp = *(const AS_GLOBAL float4 *)(pcur + idx + sy);
         const float4 prev = *(const AS_GLOBAL float4 *)(pnext + idx);
-        const float zl_row = pcur[idx - 1], zr_row = pcur[idx + 4];
         float zl = __shfl_up(pc.w, 1), zr = __shfl_down(pc.x, 1);
-        if (lane == 0) zl = zl_row;
-        if (lane == nk4 - 1) zr = zr_row;
-        float4 o;
-        o.x = 2.f * pc.x - prev.x + lam2 * (((pm.x + pp.x) + (ym.x + yp.x) + (zl + pc.y)) - 6.f * pc.x);
-        o.y = 2.f * pc.y - prev.y + lam2 * (((pm.y + pp.y) + (ym.y + yp.y) + (pc.x + pc.z)) - 6.f * pc.y);
-        o.z = 2.f * pc.z - prev.z + lam2 * (((pm.z + pp.z) + (ym.z + yp.z) + (pc.y + pc.w)) - 6.f * pc.z);
-        o.w = 2.f * pc.w - prev.w + lam2 * (((pm.w + pp.w) + (ym.w + yp.w) + (pc.z + zr)) - 6.f * pc.w);
+        if (lane == 0) zl = pcur[idx - 1];
+        if (lane == nk4 - 1) zr = pcur[idx + 4];
+        const float4 o = leapfrog4(pc, prev, pm, pp, ym, yp, zl, zr, lam2);
         if (act) {
             *(AS_GLOBAL float4 *)(pnext + idx) = o;
             if (t.te > t.tb) {
@@ -303,9 +335,9 @@ __device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc 
     const int fyM = lj >= 1 ? fy[j - 1] : 0, fzM = lk >= 1 ? fz[k - 1] : 0;
     const bool need_yL = sy || (lj >= 1 && !(mg && (A == 1 || (fyM & 8)) && pz));
     const bool need_zL = sz || (lk >= 1 && !(mg && py && (A == 2 || (fzM & 8))));
-    // merged state: q in the lower transverse axis's array, its increment d in the higher one's
-    AS_GLOBAL float *__restrict__ qarr = A == 0 ? pd1 : pd0;
-    AS_GLOBAL float *__restrict__ darr = A == 2 ? pd1 : pd2;
+    // merged state: the increment d in the lower transverse axis's array; q
+    // itself is p - pDir_A
+    AS_GLOBAL float *__restrict__ darr = A == 0 ? pd1 : pd0;
 
     // byte offsets
     const uint32_t sx4 = 4u * (uint32_t)S.sx, sy4 = 4u * (uint32_t)S.sy;
@@ -331,13 +363,13 @@ __device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc 
         const float ym = ldb(pcur, po - ym4), yp = ldb(pcur, po + yp4);
         const float zm = ldb(pcur, po - zm4), zp = ldb(pcur, po + zp4);
         float vxr_old = 0.f, vyl_old = 0.f, vyr_old = 0.f, vzl_old = 0.f, vzr_old = 0.f;
-        float q0 = 0.f, q1 = 0.f, q2 = 0.f, qc = 0.f, qd = 0.f;
+        float q0 = 0.f, q1 = 0.f, q2 = 0.f, qd = 0.f;
         if (sx) { vxr_old = ldb(vi0, co + cs4); q0 = ldb(pd0, co); }
         if (need_yL) vyl_old = ldb(vi1, yo);
         if (sy) { vyr_old = ldb(vi1, yo + e24); q1 = ldb(pd1, co); }
         if (need_zL) vzl_old = ldb(vi2, zo);
         if (sz) { vzr_old = ldb(vi2, zo + 4u); q2 = ldb(pd2, co); }
-        if (m) { qc = ldb(qarr, co); qd = ldb(darr, co); }
+        if (m) qd = ldb(darr, co);
         const int fxC = fx[i], fxR = last ? 0 : fx[i + 1];
         const float4 cxC = cx[i], cxR = cx[min(i + 1, nx)];
         const bool ci = fxC & 1;
@@ -381,11 +413,12 @@ __device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc 
             if constexpr (A != 0) lap += transverse_term(cxC, cxR, pm, pc, pp);
             if constexpr (A != 1) lap += transverse_term(cyL, cyR, ym, pc, yp);
             if constexpr (A != 2) lap += transverse_term(czL, czR, zm, pc, zp);
+            // p^{n+1} = p^n + (pDir_A^{n+1} - pDir_A^n) + d^{n+1}: only
+            // increments are rounded before the last addition
             const float dn = qd + lap;
-            const float qn = qc + dn;
+            const float dA = (A == 0 ? n0 : A == 1 ? n1 : n2) - (A == 0 ? q0 : A == 1 ? q1 : q2);
             stb(darr, co, dn);
-            stb(qarr, co, qn);
-            pn = (A == 0 ? n0 : A == 1 ? n1 : n2) + qn;
+            pn = pc + (dA + dn);
         } else {
             pn = n0 + n1 + n2;
         }

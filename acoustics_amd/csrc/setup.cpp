@@ -148,7 +148,7 @@ double algorithmic_bytes(const SceneSetup &s)
 
 double state_bytes(const SceneSetup &s)
 {
-    return algorithmic_bytes(s) - 16.0 * (double)s.n_pml_merged;
+    return algorithmic_bytes(s) - 24.0 * (double)s.n_pml_merged;
 }
 
 bool axis_plain(const SceneSetup &s, int d, int i)

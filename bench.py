@@ -16,7 +16,7 @@ Prints ONE JSON line:
                bound on the kernel's mean duration); traffic from the
                committed counter summary (profiles/step_kernel_256cube_traffic.json);
                state_bytes / state_frac: the same for the bytes the engine's
-               merged PML state moves (12 N + 44 N_pml - 16 N_merged)
+               merged PML state moves (12 N + 44 N_pml - 24 N_merged)
   staggered    the reference's staggered (MAC) scheme on a 256^3 free field
   batch        64 uneven scenes advanced together vs one at a time
   serving      a Poisson request stream: continuous batching vs the plain

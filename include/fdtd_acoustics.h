@@ -180,12 +180,12 @@ typedef struct fdtd_handle fdtd_handle;
  * reference's state (MAC_Grid.cpp:383-508): three split pressures pDir_d and
  * the face velocities of every PML cell.  MERGED (default) gives the same
  * pressures up to FP32 rounding with fewer bytes: a cell damped along one
- * axis only keeps that axis's split and faces plus the sum q of the two
- * undamped splits and its increment d, since with f1 = f3 = 1 the split
- * updates of those axes reduce exactly to
+ * axis A only keeps that axis's split and faces plus the increment d of the
+ * sum q of the two undamped splits, since with f1 = f3 = 1 the split updates
+ * of those axes reduce exactly to
  *   d^{n+1} = d^n + sum_b f5/h_b (f2_b(R) dp_R - f2_b(L) dp_L),
- *   q^{n+1} = q^n + d^{n+1}
- * (40 B per such cell and step instead of 56).  Snapshots need SPLIT (they
+ *   p^{n+1} = p^n + (pDir_A^{n+1} - pDir_A^n) + d^{n+1}
+ * (32 B per such cell and step instead of 56).  Snapshots need SPLIT (they
  * write every pDir_d and velocity). */
 enum {
     FDTD_PML_STATE_MERGED = 0,
@@ -215,7 +215,7 @@ typedef struct fdtd_info {
     int64_t device_bytes;      /* device memory held by the handle             */
     int64_t n_pml_merged_cells;   /* PML cells in the merged state (pml_state)  */
     double  state_bytes_per_step; /* bytes the state representation moves per
-                                     step: bytes_per_step - 16 per merged cell */
+                                     step: bytes_per_step - 24 per merged cell */
 } fdtd_info;
 
 fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt, fdtd_handle **out);
