@@ -53,6 +53,11 @@ struct Layout {
 
 long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
 
+// Cells of one pressure slot in device storage: rows along k are padded to a
+// multiple of four floats, so every row starts 16-byte aligned and the
+// four-cells-per-lane kernels can serve any grid width.
+long long slot_cells(const SceneSetup &S) { return pad_cells((long long)S.n[0] * S.n[1] * row_stride(S)); }
+
 // Every piece of time-stepping state in one zero-initialised block: the two
 // pressure slots, the split pressures of the PML cells, and both buffers of
 // the PML face velocities.  Each array starts on a 256-byte boundary.
@@ -74,7 +79,7 @@ size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6])
 void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
 {
     size_t off[10];
-    L.state_floats = state_layout(S, pad_cells(S.N), off);
+    L.state_floats = state_layout(S, slot_cells(S), off);
     L.field = c.take<float>(L.state_floats);
     for (int d = 0; d < 3; ++d) L.pd[d] = L.field ? L.field + off[1 + d] : nullptr;
     for (int b = 0; b < 2; ++b)
@@ -84,7 +89,7 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
         L.coef[d] = c.take<float4>(S.n[d] + 1);
         L.flags[d] = c.take<int>(S.n[d] + 1);
     }
-    L.skip = c.take<uint8_t>(S.objs.empty() ? 0 : (size_t)S.N);
+    L.skip = c.take<uint8_t>(S.objs.empty() ? 0 : (size_t)slot_cells(S));
     L.gtab = c.take<float>(S.gtab.size());
     L.psrc = c.take<PSrcDev>(S.psrc.size());
     L.isrc = c.take<ImpDev>(S.isrc.size());
@@ -148,7 +153,8 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     const long long target = target_steps;
     this->target = target_steps;
     series_cap = cap;
-    npad = pad_cells(S.N);
+    npad = slot_cells(S);
+    sy_dev = row_stride(S);
     Carver c;
     Layout L;
     carve(S, cap, c, L);
@@ -181,11 +187,13 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
         FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size() * sizeof(int)));
     }
+    // the host's cell indices are dense (k fastest, row length n2); device
+    // arrays index padded rows
     std::vector<uint8_t> skip;
     if (!S.objs.empty()) {
-        skip.assign((size_t)S.N, 0);
-        for (int64_t i = 0; i < S.N; ++i) skip[i] = S.bulk[i] ? 0 : 1;
-        for (const SpecialRec &r : S.special) skip[r.cell] = 1;
+        skip.assign((size_t)npad, 0);
+        for (int64_t i = 0; i < S.N; ++i) skip[to_dev(i)] = S.bulk[i] ? 0 : 1;
+        for (const SpecialRec &r : S.special) skip[to_dev(r.cell)] = 1;
         FDTD_HIP_TRY(up(L.skip, skip.data(), skip.size()));
     }
     FDTD_HIP_TRY(up(L.gtab, S.gtab.data(), S.gtab.size() * 4));
@@ -201,12 +209,16 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     std::vector<ImpDev> is;
     for (const ImpulseHost &h : S.isrc) is.push_back({h.obj, 0, h.A, h.w, h.ton});
     FDTD_HIP_TRY(up(L.isrc, is.data(), is.size() * sizeof(ImpDev)));
-    FDTD_HIP_TRY(up(L.recs, S.special.data(), S.special.size() * sizeof(SpecialRec)));
+    std::vector<SpecialRec> recs(S.special);
+    for (SpecialRec &r : recs) r.cell = (int32_t)to_dev(r.cell);
+    FDTD_HIP_TRY(up(L.recs, recs.data(), recs.size() * sizeof(SpecialRec)));
     std::vector<GhostDev> gd;
-    for (const GhostEntry &g : S.ghosts) gd.push_back({g.nbr, g.obj, g.coupled, 0, g.k_raster, g.c_acc});
+    for (const GhostEntry &g : S.ghosts) gd.push_back({(int)to_dev(g.nbr), g.obj, g.coupled, 0, g.k_raster, g.c_acc});
     FDTD_HIP_TRY(up(L.ghosts, gd.data(), gd.size() * sizeof(GhostDev)));
     FDTD_HIP_TRY(up(L.gk_ptr, S.gk_ptr.data(), S.gk_ptr.size() * 4));
-    FDTD_HIP_TRY(up(L.gk_idx, S.gk_idx.data(), S.gk_idx.size() * 4));
+    std::vector<int32_t> gk_idx(S.gk_idx);
+    for (int32_t &c : gk_idx) c = (int32_t)to_dev(c);
+    FDTD_HIP_TRY(up(L.gk_idx, gk_idx.data(), gk_idx.size() * 4));
     FDTD_HIP_TRY(up(L.gk_w, S.gk_w.data(), S.gk_w.size() * 8));
     FDTD_HIP_TRY(up(L.gu_ptr, S.gu_ptr.data(), S.gu_ptr.size() * 4));
     FDTD_HIP_TRY(up(L.gu_idx, S.gu_idx.data(), S.gu_idx.size() * 4));
@@ -222,8 +234,8 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     SceneDev &D = dev;
     std::memset(&D, 0, sizeof D);
     for (int d = 0; d < 3; ++d) D.n[d] = S.n[d];
-    D.sy = S.n[2];
-    D.sx = S.n[1] * S.n[2];
+    D.sy = sy_dev;
+    D.sx = S.n[1] * sy_dev;
     D.raster = S.boundary == FDTD_BOUNDARY_RASTERIZE ? 1 : 0;
     D.scheme = S.scheme;
     D.npad = npad;
@@ -267,6 +279,18 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     FDTD_HIP_TRY(fill_amplitudes(st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     return FDTD_OK;
+}
+
+hipError_t DeviceScene::download_grid(const float *src, float *dense) const
+{
+    return hipMemcpy2D(dense, sizeof(float) * S.n[2], src, sizeof(float) * sy_dev, sizeof(float) * S.n[2],
+                       (size_t)S.n[0] * S.n[1], hipMemcpyDeviceToHost);
+}
+
+hipError_t DeviceScene::upload_grid(float *dst, const float *dense) const
+{
+    return hipMemcpy2D(dst, sizeof(float) * sy_dev, dense, sizeof(float) * S.n[2], sizeof(float) * S.n[2],
+                       (size_t)S.n[0] * S.n[1], hipMemcpyHostToDevice);
 }
 
 fdtd_status DeviceScene::read_series(long long ring_row, long long nrows, double *out) const
@@ -359,7 +383,9 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     const int TK = 1 << tk, TJ = 256 >> tk;
     const int kw = b[2] - a[2];
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
-    const bool vec = !mac && plan.interior_vec && n[2] % 4 == 0 && a[2] % 4 == 0 && kw % 4 == 0 && kw <= 256;
+    // float4 rows start at a[2], a multiple of four; the row stride is one
+    // too, and a row's last group may be partial
+    const bool vec = !mac && plan.interior_vec && a[2] % 4 == 0 && kw <= 256;
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
@@ -378,8 +404,10 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     for (int i0 = a[0]; i0 < b[0]; i0 += plan.interior_ti) {
         const int i1 = std::min(i0 + plan.interior_ti, b[0]);
         if (!vec) { scalar_tiles(i0, i1, a[1], b[1]); continue; }
-        for (int j0 = a[1]; j0 < b[1]; j0 += 4) {
-            const int j1 = std::min(j0 + 4, b[1]);
+        // short rows share a wavefront: rpw rows of kw/4 lanes each
+        const int rpw = vec ? std::max(1, std::min(8, 64 / ((kw + 3) / 4))) : 1;
+        for (int j0 = a[1]; j0 < b[1]; j0 += 4 * rpw) {
+            const int j1 = std::min(j0 + 4 * rpw, b[1]);
             const int lo[3] = {i0, j0, a[2]}, hi[3] = {i1, j1, b[2]};
             if (special_hit(lo, hi) || src_hit(lo, hi)) { scalar_tiles(i0, i1, j0, j1); continue; }
             TileDesc t{};
@@ -387,6 +415,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
             t.type = TILE_INTERIOR_VEC;
             t.i0 = i0; t.i1 = i1; t.j0 = j0; t.j1 = j1; t.k0 = a[2]; t.k1 = b[2];
             t.tk_log2 = 6;
+            t.rpw = rpw;
             out.push_back(t);
         }
     }
@@ -404,19 +433,35 @@ void DeviceScene::assign_terms(int scene_id, size_t base, std::vector<TileDesc> 
     std::unordered_map<int, int> record_of;
     for (int r = 0; r < (int)S.special.size(); ++r) record_of[S.special[r].cell] = r;
     const int sy = S.n[2], sx = S.n[1] * S.n[2];
+    // the cell tiles by (i, j) column, and the record tiles by their first
+    // record: each term then looks at the few tiles over its column
+    std::vector<std::vector<int>> column((size_t)S.n[0] * S.n[1]);
+    std::vector<std::pair<int, int>> rec_tiles;   // (first record, tile)
+    for (size_t q = base; q < out.size(); ++q) {
+        const TileDesc &t = out[q];
+        if (t.type == TILE_SPECIAL) { rec_tiles.push_back({t.i0, (int)q}); continue; }
+        if (t.type == TILE_EMPTY) continue;
+        for (int i = t.i0; i < t.i1; ++i)
+            for (int j = t.j0; j < t.j1; ++j) column[(size_t)i * S.n[1] + j].push_back((int)q);
+    }
+    std::sort(rec_tiles.begin(), rec_tiles.end());
     auto tile_of_cell = [&](int cell) -> int {
         auto it = record_of.find(cell);
-        for (size_t q = base; q < out.size(); ++q) {
-            const TileDesc &t = out[q];
-            if (t.type == TILE_SPECIAL) {
-                if (it != record_of.end() && it->second >= t.i0 && it->second < t.i1) return (int)q;
-                continue;
-            }
-            if (it != record_of.end()) continue;
-            const int i = cell / sx, j = (cell - i * sx) / sy, k = cell - i * sx - j * sy;
-            if (i >= t.i0 && i < t.i1 && j >= t.j0 && j < t.j1 && k >= t.k0 && k < t.k1) return (int)q;
+        if (it != record_of.end()) {
+            // the record tile whose range [i0, i1) holds the record
+            auto r = std::upper_bound(rec_tiles.begin(), rec_tiles.end(), std::make_pair(it->second, INT_MAX));
+            if (r == rec_tiles.begin()) return -1;
+            --r;
+            const TileDesc &t = out[r->second];
+            return (it->second >= t.i0 && it->second < t.i1) ? r->second : -1;
         }
-        return -1;
+        const int i = cell / sx, j = (cell - i * sx) / sy, k = cell - i * sx - j * sy;
+        int best = -1;
+        for (int q : column[(size_t)i * S.n[1] + j]) {
+            const TileDesc &t = out[q];
+            if (k >= t.k0 && k < t.k1 && (best < 0 || q < best)) best = q;
+        }
+        return best;
     };
     for (int q = 0; q < nterm; ++q) {
         const ListenerTerm &lt = S.lst_terms[q];
@@ -430,7 +475,7 @@ void DeviceScene::assign_terms(int scene_id, size_t base, std::vector<TileDesc> 
         t.tb = (int)terms.size();
         for (int qq : by_tile[q - base]) {
             const ListenerTerm &lt = S.lst_terms[qq];
-            terms.push_back({scene_id, qq, lt.kind, lt.index, lt.beta});
+            terms.push_back({scene_id, qq, lt.kind, lt.kind == 0 ? (int)to_dev(lt.index) : lt.index, lt.beta});
         }
         t.te = (int)terms.size();
     }
@@ -483,8 +528,10 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         if (pml_class) {
             if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) return "PML tile too large";
         } else if (t.type == TILE_INTERIOR_VEC) {
-            if (t.j1 - t.j0 > 4 || (t.k0 & 3) || ((t.k1 - t.k0) & 3) || t.k1 - t.k0 > 256 || (d.sy & 3))
-                return "float4 row tile misaligned or too wide";
+            // a partial last group reads up to k0 + 4 ceil((k1 - k0) / 4) <= the row stride
+            if (t.rpw < 1 || t.j1 - t.j0 > 4 * t.rpw || (t.k0 & 3) || t.k1 - t.k0 > 256 ||
+                t.rpw * ((t.k1 - t.k0 + 3) >> 2) > 64 || (d.sy & 3) || t.k0 + ((t.k1 - t.k0 + 3) & ~3) > d.sy)
+                return "float4 row tile misaligned, too wide or with too many rows";
         } else if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) {
             return "interior tile too large";
         }
@@ -575,9 +622,6 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     order_clock = scenes.empty() ? nullptr : scenes[0].clock + CLK_STEP;
-    FDTD_HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    FDTD_HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    FDTD_HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     return FDTD_OK;
 }
 
@@ -590,11 +634,6 @@ void StepProgram::release()
     d_ghost = nullptr;
     nghost = 0;
     nscenes = 0;
-    if (fork) (void)hipEventDestroy(fork);
-    if (join) (void)hipEventDestroy(join);
-    if (side) (void)hipStreamDestroy(side);
-    fork = join = nullptr;
-    side = nullptr;
     order_clock = nullptr;
     fused = false;
 }
@@ -607,18 +646,9 @@ fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hip
         if (fused) return kind == KIND_INTERIOR ? launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s) : hipSuccess;
         return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
     };
-    const bool split = concurrent && side && ntiles[KIND_PML] > 0 && (ntiles[KIND_INTERIOR] + ntiles[KIND_SPECIAL]) > 0;
-    if (split) {
-        // the PML slabs carry most of the bytes: they run alone on the main
-        // stream while the object cells and then the interior run beside them
-        FDTD_HIP_TRY(hipEventRecord(fork, st));
-        FDTD_HIP_TRY(hipStreamWaitEvent(side, fork, 0));
-        FDTD_HIP_TRY(launch(KIND_SPECIAL, side));
-        FDTD_HIP_TRY(launch(KIND_INTERIOR, side));
-        FDTD_HIP_TRY(hipEventRecord(join, side));
-        FDTD_HIP_TRY(launch(KIND_PML, st));
-        FDTD_HIP_TRY(hipStreamWaitEvent(st, join, 0));
-    } else if ((step & 1) == 0) {
+    // (running the PML slabs on a second stream beside the interior was
+    // measured slower than one stream and is not offered)
+    if ((step & 1) == 0) {
         FDTD_HIP_TRY(launch(KIND_SPECIAL, st));
         FDTD_HIP_TRY(launch(KIND_INTERIOR, st));
         FDTD_HIP_TRY(launch(KIND_PML, st));

@@ -41,7 +41,8 @@ struct DeviceScene {
     size_t arena_bytes = 0;
     SceneDev dev{};            // host copy of the scene block (pointers into the arena)
     SceneDev *self = nullptr;  // device copy of dev (in the arena), for the scene's own launches
-    long long npad = 0;
+    long long npad = 0;        // floats per pressure slot (padded rows, rounded up to 64)
+    int sy_dev = 0;            // row stride of the device grid (n2 rounded up to 4)
     size_t state_floats = 0;   // floats of time-stepping state from dev.field on
     int series_cap = 0;
 
@@ -52,6 +53,15 @@ struct DeviceScene {
     fdtd_status allocate_and_upload(hipStream_t stream, int series_cap, long long target);
     void release();
     float *slot(int s) const { return dev.field + (size_t)s * (size_t)npad; }
+    // device index of a dense host cell index (k fastest, rows of n2)
+    int64_t to_dev(int64_t dense) const
+    {
+        const int64_t row = dense / S.n[2];
+        return row * sy_dev + (dense - row * S.n[2]);
+    }
+    // a full-grid device array (padded rows) to / from a dense host array
+    hipError_t download_grid(const float *src, float *dense) const;
+    hipError_t upload_grid(float *dst, const float *dense) const;
     long long target = 0;      // steps after which the kernels stop advancing this scene
     // zeroes every time-stepping array (pressures, split pressures, velocities)
     hipError_t zero_state(hipStream_t st) const { return hipMemsetAsync(dev.field, 0, sizeof(float) * state_floats, st); }
@@ -86,14 +96,11 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
 // solve (fully-coupled scenes) -> the step kernels, which also write the
 // listener terms; per block of up to AMP_ROWS steps, one step advance (one
 // workgroup per scene).
-// The step kernels write disjoint cells.  By default they run back to back
-// on one stream in an order that alternates with the step parity (object
-// cells, interior, PML on even steps; the reverse on odd steps), and the
-// tiles inside each kernel alternate direction too, so each step begins on
-// the data the previous step touched last.  With `concurrent` set, the PML
-// slabs run on the main stream while the object cells and the interior run
-// on a side stream (a fork/join that graph capture keeps as parallel
-// branches).
+// The step kernels write disjoint cells.  They run back to back on one
+// stream in an order that alternates with the step parity (object cells,
+// interior, PML on even steps; the reverse on odd steps), and the tiles
+// inside each kernel alternate direction too, so each step begins on the
+// data the previous step touched last.
 struct StepProgram {
     SceneDev *d_scenes = nullptr;
     TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
@@ -101,11 +108,8 @@ struct StepProgram {
     TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
     int nghost = 0, nscenes = 0;
-    bool concurrent = false;
     bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
     // tiles of every kind in one list; split by type here (or kept in one
     // launch when `fused_launch` is set)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,

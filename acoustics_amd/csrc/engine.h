@@ -162,6 +162,8 @@ bool axis_plain(const SceneSetup &s, int d, int i);
 // for the collocated scheme) or the reference's per-axis split state, which
 // snapshots need.  Sets PmlSlab::merge and n_pml_merged.
 void set_pml_state(SceneSetup &s, bool merged);
+// Row stride of the device grid: n2 rounded up to a multiple of four.
+inline int row_stride(const SceneSetup &s) { return (s.n[2] + 3) & ~3; }
 // Algorithmic bytes per step (FP32 fields): collocated 12 N + 44 N_pml;
 // staggered 32 N + 24 N_pml.
 double algorithmic_bytes(const SceneSetup &s);

@@ -114,11 +114,11 @@ struct TileDesc {
     int scene;
     int type;
     int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records
-    int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK (TILE_INTERIOR_VEC: 4 whole rows)
+    int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK (TILE_INTERIOR_VEC: whole rows, see rpw)
     int src;                     // tile intersects a pressure-source box
     int slab;                    // TILE_PML: index into SceneDev::slabs
     int tb, te;                  // listener terms [tb, te) of the program's TermRef table
-    int pad;
+    int rpw;                     // TILE_INTERIOR_VEC: rows per wavefront (the tile holds 4 * rpw rows)
 };
 
 struct GhostDesc {

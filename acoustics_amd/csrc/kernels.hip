@@ -115,6 +115,16 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     const float lam2 = S.lam2f;
 
     int idx = t.i0 * sx + j * sy + k;
+    if (!t.src && !skip_check && t.te == t.tb && t.i1 - t.i0 == 2) {
+        // two planes with every load issued before any arithmetic
+        const float a0 = pcur[idx - sx], a1 = pcur[idx], a2 = pcur[idx + sx], a3 = pcur[idx + 2 * sx];
+        const float ym1 = pcur[idx - sy], yp1 = pcur[idx + sy], zm1 = pcur[idx - 1], zp1 = pcur[idx + 1];
+        const float ym2 = pcur[idx + sx - sy], yp2 = pcur[idx + sx + sy], zm2 = pcur[idx + sx - 1], zp2 = pcur[idx + sx + 1];
+        const float pv1 = pprev[idx], pv2 = pprev[idx + sx];
+        pnext[idx] = 2.f * a1 - pv1 + lam2 * (((a0 + a2) + (ym1 + yp1) + (zm1 + zp1)) - 6.f * a1);
+        pnext[idx + sx] = 2.f * a2 - pv2 + lam2 * (((a1 + a3) + (ym2 + yp2) + (zm2 + zp2)) - 6.f * a2);
+        return;
+    }
     float pm = pcur[idx - sx];
     float pc = pcur[idx];
     if (!t.src && !skip_check && t.te == t.tb) {
@@ -152,6 +162,18 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     }
 }
 
+// Stores the first n (1..4) components of v at p (16-byte aligned).
+__device__ __forceinline__ void store4(AS_GLOBAL float *p, float4 v, int n)
+{
+    if (n == 4) {
+        *(AS_GLOBAL float4 *)p = v;
+    } else {
+        p[0] = v.x;
+        if (n > 1) p[1] = v.y;
+        if (n > 2) p[2] = v.z;
+    }
+}
+
 // Leapfrog update of four cells along k: centre c, previous level prev, x
 // neighbours xm/xp, y neighbours ym/yp, and the k neighbours beyond the four
 // (zl below c.x, zr above c.w).
@@ -166,18 +188,26 @@ __device__ __forceinline__ float4 leapfrog4(float4 c, float4 prev, float4 xm, fl
     return o;
 }
 
-// Whole rows, four floats per lane: wavefront w of the workgroup owns row
-// j0 + w, lane l the cells k0 + 4l .. k0 + 4l + 3, so one load moves up to
-// 1 KiB of a row.  The k neighbours come from the adjacent lanes; the two row
-// ends read theirs from memory.  Plain cells only (no source, no object
-// cell); k0 and the row stride are multiples of four.
+// Whole rows, four floats per lane: wavefront w of the workgroup owns rows
+// j0 + w R .. j0 + w R + R - 1 (R = rpw rows of nk4 lanes each, R nk4 <= 64,
+// so short rows still fill the wavefront), lane l of a row the cells
+// k0 + 4l .. k0 + 4l + 3; one load moves up to 1 KiB.  The k neighbours come
+// from the adjacent lanes; the two row ends read theirs from memory.  Plain
+// cells only (no source, no object cell); k0 and the row stride are
+// multiples of four.
 __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
-    const int lane = threadIdx.x & 63, j = t.j0 + (int)(threadIdx.x >> 6);
-    if (j >= t.j1) return;
-    const int nk4 = (t.k1 - t.k0) >> 2;
-    const bool act = lane < nk4;
-    const int k = t.k0 + 4 * (act ? lane : nk4 - 1);
+    // the last lane of a row may hold a partial group: its cells past k1
+    // (PML or row padding) are read, used as k neighbours, and not written
+    const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
+    const int l64 = threadIdx.x & 63, row = l64 / nk4;
+    const int lane = l64 - row * nk4;             // lane within its row
+    const int jr = t.j0 + (int)(threadIdx.x >> 6) * R + row;
+    // lanes past the wavefront's rows, and rows past the tile, load a valid
+    // row and store nothing
+    const bool act = row < R && jr < t.j1;
+    const int j = act ? jr : t.j0;
+    const int k = t.k0 + 4 * lane;
     const int sx = S.sx, sy = S.sy;
     const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
     AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
@@ -201,8 +231,9 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         if (lane == 0) { zl1 = pcur[idx - 1]; zl2 = pcur[idx + sx - 1]; }
         if (lane == nk4 - 1) { zr1 = pcur[idx + 4]; zr2 = pcur[idx + sx + 4]; }
         if (act) {
-            *(AS_GLOBAL float4 *)(pnext + idx) = leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2);
-            *(AS_GLOBAL float4 *)(pnext + idx + sx) = leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2);
+            const int n = (lane == nk4 - 1 && rem) ? rem : 4;
+            store4(pnext + idx, leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2), n);
+            store4(pnext + idx + sx, leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2), n);
         }
         return;
     }
@@ -218,12 +249,13 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         if (lane == nk4 - 1) zr = pcur[idx + 4];
         const float4 o = leapfrog4(pc, prev, pm, pp, ym, yp, zl, zr, lam2);
         if (act) {
-            *(AS_GLOBAL float4 *)(pnext + idx) = o;
+            const int n = (lane == nk4 - 1 && rem) ? rem : 4;
+            store4(pnext + idx, o, n);
             if (t.te > t.tb) {
                 emit_cell_terms(S, t, terms, step, idx, o.x);
-                emit_cell_terms(S, t, terms, step, idx + 1, o.y);
-                emit_cell_terms(S, t, terms, step, idx + 2, o.z);
-                emit_cell_terms(S, t, terms, step, idx + 3, o.w);
+                if (n > 1) emit_cell_terms(S, t, terms, step, idx + 1, o.y);
+                if (n > 2) emit_cell_terms(S, t, terms, step, idx + 2, o.z);
+                if (n > 3) emit_cell_terms(S, t, terms, step, idx + 3, o.w);
             }
         }
         pm = pc;

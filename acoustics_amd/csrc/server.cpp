@@ -6,6 +6,8 @@
 // different steps -- each carries its own device step counter), and retires
 // finished requests (copy the FP64 listener series, check it is finite, free
 // the memory).  Requests arriving while others run join at the next block.
+// Each request's tile list is built once, when it is admitted; a rebuild
+// concatenates the resident lists.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -39,6 +41,10 @@ struct Request {
     std::vector<double> result;   // [steps][L]
     std::unique_ptr<DeviceScene> dev;
     long long launched = 0;       // steps launched since admission
+    // the request's tiles and listener terms, built once at admission with
+    // scene id 0 and term indices from 0; a rebuild only renumbers them
+    std::vector<TileDesc> tiles;
+    std::vector<TermRef> terms;
 };
 
 }  // namespace
@@ -57,9 +63,11 @@ struct fdtd_server {
     bool stop = false;
     fdtd_server_stats stats{};
     int64_t resident = 0;
-    // step program of the resident set
+    // step program of the resident set; a block's launches are enqueued
+    // directly (each step kernel runs for hundreds of microseconds, so the
+    // host enqueues a block far faster than the device runs it, and a
+    // rebuild needs no graph capture)
     StepProgram prog;
-    hipGraphExec_t exec = nullptr;
     bool dirty = true;
     int block = 64;
 
@@ -81,7 +89,6 @@ void fdtd_server::finish(Request *r, fdtd_status st)
 
 fdtd_status fdtd_server::rebuild()
 {
-    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
     prog.release();
     if (active.empty()) return FDTD_OK;
     std::vector<SceneDev> devs;
@@ -91,21 +98,12 @@ fdtd_status fdtd_server::rebuild()
     for (Request *r : active) {
         const int id = (int)devs.size();
         devs.push_back(r->dev->dev);
-        r->dev->append_tiles(id, tile_plan(0), tiles, terms);
+        const int tb = (int)terms.size();
+        for (TermRef q : r->terms) { q.scene = id; terms.push_back(q); }
+        for (TileDesc t : r->tiles) { t.scene = id; t.tb += tb; t.te += tb; tiles.push_back(t); }
         if (r->dev->needs_ghost_kernel()) ghost.push_back({id});
     }
-    fdtd_status s = prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused);
-    if (s != FDTD_OK) return s;
-    hipGraph_t g = nullptr;
-    FDTD_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    s = prog.enqueue(stream, 0, block);
-    hipError_t e = hipStreamEndCapture(stream, &g);
-    if (s != FDTD_OK) { if (g) (void)hipGraphDestroy(g); return s; }
-    if (e != hipSuccess) { set_last_error(std::string("graph capture: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
-    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { set_last_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return FDTD_ERR_DEVICE; }
-    return FDTD_OK;
+    return prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused);
 }
 
 void fdtd_server::run()
@@ -145,6 +143,7 @@ void fdtd_server::run()
             r->dev = std::make_unique<DeviceScene>();
             r->dev->S = std::move(r->setup);
             fdtd_status s = r->dev->allocate_and_upload(stream, std::max(1, r->steps), r->steps);
+            if (s == FDTD_OK) r->dev->append_tiles(0, tile_plan(0), r->tiles, r->terms);
             std::lock_guard<std::mutex> lk(mu);
             if (s != FDTD_OK) { finish(r, s); done_cv.notify_all(); continue; }
             active.push_back(r);
@@ -162,7 +161,7 @@ void fdtd_server::run()
                 continue;
             }
         }
-        if (hipGraphLaunch(exec, stream) != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
+        if (prog.enqueue(stream, 0, block) != FDTD_OK || hipStreamSynchronize(stream) != hipSuccess) {
             std::lock_guard<std::mutex> lk(mu);
             set_last_error("step block failed on the device");
             for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
@@ -189,7 +188,6 @@ void fdtd_server::run()
         stats.active = (int64_t)active.size();
         done_cv.notify_all();
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
     prog.release();
     std::lock_guard<std::mutex> lk(mu);
     for (Request *r : active) finish(r, FDTD_ERR_CANCELLED);
@@ -227,10 +225,11 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     const int plan = (tile_config >= 0 && tile_config < kNumTilePlans) ? tile_config : 0;
     const TilePlan tp = tile_plan(plan);
     // the scene block as the kernels see it, without device arrays
+    D.sy_dev = row_stride(D.S);
     SceneDev d{};
     for (int a = 0; a < 3; ++a) d.n[a] = D.S.n[a];
-    d.sy = D.S.n[2];
-    d.sx = D.S.n[1] * D.S.n[2];
+    d.sy = D.sy_dev;
+    d.sx = D.S.n[1] * D.sy_dev;
     d.n_special = (int)D.S.special.size();
     d.scheme = D.S.scheme;
     std::vector<TileDesc> tiles;

@@ -312,7 +312,7 @@ fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out)
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const int slot = slot_cur(h->step);  // p^{step}: the newest field
     std::vector<float> tmp((size_t)h->D.S.N);
-    FDTD_HIP_TRY(hipMemcpy(tmp.data(), h->D.slot(slot), tmp.size() * 4, hipMemcpyDeviceToHost));
+    FDTD_HIP_TRY(h->D.download_grid(h->D.slot(slot), tmp.data()));
     for (size_t i = 0; i < tmp.size(); ++i) out[i] = tmp[i];
     return FDTD_OK;
 }
@@ -418,9 +418,9 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
     std::vector<float> f(N);
     std::vector<double> cur(N), prev(N), zero(N, 0.0);
-    FDTD_HIP_TRY(hipMemcpy(f.data(), h->D.slot(slot_cur(s)), N * 4, hipMemcpyDeviceToHost));
+    FDTD_HIP_TRY(h->D.download_grid(h->D.slot(slot_cur(s)), f.data()));
     for (long long c = 0; c < N; ++c) cur[c] = f[c];
-    FDTD_HIP_TRY(hipMemcpy(f.data(), h->D.slot(slot_next(s)), N * 4, hipMemcpyDeviceToHost));
+    FDTD_HIP_TRY(h->D.download_grid(h->D.slot(slot_next(s)), f.data()));
     for (long long c = 0; c < N; ++c) prev[c] = f[c];
     auto put = [&](const std::string &what, const std::vector<double> &v, long long rows) {
         return fdtd_write_matrix(snap_name(pattern, simulator_id, numbering, what).c_str(), v.data(), (int)rows, 1);
@@ -452,7 +452,7 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
         std::vector<double> full(NV, 0.0);
         if (mac) {
             comp.resize(N);
-            FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.vel[vb][d], N * 4, hipMemcpyDeviceToHost));
+            FDTD_HIP_TRY(h->D.download_grid(h->D.dev.vel[vb][d], comp.data()));
             for (long long i = 0; i < S.n[0]; ++i)
                 for (long long j = 0; j < S.n[1]; ++j)
                     for (long long k = 0; k < S.n[2]; ++k)
@@ -507,9 +507,9 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     std::vector<float> f(N);
     for (long long c = 0; c < N; ++c) f[c] = (float)cur[c];
-    FDTD_HIP_TRY(hipMemcpy(h->D.slot(slot_cur(s)), f.data(), N * 4, hipMemcpyHostToDevice));
+    FDTD_HIP_TRY(h->D.upload_grid(h->D.slot(slot_cur(s)), f.data()));
     for (long long c = 0; c < N; ++c) f[c] = (float)prev[c];
-    FDTD_HIP_TRY(hipMemcpy(h->D.slot(slot_next(s)), f.data(), N * 4, hipMemcpyHostToDevice));
+    FDTD_HIP_TRY(h->D.upload_grid(h->D.slot(slot_next(s)), f.data()));
     std::vector<double> full;
     std::vector<float> comp;
     for (int d = 0; d < 3; ++d) {
@@ -530,7 +530,7 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
                 for (long long j = 0; j < S.n[1]; ++j)
                     for (long long k = 0; k < S.n[2]; ++k)
                         comp[(i * S.n[1] + j) * S.n[2] + k] = (float)full[face_index(S.n, d, i, j, k)];
-            FDTD_HIP_TRY(hipMemcpy(h->D.dev.vel[vb][d], comp.data(), N * 4, hipMemcpyHostToDevice));
+            FDTD_HIP_TRY(h->D.upload_grid(h->D.dev.vel[vb][d], comp.data()));
         } else {
             comp.assign(std::max<int64_t>(1, S.n_pml_faces[d]), 0.f);
             for_pml_faces(S, d, [&](long long c, long long g) { comp[c] = (float)full[g]; });

@@ -126,3 +126,34 @@ def test_listener_next_to_object(hip_lib):
     err = rel_l2(got, ref)
     print("near-object listeners rel L2", err)
     assert err <= TOL
+
+
+@pytest.mark.parametrize("boundary", [0, FDTD_BOUNDARY_FULLY_COUPLED])
+def test_odd_sized_grid(hip_lib, boundary):
+    """Rows whose length is not a multiple of four: the device pads each row
+    to a multiple of four floats, float4 row tiles end in a partial group, and
+    object cells, ghost neighbours and listener cells are addressed through
+    the padded rows."""
+    rng = np.random.default_rng(41)
+    div = (45, 39, 53)
+    s = scenes.base_settings(div, 0.005, 8, boundary)
+    lo, hi = scenes.interior_bounds(s, 2)
+    c = (lo + hi) / 2 + rng.uniform(-0.005, 0.005, 3)
+    obj = dict(type=1, center=tuple(c), semi_axes=(0.022, 0.015, 0.018), rotation=scenes.random_rotation(rng))
+    src = dict(object=0, amplitude=1e3, width=1e-4, onset=4e-4)
+    ps = scenes.gaussian_pressure_source(lo + 0.3 * (hi - lo), 2e-4)
+    lst = np.vstack([scenes.fibonacci_shell(c, 0.032, 10), rng.uniform(lo, hi, size=(4, 3))])
+    from acoustics_amd.capi import SceneBuffers
+    sb = SceneBuffers(s, objects=[obj], impulse_sources=[src], pressure_sources=[ps], listeners=lst)
+    got, ref = _run_pair(sb, 260)
+    err = rel_l2(got, ref)
+    print("odd-sized grid rel L2", err)
+    assert err <= TOL
+    with Solver(sb) as a:
+        a.step(40)
+        pa = a.pressure()
+    o = Oracle(sb, reclassify_each_step=False)
+    o.step(40, record=False)
+    assert pa.shape == tuple(div)
+    assert rel_l2(pa, o.pressure()) <= TOL
+    o.close()
