@@ -200,13 +200,16 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
     // the last lane of a row may hold a partial group: its cells past k1
     // (PML or row padding) are read, used as k neighbours, and not written
     const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
-    const int l64 = threadIdx.x & 63, row = l64 / nk4;
-    const int lane = l64 - row * nk4;             // lane within its row
-    const int jr = t.j0 + (int)(threadIdx.x >> 6) * R + row;
-    // lanes past the wavefront's rows, and rows past the tile, load a valid
-    // row and store nothing
-    const bool act = row < R && jr < t.j1;
+    const int j0w = t.j0 + (int)(threadIdx.x >> 6) * R;   // first row of this wavefront
+    if (j0w >= t.j1) return;                                 // (whole wavefront)
+    const int l64 = threadIdx.x & 63, row = R == 1 ? 0 : l64 / nk4;
+    const int lr = l64 - row * nk4;               // lane within its row
+    const int jr = j0w + row;
+    // lanes past the wavefront's rows or past a row's last group, and rows
+    // past the tile, load a valid group and store nothing
+    const bool act = row < R && lr < nk4 && jr < t.j1;
     const int j = act ? jr : t.j0;
+    const int lane = act ? lr : 0;
     const int k = t.k0 + 4 * lane;
     const int sx = S.sx, sy = S.sy;
     const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
@@ -230,10 +233,17 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         float zl2 = __shfl_up(a2.w, 1), zr2 = __shfl_down(a2.x, 1);
         if (lane == 0) { zl1 = pcur[idx - 1]; zl2 = pcur[idx + sx - 1]; }
         if (lane == nk4 - 1) { zr1 = pcur[idx + 4]; zr2 = pcur[idx + sx + 4]; }
-        if (act) {
-            const int n = (lane == nk4 - 1 && rem) ? rem : 4;
-            store4(pnext + idx, leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2), n);
-            store4(pnext + idx + sx, leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2), n);
+        const float4 o1 = leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2);
+        const float4 o2 = leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2);
+        if (rem == 0) {   // whole groups only (uniform over the tile)
+            if (act) {
+                *(AS_GLOBAL float4 *)(pnext + idx) = o1;
+                *(AS_GLOBAL float4 *)(pnext + idx + sx) = o2;
+            }
+        } else if (act) {
+            const int n = lane == nk4 - 1 ? rem : 4;
+            store4(pnext + idx, o1, n);
+            store4(pnext + idx + sx, o2, n);
         }
         return;
     }

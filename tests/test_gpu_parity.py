@@ -157,3 +157,32 @@ def test_odd_sized_grid(hip_lib, boundary):
     assert pa.shape == tuple(div)
     assert rel_l2(pa, o.pressure()) <= TOL
     o.close()
+
+
+@pytest.mark.parametrize("n2", [168, 100, 257])
+def test_row_widths(hip_lib, n2):
+    """Float4 row tiles at widths that leave lanes idle: one row per
+    wavefront with 38 of 64 lanes (n2 = 168), three rows of 21 lanes
+    (n2 = 100), and 61 lanes ending in a partial group with a padded row
+    (n2 = 257).  The whole field is compared, so a lane writing outside its
+    row's interior cells shows up in the PML."""
+    rng = np.random.default_rng(n2)
+    div = (22, 20, n2)
+    s = scenes.base_settings(div, 0.005, 8)
+    lo, hi = scenes.interior_bounds(s, 1)
+    src = scenes.gaussian_pressure_source((lo + hi) / 2, 2e-4)
+    from acoustics_amd.capi import SceneBuffers
+    sb = SceneBuffers(s, pressure_sources=[src], listeners=rng.uniform(lo, hi, size=(6, 3)))
+    print(f"case: row widths divisions={div}", flush=True)
+    o = Oracle(sb, reclassify_each_step=False)
+    ref = o.step(120)
+    p_ref = o.pressure()
+    o.close()
+    with Solver(sb) as a:
+        a.step(120)
+        got, p = a.series(), a.pressure()
+    assert rel_l2(got, ref) <= TOL, rel_l2(got, ref)
+    assert rel_l2(p, p_ref) <= TOL, rel_l2(p, p_ref)
+    pml = np.ones(div, dtype=bool)
+    pml[8:-8, 8:-8, 8:-8] = False
+    assert rel_l2(p[pml], p_ref[pml]) <= TOL, rel_l2(p[pml], p_ref[pml])
