@@ -34,6 +34,7 @@ struct Layout {
     PmlSlab *slabs;
     float *pd[3];
     float *vel[2][3];
+    float *lapv;
     float4 *coef[3];
     int *flags[3];
     uint8_t *skip;
@@ -59,12 +60,15 @@ long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
 long long slot_cells(const SceneSetup &S) { return pad_cells((long long)S.n[0] * S.n[1] * row_stride(S)); }
 
 // Every piece of time-stepping state in one zero-initialised block: the two
-// pressure slots, the split pressures of the PML cells, and both buffers of
-// the PML face velocities.  Each array starts on a 256-byte boundary.
-size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6])
+// pressure slots, the split pressures of the PML cells, both buffers of the
+// PML face velocities, and (air viscosity) the bulk cells' previous
+// Laplacian.  Each array starts on a 256-byte boundary.
+size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6 + 1])
 {
     size_t o = (size_t)NUM_P_SLOTS * (size_t)npad;
     auto al = [](size_t x) { return (x + 63) / 64 * 64; };
+    off[10] = o;
+    if (S.visc) o += al((size_t)npad);
     for (int d = 0; d < 3; ++d) { off[1 + d] = o; o += al((size_t)S.n_pml); }
     // face velocities: the PML's compact faces (collocated) or every face of the grid (staggered)
     for (int b = 0; b < 2; ++b)
@@ -78,9 +82,10 @@ size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6])
 
 void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
 {
-    size_t off[10];
+    size_t off[11];
     L.state_floats = state_layout(S, slot_cells(S), off);
     L.field = c.take<float>(L.state_floats);
+    L.lapv = (L.field && S.visc) ? L.field + off[10] : nullptr;
     for (int d = 0; d < 3; ++d) L.pd[d] = L.field ? L.field + off[1 + d] : nullptr;
     for (int b = 0; b < 2; ++b)
         for (int d = 0; d < 3; ++d) L.vel[b][d] = L.field ? L.field + off[4 + 3 * b + d] : nullptr;
@@ -250,6 +255,13 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.skip = S.objs.empty() ? nullptr : L.skip;
     D.lam2 = S.lam2;
     D.lam2f = (float)S.lam2;
+    D.lapv = L.lapv;
+    D.kcalp = S.kcalp;
+    D.c2k2 = S.c2k2;
+    D.inv_h2 = 1.0 / (S.h * S.h);
+    D.kcalpf = (float)S.kcalp;
+    D.c2k2_kcalpf = (float)(S.c2k2 + S.kcalp);
+    D.inv_h2f = (float)(1.0 / (S.h * S.h));
     D.t_start = S.t_start;
     D.dt = S.dt;
     D.n_psrc = (int)S.psrc.size();
@@ -385,7 +397,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
     // float4 rows start at a[2], a multiple of four; the row stride is one
     // too, and a row's last group may be partial
-    const bool vec = !mac && plan.interior_vec && a[2] % 4 == 0 && kw <= 256;
+    const bool vec = !mac && !S.visc && plan.interior_vec && a[2] % 4 == 0 && kw <= 256;
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {

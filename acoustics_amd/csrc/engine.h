@@ -115,6 +115,8 @@ struct SceneSetup {
     int64_t N = 0;
     double h = 0, dt = 0, c = 0, rho = 0, t_start = 0, stop_time = 0;
     double lam2 = 0;             // (c dt / h)^2
+    bool visc = false;           // air viscosity: the bulk update keeps the previous step's Laplacian
+    double kcalp = 0, c2k2 = 0;  // c alpha dt and (c dt)^2
     int boundary = 0;
     int scheme = 0;                 // FDTD_SCHEME_*
     // PML, separable per axis

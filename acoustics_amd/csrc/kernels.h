@@ -71,6 +71,12 @@ struct SceneDev {
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
     double lam2, t_start, dt;
     float lam2f;
+    // air viscosity (MAC_Grid.cpp:926-930): the bulk cells' Laplacian of the
+    // previous step, per cell (null without viscosity), and the coefficients
+    // c alpha dt, (c dt)^2, 1/h^2 in FP64 and FP32
+    float *lapv;
+    double kcalp, c2k2, inv_h2;
+    float kcalpf, c2k2_kcalpf, inv_h2f;
     int n_psrc, n_isrc, n_special;
     const PSrcDev *psrc;
     const float *gtab;

@@ -115,6 +115,31 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     const float lam2 = S.lam2f;
 
     int idx = t.i0 * sx + j * sy + k;
+    if (S.lapv) {
+        // air viscosity (MAC_Grid.cpp:926-930): with L = (sum of the six
+        // neighbours - 6 p) / h^2, p^{n+1} = 2 p^n - p^{n-1} - c alpha dt L^{n-1}
+        // + ((c dt)^2 + c alpha dt) L^n; L^n replaces L^{n-1} in place
+        AS_GLOBAL float *lapv = gmem(S.lapv);
+        for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+            const float c = pcur[idx];
+            const float lc = (((pcur[idx - sx] + pcur[idx + sx]) + (pcur[idx - sy] + pcur[idx + sy]) +
+                               (pcur[idx - 1] + pcur[idx + 1])) - 6.f * c) * S.inv_h2f;
+            const float ll = lapv[idx];
+            float pn = (2.f * c - pprev[idx]) - S.kcalpf * ll + S.c2k2_kcalpf * lc;
+            if (t.src)
+                for (int s = 0; s < S.n_psrc; ++s) {
+                    const PSrcDev ps = cmem(S.psrc)[s];
+                    const AS_CONST float *g = cmem(S.gtab);
+                    pn += gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
+                }
+            if (!skip_check || !gmem(S.skip)[idx]) {
+                pnext[idx] = pn;
+                lapv[idx] = lc;
+                if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, pn);
+            }
+        }
+        return;
+    }
     if (!t.src && !skip_check && t.te == t.tb && t.i1 - t.i0 == 2) {
         // two planes with every load issued before any arithmetic
         const float a0 = pcur[idx - sx], a1 = pcur[idx], a2 = pcur[idx + sx], a3 = pcur[idx + 2 * sx];
@@ -622,7 +647,15 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
             else if (gn)       lap += (0.75 * nb[2 * d + 1] + g[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
             else               lap += (nb[2 * d + 1] + nb[2 * d] - 2.0 * c);
         }
-        pn = 2.0 * c - (double)pprev[idx] + lap * S.lam2;
+        if (S.lapv) {
+            // air viscosity: the ghost-aware Laplacian L^n = lap / h^2 (FP64)
+            // and the previous step's L^{n-1} of this cell
+            const double lc = lap * S.inv_h2;
+            pn = 2.0 * c - (double)pprev[idx] - S.kcalp * (double)gmem(S.lapv)[idx] + (S.c2k2 + S.kcalp) * lc;
+            gmem(S.lapv)[idx] = (float)lc;
+        } else {
+            pn = 2.0 * c - (double)pprev[idx] + lap * S.lam2;
+        }
     }
     if (S.n_psrc > 0) {
         const int i = idx / sx, j = (idx - i * sx) / sy, k = idx - i * sx - j * sy;

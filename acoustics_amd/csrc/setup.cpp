@@ -190,7 +190,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         return FDTD_ERR_INVALID;
     }
     if (!st.use_mesh) { err = "use_mesh=0 selects the Engquist boundary without a PML; this engine needs the PML"; return FDTD_ERR_UNSUPPORTED; }
-    if (st.air_viscosity_alpha != 0.0) { err = "air viscosity (air_viscosity_alpha > 0) is not implemented on the device path"; return FDTD_ERR_UNSUPPORTED; }
+    if (!(st.air_viscosity_alpha >= 0.0)) { err = "air_viscosity_alpha must be >= 0"; return FDTD_ERR_INVALID; }
     if (st.boundary_handling != FDTD_BOUNDARY_RASTERIZE && st.boundary_handling != FDTD_BOUNDARY_FULLY_COUPLED) {
         err = "boundary_handling must be rasterize or fully_coupled";
         return FDTD_ERR_INVALID;
@@ -205,6 +205,11 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     S.h = st.cell_size; S.dt = st.time_step; S.c = st.sound_speed; S.rho = st.density;
     S.t_start = st.start_time; S.stop_time = st.stop_time;
     S.lam2 = (S.c * S.dt / S.h) * (S.c * S.dt / S.h);
+    // ImpulseResponseParser.cpp:540-549: alpha > 0 turns the viscous bulk
+    // update on; the staggered scheme (stepLeapfrog) has no viscous term
+    S.visc = st.air_viscosity_alpha > 0.0 && st.scheme == FDTD_SCHEME_COLLOCATED;
+    S.kcalp = S.c * st.air_viscosity_alpha * S.dt;   // MAC_Grid.cpp:486
+    S.c2k2 = (S.c * S.dt) * (S.c * S.dt);
     S.boundary = st.boundary_handling;
     S.scheme = st.scheme;
     if (st.scheme != FDTD_SCHEME_COLLOCATED && st.scheme != FDTD_SCHEME_STAGGERED) { err = "unknown scheme"; return FDTD_ERR_INVALID; }

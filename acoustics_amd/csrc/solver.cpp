@@ -406,6 +406,11 @@ extern "C" {
 fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.visc) {
+        set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
+                       "(_SaveSimulationSnapshot writes pressures and PML state only)");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (h->D.S.n_pml_merged > 0) {
         set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
                        "FDTD_PML_STATE_SPLIT");
@@ -471,6 +476,11 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
 fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.visc) {
+        set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
+                       "(_SaveSimulationSnapshot writes pressures and PML state only)");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (h->D.S.n_pml_merged > 0) {
         set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
                        "FDTD_PML_STATE_SPLIT");

@@ -8,7 +8,7 @@ must agree bit for bit, and reruns must be bitwise deterministic.
 import numpy as np
 import pytest
 
-from acoustics_amd import scenes
+from acoustics_amd import capi, scenes
 from acoustics_amd.capi import FDTD_BOUNDARY_FULLY_COUPLED
 from acoustics_amd.solver import Solver, rel_l2
 from oracle.oracle import Oracle
@@ -186,3 +186,36 @@ def test_row_widths(hip_lib, n2):
     pml = np.ones(div, dtype=bool)
     pml[8:-8, 8:-8, 8:-8] = False
     assert rel_l2(p[pml], p_ref[pml]) <= TOL, rel_l2(p[pml], p_ref[pml])
+
+
+@pytest.mark.parametrize("case", ["sphere_raster", "ellipsoids_coupled", "free_field_source"])
+def test_air_viscosity(hip_lib, case):
+    """air_viscosity_alpha > 0 (ImpulseResponseParser.cpp:540-549): the bulk
+    cells keep the previous step's Laplacian (MAC_Grid.cpp:926-930), in the
+    2D-chunk interior tiles, the tiles with a pressure source and the
+    object-adjacent cells (FP64 ghost-aware Laplacian)."""
+    alpha = 2e-4
+    if case == "sphere_raster":
+        sb0, _ = scenes.config2(seed=27, n=44, radius=0.025, shell=0.06, pml=8, n_listeners=10)
+    elif case == "ellipsoids_coupled":
+        sb0, _ = scenes.config3(seed=33, n=56, h=0.004, pml=8, n_obj=2, n_listeners=10, axis_lo=0.01,
+                                axis_hi=0.02, boundary=FDTD_BOUNDARY_FULLY_COUPLED)
+    else:
+        sb0, _ = scenes.config1(seed=13, n=44)
+    s = dict(sb0.settings_dict)
+    s["air_viscosity_alpha"] = alpha
+    from acoustics_amd.capi import SceneBuffers
+    sb = SceneBuffers(s, sb0.objects_list, sb0.impulse_list, sb0.pressure_list, sb0.listeners_np)
+    got, ref = _run_pair(sb, 260)
+    err = rel_l2(got, ref)
+    print(case, "viscous rel L2", err)
+    assert err <= TOL
+    # the viscous term is visible: the inviscid series differs from the
+    # viscous one far more than the device differs from the oracle
+    ref0 = Oracle(sb0, reclassify_each_step=False).step(260)
+    assert rel_l2(ref0, ref) > max(10 * TOL, 1000 * err), (rel_l2(ref0, ref), err)
+    import tempfile
+    with Solver(sb, pml_state=capi.FDTD_PML_STATE_SPLIT) as a, tempfile.TemporaryDirectory() as d:
+        a.step(2)
+        with pytest.raises(RuntimeError, match="Laplacian"):
+            a.save_snapshot(d + "/s_%s", "0", 0)
