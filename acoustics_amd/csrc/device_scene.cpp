@@ -255,6 +255,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.skip = S.objs.empty() ? nullptr : L.skip;
     D.lam2 = S.lam2;
     D.lam2f = (float)S.lam2;
+    D.lam = S.c * S.dt / S.h;
     D.lapv = L.lapv;
     D.kcalp = S.kcalp;
     D.c2k2 = S.c2k2;
@@ -379,6 +380,34 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                     }
                     out.push_back(t);
                 }
+    }
+    // Engquist boundary: the layer-1 shell as six disjoint boxes (the x faces
+    // whole, the y faces inside x, the z faces inside x and y), cut into
+    // tiles of at most 256 cells
+    if (S.abc) {
+        const int boxes[6][6] = {
+            {1, 2, 1, n[1] - 1, 1, n[2] - 1},        {n[0] - 2, n[0] - 1, 1, n[1] - 1, 1, n[2] - 1},
+            {2, n[0] - 2, 1, 2, 1, n[2] - 1},        {2, n[0] - 2, n[1] - 2, n[1] - 1, 1, n[2] - 1},
+            {2, n[0] - 2, 2, n[1] - 2, 1, 2},        {2, n[0] - 2, 2, n[1] - 2, n[2] - 2, n[2] - 1},
+        };
+        for (const auto &bx : boxes) {
+            const int ek = bx[5] - bx[4], ej = bx[3] - bx[2], ei = bx[1] - bx[0];
+            const int tk = std::min(ek, 256), tj = std::min(ej, std::max(1, 256 / tk));
+            const int ti = std::min(ei, std::max(1, 256 / (tk * tj)));
+            for (int i0 = bx[0]; i0 < bx[1]; i0 += ti)
+                for (int j0 = bx[2]; j0 < bx[3]; j0 += tj)
+                    for (int k0 = bx[4]; k0 < bx[5]; k0 += tk) {
+                        TileDesc t{};
+                        t.scene = scene_id;
+                        t.type = TILE_ABC;
+                        t.i0 = i0; t.i1 = std::min(i0 + ti, bx[1]);
+                        t.j0 = j0; t.j1 = std::min(j0 + tj, bx[3]);
+                        t.k0 = k0; t.k1 = std::min(k0 + tk, bx[5]);
+                        const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+                        t.src = src_hit(lo, hi);
+                        out.push_back(t);
+                    }
+        }
     }
     // object-adjacent cells
     for (int r0 = 0; r0 < (int)S.special.size(); r0 += 256) {
@@ -518,7 +547,7 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         if (t.scene < 0 || t.scene >= (int)scenes.size()) return "tile scene out of range";
         const SceneDev &d = scenes[t.scene];
         if (t.tb < 0 || t.te < t.tb || (size_t)t.te > nterms) return "tile term range out of range";
-        const bool pml_class = t.type == TILE_PML || t.type == TILE_MAC_PML;
+        const bool pml_class = t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC;
         const bool ok_kind = kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
                              (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL);
         if ((t.type == TILE_MAC || t.type == TILE_MAC_PML) != (d.scheme == FDTD_SCHEME_STAGGERED))
@@ -526,6 +555,19 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         if (!ok_kind) return "tile class does not match its launch";
         if (t.type == TILE_SPECIAL) {
             if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";
+            continue;
+        }
+        if (t.type == TILE_ABC) {
+            // a box inside [1, n - 2] lying in one layer-1 plane: every cell is
+            // a boundary cell and every neighbour it reads or writes is in the grid
+            bool plane = false;
+            const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+            for (int a = 0; a < 3; ++a) {
+                if (lo[a] < 1 || hi[a] > d.n[a] - 1 || lo[a] >= hi[a]) return "boundary tile outside layers 1 .. n-2";
+                plane = plane || (hi[a] - lo[a] == 1 && (lo[a] == 1 || lo[a] == d.n[a] - 2));
+            }
+            if (!plane) return "boundary tile off the layer-1 shell";
+            if ((int64_t)(t.i1 - t.i0) * (t.j1 - t.j0) * (t.k1 - t.k0) > 256) return "boundary tile too large";
             continue;
         }
         const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
@@ -558,7 +600,7 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
-        else by_kind[(t.type == TILE_PML || t.type == TILE_MAC_PML) ? KIND_PML : KIND_INTERIOR].push_back(t);
+        else by_kind[(t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC) ? KIND_PML : KIND_INTERIOR].push_back(t);
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
     xcd_permute(by_kind[KIND_INTERIOR]);

@@ -125,7 +125,11 @@ struct SceneSetup {
     std::vector<float> f1[3], f2[3];   // face tables (n_d + 1)
     std::vector<float> f3[3], f5h[3];  // cell tables (n_d), f5h = f5 / h
     std::vector<uint8_t> sig0[3];      // cell: sigma_d <= 1e-12 (staggered source goes into pDir_d)
-    int inner_lo[3], inner_hi[3];      // non-PML index box
+    int inner_lo[3], inner_hi[3];      // non-PML index box (with the Engquist boundary: [2, n - 2))
+    // Engquist boundary (no PML cell anywhere: use_mesh = 0, or a PML width
+    // of zero): the layer-1 cells take the absorbing update and layer 0 is
+    // never updated (MAC_Grid.cpp:521-919)
+    bool abc = false;
     int64_t n_pml = 0;
     std::vector<PmlSlab> slabs;        // the PML as up to six disjoint boxes
     int64_t n_pml_faces[3] = {0, 0, 0};  // compact face-velocity entries per axis
@@ -164,6 +168,16 @@ bool axis_plain(const SceneSetup &s, int d, int i);
 // for the collocated scheme) or the reference's per-axis split state, which
 // snapshots need.  Sets PmlSlab::merge and n_pml_merged.
 void set_pml_state(SceneSetup &s, bool merged);
+// Engquist boundary state on the device: a face cell of layer 1 reads
+// p^{n-1} of its inner neighbour (two cells in from the exterior cell),
+// which the interior update overwrites in place during the same step, so the
+// face cell keeps a copy in the slot p^{n+1} goes to, at its exterior cell
+// (that slot's exterior value, p^{n-1} there, is read first).  Between steps
+// the exterior cell of every face cell therefore holds p^{n-1} of the inner
+// neighbour in the current slot and the extrapolated exterior pressure in the
+// previous slot.  Writes that copy into cur from prev (full grid, dense
+// indices) for a state loaded from elsewhere; a no-op without the boundary.
+void abc_pack_state(const SceneSetup &s, const double *prev, double *cur);
 // Row stride of the device grid: n2 rounded up to a multiple of four.
 inline int row_stride(const SceneSetup &s) { return (s.n[2] + 3) & ~3; }
 // Algorithmic bytes per step (FP32 fields): collocated 12 N + 44 N_pml;

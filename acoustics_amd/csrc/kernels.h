@@ -71,6 +71,7 @@ struct SceneDev {
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
     double lam2, t_start, dt;
     float lam2f;
+    double lam;                 // c dt / h (the Engquist boundary's coefficients)
     // air viscosity (MAC_Grid.cpp:926-930): the bulk cells' Laplacian of the
     // previous step, per cell (null without viscosity), and the coefficients
     // c alpha dt, (c dt)^2, 1/h^2 in FP64 and FP32
@@ -114,12 +115,13 @@ enum : int {
     TILE_INTERIOR = 0, TILE_INTERIOR_SKIP = 1, TILE_PML = 2, TILE_SPECIAL = 3, TILE_EMPTY = 4, TILE_INTERIOR_VEC = 5,
     TILE_MAC = 6,      // staggered scheme, cells outside the PML
     TILE_MAC_PML = 7,  // staggered scheme, a PML slab (split pressures in the slab's compact storage)
+    TILE_ABC = 8,      // Engquist boundary: a box of layer-1 cells, one thread per cell (k fastest)
 };
 
 struct TileDesc {
     int scene;
     int type;
-    int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records
+    int i0, i1, j0, j1, k0, k1;  // TILE_SPECIAL: [i0, i1) is a range of special records; TILE_ABC: at most 256 cells
     int tk_log2;                 // TK = 1 << tk_log2, TJ = 256 / TK (TILE_INTERIOR_VEC: whole rows, see rpw)
     int src;                     // tile intersects a pressure-source box
     int slab;                    // TILE_PML: index into SceneDev::slabs

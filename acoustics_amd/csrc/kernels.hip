@@ -669,6 +669,95 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
     if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
 }
 
+// ---------------------------------------------------------------------------
+// Engquist-Majda absorbing boundary on the layer-1 cells of a grid without a
+// PML (MAC_Grid.cpp:521-919): order 2 on the faces, order 1 on edges and
+// corners, FP64 arithmetic on FP32 fields.  The update extrapolates the
+// pressures of the exterior (layer-0) neighbours into p^n, where the next
+// step reads them back as p^{n-1}; edges and corners also zero the diagonal
+// exterior cell.  Layer 0 itself is never updated.
+//
+// A face cell also reads p^{n-1} of its inner neighbour, which the interior
+// update overwrites in place during the same step.  The face cell keeps that
+// value itself: after reading p^{n-1} at its exterior cell from the slot
+// p^{n+1} goes to, it stores the inner neighbour's p^n there, and reads it
+// back in the next step from the same cell, then the current slot.  Only the
+// face cell's own thread touches its exterior cell, in either slot.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void abc_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
+{
+    const int nk = t.k1 - t.k0, nj = t.j1 - t.j0;
+    const int q = (int)threadIdx.x, r = q / nk;
+    const int k = t.k0 + (q - r * nk), j = t.j0 + r % nj, i = t.i0 + r / nj;
+    if (i >= t.i1) return;
+    const int ijk[3] = {i, j, k};
+    const int st[3] = {S.sx, S.sy, 1};
+    int nml[3];
+    for (int d = 0; d < 3; ++d) nml[d] = ijk[d] == 1 ? -1 : (ijk[d] == S.n[d] - 2 ? 1 : 0);
+    const int btype = abs(nml[0]) + abs(nml[1]) + abs(nml[2]);
+    if (btype == 0) return;
+    AS_GLOBAL float *pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    const int idx = i * S.sx + j * S.sy + k;
+    const double lam = S.lam, lam2 = lam * lam;
+    const double pc = pcur[idx], pl = pnext[idx];
+    double pn;
+    if (btype == 1) {
+        const int di = nml[0] ? 0 : (nml[1] ? 1 : 2);
+        const int dj = di == 2 ? 0 : di + 1, dk = di == 0 ? 2 : di - 1;
+        const int out = nml[di] * st[di];
+        const int jp = ijk[dj] < S.n[dj] - 1 ? st[dj] : 0, jm = ijk[dj] > 0 ? st[dj] : 0;
+        const int kp = ijk[dk] < S.n[dk] - 1 ? st[dk] : 0, km = ijk[dk] > 0 ? st[dk] : 0;
+        const double pl_out = pnext[idx + out];   // p^{n-1} of the exterior cell
+        const float pin_f = pcur[idx - out];
+        const double pl_in = pcur[idx + out];     // the kept p^{n-1} of the inner neighbour
+        const double p_in = pin_f;
+        const double tang = ((double)pcur[idx + jp] + (double)pcur[idx - jm]) + ((double)pcur[idx + kp] + (double)pcur[idx - km]);
+        double p1 = (2. / lam2 + 4. / lam - 6. - 4. * lam) * pc - (1. / lam2 + 2. / lam) * pl + pl_out - pl_in + 2. * p_in +
+                    (lam + 1.) * tang;
+        p1 *= lam2 / (1. + 2. * lam);
+        const double p_out = pl_out + p_in - pl_in - 2. / lam * (p1 + pl - 2. * pc) + lam * (tang - 4. * pc);
+        pcur[idx + out] = (float)p_out;
+        pnext[idx + out] = pin_f;
+        pn = 2. * pc - pl + lam2 * (p_in + tang + p_out - 6. * pc);
+    } else if (btype == 2) {
+        const int di = nml[0] ? 0 : 1;
+        const int dj = (di == 0 && nml[1]) ? 1 : 2;
+        const int dk = 3 - di - dj;
+        const int oi = nml[di] * st[di], oj = nml[dj] * st[dj];
+        const int kp = ijk[dk] < S.n[dk] - 1 ? st[dk] : 0, km = ijk[dk] > 0 ? st[dk] : 0;
+        const double pi_in = pcur[idx - oi], pj_in = pcur[idx - oj];
+        const double pk = (double)pcur[idx + kp] + (double)pcur[idx - km];
+        double p1 = (2. / lam2 - 6.) * pc + (2. * lam - 1.) / lam2 * pl + 2. * pi_in + 2. * pj_in + pk;
+        p1 *= lam2 / (1. + 2. * lam);
+        const double pi_out = (pl - p1) / lam + pi_in, pj_out = (pl - p1) / lam + pj_in;
+        pcur[idx + oi] = (float)pi_out;
+        pcur[idx + oj] = (float)pj_out;
+        pcur[idx + oi + oj] = 0.f;
+        pn = 2. * pc - pl + lam2 * (pi_in + pj_in + pk + pi_out + pj_out - 6. * pc);
+    } else {
+        const int oi = nml[0] * st[0], oj = nml[1] * st[1], ok = nml[2] * st[2];
+        const double pi_in = pcur[idx - oi], pj_in = pcur[idx - oj], pk_in = pcur[idx - ok];
+        double p1 = (2. / lam2 - 6.) * pc + (3. * lam - 1.) / lam2 * pl + 2. * pi_in + 2. * pj_in + 2. * pk_in;
+        p1 *= lam2 / (1. + 3. * lam);
+        const double e = (pl - p1) / lam;
+        const double pi_out = e + pi_in, pj_out = e + pj_in, pk_out = e + pk_in;
+        pcur[idx + oi] = (float)pi_out;
+        pcur[idx + oj] = (float)pj_out;
+        pcur[idx + ok] = (float)pk_out;
+        pcur[idx + oi + oj + ok] = 0.f;
+        pn = 2. * pc - pl + lam2 * (pi_in + pj_in + pk_in + pi_out + pj_out + pk_out - 6. * pc);
+    }
+    if (t.src)
+        for (int s = 0; s < S.n_psrc; ++s) {
+            const PSrcDev ps = cmem(S.psrc)[s];
+            const AS_CONST float *g = cmem(S.gtab);
+            pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k]);
+        }
+    pnext[idx] = (float)pn;
+    if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
+}
+
 // Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
 // eight and arranged so that workgroup w (dispatched to XCD w % 8) runs chunk
 // w % 8; the reversed order keeps that assignment and reverses each chunk.
@@ -728,6 +817,7 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     } else if constexpr (KIND == KIND_PML) {
         if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
+        else if (t.type == TILE_ABC) abc_tile(S, t, terms, step);
         else pml_tile(S, t, terms, step);
     } else if constexpr (KIND == KIND_SPECIAL) {
         special_tile(S, t, terms, step);
@@ -737,6 +827,7 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
         else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
         else if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
+        else if (t.type == TILE_ABC) abc_tile(S, t, terms, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     }
     if (t.te > t.tb) emit_ghost_terms(S, t, terms, step);

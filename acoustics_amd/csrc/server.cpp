@@ -265,8 +265,13 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
         set_last_error("plan " + std::to_string(plan) + ": " + why);
         return FDTD_ERR_INVALID;
     };
+    // (with the Engquist boundary the outermost layer is in no tile)
+    const int *n = D.S.n;
     for (int64_t c = 0; c < D.S.N; ++c) {
-        if (hit[c] != 1) return fail("a cell is covered " + std::to_string(hit[c]) + " times");
+        const int i = (int)(c / ((int64_t)n[1] * n[2])), j = (int)((c / n[2]) % n[1]), k = (int)(c % n[2]);
+        const bool outer = i == 0 || j == 0 || k == 0 || i == n[0] - 1 || j == n[1] - 1 || k == n[2] - 1;
+        const int want = (D.S.abc && outer) ? 0 : 1;
+        if (hit[c] != want) return fail("a cell is covered " + std::to_string(hit[c]) + " times");
         if (object_cell[c] && !skips[c]) return fail("an object cell lies in a tile that would overwrite it");
     }
     for (int h : rec_hits)

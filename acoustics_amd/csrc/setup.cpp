@@ -151,6 +151,28 @@ double state_bytes(const SceneSetup &s)
     return algorithmic_bytes(s) - 24.0 * (double)s.n_pml_merged;
 }
 
+void abc_pack_state(const SceneSetup &s, const double *prev, double *cur)
+{
+    if (!s.abc) return;
+    const int *n = s.n;
+    auto flat = [&](int i, int j, int k) { return ((int64_t)i * n[1] + j) * n[2] + k; };
+    // exterior cell e of the face cell one step in along axis d, and the inner
+    // neighbour two steps in; face cells have both other coordinates in [2, n - 3]
+    for (int d = 0; d < 3; ++d) {
+        const int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+        for (int side = 0; side < 2; ++side)
+            for (int u = 2; u < n[e1] - 2; ++u)
+                for (int v = 2; v < n[e2] - 2; ++v) {
+                    int ext[3], in[3];
+                    ext[d] = side ? n[d] - 1 : 0;
+                    in[d] = side ? n[d] - 3 : 2;
+                    ext[e1] = in[e1] = u;
+                    ext[e2] = in[e2] = v;
+                    cur[flat(ext[0], ext[1], ext[2])] = prev[flat(in[0], in[1], in[2])];
+                }
+    }
+}
+
 bool axis_plain(const SceneSetup &s, int d, int i)
 {
     const int n = s.n[d];
@@ -189,7 +211,11 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         err = "cell_size, time_step, sound_speed and density must be positive";
         return FDTD_ERR_INVALID;
     }
-    if (!st.use_mesh) { err = "use_mesh=0 selects the Engquist boundary without a PML; this engine needs the PML"; return FDTD_ERR_UNSUPPORTED; }
+    if (!st.use_mesh && sc.n_objects > 0) {
+        // MAC_Grid::classifyCells returns before any object is rasterised
+        err = "use_mesh=0 classifies no object cells; objects need use_mesh=1";
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (!(st.air_viscosity_alpha >= 0.0)) { err = "air_viscosity_alpha must be >= 0"; return FDTD_ERR_INVALID; }
     if (st.boundary_handling != FDTD_BOUNDARY_RASTERIZE && st.boundary_handling != FDTD_BOUNDARY_FULLY_COUPLED) {
         err = "boundary_handling must be rasterize or fully_coupled";
@@ -262,7 +288,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         for (int i = 0; i < n; ++i) {
             double x = bmin[d];
             x += (0.5 + (double)i) * h;
-            S.cpml[d][i] = inside_axis(d, x);
+            S.cpml[d][i] = st.use_mesh && inside_axis(d, x);
             const double sg = sigma_axis(d, x);
             S.sig0[d][i] = sg > 1e-12 ? 0 : 1;   // MAC_Grid.cpp:966-967 inPML test
             const double D = 1.0 / dt + sg / 2.0;
@@ -273,7 +299,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         for (int i = 0; i <= n; ++i) {
             double x = vmin;
             x += (0.5 + (double)i) * h;
-            S.fpml[d][i] = inside_axis(d, x);
+            S.fpml[d][i] = st.use_mesh && inside_axis(d, x);
             const double sg = sigma_axis(d, x);
             double f1 = 1.0 / dt - sg / 2.0;
             f1 /= 1.0 / dt + sg / 2.0;
@@ -282,6 +308,17 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             S.f1[d][i] = (float)f1;
             S.f2[d][i] = (float)(-1.0 / f2);
         }
+    }
+    // no PML cell at all: the Engquist boundary on layer 1 (MAC_Grid.cpp:521-919)
+    S.abc = true;
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < S.n[d]; ++i) S.abc = S.abc && !S.cpml[d][i];
+    if (S.abc && st.scheme != FDTD_SCHEME_COLLOCATED) {
+        err = "the Engquist boundary (no PML) runs with the collocated scheme; give the staggered scheme a PML";
+        return FDTD_ERR_UNSUPPORTED;
+    }
+    for (int d = 0; d < 3 && !S.abc; ++d) {
+        const int n = S.n[d];
         // non-PML cells along d must form one interval
         int lo = -1, hi = -1;
         for (int i = 0; i < n; ++i)
@@ -290,8 +327,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         for (int i = lo; i < hi; ++i)
             if (S.cpml[d][i]) { err = "non-contiguous PML interior"; return FDTD_ERR_UNSUPPORTED; }
         if (lo < 2 || hi > n - 2) {
-            err = "the PML must cover the two outermost cell layers on every face (boundary preset 0 with PML_width >= 2); "
-                  "the Engquist layer-1 boundary is not implemented";
+            err = "the PML must cover the two outermost cell layers on every face it damps (PML_width >= 2 with "
+                  "boundary preset 0), or no face at all (the Engquist boundary)";
             return FDTD_ERR_UNSUPPORTED;
         }
         S.inner_lo[d] = lo; S.inner_hi[d] = hi;
@@ -302,8 +339,13 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 return FDTD_ERR_UNSUPPORTED;
             }
     }
-    S.n_pml = S.N - (int64_t)(S.inner_hi[0] - S.inner_lo[0]) * (S.inner_hi[1] - S.inner_lo[1]) * (S.inner_hi[2] - S.inner_lo[2]);
-    {
+    if (S.abc)
+        for (int d = 0; d < 3; ++d) { S.inner_lo[d] = 2; S.inner_hi[d] = S.n[d] - 2; }
+    S.n_pml = S.abc ? 0 : S.N - (int64_t)(S.inner_hi[0] - S.inner_lo[0]) * (S.inner_hi[1] - S.inner_lo[1]) * (S.inner_hi[2] - S.inner_lo[2]);
+    S.slabs.clear();
+    for (int d = 0; d < 3; ++d) S.n_pml_faces[d] = 0;
+    S.n_pml_merged = 0;
+    if (!S.abc) {
         // x slabs span the grid, y slabs the inner x range, z slabs the inner x and y ranges
         const int *n = S.n, *a = S.inner_lo, *b = S.inner_hi;
         const int boxes[6][6] = {
@@ -433,13 +475,21 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         std::sort(solid.begin(), solid.end());
         S.n_solid = (int64_t)solid.size();
         // a solid cell in the PML or on the outer layers is outside what the
-        // device path restates
+        // device path restates; with the Engquist boundary the cells next to
+        // an object stay off layer 1 (layers 0-2 hold no solid cell)
         for (int64_t ci : solid) {
             const int i = (int)(ci / ((int64_t)ny * nz)), j = (int)((ci / nz) % ny), k = (int)(ci % nz);
             if (S.cpml[0][i] || S.cpml[1][j] || S.cpml[2][k]) {
                 err = "an object reaches into the PML; objects must lie inside the non-PML interior";
                 return FDTD_ERR_UNSUPPORTED;
             }
+            const int ijk[3] = {i, j, k};
+            for (int d = 0; d < 3 && S.abc; ++d)
+                if (ijk[d] < 3 || ijk[d] > S.n[d] - 4) {
+                    err = "an object comes within three cells of the grid boundary; the Engquist boundary needs "
+                          "layers 0-2 free of solid cells";
+                    return FDTD_ERR_UNSUPPORTED;
+                }
         }
         // ghost entries (MakeGhostCell), topology order -x,+x,-y,+y,-z,+z
         std::vector<std::pair<uint64_t, GhostEntry>> tmp;
@@ -709,6 +759,15 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             }
         int64_t cells[8];
         const int ne = enclosing_cells(S, bmin, p, cells);
+        for (int q = 0; q < ne && S.abc; ++q) {
+            const int ijk[3] = {(int)(cells[q] / ((int64_t)ny * nz)), (int)((cells[q] / nz) % ny), (int)(cells[q] % nz)};
+            for (int d = 0; d < 3; ++d)
+                if (ijk[d] == 0 || ijk[d] == S.n[d] - 1) {
+                    err = "listener " + std::to_string(l) + " reads a cell of the outermost layer, which the Engquist "
+                          "boundary never updates";
+                    return FDTD_ERR_UNSUPPORTED;
+                }
+        }
         struct Pt { double x[3]; int kind; int index; };
         std::vector<Pt> pts;
         for (int q = 0; q < ne; ++q) {

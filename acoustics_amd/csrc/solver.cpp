@@ -512,6 +512,9 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
         if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_collocated_" + std::to_string((ind + 2) % 3) + ".dat"), N, 1, prev)) != FDTD_OK)
             return st;
     }
+    // the Engquist boundary keeps p^{n-1} of each face cell's inner neighbour
+    // at the face cell's exterior cell of the current slot
+    abc_pack_state(S, prev.data(), cur.data());
     // the whole state starts from zero: faces and cells the snapshot does not hold stay at rest
     FDTD_HIP_TRY(h->D.zero_state(h->stream));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));

@@ -137,6 +137,40 @@ def staggered_free_field(seed=7, div=(40, 40, 40), h=0.005, pml=6, steps=300, n_
     return SceneBuffers(s, pressure_sources=[src], listeners=listeners), steps
 
 
+def engquist_free_field(seed=8, div=(40, 44, 48), h=0.005, steps=300, n_listeners=8, use_mesh=0):
+    """No PML: the Engquist absorbing boundary on the layer-1 cells
+    (use_mesh = 0, MAC_Grid.cpp:521-919, 2457-2473).  One Gaussian pressure
+    pulse (width 2e-4 s) at a seeded point, seeded listeners, plus one
+    listener whose MLS cells include the face cells of layer 1 and one next
+    to the (1, 1, 1) corner."""
+    rng = np.random.default_rng(seed)
+    s = base_settings(div, h, 0)
+    s["use_mesh"] = use_mesh
+    lo, hi = interior_bounds(s, 4)
+    src = gaussian_pressure_source(rng.uniform(lo, hi), 2e-4)
+    mb = np.array(s["min_bound"])
+    mid = mb + 0.5 * h * np.array(div)
+    near_face = mid.copy()
+    near_face[0] = mb[0] + 1.7 * h
+    near_corner = mb + 1.6 * h
+    listeners = np.vstack([rng.uniform(lo, hi, size=(n_listeners - 2, 3)), near_face, near_corner])
+    return SceneBuffers(s, pressure_sources=[src], listeners=listeners), steps
+
+
+def engquist_sphere(seed=9, n=44, h=0.005, radius=0.025, shell=0.06, steps=300, n_listeners=10,
+                    boundary=FDTD_BOUNDARY_RASTERIZE):
+    """A rigid sphere with an impulse (as in config2) in a grid with a PML
+    width of zero: object cells are classified (use_mesh = 1) and the layer-1
+    cells take the Engquist boundary."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, 0, boundary)
+    c = rng.uniform(-0.2, 0.2, size=3) * (n * h / 2 - radius - 4 * h)
+    obj = dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(radius, radius, radius), rotation=np.eye(3))
+    src = dict(object=0, amplitude=1e3, width=1e-4, onset=4e-4)
+    listeners = fibonacci_shell(c, shell, n_listeners)
+    return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners), steps
+
+
 def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
             n_listeners=64, shell=0.2):
     """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse

@@ -105,7 +105,11 @@ typedef struct fdtd_settings {
     double start_time;                /* fast_forward_to_event_time            */
     int    boundary_condition_preset; /* 0 no wall, 1, 2 (MAC_Grid::InsidePML) */
     int    boundary_handling;         /* FDTD_BOUNDARY_*                       */
-    int    use_mesh;                  /* must be 1: builds the PML lists       */
+    int    use_mesh;                  /* 1: classify PML and object cells; 0:
+                                         no PML and no objects, the Engquist
+                                         boundary on layer 1 (MAC_Grid.cpp:
+                                         521-919, 2457-2473).  A PML width of
+                                         0 also selects that boundary.       */
     int    divisions[3];              /* pressure cells per axis               */
     double min_bound[3];              /* minimum corner of the pressure grid   */
     int    scheme;                    /* FDTD_SCHEME_*                         */
@@ -229,7 +233,10 @@ fdtd_status fdtd_listen(fdtd_handle *h, double *out);
 /* Listener time series rows [first_step, first_step + nsteps): FP64 [nsteps][L].
  * Row s holds the listener pressures after step s + 1. */
 fdtd_status fdtd_read_listener_series(fdtd_handle *h, int64_t first_step, int64_t nsteps, double *out);
-/* Current pressure field (FP64 copy of the FP32 field, z fastest). */
+/* Current pressure field (FP64 copy of the FP32 field, z fastest).  With the
+ * Engquist boundary (a scene without PML cells) the outermost cell layer is
+ * never updated (the reference skips it too) and holds boundary state instead
+ * of pressures. */
 fdtd_status fdtd_read_pressure(fdtd_handle *h, double *out);
 fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *out);
 /* Device timing of the next fdtd_step calls with HIP events on the handle's

@@ -612,7 +612,9 @@ static void classify_static(oracle *o)
         pidx3(o, ci, ijk); ppos(o, ijk, x);
         o->bulk[ci] = 1;
         o->pmlc[ci] = 0;
-        if (inside_pml(s, x, o->W)) {
+        /* MAC_Grid::classifyCells (MAC_Grid.cpp:2457-2473): without use_mesh
+           every cell is bulk and no PML cell is made */
+        if (s->use_mesh && inside_pml(s, x, o->W)) {
             pml_pcell *pc = &o->pp[o->npp++];
             pc->index = (int)ci;
             for (int d = 0; d < 3; ++d) {
@@ -704,6 +706,8 @@ static int cmp_ghost(const void *a, const void *b)
 static void classify_fast(oracle *o)
 {
     const int nobj = o->sc.n_objects;
+    /* classifyCellsDynamic_FAST returns at once without use_mesh (MAC_Grid.cpp:3014) */
+    if (!o->sc.settings.use_mesh) { memcpy(o->history, o->bulk, (size_t)o->N); return; }
     memcpy(o->history, o->bulk, (size_t)o->N);
     memset(o->bulk, 1, (size_t)o->N);
     memset(o->cavity, 0, (size_t)o->N);
@@ -1071,8 +1075,81 @@ static double pressure_sources(const oracle *o, const double *x, double t)
 }
 
 /* ======================================================================== */
+/* The Engquist-Majda absorbing boundary of the layer-1 cells that no PML    */
+/* covers (MAC_Grid.cpp:521-919; ENGQUIST_ORDER 2, MAC_Grid.cpp:33): order 2 */
+/* on faces, order 1 on edges and corners.  The extrapolated pressures of    */
+/* the exterior (layer-0) neighbours are written into pCurr, where the next  */
+/* step reads them back as pLast; edges and corners also zero the diagonal   */
+/* exterior cell.  Returns p^{n+1} of the cell.  No interface blending (one  */
+/* simulation unit) and no ghost cells next to layer 1 (the caller checks).  */
+/* ======================================================================== */
+static long long abc_at(const oracle *o, const int *ijk, int da, int a, int db, int b, int dc, int c)
+{
+    int q[3] = {ijk[0], ijk[1], ijk[2]};
+    q[da] += a; q[db] += b; q[dc] += c;
+    return pidx(o, q[0], q[1], q[2]);
+}
+
+static double engquist(const oracle *o, const int *ijk, const int *nml, const double *pLast, double *pCurr)
+{
+    const double lam = o->c * o->dt / o->h, lam2 = lam * lam;
+    const long long ci = pidx(o, ijk[0], ijk[1], ijk[2]);
+    const double pc = pCurr[ci], pl = pLast[ci];
+    const int btype = abs(nml[0]) + abs(nml[1]) + abs(nml[2]);
+    if (btype == 1) {
+        int di = 0;
+        while (!nml[di]) ++di;
+        const int dj = (di + 1) % 3, dk = (di + 2) % 3, s = nml[di];
+        const int jm = ijk[dj] > 0 ? -1 : 0, jp = ijk[dj] < o->n[dj] - 1 ? 1 : 0;
+        const int km = ijk[dk] > 0 ? -1 : 0, kp = ijk[dk] < o->n[dk] - 1 ? 1 : 0;
+        const double pl_out = pLast[abc_at(o, ijk, di, s, dj, 0, dk, 0)];
+        const double p_in = pCurr[abc_at(o, ijk, di, -s, dj, 0, dk, 0)];
+        const double pl_in = pLast[abc_at(o, ijk, di, -s, dj, 0, dk, 0)];
+        const double tang = pCurr[abc_at(o, ijk, di, 0, dj, jp, dk, 0)] + pCurr[abc_at(o, ijk, di, 0, dj, jm, dk, 0)] +
+                            pCurr[abc_at(o, ijk, di, 0, dj, 0, dk, kp)] + pCurr[abc_at(o, ijk, di, 0, dj, 0, dk, km)];
+        double p1 = (2. / lam2 + 4. / lam - 6. - 4. * lam) * pc - (1. / lam2 + 2. / lam) * pl + pl_out - pl_in + 2. * p_in +
+                    (lam + 1.) * tang;
+        p1 *= lam2 / (1. + 2. * lam);
+        const double p_out = pl_out + p_in - pl_in - 2. / lam * (p1 + pl - 2. * pc) + lam * (tang - 4. * pc);
+        pCurr[abc_at(o, ijk, di, s, dj, 0, dk, 0)] = p_out;
+        return 2.0 * pc - pl + lam2 * (p_in + tang + p_out - 6. * pc);
+    }
+    if (btype == 2) {
+        int di = 0;
+        while (!nml[di]) ++di;
+        int dj = di + 1;
+        while (!nml[dj]) ++dj;
+        const int dk = (dj != di + 1) ? (di + 1) % 3 : (di + 2) % 3;
+        const int si = nml[di], sj = nml[dj];
+        const int km = ijk[dk] > 0 ? -1 : 0, kp = ijk[dk] < o->n[dk] - 1 ? 1 : 0;
+        const double pi_in = pCurr[abc_at(o, ijk, di, -si, dj, 0, dk, 0)];
+        const double pj_in = pCurr[abc_at(o, ijk, di, 0, dj, -sj, dk, 0)];
+        const double pk = pCurr[abc_at(o, ijk, di, 0, dj, 0, dk, kp)] + pCurr[abc_at(o, ijk, di, 0, dj, 0, dk, km)];
+        double p1 = (2. / lam2 - 6.) * pc + (2.0 * lam - 1.0) / lam2 * pl + 2. * pi_in + 2. * pj_in + pk;
+        p1 *= lam2 / (1. + 2. * lam);
+        const double pi_out = (pl - p1) / lam + pi_in, pj_out = (pl - p1) / lam + pj_in;
+        pCurr[abc_at(o, ijk, di, si, dj, 0, dk, 0)] = pi_out;
+        pCurr[abc_at(o, ijk, di, 0, dj, sj, dk, 0)] = pj_out;
+        pCurr[abc_at(o, ijk, di, si, dj, sj, dk, 0)] = 0.0;
+        return 2.0 * pc - pl + lam2 * (pi_in + pj_in + pk + pi_out + pj_out - 6. * pc);
+    }
+    /* corner */
+    const double pi_in = pCurr[abc_at(o, ijk, 0, -nml[0], 1, 0, 2, 0)];
+    const double pj_in = pCurr[abc_at(o, ijk, 0, 0, 1, -nml[1], 2, 0)];
+    const double pk_in = pCurr[abc_at(o, ijk, 0, 0, 1, 0, 2, -nml[2])];
+    double p1 = (2. / lam2 - 6.) * pc + (3.0 * lam - 1.0) / lam2 * pl + 2. * pi_in + 2. * pj_in + 2. * pk_in;
+    p1 *= lam2 / (1. + 3. * lam);
+    const double pi_out = (pl - p1) / lam + pi_in, pj_out = (pl - p1) / lam + pj_in, pk_out = (pl - p1) / lam + pk_in;
+    pCurr[abc_at(o, ijk, 0, nml[0], 1, 0, 2, 0)] = pi_out;
+    pCurr[abc_at(o, ijk, 0, 0, 1, nml[1], 2, 0)] = pj_out;
+    pCurr[abc_at(o, ijk, 0, 0, 1, 0, 2, nml[2])] = pk_out;
+    pCurr[abc_at(o, ijk, 0, nml[0], 1, nml[1], 2, nml[2])] = 0.0;
+    return 2.0 * pc - pl + lam2 * (pi_in + pj_in + pk_in + pi_out + pj_out + pk_out - 6. * pc);
+}
+
+/* ======================================================================== */
 /* MAC_Grid::PML_pressureUpdateCollocated (MAC_Grid.cpp:481-946), PML slab   */
-/* plus the non-PML bulk cells (the Engquist branch needs no PML, rejected)  */
+/* plus the non-PML bulk cells, the Engquist boundary at layer 1 included    */
 /* ======================================================================== */
 static void pressure_update(oracle *o, double *pLast, double *pCurr, double *pNext, const double *lapCurr, const double *lapLast)
 {
@@ -1100,10 +1177,11 @@ static void pressure_update(oracle *o, double *pLast, double *pCurr, double *pNe
         int ijk[3]; pidx3(o, ci, ijk);
         if (ijk[0] == 0 || ijk[1] == 0 || ijk[2] == 0 || ijk[0] == o->n[0]-1 || ijk[1] == o->n[1]-1 || ijk[2] == o->n[2]-1)
             continue;  /* layer 0: skipped (MAC_Grid.cpp:524-526) */
-        if (ijk[0] == 1 || ijk[1] == 1 || ijk[2] == 1 || ijk[0] == o->n[0]-2 || ijk[1] == o->n[1]-2 || ijk[2] == o->n[2]-2) {
-            fprintf(stderr, "oracle: Engquist ABC branch reached (no PML at layer 1)\n"); abort();
-        }
-        if (o->cavity[ci] && o->history[ci])
+        int nml[3];   /* ScalarField::boundaryCellNormal(cell, 1) (ScalarField.h:202-213) */
+        for (int d = 0; d < 3; ++d) nml[d] = ijk[d] == 1 ? -1 : ijk[d] == o->n[d] - 2 ? 1 : 0;
+        if (nml[0] || nml[1] || nml[2])
+            pNext[ci] = engquist(o, ijk, nml, pLast, pCurr);
+        else if (o->cavity[ci] && o->history[ci])
             pNext[ci] = pLast[ci] = pCurr[ci];
         else if (visc && o->history[ci])
             pNext[ci] = 2.0 * pCurr[ci] - pLast[ci] - lapLast[ci] * kcalp + lapCurr[ci] * (c2_k2 + kcalp);
@@ -1270,7 +1348,10 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
 {
     const fdtd_settings *s = &scene->settings;
     if (s->divisions[0] < 4 || s->divisions[1] < 4 || s->divisions[2] < 4) { set_err("grid too small"); return NULL; }
-    if (!s->use_mesh) { set_err("use_mesh=0 (Engquist boundary, no PML) is not restated"); return NULL; }
+    if (!s->use_mesh && scene->n_objects > 0) {
+        set_err("use_mesh=0 leaves objects unclassified; the restatement takes free-field scenes there");
+        return NULL;
+    }
     if (scene->n_pressure_sources > 64) { set_err("too many pressure sources"); return NULL; }
     oracle *o = (oracle *)calloc(1, sizeof(oracle));
     o->sc = *scene;

@@ -84,3 +84,24 @@ def test_server_budget_cancel_and_reject(hip_lib):
                 continue
             st, ser, _ = srv.take(rid)
             assert np.array_equal(ser, _alone(sbs[i], steps[i]))
+
+
+def test_engquist_scenes_in_batch_and_server(hip_lib):
+    """Scenes with the Engquist boundary (no PML) share launches with PML
+    scenes in a batch and in the server, and give the single-scene series."""
+    sbs, steps = _small_batch(seed=44, n=3)
+    sbs = sbs + [scenes.engquist_free_field(seed=61, div=(30, 34, 38))[0], scenes.engquist_sphere(seed=62, n=40)[0]]
+    steps = steps + [120, 90]
+    print("batch:", [sb.divisions for sb in sbs], steps, flush=True)
+    with Batch(sbs, steps, steps_per_graph=16) as b:
+        b.run()
+        got = [b.series(i) for i in range(len(sbs))]
+    with Server(steps_per_block=16) as srv:
+        ids = [srv.submit(sb, st) for sb, st in zip(sbs, steps)]
+        for rid in ids:
+            assert srv.wait(rid, 120000) == 0
+        served = [srv.take(rid)[1] for rid in ids]
+    for i, sb in enumerate(sbs):
+        ref = _alone(sb, steps[i])
+        assert np.array_equal(got[i], ref), i
+        assert np.array_equal(served[i], ref), i

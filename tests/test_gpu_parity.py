@@ -219,3 +219,55 @@ def test_air_viscosity(hip_lib, case):
         a.step(2)
         with pytest.raises(RuntimeError, match="Laplacian"):
             a.save_snapshot(d + "/s_%s", "0", 0)
+
+
+def _engquist_scene(case):
+    if case == "free_field":
+        return scenes.engquist_free_field(seed=51, div=(40, 44, 48), n_listeners=8)[0]
+    if case == "free_field_use_mesh":
+        return scenes.engquist_free_field(seed=52, div=(37, 41, 45), n_listeners=8, use_mesh=1)[0]
+    if case == "sphere_raster":
+        return scenes.engquist_sphere(seed=53, n=44)[0]
+    return scenes.engquist_sphere(seed=54, n=46, boundary=FDTD_BOUNDARY_FULLY_COUPLED)[0]
+
+
+@pytest.mark.parametrize("case", ["free_field", "free_field_use_mesh", "sphere_raster", "sphere_coupled"])
+def test_engquist_boundary(hip_lib, case):
+    """Scenes without PML cells (use_mesh = 0, or a PML width of zero): the
+    layer-1 cells take the Engquist boundary (MAC_Grid.cpp:521-919) in the
+    boundary tiles, whose listener terms include the face and the (1, 1, 1)
+    corner cells the free-field listeners read; the pulse reaches the
+    boundary well inside the steps run."""
+    sb = _engquist_scene(case)
+    got, ref = _run_pair(sb, 400)
+    err = rel_l2(got, ref)
+    print(case, "Engquist rel L2", err)
+    assert np.all(np.isfinite(got))
+    assert err <= TOL
+    with Solver(sb) as s:
+        info = s.info()
+        assert info["n_pml_cells"] == 0 and info["n_pml_merged_cells"] == 0
+        s.step(400)
+        p = s.pressure()
+    o = Oracle(sb, reclassify_each_step=False)
+    o.step(400, record=False)
+    q = o.pressure().reshape(sb.divisions)
+    o.close()
+    # layer 0 holds boundary state on the device; every updated cell matches
+    inner, qi = p[1:-1, 1:-1, 1:-1], q[1:-1, 1:-1, 1:-1]
+    assert np.linalg.norm(inner - qi) <= 1e-4 * np.linalg.norm(qi)
+
+
+@pytest.mark.parametrize("tile", list(range(10)))
+def test_engquist_tile_plans_and_graphs_agree(hip_lib, tile):
+    sb = _engquist_scene("sphere_raster")
+    print(f"case: Engquist tile plan {tile}", flush=True)
+    with Solver(sb, tile_config=0) as a:
+        a.step(90)
+        ea, sa = a.pressure(), a.series()
+    with Solver(sb, tile_config=tile, steps_per_graph=16 if tile % 2 else 0) as b:
+        b.step(7)
+        b.step(83)
+        eb, sb_ = b.pressure(), b.series()
+    assert np.array_equal(ea, eb)
+    assert np.array_equal(sa, sb_)

@@ -2,7 +2,8 @@
 tile plan and scenes of every kind the engine runs (free field with a
 pressure source, sphere and ellipsoids in both boundary modes, uneven batch
 grids, stream requests), every tile's class matches the launch that runs it,
-every read stays inside the grid, every cell lies in exactly one tile box,
+every read stays inside the grid, every cell lies in exactly one tile box (none of the outermost layer
+with the Engquist boundary),
 object cells only in boxes that skip them, and every object-adjacent record
 in exactly one tile.  A plan that broke any of these would read or write
 outside its arrays on the device."""
@@ -26,6 +27,9 @@ def _scenes():
     stream = scenes.config5_stream(seed=5, n_requests=2, edge_lo=48, edge_hi=64)
     out += [(f"stream{i}", sb) for i, (_, sb, _) in enumerate(stream)]
     out.append(("staggered_36x40x44", scenes.staggered_free_field(seed=7, div=(36, 40, 44))[0]))
+    out.append(("engquist_free_field", scenes.engquist_free_field(seed=8, div=(37, 41, 300))[0]))
+    out.append(("engquist_sphere_coupled", scenes.engquist_sphere(seed=9, n=44,
+                                                                  boundary=capi.FDTD_BOUNDARY_FULLY_COUPLED)[0]))
     return out
 
 
@@ -55,3 +59,40 @@ def test_staggered_scheme_rejects_objects():
     sb.scene.settings.scheme = capi.FDTD_SCHEME_STAGGERED
     assert lib.fdtd_plan_check(C.byref(sb.scene), 0, None) == 6
     assert b"staggered" in lib.fdtd_last_error()
+
+
+def test_engquist_boundary_refusals():
+    """Without PML cells the layer-1 cells take the Engquist boundary; the
+    setups it cannot serve are refused with FDTD_ERR_UNSUPPORTED and a
+    message: objects with use_mesh = 0 (the reference classifies none), a
+    solid cell within three cells of the boundary, a listener reading a
+    cell of layer 0 (never updated), the staggered scheme, and a PML one
+    cell wide (layer 1 next to a PML layer)."""
+    import numpy as np
+    from acoustics_amd.capi import SceneBuffers
+    lib = capi.load_library()
+
+    def status(settings, sb, objects=None, listeners=None):
+        b = SceneBuffers(settings, sb.objects_list if objects is None else objects, sb.impulse_list,
+                         sb.pressure_list, sb.listeners_np if listeners is None else listeners)
+        return lib.fdtd_plan_check(C.byref(b.scene), 0, None), lib.fdtd_last_error()
+
+    ff, _ = scenes.engquist_free_field(seed=8)
+    sp, _ = scenes.engquist_sphere(seed=9)
+    h = ff.settings_dict["cell_size"]
+    assert status(ff.settings_dict, ff)[0] == 0 and status(sp.settings_dict, sp)[0] == 0
+    s = dict(sp.settings_dict, use_mesh=0)
+    assert status(s, sp) == (6, b"use_mesh=0 classifies no object cells; objects need use_mesh=1")
+    o = dict(sp.objects_list[0])
+    o["center"] = tuple(np.array(sp.settings_dict["min_bound"]) + np.array([0.025 + 2.5 * h, 0.11, 0.11]))
+    st, msg = status(sp.settings_dict, sp, objects=[o], listeners=np.zeros((0, 3)))
+    assert st == 6 and b"three cells" in msg
+    mb = np.array(ff.settings_dict["min_bound"])
+    st, msg = status(ff.settings_dict, ff, listeners=np.array([mb + 1.2 * h]))
+    assert st == 6 and b"outermost layer" in msg
+    s = dict(ff.settings_dict, scheme=capi.FDTD_SCHEME_STAGGERED)
+    st, msg = status(s, ff)
+    assert st == 6 and b"collocated" in msg
+    s = dict(sp.settings_dict, pml_width=1.0)
+    st, msg = status(s, sp)
+    assert st == 6 and b"two outermost cell layers" in msg
