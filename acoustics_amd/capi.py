@@ -41,7 +41,9 @@ class Settings(C.Structure):
         ("pml_strength", C.c_double), ("air_viscosity_alpha", C.c_double), ("start_time", C.c_double),
         ("boundary_condition_preset", C.c_int), ("boundary_handling", C.c_int), ("use_mesh", C.c_int),
         ("divisions", C.c_int * 3), ("min_bound", C.c_double * 3),
-        ("scheme", C.c_int), ("reserved", C.c_int),
+        ("scheme", C.c_int), ("time_parallel", C.c_int), ("num_time_chunks", C.c_int), ("reserved", C.c_int),
+        ("overlap_time", C.c_double), ("early_termination_ratio", C.c_double),
+        ("stop_boundary_acc_time", C.c_double),
     ]
 
 
@@ -132,6 +134,10 @@ SIGNATURES = {
     "fdtd_batch_steps_done": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "fdtd_batch_read_listener_series": (C.c_int, [_P, C.c_int, _D]),
     "fdtd_batch_destroy": (C.c_int, [_P]),
+    "fdtd_time_chunk": (C.c_int, [C.POINTER(Settings), C.c_int64, C.c_int, C.POINTER(Settings),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "fdtd_run_time_parallel": (C.c_int, [C.POINTER(Scene), C.c_int64, C.POINTER(BatchOptions), _D,
+                                         C.POINTER(C.c_int64)]),
     "fdtd_server_options_default": (None, [C.POINTER(ServerOptions)]),
     "fdtd_server_create": (C.c_int, [C.POINTER(ServerOptions), _PP]),
     "fdtd_server_submit": (C.c_int, [_P, C.POINTER(Scene), C.c_int, C.POINTER(C.c_int64)]),

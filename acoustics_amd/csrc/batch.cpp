@@ -6,6 +6,7 @@
 // next block boundary.  The tile plan is chosen by an autotuner that times
 // every plan on the batch itself before the first step.
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <climits>
 #include <cstring>
@@ -134,14 +135,101 @@ BatchRunner::~BatchRunner()
 
 }  // namespace fdtd
 
-using namespace fdtd;
-
 struct fdtd_batch {
-    BatchRunner R;
+    fdtd::BatchRunner R;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int device = 0;
 };
+
+namespace fdtd {
+
+fdtd_status run_time_chunks(const fdtd_scene &scene, int64_t num_steps, const fdtd_batch_options *opt,
+                            std::vector<ChunkRun> &out)
+{
+    const fdtd_settings &s0 = scene.settings;
+    if (!s0.time_parallel || s0.num_time_chunks < 1) {
+        set_last_error("time-parallel runs need time_parallel = 1 and num_time_chunks >= 1");
+        return FDTD_ERR_INVALID;
+    }
+    const int K = s0.num_time_chunks, L = scene.n_listeners;
+    std::vector<fdtd_scene> chunks(K, scene);
+    std::vector<int> steps(K);
+    out.assign(K, ChunkRun{});
+    for (int c = 0; c < K; ++c) {
+        int64_t n = 0;
+        fdtd_status st = fdtd_time_chunk(&s0, num_steps, c, &chunks[c].settings, &out[c].first, &n);
+        if (st != FDTD_OK) return st;
+        // rows past the end of the run are not part of its series
+        n = std::min<int64_t>(n, num_steps - out[c].first);
+        if (n > INT_MAX) { set_last_error("a chunk is longer than 2^31 steps"); return FDTD_ERR_INVALID; }
+        steps[c] = (int)n;
+        out[c].steps_run = n;
+        out[c].start = chunks[c].settings.start_time;
+        out[c].stop = chunks[c].settings.stop_boundary_acc_time;
+    }
+    fdtd_batch *b = nullptr;
+    fdtd_status st = fdtd_batch_create(chunks.data(), steps.data(), K, opt, &b);
+    if (st != FDTD_OK) return st;
+    BatchRunner &R = b->R;
+    // SimWorld::TerminationMonitor (SimWorld.h:35-42; SimWorld.cpp:556-590):
+    // after the chunk's sources are off, the step's peak listener pressure
+    // (the running max from 0 of the signed values) counts as loud above
+    // ratio times the chunk's peak and 2e-5 Pa; 0.05 s after the last loud
+    // step the chunk stops (that step included).  The monitor arms only once
+    // a loud step has been seen after the gate.
+    const double window = 0.05, floor_pa = 2e-5, ratio = s0.early_termination_ratio;
+    struct Monitor { double p_chunk = 0, last = DBL_MAX, time = 0; int64_t seen = 0; bool live = true; };
+    std::vector<Monitor> mon(K);
+    for (int c = 0; c < K; ++c) mon[c].time = out[c].start;
+    const int block = R.graph_steps > 0 ? R.graph_steps : 64;
+    std::vector<double> rows;
+    for (;;) {
+        bool any = false;
+        for (int c = 0; c < K; ++c) any = any || (mon[c].live && R.taken < R.targets[c]);
+        if (!any) break;
+        st = R.advance(b->stream, block);
+        if (st != FDTD_OK) break;
+        if (hipStreamSynchronize(b->stream) != hipSuccess) { set_last_error("stream synchronize failed"); st = FDTD_ERR_DEVICE; break; }
+        for (int c = 0; c < K && st == FDTD_OK; ++c) {
+            Monitor &m = mon[c];
+            if (!m.live) continue;
+            const int64_t upto = std::min<int64_t>(R.taken, R.targets[c]);
+            if (upto > m.seen && L) {
+                rows.resize((size_t)(upto - m.seen) * L);
+                st = R.scenes[c]->read_series(m.seen, upto - m.seen, rows.data());
+            }
+            for (int64_t q = m.seen; q < upto && st == FDTD_OK && m.live; ++q) {
+                double peak = 0.0;
+                for (int l = 0; l < L; ++l) peak = std::max(peak, rows[(size_t)(q - m.seen) * L + l]);
+                m.time += chunks[c].settings.time_step;
+                m.p_chunk = std::max(m.p_chunk, peak);
+                if (m.time > out[c].stop) {
+                    if (peak > ratio * m.p_chunk && peak > floor_pa) m.last = m.time;
+                    if (m.time - m.last > window) {
+                        out[c].steps_run = q + 1;
+                        R.targets[c] = std::min<long long>(R.targets[c], q + 1);
+                        m.live = false;
+                    }
+                }
+            }
+            m.seen = upto;
+            if (upto >= R.targets[c]) m.live = false;
+        }
+        if (st != FDTD_OK) break;
+    }
+    for (int c = 0; c < K && st == FDTD_OK; ++c) {
+        out[c].series.assign((size_t)out[c].steps_run * L, 0.0);
+        if (out[c].steps_run > 0 && L) st = R.scenes[c]->read_series(0, out[c].steps_run, out[c].series.data());
+    }
+    fdtd_batch_destroy(b);
+    return st;
+}
+
+}  // namespace fdtd
+
+using namespace fdtd;
+
 
 extern "C" {
 
@@ -223,6 +311,54 @@ fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *ou
     const DeviceScene &D = *b->R.scenes[scene];
     const long long rows = std::min<long long>(b->R.taken, b->R.targets[scene]);
     return D.read_series(0, rows, out);
+}
+
+fdtd_status fdtd_time_chunk(const fdtd_settings *in, int64_t num_steps, int chunk, fdtd_settings *out,
+                            int64_t *first_step, int64_t *chunk_steps)
+{
+    if (!in || !out || num_steps < 0) { set_last_error("bad time-chunk arguments"); return FDTD_ERR_INVALID; }
+    if (!in->time_parallel || in->num_time_chunks < 1) {
+        set_last_error("time chunks need time_parallel = 1 and num_time_chunks >= 1");
+        return FDTD_ERR_INVALID;
+    }
+    if (chunk < 0 || chunk >= in->num_time_chunks || !(in->time_step > 0) || !(in->overlap_time >= 0)) {
+        set_last_error("chunk index, time_step or overlap_time out of range");
+        return FDTD_ERR_INVALID;
+    }
+    // SimWorld.cpp:151-166: the chunk's start and source stop times come from
+    // adding time_step one step at a time, as the solver's clock does
+    const int64_t per = num_steps / in->num_time_chunks;
+    double start = in->start_time;
+    for (int64_t i = 0; i < per * chunk; ++i) start += in->time_step;
+    double stop = start;
+    for (int64_t i = 0; i < per; ++i) stop += in->time_step;
+    *out = *in;
+    out->start_time = start;
+    out->stop_boundary_acc_time = stop;
+    const double steps = (double)per + in->overlap_time / in->time_step;   // int += double, truncated
+    if (first_step) *first_step = per * chunk;
+    if (chunk_steps) *chunk_steps = (int64_t)steps;
+    return FDTD_OK;
+}
+
+fdtd_status fdtd_run_time_parallel(const fdtd_scene *scene, int64_t num_steps, const fdtd_batch_options *opt,
+                                   double *out, int64_t *steps_run)
+{
+    if (!scene || num_steps <= 0 || (!out && scene->n_listeners > 0)) { set_last_error("bad time-parallel arguments"); return FDTD_ERR_INVALID; }
+    std::vector<ChunkRun> chunks;
+    fdtd_status st = run_time_chunks(*scene, num_steps, opt, chunks);
+    if (st != FDTD_OK) return st;
+    const int L = scene->n_listeners;
+    if (L) {
+        // linear superposition, chunk by chunk in order
+        std::fill(out, out + (size_t)num_steps * L, 0.0);
+        for (const ChunkRun &c : chunks)
+            for (int64_t q = 0; q < c.steps_run; ++q)
+                for (int l = 0; l < L; ++l) out[(size_t)(c.first + q) * L + l] += c.series[(size_t)q * L + l];
+    }
+    if (steps_run)
+        for (size_t c = 0; c < chunks.size(); ++c) steps_run[c] = chunks[c].steps_run;
+    return FDTD_OK;
 }
 
 fdtd_status fdtd_batch_destroy(fdtd_batch *b)

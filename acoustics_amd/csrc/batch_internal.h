@@ -36,4 +36,17 @@ struct BatchRunner {
     ~BatchRunner();
 };
 
+// One chunk of a time-parallel run (SimWorld.cpp:141-221): its start and
+// source stop times, its first step in the whole run, the steps it ran and
+// its listener series [steps_run][L].
+struct ChunkRun {
+    double start = 0, stop = 0;
+    int64_t first = 0, steps_run = 0;
+    std::vector<double> series;
+};
+// Runs every chunk of a time-parallel scene as one batch, with the
+// reference's early-termination monitor per chunk.
+fdtd_status run_time_chunks(const fdtd_scene &scene, int64_t num_steps, const fdtd_batch_options *opt,
+                            std::vector<ChunkRun> &chunks);
+
 }  // namespace fdtd

@@ -256,6 +256,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.lam2 = S.lam2;
     D.lam2f = (float)S.lam2;
     D.lam = S.c * S.dt / S.h;
+    D.src_steps = S.src_steps;
     D.lapv = L.lapv;
     D.kcalp = S.kcalp;
     D.c2k2 = S.c2k2;

@@ -116,6 +116,10 @@ struct SceneSetup {
     double h = 0, dt = 0, c = 0, rho = 0, t_start = 0, stop_time = 0;
     double lam2 = 0;             // (c dt / h)^2
     bool visc = false;           // air viscosity: the bulk update keeps the previous step's Laplacian
+    // time-parallel chunk: every source acts only in steps s < src_steps
+    // (-1: always), the first step whose start time, counted by repeated
+    // addition of dt, reaches stop_boundary_acc_time (SimWorld.cpp:474)
+    int64_t src_steps = -1;
     double kcalp = 0, c2k2 = 0;  // c alpha dt and (c dt)^2
     int boundary = 0;
     int scheme = 0;                 // FDTD_SCHEME_*

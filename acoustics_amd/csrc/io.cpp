@@ -7,9 +7,11 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
+#include "batch_internal.h"
 #include "engine.h"
 
 namespace {
@@ -111,6 +113,37 @@ fdtd_status fdtd_run_xml(const char *xml_path, const char *simulator_id, const f
     if (L > 0) {
         st = write_matrix(composite(pattern.c_str(), simulator_id, "listening_position.dat"), sc->listeners, L, 3);
         if (st != FDTD_OK) { fdtd_parsed_free(P); return st; }
+    }
+
+    if (s.time_parallel) {
+        // SimWorld.cpp:354-405, 870-911: each chunk writes its own audio
+        // stream "%05d_all_audio.dat" and "%05d_start_time", "%05d_time_range"
+        // and "%05d_sim-done" under the output pattern; all chunks run here as
+        // one batch (the reference runs a process per chunk)
+        std::vector<fdtd::ChunkRun> chunks;
+        fdtd_batch_options bo;
+        fdtd_batch_options_default(&bo);
+        if (opt) bo.tile_config = opt->tile_config;
+        st = fdtd::run_time_chunks(*sc, steps, &bo, chunks);
+        char tag[32];
+        for (size_t c = 0; c < chunks.size() && st == FDTD_OK; ++c) {
+            const fdtd::ChunkRun &ch = chunks[c];
+            std::snprintf(tag, sizeof tag, "%05d_", (int)c);
+            const std::string audio = composite(pattern.c_str(), nullptr, std::string(tag) + "all_audio.dat");
+            std::ofstream out(audio, std::ios::binary);
+            if (!out) { fdtd::set_last_error("cannot open " + audio + " for writing"); st = FDTD_ERR_INVALID; break; }
+            if (L) out.write(reinterpret_cast<const char *>(ch.series.data()), sizeof(double) * ch.series.size());
+            double end = ch.start;
+            for (int64_t q = 0; q < ch.steps_run; ++q) end += s.time_step;
+            std::ofstream t0(composite(pattern.c_str(), nullptr, std::string(tag) + "start_time"));
+            t0 << std::setprecision(18) << std::fixed << ch.start << std::endl;
+            std::ofstream tr(composite(pattern.c_str(), nullptr, std::string(tag) + "time_range"));
+            tr << std::setprecision(18) << std::fixed << ch.start << std::endl << ch.stop << std::endl << end << std::endl;
+            std::ofstream done(composite(pattern.c_str(), nullptr, std::string(tag) + "sim-done"));
+            if (steps_run) *steps_run += ch.steps_run;
+        }
+        fdtd_parsed_free(P);
+        return st;
     }
 
     fdtd_handle *h = nullptr;

@@ -99,6 +99,7 @@ struct SceneDev {
     // time-dependent amplitudes, one ring row per step (amp_row):
     double *acc_tab;            // [AMP_ROWS][n_obj] normal acceleration a_n(obj, t_s)
     float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
+    long long src_steps;        // time-parallel chunk: sources act in steps < src_steps (-1: always)
 };
 
 // A listener term handled by the tile that produces its sample: kind 0 is

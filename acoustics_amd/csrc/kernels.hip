@@ -18,6 +18,8 @@
 // No MFMA anywhere: a 7-point stencil is not a contraction.
 #include "kernels.h"
 
+#include <climits>
+
 namespace fdtd {
 
 // Pointers read out of a SceneDev are generic to the compiler, which then
@@ -780,15 +782,17 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
     const long long step = (long long)clk[CLK_STEP], target = (long long)clk[CLK_TARGET];
     if (step >= target) return;
     const long long base = min(step + (long long)n, target);
+    // a time-parallel chunk's sources are off from step src_steps on
+    const long long off = S.src_steps < 0 ? LLONG_MAX : S.src_steps;
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_obj; q += blockDim.x) {
         const long long s = base + q / S.n_obj;
         const int o = q % S.n_obj;
-        gmem(S.acc_tab)[amp_row(s) * S.n_obj + o] = impulse_accel(S.isrc, S.n_isrc, o, S.t_start + (double)s * S.dt);
+        gmem(S.acc_tab)[amp_row(s) * S.n_obj + o] = s < off ? impulse_accel(S.isrc, S.n_isrc, o, S.t_start + (double)s * S.dt) : 0.0;
     }
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_psrc; q += blockDim.x) {
         const long long s = base + q / S.n_psrc;
         const int p = q % S.n_psrc;
-        gmem(S.src_tab)[amp_row(s) * S.n_psrc + p] = source_amp(S.psrc[p], S.t_start + (double)s * S.dt, S.dt, S.scheme);
+        gmem(S.src_tab)[amp_row(s) * S.n_psrc + p] = s < off ? source_amp(S.psrc[p], S.t_start + (double)s * S.dt, S.dt, S.scheme) : 0.f;
     }
     __syncthreads();
     if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)base;

@@ -238,6 +238,19 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     S.c2k2 = (S.c * S.dt) * (S.c * S.dt);
     S.boundary = st.boundary_handling;
     S.scheme = st.scheme;
+    S.src_steps = -1;
+    if (st.time_parallel) {
+        // SimWorld::UpdateObjectState switches every source off once the
+        // clock (start time plus dt, added step by step) reaches the stop time
+        if (!std::isfinite(st.stop_boundary_acc_time)) { err = "time_parallel needs a finite stop_boundary_acc_time"; return FDTD_ERR_INVALID; }
+        double t = st.start_time;
+        int64_t k = 0;
+        while (!(t >= st.stop_boundary_acc_time)) {
+            t += st.time_step;
+            if (++k > (int64_t)INT32_MAX) { err = "stop_boundary_acc_time lies more than 2^31 steps after the start"; return FDTD_ERR_INVALID; }
+        }
+        S.src_steps = k;
+    }
     if (st.scheme != FDTD_SCHEME_COLLOCATED && st.scheme != FDTD_SCHEME_STAGGERED) { err = "unknown scheme"; return FDTD_ERR_INVALID; }
     if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0)) {
         err = "the staggered scheme runs free-field scenes only (objects need the reference's mesh distance-field "

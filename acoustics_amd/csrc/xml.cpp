@@ -200,6 +200,13 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
         s.use_mesh = opt_int(solver, "use_mesh", 1) != 0;
         s.boundary_condition_preset = opt_int(solver, "boundary_condition_preset", 0);
         s.start_time = opt_real(solver, "fast_forward_to_event_time", 0.0);
+        // time parallelization (ImpulseResponseParser.cpp:525-533)
+        s.time_parallel = opt_int(solver, "time_parallel", 0) != 0;
+        if (s.time_parallel) {
+            s.num_time_chunks = req_int(solver, "num_time_chunks");
+            s.overlap_time = opt_real(solver, "overlap_time", 6e-3);
+            s.early_termination_ratio = opt_real(solver, "early_termination_ratio", 1e-3);
+        }
         const std::string bh = opt_str(solver, "boundary_handling", "rasterize");
         if (bh == "rasterize") s.boundary_handling = FDTD_BOUNDARY_RASTERIZE;
         else if (bh == "fully_coupled") s.boundary_handling = FDTD_BOUNDARY_FULLY_COUPLED;

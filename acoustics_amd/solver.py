@@ -183,6 +183,40 @@ class Batch:
             pass
 
 
+def time_chunk(settings, num_steps, chunk):
+    """Settings of one time-parallel chunk (fdtd_time_chunk): returns
+    (settings dict, first step in the whole run, steps the chunk runs)."""
+    lib = capi.load_library()
+    src = capi.SceneBuffers(settings).scene.settings
+    out = capi.Settings()
+    first, n = C.c_int64(), C.c_int64()
+    capi.check(lib.fdtd_time_chunk(C.byref(src), int(num_steps), int(chunk), C.byref(out), C.byref(first),
+                                   C.byref(n)), "fdtd_time_chunk")
+    d = dict(settings)
+    d["start_time"] = out.start_time
+    d["stop_boundary_acc_time"] = out.stop_boundary_acc_time
+    return d, first.value, n.value
+
+
+def time_parallel_run(scene_buffers, num_steps, steps_per_graph=32, tile_config=-1):
+    """Runs every chunk of a time-parallel scene as one device batch and
+    sums the chunks' listener series (fdtd_run_time_parallel).  Returns
+    (series [num_steps][L], steps each chunk ran)."""
+    lib = capi.load_library()
+    sb = scene_buffers
+    k = sb.scene.settings.num_time_chunks
+    opt = capi.BatchOptions()
+    lib.fdtd_batch_options_default(C.byref(opt))
+    opt.steps_per_graph = steps_per_graph
+    opt.tile_config = tile_config
+    out = np.zeros((int(num_steps), sb.n_listeners))
+    run = (C.c_int64 * max(1, k))()
+    capi.check(lib.fdtd_run_time_parallel(C.byref(sb.scene), int(num_steps), C.byref(opt),
+                                          out.ctypes.data_as(C.POINTER(C.c_double)), run),
+               "fdtd_run_time_parallel")
+    return out, [run[i] for i in range(k)]
+
+
 class Server:
     """Continuous-batching request loop (fdtd_server_*)."""
 

@@ -113,7 +113,17 @@ typedef struct fdtd_settings {
     int    divisions[3];              /* pressure cells per axis               */
     double min_bound[3];              /* minimum corner of the pressure grid   */
     int    scheme;                    /* FDTD_SCHEME_*                         */
+    /* time-parallel chunks (ImpulseResponseParser.cpp:525-533, SimWorld.cpp:
+     * 151-166): with time_parallel set, every source (pressure sources and
+     * boundary accelerations) acts only in the steps that start before
+     * stop_boundary_acc_time (counted from start_time by repeated addition of
+     * time_step, as the reference's clock runs; SimWorld.cpp:474). */
+    int    time_parallel;             /* time_parallel (0: off)                */
+    int    num_time_chunks;           /* num_time_chunks                       */
     int    reserved;
+    double overlap_time;              /* overlap_time (s), default 6e-3        */
+    double early_termination_ratio;   /* early_termination_ratio, default 1e-3 */
+    double stop_boundary_acc_time;    /* set per chunk by fdtd_time_chunk      */
 } fdtd_settings;
 
 /* Rigid object with an analytic signed distance.  The rotation maps object
@@ -277,6 +287,30 @@ fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *ou
 fdtd_status fdtd_batch_destroy(fdtd_batch *b);
 
 /* ------------------------------------------------------------------------- */
+/* time-parallel chunks (SimWorld.cpp:141-221, 474-478, 556-590)              */
+/* ------------------------------------------------------------------------- */
+/* Settings of chunk `chunk` of a num_steps run split in settings->
+ * num_time_chunks chunks, as SimWorld sets them up for its chunk index
+ * (SimWorld.cpp:151-166): S = num_steps / K steps per chunk; the chunk starts
+ * at start_time + S * chunk steps and keeps its sources on for S steps (both
+ * times by repeated addition of time_step), and runs S + overlap_time /
+ * time_step steps.  first_step gets the chunk's first step in the whole run,
+ * chunk_steps its step count.  Host only. */
+fdtd_status fdtd_time_chunk(const fdtd_settings *settings, int64_t num_steps, int chunk, fdtd_settings *out,
+                            int64_t *first_step, int64_t *chunk_steps);
+/* Runs every chunk of a time-parallel scene (settings.time_parallel = 1) as
+ * one batch on the device (the reference runs one process per chunk) and sums
+ * the chunks' listener series into the run's series by linear superposition:
+ * out [num_steps][L], row r = listener pressures after step r + 1 of the
+ * whole run.  A chunk stops early, after its sources are off, once its
+ * listeners' peak pressure stays below early_termination_ratio times the
+ * chunk's running peak (and below 2e-5 Pa) for 0.05 s of simulated time
+ * (SimWorld.cpp:556-590).  steps_run (NULL or num_time_chunks entries) gets
+ * the steps each chunk ran. */
+fdtd_status fdtd_run_time_parallel(const fdtd_scene *scene, int64_t num_steps, const fdtd_batch_options *opt,
+                                   double *out, int64_t *steps_run);
+
+/* ------------------------------------------------------------------------- */
 /* serving: continuous batching of impact-sound solve requests                */
 /* ------------------------------------------------------------------------- */
 typedef struct fdtd_server fdtd_server;
@@ -328,7 +362,11 @@ fdtd_status fdtd_write_matrix(const char *filename, const double *data, int rows
  * file (input_control_file.xml) and listening_position.dat under the scene's
  * output_pattern, steps until stop_time (or number_of_timesteps), and writes
  * all_audio.dat as one row of L doubles per step (AudioOutput.cpp:55-64,
- * read by scripts/wavesolver_results.py). */
+ * read by scripts/wavesolver_results.py).  A time_parallel scene runs all
+ * its chunks as one batch and writes, per chunk c, what the reference's chunk
+ * process writes (SimWorld.cpp:354-405, 870-911): %05d_all_audio.dat,
+ * %05d_start_time, %05d_time_range and %05d_sim-done; steps_run then counts
+ * the steps of all chunks. */
 fdtd_status fdtd_run_xml(const char *xml_path, const char *simulator_id, const fdtd_options *opt, int64_t *steps_run);
 /* Snapshot of the time-stepping state in the file set of
  * FDTD_AcousticSimulator::_SaveSimulationSnapshot (FDTD_AcousticSimulator.cpp:

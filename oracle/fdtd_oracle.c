@@ -521,6 +521,7 @@ struct oracle {
 
     double t;
     long long step;
+    int evals_off;   /* time-parallel chunk past its stop_boundary_acc_time */
     /* staggered (MAC) scheme: pFull; pdir[3] and v[3] are the full-grid fields */
     int scheme;
     double *pfull;
@@ -778,9 +779,12 @@ static void classify_fast(oracle *o)
 
 /* boundary acceleration vector at a surface point: sum of the object's
  * Gaussian-impulse sources times the outward normal */
+static void update_evals(oracle *o);
+
 static double object_an(const oracle *o, int obj, double t)
 {
     double a = 0.0;
+    if (o->evals_off) return 0.0;   /* FDTD_RigidObject::EvaluateBoundaryAcceleration, _disableEvals */
     for (int i = 0; i < o->sc.n_impulse_sources; ++i) {
         const fdtd_impulse_source *s = &o->isrc[i];
         if (s->object != obj) continue;
@@ -1059,6 +1063,7 @@ static void laplacian_ghost(oracle *o, const double *val, double *lap)
 static double pressure_sources(const oracle *o, const double *x, double t)
 {
     double acc = 0;
+    if (o->evals_off) return 0.0;   /* FDTD_Objects::EvaluatePressureSources, _disableAllEvals */
     for (int i = 0; i < o->sc.n_pressure_sources; ++i) {
         const fdtd_pressure_source *s = &o->psrc[i];
         const double *mn = o->src_bmin[i], *mx = o->src_bmax[i];
@@ -1249,6 +1254,7 @@ static void fetch_listeners(const oracle *o, const double *scalar, double *out)
  * amplitude is the normalisation constant itself. */
 static double pressure_sources_mac(const oracle *o, const double *x, double t)
 {
+    if (o->evals_off) return 0.0;
     const double sqrt_2 = sqrt(2.0);   /* Wavesolver_ConstantsAndTypes.h:75 */
     double acc = 0;
     for (int i = 0; i < o->sc.n_pressure_sources; ++i) {
@@ -1389,6 +1395,8 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     o->pv = (pml_vcell *)malloc(sizeof(pml_vcell) * (o->Nv[0] + o->Nv[1] + o->Nv[2]));
     o->pc_ind = 1; o->lap_ind = 1;
     o->t = s->start_time;
+    o->evals_off = 0;
+    update_evals(o);
     o->reclass = reclassify_each_step;
     o->scheme = s->scheme;
     if (o->scheme == FDTD_SCHEME_STAGGERED) {
@@ -1419,6 +1427,15 @@ void oracle_free(oracle *o)
     free(o);
 }
 
+/* SimWorld::UpdateObjectState (SimWorld.cpp:468-479), after the clock moved
+ * on: a time-parallel chunk switches every source off for good once the
+ * clock reaches its stop_boundary_acc_time */
+static void update_evals(oracle *o)
+{
+    const fdtd_settings *s = &o->sc.settings;
+    if (s->time_parallel && o->t >= s->stop_boundary_acc_time) o->evals_off = 1;
+}
+
 /* PML_WaveSolver::stepCollocated (PML_WaveSolver.cpp:775-826) */
 int oracle_step(oracle *o, int nsteps, double *rows)
 {
@@ -1428,6 +1445,7 @@ int oracle_step(oracle *o, int nsteps, double *rows)
             mac_pressure(o);
             o->t += o->dt;
             o->step += 1;
+            update_evals(o);
             if (rows) fetch_listeners(o, o->pfull, rows + (size_t)st * o->sc.n_listeners);
         }
         return 0;
@@ -1451,6 +1469,7 @@ int oracle_step(oracle *o, int nsteps, double *rows)
         o->lap_ind = (o->lap_ind + 1) % 2;
         o->t += o->dt;
         o->step += 1;
+        update_evals(o);
         if (rows) fetch_listeners(o, o->pc[o->pc_ind], rows + (size_t)st * o->sc.n_listeners);
     }
     return 0;

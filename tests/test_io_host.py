@@ -66,3 +66,29 @@ def test_xml_output_pattern_and_scheme(tmp_path):
     lib.fdtd_parsed_free(p)
     q.write_text(text.replace('scheme="staggered"', 'scheme="leapfrog"'))
     assert lib.fdtd_parse_xml(str(q).encode(), C.byref(p)) == 5
+
+
+def test_xml_time_parallel_keys(tmp_path):
+    """time_parallel, num_time_chunks, overlap_time and early_termination_ratio
+    (ImpulseResponseParser.cpp:525-533), with the reference's defaults."""
+    lib = capi.load_library()
+    text = open(os.path.join(ROOT, "tests", "golden", "scene_sphere.xml")).read()
+    p = C.c_void_p()
+    q = tmp_path / "tp.xml"
+    q.write_text(text.replace("<solver ", '<solver time_parallel="1" num_time_chunks="5" '))
+    assert lib.fdtd_parse_xml(str(q).encode(), C.byref(p)) == 0
+    s = lib.fdtd_parsed_scene(p).contents.settings
+    assert (s.time_parallel, s.num_time_chunks, s.overlap_time, s.early_termination_ratio) == (1, 5, 6e-3, 1e-3)
+    lib.fdtd_parsed_free(p)
+    q.write_text(text.replace("<solver ", '<solver time_parallel="1" num_time_chunks="2" overlap_time="0.001" '
+                                          'early_termination_ratio="0.01" '))
+    assert lib.fdtd_parse_xml(str(q).encode(), C.byref(p)) == 0
+    s = lib.fdtd_parsed_scene(p).contents.settings
+    assert (s.num_time_chunks, s.overlap_time, s.early_termination_ratio) == (2, 0.001, 0.01)
+    lib.fdtd_parsed_free(p)
+    q.write_text(text.replace("<solver ", '<solver time_parallel="1" '))
+    assert lib.fdtd_parse_xml(str(q).encode(), C.byref(p)) == 5       # num_time_chunks is required
+    p = C.c_void_p()
+    assert lib.fdtd_parse_xml(os.path.join(ROOT, "tests", "golden", "scene_sphere.xml").encode(), C.byref(p)) == 0
+    assert lib.fdtd_parsed_scene(p).contents.settings.time_parallel == 0
+    lib.fdtd_parsed_free(p)
