@@ -51,6 +51,7 @@ class Object(C.Structure):
     _fields_ = [
         ("type", C.c_int), ("reserved", C.c_int), ("center", C.c_double * 3),
         ("semi_axes", C.c_double * 3), ("rotation", C.c_double * 9),
+        ("velocity", C.c_double * 3), ("angular_velocity", C.c_double * 3),
     ]
 
 
@@ -220,6 +221,9 @@ class SceneBuffers:
             rot = np.asarray(o.get("rotation", np.eye(3)), dtype=np.float64).reshape(9)
             for q in range(9):
                 self._objs[i].rotation[q] = rot[q]
+            for d in range(3):
+                self._objs[i].velocity[d] = o.get("velocity", (0.0, 0.0, 0.0))[d]
+                self._objs[i].angular_velocity[d] = o.get("angular_velocity", (0.0, 0.0, 0.0))[d]
         self._isrc = (ImpulseSource * max(1, len(self.impulse_list)))()
         for i, s_ in enumerate(self.impulse_list):
             self._isrc[i].object = s_["object"]

@@ -127,13 +127,21 @@ typedef struct fdtd_settings {
 } fdtd_settings;
 
 /* Rigid object with an analytic signed distance.  The rotation maps object
- * coordinates to world coordinates (row-major 3x3, orthonormal). */
+ * coordinates to world coordinates (row-major 3x3, orthonormal).  center and
+ * rotation give the pose at simulation time 0; an object with a nonzero
+ * velocity or angular velocity moves rigidly: c(t) = center + velocity t,
+ * R(t) = Rot(angular_velocity, |angular_velocity| t) rotation, about its
+ * centre (stands in for the rigid-body animator, FDTD_Objects.cpp:551-582).
+ * Moving objects are reclassified every step, with fresh cells filled by
+ * MAC_Grid::InterpolateFreshPressureCell's rule. */
 typedef struct fdtd_object {
     int    type;          /* FDTD_OBJ_SPHERE | FDTD_OBJ_ELLIPSOID        */
     int    reserved;
-    double center[3];     /* world position of the centre (m)            */
+    double center[3];     /* world position of the centre at t = 0 (m)   */
     double semi_axes[3];  /* sphere: radius in semi_axes[0]              */
     double rotation[9];   /* world-from-object rotation, row-major       */
+    double velocity[3];   /* m/s, world frame                            */
+    double angular_velocity[3]; /* rad/s, world frame, about the centre   */
 } fdtd_object;
 
 /* Normal boundary acceleration a_n(t) = A exp(-(t - t_on)^2 / (2 w^2)) on the

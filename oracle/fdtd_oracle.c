@@ -522,6 +522,9 @@ struct oracle {
     double t;
     long long step;
     int evals_off;   /* time-parallel chunk past its stop_boundary_acc_time */
+    fdtd_object *objs0;  /* poses at t = 0 of moving objects                */
+    int moving;          /* some object has a velocity or angular velocity  */
+    int fresh_error;     /* a fresh cell lay inside an object               */
     /* staggered (MAC) scheme: pFull; pdir[3] and v[3] are the full-grid fields */
     int scheme;
     double *pfull;
@@ -780,6 +783,7 @@ static void classify_fast(oracle *o)
 /* boundary acceleration vector at a surface point: sum of the object's
  * Gaussian-impulse sources times the outward normal */
 static void update_evals(oracle *o);
+static void animate_objects(oracle *o, double t);
 
 static double object_an(const oracle *o, int obj, double t)
 {
@@ -1370,6 +1374,12 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     o->listeners = (double *)malloc(sizeof(double) * 3 * (scene->n_listeners + 1));
     memcpy(o->listeners, scene->listeners, sizeof(double) * 3 * scene->n_listeners);
     o->sc.objects = o->objs; o->sc.impulse_sources = o->isrc; o->sc.pressure_sources = o->psrc; o->sc.listeners = o->listeners;
+    o->objs0 = (fdtd_object *)malloc(sizeof(fdtd_object) * (scene->n_objects + 1));
+    memcpy(o->objs0, scene->objects, sizeof(fdtd_object) * scene->n_objects);
+    for (int i = 0; i < scene->n_objects; ++i)
+        for (int d = 0; d < 3; ++d)
+            o->moving |= scene->objects[i].velocity[d] != 0.0 || scene->objects[i].angular_velocity[d] != 0.0;
+    if (o->moving) animate_objects(o, s->start_time);
     for (int i = 0; i < scene->n_pressure_sources; ++i) {
         /* GaussianPressureSource ctor: bbox = pos -/+ ws*2, position = bbox centroid */
         for (int d = 0; d < 3; ++d) {
@@ -1397,7 +1407,7 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     o->t = s->start_time;
     o->evals_off = 0;
     update_evals(o);
-    o->reclass = reclassify_each_step;
+    o->reclass = reclassify_each_step || o->moving;
     o->scheme = s->scheme;
     if (o->scheme == FDTD_SCHEME_STAGGERED) {
         if (scene->n_objects > 0) {
@@ -1422,9 +1432,110 @@ void oracle_free(oracle *o)
     for (int d = 0; d < 3; ++d) { free(o->pdir[d]); free(o->v[d]); }
     free(o->bulk); free(o->ghostc); free(o->pmlc); free(o->cavity); free(o->history); free(o->classified);
     free(o->pp); free(o->pv); free(o->gh); free(o->hkeys); free(o->hvals);
-    free(o->objs); free(o->isrc); free(o->psrc); free(o->listeners);
+    free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
     free(o);
+}
+
+/* Rigid motion with constant velocity v and angular velocity w about the
+ * centre, from the pose at t = 0: c(t) = c0 + v t, R(t) = Rot(w/|w|, |w| t) R0
+ * (Rodrigues' formula); stands in for FDTD_Objects::AnimateObjects with the
+ * animator's GetRigidObjectTransform (FDTD_Objects.cpp:565-582). */
+static void animate_objects(oracle *o, double t)
+{
+    for (int i = 0; i < o->sc.n_objects; ++i) {
+        const fdtd_object *a = &o->objs0[i];
+        fdtd_object *b = &o->objs[i];
+        for (int d = 0; d < 3; ++d) b->center[d] = a->center[d] + a->velocity[d] * t;
+        const double *w = a->angular_velocity;
+        const double wn = sqrt(w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
+        if (wn == 0.0) { memcpy(b->rotation, a->rotation, sizeof(b->rotation)); continue; }
+        const double k[3] = {w[0]/wn, w[1]/wn, w[2]/wn}, th = wn * t, sn = sin(th), cs = 1.0 - cos(th);
+        const double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+        double Rt[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double k2 = 0;
+                for (int m = 0; m < 3; ++m) k2 += K[r*3+m] * K[m*3+c];
+                Rt[r*3+c] = (r == c ? 1.0 : 0.0) + sn * K[r*3+c] + cs * k2;
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double v = 0;
+                for (int m = 0; m < 3; ++m) v += Rt[r*3+m] * a->rotation[m*3+c];
+                b->rotation[r*3+c] = v;
+            }
+    }
+}
+
+/* pose of a moving object at time t (test hook of animate_objects) */
+void oracle_object_pose(const fdtd_object *obj, double t, fdtd_object *out)
+{
+    oracle tmp;
+    memset(&tmp, 0, sizeof tmp);
+    tmp.sc.n_objects = 1;
+    tmp.objs0 = (fdtd_object *)obj;
+    *out = *obj;
+    tmp.objs = out;
+    animate_objects(&tmp, t);
+}
+
+/* MAC_Grid::InterpolateFreshPressureCell (MAC_Grid.cpp:1993-2254), its MLS
+ * branch: a bulk cell without valid history (solid in the previous step)
+ * takes the order-1 MLS value at its image point across the nearest object's
+ * boundary, over the enclosing cells with valid history, plus rho a_n |d|
+ * (d the cell's distance to the boundary, a_n the boundary acceleration along
+ * the erected normal); fewer than four valid cells: the one nearest the image
+ * point, else the first bulk axis neighbour (+x, -x, +y, ...), else zero.
+ * Cells are visited in index order, as in the reference. */
+static void fresh_fill(oracle *o, double *p)
+{
+    if (o->sc.n_objects == 0) return;
+    for (long long ci = 0; ci < o->N; ++ci) {
+        if (o->history[ci] || !o->bulk[ci]) continue;
+        int ijk[3]; double x[3];
+        pidx3(o, ci, ijk); ppos(o, ijk, x);
+        double dist; int id;
+        lowest_object_distance(o, x, &dist, &id);
+        if (dist < 0.0) { o->fresh_error = 1; continue; }   /* DISTANCE_TOLERANCE = 0: the reference throws */
+        double image[3], bp[3], en[3], nb[3];
+        const double travelled = reflect_normal(&o->objs[id], x, image, bp, en, nb);
+        long long cells[8];
+        const int ne = enclosing(o, image, cells);
+        long long valid[8]; int nv = 0;
+        double pts[24], vals[8];
+        for (int q = 0; q < ne; ++q) {
+            if (!o->history[cells[q]]) continue;
+            int cq[3]; pidx3(o, cells[q], cq); ppos(o, cq, &pts[3*nv]);
+            vals[nv] = p[cells[q]];
+            valid[nv++] = cells[q];
+        }
+        if (nv < 4) {
+            if (nv > 0) {
+                double bd = DBL_MAX; long long best = -1;
+                for (int q = 0; q < nv; ++q) {
+                    const double *y = &pts[3*q];
+                    const double d2 = (image[0]-y[0])*(image[0]-y[0]) + (image[1]-y[1])*(image[1]-y[1]) + (image[2]-y[2])*(image[2]-y[2]);
+                    if (d2 < bd) { bd = d2; best = valid[q]; }
+                }
+                p[ci] = p[best];
+            } else {
+                long long best = -1;
+                for (int d = 0; d < 3 && best < 0; ++d)
+                    for (int sg = 1; sg >= -1 && best < 0; sg -= 2) {
+                        int t3[3] = {ijk[0], ijk[1], ijk[2]};
+                        t3[d] += sg;
+                        const long long nbc = pidx(o, t3[0], t3[1], t3[2]);
+                        if (o->bulk[nbc]) best = nbc;
+                    }
+                p[ci] = best >= 0 ? p[best] : 0.0;
+            }
+        } else {
+            const double mls = oracle_mls_lookup(image, nv, pts, vals);
+            const double an = object_an(o, id, o->t) * (nb[0]*en[0] + nb[1]*en[1] + nb[2]*en[2]);
+            p[ci] = mls + o->rho * an * fabs(travelled);
+        }
+    }
 }
 
 /* SimWorld::UpdateObjectState (SimWorld.cpp:468-479), after the clock moved
@@ -1456,8 +1567,12 @@ int oracle_step(oracle *o, int nsteps, double *rows)
         double *pNext = o->pc[(o->pc_ind + 1) % 3];
         double *lapLast = o->lap[(o->lap_ind + 1) % 2];
         double *lapCurr = o->lap[o->lap_ind];
-        if (o->reclass) classify_fast(o);
-        else {
+        if (o->reclass) {
+            classify_fast(o);
+            fresh_fill(o, pLast);
+            fresh_fill(o, pCurr);
+            if (o->fresh_error) { set_err("a fresh cell lies inside an object"); return -1; }
+        } else {
             memcpy(o->history, o->bulk, (size_t)o->N);
             memset(o->cavity, 0, (size_t)o->N);
         }
@@ -1471,6 +1586,7 @@ int oracle_step(oracle *o, int nsteps, double *rows)
         o->step += 1;
         update_evals(o);
         if (rows) fetch_listeners(o, o->pc[o->pc_ind], rows + (size_t)st * o->sc.n_listeners);
+        if (o->moving) animate_objects(o, o->t);   /* SimWorld::UpdateObjectState after the step */
     }
     return 0;
 }

@@ -81,6 +81,10 @@ double  oracle_vandermonde_weights(const double *rows, int s_i, const double *bp
 double  oracle_pml_sigma(const fdtd_settings *s, const double *x, int dim);
 int     oracle_inside_pml(const fdtd_settings *s, const double *x);
 
+/* Pose at time t of an object moving with its velocity and angular velocity
+ * (test hook of the per-step object motion). */
+void oracle_object_pose(const fdtd_object *obj, double t, fdtd_object *out);
+
 #ifdef __cplusplus
 }
 #endif

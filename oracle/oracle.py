@@ -51,6 +51,8 @@ def lib():
         L.oracle_object_closest.restype = C.c_double
         L.oracle_object_closest.argtypes = [C.POINTER(Object), D, D, D]
         L.oracle_reflect.restype = C.c_double
+        L.oracle_object_pose.argtypes = [C.POINTER(Object), C.c_double, C.POINTER(Object)]
+        L.oracle_object_pose.restype = None
         L.oracle_reflect.argtypes = [C.POINTER(Object), D, D, D, D]
         L.oracle_vandermonde_weights.restype = C.c_double
         L.oracle_vandermonde_weights.argtypes = [D, C.c_int, D, D, D, D, C.c_double, D]
@@ -80,10 +82,14 @@ class Oracle:
         nl = self.sb.n_listeners
         if record and nl:
             rows = np.zeros((nsteps, nl), dtype=np.float64)
-            self.L.oracle_step(self.h, nsteps, rows.ctypes.data_as(C.POINTER(C.c_double)))
+            self._check(self.L.oracle_step(self.h, nsteps, rows.ctypes.data_as(C.POINTER(C.c_double))))
             return rows
-        self.L.oracle_step(self.h, nsteps, None)
+        self._check(self.L.oracle_step(self.h, nsteps, None))
         return None
+
+    def _check(self, status):
+        if status != 0:
+            raise RuntimeError("oracle step failed: " + self.L.oracle_error().decode())
 
     def pressure(self):
         out = np.zeros(self.sb.n_cells, dtype=np.float64)
