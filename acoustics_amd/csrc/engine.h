@@ -12,23 +12,14 @@
 #include <vector>
 
 #include "../../include/fdtd_acoustics.h"
+#include "geom.h"
 
 namespace fdtd {
 
 // --------------------------------------------------------------------------
 // geometry (analytic rigid objects; stand-in for FDTD_RigidObject queries)
 // --------------------------------------------------------------------------
-struct Geom {
-    int type;
-    double c[3];
-    double a[3];
-    double R[9];  // world-from-object, row-major
-};
 Geom make_geom(const fdtd_object &o);
-// signed distance (negative inside), closest surface point, outward unit normal
-double geom_closest(const Geom &g, const double *x, double *xb, double *n);
-// strict inside test (distance < 0)
-bool geom_inside(const Geom &g, const double *x);
 void geom_aabb(const Geom &g, double *mn, double *mx);
 
 // --------------------------------------------------------------------------
