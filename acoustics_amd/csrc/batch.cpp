@@ -25,6 +25,7 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
     std::vector<GhostDesc> ghost;
+    std::vector<MotionDesc> motion;
     members.clear();
     for (size_t i = 0; i < scenes.size(); ++i) {
         if (!scenes[i] || done[i]) continue;
@@ -32,10 +33,11 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
         devs.push_back(scenes[i]->dev);
         scenes[i]->append_tiles(id, tile_plan(plan), tiles, terms);
         if (scenes[i]->needs_ghost_kernel()) ghost.push_back({id});
+        if (scenes[i]->needs_motion_kernel()) motion.push_back({id});
         members.push_back((int)i);
     }
     if (devs.empty()) { prog.release(); return FDTD_OK; }
-    return prog.upload(devs, tiles, terms, ghost, st, tile_plan(plan).fused);
+    return prog.upload(devs, tiles, terms, ghost, st, tile_plan(plan).fused, motion);
 }
 
 void BatchRunner::drop_graph()
@@ -265,6 +267,10 @@ fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, in
         std::string err;
         fdtd_status st = build_setup(scenes[i], ds->S, err);
         if (st != FDTD_OK) { set_last_error("scene " + std::to_string(i) + ": " + err); return st; }
+        if (ds->S.moving && num_steps[i] > ds->S.mv_steps) {
+            set_last_error("scene " + std::to_string(i) + ": moving objects are planned up to stop_time");
+            return FDTD_ERR_INVALID;
+        }
         st = ds->allocate_and_upload(b->stream, std::max(1, num_steps[i]), num_steps[i]);
         if (st != FDTD_OK) { ds->release(); R.release_scenes(); return st; }
         R.scenes.push_back(std::move(ds));

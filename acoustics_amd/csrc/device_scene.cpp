@@ -49,8 +49,18 @@ struct Layout {
     unsigned long long *clock;
     double *acc_tab;
     float *src_tab;
+    uint8_t *ucls;
+    MoveDev *mobj;
+    Geom *pose;
     SceneDev *self;
 };
+
+// cells of a moving scene's reclassified box
+static size_t moving_cells(const SceneSetup &S)
+{
+    if (!S.moving) return 0;
+    return (size_t)(S.mv_hi[0] - S.mv_lo[0]) * (S.mv_hi[1] - S.mv_lo[1]) * (S.mv_hi[2] - S.mv_lo[2]);
+}
 
 long long pad_cells(long long N) { return (N + 63) / 64 * 64; }
 
@@ -113,6 +123,9 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.clock = c.take<unsigned long long>(CLK_WORDS);
     L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
     L.src_tab = c.take<float>((size_t)AMP_ROWS * S.psrc.size());
+    L.ucls = c.take<uint8_t>(2 * moving_cells(S));
+    L.mobj = c.take<MoveDev>(S.moving ? S.objs.size() : 0);
+    L.pose = c.take<Geom>(S.moving ? S.objs.size() : 0);
     L.self = c.take<SceneDev>(1);
 }
 
@@ -149,6 +162,8 @@ fdtd_status DeviceScene::reset_to_start(hipStream_t st)
         FDTD_HIP_TRY(hipMemsetAsync(dev.gval[0], 0, S.ghosts.size() * 8, st));
         FDTD_HIP_TRY(hipMemsetAsync(dev.gval[1], 0, S.ghosts.size() * 8, st));
     }
+    if (S.moving)
+        FDTD_HIP_TRY(hipMemcpyAsync(dev.ucls + moving_cells(S), S.mv_cls0.data(), S.mv_cls0.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     return FDTD_OK;
 }
@@ -199,7 +214,22 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
         skip.assign((size_t)npad, 0);
         for (int64_t i = 0; i < S.N; ++i) skip[to_dev(i)] = S.bulk[i] ? 0 : 1;
         for (const SpecialRec &r : S.special) skip[to_dev(r.cell)] = 1;
+        // a moving scene's reclassified box belongs to its own tiles
+        if (S.moving)
+            for (int i = S.mv_lo[0]; i < S.mv_hi[0]; ++i)
+                for (int j = S.mv_lo[1]; j < S.mv_hi[1]; ++j)
+                    for (int k = S.mv_lo[2]; k < S.mv_hi[2]; ++k) skip[(size_t)i * S.n[1] * sy_dev + (size_t)j * sy_dev + k] = 1;
         FDTD_HIP_TRY(up(L.skip, skip.data(), skip.size()));
+    }
+    if (S.moving) {
+        std::vector<MoveDev> mv(S.objs.size());
+        for (size_t q = 0; q < S.objs.size(); ++q) {
+            mv[q].g0 = S.objs[q];
+            for (int d = 0; d < 3; ++d) { mv[q].v[d] = S.mv_v[3 * q + d]; mv[q].w[d] = S.mv_w[3 * q + d]; }
+        }
+        FDTD_HIP_TRY(up(L.mobj, mv.data(), mv.size() * sizeof(MoveDev)));
+        // the flags of step -1 (row 1): the classification at the start pose
+        FDTD_HIP_TRY(up(L.ucls + moving_cells(S), S.mv_cls0.data(), S.mv_cls0.size()));
     }
     FDTD_HIP_TRY(up(L.gtab, S.gtab.data(), S.gtab.size() * 4));
     FDTD_HIP_TRY(up(L.slabs, S.slabs.data(), S.slabs.size() * sizeof(PmlSlab)));
@@ -288,6 +318,17 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.n_obj = (int)S.objs.size();
     D.acc_tab = L.acc_tab;
     D.src_tab = L.src_tab;
+    D.mv = S.moving ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        D.u_lo[d] = S.mv_lo[d];
+        D.u_n[d] = S.moving ? S.mv_hi[d] - S.mv_lo[d] : 0;
+        D.bmin[d] = S.bmin[d];
+    }
+    D.ucls = L.ucls;
+    D.mobj = L.mobj;
+    D.pose = L.pose;
+    D.h = S.h;
+    D.rho = S.rho;
     self = L.self;
     FDTD_HIP_TRY(hipMemcpyAsync(self, &D, sizeof D, hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(fill_amplitudes(st));
@@ -357,7 +398,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     auto special_hit = [&](const int *lo, const int *hi) {
         for (const SpecialBox &bx : S.boxes)
             if (box_overlap(lo, hi, bx.lo, bx.hi)) return true;
-        return false;
+        return S.moving && box_overlap(lo, hi, S.mv_lo, S.mv_hi);
     };
     // PML slabs first (heaviest tiles start first)
     for (int q = 0; q < (int)S.slabs.size(); ++q) {
@@ -409,6 +450,26 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                         out.push_back(t);
                     }
         }
+    }
+    // moving objects: the reclassified box in tiles of at most 256 cells
+    if (S.moving) {
+        const int *bl = S.mv_lo, *bh = S.mv_hi;
+        const int ek = bh[2] - bl[2], ej = bh[1] - bl[1], ei = bh[0] - bl[0];
+        const int tk = std::min(ek, 256), tj = std::min(ej, std::max(1, 256 / tk));
+        const int ti = std::min(ei, std::max(1, 256 / (tk * tj)));
+        for (int i0 = bl[0]; i0 < bh[0]; i0 += ti)
+            for (int j0 = bl[1]; j0 < bh[1]; j0 += tj)
+                for (int k0 = bl[2]; k0 < bh[2]; k0 += tk) {
+                    TileDesc t{};
+                    t.scene = scene_id;
+                    t.type = TILE_MOVING;
+                    t.i0 = i0; t.i1 = std::min(i0 + ti, bh[0]);
+                    t.j0 = j0; t.j1 = std::min(j0 + tj, bh[1]);
+                    t.k0 = k0; t.k1 = std::min(k0 + tk, bh[2]);
+                    const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+                    t.src = src_hit(lo, hi);
+                    out.push_back(t);
+                }
     }
     // object-adjacent cells
     for (int r0 = 0; r0 < (int)S.special.size(); r0 += 256) {
@@ -549,13 +610,24 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         const SceneDev &d = scenes[t.scene];
         if (t.tb < 0 || t.te < t.tb || (size_t)t.te > nterms) return "tile term range out of range";
         const bool pml_class = t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC;
-        const bool ok_kind = kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
-                             (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL);
+        const bool moving = t.type == TILE_MOVING;
+        const bool ok_kind = moving ? kind == KIND_SPECIAL
+                                    : (kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
+                                       (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL));
         if ((t.type == TILE_MAC || t.type == TILE_MAC_PML) != (d.scheme == FDTD_SCHEME_STAGGERED))
             return "tile class does not match the scene's scheme";
         if (!ok_kind) return "tile class does not match its launch";
         if (t.type == TILE_SPECIAL) {
             if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";
+            continue;
+        }
+        if (moving) {
+            // inside the scene's reclassified box, which keeps a cell from the grid's edge
+            const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+            for (int a = 0; a < 3; ++a)
+                if (!d.mv || lo[a] < d.u_lo[a] || hi[a] > d.u_lo[a] + d.u_n[a] || lo[a] >= hi[a] || lo[a] < 1 || hi[a] > d.n[a] - 1)
+                    return "moving-object tile outside the scene's reclassified box";
+            if ((int64_t)(t.i1 - t.i0) * (t.j1 - t.j0) * (t.k1 - t.k0) > 256) return "moving-object tile too large";
             continue;
         }
         if (t.type == TILE_ABC) {
@@ -601,6 +673,7 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
+        else if (t.type == TILE_MOVING) by_kind[KIND_SPECIAL].push_back(t);   // a launch of their own
         else by_kind[(t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC) ? KIND_PML : KIND_INTERIOR].push_back(t);
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
@@ -628,7 +701,7 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
                                const std::vector<SceneDev> &scenes, size_t nterms)
 {
     for (int k = 0; k < NUM_KINDS; ++k) {
-        const std::string e = check_tiles(by_kind[k], fused_launch ? KIND_ALL : k, scenes, nterms);
+        const std::string e = check_tiles(by_kind[k], (fused_launch && k != KIND_SPECIAL) ? KIND_ALL : k, scenes, nterms);
         if (!e.empty()) return e;
     }
     return std::string();
@@ -636,7 +709,7 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
 
 fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                                 const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
-                                bool fused_launch)
+                                bool fused_launch, const std::vector<MotionDesc> &motion)
 {
     release();
     std::vector<TileDesc> by_kind[NUM_KINDS];
@@ -648,12 +721,17 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     }
     const std::vector<SceneDev> &devs = scenes;
     nscenes = (int)devs.size();
+    for (const MotionDesc &m : motion)
+        if (m.scene < 0 || m.scene >= (int)devs.size() || !devs[m.scene].mv) {
+            set_last_error("step program: motion list names a scene without moving objects");
+            return FDTD_ERR_INVALID;
+        }
     size_t bytes[4 + NUM_KINDS];
     bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, devs.size()));
     bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, terms.size()));
     bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
-    bytes[3] = 0;
-    size_t total = bytes[0] + bytes[1] + bytes[2];
+    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, motion.size()));
+    size_t total = bytes[0] + bytes[1] + bytes[2] + bytes[3];
     for (int k = 0; k < NUM_KINDS; ++k) {
         bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
         total += bytes[4 + k];
@@ -663,7 +741,8 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     d_scenes = reinterpret_cast<SceneDev *>(base);
     d_terms = reinterpret_cast<TermRef *>(base + bytes[0]);
     d_ghost = reinterpret_cast<GhostDesc *>(base + bytes[0] + bytes[1]);
-    size_t off = bytes[0] + bytes[1] + bytes[2];
+    d_motion = reinterpret_cast<MotionDesc *>(base + bytes[0] + bytes[1] + bytes[2]);
+    size_t off = bytes[0] + bytes[1] + bytes[2] + bytes[3];
     for (int k = 0; k < NUM_KINDS; ++k) {
         d_tiles[k] = reinterpret_cast<TileDesc *>(base + off);
         off += bytes[4 + k];
@@ -672,6 +751,8 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
             FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
     }
     nghost = (int)ghost.size();
+    nmotion = (int)motion.size();
+    if (!motion.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, motion.data(), sizeof(MotionDesc) * motion.size(), hipMemcpyHostToDevice, st));
     if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
     if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
@@ -687,7 +768,9 @@ void StepProgram::release()
     for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; }
     d_terms = nullptr;
     d_ghost = nullptr;
+    d_motion = nullptr;
     nghost = 0;
+    nmotion = 0;
     nscenes = 0;
     order_clock = nullptr;
     fused = false;
@@ -695,10 +778,13 @@ void StepProgram::release()
 
 fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
+    if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, k, st));
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {
-        if (fused) return kind == KIND_INTERIOR ? launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s) : hipSuccess;
+        // fused: every class in the interior list, except the moving-object tiles
+        if (fused && kind == KIND_INTERIOR) return launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
+        if (fused && kind == KIND_PML) return hipSuccess;
         return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
     };
     // (running the PML slabs on a second stream beside the interior was

@@ -79,6 +79,7 @@ struct DeviceScene {
     // each the sum of the listener's terms in term order.
     fdtd_status read_series(long long ring_row, long long nrows, double *out) const;
     bool needs_ghost_kernel() const { return S.boundary == FDTD_BOUNDARY_FULLY_COUPLED && !S.ghosts.empty(); }
+    bool needs_motion_kernel() const { return S.moving; }
 };
 
 // Reorders a spatially ordered tile list so workgroup w (dispatched to XCD
@@ -92,8 +93,9 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
 std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
                                const std::vector<SceneDev> &scenes, size_t nterms);
 
-// The launch sequence of a set of scenes, uploaded once.  Per step: ghost
-// solve (fully-coupled scenes) -> the step kernels, which also write the
+// The launch sequence of a set of scenes, uploaded once.  Per step: object
+// motion (scenes with moving objects) -> ghost solve (fully-coupled scenes)
+// -> the step kernels, which also write the
 // listener terms; per block of up to AMP_ROWS steps, one step advance (one
 // workgroup per scene).
 // The step kernels write disjoint cells.  They run back to back on one
@@ -105,16 +107,17 @@ struct StepProgram {
     SceneDev *d_scenes = nullptr;
     TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
     GhostDesc *d_ghost = nullptr;
+    MotionDesc *d_motion = nullptr;
     TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
-    int nghost = 0, nscenes = 0;
+    int nghost = 0, nmotion = 0, nscenes = 0;
     bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
     // tiles of every kind in one list; split by type here (or kept in one
     // launch when `fused_launch` is set)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                        const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
-                       bool fused_launch = false);
+                       bool fused_launch = false, const std::vector<MotionDesc> &motion = {});
     void release();
     // Enqueues n steps from `step` (the host's count of the step the first
     // runs; its parity picks the kernel order of each step): the step

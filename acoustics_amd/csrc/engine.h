@@ -134,7 +134,19 @@ struct SceneSetup {
     std::vector<float> gtab;
     std::vector<ImpulseHost> isrc;
     // objects
-    std::vector<Geom> objs;
+    std::vector<Geom> objs;             // poses at t = 0
+    // moving objects (some object has a velocity or angular velocity): the
+    // cells any object can reach up to stop_time, with a two-cell margin,
+    // form the box [mv_lo, mv_hi); the device reclassifies them every step
+    // (MAC_Grid::classifyCells_FAST) and fills fresh cells
+    // (InterpolateFreshPressureCell).  mv_cls0: bulk flags of the box at the
+    // start pose, z fastest.
+    double bmin[3] = {0, 0, 0};         // minimum corner of the pressure grid
+    bool moving = false;
+    int mv_lo[3] = {0, 0, 0}, mv_hi[3] = {0, 0, 0};
+    std::vector<double> mv_v, mv_w;     // 3 per object
+    int64_t mv_steps = 0;               // steps the planned path covers (start to stop_time)
+    std::vector<uint8_t> mv_cls0;
     std::vector<uint8_t> bulk;          // full grid (host only)
     std::vector<GhostEntry> ghosts;     // sorted by (owner, key)
     std::vector<SpecialRec> special;

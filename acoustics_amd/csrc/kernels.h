@@ -53,6 +53,12 @@ enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_WORDS = 4 };
 enum : int { AMP_ROWS = 64 };
 __host__ __device__ inline int amp_row(long long s) { return (int)(s & (AMP_ROWS - 1)); }
 
+// a moving object: its pose at t = 0 and its constant velocity and angular velocity
+struct MoveDev {
+    Geom g0;
+    double v[3], w[3];
+};
+
 struct SceneDev {
     int n[3];
     int sx, sy;                 // strides of i and j (k fastest)
@@ -100,6 +106,15 @@ struct SceneDev {
     double *acc_tab;            // [AMP_ROWS][n_obj] normal acceleration a_n(obj, t_s)
     float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
     long long src_steps;        // time-parallel chunk: sources act in steps < src_steps (-1: always)
+    // moving objects: the box of cells reclassified every step, their bulk
+    // flags by step parity (row s & 1 holds step s's classification), the
+    // objects' motion and their pose at the step being run
+    int mv;
+    int u_lo[3], u_n[3];
+    uint8_t *ucls;              // [2][u_n0 u_n1 u_n2], z fastest: 1 bulk, 0 solid
+    const MoveDev *mobj;        // [n_obj]
+    Geom *pose;                 // [n_obj], written by motion_kernel each step
+    double h, rho, bmin[3];
 };
 
 // A listener term handled by the tile that produces its sample: kind 0 is
@@ -117,6 +132,7 @@ enum : int {
     TILE_MAC = 6,      // staggered scheme, cells outside the PML
     TILE_MAC_PML = 7,  // staggered scheme, a PML slab (split pressures in the slab's compact storage)
     TILE_ABC = 8,      // Engquist boundary: a box of layer-1 cells, one thread per cell (k fastest)
+    TILE_MOVING = 9,   // a box (at most 256 cells) of a moving scene's reclassified cells, one thread per cell
 };
 
 struct TileDesc {
@@ -134,6 +150,10 @@ struct GhostDesc {
     int scene;
 };
 
+struct MotionDesc {
+    int scene;
+};
+
 // KIND_ALL: one launch carrying every tile class (dispatch per workgroup on
 // the tile type); the other kinds carry one class each.
 enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3, KIND_ALL = 3 };
@@ -148,5 +168,9 @@ hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, 
 // fills the ring.
 hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s);
 hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s);
+// Moving objects, before the step kernels of step clock + k: poses, per-step
+// reclassification of each scene's box, zeroed new solid cells and filled
+// fresh cells; one workgroup per scene.
+hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int k, hipStream_t s);
 
 }  // namespace fdtd

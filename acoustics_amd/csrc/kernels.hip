@@ -18,7 +18,10 @@
 // No MFMA anywhere: a 7-point stencil is not a contraction.
 #include "kernels.h"
 
+#include <cfloat>
 #include <climits>
+
+#include "linalg.h"
 
 namespace fdtd {
 
@@ -760,6 +763,247 @@ __device__ __forceinline__ void abc_tile(const SceneDev &S, const TileDesc &t, c
     if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
 }
 
+// ---------------------------------------------------------------------------
+// moving objects (FP64): per-step reclassification of the cells the objects
+// can reach (MAC_Grid::classifyCells_FAST), fresh cells
+// (MAC_Grid::InterpolateFreshPressureCell, MAC_Grid.cpp:1993-2254) and the
+// ghost-aware update of the fluid cells among them
+// ---------------------------------------------------------------------------
+__device__ inline void mv_centre(const SceneDev &S, int i, int j, int k, double *x)
+{
+    const int ijk[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d) x[d] = S.bmin[d] + (0.5 + (double)ijk[d]) * S.h;
+}
+
+// fluid by the 7-sample rule: the centre and the six points a quarter cell
+// away along the axes all lie outside every object
+__device__ inline bool mv_fluid_at(const SceneDev &S, const double *x)
+{
+    const double q4 = 0.25 * S.h;
+    for (int smp = 0; smp < 7; ++smp) {
+        double y[3] = {x[0], x[1], x[2]};
+        if (smp > 0) y[(smp - 1) >> 1] += ((smp - 1) & 1) ? q4 : -q4;
+        for (int o = 0; o < S.n_obj; ++o)
+            if (geom_inside(S.pose[o], y)) return false;
+    }
+    return true;
+}
+
+__device__ inline int mv_uidx(const SceneDev &S, int i, int j, int k)
+{
+    const int a = i - S.u_lo[0], b = j - S.u_lo[1], c = k - S.u_lo[2];
+    if (a < 0 || b < 0 || c < 0 || a >= S.u_n[0] || b >= S.u_n[1] || c >= S.u_n[2]) return -1;
+    return (a * S.u_n[1] + b) * S.u_n[2] + c;
+}
+
+// LowestObjectDistance (FDTD_Objects.cpp:240-257): the object nearest to x by |distance|
+__device__ inline int mv_nearest(const SceneDev &S, const double *x, double *dist)
+{
+    double best = DBL_MAX;
+    int id = 0;
+    for (int o = 0; o < S.n_obj; ++o) {
+        double xb[3], n[3];
+        const double d = geom_closest(S.pose[o], x, xb, n);
+        if (fabs(d) < fabs(best)) { best = d; id = o; }
+    }
+    *dist = best;
+    return id;
+}
+
+// fresh value of cell (i, j, k) in field p: the cells of the box hold this
+// step's classification in now_f and the previous step's in prev_f (cells
+// outside the box are fluid in both)
+__device__ double mv_fresh(const SceneDev &S, const float *p, const uint8_t *now_f, const uint8_t *prev_f, int i, int j,
+                           int k, long long step)
+{
+    double x[3];
+    mv_centre(S, i, j, k, x);
+    double dist;
+    const int id = mv_nearest(S, x, &dist);
+    if (dist < 0.0) return __builtin_nan("");   // the reference throws: a fresh cell inside an object
+    // ReflectAgainstBoundary (FDTD_RigidObject.cpp:429-536) for a point outside
+    double xb[3], nb[3], en[3], image[3];
+    const double travelled = geom_closest(S.pose[id], x, xb, nb);
+    const double dd = sqrt((xb[0] - x[0]) * (xb[0] - x[0]) + (xb[1] - x[1]) * (xb[1] - x[1]) + (xb[2] - x[2]) * (xb[2] - x[2]));
+    if (dd > 1e-10) for (int e = 0; e < 3; ++e) { en[e] = x[e] - xb[e]; image[e] = x[e] + en[e]; }
+    else for (int e = 0; e < 3; ++e) { en[e] = nb[e]; image[e] = x[e]; }
+    const double enn = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
+    for (int e = 0; e < 3; ++e) en[e] /= enn;
+    // enclosing cells of the image point (ScalarField::enclosingNeighbours), sorted and unique
+    int lo3[3], hi3[3];
+    for (int d = 0; d < 3; ++d) {
+        double diff = image[d] - S.bmin[d];
+        diff -= S.h / 2.0;
+        int v0 = (int)(diff / S.h);
+        v0 = min(max(v0, 0), S.n[d] - 1);
+        lo3[d] = v0;
+        hi3[d] = diff < 0.0 ? 0 : min(v0 + 1, S.n[d] - 1);
+    }
+    int cells[8][3], nc = 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) {
+                const int q[3] = {a ? hi3[0] : lo3[0], b ? hi3[1] : lo3[1], c ? hi3[2] : lo3[2]};
+                bool dup = false;
+                for (int r = 0; r < nc; ++r) dup = dup || (cells[r][0] == q[0] && cells[r][1] == q[1] && cells[r][2] == q[2]);
+                if (!dup) { cells[nc][0] = q[0]; cells[nc][1] = q[1]; cells[nc][2] = q[2]; ++nc; }
+            }
+    // (the enumeration above is already in flat-index order: i, then j, then k)
+    double pts[24], vals[8];
+    int nv = 0;
+    for (int r = 0; r < nc; ++r) {
+        const int u = mv_uidx(S, cells[r][0], cells[r][1], cells[r][2]);
+        if (u >= 0 && !prev_f[u]) continue;   // no valid history
+        mv_centre(S, cells[r][0], cells[r][1], cells[r][2], &pts[3 * nv]);
+        // a cell that turned solid this step was zeroed
+        const bool fluid = u < 0 || now_f[u];
+        vals[nv] = fluid ? (double)p[cells[r][0] * S.sx + cells[r][1] * S.sy + cells[r][2]] : 0.0;
+        ++nv;
+    }
+    if (nv >= 4) {
+        // MLS at the image point plus rho a_n |d|, a_n along the erected normal
+        const double m = mls_value(nv, pts, vals, image, S.h, 0.018);
+        const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
+        const double an = a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]);
+        return m + S.rho * an * fabs(travelled);
+    }
+    if (nv > 0) {
+        double bd = DBL_MAX, v = 0.0;
+        for (int r = 0; r < nv; ++r) {
+            const double *y = &pts[3 * r];
+            const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
+            if (d2 < bd) { bd = d2; v = vals[r]; }
+        }
+        return v;
+    }
+    // no valid enclosing cell: the first fluid axis neighbour (+x, -x, +y, ...);
+    // a neighbour that is itself fresh this step counts with the zero it held
+    const int ijk[3] = {i, j, k};
+    for (int d = 0; d < 3; ++d)
+        for (int sg = 1; sg >= -1; sg -= 2) {
+            int q[3] = {ijk[0], ijk[1], ijk[2]};
+            q[d] += sg;
+            const int u = mv_uidx(S, q[0], q[1], q[2]);
+            if (u >= 0 && !now_f[u]) continue;
+            if (u >= 0 && !prev_f[u]) return 0.0;
+            return (double)p[q[0] * S.sx + q[1] * S.sy + q[2]];
+        }
+    return 0.0;
+}
+
+// One workgroup per moving scene, before the step kernels: the objects' poses
+// at t_s, the box's bulk flags of step s (the 7-sample rule), zeroed pressures
+// of cells that turned solid (in both slots, as classifyCells_FAST zeroes
+// every pressure buffer) and filled fresh cells (p^{s-1}, then p^s).  No
+// thread reads a cell another thread writes: fresh values come from cells
+// with valid history that are still fluid, and the cells that turned solid
+// count as zero without being read.
+__global__ __launch_bounds__(1024) void motion_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
+{
+    const SceneDev &S = scenes[list[blockIdx.x].scene];
+    const long long step = (long long)S.clock[CLK_STEP] + k;
+    if (step >= (long long)S.clock[CLK_TARGET]) return;
+    const double t = S.t_start + (double)step * S.dt;
+    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) S.pose[o] = geom_at(S.mobj[o].g0, S.mobj[o].v, S.mobj[o].w, t);
+    __syncthreads();
+    const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
+    uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
+    const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
+    float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    float *plast = S.field + (size_t)slot_next(step) * S.npad;
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int kk = c % S.u_n[2], r = c / S.u_n[2];
+        const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
+        double x[3];
+        mv_centre(S, i, j, kz, x);
+        now_f[c] = mv_fluid_at(S, x) ? 1 : 0;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int kk = c % S.u_n[2], r = c / S.u_n[2];
+        const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
+        const int idx = i * S.sx + j * S.sy + kz;
+        const bool now = now_f[c], prev = prev_f[c];
+        if (!now && prev) {
+            pcur[idx] = 0.f;
+            plast[idx] = 0.f;
+        } else if (now && !prev) {
+            const double fl = mv_fresh(S, plast, now_f, prev_f, i, j, kz, step);
+            const double fc = mv_fresh(S, pcur, now_f, prev_f, i, j, kz, step);
+            plast[idx] = (float)fl;
+            pcur[idx] = (float)fc;
+        }
+    }
+}
+
+// A box of a moving scene's cells (one thread per cell): solid cells are
+// left alone; a fluid cell takes the ghost-aware Laplacian (MAC_Grid.cpp:
+// 187-299) with rasterized ghost values p + 0.75 h rho a_n (n_b . n_cell) for
+// its solid neighbours (the ghost sits a quarter cell from the solid cell's
+// centre towards this cell; its object is the one nearest the solid cell's
+// centre, n_b the outward normal at the ghost's closest boundary point),
+// and a cell with ghosts on both sides along two axes keeps p^n when it was
+// fluid in the previous step.
+__device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step)
+{
+    const int nk = t.k1 - t.k0, nj = t.j1 - t.j0;
+    const int q = (int)threadIdx.x, r = q / nk;
+    const int k = t.k0 + (q - r * nk), j = t.j0 + r % nj, i = t.i0 + r / nj;
+    if (i >= t.i1) return;
+    const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
+    const uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
+    const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
+    const int u = mv_uidx(S, i, j, k);
+    if (u < 0 || !now_f[u]) return;
+    const AS_GLOBAL float *pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    const int st[3] = {S.sx, S.sy, 1};
+    const int idx = i * S.sx + j * S.sy + k;
+    const double c = pcur[idx];
+    const int ijk[3] = {i, j, k};
+    double lap = 0.0;
+    int surrounded = 0;
+    for (int d = 0; d < 3; ++d) {
+        double g[2], nbv[2];
+        bool solid[2];
+        for (int side = 0; side < 2; ++side) {
+            const int sg = side ? 1 : -1;
+            int m[3] = {ijk[0], ijk[1], ijk[2]};
+            m[d] += sg;
+            const int um = mv_uidx(S, m[0], m[1], m[2]);
+            solid[side] = um >= 0 && !now_f[um];
+            nbv[side] = pcur[idx + sg * st[d]];
+            g[side] = 0.0;
+            if (solid[side]) {
+                double xo[3], gp[3], dist, xb[3], nb[3];
+                mv_centre(S, m[0], m[1], m[2], xo);
+                const int id = mv_nearest(S, xo, &dist);
+                for (int e = 0; e < 3; ++e) gp[e] = xo[e];
+                gp[d] -= sg * 0.25 * S.h;             // towards this cell
+                geom_closest(S.pose[id], gp, xb, nb);
+                const double ncell = -(double)sg;     // from the solid cell to this one, along d
+                const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
+                g[side] = c + 0.75 * S.h * S.rho * a * nb[d] * ncell;
+            }
+        }
+        // side 0 is the negative neighbour, side 1 the positive one
+        if (solid[1] && solid[0]) { lap += (g[1] + g[0] - 2.0 * c) / (9.0 / 16.0); ++surrounded; }
+        else if (solid[1])        lap += (g[1] + 0.75 * nbv[0] - (7.0 / 4.0) * c) / (21.0 / 32.0);
+        else if (solid[0])        lap += (0.75 * nbv[1] + g[0] - (7.0 / 4.0) * c) / (21.0 / 32.0);
+        else                      lap += (nbv[1] + nbv[0] - 2.0 * c);
+    }
+    double pn;
+    if (surrounded > 1 && prev_f[u]) pn = c;   // MAC_Grid.cpp:924-925
+    else pn = 2.0 * c - (double)pnext[idx] + lap * S.lam2;
+    if (t.src)
+        for (int s = 0; s < S.n_psrc; ++s) {
+            const PSrcDev ps = cmem(S.psrc)[s];
+            const AS_CONST float *g3 = cmem(S.gtab);
+            pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
+        }
+    pnext[idx] = (float)pn;
+}
+
 // Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
 // eight and arranged so that workgroup w (dispatched to XCD w % 8) runs chunk
 // w % 8; the reversed order keeps that assignment and reverses each chunk.
@@ -824,7 +1068,10 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         else if (t.type == TILE_ABC) abc_tile(S, t, terms, step);
         else pml_tile(S, t, terms, step);
     } else if constexpr (KIND == KIND_SPECIAL) {
-        special_tile(S, t, terms, step);
+        // the moving-object tiles run in this launch of their own, so their
+        // FP64 geometry does not set the register budget of the streaming tiles
+        if (t.type == TILE_MOVING) moving_tile(S, t, step);
+        else special_tile(S, t, terms, step);
     } else {
         if (t.type == TILE_PML) pml_tile(S, t, terms, step);
         else if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
@@ -894,6 +1141,13 @@ hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_
 {
     if (nscenes <= 0) return hipSuccess;
     hipLaunchKernelGGL(advance_kernel, dim3(nscenes), dim3(256), 0, s, scenes, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int k, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(motion_kernel, dim3(n), dim3(1024), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

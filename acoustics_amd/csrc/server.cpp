@@ -95,6 +95,7 @@ fdtd_status fdtd_server::rebuild()
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
     std::vector<GhostDesc> ghost;
+    std::vector<MotionDesc> motion;
     for (Request *r : active) {
         const int id = (int)devs.size();
         devs.push_back(r->dev->dev);
@@ -102,8 +103,9 @@ fdtd_status fdtd_server::rebuild()
         for (TermRef q : r->terms) { q.scene = id; terms.push_back(q); }
         for (TileDesc t : r->tiles) { t.scene = id; t.tb += tb; t.te += tb; tiles.push_back(t); }
         if (r->dev->needs_ghost_kernel()) ghost.push_back({id});
+        if (r->dev->needs_motion_kernel()) motion.push_back({id});
     }
-    return prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused);
+    return prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused, motion);
 }
 
 void fdtd_server::run()
@@ -215,6 +217,12 @@ int64_t fdtd_scene_device_bytes(const fdtd_scene *scene)
     return (int64_t)DeviceScene::bytes_needed(S, 1);
 }
 
+static fdtd_status fail_early(int plan, const std::string &why)
+{
+    set_last_error("plan " + std::to_string(plan) + ": " + why);
+    return FDTD_ERR_INVALID;
+}
+
 fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n_tiles)
 {
     if (!scene) return FDTD_ERR_INVALID;
@@ -232,6 +240,8 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     d.sx = D.S.n[1] * D.sy_dev;
     d.n_special = (int)D.S.special.size();
     d.scheme = D.S.scheme;
+    d.mv = D.S.moving ? 1 : 0;
+    for (int a = 0; a < 3; ++a) { d.u_lo[a] = D.S.mv_lo[a]; d.u_n[a] = D.S.mv_hi[a] - D.S.mv_lo[a]; }
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
     D.append_tiles(0, tp, tiles, terms);
@@ -241,9 +251,15 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     if (!e.empty()) { set_last_error("plan " + std::to_string(plan) + ": " + e); return FDTD_ERR_INVALID; }
     // every cell lies in exactly one box tile; object cells (solid or next to
     // an object) only in boxes that skip them, and each record in one
-    // object-cell tile
+    // object-cell tile.  A moving scene's reclassified box is written by its
+    // moving-object tiles only: the interior tiles over it skip it.
     std::vector<uint8_t> hit((size_t)D.S.N, 0), skips((size_t)D.S.N, 0);
     std::vector<int> rec_hits(D.S.special.size(), 0);
+    std::vector<uint8_t> in_box((size_t)D.S.N, 0);
+    if (D.S.moving)
+        for (int i = D.S.mv_lo[0]; i < D.S.mv_hi[0]; ++i)
+            for (int j = D.S.mv_lo[1]; j < D.S.mv_hi[1]; ++j)
+                for (int k = D.S.mv_lo[2]; k < D.S.mv_hi[2]; ++k) in_box[((int64_t)i * D.S.n[1] + j) * D.S.n[2] + k] = 1;
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_SPECIAL) {
             for (int r = t.i0; r < t.i1; ++r) ++rec_hits[r];
@@ -253,8 +269,13 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
             for (int j = t.j0; j < t.j1; ++j)
                 for (int k = t.k0; k < t.k1; ++k) {
                     const int64_t c = ((int64_t)i * D.S.n[1] + j) * D.S.n[2] + k;
+                    if (in_box[c] && t.type != TILE_MOVING) {
+                        if (t.type != TILE_INTERIOR_SKIP) return fail_early(plan, "a moving scene's box lies in a tile that would overwrite it");
+                        continue;
+                    }
+                    if (!in_box[c] && t.type == TILE_MOVING) return fail_early(plan, "a moving-object tile leaves its box");
                     ++hit[c];
-                    skips[c] = t.type == TILE_INTERIOR_SKIP;
+                    skips[c] = t.type == TILE_INTERIOR_SKIP || t.type == TILE_MOVING;
                 }
     }
     std::vector<uint8_t> object_cell((size_t)D.S.N, 0);
@@ -312,6 +333,10 @@ fdtd_status fdtd_server_submit(fdtd_server *s, const fdtd_scene *scene, int num_
     std::string err;
     fdtd_status st = build_setup(*scene, r->setup, err);
     if (st != FDTD_OK) { set_last_error(err); return st; }
+    if (r->setup.moving && num_steps > r->setup.mv_steps) {
+        set_last_error("moving objects are planned up to stop_time");
+        return FDTD_ERR_INVALID;
+    }
     r->steps = num_steps;
     r->bytes = DeviceScene::bytes_needed(r->setup, num_steps);
     r->submitted = Clock::now();

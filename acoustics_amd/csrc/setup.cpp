@@ -27,6 +27,7 @@
 #include <numeric>
 
 #include "engine.h"
+#include "linalg.h"
 
 namespace fdtd {
 
@@ -41,68 +42,6 @@ constexpr double kCheckBound = 2.0;       // GAUSSIAN_CHECK_BOUND
 constexpr double kCondLimit = 25.0;       // CONDN_LIM
 constexpr double kMlsRadius = 0.018;      // MLS_FLEISHMAN_RAD
 constexpr double kRefEps = 1e-10;         // EPS, config.h
-
-// ---- small dense linear algebra (symmetric Jacobi eigen-decomposition) ----
-// A (n x n symmetric, row-major) -> eigenvalues lam, eigenvectors V (columns)
-void sym_eigen(int n, const double *A_in, double *lam, double *V)
-{
-    double A[64];
-    std::memcpy(A, A_in, sizeof(double) * n * n);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) V[i * n + j] = (i == j);
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0, diag = 0;
-        for (int i = 0; i < n; ++i) {
-            diag += A[i * n + i] * A[i * n + i];
-            for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
-        }
-        if (off <= 1e-34 * diag || off == 0) break;
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                const double apq = A[p * n + q];
-                if (apq == 0) continue;
-                const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
-                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < n; ++k) {
-                    const double akp = A[k * n + p], akq = A[k * n + q];
-                    A[k * n + p] = c * akp - s * akq;
-                    A[k * n + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double apk = A[p * n + k], aqk = A[q * n + k];
-                    A[p * n + k] = c * apk - s * aqk;
-                    A[q * n + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; ++k) {
-                    const double vkp = V[k * n + p], vkq = V[k * n + q];
-                    V[k * n + p] = c * vkp - s * vkq;
-                    V[k * n + q] = s * vkp + c * vkq;
-                }
-            }
-    }
-    for (int i = 0; i < n; ++i) lam[i] = A[i * n + i];
-}
-
-// pseudo-inverse of a symmetric PSD Gram matrix G = B^T B with the rank rule
-// sigma > sigma_max * rank_scale * eps (sigma = sqrt(lambda)); returns cond.
-double gram_pinv(int n, const double *G, double rank_scale, double *Gp)
-{
-    double lam[8], V[64];
-    sym_eigen(n, G, lam, V);
-    double lmax = 0, lmin = INFINITY;
-    for (int i = 0; i < n; ++i) { lmax = std::max(lmax, lam[i]); lmin = std::min(lmin, lam[i]); }
-    const double smax = std::sqrt(std::max(lmax, 0.0));
-    double thr = smax * rank_scale * DBL_EPSILON;
-    thr = std::max(thr * thr, DBL_MIN);
-    for (int i = 0; i < n * n; ++i) Gp[i] = 0;
-    for (int k = 0; k < n; ++k) {
-        if (!(lam[k] > thr)) continue;
-        for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j) Gp[i * n + j] += V[i * n + k] * V[j * n + k] / lam[k];
-    }
-    return lmin > 0 ? std::sqrt(lmax / lmin) : INFINITY;
-}
 
 // ScalarField::enclosingNeighbours (src/field/ScalarField.cpp:261-286)
 int enclosing_cells(const SceneSetup &S, const double *bmin, const double *p, int64_t *out)
@@ -264,6 +203,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
 
     double bmin[3], bmax[3];
     for (int d = 0; d < 3; ++d) { bmin[d] = st.min_bound[d]; bmax[d] = bmin[d]; bmax[d] += h * (double)S.n[d]; }
+    for (int d = 0; d < 3; ++d) S.bmin[d] = bmin[d];
 
     // ---- PML tables -------------------------------------------------------
     auto inside_axis = [&](int d, double x) {
@@ -433,12 +373,71 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         const int ijk[3] = {i, j, k};
         for (int d = 0; d < 3; ++d) { x[d] = bmin[d]; x[d] += (0.5 + (double)ijk[d]) * h; }
     };
+    S.moving = false;
+    S.mv_v.clear(); S.mv_w.clear(); S.mv_cls0.clear();
+    for (int q = 0; q < sc.n_objects; ++q)
+        for (int d = 0; d < 3; ++d)
+            S.moving = S.moving || sc.objects[q].velocity[d] != 0.0 || sc.objects[q].angular_velocity[d] != 0.0;
+    if (S.moving) {
+        if (st.scheme != FDTD_SCHEME_COLLOCATED || S.boundary != FDTD_BOUNDARY_RASTERIZE || S.visc) {
+            err = "moving objects run with the collocated scheme, rasterized ghost cells and no air viscosity";
+            return FDTD_ERR_UNSUPPORTED;
+        }
+        if (!(st.stop_time > st.start_time)) {
+            err = "moving objects need a stop_time after the start: the cells they sweep are planned up to it";
+            return FDTD_ERR_INVALID;
+        }
+        for (int q = 0; q < sc.n_objects; ++q)
+            for (int d = 0; d < 3; ++d) { S.mv_v.push_back(sc.objects[q].velocity[d]); S.mv_w.push_back(sc.objects[q].angular_velocity[d]); }
+        // steps whose start time t_start + s dt lies within [start, stop]
+        S.mv_steps = (int64_t)std::floor((st.stop_time - st.start_time) / S.dt) + 1;
+        // swept box: the centre's path from start to stop, widened by the
+        // largest semi-axis (the shape turns about its centre)
+        double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int q = 0; q < sc.n_objects; ++q) {
+            const Geom &g = S.objs[q];
+            const double r = std::max(g.a[0], std::max(g.a[1], g.a[2]));
+            for (int d = 0; d < 3; ++d) {
+                const double c0 = g.c[d] + S.mv_v[3 * q + d] * st.start_time, c1 = g.c[d] + S.mv_v[3 * q + d] * st.stop_time;
+                lo[d] = std::min(lo[d], std::min(c0, c1) - r);
+                hi[d] = std::max(hi[d], std::max(c0, c1) + r);
+            }
+        }
+        const int edge = S.abc ? 3 : 0;
+        for (int d = 0; d < 3; ++d) {
+            // cells whose centre lies within two cells of the swept box
+            S.mv_lo[d] = (int)std::floor((lo[d] - bmin[d]) / h - 0.5) - 2;
+            S.mv_hi[d] = (int)std::ceil((hi[d] - bmin[d]) / h - 0.5) + 3;
+            if (S.mv_lo[d] < std::max(S.inner_lo[d], edge) || S.mv_hi[d] > std::min(S.inner_hi[d], S.n[d] - edge)) {
+                err = "a moving object's path (with a two-cell margin) leaves the non-PML interior before stop_time";
+                return FDTD_ERR_UNSUPPORTED;
+            }
+        }
+        // bulk flags at the start pose (7-sample rule, as the device reclassifies)
+        std::vector<Geom> g0(sc.n_objects);
+        for (int q = 0; q < sc.n_objects; ++q) g0[q] = geom_at(S.objs[q], &S.mv_v[3 * q], &S.mv_w[3 * q], st.start_time);
+        const double q4 = 0.25 * h;
+        for (int i = S.mv_lo[0]; i < S.mv_hi[0]; ++i)
+            for (int j = S.mv_lo[1]; j < S.mv_hi[1]; ++j)
+                for (int k = S.mv_lo[2]; k < S.mv_hi[2]; ++k) {
+                    double x[3];
+                    cpos(i, j, k, x);
+                    bool fluid = true;
+                    for (int smp = 0; smp < 7 && fluid; ++smp) {
+                        double y[3] = {x[0], x[1], x[2]};
+                        if (smp > 0) y[(smp - 1) / 2] += ((smp - 1) % 2 ? q4 : -q4);
+                        for (const Geom &g : g0) fluid = fluid && !geom_inside(g, y);
+                    }
+                    S.mv_cls0.push_back(fluid ? 1 : 0);
+                }
+    }
+    const bool static_objs = !S.objs.empty() && !S.moving;
     S.bulk.assign(S.objs.empty() ? 0 : N, 1);
     S.ghosts.clear(); S.special.clear(); S.boxes.clear(); S.cls.clear();
     S.n_solid = 0; S.n_cavity = 0;
     std::vector<int64_t> solid;
     std::vector<std::array<int, 6>> itbox(S.objs.size());
-    if (!S.objs.empty()) {
+    if (static_objs) {
         std::vector<uint8_t> classified(N, 0);
         auto occupied = [&](const double *x) {
             for (const Geom &g : S.objs)
@@ -752,7 +751,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 in |= ijk[0] >= b.lo[0] && ijk[0] < b.hi[0] && ijk[1] >= b.lo[1] && ijk[1] < b.hi[1] && ijk[2] >= b.lo[2] && ijk[2] < b.hi[2];
             if (!in) { err = "internal: solid cell outside every object box"; return FDTD_ERR_INVALID; }
         }
-    } else if (S.n_solid > 0) {
+    } else if (S.n_solid > 0 && static_objs) {
         // solid cells with no bulk neighbour at all cannot happen: the
         // object would fill the grid
         err = "object covers the whole interior";
@@ -772,6 +771,15 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             }
         int64_t cells[8];
         const int ne = enclosing_cells(S, bmin, p, cells);
+        for (int q = 0; q < ne && S.moving; ++q) {
+            const int ijk[3] = {(int)(cells[q] / ((int64_t)ny * nz)), (int)((cells[q] / nz) % ny), (int)(cells[q] % nz)};
+            bool in = true;
+            for (int d = 0; d < 3; ++d) in = in && ijk[d] >= S.mv_lo[d] && ijk[d] < S.mv_hi[d];
+            if (in) {
+                err = "listener " + std::to_string(l) + " reads a cell that a moving object's path reaches";
+                return FDTD_ERR_UNSUPPORTED;
+            }
+        }
         for (int q = 0; q < ne && S.abc; ++q) {
             const int ijk[3] = {(int)(cells[q] / ((int64_t)ny * nz)), (int)((cells[q] / nz) % ny), (int)(cells[q] % nz)};
             for (int d = 0; d < 3; ++d)

@@ -126,7 +126,9 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
         std::vector<GhostDesc> ghost;
         h->D.append_tiles(0, tile_plan(h->tile), tiles, terms);
         if (h->D.needs_ghost_kernel()) ghost.push_back({0});
-        st = h->prog.upload({h->D.dev}, tiles, terms, ghost, h->stream, tile_plan(h->tile).fused);
+        std::vector<MotionDesc> motion;
+        if (h->D.needs_motion_kernel()) motion.push_back({0});
+        st = h->prog.upload({h->D.dev}, tiles, terms, ghost, h->stream, tile_plan(h->tile).fused, motion);
     }
     if (st != FDTD_OK) {
         h->prog.release();
@@ -200,6 +202,10 @@ static fdtd_status build_graph(fdtd_handle *h)
 fdtd_status fdtd_step(fdtd_handle *h, int nsteps)
 {
     if (!h || nsteps < 0) return FDTD_ERR_INVALID;
+    if (h->D.S.moving && h->step + nsteps > h->D.S.mv_steps) {
+        set_last_error("moving objects are planned up to stop_time (" + std::to_string(h->D.S.mv_steps) + " steps)");
+        return FDTD_ERR_INVALID;
+    }
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     const int cap = h->D.series_cap;
     const bool per_step = h->timing == FDTD_TIMING_PER_STEP;
@@ -411,6 +417,10 @@ extern "C" {
 fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.moving) {
+        set_last_error("snapshots do not hold the per-step classification of moving objects");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (h->D.S.visc) {
         set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
                        "(_SaveSimulationSnapshot writes pressures and PML state only)");
@@ -481,6 +491,10 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
 fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering)
 {
     if (!h) return FDTD_ERR_INVALID;
+    if (h->D.S.moving) {
+        set_last_error("snapshots do not hold the per-step classification of moving objects");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (h->D.S.visc) {
         set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
                        "(_SaveSimulationSnapshot writes pressures and PML state only)");

@@ -171,6 +171,45 @@ def engquist_sphere(seed=9, n=44, h=0.005, radius=0.025, shell=0.06, steps=300, 
     return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners), steps
 
 
+def moving_object(seed=10, n=48, h=0.005, pml=8, steps=400, kind="sphere", speed=20.0, spin=0.0, n_listeners=8,
+                  pml_free=False):
+    """One rigid object moving through the grid for `steps` steps (constant
+    velocity of `speed` m/s in a seeded direction, and for an ellipsoid a
+    constant angular velocity of `spin` rad/s about a seeded axis), with a
+    Gaussian impulse (1e3 m/s^2, 1e-4 s, onset 3e-4 s); stop_time bounds the
+    planned path.  Listeners on a shell around the path's midpoint, clear of
+    every cell the object can reach.  pml_free: a PML width of zero (the
+    Engquist boundary) instead of `pml` cells."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, 0 if pml_free else pml)
+    dt = s["time_step"]
+    s["stop_time"] = s["start_time"] + steps * dt
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    v = speed * u
+    T = steps * dt
+    mid = rng.uniform(-0.2, 0.2, size=3) * h
+    c0 = mid - 0.5 * v * T
+    if kind == "sphere":
+        r = 0.02
+        obj = dict(type=FDTD_OBJ_SPHERE, center=tuple(c0), semi_axes=(r, r, r), rotation=np.eye(3), velocity=tuple(v))
+    else:
+        axes = (0.024, 0.012, 0.017)
+        r = max(axes)
+        w = rng.normal(size=3)
+        w = spin * w / np.linalg.norm(w)
+        obj = dict(type=FDTD_OBJ_ELLIPSOID, center=tuple(c0), semi_axes=axes, rotation=random_rotation(rng),
+                   velocity=tuple(v), angular_velocity=tuple(w))
+    src = dict(object=0, amplitude=1e3, width=1e-4, onset=3e-4)
+    # outside the path's box (the object's reach along each axis plus the
+    # two-cell margin and the listener's own interpolation cells)
+    half = 0.5 * np.abs(v) * T + r + 4.5 * h
+    pts = fibonacci_shell(mid, float(np.max(half)) + 1.5 * h, 16 * n_listeners)
+    keep = [p for p in pts if np.any(np.abs(p - mid) > half)]
+    listeners = np.array(keep[::max(1, len(keep) // n_listeners)][:n_listeners])
+    return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners), steps
+
+
 def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
             n_listeners=64, shell=0.2):
     """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse

@@ -28,6 +28,9 @@ def _scenes():
     out += [(f"stream{i}", sb) for i, (_, sb, _) in enumerate(stream)]
     out.append(("staggered_36x40x44", scenes.staggered_free_field(seed=7, div=(36, 40, 44))[0]))
     out.append(("engquist_free_field", scenes.engquist_free_field(seed=8, div=(37, 41, 300))[0]))
+    out.append(("moving_sphere", scenes.moving_object(seed=10)[0]))
+    out.append(("moving_ellipsoid_engquist", scenes.moving_object(seed=11, kind="ellipsoid", speed=12.0, spin=300.0,
+                                                                  pml_free=True, n=46)[0]))
     out.append(("engquist_sphere_coupled", scenes.engquist_sphere(seed=9, n=44,
                                                                   boundary=capi.FDTD_BOUNDARY_FULLY_COUPLED)[0]))
     return out
@@ -96,3 +99,31 @@ def test_engquist_boundary_refusals():
     s = dict(sp.settings_dict, pml_width=1.0)
     st, msg = status(s, sp)
     assert st == 6 and b"two outermost cell layers" in msg
+
+
+def test_moving_object_refusals():
+    """Moving objects need the collocated scheme with rasterized ghost cells
+    and no air viscosity, a stop_time to plan their path up to, a path (with
+    a two-cell margin) inside the non-PML interior, and listeners clear of
+    every cell the path reaches."""
+    import numpy as np
+    from acoustics_amd.capi import SceneBuffers
+    lib = capi.load_library()
+    sb, _ = scenes.moving_object(seed=10)
+
+    def status(settings=None, objects=None, listeners=None):
+        b = SceneBuffers(sb.settings_dict if settings is None else settings, sb.objects_list if objects is None else objects,
+                         sb.impulse_list, sb.pressure_list, sb.listeners_np if listeners is None else listeners)
+        return lib.fdtd_plan_check(C.byref(b.scene), 0, None), lib.fdtd_last_error()
+
+    assert status()[0] == 0
+    s = dict(sb.settings_dict, boundary_handling=capi.FDTD_BOUNDARY_FULLY_COUPLED)
+    assert status(settings=s) == (6, b"moving objects run with the collocated scheme, rasterized ghost cells and no air viscosity")
+    assert status(settings=dict(sb.settings_dict, air_viscosity_alpha=1e-4))[0] == 6
+    st, msg = status(settings=dict(sb.settings_dict, stop_time=0.0))
+    assert st == 1 and b"stop_time" in msg
+    fast = [dict(sb.objects_list[0], velocity=(400.0, 0.0, 0.0))]
+    st, msg = status(objects=fast, listeners=np.zeros((0, 3)))
+    assert st == 6 and b"leaves the non-PML interior" in msg
+    st, msg = status(listeners=np.array([sb.objects_list[0]["center"]]))
+    assert st == 6 and b"path reaches" in msg
