@@ -18,6 +18,9 @@ Prints ONE JSON line:
                state_bytes / state_frac: the same for the bytes the engine's
                merged PML state moves (12 N + 44 N_pml - 24 N_merged)
   staggered    the reference's staggered (MAC) scheme on a 256^3 free field
+  engquist     a 256^3 free field without PML (the Engquist boundary)
+  moving       a 256^3 scene with a moving, spinning ellipsoid, and the same
+               ellipsoid held still
   batch        64 uneven scenes advanced together vs one at a time
   serving      a Poisson request stream: continuous batching vs the plain
                one-request-at-a-time path
@@ -55,6 +58,8 @@ def parse():
     ap.add_argument("--staggered-steps", type=int, default=200,
                     help="steps of the staggered-scheme measurement (0: skip)")
     ap.add_argument("--batch", type=int, default=64, help="scenes in the batched measurement (0: skip)")
+    ap.add_argument("--other-steps", type=int, default=200,
+                    help="steps of the Engquist-boundary and moving-object measurements (0: skip)")
     return ap.parse_args()
 
 
@@ -104,6 +109,53 @@ def staggered(steps, n=256):
             "value": round(info["n_cells"] * steps / wall / 1e9, 3), "unit": "Gcell-updates/s", "steps": steps,
             "bytes_per_step": info["bytes_per_step"], "kernel_ms": round(kern_ms, 5), "step_ms": round(step_ms, 5),
             "achieved_GBps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4), "listeners_finite": finite}
+
+
+def _timed(sb, steps, warm=20):
+    from acoustics_amd import capi
+    from acoustics_amd.solver import Solver
+    with Solver(sb, series_capacity=max(1024, steps + warm + 64)) as s:
+        info = s.info()
+        s.set_timing(capi.FDTD_TIMING_SPAN)
+        s.step(warm)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        _, step_ms = s.last_kernel_ms()
+        finite = bool(np.isfinite(s.series()).all())
+    return info, wall, step_ms, finite
+
+
+def other_paths(steps, n=256):
+    """Two more paths at n^3 (h = 0.64/n): the Engquist boundary on a
+    free field without PML (one pressure pulse, 8 listeners), and a moving
+    ellipsoid (10 m/s, spinning at 200 rad/s, 16-cell PML) beside the same
+    ellipsoid held still.  Whole-job Gcell-updates/s and device time per step
+    (HIP events around the timed steps; for the moving scene this includes
+    the per-step motion kernel and the moving-object tiles)."""
+    from acoustics_amd import scenes
+    from acoustics_amd.capi import SceneBuffers
+    warm = 20
+    out = {}
+    sb, _ = scenes.engquist_free_field(seed=8, div=(n, n, n), h=0.64 / n, n_listeners=8)
+    info, wall, step_ms, finite = _timed(sb, steps, warm)
+    out["engquist"] = {"workload": f"{n}^3 free field, no PML (Engquist boundary on layer 1), 1 pressure source, 8 listeners",
+                       "value": round(info["n_cells"] * steps / wall / 1e9, 3), "unit": "Gcell-updates/s",
+                       "steps": steps, "step_ms": round(step_ms, 5), "listeners_finite": finite}
+    mv, _ = scenes.moving_object(seed=13, n=n, h=0.64 / n, pml=16, steps=steps + warm + 1, kind="ellipsoid",
+                                 speed=10.0, spin=200.0)
+    info, wall, step_ms, finite = _timed(mv, steps, warm)
+    still = SceneBuffers(mv.settings_dict, [dict(o, velocity=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0))
+                                            for o in mv.objects_list], mv.impulse_list, mv.pressure_list, mv.listeners_np)
+    _, wall0, step0_ms, _ = _timed(still, steps, warm)
+    out["moving"] = {"workload": f"{n}^3 MAC grid, 16-cell PML, one ellipsoid moving at 10 m/s and spinning at "
+                                 f"200 rad/s (per-step reclassification and fresh cells), 8 listeners",
+                     "value": round(info["n_cells"] * steps / wall / 1e9, 3), "unit": "Gcell-updates/s",
+                     "steps": steps, "step_ms": round(step_ms, 5), "step_ms_static": round(step0_ms, 5),
+                     "value_static": round(info["n_cells"] * steps / wall0 / 1e9, 3), "listeners_finite": finite}
+    return out
 
 
 def batched(n_scenes):
@@ -286,6 +338,8 @@ def main():
     }
     if a.staggered_steps > 0:
         out["staggered"] = staggered(a.staggered_steps, n)
+    if a.other_steps > 0:
+        out.update(other_paths(a.other_steps, n))
     if a.batch > 0:
         out["batch"] = batched(a.batch)
     if a.serve > 0:
