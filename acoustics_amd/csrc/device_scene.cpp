@@ -752,6 +752,9 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     }
     nghost = (int)ghost.size();
     nmotion = (int)motion.size();
+    motion_cells = 0;
+    for (const MotionDesc &m : motion)
+        motion_cells = std::max(motion_cells, devs[m.scene].u_n[0] * devs[m.scene].u_n[1] * devs[m.scene].u_n[2]);
     if (!motion.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, motion.data(), sizeof(MotionDesc) * motion.size(), hipMemcpyHostToDevice, st));
     if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
     if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
@@ -771,6 +774,7 @@ void StepProgram::release()
     d_motion = nullptr;
     nghost = 0;
     nmotion = 0;
+    motion_cells = 0;
     nscenes = 0;
     order_clock = nullptr;
     fused = false;
@@ -778,7 +782,7 @@ void StepProgram::release()
 
 fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
-    if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, k, st));
+    if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, motion_cells, k, st));
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {

@@ -19,6 +19,7 @@ namespace fdtd {
 // --------------------------------------------------------------------------
 // geometry (analytic rigid objects; stand-in for FDTD_RigidObject queries)
 // --------------------------------------------------------------------------
+enum : int { MAX_MOVING_OBJ = 16 };   // objects of a scene with moving objects
 Geom make_geom(const fdtd_object &o);
 void geom_aabb(const Geom &g, double *mn, double *mx);
 

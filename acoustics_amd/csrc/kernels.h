@@ -170,7 +170,8 @@ hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_
 hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s);
 // Moving objects, before the step kernels of step clock + k: poses, per-step
 // reclassification of each scene's box, zeroed new solid cells and filled
-// fresh cells; one workgroup per scene.
-hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int k, hipStream_t s);
+// fresh cells; workgroups of 256 cells along x (max_cells: the largest box of
+// the list), one scene per y.
+hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s);
 
 }  // namespace fdtd

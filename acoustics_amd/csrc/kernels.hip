@@ -777,14 +777,14 @@ __device__ inline void mv_centre(const SceneDev &S, int i, int j, int k, double 
 
 // fluid by the 7-sample rule: the centre and the six points a quarter cell
 // away along the axes all lie outside every object
-__device__ inline bool mv_fluid_at(const SceneDev &S, const double *x)
+__device__ inline bool mv_fluid_at(const SceneDev &S, const Geom *pose, const double *x)
 {
     const double q4 = 0.25 * S.h;
     for (int smp = 0; smp < 7; ++smp) {
         double y[3] = {x[0], x[1], x[2]};
         if (smp > 0) y[(smp - 1) >> 1] += ((smp - 1) & 1) ? q4 : -q4;
         for (int o = 0; o < S.n_obj; ++o)
-            if (geom_inside(S.pose[o], y)) return false;
+            if (geom_inside(pose[o], y)) return false;
     }
     return true;
 }
@@ -799,6 +799,7 @@ __device__ inline int mv_uidx(const SceneDev &S, int i, int j, int k)
 // LowestObjectDistance (FDTD_Objects.cpp:240-257): the object nearest to x by |distance|
 __device__ inline int mv_nearest(const SceneDev &S, const double *x, double *dist)
 {
+    if (S.n_obj == 1) { *dist = 0.0; return 0; }   // the only object (the distance is not used then)
     double best = DBL_MAX;
     int id = 0;
     for (int o = 0; o < S.n_obj; ++o) {
@@ -810,26 +811,28 @@ __device__ inline int mv_nearest(const SceneDev &S, const double *x, double *dis
     return id;
 }
 
-// fresh value of cell (i, j, k) in field p: the cells of the box hold this
-// step's classification in now_f and the previous step's in prev_f (cells
-// outside the box are fluid in both)
-__device__ double mv_fresh(const SceneDev &S, const float *p, const uint8_t *now_f, const uint8_t *prev_f, int i, int j,
-                           int k, long long step)
+// fresh values of cell (i, j, k) in both pressure slots (p^{s-1} in plast,
+// p^s in pcur; one geometry and one set of MLS weights for both): the cells
+// of the box hold this step's classification in now_f and the previous
+// step's in prev_f (cells outside the box are fluid in both)
+__device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcur, const uint8_t *now_f,
+                         const uint8_t *prev_f, int i, int j, int k, long long step, double *vl, double *vc)
 {
     double x[3];
     mv_centre(S, i, j, k, x);
     double dist;
     const int id = mv_nearest(S, x, &dist);
-    if (dist < 0.0) return __builtin_nan("");   // the reference throws: a fresh cell inside an object
     // ReflectAgainstBoundary (FDTD_RigidObject.cpp:429-536) for a point outside
     double xb[3], nb[3], en[3], image[3];
     const double travelled = geom_closest(S.pose[id], x, xb, nb);
+    if (travelled < 0.0) { *vl = *vc = __builtin_nan(""); return; }   // the reference throws: a fresh cell inside an object
     const double dd = sqrt((xb[0] - x[0]) * (xb[0] - x[0]) + (xb[1] - x[1]) * (xb[1] - x[1]) + (xb[2] - x[2]) * (xb[2] - x[2]));
     if (dd > 1e-10) for (int e = 0; e < 3; ++e) { en[e] = x[e] - xb[e]; image[e] = x[e] + en[e]; }
     else for (int e = 0; e < 3; ++e) { en[e] = nb[e]; image[e] = x[e]; }
     const double enn = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
     for (int e = 0; e < 3; ++e) en[e] /= enn;
-    // enclosing cells of the image point (ScalarField::enclosingNeighbours), sorted and unique
+    // enclosing cells of the image point (ScalarField::enclosingNeighbours), in
+    // flat-index order without repeats
     int lo3[3], hi3[3];
     for (int d = 0; d < 3; ++d) {
         double diff = image[d] - S.bmin[d];
@@ -839,42 +842,43 @@ __device__ double mv_fresh(const SceneDev &S, const float *p, const uint8_t *now
         lo3[d] = v0;
         hi3[d] = diff < 0.0 ? 0 : min(v0 + 1, S.n[d] - 1);
     }
-    int cells[8][3], nc = 0;
+    double pts[24], vl8[8], vc8[8];
+    int nv = 0;
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int c = 0; c < 2; ++c) {
                 const int q[3] = {a ? hi3[0] : lo3[0], b ? hi3[1] : lo3[1], c ? hi3[2] : lo3[2]};
-                bool dup = false;
-                for (int r = 0; r < nc; ++r) dup = dup || (cells[r][0] == q[0] && cells[r][1] == q[1] && cells[r][2] == q[2]);
-                if (!dup) { cells[nc][0] = q[0]; cells[nc][1] = q[1]; cells[nc][2] = q[2]; ++nc; }
+                if ((a && hi3[0] == lo3[0]) || (b && hi3[1] == lo3[1]) || (c && hi3[2] == lo3[2])) continue;
+                const int u = mv_uidx(S, q[0], q[1], q[2]);
+                if (u >= 0 && !prev_f[u]) continue;   // no valid history
+                mv_centre(S, q[0], q[1], q[2], &pts[3 * nv]);
+                // a cell that turned solid this step was zeroed
+                const bool fluid = u < 0 || now_f[u];
+                const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
+                vl8[nv] = fluid ? (double)plast[qi] : 0.0;
+                vc8[nv] = fluid ? (double)pcur[qi] : 0.0;
+                ++nv;
             }
-    // (the enumeration above is already in flat-index order: i, then j, then k)
-    double pts[24], vals[8];
-    int nv = 0;
-    for (int r = 0; r < nc; ++r) {
-        const int u = mv_uidx(S, cells[r][0], cells[r][1], cells[r][2]);
-        if (u >= 0 && !prev_f[u]) continue;   // no valid history
-        mv_centre(S, cells[r][0], cells[r][1], cells[r][2], &pts[3 * nv]);
-        // a cell that turned solid this step was zeroed
-        const bool fluid = u < 0 || now_f[u];
-        vals[nv] = fluid ? (double)p[cells[r][0] * S.sx + cells[r][1] * S.sy + cells[r][2]] : 0.0;
-        ++nv;
-    }
     if (nv >= 4) {
         // MLS at the image point plus rho a_n |d|, a_n along the erected normal
-        const double m = mls_value(nv, pts, vals, image, S.h, 0.018);
+        double beta[8];
+        mls_weights(nv, pts, image, S.h, 0.018, beta);
+        double ml = 0.0, mc = 0.0;
+        for (int r = 0; r < nv; ++r) { ml += beta[r] * vl8[r]; mc += beta[r] * vc8[r]; }
         const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
-        const double an = a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]);
-        return m + S.rho * an * fabs(travelled);
+        const double add = S.rho * a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]) * fabs(travelled);
+        *vl = ml + add;
+        *vc = mc + add;
+        return;
     }
     if (nv > 0) {
-        double bd = DBL_MAX, v = 0.0;
+        double bd = DBL_MAX;
         for (int r = 0; r < nv; ++r) {
             const double *y = &pts[3 * r];
             const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
-            if (d2 < bd) { bd = d2; v = vals[r]; }
+            if (d2 < bd) { bd = d2; *vl = vl8[r]; *vc = vc8[r]; }
         }
-        return v;
+        return;
     }
     // no valid enclosing cell: the first fluid axis neighbour (+x, -x, +y, ...);
     // a neighbour that is itself fresh this step counts with the zero it held
@@ -885,54 +889,75 @@ __device__ double mv_fresh(const SceneDev &S, const float *p, const uint8_t *now
             q[d] += sg;
             const int u = mv_uidx(S, q[0], q[1], q[2]);
             if (u >= 0 && !now_f[u]) continue;
-            if (u >= 0 && !prev_f[u]) return 0.0;
-            return (double)p[q[0] * S.sx + q[1] * S.sy + q[2]];
+            if (u >= 0 && !prev_f[u]) { *vl = *vc = 0.0; return; }
+            const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
+            *vl = plast[qi];
+            *vc = pcur[qi];
+            return;
         }
-    return 0.0;
+    *vl = *vc = 0.0;
 }
 
-// One workgroup per moving scene, before the step kernels: the objects' poses
-// at t_s, the box's bulk flags of step s (the 7-sample rule), zeroed pressures
-// of cells that turned solid (in both slots, as classifyCells_FAST zeroes
-// every pressure buffer) and filled fresh cells (p^{s-1}, then p^s).  No
-// thread reads a cell another thread writes: fresh values come from cells
-// with valid history that are still fluid, and the cells that turned solid
-// count as zero without being read.
-__global__ __launch_bounds__(1024) void motion_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
+// Moving objects, before the step kernels of step s, in two launches over
+// the box of each moving scene (workgroups along x, scenes along y):
+// motion_classify_kernel takes the objects' poses at t_s (into LDS, and into
+// the scene's pose array for the later kernels) and writes the box's bulk
+// flags by the 7-sample rule into the row of this step's parity;
+// motion_fresh_kernel then zeroes the pressures (both slots, as
+// classifyCells_FAST zeroes every pressure buffer) of cells that turned solid
+// and fills fresh cells (p^{s-1}, then p^s).  No thread of the second launch
+// reads a cell another thread writes: fresh values come from cells with valid
+// history that are still fluid, and cells that turned solid count as zero
+// without being read.
+__global__ __launch_bounds__(256) void motion_classify_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
 {
-    const SceneDev &S = scenes[list[blockIdx.x].scene];
+    const SceneDev &S = scenes[list[blockIdx.y].scene];
     const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const double t = S.t_start + (double)step * S.dt;
-    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) S.pose[o] = geom_at(S.mobj[o].g0, S.mobj[o].v, S.mobj[o].w, t);
-    __syncthreads();
     const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
-    uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
-    const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
-    float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
-    float *plast = S.field + (size_t)slot_next(step) * S.npad;
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int kk = c % S.u_n[2], r = c / S.u_n[2];
-        const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
-        double x[3];
-        mv_centre(S, i, j, kz, x);
-        now_f[c] = mv_fluid_at(S, x) ? 1 : 0;
+    if ((int)blockIdx.x * (int)blockDim.x >= ncell) return;
+    __shared__ Geom pose[MAX_MOVING_OBJ];
+    const double t = S.t_start + (double)step * S.dt;
+    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) {
+        const Geom g = geom_at(S.mobj[o].g0, S.mobj[o].v, S.mobj[o].w, t);
+        pose[o] = g;
+        if (blockIdx.x == 0) S.pose[o] = g;
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int kk = c % S.u_n[2], r = c / S.u_n[2];
-        const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
-        const int idx = i * S.sx + j * S.sy + kz;
-        const bool now = now_f[c], prev = prev_f[c];
-        if (!now && prev) {
-            pcur[idx] = 0.f;
-            plast[idx] = 0.f;
-        } else if (now && !prev) {
-            const double fl = mv_fresh(S, plast, now_f, prev_f, i, j, kz, step);
-            const double fc = mv_fresh(S, pcur, now_f, prev_f, i, j, kz, step);
-            plast[idx] = (float)fl;
-            pcur[idx] = (float)fc;
-        }
+    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (c >= ncell) return;
+    const int kk = c % S.u_n[2], r = c / S.u_n[2];
+    const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1];
+    double x[3];
+    mv_centre(S, i, j, S.u_lo[2] + kk, x);
+    S.ucls[(size_t)(step & 1) * ncell + c] = mv_fluid_at(S, pose, x) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
+{
+    const SceneDev &S = scenes[list[blockIdx.y].scene];
+    const long long step = (long long)S.clock[CLK_STEP] + k;
+    if (step >= (long long)S.clock[CLK_TARGET]) return;
+    const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
+    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (c >= ncell) return;
+    const uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
+    const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
+    const bool now = now_f[c], prev = prev_f[c];
+    if (now == prev) return;
+    const int kk = c % S.u_n[2], r = c / S.u_n[2];
+    const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
+    const int idx = i * S.sx + j * S.sy + kz;
+    float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
+    float *plast = S.field + (size_t)slot_next(step) * S.npad;
+    if (!now) {
+        pcur[idx] = 0.f;
+        plast[idx] = 0.f;
+    } else {
+        double fl, fc;
+        mv_fresh(S, plast, pcur, now_f, prev_f, i, j, kz, step, &fl, &fc);
+        plast[idx] = (float)fl;
+        pcur[idx] = (float)fc;
     }
 }
 
@@ -1144,10 +1169,12 @@ hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int k, hipStream_t s)
+hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(motion_kernel, dim3(n), dim3(1024), 0, s, scenes, list, k);
+    if (n <= 0 || max_cells <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((max_cells + 255) / 256), (unsigned)n);
+    hipLaunchKernelGGL(motion_classify_kernel, grid, dim3(256), 0, s, scenes, list, k);
+    hipLaunchKernelGGL(motion_fresh_kernel, grid, dim3(256), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

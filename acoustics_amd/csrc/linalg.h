@@ -72,11 +72,12 @@ FDTD_HD inline double gram_pinv(int n, const double *G, double rank_scale, doubl
     return lmin > 0 ? sqrt(lmax / lmin) : INFINITY;
 }
 
-// Order-1 MLS value at p over m points (MLSModeInterpolator::lookup with
-// Fleishman weights w = exp(-d^2 / (2 r^2))): the weighted design rows
+// Order-1 MLS weights at p over m <= 8 points (MLSModeInterpolator::lookup
+// with Fleishman weights w = exp(-d^2 / (2 r^2))): the weighted design rows
 // [w, w dx/h, w dy/h, w dz/h] with d = point - p, the Gram pseudo-inverse
-// with the rank rule of gram_pinv, and c0 of the min-norm solution.
-FDTD_HD inline double mls_value(int m, const double *pts, const double *vals, const double *p, double h, double radius)
+// with the rank rule of gram_pinv, and c0 of the min-norm solution as
+// sum_i beta_i val_i.
+FDTD_HD inline void mls_weights(int m, const double *pts, const double *p, double h, double radius, double *beta)
 {
     double A[32], w[8], G[16], Gp[16];
     const double r2 = radius * radius;
@@ -92,13 +93,11 @@ FDTD_HD inline double mls_value(int m, const double *pts, const double *vals, co
             G[4 * a + b] = s;
         }
     gram_pinv(4, G, (double)(m < 4 ? m : 4), Gp);
-    double v = 0;
     for (int i = 0; i < m; ++i) {
         double s = 0;
         for (int b = 0; b < 4; ++b) s += Gp[b] * A[4 * i + b];
-        v += s * w[i] * vals[i];
+        beta[i] = s * w[i];
     }
-    return v;
 }
 
 }  // namespace fdtd

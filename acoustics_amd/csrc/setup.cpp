@@ -383,6 +383,10 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             err = "moving objects run with the collocated scheme, rasterized ghost cells and no air viscosity";
             return FDTD_ERR_UNSUPPORTED;
         }
+        if (sc.n_objects > MAX_MOVING_OBJ) {
+            err = "a scene with moving objects holds at most " + std::to_string((int)MAX_MOVING_OBJ) + " objects";
+            return FDTD_ERR_UNSUPPORTED;
+        }
         if (!(st.stop_time > st.start_time)) {
             err = "moving objects need a stop_time after the start: the cells they sweep are planned up to it";
             return FDTD_ERR_INVALID;
