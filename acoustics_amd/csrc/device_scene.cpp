@@ -52,6 +52,7 @@ struct Layout {
     uint8_t *ucls;
     MoveDev *mobj;
     Geom *pose;
+    double *ughost;
     SceneDev *self;
 };
 
@@ -126,6 +127,7 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.ucls = c.take<uint8_t>(2 * moving_cells(S));
     L.mobj = c.take<MoveDev>(S.moving ? S.objs.size() : 0);
     L.pose = c.take<Geom>(S.moving ? S.objs.size() : 0);
+    L.ughost = c.take<double>(6 * moving_cells(S));
     L.self = c.take<SceneDev>(1);
 }
 
@@ -327,6 +329,7 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.ucls = L.ucls;
     D.mobj = L.mobj;
     D.pose = L.pose;
+    D.ughost = L.ughost;
     D.h = S.h;
     D.rho = S.rho;
     self = L.self;

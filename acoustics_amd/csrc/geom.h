@@ -41,13 +41,16 @@ FDTD_HD inline void geom_world_vec(const Geom &g, const double *y, double *x)
     for (int i = 0; i < 3; ++i) x[i] = g.R[3 * i + 0] * y[0] + g.R[3 * i + 1] * y[1] + g.R[3 * i + 2] * y[2];
 }
 
+// F(t) and F'(t), one division per axis (FP64 division is a long dependent
+// instruction sequence on the GPU)
 FDTD_HD inline double geom_root_f(int m, const double *e, const double *y, double t, double *dF)
 {
     double f = -1.0, d = 0.0;
     for (int i = 0; i < m; ++i) {
-        const double q = e[i] * y[i] / (t + e[i] * e[i]);
+        const double r = 1.0 / (t + e[i] * e[i]);
+        const double q = e[i] * y[i] * r;
         f += q * q;
-        d += -2.0 * q * q / (t + e[i] * e[i]);
+        d += -2.0 * q * q * r;
     }
     if (dF) *dF = d;
     return f;

@@ -113,7 +113,8 @@ struct SceneDev {
     int u_lo[3], u_n[3];
     uint8_t *ucls;              // [2][u_n0 u_n1 u_n2], z fastest: 1 bulk, 0 solid
     const MoveDev *mobj;        // [n_obj]
-    Geom *pose;                 // [n_obj], written by motion_kernel each step
+    Geom *pose;                 // [n_obj], written by motion_classify_kernel each step
+    double *ughost;             // [6][u cells]: ghost term of each solid neighbour of a fluid cell (-x, +x, ..., +z)
     double h, rho, bmin[3];
 };
 

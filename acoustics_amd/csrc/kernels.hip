@@ -904,8 +904,9 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
 // the scene's pose array for the later kernels) and writes the box's bulk
 // flags by the 7-sample rule into the row of this step's parity;
 // motion_fresh_kernel then zeroes the pressures (both slots, as
-// classifyCells_FAST zeroes every pressure buffer) of cells that turned solid
-// and fills fresh cells (p^{s-1}, then p^s).  No thread of the second launch
+// classifyCells_FAST zeroes every pressure buffer) of cells that turned solid,
+// fills fresh cells (p^{s-1}, then p^s) and writes the rasterized ghost term
+// of every solid neighbour of a fluid cell, six threads per cell.  No thread of the second launch
 // reads a cell another thread writes: fresh values come from cells with valid
 // history that are still fluid, and cells that turned solid count as zero
 // without being read.
@@ -939,36 +940,56 @@ __global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__res
     const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
     const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
-    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-    if (c >= ncell) return;
+    const int tid = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (tid >= 6 * ncell) return;
+    // thread (q, c): direction q (-x, +x, -y, +y, -z, +z) of cell c; q = 0
+    // also does the cell's fresh fill or zeroing
+    const int q = tid / ncell, c = tid - q * ncell;
     const uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
     const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
     const bool now = now_f[c], prev = prev_f[c];
-    if (now == prev) return;
     const int kk = c % S.u_n[2], r = c / S.u_n[2];
     const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
-    const int idx = i * S.sx + j * S.sy + kz;
-    float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
-    float *plast = S.field + (size_t)slot_next(step) * S.npad;
-    if (!now) {
-        pcur[idx] = 0.f;
-        plast[idx] = 0.f;
-    } else {
-        double fl, fc;
-        mv_fresh(S, plast, pcur, now_f, prev_f, i, j, kz, step, &fl, &fc);
-        plast[idx] = (float)fl;
-        pcur[idx] = (float)fc;
+    if (q == 0 && now != prev) {
+        const int idx = i * S.sx + j * S.sy + kz;
+        float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
+        float *plast = S.field + (size_t)slot_next(step) * S.npad;
+        if (!now) {
+            pcur[idx] = 0.f;
+            plast[idx] = 0.f;
+        } else {
+            double fl, fc;
+            mv_fresh(S, plast, pcur, now_f, prev_f, i, j, kz, step, &fl, &fc);
+            plast[idx] = (float)fl;
+            pcur[idx] = (float)fc;
+        }
     }
+    // rasterized ghost term of a fluid cell's solid neighbour in direction q:
+    // 0.75 h rho a_n (n_b . n_cell), the ghost a quarter cell from the solid
+    // cell's centre towards this cell, its object the one nearest the solid
+    // cell's centre, n_b the outward normal at the ghost's closest boundary point
+    if (!now) return;
+    const int d = q >> 1, sg = (q & 1) ? 1 : -1;
+    int m[3] = {i, j, kz};
+    m[d] += sg;
+    const int um = mv_uidx(S, m[0], m[1], m[2]);
+    if (um < 0 || now_f[um]) return;
+    double xo[3], gp[3], dist, xb[3], nb[3];
+    mv_centre(S, m[0], m[1], m[2], xo);
+    const int id = mv_nearest(S, xo, &dist);
+    for (int e = 0; e < 3; ++e) gp[e] = xo[e];
+    gp[d] -= sg * 0.25 * S.h;
+    geom_closest(S.pose[id], gp, xb, nb);
+    const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
+    S.ughost[(size_t)q * ncell + c] = 0.75 * S.h * S.rho * a * nb[d] * -(double)sg;
 }
 
 // A box of a moving scene's cells (one thread per cell): solid cells are
 // left alone; a fluid cell takes the ghost-aware Laplacian (MAC_Grid.cpp:
-// 187-299) with rasterized ghost values p + 0.75 h rho a_n (n_b . n_cell) for
-// its solid neighbours (the ghost sits a quarter cell from the solid cell's
-// centre towards this cell; its object is the one nearest the solid cell's
-// centre, n_b the outward normal at the ghost's closest boundary point),
-// and a cell with ghosts on both sides along two axes keeps p^n when it was
-// fluid in the previous step.
+// 187-299) with rasterized ghost values p + (the ghost term motion_fresh_kernel
+// wrote for that direction) for its solid neighbours, and a cell with ghosts
+// on both sides along two axes keeps p^n when it was fluid in the previous
+// step.
 __device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step)
 {
     const int nk = t.k1 - t.k0, nj = t.j1 - t.j0;
@@ -999,17 +1020,7 @@ __device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step
             solid[side] = um >= 0 && !now_f[um];
             nbv[side] = pcur[idx + sg * st[d]];
             g[side] = 0.0;
-            if (solid[side]) {
-                double xo[3], gp[3], dist, xb[3], nb[3];
-                mv_centre(S, m[0], m[1], m[2], xo);
-                const int id = mv_nearest(S, xo, &dist);
-                for (int e = 0; e < 3; ++e) gp[e] = xo[e];
-                gp[d] -= sg * 0.25 * S.h;             // towards this cell
-                geom_closest(S.pose[id], gp, xb, nb);
-                const double ncell = -(double)sg;     // from the solid cell to this one, along d
-                const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
-                g[side] = c + 0.75 * S.h * S.rho * a * nb[d] * ncell;
-            }
+            if (solid[side]) g[side] = c + S.ughost[(size_t)(2 * d + side) * ncell + u];
         }
         // side 0 is the negative neighbour, side 1 the positive one
         if (solid[1] && solid[0]) { lap += (g[1] + g[0] - 2.0 * c) / (9.0 / 16.0); ++surrounded; }
@@ -1173,8 +1184,9 @@ hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, 
 {
     if (n <= 0 || max_cells <= 0) return hipSuccess;
     const dim3 grid((unsigned)((max_cells + 255) / 256), (unsigned)n);
+    const dim3 grid6((unsigned)((6 * max_cells + 255) / 256), (unsigned)n);
     hipLaunchKernelGGL(motion_classify_kernel, grid, dim3(256), 0, s, scenes, list, k);
-    hipLaunchKernelGGL(motion_fresh_kernel, grid, dim3(256), 0, s, scenes, list, k);
+    hipLaunchKernelGGL(motion_fresh_kernel, grid6, dim3(256), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

@@ -92,10 +92,41 @@ FDTD_HD inline void mls_weights(int m, const double *pts, const double *p, doubl
             for (int i = 0; i < m; ++i) s += A[4 * i + a] * A[4 * i + b];
             G[4 * a + b] = s;
         }
-    gram_pinv(4, G, (double)(m < 4 ? m : 4), Gp);
+    // row 0 of G^+: by Cholesky when every pivot keeps more than 1e-10 of its
+    // diagonal (G is then well inside full rank and G^+ = G^-1), else from
+    // the eigen-decomposition with the rank rule
+    double z[4], L[16];
+    bool chol = m >= 4;
+    for (int j = 0; j < 4 && chol; ++j) {
+        double d = G[5 * j];
+        for (int q = 0; q < j; ++q) d -= L[4 * j + q] * L[4 * j + q];
+        if (!(d > 1e-10 * G[5 * j])) { chol = false; break; }
+        L[5 * j] = sqrt(d);
+        for (int i = j + 1; i < 4; ++i) {
+            double v = G[4 * i + j];
+            for (int q = 0; q < j; ++q) v -= L[4 * i + q] * L[4 * j + q];
+            L[4 * i + j] = v / L[5 * j];
+        }
+    }
+    if (chol) {
+        double y[4];   // L y = e0, then L^T z = y
+        for (int i = 0; i < 4; ++i) {
+            double v = i == 0 ? 1.0 : 0.0;
+            for (int q = 0; q < i; ++q) v -= L[4 * i + q] * y[q];
+            y[i] = v / L[5 * i];
+        }
+        for (int i = 3; i >= 0; --i) {
+            double v = y[i];
+            for (int q = i + 1; q < 4; ++q) v -= L[4 * q + i] * z[q];
+            z[i] = v / L[5 * i];
+        }
+    } else {
+        gram_pinv(4, G, (double)(m < 4 ? m : 4), Gp);
+        for (int b = 0; b < 4; ++b) z[b] = Gp[b];
+    }
     for (int i = 0; i < m; ++i) {
         double s = 0;
-        for (int b = 0; b < 4; ++b) s += Gp[b] * A[4 * i + b];
+        for (int b = 0; b < 4; ++b) s += z[b] * A[4 * i + b];
         beta[i] = s * w[i];
     }
 }
