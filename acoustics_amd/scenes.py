@@ -210,6 +210,31 @@ def moving_object(seed=10, n=48, h=0.005, pml=8, steps=400, kind="sphere", speed
     return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners), steps
 
 
+def two_moving_objects(seed=14, n=56, h=0.005, pml=8, steps=360, n_listeners=8):
+    """A sphere and a spinning ellipsoid passing each other (opposite
+    velocities of 12 m/s, 4 cells apart across the path), each with its own
+    impulse, plus a Gaussian pressure pulse between them; listeners outside the
+    box the two paths sweep."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml)
+    dt = s["time_step"]
+    s["stop_time"] = s["start_time"] + steps * dt
+    T = steps * dt
+    v = np.array([12.0, 0.0, 0.0])
+    off = np.array([0.0, 0.0, 0.014 + 2 * h])
+    sph = dict(type=FDTD_OBJ_SPHERE, center=tuple(-0.5 * v * T - off), semi_axes=(0.012, 0.012, 0.012),
+               rotation=np.eye(3), velocity=tuple(v))
+    ell = dict(type=FDTD_OBJ_ELLIPSOID, center=tuple(0.5 * v * T + off), semi_axes=(0.016, 0.009, 0.012),
+               rotation=random_rotation(rng), velocity=tuple(-v), angular_velocity=(0.0, 250.0, 120.0))
+    srcs = [dict(object=0, amplitude=1e3, width=1e-4, onset=2e-4), dict(object=1, amplitude=-8e2, width=1e-4, onset=5e-4)]
+    ps = gaussian_pressure_source((0.0, 0.01, 0.0), 1e-4, offset=4e-4)
+    half = np.array([0.5 * 12.0 * T + 0.016, 0.016, 0.014 + 2 * h + 0.016]) + 4.5 * h
+    pts = fibonacci_shell(np.zeros(3), float(np.max(half)) + 1.5 * h, 16 * n_listeners)
+    keep = [p for p in pts if np.any(np.abs(p) > half)]
+    listeners = np.array(keep[::max(1, len(keep) // n_listeners)][:n_listeners])
+    return SceneBuffers(s, objects=[sph, ell], impulse_sources=srcs, pressure_sources=[ps], listeners=listeners), steps
+
+
 def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
             n_listeners=64, shell=0.2):
     """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse

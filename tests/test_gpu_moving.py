@@ -3,8 +3,10 @@ an object can reach (MAC_Grid::classifyCells_FAST), fresh cells filled by
 InterpolateFreshPressureCell's rule (MAC_Grid.cpp:1993-2254) and the
 ghost-aware update of the fluid cells among them, against the FP64 oracle's
 restatement (tolerance: listener relative L2 <= 1e-4, as every parity test
-here), with a translating sphere, a translating and spinning ellipsoid, and
-a sphere in a grid with the Engquist boundary.  Graph replay, every tile plan,
+here), with a translating sphere, a translating and spinning ellipsoid, a
+sphere in a grid with the Engquist boundary, and two objects passing each
+other (the nearest-object choice for ghosts and fresh cells) beside a
+pressure source.  Graph replay, every tile plan,
 batches and the server must give the single-scene result bit for bit."""
 import numpy as np
 import pytest
@@ -23,10 +25,12 @@ def _scene(case):
         return scenes.moving_object(seed=10)
     if case == "ellipsoid_spinning":
         return scenes.moving_object(seed=11, kind="ellipsoid", speed=12.0, spin=300.0)
+    if case == "two_objects_and_source":
+        return scenes.two_moving_objects(seed=14)
     return scenes.moving_object(seed=12, pml_free=True, n=44)
 
 
-@pytest.mark.parametrize("case", ["sphere", "ellipsoid_spinning", "sphere_engquist"])
+@pytest.mark.parametrize("case", ["sphere", "ellipsoid_spinning", "sphere_engquist", "two_objects_and_source"])
 def test_moving_object_matches_oracle(hip_lib, case):
     sb, steps = _scene(case)
     print(f"case: moving {case} divisions={sb.divisions} steps={steps}", flush=True)
