@@ -489,9 +489,11 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     const int TK = 1 << tk, TJ = 256 >> tk;
     const int kw = b[2] - a[2];
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
-    // float4 rows start at a[2], a multiple of four; the row stride is one
-    // too, and a row's last group may be partial
-    const bool vec = !mac && !S.visc && plan.interior_vec && a[2] % 4 == 0 && kw <= 256;
+    // float4 rows start at the multiple of four at or below a[2] (the cells
+    // before a[2] are read and not written); the row stride is a multiple of
+    // four too, and a row's last group may be partial
+    const int kv0 = a[2] & ~3, lead = a[2] - kv0, kwv = b[2] - kv0;
+    const bool vec = !mac && !S.visc && plan.interior_vec && kwv <= 256;
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
@@ -510,8 +512,8 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     for (int i0 = a[0]; i0 < b[0]; i0 += plan.interior_ti) {
         const int i1 = std::min(i0 + plan.interior_ti, b[0]);
         if (!vec) { scalar_tiles(i0, i1, a[1], b[1]); continue; }
-        // short rows share a wavefront: rpw rows of kw/4 lanes each
-        const int rpw = vec ? std::max(1, std::min(8, 64 / ((kw + 3) / 4))) : 1;
+        // short rows share a wavefront: rpw rows of kwv/4 lanes each
+        const int rpw = vec ? std::max(1, std::min(8, 64 / ((kwv + 3) / 4))) : 1;
         for (int j0 = a[1]; j0 < b[1]; j0 += 4 * rpw) {
             const int j1 = std::min(j0 + 4 * rpw, b[1]);
             const int lo[3] = {i0, j0, a[2]}, hi[3] = {i1, j1, b[2]};
@@ -519,9 +521,10 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
             TileDesc t{};
             t.scene = scene_id;
             t.type = TILE_INTERIOR_VEC;
-            t.i0 = i0; t.i1 = i1; t.j0 = j0; t.j1 = j1; t.k0 = a[2]; t.k1 = b[2];
+            t.i0 = i0; t.i1 = i1; t.j0 = j0; t.j1 = j1; t.k0 = kv0; t.k1 = b[2];
             t.tk_log2 = 6;
             t.rpw = rpw;
+            t.lead = lead;
             out.push_back(t);
         }
     }
@@ -565,7 +568,8 @@ void DeviceScene::assign_terms(int scene_id, size_t base, std::vector<TileDesc> 
         int best = -1;
         for (int q : column[(size_t)i * S.n[1] + j]) {
             const TileDesc &t = out[q];
-            if (k >= t.k0 && k < t.k1 && (best < 0 || q < best)) best = q;
+            const int kfirst = t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0);   // cells the tile writes
+            if (k >= kfirst && k < t.k1 && (best < 0 || q < best)) best = q;
         }
         return best;
     };
@@ -646,7 +650,8 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
             if ((int64_t)(t.i1 - t.i0) * (t.j1 - t.j0) * (t.k1 - t.k0) > 256) return "boundary tile too large";
             continue;
         }
-        const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
+        // a float4 row tile reads from k0 and writes from k0 + lead
+        const int lo[3] = {t.i0, t.j0, t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0)}, hi[3] = {t.i1, t.j1, t.k1};
         for (int a = 0; a < 3; ++a) {
             if (lo[a] >= hi[a]) return "empty tile box";
             if (pml_class) {
@@ -659,7 +664,7 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
             if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) return "PML tile too large";
         } else if (t.type == TILE_INTERIOR_VEC) {
             // a partial last group reads up to k0 + 4 ceil((k1 - k0) / 4) <= the row stride
-            if (t.rpw < 1 || t.j1 - t.j0 > 4 * t.rpw || (t.k0 & 3) || t.k1 - t.k0 > 256 ||
+            if (t.rpw < 1 || t.j1 - t.j0 > 4 * t.rpw || (t.k0 & 3) || t.lead < 0 || t.lead > 3 || t.k1 - t.k0 > 256 ||
                 t.rpw * ((t.k1 - t.k0 + 3) >> 2) > 64 || (d.sy & 3) || t.k0 + ((t.k1 - t.k0 + 3) & ~3) > d.sy)
                 return "float4 row tile misaligned, too wide or with too many rows";
         } else if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) {

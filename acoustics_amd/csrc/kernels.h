@@ -145,6 +145,8 @@ struct TileDesc {
     int slab;                    // TILE_PML: index into SceneDev::slabs
     int tb, te;                  // listener terms [tb, te) of the program's TermRef table
     int rpw;                     // TILE_INTERIOR_VEC: rows per wavefront (the tile holds 4 * rpw rows)
+    int lead;                    // TILE_INTERIOR_VEC: cells at the start of each row's first group (from k0,
+                                 // a multiple of four) that belong to another tile and are not written
 };
 
 struct GhostDesc {

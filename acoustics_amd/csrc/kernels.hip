@@ -192,15 +192,16 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
     }
 }
 
-// Stores the first n (1..4) components of v at p (16-byte aligned).
-__device__ __forceinline__ void store4(AS_GLOBAL float *p, float4 v, int n)
+// Stores components [lo, n) of v at p (16-byte aligned; 0 <= lo < n <= 4).
+__device__ __forceinline__ void store4(AS_GLOBAL float *p, float4 v, int n, int lo = 0)
 {
-    if (n == 4) {
+    if (n == 4 && lo == 0) {
         *(AS_GLOBAL float4 *)p = v;
     } else {
-        p[0] = v.x;
-        if (n > 1) p[1] = v.y;
-        if (n > 2) p[2] = v.z;
+        if (lo == 0) p[0] = v.x;
+        if (n > 1 && lo <= 1) p[1] = v.y;
+        if (n > 2 && lo <= 2) p[2] = v.z;
+        if (n > 3) p[3] = v.w;
     }
 }
 
@@ -228,7 +229,9 @@ __device__ __forceinline__ float4 leapfrog4(float4 c, float4 prev, float4 xm, fl
 __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     // the last lane of a row may hold a partial group: its cells past k1
-    // (PML or row padding) are read, used as k neighbours, and not written
+    // (PML or row padding) are read, used as k neighbours, and not written;
+    // likewise the first lane's first t.lead cells when the row's own cells
+    // start past the float4 boundary k0
     const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
     const int j0w = t.j0 + (int)(threadIdx.x >> 6) * R;   // first row of this wavefront
     if (j0w >= t.j1) return;                                 // (whole wavefront)
@@ -265,15 +268,15 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         if (lane == nk4 - 1) { zr1 = pcur[idx + 4]; zr2 = pcur[idx + sx + 4]; }
         const float4 o1 = leapfrog4(a1, pv1, a0, a2, ym1, yp1, zl1, zr1, lam2);
         const float4 o2 = leapfrog4(a2, pv2, a1, a3, ym2, yp2, zl2, zr2, lam2);
-        if (rem == 0) {   // whole groups only (uniform over the tile)
+        if (rem == 0 && t.lead == 0) {   // whole groups only (uniform over the tile)
             if (act) {
                 *(AS_GLOBAL float4 *)(pnext + idx) = o1;
                 *(AS_GLOBAL float4 *)(pnext + idx + sx) = o2;
             }
         } else if (act) {
-            const int n = lane == nk4 - 1 ? rem : 4;
-            store4(pnext + idx, o1, n);
-            store4(pnext + idx + sx, o2, n);
+            const int n = (lane == nk4 - 1 && rem) ? rem : 4, lo = lane == 0 ? t.lead : 0;
+            store4(pnext + idx, o1, n, lo);
+            store4(pnext + idx + sx, o2, n, lo);
         }
         return;
     }
@@ -289,12 +292,12 @@ __device__ __forceinline__ void interior_vec_tile(const SceneDev &S, const TileD
         if (lane == nk4 - 1) zr = pcur[idx + 4];
         const float4 o = leapfrog4(pc, prev, pm, pp, ym, yp, zl, zr, lam2);
         if (act) {
-            const int n = (lane == nk4 - 1 && rem) ? rem : 4;
-            store4(pnext + idx, o, n);
+            const int n = (lane == nk4 - 1 && rem) ? rem : 4, lo = lane == 0 ? t.lead : 0;
+            store4(pnext + idx, o, n, lo);
             if (t.te > t.tb) {
-                emit_cell_terms(S, t, terms, step, idx, o.x);
-                if (n > 1) emit_cell_terms(S, t, terms, step, idx + 1, o.y);
-                if (n > 2) emit_cell_terms(S, t, terms, step, idx + 2, o.z);
+                if (lo == 0) emit_cell_terms(S, t, terms, step, idx, o.x);
+                if (n > 1 && lo <= 1) emit_cell_terms(S, t, terms, step, idx + 1, o.y);
+                if (n > 2 && lo <= 2) emit_cell_terms(S, t, terms, step, idx + 2, o.z);
                 if (n > 3) emit_cell_terms(S, t, terms, step, idx + 3, o.w);
             }
         }

@@ -265,9 +265,10 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
             for (int r = t.i0; r < t.i1; ++r) ++rec_hits[r];
             continue;
         }
+        const int kfirst = t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0);   // cells the tile writes
         for (int i = t.i0; i < t.i1; ++i)
             for (int j = t.j0; j < t.j1; ++j)
-                for (int k = t.k0; k < t.k1; ++k) {
+                for (int k = kfirst; k < t.k1; ++k) {
                     const int64_t c = ((int64_t)i * D.S.n[1] + j) * D.S.n[2] + k;
                     if (in_box[c] && t.type != TILE_MOVING) {
                         if (t.type != TILE_INTERIOR_SKIP) return fail_early(plan, "a moving scene's box lies in a tile that would overwrite it");
