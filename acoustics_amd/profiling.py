@@ -44,10 +44,13 @@ def copy(mib, iters):
     print(f"copy: {iters} x {mib} MiB read + {mib} MiB written", flush=True)
 
 
-def step(n, steps, tile):
+def step(n, steps, tile, scheme="collocated"):
     from . import scenes
     from .solver import Solver
-    sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n)
+    if scheme == "staggered":
+        sb, _ = scenes.staggered_free_field(seed=7, div=(n, n, n), h=0.64 / n, pml=16, n_listeners=8)
+    else:
+        sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n)
     with Solver(sb, tile_config=tile) as s:
         # eager (one dispatch per kernel per step, one advance per 64 steps),
         # no timing events between the kernels
@@ -121,6 +124,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--scheme", choices=["collocated", "staggered"], default="collocated")
     ap.add_argument("--step-fetch")
     ap.add_argument("--step-write")
     ap.add_argument("--copy-fetch")
@@ -132,7 +136,7 @@ def main(argv=None):
     if a.what == "copy":
         copy(a.mib, a.iters)
     elif a.what == "step":
-        step(a.n, a.steps, a.tile)
+        step(a.n, a.steps, a.tile, a.scheme)
     elif a.what == "traffic":
         traffic(a)
     else:
