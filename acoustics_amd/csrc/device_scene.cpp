@@ -493,7 +493,8 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     // before a[2] are read and not written); the row stride is a multiple of
     // four too, and a row's last group may be partial
     const int kv0 = a[2] & ~3, lead = a[2] - kv0, kwv = b[2] - kv0;
-    const bool vec = !mac && !S.visc && plan.interior_vec && kwv <= 256;
+    // (the staggered scheme's float4 rows run in the interior launch only)
+    const bool vec = !S.visc && plan.interior_vec && kwv <= 256 && (!mac || !plan.fused);
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
@@ -520,7 +521,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
             if (special_hit(lo, hi) || src_hit(lo, hi)) { scalar_tiles(i0, i1, j0, j1); continue; }
             TileDesc t{};
             t.scene = scene_id;
-            t.type = TILE_INTERIOR_VEC;
+            t.type = mac ? TILE_MAC_VEC : TILE_INTERIOR_VEC;
             t.i0 = i0; t.i1 = i1; t.j0 = j0; t.j1 = j1; t.k0 = kv0; t.k1 = b[2];
             t.tk_log2 = 6;
             t.rpw = rpw;
@@ -568,7 +569,7 @@ void DeviceScene::assign_terms(int scene_id, size_t base, std::vector<TileDesc> 
         int best = -1;
         for (int q : column[(size_t)i * S.n[1] + j]) {
             const TileDesc &t = out[q];
-            const int kfirst = t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0);   // cells the tile writes
+            const int kfirst = t.k0 + ((t.type == TILE_INTERIOR_VEC || t.type == TILE_MAC_VEC) ? t.lead : 0);   // cells the tile writes
             if (k >= kfirst && k < t.k1 && (best < 0 || q < best)) best = q;
         }
         return best;
@@ -621,8 +622,9 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         const bool ok_kind = moving ? kind == KIND_SPECIAL
                                     : (kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
                                        (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL));
-        if ((t.type == TILE_MAC || t.type == TILE_MAC_PML) != (d.scheme == FDTD_SCHEME_STAGGERED))
+        if ((t.type == TILE_MAC || t.type == TILE_MAC_PML || t.type == TILE_MAC_VEC) != (d.scheme == FDTD_SCHEME_STAGGERED))
             return "tile class does not match the scene's scheme";
+        if (t.type == TILE_MAC_VEC && kind != KIND_INTERIOR) return "staggered float4 rows outside the interior launch";
         if (!ok_kind) return "tile class does not match its launch";
         if (t.type == TILE_SPECIAL) {
             if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";
@@ -651,7 +653,8 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
             continue;
         }
         // a float4 row tile reads from k0 and writes from k0 + lead
-        const int lo[3] = {t.i0, t.j0, t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0)}, hi[3] = {t.i1, t.j1, t.k1};
+        const bool vec_row = t.type == TILE_INTERIOR_VEC || t.type == TILE_MAC_VEC;
+        const int lo[3] = {t.i0, t.j0, t.k0 + (vec_row ? t.lead : 0)}, hi[3] = {t.i1, t.j1, t.k1};
         for (int a = 0; a < 3; ++a) {
             if (lo[a] >= hi[a]) return "empty tile box";
             if (pml_class) {
@@ -662,7 +665,7 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         }
         if (pml_class) {
             if ((t.k1 - t.k0) > (1 << t.tk_log2) || (t.j1 - t.j0) * (1 << t.tk_log2) > 256) return "PML tile too large";
-        } else if (t.type == TILE_INTERIOR_VEC) {
+        } else if (vec_row) {
             // a partial last group reads up to k0 + 4 ceil((k1 - k0) / 4) <= the row stride
             if (t.rpw < 1 || t.j1 - t.j0 > 4 * t.rpw || (t.k0 & 3) || t.lead < 0 || t.lead > 3 || t.k1 - t.k0 > 256 ||
                 t.rpw * ((t.k1 - t.k0 + 3) >> 2) > 64 || (d.sy & 3) || t.k0 + ((t.k1 - t.k0 + 3) & ~3) > d.sy)

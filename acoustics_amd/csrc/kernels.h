@@ -134,6 +134,7 @@ enum : int {
     TILE_MAC_PML = 7,  // staggered scheme, a PML slab (split pressures in the slab's compact storage)
     TILE_ABC = 8,      // Engquist boundary: a box of layer-1 cells, one thread per cell (k fastest)
     TILE_MOVING = 9,   // a box (at most 256 cells) of a moving scene's reclassified cells, one thread per cell
+    TILE_MAC_VEC = 10, // staggered scheme, cells outside the PML, whole rows with four cells per lane (as TILE_INTERIOR_VEC)
 };
 
 struct TileDesc {

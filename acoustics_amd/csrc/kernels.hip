@@ -586,7 +586,9 @@ __device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, c
         vo0[idx] = vxL;
         vo1[idx] = vyL;
         vo2[idx] = vzL;
-        const float ix = cxC.w * (vxR - vxL), iy = cyL.w * (vyR - vyL), iz = czL.w * (vzR - vzL);
+        // products and sums rounded one by one (no contraction), as in the
+        // float4 row tiles, so every tile plan gives the same bits
+        const float ix = __fmul_rn(cxC.w, vxR - vxL), iy = __fmul_rn(cyL.w, vyR - vyL), iz = __fmul_rn(czL.w, vzR - vzL);
         float src = 0.f;
         if (t.src)
             for (int s = 0; s < S.n_psrc; ++s) {
@@ -605,10 +607,98 @@ __device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, c
             pd2[c] = n2;
             pn = n0 + n1 + n2;
         } else {
-            pn = pc + fmaf(3.f, src, (ix + iy) + iz);
+            pn = __fadd_rn(pc, fmaf(3.f, src, __fadd_rn(__fadd_rn(ix, iy), iz)));
         }
         pnext[idx] = pn;
         if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, pn);
+        vxL = vxR;
+        pm = pc;
+        pc = pp;
+    }
+}
+
+// Staggered scheme, cells outside the PML, whole rows with four cells per
+// lane (the lane and row layout of interior_vec_tile).  No face there is a
+// wall (walls lie in the PML); each lane updates the five z faces k .. k + 4 of its
+// group (the fifth is the next lane's first, recomputed the same way), its
+// cells' low x and y faces (the high ones are recomputed by the neighbours
+// that own them) and its four pressures, marching the tile's planes with the
+// x face carried.  Arithmetic as mac_tile<false>, cell by cell.
+__device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &t, long long step)
+{
+    const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
+    const int j0w = t.j0 + (int)(threadIdx.x >> 6) * R;
+    if (j0w >= t.j1) return;
+    const int l64 = threadIdx.x & 63, row = R == 1 ? 0 : l64 / nk4;
+    const int lr = l64 - row * nk4, jr = j0w + row;
+    const bool act = row < R && lr < nk4 && jr < t.j1;
+    const int j = act ? jr : t.j0;
+    const int lane = act ? lr : 0;
+    const int k = t.k0 + 4 * lane;
+    const int sx = S.sx, sy = S.sy;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    AS_GLOBAL float *__restrict__ pnext = gmem(S.field + (size_t)slot_next(step) * S.npad);
+    const int vb = (int)(step & 1);
+    const AS_GLOBAL float *__restrict__ vi0 = gmem(S.vel[vb][0]);
+    const AS_GLOBAL float *__restrict__ vi1 = gmem(S.vel[vb][1]);
+    const AS_GLOBAL float *__restrict__ vi2 = gmem(S.vel[vb][2]);
+    AS_GLOBAL float *__restrict__ vo0 = gmem(S.vel[vb ^ 1][0]);
+    AS_GLOBAL float *__restrict__ vo1 = gmem(S.vel[vb ^ 1][1]);
+    AS_GLOBAL float *__restrict__ vo2 = gmem(S.vel[vb ^ 1][2]);
+    const AS_CONST float4 *__restrict__ cx = cmem(S.coef[0]);
+    const AS_CONST float4 *cy = cmem(S.coef[1]), *cz = cmem(S.coef[2]);
+    const float4 cyL = cy[j], cyR = cy[j + 1];
+    // the z faces k .. k + 4 and their cells' tables (index k + m: face on
+    // the low side of cell k + m)
+    float4 czf[5];
+    for (int m = 0; m < 5; ++m) czf[m] = cz[min(k + m, S.n[2])];   // past the row end: read, not used
+    int idx = t.i0 * sx + j * sy + k;
+    float4 pm = *(const AS_GLOBAL float4 *)(pcur + idx - sx);
+    float4 pc = *(const AS_GLOBAL float4 *)(pcur + idx);
+    float4 vxL;
+    {
+        const float4 v = *(const AS_GLOBAL float4 *)(vi0 + idx);
+        const float4 c = cx[t.i0];
+        vxL = make_float4(mac_face(c, v.x, pm.x, pc.x), mac_face(c, v.y, pm.y, pc.y), mac_face(c, v.z, pm.z, pc.z),
+                          mac_face(c, v.w, pm.w, pc.w));
+    }
+    for (int i = t.i0; i < t.i1; ++i, idx += sx) {
+        const float4 pp = *(const AS_GLOBAL float4 *)(pcur + idx + sx);
+        const float4 ym = *(const AS_GLOBAL float4 *)(pcur + idx - sy);
+        const float4 yp = *(const AS_GLOBAL float4 *)(pcur + idx + sy);
+        const float4 v0R = *(const AS_GLOBAL float4 *)(vi0 + idx + sx);
+        const float4 v1L = *(const AS_GLOBAL float4 *)(vi1 + idx);
+        const float4 v1R = *(const AS_GLOBAL float4 *)(vi1 + idx + sy);
+        const float4 v2 = *(const AS_GLOBAL float4 *)(vi2 + idx);
+        float zl = __shfl_up(pc.w, 1), zr = __shfl_down(pc.x, 1), v2n = __shfl_down(v2.x, 1);
+        if (lane == 0) zl = pcur[idx - 1];
+        if (lane == nk4 - 1) { zr = pcur[idx + 4]; v2n = vi2[idx + 4]; }
+        const float4 cxC = cx[i], cxR = cx[i + 1];
+        const float4 vxR = make_float4(mac_face(cxR, v0R.x, pc.x, pp.x), mac_face(cxR, v0R.y, pc.y, pp.y),
+                                       mac_face(cxR, v0R.z, pc.z, pp.z), mac_face(cxR, v0R.w, pc.w, pp.w));
+        const float4 vyL = make_float4(mac_face(cyL, v1L.x, ym.x, pc.x), mac_face(cyL, v1L.y, ym.y, pc.y),
+                                       mac_face(cyL, v1L.z, ym.z, pc.z), mac_face(cyL, v1L.w, ym.w, pc.w));
+        const float4 vyR = make_float4(mac_face(cyR, v1R.x, pc.x, yp.x), mac_face(cyR, v1R.y, pc.y, yp.y),
+                                       mac_face(cyR, v1R.z, pc.z, yp.z), mac_face(cyR, v1R.w, pc.w, yp.w));
+        const float f0 = mac_face(czf[0], v2.x, zl, pc.x), f1 = mac_face(czf[1], v2.y, pc.x, pc.y);
+        const float f2 = mac_face(czf[2], v2.z, pc.y, pc.z), f3 = mac_face(czf[3], v2.w, pc.z, pc.w);
+        const float f4 = mac_face(czf[4], v2n, pc.w, zr);
+        float4 pn;
+        pn.x = __fadd_rn(pc.x, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.x - vxL.x), __fmul_rn(cyL.w, vyR.x - vyL.x)),
+                                                      __fmul_rn(czf[0].w, f1 - f0))));
+        pn.y = __fadd_rn(pc.y, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.y - vxL.y), __fmul_rn(cyL.w, vyR.y - vyL.y)),
+                                                      __fmul_rn(czf[1].w, f2 - f1))));
+        pn.z = __fadd_rn(pc.z, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.z - vxL.z), __fmul_rn(cyL.w, vyR.z - vyL.z)),
+                                                      __fmul_rn(czf[2].w, f3 - f2))));
+        pn.w = __fadd_rn(pc.w, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.w - vxL.w), __fmul_rn(cyL.w, vyR.w - vyL.w)),
+                                                      __fmul_rn(czf[3].w, f4 - f3))));
+        if (act) {
+            const int n = (lane == nk4 - 1 && rem) ? rem : 4, lo = lane == 0 ? t.lead : 0;
+            store4(vo0 + idx, vxL, n, lo);
+            store4(vo1 + idx, vyL, n, lo);
+            store4(vo2 + idx, make_float4(f0, f1, f2, f3), n, lo);
+            store4(pnext + idx, pn, n, lo);
+        }
         vxL = vxR;
         pm = pc;
         pc = pp;
@@ -1101,6 +1191,7 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
         else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
+        else if (t.type == TILE_MAC_VEC) mac_vec_tile(S, t, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     } else if constexpr (KIND == KIND_PML) {
         if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);

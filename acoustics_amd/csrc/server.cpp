@@ -265,7 +265,7 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
             for (int r = t.i0; r < t.i1; ++r) ++rec_hits[r];
             continue;
         }
-        const int kfirst = t.k0 + (t.type == TILE_INTERIOR_VEC ? t.lead : 0);   // cells the tile writes
+        const int kfirst = t.k0 + ((t.type == TILE_INTERIOR_VEC || t.type == TILE_MAC_VEC) ? t.lead : 0);   // cells the tile writes
         for (int i = t.i0; i < t.i1; ++i)
             for (int j = t.j0; j < t.j1; ++j)
                 for (int k = kfirst; k < t.k1; ++k) {
