@@ -65,6 +65,12 @@ class PressureSource(C.Structure):
                 ("offset_time", C.c_double), ("normalize_constant", C.c_double)]
 
 
+class SpeakerSource(C.Structure):
+    _fields_ = [("position", C.c_double * 3), ("width_space", C.c_double), ("start_time", C.c_double),
+                ("sample_rate", C.c_double), ("n_samples", C.c_int), ("reserved", C.c_int),
+                ("samples", C.POINTER(C.c_double))]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("settings", Settings),
@@ -72,6 +78,7 @@ class Scene(C.Structure):
         ("n_impulse_sources", C.c_int), ("impulse_sources", C.POINTER(ImpulseSource)),
         ("n_pressure_sources", C.c_int), ("pressure_sources", C.POINTER(PressureSource)),
         ("n_listeners", C.c_int), ("listeners", C.POINTER(C.c_double)),
+        ("n_speaker_sources", C.c_int), ("speaker_sources", C.POINTER(SpeakerSource)),
     ]
 
 
@@ -199,7 +206,8 @@ def check(status, what=""):
 class SceneBuffers:
     """Owns the ctypes arrays a Scene points at (keeps them alive)."""
 
-    def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None):
+    def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None,
+                 speaker_sources=()):
         self.settings_dict = dict(settings)
         s = Settings()
         for k, v in settings.items():
@@ -250,6 +258,20 @@ class SceneBuffers:
         self.scene.pressure_sources = C.cast(self._psrc, C.POINTER(PressureSource))
         self.scene.n_listeners = self.listeners_np.shape[0]
         self.scene.listeners = self.listeners_np.ctypes.data_as(C.POINTER(C.c_double))
+        # speaker sources: dict(position, width_space, start_time, sample_rate, samples)
+        self.speaker_list = list(speaker_sources)
+        self._spk_samples = [np.ascontiguousarray(sp["samples"], dtype=np.float64) for sp in self.speaker_list]
+        self._spk = (SpeakerSource * max(1, len(self.speaker_list)))()
+        for i, sp in enumerate(self.speaker_list):
+            for d in range(3):
+                self._spk[i].position[d] = sp["position"][d]
+            self._spk[i].width_space = sp["width_space"]
+            self._spk[i].start_time = sp["start_time"]
+            self._spk[i].sample_rate = sp["sample_rate"]
+            self._spk[i].n_samples = self._spk_samples[i].shape[0]
+            self._spk[i].samples = self._spk_samples[i].ctypes.data_as(C.POINTER(C.c_double))
+        self.scene.n_speaker_sources = len(self.speaker_list)
+        self.scene.speaker_sources = C.cast(self._spk, C.POINTER(SpeakerSource))
 
     @property
     def n_listeners(self):

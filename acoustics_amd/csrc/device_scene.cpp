@@ -49,6 +49,8 @@ struct Layout {
     unsigned long long *clock;
     double *acc_tab;
     float *src_tab;
+    double *spk, *spk_samples;
+    float *spk_field;
     uint8_t *ucls;
     MoveDev *mobj;
     Geom *pose;
@@ -124,6 +126,9 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.clock = c.take<unsigned long long>(CLK_WORDS);
     L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
     L.src_tab = c.take<float>((size_t)AMP_ROWS * S.psrc.size());
+    L.spk = c.take<double>(8 * S.spk.size());
+    L.spk_samples = c.take<double>(S.spk_samples.size());
+    L.spk_field = c.take<float>(S.spk.empty() ? 0 : (size_t)(S.spk_hi[0] - S.spk_lo[0]) * (S.spk_hi[1] - S.spk_lo[1]) * (S.spk_hi[2] - S.spk_lo[2]));
     L.ucls = c.take<uint8_t>(2 * moving_cells(S));
     L.mobj = c.take<MoveDev>(S.moving ? S.objs.size() : 0);
     L.pose = c.take<Geom>(S.moving ? S.objs.size() : 0);
@@ -223,6 +228,15 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
                     for (int k = S.mv_lo[2]; k < S.mv_hi[2]; ++k) skip[(size_t)i * S.n[1] * sy_dev + (size_t)j * sy_dev + k] = 1;
         FDTD_HIP_TRY(up(L.skip, skip.data(), skip.size()));
     }
+    if (!S.spk.empty()) {
+        std::vector<double> sp;
+        for (const SceneSetup::SpeakerHost &q : S.spk) {
+            const double v[8] = {q.p[0], q.p[1], q.p[2], q.w, q.t0, q.period, (double)q.off, (double)q.n};
+            sp.insert(sp.end(), v, v + 8);
+        }
+        FDTD_HIP_TRY(up(L.spk, sp.data(), sp.size() * 8));
+        FDTD_HIP_TRY(up(L.spk_samples, S.spk_samples.data(), S.spk_samples.size() * 8));
+    }
     if (S.moving) {
         std::vector<MoveDev> mv(S.objs.size());
         for (size_t q = 0; q < S.objs.size(); ++q) {
@@ -320,6 +334,12 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.n_obj = (int)S.objs.size();
     D.acc_tab = L.acc_tab;
     D.src_tab = L.src_tab;
+    D.n_spk = (int)S.spk.size();
+    D.spk = L.spk;
+    D.spk_samples = L.spk_samples;
+    D.c = S.c;
+    for (int d = 0; d < 3; ++d) { D.sp_lo[d] = S.spk_lo[d]; D.sp_n[d] = S.spk.empty() ? 0 : S.spk_hi[d] - S.spk_lo[d]; }
+    D.spk_field = L.spk_field;
     D.mv = S.moving ? 1 : 0;
     for (int d = 0; d < 3; ++d) {
         D.u_lo[d] = S.mv_lo[d];
@@ -395,6 +415,8 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     const int *a = S.inner_lo, *b = S.inner_hi;
     auto src_hit = [&](const int *lo, const int *hi) {
         for (const PSrcHost &p : S.psrc)
+            if (box_overlap(lo, hi, p.lo, p.hi)) return 1;
+        for (const SceneSetup::SpeakerHost &p : S.spk)
             if (box_overlap(lo, hi, p.lo, p.hi)) return 1;
         return 0;
     };
@@ -722,6 +744,10 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
                                 const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
                                 bool fused_launch, const std::vector<MotionDesc> &motion)
 {
+    // scenes with speaker sources get the speaker pass before their step kernels
+    std::vector<MotionDesc> speaker;
+    for (size_t q = 0; q < scenes.size(); ++q)
+        if (scenes[q].n_spk) speaker.push_back({(int)q});
     release();
     std::vector<TileDesc> by_kind[NUM_KINDS];
     build_launch_lists(tiles, fused_launch, by_kind);
@@ -741,7 +767,7 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, devs.size()));
     bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, terms.size()));
     bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
-    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, motion.size()));
+    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, motion.size() + speaker.size()));
     size_t total = bytes[0] + bytes[1] + bytes[2] + bytes[3];
     for (int k = 0; k < NUM_KINDS; ++k) {
         bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
@@ -766,7 +792,13 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     motion_cells = 0;
     for (const MotionDesc &m : motion)
         motion_cells = std::max(motion_cells, devs[m.scene].u_n[0] * devs[m.scene].u_n[1] * devs[m.scene].u_n[2]);
-    if (!motion.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, motion.data(), sizeof(MotionDesc) * motion.size(), hipMemcpyHostToDevice, st));
+    nspeaker = (int)speaker.size();
+    speaker_cells = 0;
+    for (const MotionDesc &m : speaker)
+        speaker_cells = std::max(speaker_cells, devs[m.scene].sp_n[0] * devs[m.scene].sp_n[1] * devs[m.scene].sp_n[2]);
+    std::vector<MotionDesc> pre(motion);
+    pre.insert(pre.end(), speaker.begin(), speaker.end());
+    if (!pre.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, pre.data(), sizeof(MotionDesc) * pre.size(), hipMemcpyHostToDevice, st));
     if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
     if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
     if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
@@ -786,6 +818,8 @@ void StepProgram::release()
     nghost = 0;
     nmotion = 0;
     motion_cells = 0;
+    nspeaker = 0;
+    speaker_cells = 0;
     nscenes = 0;
     order_clock = nullptr;
     fused = false;
@@ -794,6 +828,7 @@ void StepProgram::release()
 fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hipEvent_t ev_begin, hipEvent_t ev_end) const
 {
     if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, motion_cells, k, st));
+    if (nspeaker) FDTD_HIP_TRY(launch_speaker(d_scenes, d_motion + nmotion, nspeaker, speaker_cells, k, st));
     if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {

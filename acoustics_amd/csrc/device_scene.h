@@ -110,7 +110,7 @@ struct StepProgram {
     MotionDesc *d_motion = nullptr;
     TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
-    int nghost = 0, nmotion = 0, motion_cells = 0, nscenes = 0;
+    int nghost = 0, nmotion = 0, motion_cells = 0, nspeaker = 0, speaker_cells = 0, nscenes = 0;
     bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
     // tiles of every kind in one list; split by type here (or kept in one

@@ -134,6 +134,12 @@ struct SceneSetup {
     std::vector<PSrcHost> psrc;
     std::vector<float> gtab;
     std::vector<ImpulseHost> isrc;
+    // speaker sources (collocated scheme): parameters and samples, and the
+    // index box within 8 widths of each, where tiles evaluate them
+    struct SpeakerHost { double p[3], w, t0, period; int off, n; int lo[3], hi[3]; };
+    std::vector<SpeakerHost> spk;
+    std::vector<double> spk_samples;
+    int spk_lo[3] = {0, 0, 0}, spk_hi[3] = {0, 0, 0};   // union of the speakers' boxes
     // objects
     std::vector<Geom> objs;             // poses at t = 0
     // moving objects (some object has a velocity or angular velocity): the

@@ -106,6 +106,13 @@ struct SceneDev {
     double *acc_tab;            // [AMP_ROWS][n_obj] normal acceleration a_n(obj, t_s)
     float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
     long long src_steps;        // time-parallel chunk: sources act in steps < src_steps (-1: always)
+    // speaker sources: per source {px, py, pz, w, t0, period, off, n} and the samples
+    int n_spk;
+    const double *spk;          // [n_spk][8]
+    const double *spk_samples;
+    double c;                   // sound speed
+    int sp_lo[3], sp_n[3];      // box of cells within 8 widths of any speaker
+    float *spk_field;           // [sp_n0 sp_n1 sp_n2]: the speakers' S dt of the step being run
     // moving objects: the box of cells reclassified every step, their bulk
     // flags by step parity (row s & 1 holds step s's classification), the
     // objects' motion and their pose at the step being run
@@ -177,5 +184,8 @@ hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, in
 // fresh cells; workgroups of 256 cells along x (max_cells: the largest box of
 // the list), one scene per y.
 hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s);
+// Speaker sources, before the step kernels of step clock + k: the speakers'
+// S dt over each listed scene's speaker box.
+hipError_t launch_speaker(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s);
 
 }  // namespace fdtd

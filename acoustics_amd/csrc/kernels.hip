@@ -71,6 +71,39 @@ __device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double
     return (float)(exp(-(u * u) / 2.0) * ps.amp * dt);
 }
 
+// Speaker sources at cell (i, j, k) in step `step`, times dt
+// (SpeakerPressureSource::Evaluate, SpeakerPressureSource.cpp:91-113, at
+// t_n + dt/2): exp(-r^2 / (2 w^2)) samples[(int)((t - r/c - t0) / period)],
+// zero past either end of the samples.
+__device__ inline double speaker_terms(const SceneDev &S, long long step, int i, int j, int k)
+{
+    if (S.n_spk == 0 || (S.src_steps >= 0 && step >= S.src_steps)) return 0.0;
+    const double t = S.t_start + (double)step * S.dt + 0.5 * S.dt;
+    const int ijk[3] = {i, j, k};
+    double x[3];
+    for (int d = 0; d < 3; ++d) x[d] = S.bmin[d] + (0.5 + (double)ijk[d]) * S.h;
+    double v = 0.0;
+    for (int s = 0; s < S.n_spk; ++s) {
+        const double *q = S.spk + 8 * s;
+        const double r2 = (x[0] - q[0]) * (x[0] - q[0]) + (x[1] - q[1]) * (x[1] - q[1]) + (x[2] - q[2]) * (x[2] - q[2]);
+        const double g = exp(-r2 / (2.0 * q[3] * q[3]));
+        const double delay = t - sqrt(r2) / S.c;
+        const int idx = (int)((delay - q[4]) / q[5]);
+        if (idx >= 0 && idx < (int)q[7]) v += g * S.spk_samples[(int)q[6] + idx];
+    }
+    return v * S.dt;
+}
+
+// The speaker term of cell (i, j, k) for the step being run, from the field
+// speaker_kernel wrote over the speakers' box before the step kernels (zero
+// outside the box, where every speaker is below exp(-32) of its peak).
+__device__ __forceinline__ float speaker_at(const SceneDev &S, int i, int j, int k)
+{
+    const int a = i - S.sp_lo[0], b = j - S.sp_lo[1], c = k - S.sp_lo[2];
+    if (a < 0 || b < 0 || c < 0 || a >= S.sp_n[0] || b >= S.sp_n[1] || c >= S.sp_n[2]) return 0.f;
+    return gmem(S.spk_field)[(a * S.sp_n[1] + b) * S.sp_n[2] + c];
+}
+
 // ---------------------------------------------------------------------------
 // listener terms (PML_WaveSolver::FetchScalarData, MLSModeInterpolator::lookup
 // with precomputed weights): the thread that produces a term's sample writes
@@ -137,6 +170,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
                     const AS_CONST float *g = cmem(S.gtab);
                     pn += gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
                 }
+            if (t.src && S.n_spk) pn += speaker_at(S, i, j, k);
             if (!skip_check || !gmem(S.skip)[idx]) {
                 pnext[idx] = pn;
                 lapv[idx] = lc;
@@ -183,6 +217,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
                 const AS_CONST float *g = cmem(S.gtab);
                 pn += gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k];
             }
+        if (t.src && S.n_spk) pn += speaker_at(S, i, j, k);
         if (!skip_check || !gmem(S.skip)[idx]) {
             pnext[idx] = pn;
             if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, pn);
@@ -763,6 +798,10 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
             pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
         }
     }
+    if (S.n_spk) {
+        const int i = idx / sx, j = (idx - i * sx) / sy, k = idx - i * sx - j * sy;
+        pn += (double)speaker_at(S, i, j, k);
+    }
     pnext[idx] = (float)pn;
     if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
 }
@@ -852,6 +891,7 @@ __device__ __forceinline__ void abc_tile(const SceneDev &S, const TileDesc &t, c
             const AS_CONST float *g = cmem(S.gtab);
             pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g[ps.goff[0] + i] * g[ps.goff[1] + j] * g[ps.goff[2] + k]);
         }
+    if (t.src && S.n_spk) pn += (double)speaker_at(S, i, j, k);
     pnext[idx] = (float)pn;
     if (t.te > t.tb) emit_cell_terms(S, t, terms, step, idx, (float)pn);
 }
@@ -1130,6 +1170,7 @@ __device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step
             const AS_CONST float *g3 = cmem(S.gtab);
             pn += (double)(gmem(S.src_tab)[amp_row(step) * S.n_psrc + s] * g3[ps.goff[0] + i] * g3[ps.goff[1] + j] * g3[ps.goff[2] + k]);
         }
+    if (t.src && S.n_spk) pn += (double)speaker_at(S, i, j, k);
     pnext[idx] = (float)pn;
 }
 
@@ -1271,6 +1312,28 @@ hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_
 {
     if (nscenes <= 0) return hipSuccess;
     hipLaunchKernelGGL(advance_kernel, dim3(nscenes), dim3(256), 0, s, scenes, n);
+    return hipGetLastError();
+}
+
+// The speaker field of step clock + k over each listed scene's speaker box
+// (workgroups of 256 cells along x, one scene per y), before the step kernels.
+__global__ __launch_bounds__(256) void speaker_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
+{
+    const SceneDev &S = scenes[list[blockIdx.y].scene];
+    const long long step = (long long)S.clock[CLK_STEP] + k;
+    if (step >= (long long)S.clock[CLK_TARGET]) return;
+    const int ncell = S.sp_n[0] * S.sp_n[1] * S.sp_n[2];
+    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (c >= ncell) return;
+    const int kk = c % S.sp_n[2], r = c / S.sp_n[2];
+    const int j = S.sp_lo[1] + r % S.sp_n[1], i = S.sp_lo[0] + r / S.sp_n[1];
+    S.spk_field[c] = (float)speaker_terms(S, step, i, j, S.sp_lo[2] + kk);
+}
+
+hipError_t launch_speaker(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s)
+{
+    if (n <= 0 || max_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(speaker_kernel, dim3((unsigned)((max_cells + 255) / 256), (unsigned)n), dim3(256), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

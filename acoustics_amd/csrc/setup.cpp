@@ -355,6 +355,36 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         }
         S.psrc.push_back(P);
     }
+    S.spk.clear(); S.spk_samples.clear();
+    if (sc.n_speaker_sources < 0 || (sc.n_speaker_sources > 0 && !sc.speaker_sources)) { err = "bad speaker source list"; return FDTD_ERR_INVALID; }
+    if (sc.n_speaker_sources > 0 && st.scheme != FDTD_SCHEME_COLLOCATED) {
+        err = "speaker sources run with the collocated scheme";
+        return FDTD_ERR_UNSUPPORTED;
+    }
+    for (int q = 0; q < sc.n_speaker_sources; ++q) {
+        const fdtd_speaker_source &sp = sc.speaker_sources[q];
+        if (!(sp.width_space > 0) || !(sp.sample_rate > 0) || sp.n_samples < 0 || (sp.n_samples > 0 && !sp.samples)) {
+            err = "speaker source: width_space and sample_rate must be positive, samples given";
+            return FDTD_ERR_INVALID;
+        }
+        SceneSetup::SpeakerHost H{};
+        for (int d = 0; d < 3; ++d) H.p[d] = sp.position[d];
+        H.w = sp.width_space; H.t0 = sp.start_time;
+        H.period = 1.0 / sp.sample_rate;   // WavReader::SampleRate returns the sample period
+        H.off = (int)S.spk_samples.size(); H.n = sp.n_samples;
+        S.spk_samples.insert(S.spk_samples.end(), sp.samples, sp.samples + sp.n_samples);
+        for (int d = 0; d < 3; ++d) {
+            // cells within 8 w of the speaker (exp(-32) of its peak beyond)
+            H.lo[d] = std::max(0, (int)std::floor((H.p[d] - 8.0 * H.w - bmin[d]) / h - 0.5));
+            H.hi[d] = std::min(S.n[d], (int)std::ceil((H.p[d] + 8.0 * H.w - bmin[d]) / h - 0.5) + 1);
+            if (H.hi[d] <= H.lo[d]) { H.lo[d] = 0; H.hi[d] = 0; }
+        }
+        S.spk.push_back(H);
+        for (int d = 0; d < 3; ++d) {
+            S.spk_lo[d] = S.spk.size() == 1 ? H.lo[d] : std::min(S.spk_lo[d], H.lo[d]);
+            S.spk_hi[d] = S.spk.size() == 1 ? H.hi[d] : std::max(S.spk_hi[d], H.hi[d]);
+        }
+    }
     S.isrc.clear();
     for (int q = 0; q < sc.n_impulse_sources; ++q) {
         const fdtd_impulse_source &is = sc.impulse_sources[q];

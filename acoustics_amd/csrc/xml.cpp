@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -153,10 +154,68 @@ struct fdtd_parsed {
     std::vector<fdtd_object> objects;
     std::vector<fdtd_impulse_source> impulses;
     std::vector<fdtd_pressure_source> psrc;
+    std::vector<fdtd_speaker_source> spk;
+    std::vector<std::vector<double>> spk_samples;
     std::vector<double> listeners;
     int num_steps = 0;
     std::string output_pattern = "%s";
 };
+
+namespace {
+
+// Channel 0 of a RIFF/WAVE file as the reference's WavReader gets it from
+// libsndfile (sndgen/WavReader.hpp:96-120): integer PCM scaled to [-1, 1)
+// (8-bit unsigned about 128, 16/24/32-bit signed over 2^(bits-1)), IEEE
+// floats as stored.  ReadChannel reads `frames` values of the interleaved
+// data, so with more than one channel only the first frames / channels
+// frames carry samples and the rest are zero, as there.
+bool read_wav_channel0(const std::string &path, std::vector<double> &out, double &rate, std::string &err)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { err = "cannot open speaker file " + path; return false; }
+    std::vector<unsigned char> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    auto u16 = [&](size_t o) { return (unsigned)b[o] | ((unsigned)b[o + 1] << 8); };
+    auto u32 = [&](size_t o) { return (uint32_t)b[o] | ((uint32_t)b[o + 1] << 8) | ((uint32_t)b[o + 2] << 16) | ((uint32_t)b[o + 3] << 24); };
+    if (b.size() < 12 || std::memcmp(&b[0], "RIFF", 4) || std::memcmp(&b[8], "WAVE", 4)) { err = path + " is not a RIFF/WAVE file"; return false; }
+    unsigned fmt = 0, ch = 0, bits = 0;
+    size_t data = 0, dlen = 0;
+    for (size_t o = 12; o + 8 <= b.size();) {
+        const uint32_t len = u32(o + 4);
+        if (!std::memcmp(&b[o], "fmt ", 4) && o + 24 <= b.size()) {
+            fmt = u16(o + 8); ch = u16(o + 10); rate = (double)u32(o + 12); bits = u16(o + 22);
+            if (fmt == 0xFFFE && len >= 26 && o + 34 <= b.size()) fmt = u16(o + 32);   // WAVE_FORMAT_EXTENSIBLE
+        } else if (!std::memcmp(&b[o], "data", 4)) {
+            data = o + 8;
+            dlen = std::min<size_t>(len, b.size() - data);
+        }
+        o += 8 + (size_t)len + (len & 1);
+    }
+    const unsigned bps = bits / 8;
+    if (!data || !ch || !(rate > 0) || !bps || !((fmt == 1 && bps >= 1 && bps <= 4) || (fmt == 3 && (bps == 4 || bps == 8)))) {
+        err = path + ": unsupported WAV layout (PCM 8/16/24/32-bit or IEEE float 32/64 expected)";
+        return false;
+    }
+    const size_t frames = dlen / (bps * ch);
+    auto value = [&](size_t item) -> double {
+        const size_t o = data + item * bps;
+        if (fmt == 3) {
+            if (bps == 4) { float v; std::memcpy(&v, &b[o], 4); return v; }
+            double v; std::memcpy(&v, &b[o], 8); return v;
+        }
+        if (bps == 1) return ((double)b[o] - 128.0) / 128.0;
+        uint32_t raw = 0;
+        for (unsigned q = 0; q < bps; ++q) raw |= (uint32_t)b[o + q] << (8 * q);
+        int64_t v = raw;
+        if (raw & (1u << (8 * bps - 1))) v -= (int64_t)1 << (8 * bps);
+        return (double)v / (double)((int64_t)1 << (8 * bps - 1));
+    };
+    out.assign(frames, 0.0);
+    for (size_t fr = 0; fr < frames; ++fr)
+        if (fr * ch < frames) out[fr] = value(fr * ch);
+    return true;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -241,8 +300,35 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
                 q.position[2] = req_real(g, "source_position_z");
                 P->psrc.push_back(q);
             }
-            if (!ps->children("speaker_pressure_source").empty())
-                throw ParseError{"speaker_pressure_source is not supported (needs an audio file reader)"};
+            // ImpulseResponseParser.cpp:676-717 (static positioning)
+            for (const Node *g : ps->children("speaker_pressure_source")) {
+                fdtd_speaker_source q{};
+                q.start_time = req_real(g, "start_time");
+                q.width_space = req_real(g, "width_space");
+                const std::string file = opt_str(g, "speaker_file", "");
+                if (file.empty()) throw ParseError{"speaker_pressure_source needs speaker_file"};
+                const std::string type = opt_str(g, "positioning_type", "static");
+                if (type == "markers") throw ParseError{"speaker positioning by marker meshes is not supported"};
+                if (type != "static") throw ParseError{"speaker position type not understood"};
+                q.position[0] = req_real(g, "position_x");
+                q.position[1] = req_real(g, "position_y");
+                q.position[2] = req_real(g, "position_z");
+                // the file as given, else next to the XML file
+                std::vector<double> samples;
+                double rate = 0;
+                std::string err;
+                std::string where = file;
+                if (!std::ifstream(where).good() && file[0] != '/') {
+                    const std::string xml(path);
+                    const size_t slash = xml.find_last_of('/');
+                    if (slash != std::string::npos) where = xml.substr(0, slash + 1) + file;
+                }
+                if (!read_wav_channel0(where, samples, rate, err)) throw ParseError{err};
+                q.sample_rate = rate;
+                q.n_samples = (int)samples.size();
+                P->spk.push_back(q);
+                P->spk_samples.push_back(std::move(samples));
+            }
         }
         if (const Node *ll = root->child("listening_point_list"))
             for (const Node *l : ll->children("listening_point")) {
@@ -305,6 +391,9 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
     sc.pressure_sources = P->psrc.data();
     sc.n_listeners = (int)(P->listeners.size() / 3);
     sc.listeners = P->listeners.data();
+    for (size_t q = 0; q < P->spk.size(); ++q) P->spk[q].samples = P->spk_samples[q].data();
+    sc.n_speaker_sources = (int)P->spk.size();
+    sc.speaker_sources = P->spk.data();
     *out = P.release();
     return FDTD_OK;
 }

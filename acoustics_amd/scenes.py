@@ -235,6 +235,30 @@ def two_moving_objects(seed=14, n=56, h=0.005, pml=8, steps=360, n_listeners=8):
     return SceneBuffers(s, objects=[sph, ell], impulse_sources=srcs, pressure_sources=[ps], listeners=listeners), steps
 
 
+def tone_burst(fs=44100, freq=2000.0, cycles=4, amplitude=0.8):
+    """A Hann-windowed sine burst (the samples of a speaker file)."""
+    n = int(round(cycles * fs / freq))
+    t = np.arange(n) / fs
+    return amplitude * np.sin(2 * np.pi * freq * t) * np.sin(np.pi * np.arange(n) / n) ** 2
+
+
+def speaker_scene(seed=15, n=48, h=0.005, pml=8, steps=300, n_listeners=8, with_sphere=False):
+    """A speaker pressure source (SpeakerPressureSource, static position,
+    a 2 kHz tone burst at 44.1 kHz starting at 1e-4 s, width 8 mm) at a seeded
+    interior point, seeded listeners; optionally a rigid sphere next to it."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml)
+    lo, hi = interior_bounds(s, 6)
+    pos = rng.uniform(lo, hi)
+    spk = dict(position=tuple(pos), width_space=0.008, start_time=1e-4, sample_rate=44100.0, samples=tone_burst())
+    objs, imps = [], []
+    if with_sphere:
+        c = pos + np.array([0.035, 0.0, 0.0])
+        objs = [dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(0.015,) * 3, rotation=np.eye(3))]
+    listeners = _random_listeners(rng, s, objs, n_listeners)
+    return SceneBuffers(s, objects=objs, impulse_sources=imps, listeners=listeners, speaker_sources=[spk]), steps
+
+
 def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
             n_listeners=64, shell=0.2):
     """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse

@@ -165,6 +165,22 @@ typedef struct fdtd_pressure_source {
     double normalize_constant;  /* K (default 1/(sqrt(2 pi) ws)^3)       */
 } fdtd_pressure_source;
 
+/* Speaker pressure source (SpeakerPressureSource.cpp:91-113, static
+ * positioning): S(x, t) = exp(-|x - p|^2 / (2 w^2)) * samples[idx] with
+ * idx = (int)((t - |x - p| / c - start_time) * sample period^-1), zero outside
+ * [0, n_samples); added as S dt to every non-PML fluid cell like the Gaussian
+ * source.  samples: channel 0 of the speaker file, scaled to [-1, 1) as
+ * libsndfile reads integer PCM. */
+typedef struct fdtd_speaker_source {
+    double        position[3];   /* position_x/y/z                          */
+    double        width_space;   /* width_space w (m)                       */
+    double        start_time;    /* start_time (s)                          */
+    double        sample_rate;   /* samples per second of the file          */
+    int           n_samples;
+    int           reserved;
+    const double *samples;       /* n_samples values (the caller owns them) */
+} fdtd_speaker_source;
+
 typedef struct fdtd_scene {
     fdtd_settings                settings;
     int                          n_objects;
@@ -175,6 +191,8 @@ typedef struct fdtd_scene {
     const fdtd_pressure_source  *pressure_sources;
     int                          n_listeners;
     const double                *listeners;          /* [n_listeners][3] */
+    int                          n_speaker_sources;
+    const fdtd_speaker_source   *speaker_sources;
 } fdtd_scene;
 
 /* Fills every field of a settings block with the parser defaults

@@ -523,6 +523,7 @@ struct oracle {
     long long step;
     int evals_off;   /* time-parallel chunk past its stop_boundary_acc_time */
     fdtd_object *objs0;  /* poses at t = 0 of moving objects                */
+    fdtd_speaker_source *spk;  /* speaker sources with copied samples       */
     int moving;          /* some object has a velocity or angular velocity  */
     int fresh_error;     /* a fresh cell lay inside an object               */
     /* staggered (MAC) scheme: pFull; pdir[3] and v[3] are the full-grid fields */
@@ -1080,6 +1081,22 @@ static double pressure_sources(const oracle *o, const double *x, double t)
         value *= 1. / 3.6836 * s->normalize_constant;
         acc += value;
     }
+    /* SpeakerPressureSource::Evaluate (SpeakerPressureSource.cpp:91-113): the
+       bounding box is never set for this source type, so every cell counts;
+       WavReader::SampleRate is the sample period */
+    for (int i = 0; i < o->sc.n_speaker_sources; ++i) {
+        const fdtd_speaker_source *sp = &o->spk[i];
+        const double *q = sp->position;
+        double value = pow(x[0] - q[0], 2.0) + pow(x[1] - q[1], 2.0) + pow(x[2] - q[2], 2.0);
+        const double r = sqrt(value);
+        value = -value / (2.0 * pow(sp->width_space, 2.0));
+        value = exp(value);
+        const double delay = t - r / o->c;
+        const int idx = (int)((delay - sp->start_time) / (1.0 / sp->sample_rate));
+        if (idx >= 0 && idx < sp->n_samples) value *= sp->samples[idx];
+        else value = 0.0;
+        acc += value;
+    }
     return acc;
 }
 
@@ -1178,7 +1195,7 @@ static void pressure_update(oracle *o, double *pLast, double *pCurr, double *pNe
         }
         pNext[ci] = o->pdir[0][ci] + o->pdir[1][ci] + o->pdir[2][ci];
     }
-    const int has_src = o->sc.n_pressure_sources > 0;
+    const int has_src = o->sc.n_pressure_sources > 0 || o->sc.n_speaker_sources > 0;
     const double tsrc = o->t + 0.5 * dt;
 #pragma omp parallel for schedule(static)
     for (long long ci = 0; ci < o->N; ++ci) {
@@ -1321,7 +1338,7 @@ static void mac_pressure(oracle *o)
     const fdtd_settings *s = &o->sc.settings;
     const double nrc = -o->rho * o->c * o->c * o->dt;
     const double hinv = 1.0 / o->h;
-    const int has_src = o->sc.n_pressure_sources > 0;
+    const int has_src = o->sc.n_pressure_sources > 0 || o->sc.n_speaker_sources > 0;
     for (int d = 0; d < 3; ++d) {
         #pragma omp parallel for schedule(static)
         for (int i = 0; i < o->n[0]; ++i)
@@ -1374,6 +1391,14 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     o->listeners = (double *)malloc(sizeof(double) * 3 * (scene->n_listeners + 1));
     memcpy(o->listeners, scene->listeners, sizeof(double) * 3 * scene->n_listeners);
     o->sc.objects = o->objs; o->sc.impulse_sources = o->isrc; o->sc.pressure_sources = o->psrc; o->sc.listeners = o->listeners;
+    o->spk = (fdtd_speaker_source *)calloc((size_t)scene->n_speaker_sources + 1, sizeof(fdtd_speaker_source));
+    for (int i = 0; i < scene->n_speaker_sources; ++i) {
+        o->spk[i] = scene->speaker_sources[i];
+        double *copy = (double *)malloc(sizeof(double) * ((size_t)o->spk[i].n_samples + 1));
+        memcpy(copy, scene->speaker_sources[i].samples, sizeof(double) * (size_t)o->spk[i].n_samples);
+        o->spk[i].samples = copy;
+    }
+    o->sc.speaker_sources = o->spk;
     o->objs0 = (fdtd_object *)malloc(sizeof(fdtd_object) * (scene->n_objects + 1));
     memcpy(o->objs0, scene->objects, sizeof(fdtd_object) * scene->n_objects);
     for (int i = 0; i < scene->n_objects; ++i)
@@ -1432,6 +1457,8 @@ void oracle_free(oracle *o)
     for (int d = 0; d < 3; ++d) { free(o->pdir[d]); free(o->v[d]); }
     free(o->bulk); free(o->ghostc); free(o->pmlc); free(o->cavity); free(o->history); free(o->classified);
     free(o->pp); free(o->pv); free(o->gh); free(o->hkeys); free(o->hvals);
+    for (int i = 0; o->spk && i < o->sc.n_speaker_sources; ++i) free((void *)o->spk[i].samples);
+    free(o->spk);
     free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
     free(o);

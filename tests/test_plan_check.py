@@ -29,6 +29,7 @@ def _scenes():
     out.append(("staggered_36x40x44", scenes.staggered_free_field(seed=7, div=(36, 40, 44))[0]))
     out.append(("engquist_free_field", scenes.engquist_free_field(seed=8, div=(37, 41, 300))[0]))
     out.append(("moving_sphere", scenes.moving_object(seed=10)[0]))
+    out.append(("speaker_sphere", scenes.speaker_scene(seed=15, with_sphere=True)[0]))
     out.append(("moving_ellipsoid_engquist", scenes.moving_object(seed=11, kind="ellipsoid", speed=12.0, spin=300.0,
                                                                   pml_free=True, n=46)[0]))
     out.append(("engquist_sphere_coupled", scenes.engquist_sphere(seed=9, n=44,
