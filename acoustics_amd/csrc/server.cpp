@@ -5,7 +5,9 @@
 // for every resident request (requests admitted at different times are at
 // different steps -- each carries its own device step counter), and retires
 // finished requests (copy the FP64 listener series, check it is finite, free
-// the memory).  Requests arriving while others run join at the next block.
+// the memory).  Requests arriving while others run join at the next block:
+// their allocation and upload run on a second stream while the resident
+// set's block runs, and the work list is rebuilt after that block.
 // Each request's tile list is built once, when it is admitted; a rebuild
 // concatenates the resident lists.
 #include <algorithm>
@@ -53,6 +55,7 @@ struct fdtd_server {
     fdtd_server_options opt{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t upload = nullptr;   // admissions upload here while a block runs on `stream`
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -141,18 +144,65 @@ void fdtd_server::run()
             stats.queued = (int64_t)queue.size();
             stats.peak_resident_bytes = std::max(stats.peak_resident_bytes, resident);
         }
+        // the block of the current resident set starts first; the admitted
+        // requests upload on their own stream while it runs, and join the
+        // work list at the next rebuild
+        const bool run_block = !active.empty() && !dirty;
+        if (run_block && prog.enqueue(stream, 0, block) != FDTD_OK) {
+            (void)hipStreamSynchronize(stream);
+            std::lock_guard<std::mutex> lk(mu);
+            set_last_error("step block failed on the device");
+            for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
+            for (Request *r : admit) finish(r, FDTD_ERR_DEVICE);
+            active.clear();
+            done_cv.notify_all();
+            continue;
+        }
+        std::vector<Request *> fresh;
         for (Request *r : admit) {
             r->dev = std::make_unique<DeviceScene>();
             r->dev->S = std::move(r->setup);
-            fdtd_status s = r->dev->allocate_and_upload(stream, std::max(1, r->steps), r->steps);
+            fdtd_status s = r->dev->allocate_and_upload(upload, std::max(1, r->steps), r->steps);
             if (s == FDTD_OK) r->dev->append_tiles(0, tile_plan(0), r->tiles, r->terms);
             std::lock_guard<std::mutex> lk(mu);
             if (s != FDTD_OK) { finish(r, s); done_cv.notify_all(); continue; }
-            active.push_back(r);
-            dirty = true;
+            fresh.push_back(r);
         }
-        if (active.empty()) continue;
-        if (dirty) {
+        if (run_block) {
+            if (hipStreamSynchronize(stream) != hipSuccess) {
+                std::lock_guard<std::mutex> lk(mu);
+                set_last_error("step block failed on the device");
+                for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
+                for (Request *r : fresh) finish(r, FDTD_ERR_DEVICE);
+                active.clear();
+                done_cv.notify_all();
+                continue;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            ++stats.blocks_run;
+            for (Request *r : active) {
+                r->launched += block;
+                if (r->launched < r->steps) continue;
+                const int L = r->dev->S.L;
+                r->result.resize((size_t)r->steps * L);
+                fdtd_status s = FDTD_OK;
+                if (r->steps && L) s = r->dev->read_series(0, r->steps, r->result.data());
+                for (double v : r->result)
+                    if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
+                finish(r, s);
+                dirty = true;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!fresh.empty()) dirty = true;
+            active.insert(active.end(), fresh.begin(), fresh.end());
+            active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->status != FDTD_ERR_BUSY; }),
+                         active.end());
+            stats.active = (int64_t)active.size();
+            done_cv.notify_all();
+        }
+        if (dirty && !active.empty()) {
             fdtd_status s = rebuild();
             dirty = false;
             if (s != FDTD_OK) {
@@ -160,35 +210,8 @@ void fdtd_server::run()
                 for (Request *r : active) finish(r, s);
                 active.clear();
                 done_cv.notify_all();
-                continue;
             }
         }
-        if (prog.enqueue(stream, 0, block) != FDTD_OK || hipStreamSynchronize(stream) != hipSuccess) {
-            std::lock_guard<std::mutex> lk(mu);
-            set_last_error("step block failed on the device");
-            for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
-            active.clear();
-            done_cv.notify_all();
-            continue;
-        }
-        std::lock_guard<std::mutex> lk(mu);
-        ++stats.blocks_run;
-        for (Request *r : active) {
-            r->launched += block;
-            if (r->launched < r->steps) continue;
-            const int L = r->dev->S.L;
-            r->result.resize((size_t)r->steps * L);
-            fdtd_status s = FDTD_OK;
-            if (r->steps && L) s = r->dev->read_series(0, r->steps, r->result.data());
-            for (double v : r->result)
-                if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
-            finish(r, s);
-            dirty = true;
-        }
-        active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->status != FDTD_ERR_BUSY; }),
-                     active.end());
-        stats.active = (int64_t)active.size();
-        done_cv.notify_all();
     }
     prog.release();
     std::lock_guard<std::mutex> lk(mu);
@@ -321,6 +344,7 @@ fdtd_status fdtd_server_create(const fdtd_server_options *opt_in, fdtd_server **
     if (s->opt.device >= 0) FDTD_HIP_TRY(hipSetDevice(s->opt.device));
     FDTD_HIP_TRY(hipGetDevice(&s->device));
     FDTD_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    FDTD_HIP_TRY(hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking));
     fdtd_server *raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
     *out = raw;
@@ -424,6 +448,7 @@ fdtd_status fdtd_server_destroy(fdtd_server *s)
     if (s->worker.joinable()) s->worker.join();
     (void)hipStreamSynchronize(s->stream);
     (void)hipStreamDestroy(s->stream);
+    (void)hipStreamDestroy(s->upload);
     delete s;
     return FDTD_OK;
 }
