@@ -516,7 +516,16 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     // four too, and a row's last group may be partial
     const int kv0 = a[2] & ~3, lead = a[2] - kv0, kwv = b[2] - kv0;
     // (the staggered scheme's float4 rows run in the interior launch only)
-    const bool vec = !S.visc && plan.interior_vec && kwv <= 256 && (!mac || !plan.fused);
+    // staggered rows take one coefficient entry per axis: every face of the
+    // box (a .. b inclusive) and every cell must hold the same table values
+    bool mac_uniform = true;
+    if (mac)
+        for (int d = 0; d < 3; ++d)
+            for (int i = a[d]; i <= b[d]; ++i) {
+                mac_uniform = mac_uniform && S.f1[d][i] == S.f1[d][a[d]] && S.f2[d][i] == S.f2[d][a[d]];
+                if (i < b[d]) mac_uniform = mac_uniform && S.f5h[d][i] == S.f5h[d][a[d]];
+            }
+    const bool vec = !S.visc && plan.interior_vec && kwv <= 256 && (!mac || (!plan.fused && mac_uniform));
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {

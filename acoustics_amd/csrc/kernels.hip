@@ -658,7 +658,10 @@ __device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, c
 // group (the fifth is the next lane's first, recomputed the same way), its
 // cells' low x and y faces (the high ones are recomputed by the neighbours
 // that own them) and its four pressures, marching the tile's planes with the
-// x face carried.  Arithmetic as mac_tile<false>, cell by cell.
+// x face carried.  Arithmetic as mac_tile<false>, cell by cell.  The host
+// builds these rows only where each axis's coefficient table holds a single
+// value over the tile's faces and cells (outside the PML: f1 = 1, sigma = 0),
+// so one entry per axis is held instead of one per face.
 __device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &t, long long step)
 {
     const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
@@ -682,20 +685,15 @@ __device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &
     AS_GLOBAL float *__restrict__ vo2 = gmem(S.vel[vb ^ 1][2]);
     const AS_CONST float4 *__restrict__ cx = cmem(S.coef[0]);
     const AS_CONST float4 *cy = cmem(S.coef[1]), *cz = cmem(S.coef[2]);
-    const float4 cyL = cy[j], cyR = cy[j + 1];
-    // the z faces k .. k + 4 and their cells' tables (index k + m: face on
-    // the low side of cell k + m)
-    float4 czf[5];
-    for (int m = 0; m < 5; ++m) czf[m] = cz[min(k + m, S.n[2])];   // past the row end: read, not used
+    const float4 cxv = cx[t.i0], cyv = cy[j], czv = cz[t.k0 + t.lead];
     int idx = t.i0 * sx + j * sy + k;
     float4 pm = *(const AS_GLOBAL float4 *)(pcur + idx - sx);
     float4 pc = *(const AS_GLOBAL float4 *)(pcur + idx);
     float4 vxL;
     {
         const float4 v = *(const AS_GLOBAL float4 *)(vi0 + idx);
-        const float4 c = cx[t.i0];
-        vxL = make_float4(mac_face(c, v.x, pm.x, pc.x), mac_face(c, v.y, pm.y, pc.y), mac_face(c, v.z, pm.z, pc.z),
-                          mac_face(c, v.w, pm.w, pc.w));
+        vxL = make_float4(mac_face(cxv, v.x, pm.x, pc.x), mac_face(cxv, v.y, pm.y, pc.y), mac_face(cxv, v.z, pm.z, pc.z),
+                          mac_face(cxv, v.w, pm.w, pc.w));
     }
     for (int i = t.i0; i < t.i1; ++i, idx += sx) {
         const float4 pp = *(const AS_GLOBAL float4 *)(pcur + idx + sx);
@@ -708,25 +706,24 @@ __device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &
         float zl = __shfl_up(pc.w, 1), zr = __shfl_down(pc.x, 1), v2n = __shfl_down(v2.x, 1);
         if (lane == 0) zl = pcur[idx - 1];
         if (lane == nk4 - 1) { zr = pcur[idx + 4]; v2n = vi2[idx + 4]; }
-        const float4 cxC = cx[i], cxR = cx[i + 1];
-        const float4 vxR = make_float4(mac_face(cxR, v0R.x, pc.x, pp.x), mac_face(cxR, v0R.y, pc.y, pp.y),
-                                       mac_face(cxR, v0R.z, pc.z, pp.z), mac_face(cxR, v0R.w, pc.w, pp.w));
-        const float4 vyL = make_float4(mac_face(cyL, v1L.x, ym.x, pc.x), mac_face(cyL, v1L.y, ym.y, pc.y),
-                                       mac_face(cyL, v1L.z, ym.z, pc.z), mac_face(cyL, v1L.w, ym.w, pc.w));
-        const float4 vyR = make_float4(mac_face(cyR, v1R.x, pc.x, yp.x), mac_face(cyR, v1R.y, pc.y, yp.y),
-                                       mac_face(cyR, v1R.z, pc.z, yp.z), mac_face(cyR, v1R.w, pc.w, yp.w));
-        const float f0 = mac_face(czf[0], v2.x, zl, pc.x), f1 = mac_face(czf[1], v2.y, pc.x, pc.y);
-        const float f2 = mac_face(czf[2], v2.z, pc.y, pc.z), f3 = mac_face(czf[3], v2.w, pc.z, pc.w);
-        const float f4 = mac_face(czf[4], v2n, pc.w, zr);
+        const float4 vxR = make_float4(mac_face(cxv, v0R.x, pc.x, pp.x), mac_face(cxv, v0R.y, pc.y, pp.y),
+                                       mac_face(cxv, v0R.z, pc.z, pp.z), mac_face(cxv, v0R.w, pc.w, pp.w));
+        const float4 vyL = make_float4(mac_face(cyv, v1L.x, ym.x, pc.x), mac_face(cyv, v1L.y, ym.y, pc.y),
+                                       mac_face(cyv, v1L.z, ym.z, pc.z), mac_face(cyv, v1L.w, ym.w, pc.w));
+        const float4 vyR = make_float4(mac_face(cyv, v1R.x, pc.x, yp.x), mac_face(cyv, v1R.y, pc.y, yp.y),
+                                       mac_face(cyv, v1R.z, pc.z, yp.z), mac_face(cyv, v1R.w, pc.w, yp.w));
+        const float f0 = mac_face(czv, v2.x, zl, pc.x), f1 = mac_face(czv, v2.y, pc.x, pc.y);
+        const float f2 = mac_face(czv, v2.z, pc.y, pc.z), f3 = mac_face(czv, v2.w, pc.z, pc.w);
+        const float f4 = mac_face(czv, v2n, pc.w, zr);
         float4 pn;
-        pn.x = __fadd_rn(pc.x, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.x - vxL.x), __fmul_rn(cyL.w, vyR.x - vyL.x)),
-                                                      __fmul_rn(czf[0].w, f1 - f0))));
-        pn.y = __fadd_rn(pc.y, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.y - vxL.y), __fmul_rn(cyL.w, vyR.y - vyL.y)),
-                                                      __fmul_rn(czf[1].w, f2 - f1))));
-        pn.z = __fadd_rn(pc.z, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.z - vxL.z), __fmul_rn(cyL.w, vyR.z - vyL.z)),
-                                                      __fmul_rn(czf[2].w, f3 - f2))));
-        pn.w = __fadd_rn(pc.w, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxC.w, vxR.w - vxL.w), __fmul_rn(cyL.w, vyR.w - vyL.w)),
-                                                      __fmul_rn(czf[3].w, f4 - f3))));
+        pn.x = __fadd_rn(pc.x, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxv.w, vxR.x - vxL.x), __fmul_rn(cyv.w, vyR.x - vyL.x)),
+                                                      __fmul_rn(czv.w, f1 - f0))));
+        pn.y = __fadd_rn(pc.y, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxv.w, vxR.y - vxL.y), __fmul_rn(cyv.w, vyR.y - vyL.y)),
+                                                      __fmul_rn(czv.w, f2 - f1))));
+        pn.z = __fadd_rn(pc.z, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxv.w, vxR.z - vxL.z), __fmul_rn(cyv.w, vyR.z - vyL.z)),
+                                                      __fmul_rn(czv.w, f3 - f2))));
+        pn.w = __fadd_rn(pc.w, fmaf(3.f, 0.f, __fadd_rn(__fadd_rn(__fmul_rn(cxv.w, vxR.w - vxL.w), __fmul_rn(cyv.w, vyR.w - vyL.w)),
+                                                      __fmul_rn(czv.w, f4 - f3))));
         if (act) {
             const int n = (lane == nk4 - 1 && rem) ? rem : 4, lo = lane == 0 ? t.lead : 0;
             store4(vo0 + idx, vxL, n, lo);
