@@ -43,8 +43,8 @@ struct Layout {
     ImpDev *isrc;
     SpecialRec *recs;
     GhostDev *ghosts;
-    int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
-    double *gk_w, *gu_w, *gval0, *gval1, *gtmp;
+    int32_t *op_ptr, *op_idx, *oc_ptr, *oc_obj;
+    double *op_w, *oc_w, *gval0, *gval1;
     double *terms;
     unsigned long long *clock;
     double *acc_tab;
@@ -93,11 +93,13 @@ size_t state_layout(const SceneSetup &S, long long npad, size_t off[1 + 3 + 6 + 
     return o;
 }
 
-void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
+// The time-stepping state (L.field) comes from `cs`, every table, ring and
+// counter from `c`; both are the same carver for a single arena.
+void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
 {
     size_t off[11];
     L.state_floats = state_layout(S, slot_cells(S), off);
-    L.field = c.take<float>(L.state_floats);
+    L.field = cs.take<float>(L.state_floats);
     L.lapv = (L.field && S.visc) ? L.field + off[10] : nullptr;
     for (int d = 0; d < 3; ++d) L.pd[d] = L.field ? L.field + off[1 + d] : nullptr;
     for (int b = 0; b < 2; ++b)
@@ -113,15 +115,14 @@ void carve(const SceneSetup &S, int cap, Carver &c, Layout &L)
     L.isrc = c.take<ImpDev>(S.isrc.size());
     L.recs = c.take<SpecialRec>(S.special.size());
     L.ghosts = c.take<GhostDev>(S.ghosts.size());
-    L.gk_ptr = c.take<int32_t>(S.gk_ptr.size());
-    L.gk_idx = c.take<int32_t>(S.gk_idx.size());
-    L.gk_w = c.take<double>(S.gk_w.size());
-    L.gu_ptr = c.take<int32_t>(S.gu_ptr.size());
-    L.gu_idx = c.take<int32_t>(S.gu_idx.size());
-    L.gu_w = c.take<double>(S.gu_w.size());
+    L.op_ptr = c.take<int32_t>(S.op_ptr.size());
+    L.op_idx = c.take<int32_t>(S.op_idx.size());
+    L.op_w = c.take<double>(S.op_w.size());
+    L.oc_ptr = c.take<int32_t>(S.oc_ptr.size());
+    L.oc_obj = c.take<int32_t>(S.oc_obj.size());
+    L.oc_w = c.take<double>(S.oc_w.size());
     L.gval0 = c.take<double>(S.ghosts.size());
     L.gval1 = c.take<double>(S.ghosts.size());
-    L.gtmp = c.take<double>(2 * S.ghosts.size());
     L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
     L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
@@ -154,8 +155,17 @@ size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)
 {
     Carver c;
     Layout L;
-    carve(S, series_cap, c, L);
+    carve(S, series_cap, c, c, L);
     return c.off;
+}
+
+void DeviceScene::region_bytes(const SceneSetup &S, int series_cap, size_t *state, size_t *tables)
+{
+    Carver cs, ct;
+    Layout L;
+    carve(S, series_cap, cs, ct, L);
+    *state = cs.off;
+    *tables = ct.off;
 }
 
 fdtd_status DeviceScene::reset_to_start(hipStream_t st)
@@ -177,19 +187,25 @@ fdtd_status DeviceScene::reset_to_start(hipStream_t st)
 
 fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long target_steps)
 {
+    size_t state_b = 0, table_b = 0;
+    region_bytes(S, cap, &state_b, &table_b);
+    arena_bytes = state_b + table_b;
+    FDTD_HIP_TRY(hipMalloc(&arena, arena_bytes));
+    return place(st, cap, target_steps, arena, arena + state_b);
+}
+
+fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, char *state_base, char *table_base)
+{
     const long long target = target_steps;
     this->target = target_steps;
     series_cap = cap;
     npad = slot_cells(S);
     sy_dev = row_stride(S);
-    Carver c;
+    Carver cs, c;
+    cs.base = state_base;
+    c.base = table_base;
     Layout L;
-    carve(S, cap, c, L);
-    arena_bytes = c.off;
-    FDTD_HIP_TRY(hipMalloc(&arena, arena_bytes));
-    c.base = arena;
-    c.off = 0;
-    carve(S, cap, c, L);
+    carve(S, cap, cs, c, L);
     state_floats = L.state_floats;
     // fields start at rest: pressures, split pressures and both face-velocity buffers
     FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * state_floats, st));
@@ -266,14 +282,14 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     std::vector<GhostDev> gd;
     for (const GhostEntry &g : S.ghosts) gd.push_back({(int)to_dev(g.nbr), g.obj, g.coupled, 0, g.k_raster, g.c_acc});
     FDTD_HIP_TRY(up(L.ghosts, gd.data(), gd.size() * sizeof(GhostDev)));
-    FDTD_HIP_TRY(up(L.gk_ptr, S.gk_ptr.data(), S.gk_ptr.size() * 4));
-    std::vector<int32_t> gk_idx(S.gk_idx);
-    for (int32_t &c : gk_idx) c = (int32_t)to_dev(c);
-    FDTD_HIP_TRY(up(L.gk_idx, gk_idx.data(), gk_idx.size() * 4));
-    FDTD_HIP_TRY(up(L.gk_w, S.gk_w.data(), S.gk_w.size() * 8));
-    FDTD_HIP_TRY(up(L.gu_ptr, S.gu_ptr.data(), S.gu_ptr.size() * 4));
-    FDTD_HIP_TRY(up(L.gu_idx, S.gu_idx.data(), S.gu_idx.size() * 4));
-    FDTD_HIP_TRY(up(L.gu_w, S.gu_w.data(), S.gu_w.size() * 8));
+    FDTD_HIP_TRY(up(L.op_ptr, S.op_ptr.data(), S.op_ptr.size() * 4));
+    std::vector<int32_t> op_idx(S.op_idx);
+    for (int32_t &c : op_idx) c = (int32_t)to_dev(c);
+    FDTD_HIP_TRY(up(L.op_idx, op_idx.data(), op_idx.size() * 4));
+    FDTD_HIP_TRY(up(L.op_w, S.op_w.data(), S.op_w.size() * 8));
+    FDTD_HIP_TRY(up(L.oc_ptr, S.oc_ptr.data(), S.oc_ptr.size() * 4));
+    FDTD_HIP_TRY(up(L.oc_obj, S.oc_obj.data(), S.oc_obj.size() * 4));
+    FDTD_HIP_TRY(up(L.oc_w, S.oc_w.data(), S.oc_w.size() * 8));
     if (!S.ghosts.empty()) {
         FDTD_HIP_TRY(hipMemsetAsync(L.gval0, 0, S.ghosts.size() * 8, st));
         FDTD_HIP_TRY(hipMemsetAsync(L.gval1, 0, S.ghosts.size() * 8, st));
@@ -322,11 +338,9 @@ fdtd_status DeviceScene::allocate_and_upload(hipStream_t st, int cap, long long 
     D.ghosts = L.ghosts;
     D.gval[0] = L.gval0;
     D.gval[1] = L.gval1;
-    D.gtmp = L.gtmp;
     D.G = (int)S.ghosts.size();
-    D.coupled_iters = S.coupled_iters;
-    D.gk_ptr = L.gk_ptr; D.gk_idx = L.gk_idx; D.gk_w = L.gk_w;
-    D.gu_ptr = L.gu_ptr; D.gu_idx = L.gu_idx; D.gu_w = L.gu_w;
+    D.op_ptr = L.op_ptr; D.op_idx = L.op_idx; D.op_w = L.op_w;
+    D.oc_ptr = L.oc_ptr; D.oc_obj = L.oc_obj; D.oc_w = L.oc_w;
     D.T = (int)S.lst_terms.size();
     D.cap = cap;
     D.terms = L.terms;
@@ -749,9 +763,36 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
     return std::string();
 }
 
+// Bytes of the device block of a step program: scene blocks, listener term
+// references, ghost and pre-step (motion, speaker) lists, and the tile list
+// of each launch.
+static void program_bytes(size_t nscenes, size_t nterms, size_t nghost, size_t npre,
+                          const std::vector<TileDesc> by_kind[NUM_KINDS], size_t bytes[4 + NUM_KINDS], size_t *total)
+{
+    bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, nscenes));
+    bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, nterms));
+    bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, nghost));
+    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, npre));
+    *total = bytes[0] + bytes[1] + bytes[2] + bytes[3];
+    for (int k = 0; k < NUM_KINDS; ++k) {
+        bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
+        *total += bytes[4 + k];
+    }
+}
+
+size_t StepProgram::bytes_needed(size_t nscenes, const std::vector<TileDesc> &tiles, bool fused_launch, size_t nterms,
+                                 size_t nghost, size_t npre)
+{
+    std::vector<TileDesc> by_kind[NUM_KINDS];
+    build_launch_lists(tiles, fused_launch, by_kind);
+    size_t bytes[4 + NUM_KINDS], total = 0;
+    program_bytes(nscenes, nterms, nghost, npre, by_kind, bytes, &total);
+    return total;
+}
+
 fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                                 const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
-                                bool fused_launch, const std::vector<MotionDesc> &motion)
+                                bool fused_launch, const std::vector<MotionDesc> &motion, char *ext_base, size_t ext_bytes)
 {
     // scenes with speaker sources get the speaker pass before their step kernels
     std::vector<MotionDesc> speaker;
@@ -772,18 +813,21 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
             set_last_error("step program: motion list names a scene without moving objects");
             return FDTD_ERR_INVALID;
         }
-    size_t bytes[4 + NUM_KINDS];
-    bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, devs.size()));
-    bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, terms.size()));
-    bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, ghost.size()));
-    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, motion.size() + speaker.size()));
-    size_t total = bytes[0] + bytes[1] + bytes[2] + bytes[3];
-    for (int k = 0; k < NUM_KINDS; ++k) {
-        bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
-        total += bytes[4 + k];
-    }
+    size_t bytes[4 + NUM_KINDS], total = 0;
+    program_bytes(devs.size(), terms.size(), ghost.size(), motion.size() + speaker.size(), by_kind, bytes, &total);
     char *base = nullptr;
-    FDTD_HIP_TRY(hipMalloc(&base, total));
+    if (ext_base) {
+        if (total > ext_bytes) {
+            set_last_error("step program needs " + std::to_string(total) + " bytes, the memory given holds " +
+                           std::to_string(ext_bytes));
+            return FDTD_ERR_OOM;
+        }
+        base = ext_base;
+        owns = false;
+    } else {
+        FDTD_HIP_TRY(hipMalloc(&base, total));
+        owns = true;
+    }
     d_scenes = reinterpret_cast<SceneDev *>(base);
     d_terms = reinterpret_cast<TermRef *>(base + bytes[0]);
     d_ghost = reinterpret_cast<GhostDesc *>(base + bytes[0] + bytes[1]);
@@ -797,6 +841,8 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
             FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
     }
     nghost = (int)ghost.size();
+    ghost_max = 0;
+    for (const GhostDesc &g : ghost) ghost_max = std::max(ghost_max, devs[g.scene].G);
     nmotion = (int)motion.size();
     motion_cells = 0;
     for (const MotionDesc &m : motion)
@@ -818,13 +864,15 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
 
 void StepProgram::release()
 {
-    if (d_scenes) (void)hipFree(d_scenes);
+    if (d_scenes && owns) (void)hipFree(d_scenes);
     d_scenes = nullptr;
+    owns = true;
     for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; }
     d_terms = nullptr;
     d_ghost = nullptr;
     d_motion = nullptr;
     nghost = 0;
+    ghost_max = 0;
     nmotion = 0;
     motion_cells = 0;
     nspeaker = 0;
@@ -838,7 +886,7 @@ fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hip
 {
     if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, motion_cells, k, st));
     if (nspeaker) FDTD_HIP_TRY(launch_speaker(d_scenes, d_motion + nmotion, nspeaker, speaker_cells, k, st));
-    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, k, st));
+    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, ghost_max, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {
         // fused: every class in the interior list, except the moving-object tiles

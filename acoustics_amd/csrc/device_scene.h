@@ -48,9 +48,17 @@ struct DeviceScene {
 
     // Device bytes a scene needs (admission control uses this).
     static size_t bytes_needed(const SceneSetup &S, int series_cap);
+    // The same split in two regions: the time-stepping state (pressure slots,
+    // PML state, viscous Laplacian) and everything else (tables, series ring,
+    // amplitude ring, clock, scene block).
+    static void region_bytes(const SceneSetup &S, int series_cap, size_t *state, size_t *tables);
     // Allocates, zeroes the fields and uploads every table; target is the
     // number of steps after which the kernels stop advancing this scene.
     fdtd_status allocate_and_upload(hipStream_t stream, int series_cap, long long target);
+    // As allocate_and_upload, in memory the caller provides (region_bytes
+    // sizes both regions; each base 256-byte aligned); allocates nothing, and
+    // release() then frees nothing.
+    fdtd_status place(hipStream_t stream, int series_cap, long long target, char *state_base, char *table_base);
     void release();
     float *slot(int s) const { return dev.field + (size_t)s * (size_t)npad; }
     // device index of a dense host cell index (k fastest, rows of n2)
@@ -110,14 +118,21 @@ struct StepProgram {
     MotionDesc *d_motion = nullptr;
     TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
-    int nghost = 0, nmotion = 0, motion_cells = 0, nspeaker = 0, speaker_cells = 0, nscenes = 0;
+    int nghost = 0, ghost_max = 0, nmotion = 0, motion_cells = 0, nspeaker = 0, speaker_cells = 0, nscenes = 0;
     bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
     // tiles of every kind in one list; split by type here (or kept in one
     // launch when `fused_launch` is set)
+    // (ext_base: place the program there, ext_bytes long, instead of
+    // allocating it; release() then frees nothing)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                        const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
-                       bool fused_launch = false, const std::vector<MotionDesc> &motion = {});
+                       bool fused_launch = false, const std::vector<MotionDesc> &motion = {}, char *ext_base = nullptr,
+                       size_t ext_bytes = 0);
+    // Bytes upload() places for these lists (npre: motion plus speaker entries).
+    static size_t bytes_needed(size_t nscenes, const std::vector<TileDesc> &tiles, bool fused_launch, size_t nterms,
+                               size_t nghost, size_t npre);
+    bool owns = true;          // the program's block came from hipMalloc
     void release();
     // Enqueues n steps from `step` (the host's count of the step the first
     // runs; its parity picks the kernel order of each step): the step

@@ -164,7 +164,13 @@ struct SceneSetup {
     // coupled ghost system (CSR over ghosts): known-cell terms and unknown-ghost terms
     std::vector<int32_t> gk_ptr, gk_idx;  std::vector<double> gk_w;
     std::vector<int32_t> gu_ptr, gu_idx;  std::vector<double> gu_w;
-    int coupled_iters = 0;
+    // the system's solution operator x = M p + m a (fully-coupled scenes):
+    // per ghost, cells and weights of M (CSR, dense cell indices) and objects
+    // and weights of m; op_max_cycle is the largest set of mutually coupled
+    // ghosts (1: the coupling graph is acyclic)
+    std::vector<int32_t> op_ptr, op_idx;  std::vector<double> op_w;
+    std::vector<int32_t> oc_ptr, oc_obj;  std::vector<double> oc_w;
+    int op_max_cycle = 0;
     int64_t n_coupled = 0;
     // listeners
     int L = 0;
@@ -175,6 +181,8 @@ struct SceneSetup {
 
 // Builds the full setup; returns FDTD_OK or an error with a message in err.
 fdtd_status build_setup(const fdtd_scene &scene, SceneSetup &out, std::string &err);
+// The solution operator of the fully-coupled ghost system (op_* and oc_*).
+fdtd_status build_coupled_operator(SceneSetup &S, std::string &err);
 // Axis d is plain at cell i: no damping on the cell or its two faces
 // (f3 = f1 = 1 in the FP32 tables) and neither face is a wall.
 bool axis_plain(const SceneSetup &s, int d, int i);

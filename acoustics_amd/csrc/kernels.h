@@ -91,10 +91,12 @@ struct SceneDev {
     const SpecialRec *recs;
     const GhostDev *ghosts;
     double *gval[2];            // ghost values by step parity (fully-coupled mode)
-    double *gtmp;               // 2G scratch of the coupled iteration
-    int G, coupled_iters;
-    const int32_t *gk_ptr, *gk_idx, *gu_ptr, *gu_idx;
-    const double *gk_w, *gu_w;
+    // fully-coupled mode: the coupled system's solution operator, per ghost g
+    // x_g = sum_q op_w[q] p[op_idx[q]] (q in [op_ptr[g], op_ptr[g+1]))
+    //     + sum_q oc_w[q] a_n(oc_obj[q])  (q in [oc_ptr[g], oc_ptr[g+1]))
+    int G;
+    const int32_t *op_ptr, *op_idx, *oc_ptr, *oc_obj;
+    const double *op_w, *oc_w;
     // listener series: row s of the ring holds beta_q * sample_q of every MLS
     // term q of every listener after step s; a listener's value is the sum
     // of its terms, formed when the rows are read out
@@ -178,7 +180,9 @@ hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, 
 // refills its amplitude ring for the AMP_ROWS steps from there; n = 0 only
 // fills the ring.
 hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s);
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s);
+// Fully-coupled ghost values of step clock + k, one thread per ghost
+// (max_ghosts: the largest ghost count of the list), one scene per y.
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int max_ghosts, int k, hipStream_t s);
 // Moving objects, before the step kernels of step clock + k: poses, per-step
 // reclassification of each scene's box, zeroed new solid cells and filled
 // fresh cells; workgroups of 256 cells along x (max_cells: the largest box of

@@ -1253,43 +1253,25 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// fully-coupled ghost values (MAC_Grid.cpp:1131-1441): Jacobi iteration
-// x <- b + W x on the static coupling matrix; one workgroup per scene.
+// fully-coupled ghost values (MAC_Grid.cpp:1131-1441): the coupled system's
+// static solution operator applied to p^n and the accelerations of step n,
+// x_g = sum_q M_gq p_q + sum_o m_go a_n(o), one thread per ghost, FP64.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list, int k)
+__global__ __launch_bounds__(256) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list, int k)
 {
-    const SceneDev &S = scenes[list[blockIdx.x].scene];
+    const SceneDev &S = scenes[list[blockIdx.y].scene];
     const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const float *__restrict__ pcur = S.field + (size_t)slot_cur(step) * S.npad;
-    double *gval = S.gval[step & 1];
-    double *b = S.gtmp, *x1 = S.gtmp + S.G;
-    for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
-        const GhostDev gd = S.ghosts[g];
-        const double a = S.acc_tab[amp_row(step) * S.n_obj + gd.obj];
-        double v;
-        if (gd.coupled) {
-            v = gd.c_acc * a;
-            for (int q = S.gk_ptr[g]; q < S.gk_ptr[g + 1]; ++q) v += S.gk_w[q] * (double)pcur[S.gk_idx[q]];
-        } else {
-            v = (double)pcur[gd.nbr] + gd.k_raster * a;
-        }
-        b[g] = v;
-        gval[g] = v;
-    }
-    __syncthreads();
-    for (int it = 0; it < S.coupled_iters; ++it) {
-        const double *src = (it & 1) ? x1 : gval;
-        double *dst = (it & 1) ? gval : x1;
-        for (int g = threadIdx.x; g < S.G; g += blockDim.x) {
-            double v = b[g];
-            for (int q = S.gu_ptr[g]; q < S.gu_ptr[g + 1]; ++q) v += S.gu_w[q] * src[S.gu_idx[q]];
-            dst[g] = v;
-        }
-        __syncthreads();
-    }
-    if (S.coupled_iters & 1)
-        for (int g = threadIdx.x; g < S.G; g += blockDim.x) gval[g] = x1[g];
+    const int g = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    if (g >= S.G) return;
+    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
+    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_obj;
+    const int q0 = gmem(S.op_ptr)[g], q1 = gmem(S.op_ptr)[g + 1];
+    double x = 0.0;
+    for (int q = q0; q < q1; ++q) x += gmem(S.op_w)[q] * (double)pcur[gmem(S.op_idx)[q]];
+    const int c0 = gmem(S.oc_ptr)[g], c1 = gmem(S.oc_ptr)[g + 1];
+    for (int q = c0; q < c1; ++q) x += gmem(S.oc_w)[q] * acc[gmem(S.oc_obj)[q]];
+    gmem(S.gval[step & 1])[g] = x;
 }
 
 hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
@@ -1344,10 +1326,10 @@ hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, 
     return hipGetLastError();
 }
 
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int k, hipStream_t s)
+hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int max_ghosts, int k, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ghost_kernel, dim3(n), dim3(1024), 0, s, scenes, list, k);
+    if (n <= 0 || max_ghosts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((max_ghosts + 255) / 256), (unsigned)n), dim3(256), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

@@ -168,33 +168,35 @@ void fdtd_server::run()
             if (s != FDTD_OK) { finish(r, s); done_cv.notify_all(); continue; }
             fresh.push_back(r);
         }
-        if (run_block) {
-            if (hipStreamSynchronize(stream) != hipSuccess) {
-                std::lock_guard<std::mutex> lk(mu);
-                set_last_error("step block failed on the device");
-                for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
-                for (Request *r : fresh) finish(r, FDTD_ERR_DEVICE);
-                active.clear();
-                done_cv.notify_all();
-                continue;
-            }
+        if (run_block && hipStreamSynchronize(stream) != hipSuccess) {
             std::lock_guard<std::mutex> lk(mu);
-            ++stats.blocks_run;
-            for (Request *r : active) {
-                r->launched += block;
-                if (r->launched < r->steps) continue;
-                const int L = r->dev->S.L;
-                r->result.resize((size_t)r->steps * L);
-                fdtd_status s = FDTD_OK;
-                if (r->steps && L) s = r->dev->read_series(0, r->steps, r->result.data());
-                for (double v : r->result)
-                    if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
-                finish(r, s);
-                dirty = true;
-            }
+            set_last_error("step block failed on the device");
+            for (Request *r : active) finish(r, FDTD_ERR_DEVICE);
+            for (Request *r : fresh) finish(r, FDTD_ERR_DEVICE);
+            active.clear();
+            done_cv.notify_all();
+            continue;
         }
         {
+            // one critical section from finish() to the removal from `active`:
+            // a client that sees a request finished may take (and free) it at
+            // once, so no worker list may still hold it when the lock drops
             std::lock_guard<std::mutex> lk(mu);
+            if (run_block) {
+                ++stats.blocks_run;
+                for (Request *r : active) {
+                    r->launched += block;
+                    if (r->launched < r->steps) continue;
+                    const int L = r->dev->S.L;
+                    r->result.resize((size_t)r->steps * L);
+                    fdtd_status s = FDTD_OK;
+                    if (r->steps && L) s = r->dev->read_series(0, r->steps, r->result.data());
+                    for (double v : r->result)
+                        if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
+                    finish(r, s);
+                    dirty = true;
+                }
+            }
             if (!fresh.empty()) dirty = true;
             active.insert(active.end(), fresh.begin(), fresh.end());
             active.erase(std::remove_if(active.begin(), active.end(), [](Request *r) { return r->status != FDTD_ERR_BUSY; }),

@@ -139,6 +139,176 @@ void set_pml_state(SceneSetup &s, bool merged)
     }
 }
 
+namespace {
+
+// a sparse row: (column, weight), sorted by column, no repeats
+using SparseRow = std::vector<std::pair<int32_t, double>>;
+
+// sum_q f_q R_q of sparse rows
+SparseRow combine(const std::vector<std::pair<double, const SparseRow *>> &terms)
+{
+    std::vector<std::pair<int32_t, double>> all;
+    for (const auto &t : terms)
+        for (const auto &e : *t.second) all.push_back({e.first, t.first * e.second});
+    std::sort(all.begin(), all.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    SparseRow out;
+    for (const auto &e : all) {
+        if (!out.empty() && out.back().first == e.first) out.back().second += e.second;
+        else out.push_back(e);
+    }
+    return out;
+}
+
+// Strongly connected components of the digraph g -> u (W_gu != 0), in the
+// order Tarjan's algorithm closes them: every component after all the
+// components it reaches (iterative, no recursion depth limit).
+std::vector<std::vector<int>> coupling_components(const SceneSetup &S)
+{
+    const int G = (int)S.ghosts.size();
+    std::vector<int> index(G, -1), low(G, 0), stack, comp_of(G, -1);
+    std::vector<char> on(G, 0);
+    std::vector<std::vector<int>> comps;
+    int counter = 0;
+    std::vector<std::pair<int, int>> call;   // (vertex, next edge)
+    for (int root = 0; root < G; ++root) {
+        if (index[root] >= 0) continue;
+        call.push_back({root, S.gu_ptr[root]});
+        index[root] = low[root] = counter++;
+        stack.push_back(root); on[root] = 1;
+        while (!call.empty()) {
+            int v = call.back().first;
+            int &e = call.back().second;
+            if (e < S.gu_ptr[v + 1]) {
+                const int w = S.gu_idx[e++];
+                if (index[w] < 0) {
+                    index[w] = low[w] = counter++;
+                    stack.push_back(w); on[w] = 1;
+                    call.push_back({w, S.gu_ptr[w]});
+                } else if (on[w]) {
+                    low[v] = std::min(low[v], index[w]);
+                }
+                continue;
+            }
+            if (low[v] == index[v]) {
+                std::vector<int> c;
+                for (;;) {
+                    const int w = stack.back(); stack.pop_back(); on[w] = 0;
+                    c.push_back(w);
+                    if (w == v) break;
+                }
+                std::sort(c.begin(), c.end());
+                comps.push_back(std::move(c));
+            }
+            call.pop_back();
+            if (!call.empty()) { const int p = call.back().first; low[p] = std::min(low[p], low[v]); }
+        }
+    }
+    return comps;
+}
+
+}  // namespace
+
+// Rows of M (cells) and m (objects) of the coupled ghost operator
+// x = M p + m a, from the coupled rows (gk: known cells, gu: unknown ghosts,
+// c_acc) and the rasterized rows (p[nbr] + k_raster a).
+fdtd_status build_coupled_operator(SceneSetup &S, std::string &err)
+{
+    const int G = (int)S.ghosts.size();
+    std::vector<SparseRow> K(G), C(G), M(G), Mo(G);
+    for (int g = 0; g < G; ++g) {
+        const GhostEntry &e = S.ghosts[g];
+        if (e.coupled) {
+            std::vector<std::pair<double, const SparseRow *>> none;
+            SparseRow r;
+            for (int q = S.gk_ptr[g]; q < S.gk_ptr[g + 1]; ++q) r.push_back({S.gk_idx[q], S.gk_w[q]});
+            K[g] = combine({{1.0, &r}});
+            C[g] = {{e.obj, e.c_acc}};
+        } else {
+            K[g] = {{e.nbr, 1.0}};
+            C[g] = {{e.obj, e.k_raster}};
+        }
+    }
+    const std::vector<std::vector<int>> comps = coupling_components(S);
+    std::vector<int> comp_of(G, -1);
+    for (size_t c = 0; c < comps.size(); ++c)
+        for (int g : comps[c]) comp_of[g] = (int)c;
+    for (size_t c = 0; c < comps.size(); ++c) {
+        const std::vector<int> &cg = comps[c];
+        const int m = (int)cg.size();
+        // B_g = K_g + sum over ghosts u outside the component of W_gu M_u
+        std::vector<SparseRow> BM(m), BO(m);
+        std::vector<double> A((size_t)m * m, 0.0);   // I - W restricted to the component
+        for (int a = 0; a < m; ++a) {
+            const int g = cg[a];
+            A[(size_t)a * m + a] = 1.0;
+            std::vector<std::pair<double, const SparseRow *>> tm = {{1.0, &K[g]}}, to = {{1.0, &C[g]}};
+            for (int q = S.gu_ptr[g]; q < S.gu_ptr[g + 1]; ++q) {
+                const int u = S.gu_idx[q];
+                if (comp_of[u] == (int)c) {
+                    const int b = (int)(std::lower_bound(cg.begin(), cg.end(), u) - cg.begin());
+                    A[(size_t)a * m + b] -= S.gu_w[q];
+                } else {
+                    tm.push_back({S.gu_w[q], &M[u]});
+                    to.push_back({S.gu_w[q], &Mo[u]});
+                }
+            }
+            BM[a] = combine(tm);
+            BO[a] = combine(to);
+        }
+        if (m == 1) {
+            const double d = A[0];
+            if (!(std::fabs(d) > 1e-12)) { err = "fully_coupled ghost system is singular (a ghost couples to itself with weight 1)"; return FDTD_ERR_UNSUPPORTED; }
+            M[cg[0]] = combine({{1.0 / d, &BM[0]}});
+            Mo[cg[0]] = combine({{1.0 / d, &BO[0]}});
+            continue;
+        }
+        if (m > 512) {
+            err = "fully_coupled ghost system: a cycle of " + std::to_string(m) + " mutually coupled ghosts (the direct solve "
+                  "takes cycles of at most 512)";
+            return FDTD_ERR_UNSUPPORTED;
+        }
+        S.op_max_cycle = std::max(S.op_max_cycle, m);
+        // inverse of A by Gauss-Jordan elimination with partial pivoting
+        std::vector<double> Inv((size_t)m * m, 0.0);
+        for (int a = 0; a < m; ++a) Inv[(size_t)a * m + a] = 1.0;
+        for (int col = 0; col < m; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < m; ++r)
+                if (std::fabs(A[(size_t)r * m + col]) > std::fabs(A[(size_t)piv * m + col])) piv = r;
+            if (!(std::fabs(A[(size_t)piv * m + col]) > 1e-12)) { err = "fully_coupled ghost system is singular"; return FDTD_ERR_UNSUPPORTED; }
+            if (piv != col)
+                for (int k = 0; k < m; ++k) {
+                    std::swap(A[(size_t)piv * m + k], A[(size_t)col * m + k]);
+                    std::swap(Inv[(size_t)piv * m + k], Inv[(size_t)col * m + k]);
+                }
+            const double inv_p = 1.0 / A[(size_t)col * m + col];
+            for (int k = 0; k < m; ++k) { A[(size_t)col * m + k] *= inv_p; Inv[(size_t)col * m + k] *= inv_p; }
+            for (int r = 0; r < m; ++r) {
+                if (r == col) continue;
+                const double f = A[(size_t)r * m + col];
+                if (f == 0.0) continue;
+                for (int k = 0; k < m; ++k) { A[(size_t)r * m + k] -= f * A[(size_t)col * m + k]; Inv[(size_t)r * m + k] -= f * Inv[(size_t)col * m + k]; }
+            }
+        }
+        for (int a = 0; a < m; ++a) {
+            std::vector<std::pair<double, const SparseRow *>> tm, to;
+            for (int b = 0; b < m; ++b) {
+                const double f = Inv[(size_t)a * m + b];
+                if (f != 0.0) { tm.push_back({f, &BM[b]}); to.push_back({f, &BO[b]}); }
+            }
+            M[cg[a]] = combine(tm);
+            Mo[cg[a]] = combine(to);
+        }
+    }
+    for (int g = 0; g < G; ++g) {
+        for (const auto &e : M[g]) if (e.second != 0.0) { S.op_idx.push_back(e.first); S.op_w.push_back(e.second); }
+        S.op_ptr.push_back((int32_t)S.op_idx.size());
+        for (const auto &e : Mo[g]) if (e.second != 0.0) { S.oc_obj.push_back(e.first); S.oc_w.push_back(e.second); }
+        S.oc_ptr.push_back((int32_t)S.oc_obj.size());
+    }
+    return FDTD_OK;
+}
+
 fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
 {
     const fdtd_settings &st = sc.settings;
@@ -706,38 +876,23 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         S.gk_ptr.push_back((int32_t)S.gk_idx.size());
         S.gu_ptr.push_back((int32_t)S.gu_idx.size());
     }
-    // Jacobi iteration count for the coupled rows: x <- b + W x converges when
-    // the spectral radius of W is below one; estimate it by power iteration.
-    // Couplings usually form chains (a ghost's image point sees ghosts further
-    // along the surface, not back), making W nilpotent: then x_k = sum_{m<=k}
-    // W^m b is exact once k reaches the chain depth.  Otherwise bound the
-    // iteration count by the spectral radius.
-    S.coupled_iters = 0;
+    // The coupled system (MAC_Grid.cpp:1400-1425: A(r,r) = -1, A(r,u) = w_u,
+    // solved by BiCGSTAB every step) is x = b + W x with a static W and a
+    // right-hand side b = K p + c a_n that is linear in the pressures and the
+    // accelerations.  Its solution is therefore a static linear operator,
+    //   x = (I - W)^{-1} (K p + c a) = M p + m a,
+    // evaluated on the device in one pass.  W couples a ghost to the ghosts in
+    // its image point's stencil, further along the surface; its graph is
+    // almost always acyclic (the rare cycles are pairs), so the rows of M
+    // follow by substitution in dependency order: M_g = K_g + sum_u W_gu M_u,
+    // with a small dense inverse of I - W over each cycle.  The result is the
+    // exact solution (to rounding) whether or not an iteration would converge.
+    S.op_ptr.assign(1, 0); S.op_idx.clear(); S.op_w.clear();
+    S.oc_ptr.assign(1, 0); S.oc_obj.clear(); S.oc_w.clear();
+    S.op_max_cycle = 0;
     if (S.n_coupled > 0) {
-        std::vector<double> x(G, 1.0), y(G);
-        double rho_est = 0;
-        int depth = 0;
-        for (int it = 0; it < 256; ++it) {
-            double nrm = 0;
-            for (int g = 0; g < G; ++g) {
-                double s = 0;
-                for (int q = S.gu_ptr[g]; q < S.gu_ptr[g + 1]; ++q) s += std::fabs(S.gu_w[q]) * x[S.gu_idx[q]];
-                y[g] = s;
-                nrm = std::max(nrm, std::fabs(s));
-            }
-            if (nrm == 0) { depth = it + 1; break; }  // |W|^{it+1} 1 = 0
-            rho_est = nrm / std::max(1e-300, *std::max_element(x.begin(), x.end()));
-            for (int g = 0; g < G; ++g) x[g] = y[g] / nrm;
-        }
-        if (depth > 0) {
-            S.coupled_iters = depth;
-        } else if (rho_est < 0.95) {
-            S.coupled_iters = std::min(256, (int)std::ceil(std::log(1e-17) / std::log(rho_est)) + 1);
-        } else {
-            err = "fully_coupled ghost system: Jacobi iteration does not contract (spectral radius estimate " +
-                  std::to_string(rho_est) + ")";
-            return FDTD_ERR_UNSUPPORTED;
-        }
+        fdtd_status ost = build_coupled_operator(S, err);
+        if (ost != FDTD_OK) return ost;
     }
 
     // ---- special cells and their boxes ------------------------------------

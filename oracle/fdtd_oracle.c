@@ -512,6 +512,8 @@ struct oracle {
     pml_vcell *pv; long long npv;
 
     ghost_t *gh; int ngh, capgh;
+    /* the last assembled ghost system (CSR), kept for inspection */
+    int sys_n, sys_nnz; int *sys_rp, *sys_ci; double *sys_vv, *sys_rhs;
     /* open-addressing hash: key -> ghost index */
     uint64_t *hkeys; int *hvals; int hcap;
     /* owner -> children range (ghost indices sorted by owner then key) */
@@ -1011,7 +1013,39 @@ static void ghost_update(oracle *o, const double *p)
     double *sol = (double *)malloc(sizeof(double) * G);
     bicgstab(G, rp, ci, vv, rhs, sol);
     for (int g = 0; g < G; ++g) o->gh[g].pressure = sol[g];
-    free(sol); free(rp); free(ci); free(vv); free(all); free(trips); free(ntr); free(rhs);
+    /* the assembled system stays readable (oracle_ghost_system) until the next update */
+    free(o->sys_rp); free(o->sys_ci); free(o->sys_vv); free(o->sys_rhs);
+    o->sys_n = G; o->sys_nnz = m;
+    o->sys_rp = rp; o->sys_ci = ci; o->sys_vv = vv; o->sys_rhs = rhs;
+    free(sol); free(all); free(trips); free(ntr);
+}
+
+int oracle_ghost_count(const oracle *o) { return o->ngh; }
+
+int oracle_ghost_update_at(oracle *o, const double *p, double t, double *val, double *pos, long long *nbr)
+{
+    const double keep = o->t;
+    o->t = t;
+    ghost_update(o, p);
+    o->t = keep;
+    for (int g = 0; g < o->ngh; ++g) {
+        if (val) val[g] = o->gh[g].pressure;
+        if (pos) for (int d = 0; d < 3; ++d) pos[3 * g + d] = o->gh[g].position[d];
+        if (nbr) nbr[g] = o->gh[g].neighbour;
+    }
+    return o->ngh;
+}
+
+int oracle_ghost_system(const oracle *o, int *n, int *nnz, int *rp, int *ci, double *vv, double *rhs)
+{
+    if (n) *n = o->sys_n;
+    if (nnz) *nnz = o->sys_nnz;
+    if (!o->sys_rp) return -1;
+    if (rp) memcpy(rp, o->sys_rp, sizeof(int) * (o->sys_n + 1));
+    if (ci) memcpy(ci, o->sys_ci, sizeof(int) * o->sys_nnz);
+    if (vv) memcpy(vv, o->sys_vv, sizeof(double) * o->sys_nnz);
+    if (rhs) memcpy(rhs, o->sys_rhs, sizeof(double) * o->sys_n);
+    return 0;
 }
 
 /* ======================================================================== */
@@ -1461,6 +1495,7 @@ void oracle_free(oracle *o)
     free(o->spk);
     free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
+    free(o->sys_rp); free(o->sys_ci); free(o->sys_vv); free(o->sys_rhs);
     free(o);
 }
 

@@ -81,6 +81,17 @@ double  oracle_vandermonde_weights(const double *rows, int s_i, const double *bp
 double  oracle_pml_sigma(const fdtd_settings *s, const double *x, int dim);
 int     oracle_inside_pml(const fdtd_settings *s, const double *x);
 
+/* Ghost-cell entries of the current classification. */
+int     oracle_ghost_count(const oracle *o);
+/* Test hook of PML_pressureUpdateGhostCells: the ghost values the update
+ * gives for pressure field p (N values) at time t (val), with each entry's
+ * position (pos, 3 per entry) and fluid neighbour (nbr); any output may be
+ * NULL.  Returns the number of entries. */
+int     oracle_ghost_update_at(oracle *o, const double *p, double t, double *val, double *pos, long long *nbr);
+/* The last assembled ghost system A x = rhs (CSR, duplicates summed): sizes
+ * into n and nnz, then the arrays when non-NULL; -1 before any update. */
+int     oracle_ghost_system(const oracle *o, int *n, int *nnz, int *rp, int *ci, double *vv, double *rhs);
+
 /* Pose at time t of an object moving with its velocity and angular velocity
  * (test hook of the per-step object motion). */
 void oracle_object_pose(const fdtd_object *obj, double t, fdtd_object *out);

@@ -56,6 +56,13 @@ def lib():
         L.oracle_reflect.argtypes = [C.POINTER(Object), D, D, D, D]
         L.oracle_vandermonde_weights.restype = C.c_double
         L.oracle_vandermonde_weights.argtypes = [D, C.c_int, D, D, D, D, C.c_double, D]
+        L.oracle_ghost_count.restype = C.c_int
+        L.oracle_ghost_count.argtypes = [C.c_void_p]
+        L.oracle_ghost_update_at.restype = C.c_int
+        L.oracle_ghost_update_at.argtypes = [C.c_void_p, D, C.c_double, D, D, C.POINTER(C.c_longlong)]
+        L.oracle_ghost_system.restype = C.c_int
+        L.oracle_ghost_system.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int), D, D]
         L.oracle_pml_sigma.restype = C.c_double
         L.oracle_pml_sigma.argtypes = [C.POINTER(Settings), D, C.c_int]
         L.oracle_inside_pml.restype = C.c_int
@@ -111,6 +118,36 @@ class Oracle:
 
     def time(self):
         return self.L.oracle_time(self.h)
+
+    def ghost_update_at(self, p, t):
+        """Ghost values of the ghost-cell update for field p at time t, with
+        the entries' positions and fluid neighbours (flat indices)."""
+        G = self.L.oracle_ghost_count(self.h)
+        a = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        assert a.size == self.sb.n_cells
+        val, pos, nbr = np.zeros(G), np.zeros((G, 3)), np.zeros(G, dtype=np.int64)
+        P = C.POINTER(C.c_double)
+        self.L.oracle_ghost_update_at(self.h, a.ctypes.data_as(P), float(t), val.ctypes.data_as(P),
+                                      pos.ctypes.data_as(P), nbr.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return val, pos, nbr
+
+    def ghost_system(self):
+        """The last assembled ghost system as (dense A, rhs)."""
+        n, nnz = C.c_int(0), C.c_int(0)
+        if self.L.oracle_ghost_system(self.h, C.byref(n), C.byref(nnz), None, None, None, None) != 0:
+            raise RuntimeError("no ghost system assembled yet")
+        rp = np.zeros(n.value + 1, dtype=np.int32)
+        ci = np.zeros(max(1, nnz.value), dtype=np.int32)
+        vv = np.zeros(max(1, nnz.value))
+        rhs = np.zeros(n.value)
+        P, I = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        self.L.oracle_ghost_system(self.h, C.byref(n), C.byref(nnz), rp.ctypes.data_as(I), ci.ctypes.data_as(I),
+                                   vv.ctypes.data_as(P), rhs.ctypes.data_as(P))
+        A = np.zeros((n.value, n.value))
+        for r in range(n.value):
+            for q in range(rp[r], rp[r + 1]):
+                A[r, ci[q]] += vv[q]
+        return A, rhs
 
     def close(self):
         if self.h:
