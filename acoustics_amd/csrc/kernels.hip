@@ -139,6 +139,21 @@ __device__ __forceinline__ void emit_ghost_terms(const SceneDev &S, const TileDe
 // ---------------------------------------------------------------------------
 // interior leapfrog
 // ---------------------------------------------------------------------------
+// The leapfrog update in increment form, p^{n+1} = p^n + ((p^n - p^{n-1}) +
+// lam2 sum_q (p_q - p^n)).  A smooth field changes little between steps and
+// between neighbours, so every difference here is exact in FP32 (Sterbenz:
+// operands within a factor of two) and the only rounding at the scale of p
+// itself is the last addition.  Forming 2 p^n - p^{n-1} and the six-neighbour
+// sum instead rounds three times at that scale per step, and the rounding
+// errors, fed back as velocity, add up along every propagation path: 1.3e-4
+// relative L2 at the listeners after 4000 steps at 256^3, against 1e-4 allowed.
+__device__ __forceinline__ float leapfrog1(float c, float prev, float xm, float xp, float ym, float yp, float zm, float zp,
+                                           float lam2)
+{
+    const float lap = ((xm - c) + (xp - c)) + ((ym - c) + (yp - c)) + ((zm - c) + (zp - c));
+    return c + fmaf(lam2, lap, c - prev);
+}
+
 __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step, bool skip_check)
 {
     const int tk = t.tk_log2;
@@ -160,10 +175,10 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
         AS_GLOBAL float *lapv = gmem(S.lapv);
         for (int i = t.i0; i < t.i1; ++i, idx += sx) {
             const float c = pcur[idx];
-            const float lc = (((pcur[idx - sx] + pcur[idx + sx]) + (pcur[idx - sy] + pcur[idx + sy]) +
-                               (pcur[idx - 1] + pcur[idx + 1])) - 6.f * c) * S.inv_h2f;
+            const float lc = (((pcur[idx - sx] - c) + (pcur[idx + sx] - c)) + ((pcur[idx - sy] - c) + (pcur[idx + sy] - c)) +
+                              ((pcur[idx - 1] - c) + (pcur[idx + 1] - c))) * S.inv_h2f;
             const float ll = lapv[idx];
-            float pn = (2.f * c - pprev[idx]) - S.kcalpf * ll + S.c2k2_kcalpf * lc;
+            float pn = c + (((c - pprev[idx]) - S.kcalpf * ll) + S.c2k2_kcalpf * lc);
             if (t.src)
                 for (int s = 0; s < S.n_psrc; ++s) {
                     const PSrcDev ps = cmem(S.psrc)[s];
@@ -185,8 +200,8 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
         const float ym1 = pcur[idx - sy], yp1 = pcur[idx + sy], zm1 = pcur[idx - 1], zp1 = pcur[idx + 1];
         const float ym2 = pcur[idx + sx - sy], yp2 = pcur[idx + sx + sy], zm2 = pcur[idx + sx - 1], zp2 = pcur[idx + sx + 1];
         const float pv1 = pprev[idx], pv2 = pprev[idx + sx];
-        pnext[idx] = 2.f * a1 - pv1 + lam2 * (((a0 + a2) + (ym1 + yp1) + (zm1 + zp1)) - 6.f * a1);
-        pnext[idx + sx] = 2.f * a2 - pv2 + lam2 * (((a1 + a3) + (ym2 + yp2) + (zm2 + zp2)) - 6.f * a2);
+        pnext[idx] = leapfrog1(a1, pv1, a0, a2, ym1, yp1, zm1, zp1, lam2);
+        pnext[idx + sx] = leapfrog1(a2, pv2, a1, a3, ym2, yp2, zm2, zp2, lam2);
         return;
     }
     float pm = pcur[idx - sx];
@@ -198,7 +213,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
             const float ym = pcur[idx - sy], yp = pcur[idx + sy];
             const float zm = pcur[idx - 1], zp = pcur[idx + 1];
             const float prev = pprev[idx];
-            pnext[idx] = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
+            pnext[idx] = leapfrog1(pc, prev, pm, pp, ym, yp, zm, zp, lam2);
             pm = pc;
             pc = pp;
         }
@@ -210,7 +225,7 @@ __device__ __forceinline__ void interior_tile(const SceneDev &S, const TileDesc 
         const float ym = pcur[idx - sy], yp = pcur[idx + sy];
         const float zm = pcur[idx - 1], zp = pcur[idx + 1];
         const float prev = pprev[idx];
-        float pn = 2.f * pc - prev + lam2 * (((pm + pp) + (ym + yp) + (zm + zp)) - 6.f * pc);
+        float pn = leapfrog1(pc, prev, pm, pp, ym, yp, zm, zp, lam2);
         if (t.src)
             for (int s = 0; s < S.n_psrc; ++s) {
                 const PSrcDev ps = cmem(S.psrc)[s];
@@ -247,10 +262,10 @@ __device__ __forceinline__ float4 leapfrog4(float4 c, float4 prev, float4 xm, fl
                                             float zr, float lam2)
 {
     float4 o;
-    o.x = 2.f * c.x - prev.x + lam2 * (((xm.x + xp.x) + (ym.x + yp.x) + (zl + c.y)) - 6.f * c.x);
-    o.y = 2.f * c.y - prev.y + lam2 * (((xm.y + xp.y) + (ym.y + yp.y) + (c.x + c.z)) - 6.f * c.y);
-    o.z = 2.f * c.z - prev.z + lam2 * (((xm.z + xp.z) + (ym.z + yp.z) + (c.y + c.w)) - 6.f * c.z);
-    o.w = 2.f * c.w - prev.w + lam2 * (((xm.w + xp.w) + (ym.w + yp.w) + (c.z + zr)) - 6.f * c.w);
+    o.x = leapfrog1(c.x, prev.x, xm.x, xp.x, ym.x, yp.x, zl, c.y, lam2);
+    o.y = leapfrog1(c.y, prev.y, xm.y, xp.y, ym.y, yp.y, c.x, c.z, lam2);
+    o.z = leapfrog1(c.z, prev.z, xm.z, xp.z, ym.z, yp.z, c.y, c.w, lam2);
+    o.w = leapfrog1(c.w, prev.w, xm.w, xp.w, ym.w, yp.w, c.z, zr, lam2);
     return o;
 }
 
