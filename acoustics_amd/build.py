@@ -26,6 +26,7 @@ SOURCES = [
     ("solver.cpp", True),
     ("batch.cpp", True),
     ("server.cpp", True),
+    ("noalloc.cpp", True),
     ("setup.cpp", False),
     ("geometry.cpp", False),
     ("xml.cpp", False),

@@ -163,6 +163,7 @@ SIGNATURES = {
     "fdtd_save_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
     "fdtd_load_snapshot": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int]),
     "fdtd_server_destroy": (C.c_int, [_P]),
+    "fdtd_solve_noalloc": (C.c_int, [C.POINTER(Scene), C.c_int, _P, C.c_int64, _P, _D, C.POINTER(C.c_int64)]),
 }
 
 _lib = None

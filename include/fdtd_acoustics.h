@@ -416,6 +416,31 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
 fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n_tiles);
 fdtd_status fdtd_server_destroy(fdtd_server *s);
 
+/* ------------------------------------------------------------------------- */
+/* no device allocation                                                      */
+/* ------------------------------------------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: the work of fdtd_create (default options) +
+ * fdtd_step(num_steps) + fdtd_read_listener_series + fdtd_destroy, with the
+ * same kernels and the same results bit for bit, calling no hipMalloc,
+ * hipFree or other allocating routine.
+ *   fields   device memory (256-byte aligned) for the time-stepping state,
+ *            zeroed here and holding the final state (engine layout) on
+ *            return; NULL only queries its size into *fields_bytes_needed
+ *            (host work only, no device call)
+ *   stream   hipStream_t to run on (NULL: the null stream); the call waits
+ *            for its work before it returns
+ *   series   host buffer [num_steps][n_listeners] (FP64)
+ * Everything else -- tables, listener term ring, amplitude ring, counters,
+ * tile lists -- lives in a static device array of 64 MiB per GPU, held until
+ * the process ends; a scene that needs more scratch returns
+ * FDTD_ERR_UNSUPPORTED before any device work.  Calls on one GPU share that
+ * array: the library serialises its own calls per device, and calls that
+ * overlap on one GPU from elsewhere would race on it. */
+fdtd_status fdtd_solve_noalloc(const fdtd_scene *scene, int num_steps, void *fields, int64_t fields_bytes,
+                               void *stream, double *series, int64_t *fields_bytes_needed);
+
 #ifdef __cplusplus
 }
 #endif

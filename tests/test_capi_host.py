@@ -97,3 +97,27 @@ def test_facade_compiles_and_links(tmp_path):
         pytest.skip("a GPU may be present; the probe runs in the GPU tests")
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 3 and "no device" in r.stdout
+
+
+def test_noalloc_size_query_and_scratch_limit():
+    """fdtd_solve_noalloc's host part: with fields NULL it only reports the
+    state bytes (the two pressure slots plus the PML state); a scene whose
+    tables exceed the 64 MiB scratch array is refused before any device call."""
+    from acoustics_amd import scenes
+    lib = capi.load_library()
+    sb, _ = scenes.config1(seed=3, n=32)
+    need = C.c_int64(0)
+    assert lib.fdtd_solve_noalloc(sb.ref(), 10, None, 0, None, None, C.byref(need)) == 0
+    n_pad = (32 * 32 * 32 + 63) // 64 * 64
+    assert need.value >= 2 * 4 * n_pad
+    # 416^3 with an object: the per-cell skip mask alone exceeds 64 MiB
+    big, _ = scenes.config2(seed=3, n=416, pml=8, radius=0.05, n_listeners=4)
+    need_big = C.c_int64(0)
+    assert lib.fdtd_solve_noalloc(big.ref(), 1, None, 0, None, None, C.byref(need_big)) == 0
+    st = lib.fdtd_solve_noalloc(big.ref(), 1, C.c_void_p(256), need_big.value, None,
+                                (C.c_double * 4)(), None)
+    assert st == 6
+    assert b"scratch" in lib.fdtd_last_error()
+    # too small a state buffer
+    st = lib.fdtd_solve_noalloc(sb.ref(), 1, C.c_void_p(256), need.value - 4, None, (C.c_double * 8)(), None)
+    assert st == 1
