@@ -24,7 +24,6 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
     std::vector<SceneDev> devs;
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
-    std::vector<GhostDesc> ghost;
     std::vector<MotionDesc> motion;
     members.clear();
     for (size_t i = 0; i < scenes.size(); ++i) {
@@ -32,12 +31,11 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
         const int id = (int)devs.size();
         devs.push_back(scenes[i]->dev);
         scenes[i]->append_tiles(id, tile_plan(plan), tiles, terms);
-        if (scenes[i]->needs_ghost_kernel()) ghost.push_back({id});
         if (scenes[i]->needs_motion_kernel()) motion.push_back({id});
         members.push_back((int)i);
     }
     if (devs.empty()) { prog.release(); return FDTD_OK; }
-    return prog.upload(devs, tiles, terms, ghost, st, tile_plan(plan).fused, motion);
+    return prog.upload(devs, tiles, terms, st, tile_plan(plan).fused, motion);
 }
 
 void BatchRunner::drop_graph()

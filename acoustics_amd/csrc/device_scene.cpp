@@ -44,7 +44,7 @@ struct Layout {
     SpecialRec *recs;
     GhostDev *ghosts;
     int32_t *op_ptr, *op_idx, *oc_ptr, *oc_obj;
-    double *op_w, *oc_w, *gval0, *gval1;
+    double *op_w, *oc_w;
     double *terms;
     unsigned long long *clock;
     double *acc_tab;
@@ -121,8 +121,6 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
     L.oc_ptr = c.take<int32_t>(S.oc_ptr.size());
     L.oc_obj = c.take<int32_t>(S.oc_obj.size());
     L.oc_w = c.take<double>(S.oc_w.size());
-    L.gval0 = c.take<double>(S.ghosts.size());
-    L.gval1 = c.take<double>(S.ghosts.size());
     L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
     L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
@@ -175,10 +173,6 @@ fdtd_status DeviceScene::reset_to_start(hipStream_t st)
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(hipMemcpyAsync(dev.clock, clk, sizeof clk, hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(fill_amplitudes(st));
-    if (!S.ghosts.empty()) {
-        FDTD_HIP_TRY(hipMemsetAsync(dev.gval[0], 0, S.ghosts.size() * 8, st));
-        FDTD_HIP_TRY(hipMemsetAsync(dev.gval[1], 0, S.ghosts.size() * 8, st));
-    }
     if (S.moving)
         FDTD_HIP_TRY(hipMemcpyAsync(dev.ucls + moving_cells(S), S.mv_cls0.data(), S.mv_cls0.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
@@ -290,10 +284,6 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     FDTD_HIP_TRY(up(L.oc_ptr, S.oc_ptr.data(), S.oc_ptr.size() * 4));
     FDTD_HIP_TRY(up(L.oc_obj, S.oc_obj.data(), S.oc_obj.size() * 4));
     FDTD_HIP_TRY(up(L.oc_w, S.oc_w.data(), S.oc_w.size() * 8));
-    if (!S.ghosts.empty()) {
-        FDTD_HIP_TRY(hipMemsetAsync(L.gval0, 0, S.ghosts.size() * 8, st));
-        FDTD_HIP_TRY(hipMemsetAsync(L.gval1, 0, S.ghosts.size() * 8, st));
-    }
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
@@ -336,8 +326,6 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     D.isrc = L.isrc;
     D.recs = L.recs;
     D.ghosts = L.ghosts;
-    D.gval[0] = L.gval0;
-    D.gval[1] = L.gval1;
     D.G = (int)S.ghosts.size();
     D.op_ptr = L.op_ptr; D.op_idx = L.op_idx; D.op_w = L.op_w;
     D.oc_ptr = L.oc_ptr; D.oc_obj = L.oc_obj; D.oc_w = L.oc_w;
@@ -722,10 +710,13 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
     return std::string();
 }
 
-void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS])
+void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS],
+                        int nhead[NUM_KINDS])
 {
     std::vector<TileDesc> special;
     for (int k = 0; k < NUM_KINDS; ++k) by_kind[k].clear();
+    if (nhead)
+        for (int k = 0; k < NUM_KINDS; ++k) nhead[k] = 0;
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
@@ -735,22 +726,28 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     // neighbouring tiles of a scene share an XCD (and its L2)
     xcd_permute(by_kind[KIND_INTERIOR]);
     xcd_permute(by_kind[KIND_PML]);
-    // the object-adjacent cells ride at the head of the interior launch, one
-    // tile per XCD in turn; the prefix is padded to a multiple of eight so
-    // each interior chunk still lands on a single XCD
+    // the object-adjacent cells (FP64 gathers, latency-bound) ride at the
+    // head of the launch that carries them, one tile per XCD in turn, and
+    // are dispatched first on every step (the kernel keeps this head out of
+    // the odd-step reversal), so their latency hides under the streaming
+    // tiles; the head is padded to a multiple of eight so every later chunk
+    // still lands on a single XCD
     if (!special.empty()) {
         TileDesc empty{};
         empty.type = TILE_EMPTY;
-        while (!by_kind[KIND_INTERIOR].empty() && special.size() % 8) special.push_back(empty);
-        by_kind[KIND_INTERIOR].insert(by_kind[KIND_INTERIOR].begin(), special.begin(), special.end());
+        while (special.size() % 8) special.push_back(empty);
     }
     if (fused_launch) {
-        // PML tiles (the longest) start first; the interior fills in behind them
-        std::vector<TileDesc> all = by_kind[KIND_PML];
+        // then the PML tiles (the longest); the interior fills in behind them
+        std::vector<TileDesc> all = special;
+        all.insert(all.end(), by_kind[KIND_PML].begin(), by_kind[KIND_PML].end());
         all.insert(all.end(), by_kind[KIND_INTERIOR].begin(), by_kind[KIND_INTERIOR].end());
         by_kind[KIND_INTERIOR].swap(all);
         by_kind[KIND_PML].clear();
+    } else {
+        by_kind[KIND_INTERIOR].insert(by_kind[KIND_INTERIOR].begin(), special.begin(), special.end());
     }
+    if (nhead) nhead[KIND_INTERIOR] = (int)special.size();
 }
 
 std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
@@ -766,32 +763,31 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
 // Bytes of the device block of a step program: scene blocks, listener term
 // references, ghost and pre-step (motion, speaker) lists, and the tile list
 // of each launch.
-static void program_bytes(size_t nscenes, size_t nterms, size_t nghost, size_t npre,
-                          const std::vector<TileDesc> by_kind[NUM_KINDS], size_t bytes[4 + NUM_KINDS], size_t *total)
+static void program_bytes(size_t nscenes, size_t nterms, size_t npre,
+                          const std::vector<TileDesc> by_kind[NUM_KINDS], size_t bytes[3 + NUM_KINDS], size_t *total)
 {
     bytes[0] = align_up(sizeof(SceneDev) * std::max<size_t>(1, nscenes));
     bytes[1] = align_up(sizeof(TermRef) * std::max<size_t>(1, nterms));
-    bytes[2] = align_up(sizeof(GhostDesc) * std::max<size_t>(1, nghost));
-    bytes[3] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, npre));
-    *total = bytes[0] + bytes[1] + bytes[2] + bytes[3];
+    bytes[2] = align_up(sizeof(MotionDesc) * std::max<size_t>(1, npre));
+    *total = bytes[0] + bytes[1] + bytes[2];
     for (int k = 0; k < NUM_KINDS; ++k) {
-        bytes[4 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
-        *total += bytes[4 + k];
+        bytes[3 + k] = align_up(sizeof(TileDesc) * std::max<size_t>(1, by_kind[k].size()));
+        *total += bytes[3 + k];
     }
 }
 
 size_t StepProgram::bytes_needed(size_t nscenes, const std::vector<TileDesc> &tiles, bool fused_launch, size_t nterms,
-                                 size_t nghost, size_t npre)
+                                 size_t npre)
 {
     std::vector<TileDesc> by_kind[NUM_KINDS];
     build_launch_lists(tiles, fused_launch, by_kind);
-    size_t bytes[4 + NUM_KINDS], total = 0;
-    program_bytes(nscenes, nterms, nghost, npre, by_kind, bytes, &total);
+    size_t bytes[3 + NUM_KINDS], total = 0;
+    program_bytes(nscenes, nterms, npre, by_kind, bytes, &total);
     return total;
 }
 
 fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
-                                const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
+                                const std::vector<TermRef> &terms, hipStream_t st,
                                 bool fused_launch, const std::vector<MotionDesc> &motion, char *ext_base, size_t ext_bytes)
 {
     // scenes with speaker sources get the speaker pass before their step kernels
@@ -800,7 +796,8 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
         if (scenes[q].n_spk) speaker.push_back({(int)q});
     release();
     std::vector<TileDesc> by_kind[NUM_KINDS];
-    build_launch_lists(tiles, fused_launch, by_kind);
+    int nhead_k[NUM_KINDS];
+    build_launch_lists(tiles, fused_launch, by_kind, nhead_k);
     fused = fused_launch;
     {
         const std::string e = check_launch_lists(by_kind, fused, scenes, terms.size());
@@ -813,8 +810,8 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
             set_last_error("step program: motion list names a scene without moving objects");
             return FDTD_ERR_INVALID;
         }
-    size_t bytes[4 + NUM_KINDS], total = 0;
-    program_bytes(devs.size(), terms.size(), ghost.size(), motion.size() + speaker.size(), by_kind, bytes, &total);
+    size_t bytes[3 + NUM_KINDS], total = 0;
+    program_bytes(devs.size(), terms.size(), motion.size() + speaker.size(), by_kind, bytes, &total);
     char *base = nullptr;
     if (ext_base) {
         if (total > ext_bytes) {
@@ -830,19 +827,16 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     }
     d_scenes = reinterpret_cast<SceneDev *>(base);
     d_terms = reinterpret_cast<TermRef *>(base + bytes[0]);
-    d_ghost = reinterpret_cast<GhostDesc *>(base + bytes[0] + bytes[1]);
-    d_motion = reinterpret_cast<MotionDesc *>(base + bytes[0] + bytes[1] + bytes[2]);
-    size_t off = bytes[0] + bytes[1] + bytes[2] + bytes[3];
+    d_motion = reinterpret_cast<MotionDesc *>(base + bytes[0] + bytes[1]);
+    size_t off = bytes[0] + bytes[1] + bytes[2];
     for (int k = 0; k < NUM_KINDS; ++k) {
         d_tiles[k] = reinterpret_cast<TileDesc *>(base + off);
-        off += bytes[4 + k];
+        off += bytes[3 + k];
         ntiles[k] = (int)by_kind[k].size();
+        head[k] = std::min(nhead_k[k], ntiles[k]);
         if (ntiles[k])
             FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
     }
-    nghost = (int)ghost.size();
-    ghost_max = 0;
-    for (const GhostDesc &g : ghost) ghost_max = std::max(ghost_max, devs[g.scene].G);
     nmotion = (int)motion.size();
     motion_cells = 0;
     for (const MotionDesc &m : motion)
@@ -856,7 +850,6 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     if (!pre.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, pre.data(), sizeof(MotionDesc) * pre.size(), hipMemcpyHostToDevice, st));
     if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
     if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
-    if (!ghost.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_ghost, ghost.data(), sizeof(GhostDesc) * ghost.size(), hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
     order_clock = scenes.empty() ? nullptr : scenes[0].clock + CLK_STEP;
     return FDTD_OK;
@@ -867,12 +860,9 @@ void StepProgram::release()
     if (d_scenes && owns) (void)hipFree(d_scenes);
     d_scenes = nullptr;
     owns = true;
-    for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; }
+    for (int k = 0; k < NUM_KINDS; ++k) { d_tiles[k] = nullptr; ntiles[k] = 0; head[k] = 0; }
     d_terms = nullptr;
-    d_ghost = nullptr;
     d_motion = nullptr;
-    nghost = 0;
-    ghost_max = 0;
     nmotion = 0;
     motion_cells = 0;
     nspeaker = 0;
@@ -886,13 +876,13 @@ fdtd_status StepProgram::enqueue_step(hipStream_t st, long long step, int k, hip
 {
     if (nmotion) FDTD_HIP_TRY(launch_motion(d_scenes, d_motion, nmotion, motion_cells, k, st));
     if (nspeaker) FDTD_HIP_TRY(launch_speaker(d_scenes, d_motion + nmotion, nspeaker, speaker_cells, k, st));
-    if (nghost) FDTD_HIP_TRY(launch_ghost(d_scenes, d_ghost, nghost, ghost_max, k, st));
     if (ev_begin) FDTD_HIP_TRY(hipEventRecord(ev_begin, st));
     auto launch = [&](int kind, hipStream_t s) {
         // fused: every class in the interior list, except the moving-object tiles
-        if (fused && kind == KIND_INTERIOR) return launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
+        if (fused && kind == KIND_INTERIOR)
+            return launch_step(KIND_ALL, d_scenes, d_tiles[kind], ntiles[kind], head[kind], d_terms, order_clock, k, s);
         if (fused && kind == KIND_PML) return hipSuccess;
-        return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], d_terms, order_clock, k, s);
+        return launch_step(kind, d_scenes, d_tiles[kind], ntiles[kind], head[kind], d_terms, order_clock, k, s);
     };
     // (running the PML slabs on a second stream beside the interior was
     // measured slower than one stream and is not offered)

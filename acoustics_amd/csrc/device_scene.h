@@ -86,7 +86,6 @@ struct DeviceScene {
     // Listener values of ring rows [ring_row, ring_row + nrows): FP64 [nrows][L],
     // each the sum of the listener's terms in term order.
     fdtd_status read_series(long long ring_row, long long nrows, double *out) const;
-    bool needs_ghost_kernel() const { return S.boundary == FDTD_BOUNDARY_FULLY_COUPLED && !S.ghosts.empty(); }
     bool needs_motion_kernel() const { return S.moving; }
 };
 
@@ -97,12 +96,15 @@ void xcd_permute(std::vector<TileDesc> &tiles, int nxcd = 8);
 // Splits the tiles of a step into the lists of its launches (per class, or
 // one list when fused) in launch order, and checks them on the host: each
 // tile's class matches its launch and its reads stay inside the grid.
-void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS]);
+// nhead (optional) gets, per launch, the tiles at its head that every step
+// dispatches first (the object-adjacent cells, padded to a multiple of eight).
+void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS],
+                        int nhead[NUM_KINDS] = nullptr);
 std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], bool fused_launch,
                                const std::vector<SceneDev> &scenes, size_t nterms);
 
 // The launch sequence of a set of scenes, uploaded once.  Per step: object
-// motion (scenes with moving objects) -> ghost solve (fully-coupled scenes)
+// motion (scenes with moving objects) -> speaker pass (scenes with speakers)
 // -> the step kernels, which also write the
 // listener terms; per block of up to AMP_ROWS steps, one step advance (one
 // workgroup per scene).
@@ -114,11 +116,11 @@ std::string check_launch_lists(const std::vector<TileDesc> by_kind[NUM_KINDS], b
 struct StepProgram {
     SceneDev *d_scenes = nullptr;
     TileDesc *d_tiles[NUM_KINDS] = {nullptr, nullptr, nullptr};
-    GhostDesc *d_ghost = nullptr;
     MotionDesc *d_motion = nullptr;
     TermRef *d_terms = nullptr;
     int ntiles[NUM_KINDS] = {0, 0, 0};
-    int nghost = 0, ghost_max = 0, nmotion = 0, motion_cells = 0, nspeaker = 0, speaker_cells = 0, nscenes = 0;
+    int head[NUM_KINDS] = {0, 0, 0};   // tiles dispatched first on every step (no odd-step reversal)
+    int nmotion = 0, motion_cells = 0, nspeaker = 0, speaker_cells = 0, nscenes = 0;
     bool fused = false;        // everything in d_tiles[KIND_INTERIOR], launched as KIND_ALL
     const unsigned long long *order_clock = nullptr;  // step word of the first scene
     // tiles of every kind in one list; split by type here (or kept in one
@@ -126,12 +128,12 @@ struct StepProgram {
     // (ext_base: place the program there, ext_bytes long, instead of
     // allocating it; release() then frees nothing)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
-                       const std::vector<TermRef> &terms, const std::vector<GhostDesc> &ghost, hipStream_t st,
+                       const std::vector<TermRef> &terms, hipStream_t st,
                        bool fused_launch = false, const std::vector<MotionDesc> &motion = {}, char *ext_base = nullptr,
                        size_t ext_bytes = 0);
     // Bytes upload() places for these lists (npre: motion plus speaker entries).
     static size_t bytes_needed(size_t nscenes, const std::vector<TileDesc> &tiles, bool fused_launch, size_t nterms,
-                               size_t nghost, size_t npre);
+                               size_t npre);
     bool owns = true;          // the program's block came from hipMalloc
     void release();
     // Enqueues n steps from `step` (the host's count of the step the first

@@ -90,7 +90,6 @@ struct SceneDev {
     const ImpDev *isrc;
     const SpecialRec *recs;
     const GhostDev *ghosts;
-    double *gval[2];            // ghost values by step parity (fully-coupled mode)
     // fully-coupled mode: the coupled system's solution operator, per ghost g
     // x_g = sum_q op_w[q] p[op_idx[q]] (q in [op_ptr[g], op_ptr[g+1]))
     //     + sum_q oc_w[q] a_n(oc_obj[q])  (q in [oc_ptr[g], oc_ptr[g+1]))
@@ -159,10 +158,6 @@ struct TileDesc {
                                  // a multiple of four) that belong to another tile and are not written
 };
 
-struct GhostDesc {
-    int scene;
-};
-
 struct MotionDesc {
     int scene;
 };
@@ -174,15 +169,14 @@ enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3, K
 // (plus k) selects the tile order -- odd steps walk each XCD's chunk of the
 // tile list backwards, so a step starts on the data the previous step touched
 // last and the Infinity Cache still holds.  k: the step's offset in its block.
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
+// nhead: the first nhead tiles (a multiple of eight) keep their place on odd
+// steps too.
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, int nhead, const TermRef *terms,
                        const unsigned long long *order_clock, int k, hipStream_t s);
 // Moves every scene of the launch n steps on (clamped to its target) and
 // refills its amplitude ring for the AMP_ROWS steps from there; n = 0 only
 // fills the ring.
 hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s);
-// Fully-coupled ghost values of step clock + k, one thread per ghost
-// (max_ghosts: the largest ghost count of the list), one scene per y.
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int max_ghosts, int k, hipStream_t s);
 // Moving objects, before the step kernels of step clock + k: poses, per-step
 // reclassification of each scene's box, zeroed new solid cells and filled
 // fresh cells; workgroups of 256 cells along x (max_cells: the largest box of

@@ -105,6 +105,37 @@ __device__ __forceinline__ float speaker_at(const SceneDev &S, int i, int j, int
 }
 
 // ---------------------------------------------------------------------------
+// fully-coupled ghost values (MAC_Grid.cpp:1131-1441): the coupled system's
+// static solution operator applied to p^n and the accelerations of step n,
+// x_g = sum_q M_gq p_q + sum_o m_go a_n(o), FP64, in row order.  Computed where
+// it is used (the object-adjacent cell next to the ghost, and listener terms),
+// four independent index loads and gathers at a time.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double coupled_ghost(const SceneDev &S, int g, const AS_GLOBAL float *__restrict__ pcur,
+                                                long long step)
+{
+    const AS_GLOBAL int32_t *idx = gmem(S.op_idx);
+    const AS_GLOBAL double *w = gmem(S.op_w);
+    int q = gmem(S.op_ptr)[g];
+    const int q1 = gmem(S.op_ptr)[g + 1];
+    double x = 0.0;
+    for (; q + 4 <= q1; q += 4) {
+        const int i0 = idx[q], i1 = idx[q + 1], i2 = idx[q + 2], i3 = idx[q + 3];
+        const double w0 = w[q], w1 = w[q + 1], w2 = w[q + 2], w3 = w[q + 3];
+        const float v0 = pcur[i0], v1 = pcur[i1], v2 = pcur[i2], v3 = pcur[i3];
+        x = fma(w0, (double)v0, x);
+        x = fma(w1, (double)v1, x);
+        x = fma(w2, (double)v2, x);
+        x = fma(w3, (double)v3, x);
+    }
+    for (; q < q1; ++q) x = fma(w[q], (double)pcur[idx[q]], x);
+    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_obj;
+    const int c1 = gmem(S.oc_ptr)[g + 1];
+    for (int c = gmem(S.oc_ptr)[g]; c < c1; ++c) x = fma(gmem(S.oc_w)[c], acc[gmem(S.oc_obj)[c]], x);
+    return x;
+}
+
+// ---------------------------------------------------------------------------
 // listener terms (PML_WaveSolver::FetchScalarData, MLSModeInterpolator::lookup
 // with precomputed weights): the thread that produces a term's sample writes
 // beta * sample into the scene's series ring; the terms of a listener are
@@ -130,7 +161,7 @@ __device__ __forceinline__ void emit_ghost_terms(const SceneDev &S, const TileDe
             const GhostDev gd = gmem(S.ghosts)[r.cell];
             v = (double)pcur[gd.nbr] + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
         } else {
-            v = gmem(S.gval[step & 1])[r.cell];
+            v = coupled_ghost(S, r.cell, pcur, step);
         }
         gmem(S.terms)[(size_t)(step % S.cap) * S.T + r.slot] = r.beta * v;
     }
@@ -776,7 +807,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
                 const GhostDev gd = gmem(S.ghosts)[gid];
                 g[q] = c + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
             } else {
-                g[q] = gmem(S.gval[step & 1])[gid];
+                g[q] = coupled_ghost(S, gid, pcur, step);
             }
         }
     }
@@ -1189,11 +1220,14 @@ __device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step
 // Tile of workgroup w.  Lists of 16 tiles or more are padded to a multiple of
 // eight and arranged so that workgroup w (dispatched to XCD w % 8) runs chunk
 // w % 8; the reversed order keeps that assignment and reverses each chunk.
-__device__ __forceinline__ int tile_of(int w, int ntiles, const unsigned long long *order_clock, int k)
+// The first nhead tiles (a multiple of eight) are dispatched first on every
+// step; the reversal applies to the rest.
+__device__ __forceinline__ int tile_of(int w, int ntiles, int nhead, const unsigned long long *order_clock, int k)
 {
-    if (!order_clock || !((*order_clock + (unsigned long long)k) & 1ULL)) return w;
-    if (ntiles >= 16 && (ntiles & 7) == 0) return (ntiles / 8 - 1 - w / 8) * 8 + (w & 7);
-    return ntiles - 1 - w;
+    if (w < nhead || !order_clock || !((*order_clock + (unsigned long long)k) & 1ULL)) return w;
+    const int r = w - nhead, nr = ntiles - nhead;
+    if (nr >= 16 && (nr & 7) == 0) return nhead + (nr / 8 - 1 - r / 8) * 8 + (r & 7);
+    return nhead + nr - 1 - r;
 }
 
 // Step advance (PML_WaveSolver::stepSystem's time update), one workgroup per
@@ -1230,10 +1264,10 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
 // disjoint cells and can run in any order.
 template <int KIND>
 __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ scenes, const TileDesc *__restrict__ tiles,
-                                                   int ntiles, const TermRef *__restrict__ terms,
+                                                   int ntiles, int nhead, const TermRef *__restrict__ terms,
                                                    const unsigned long long *__restrict__ order_clock, int k)
 {
-    const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, order_clock, k)];
+    const TileDesc t = gmem(tiles)[tile_of(blockIdx.x, ntiles, nhead, order_clock, k)];
     if (t.type == TILE_EMPTY) return;
     const SceneDev &S = scenes[t.scene];
     const long long step = (long long)S.clock[CLK_STEP] + k;
@@ -1267,37 +1301,15 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
     if (t.te > t.tb) emit_ghost_terms(S, t, terms, step);
 }
 
-// ---------------------------------------------------------------------------
-// fully-coupled ghost values (MAC_Grid.cpp:1131-1441): the coupled system's
-// static solution operator applied to p^n and the accelerations of step n,
-// x_g = sum_q M_gq p_q + sum_o m_go a_n(o), one thread per ghost, FP64.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void ghost_kernel(const SceneDev *__restrict__ scenes, const GhostDesc *__restrict__ list, int k)
-{
-    const SceneDev &S = scenes[list[blockIdx.y].scene];
-    const long long step = (long long)S.clock[CLK_STEP] + k;
-    if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const int g = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-    if (g >= S.G) return;
-    const AS_GLOBAL float *__restrict__ pcur = gmem(S.field + (size_t)slot_cur(step) * S.npad);
-    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_obj;
-    const int q0 = gmem(S.op_ptr)[g], q1 = gmem(S.op_ptr)[g + 1];
-    double x = 0.0;
-    for (int q = q0; q < q1; ++q) x += gmem(S.op_w)[q] * (double)pcur[gmem(S.op_idx)[q]];
-    const int c0 = gmem(S.oc_ptr)[g], c1 = gmem(S.oc_ptr)[g + 1];
-    for (int q = c0; q < c1; ++q) x += gmem(S.oc_w)[q] * acc[gmem(S.oc_obj)[q]];
-    gmem(S.gval[step & 1])[g] = x;
-}
-
-hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, const TermRef *terms,
+hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, int nhead, const TermRef *terms,
                        const unsigned long long *order_clock, int k, hipStream_t s)
 {
     if (ntiles <= 0) return hipSuccess;
     switch (kind) {
-    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
-    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
-    case KIND_SPECIAL: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
-    default: hipLaunchKernelGGL(step_kernel<KIND_ALL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, terms, order_clock, k); break;
+    case KIND_INTERIOR: hipLaunchKernelGGL(step_kernel<KIND_INTERIOR>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, nhead, terms, order_clock, k); break;
+    case KIND_PML: hipLaunchKernelGGL(step_kernel<KIND_PML>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, nhead, terms, order_clock, k); break;
+    case KIND_SPECIAL: hipLaunchKernelGGL(step_kernel<KIND_SPECIAL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, nhead, terms, order_clock, k); break;
+    default: hipLaunchKernelGGL(step_kernel<KIND_ALL>, dim3(ntiles), dim3(256), 0, s, scenes, tiles, ntiles, nhead, terms, order_clock, k); break;
     }
     return hipGetLastError();
 }
@@ -1341,11 +1353,5 @@ hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, 
     return hipGetLastError();
 }
 
-hipError_t launch_ghost(const SceneDev *scenes, const GhostDesc *list, int n, int max_ghosts, int k, hipStream_t s)
-{
-    if (n <= 0 || max_ghosts <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ghost_kernel, dim3((unsigned)((max_ghosts + 255) / 256), (unsigned)n), dim3(256), 0, s, scenes, list, k);
-    return hipGetLastError();
-}
 
 }  // namespace fdtd

@@ -92,13 +92,11 @@ extern "C" fdtd_status fdtd_solve_noalloc(const fdtd_scene *scene, int num_steps
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
     D.append_tiles(0, tp, tiles, terms);
-    std::vector<GhostDesc> ghost;
-    if (D.needs_ghost_kernel()) ghost.push_back({0});
     std::vector<MotionDesc> motion;
     if (D.needs_motion_kernel()) motion.push_back({0});
     const size_t npre = motion.size() + (D.S.spk.empty() ? 0 : 1);
     const size_t table_region = (table_b + 255) & ~(size_t)255;
-    const size_t prog_b = StepProgram::bytes_needed(1, tiles, tp.fused, terms.size(), ghost.size(), npre);
+    const size_t prog_b = StepProgram::bytes_needed(1, tiles, tp.fused, terms.size(), npre);
     if (table_region + prog_b > kScratchBytes) {
         set_last_error("the scene's tables and step program need " + std::to_string(table_region + prog_b) +
                        " bytes of scratch; fdtd_solve_noalloc holds at most " + std::to_string(kScratchBytes));
@@ -118,7 +116,7 @@ extern "C" fdtd_status fdtd_solve_noalloc(const fdtd_scene *scene, int num_steps
     st = D.place(s, cap, LLONG_MAX, (char *)fields, base);
     if (st != FDTD_OK) return st;
     StepProgram prog;
-    st = prog.upload({D.dev}, tiles, terms, ghost, s, tp.fused, motion, base + table_region, kScratchBytes - table_region);
+    st = prog.upload({D.dev}, tiles, terms, s, tp.fused, motion, base + table_region, kScratchBytes - table_region);
     if (st != FDTD_OK) return st;
     const int L = D.S.L;
     for (long long step = 0; step < num_steps;) {

@@ -97,7 +97,6 @@ fdtd_status fdtd_server::rebuild()
     std::vector<SceneDev> devs;
     std::vector<TileDesc> tiles;
     std::vector<TermRef> terms;
-    std::vector<GhostDesc> ghost;
     std::vector<MotionDesc> motion;
     for (Request *r : active) {
         const int id = (int)devs.size();
@@ -105,10 +104,9 @@ fdtd_status fdtd_server::rebuild()
         const int tb = (int)terms.size();
         for (TermRef q : r->terms) { q.scene = id; terms.push_back(q); }
         for (TileDesc t : r->tiles) { t.scene = id; t.tb += tb; t.te += tb; tiles.push_back(t); }
-        if (r->dev->needs_ghost_kernel()) ghost.push_back({id});
         if (r->dev->needs_motion_kernel()) motion.push_back({id});
     }
-    return prog.upload(devs, tiles, terms, ghost, stream, tile_plan(0).fused, motion);
+    return prog.upload(devs, tiles, terms, stream, tile_plan(0).fused, motion);
 }
 
 void fdtd_server::run()

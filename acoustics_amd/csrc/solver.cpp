@@ -127,12 +127,10 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
     if (st == FDTD_OK) {
         std::vector<TileDesc> tiles;
         std::vector<TermRef> terms;
-        std::vector<GhostDesc> ghost;
         h->D.append_tiles(0, tile_plan(h->tile), tiles, terms);
-        if (h->D.needs_ghost_kernel()) ghost.push_back({0});
         std::vector<MotionDesc> motion;
         if (h->D.needs_motion_kernel()) motion.push_back({0});
-        st = h->prog.upload({h->D.dev}, tiles, terms, ghost, h->stream, tile_plan(h->tile).fused, motion);
+        st = h->prog.upload({h->D.dev}, tiles, terms, h->stream, tile_plan(h->tile).fused, motion);
     }
     if (st != FDTD_OK) {
         h->prog.release();

@@ -30,7 +30,7 @@ STEP_KERNELS = ("step_kernel<0>", "step_kernel<1>", "step_kernel<2>", "step_kern
 KERNEL_ROLE = {"step_kernel<0>": "interior + object-adjacent cells", "step_kernel<1>": "PML slabs",
                "step_kernel<2>": "object-adjacent cells",
                "step_kernel<3>": "one step, all tile classes (PML, interior, object-adjacent) + listener terms",
-               "advance_kernel": "block advance (step counter, amplitude ring for the next 64 steps)", "ghost_kernel": "coupled ghost solve"}
+               "advance_kernel": "block advance (step counter, amplitude ring for the next 64 steps)"}
 
 
 def copy(mib, iters):

@@ -22,9 +22,11 @@ Prints ONE JSON line:
   moving       a 256^3 scene with a moving, spinning ellipsoid, and the same
                ellipsoid held still
   batch        64 uneven scenes advanced together vs one at a time
-  serving      a Poisson request stream: continuous batching vs the plain
-               one-request-at-a-time path
-  cpu_baseline FP64 CPU oracle (OpenMP) on a bounded sample of the same scene
+  coupled      the same scene with fully-coupled ghost cells
+  serving      a Poisson request stream (200 requests): continuous batching vs
+               the plain one-request-at-a-time path (all steps in one call)
+  cpu_baseline FP64 CPU oracle (OpenMP, all host threads granted; plus one
+               thread) on bounded samples of the three single-scene workloads
 """
 from __future__ import annotations
 
@@ -53,7 +55,9 @@ def parse():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=120)
-    ap.add_argument("--serve", type=int, default=24, help="requests in the serving comparison (0: skip)")
+    ap.add_argument("--serve", type=int, default=200, help="requests in the serving comparison (0: skip)")
+    ap.add_argument("--serve-per-step", action="store_true",
+                    help="also time the plain path in the reference's per-step driver pattern")
     ap.add_argument("--block", type=int, default=64, help="server steps per block")
     ap.add_argument("--staggered-steps", type=int, default=200,
                     help="steps of the staggered-scheme measurement (0: skip)")
@@ -63,24 +67,67 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(sb, steps):
-    """FP64 oracle on the same scene, timed over the stepping loop only."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _omp_threads(n):
+    """Sets the OpenMP thread count of the calling thread (the oracle's loops)."""
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def cpu_baseline(steps3):
+    """FP64 oracle (the CPU restatement of the reference path, OpenMP) timed
+    over its stepping loop on bounded samples of the three single-scene
+    workloads: 64^3 free field (500 steps, all), 128^3 sphere (200 of 2000
+    steps), 256^3 five ellipsoids (steps3 of 4000), each with every host
+    thread this process may use, plus a 1-thread run of the 128^3 sample.
+    Per-step reclassification as in the reference; the value is the 256^3
+    rate."""
+    from acoustics_amd import scenes
     from oracle.oracle import Oracle, build
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    nproc = os.cpu_count() or 1
     if threads <= 0:
-        threads = min(16, os.cpu_count() or 1)
-        os.environ["OMP_NUM_THREADS"] = str(threads)
+        threads = nproc
     build()
-    o = Oracle(sb, reclassify_each_step=True)
-    o.step(1)
-    t0 = time.perf_counter()
-    o.step(steps)
-    t = time.perf_counter() - t0
-    o.close()
-    n = sb.n_cells
-    return {"value": n * steps / t / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{sb.divisions[0]}^3 headline scene, {steps} FP64 steps after 1 warmup step, "
-                      f"per-step reclassification as in the reference, {t:.2f} s"}
+
+    def rate(sb, steps, nthreads):
+        _omp_threads(nthreads)
+        o = Oracle(sb, reclassify_each_step=True)
+        o.step(1, record=False)
+        t0 = time.perf_counter()
+        o.step(steps, record=False)
+        t = time.perf_counter() - t0
+        o.close()
+        return sb.n_cells * steps / t / 1e9, t
+
+    out = {}
+    samples = [("cfg1_64cube_free_field", scenes.config1(seed=1)[0], 500),
+               ("cfg2_128cube_sphere", scenes.config2(seed=2)[0], 200),
+               ("cfg3_256cube_ellipsoids", scenes.config3(seed=3)[0], steps3)]
+    for name, sb, steps in samples:
+        g, t = rate(sb, steps, threads)
+        out[name] = {"Gcell_per_s": round(g, 4), "steps": steps, "seconds": round(t, 2)}
+    g1, t1 = rate(samples[1][1], 20, 1)
+    _omp_threads(threads)
+    return {"value": out["cfg3_256cube_ellipsoids"]["Gcell_per_s"], "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"256^3 headline scene, {steps3} FP64 steps after 1 warmup step (of 4000), per-step "
+                      f"reclassification as in the reference, {threads} OpenMP threads",
+            "nproc": nproc, "cpu_model": _cpu_model(), "configs": out,
+            "single_thread": {"workload": "cfg2_128cube_sphere", "steps": 20, "Gcell_per_s": round(g1, 5),
+                              "seconds": round(t1, 2)}}
 
 
 def staggered(steps, n=256):
@@ -128,6 +175,19 @@ def _timed(sb, steps, warm=20):
     return info, wall, step_ms, finite
 
 
+def coupled(steps, n=256):
+    """The headline scene with fully-coupled ghost cells (MAC_Grid.cpp:
+    1312-1425): per step, the coupled system's static solution operator
+    (one FP64 pass over the ghost cells) runs before the step kernel."""
+    from acoustics_amd import scenes
+    from acoustics_amd.capi import FDTD_BOUNDARY_FULLY_COUPLED
+    sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n, boundary=FDTD_BOUNDARY_FULLY_COUPLED)
+    info, wall, step_ms, finite = _timed(sb, steps)
+    return {"workload": f"{n}^3 headline scene with fully-coupled ghost cells ({info['n_ghost_cells']} ghost cells)",
+            "value": round(info["n_cells"] * steps / wall / 1e9, 3), "unit": "Gcell-updates/s", "steps": steps,
+            "step_ms": round(step_ms, 5), "listeners_finite": finite}
+
+
 def other_paths(steps, n=256):
     """Two more paths at n^3 (h = 0.64/n): the Engquist boundary on a
     free field without PML (one pressure pulse, 8 listeners), and a moving
@@ -168,6 +228,11 @@ def batched(n_scenes):
     from acoustics_amd.solver import Batch, Solver
     sbs, steps = scenes.config4(seed=4, n_scenes=n_scenes)
     cells = sum(sb.n_cells * st for sb, st in zip(sbs, steps))
+
+    def step_bytes(sb):   # 12 N + 44 N_pml with the 8-cell PML on every face
+        d = np.asarray(sb.divisions)
+        return 12 * int(np.prod(d)) + 44 * (int(np.prod(d)) - int(np.prod(d - 16)))
+    alg_bytes = sum(step_bytes(sb) * st for sb, st in zip(sbs, steps))
     t0 = time.perf_counter()
     with Batch(sbs, steps) as b:
         t1 = time.perf_counter()
@@ -185,7 +250,11 @@ def batched(n_scenes):
             "batched_Gcell_per_s": round(cells / t_batch / 1e9, 3),
             "batched_wall_s": round(t_batch, 3), "batched_wall_with_setup_s": round(t_batch_all, 3),
             "one_at_a_time_Gcell_per_s": round(cells / t_plain / 1e9, 3), "one_at_a_time_wall_s": round(t_plain, 3),
-            "speedup": round(t_plain / t_batch_all, 3), "listeners_finite": ok}
+            "speedup": round(t_plain / t_batch_all, 3), "listeners_finite": ok,
+            "roofline": {"bound": "hbm", "algorithmic_bytes": alg_bytes,
+                         "achieved": round(alg_bytes / t_batch / 1e9, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": round(alg_bytes / t_batch / 1e9 / HBM_PEAK_GBPS, 4),
+                         "source": "sum over scenes of (12 N + 44 N_pml) x steps / batched wall time"}}
 
 
 def _percentile(xs, q):
@@ -193,7 +262,7 @@ def _percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q)) if xs else None
 
 
-def serving(n_requests, block):
+def serving(n_requests, block, per_step=False):
     """Poisson stream of impact-sound requests (rate 200/s, edges 48-192,
     1-3 ellipsoids, 32 listeners, 0.01-0.05 s of sound), served three ways:
       served      the continuous-batching server (all admitted requests share
@@ -258,16 +327,19 @@ def serving(n_requests, block):
         return {"solves_per_s": round(len(stream) / wall, 3), "p50_ms": round(_percentile(lats, 50), 2),
                 "p99_ms": round(_percentile(lats, 99), 2), "wall_s": round(wall, 3)}
 
-    plain_lat, t_plain = one_at_a_time(True)
     bulk_lat, t_bulk = one_at_a_time(False)
-    return {
+    out = {
         "requests": len(stream), "cell_updates": cells_steps, "steps_per_block": block,
         "served": summary(served_lat, t_served),
-        "plain": summary(plain_lat, t_plain),
         "plain_bulk": summary(bulk_lat, t_bulk),
-        "speedup_solves_per_s": round(t_plain / t_served, 3),
         "speedup_vs_plain_bulk": round(t_bulk / t_served, 3),
+        "served_Gcell_per_s": round(cells_steps / t_served / 1e9, 2),
     }
+    if per_step:
+        plain_lat, t_plain = one_at_a_time(True)
+        out["plain_per_step"] = summary(plain_lat, t_plain)
+        out["speedup_vs_plain_per_step"] = round(t_plain / t_served, 3)
+    return out
 
 
 def main():
@@ -331,6 +403,9 @@ def main():
                    "cells": N, "pml_cells": Npml, "bytes_per_step": bytes_step, "listeners_finite": finite},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "traffic_source": "not measured in this run: HBM bytes per launch from the committed counter "
+                                       "summary profiles/step_kernel_256cube_traffic.json (rocprofv3 --pmc "
+                                       "FETCH_SIZE, WRITE_SIZE, scaled by a copy kernel of known size)",
                      "kernel": "step_kernel", "kernel_ms": round(kern_ms, 5),
                      "kernel_ms_source": "HIP events around the timed steps on the solver stream, per step",
                      "state_bytes": state_bytes,
@@ -339,13 +414,15 @@ def main():
     if a.staggered_steps > 0:
         out["staggered"] = staggered(a.staggered_steps, n)
     if a.other_steps > 0:
+        out["coupled"] = coupled(a.other_steps, n)
+        out["coupled"]["ghost_ms_per_step"] = round(out["coupled"]["step_ms"] - kern_ms, 5)
         out.update(other_paths(a.other_steps, n))
     if a.batch > 0:
         out["batch"] = batched(a.batch)
     if a.serve > 0:
-        out["serving"] = serving(a.serve, a.block)
+        out["serving"] = serving(a.serve, a.block, a.serve_per_step)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sb, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_steps)
     print(json.dumps(out), flush=True)
 
 
