@@ -412,6 +412,24 @@ void for_pml_faces(const SceneSetup &S, int d, F fn)
     }
 }
 
+// A cell of a merging slab is merged when both axes other than the slab axis
+// are plain there (the kernel's rule; x is plain over y and z slabs that merge).
+inline bool merged_cell(const SceneSetup &S, const PmlSlab &sl, int j, int k)
+{
+    return sl.merge && (sl.axis == 1 || axis_plain(S, 1, j)) && (sl.axis == 2 || axis_plain(S, 2, k));
+}
+
+// The engine's own state file: a header, then the whole time-stepping state
+// region of the device as it lies there (pressure slots, PML state, viscous
+// Laplacian), so a handle of the same scene and state representation resumes
+// bit for bit.
+struct EngineStateHeader {
+    char magic[8];
+    int32_t version, pml_state, scheme, pad;
+    int64_t step, n_cells, state_floats, npad;
+};
+constexpr char kStateMagic[8] = {'F', 'D', 'T', 'D', 'S', 'T', 'A', 'T'};
+
 }  // namespace
 
 extern "C" {
@@ -423,21 +441,12 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
         set_last_error("snapshots do not hold the per-step classification of moving objects");
         return FDTD_ERR_UNSUPPORTED;
     }
-    if (h->D.S.visc) {
-        set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
-                       "(_SaveSimulationSnapshot writes pressures and PML state only)");
-        return FDTD_ERR_UNSUPPORTED;
-    }
-    if (h->D.S.n_pml_merged > 0) {
-        set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
-                       "FDTD_PML_STATE_SPLIT");
-        return FDTD_ERR_UNSUPPORTED;
-    }
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const SceneSetup &S = h->D.S;
     const long long N = S.N, s = h->step;
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
+    const bool merged = S.n_pml_merged > 0;
     std::vector<float> f(N);
     std::vector<double> cur(N), prev(N), zero(N, 0.0);
     FDTD_HIP_TRY(h->D.download_grid(h->D.slot(slot_cur(s)), f.data()));
@@ -460,33 +469,108 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
         os << s << "\n";
         if (!of || !os) { set_last_error("cannot write snapshot text files"); return FDTD_ERR_INVALID; }
     }
-    std::vector<float> comp;
+    // split pressures; a merged cell keeps pDir_A and the increment of the sum
+    // q of its transverse splits, so its files hold pDir_A on the slab axis,
+    // q = p^n - pDir_A on the lower transverse axis and zero on the other
+    std::vector<float> comp[3];
     for (int d = 0; d < 3; ++d) {
-        std::vector<double> full(N, 0.0);
-        comp.resize(std::max<int64_t>(1, S.n_pml));
-        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.pd[d], S.n_pml * 4, hipMemcpyDeviceToHost));
-        for_pml_cells(S, [&](long long c, long long g) { full[g] = comp[c]; });
-        if ((st = put("p_pml_" + std::to_string(d) + ".dat", full, N)) != FDTD_OK) return st;
+        comp[d].resize(std::max<int64_t>(1, S.n_pml));
+        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(comp[d].data(), h->D.dev.pd[d], S.n_pml * 4, hipMemcpyDeviceToHost));
     }
+    std::vector<double> full[3];
+    for (int d = 0; d < 3; ++d) full[d].assign(N, 0.0);
+    for (const PmlSlab &sl : S.slabs)
+        for (int li = 0; li < sl.e[0]; ++li)
+            for (int lj = 0; lj < sl.e[1]; ++lj)
+                for (int lk = 0; lk < sl.e[2]; ++lk) {
+                    const long long c = sl.cell_base + ((long long)li * sl.e[1] + lj) * sl.e[2] + lk;
+                    const int j = sl.lo[1] + lj, k = sl.lo[2] + lk;
+                    const long long g = ((long long)(sl.lo[0] + li) * S.n[1] + j) * S.n[2] + k;
+                    if (!merged_cell(S, sl, j, k)) {
+                        for (int d = 0; d < 3; ++d) full[d][g] = comp[d][c];
+                    } else {
+                        const int A = sl.axis, lower = A == 0 ? 1 : 0;
+                        full[A][g] = comp[A][c];
+                        full[lower][g] = cur[g] - (double)comp[A][c];
+                    }
+                }
+    for (int d = 0; d < 3; ++d)
+        if ((st = put("p_pml_" + std::to_string(d) + ".dat", full[d], N)) != FDTD_OK) return st;
     const int vb = (int)(s & 1);  // the buffer the next step reads
+    std::vector<float> vc;
     for (int d = 0; d < 3; ++d) {
         const long long NV = (long long)(S.n[0] + (d == 0)) * (S.n[1] + (d == 1)) * (S.n[2] + (d == 2));
-        std::vector<double> full(NV, 0.0);
+        std::vector<double> fv(NV, 0.0);
         if (mac) {
-            comp.resize(N);
-            FDTD_HIP_TRY(h->D.download_grid(h->D.dev.vel[vb][d], comp.data()));
+            vc.resize(N);
+            FDTD_HIP_TRY(h->D.download_grid(h->D.dev.vel[vb][d], vc.data()));
             for (long long i = 0; i < S.n[0]; ++i)
                 for (long long j = 0; j < S.n[1]; ++j)
                     for (long long k = 0; k < S.n[2]; ++k)
-                        full[face_index(S.n, d, i, j, k)] = comp[(i * S.n[1] + j) * S.n[2] + k];
+                        fv[face_index(S.n, d, i, j, k)] = vc[(i * S.n[1] + j) * S.n[2] + k];
         } else {
-            comp.resize(std::max<int64_t>(1, S.n_pml_faces[d]));
+            vc.resize(std::max<int64_t>(1, S.n_pml_faces[d]));
             if (S.n_pml_faces[d])
-                FDTD_HIP_TRY(hipMemcpy(comp.data(), h->D.dev.vel[vb][d], S.n_pml_faces[d] * 4, hipMemcpyDeviceToHost));
-            for_pml_faces(S, d, [&](long long c, long long g) { full[g] = comp[c]; });
+                FDTD_HIP_TRY(hipMemcpy(vc.data(), h->D.dev.vel[vb][d], S.n_pml_faces[d] * 4, hipMemcpyDeviceToHost));
+            // a transverse face between two merged cells is not kept by the
+            // merged state (the files hold zero there); every other face is
+            for (const PmlSlab &sl : S.slabs) {
+                const int E[3] = {sl.e[0] + (d == 0), sl.e[1] + (d == 1), sl.e[2] + (d == 2)};
+                for (int li = 0; li < E[0]; ++li)
+                    for (int lj = 0; lj < E[1]; ++lj)
+                        for (int lk = 0; lk < E[2]; ++lk) {
+                            const int i = sl.lo[0] + li, j = sl.lo[1] + lj, k = sl.lo[2] + lk;
+                            if (d != sl.axis && sl.merge) {
+                                // the cells on either side of the face inside this box
+                                const int lo_l = (d == 1 ? lj : lk) - 1, hi_l = (d == 1 ? lj : lk);
+                                const int ext = d == 1 ? sl.e[1] : sl.e[2];
+                                bool kept = false;
+                                for (int side = 0; side < 2; ++side) {
+                                    const int l = side ? hi_l : lo_l;
+                                    if (l < 0 || l >= ext) continue;
+                                    const int jj = d == 1 ? sl.lo[1] + l : j, kk = d == 2 ? sl.lo[2] + l : k;
+                                    kept = kept || !merged_cell(S, sl, jj, kk);
+                                }
+                                if (!kept) continue;
+                            }
+                            const long long c = sl.v_base[d] + ((long long)li * E[1] + lj) * E[2] + lk;
+                            fv[face_index(S.n, d, i, j, k)] = vc[c];
+                        }
+            }
         }
-        if ((st = put("v_pml_" + std::to_string(d) + ".dat", full, NV)) != FDTD_OK) return st;
+        if ((st = put("v_pml_" + std::to_string(d) + ".dat", fv, NV)) != FDTD_OK) return st;
     }
+    // the engine's own state, for a bit-exact resume of any state representation
+    {
+        EngineStateHeader hd{};
+        std::memcpy(hd.magic, kStateMagic, 8);
+        hd.version = 1;
+        hd.pml_state = merged ? FDTD_PML_STATE_MERGED : FDTD_PML_STATE_SPLIT;
+        hd.scheme = S.scheme;
+        hd.step = s;
+        hd.n_cells = N;
+        hd.state_floats = (int64_t)h->D.state_floats;
+        hd.npad = h->D.npad;
+        std::vector<float> raw(h->D.state_floats);
+        FDTD_HIP_TRY(hipMemcpy(raw.data(), h->D.dev.field, raw.size() * 4, hipMemcpyDeviceToHost));
+        std::ofstream os(snap_name(pattern, simulator_id, numbering, "engine_state.bin"), std::ios::binary);
+        os.write(reinterpret_cast<const char *>(&hd), sizeof hd);
+        os.write(reinterpret_cast<const char *>(raw.data()), (std::streamsize)(raw.size() * 4));
+        if (!os) { set_last_error("cannot write the engine state file"); return FDTD_ERR_INVALID; }
+    }
+    return FDTD_OK;
+}
+
+// Sets the clock of step s and the host's series bookkeeping after a load.
+static fdtd_status resume_at(fdtd_handle *h, long long s)
+{
+    const unsigned long long clk[CLK_WORDS] = {(unsigned long long)s, (unsigned long long)LLONG_MAX, 0ULL, 0ULL};
+    FDTD_HIP_TRY(hipMemcpy(h->D.dev.clock, clk, sizeof clk, hipMemcpyHostToDevice));
+    FDTD_HIP_TRY(h->D.fill_amplitudes(h->stream));
+    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
+    h->step = s;
+    h->flushed = s;
+    h->host_series.assign((size_t)s * h->D.S.L, 0.0);
     return FDTD_OK;
 }
 
@@ -497,21 +581,38 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
         set_last_error("snapshots do not hold the per-step classification of moving objects");
         return FDTD_ERR_UNSUPPORTED;
     }
-    if (h->D.S.visc) {
-        set_last_error("snapshots do not hold the previous step's Laplacian that air viscosity needs "
-                       "(_SaveSimulationSnapshot writes pressures and PML state only)");
-        return FDTD_ERR_UNSUPPORTED;
-    }
-    if (h->D.S.n_pml_merged > 0) {
-        set_last_error("snapshots hold the per-axis split PML state: create the handle with pml_state = "
-                       "FDTD_PML_STATE_SPLIT");
-        return FDTD_ERR_UNSUPPORTED;
-    }
     FDTD_HIP_TRY(hipSetDevice(h->D.device));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
     const SceneSetup &S = h->D.S;
     const long long N = S.N;
     const bool mac = S.scheme == FDTD_SCHEME_STAGGERED;
+    const bool merged = S.n_pml_merged > 0;
+    // the engine's own state file, when it matches this handle, resumes bit for bit
+    {
+        std::ifstream is(snap_name(pattern, simulator_id, numbering, "engine_state.bin"), std::ios::binary);
+        EngineStateHeader hd{};
+        if (is && is.read(reinterpret_cast<char *>(&hd), sizeof hd) && std::memcmp(hd.magic, kStateMagic, 8) == 0 &&
+            hd.version == 1 && hd.n_cells == N && hd.scheme == S.scheme && hd.state_floats == (int64_t)h->D.state_floats &&
+            hd.npad == h->D.npad && hd.pml_state == (merged ? FDTD_PML_STATE_MERGED : FDTD_PML_STATE_SPLIT) && hd.step >= 0) {
+            std::vector<float> raw(h->D.state_floats);
+            if (!is.read(reinterpret_cast<char *>(raw.data()), (std::streamsize)(raw.size() * 4))) {
+                set_last_error("engine state file: short file");
+                return FDTD_ERR_INVALID;
+            }
+            FDTD_HIP_TRY(hipMemcpy(h->D.dev.field, raw.data(), raw.size() * 4, hipMemcpyHostToDevice));
+            return resume_at(h, hd.step);
+        }
+        if (std::memcmp(hd.magic, kStateMagic, 8) == 0 && hd.pml_state == FDTD_PML_STATE_MERGED && !merged) {
+            set_last_error("snapshot written by a handle with the merged PML state, whose reference files lack the "
+                           "transverse face velocities of merged cells; load it into a merged handle of the same scene");
+            return FDTD_ERR_UNSUPPORTED;
+        }
+    }
+    if (S.visc) {
+        set_last_error("the reference's snapshot files do not hold the previous step's Laplacian that air viscosity "
+                       "needs; resume a viscous scene from the engine state file of a handle like this one");
+        return FDTD_ERR_UNSUPPORTED;
+    }
     long long s = -1;
     int ind = -1;
     {
@@ -544,43 +645,65 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
     FDTD_HIP_TRY(h->D.upload_grid(h->D.slot(slot_cur(s)), f.data()));
     for (long long c = 0; c < N; ++c) f[c] = (float)prev[c];
     FDTD_HIP_TRY(h->D.upload_grid(h->D.slot(slot_next(s)), f.data()));
-    std::vector<double> full;
-    std::vector<float> comp;
-    for (int d = 0; d < 3; ++d) {
-        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_pml_" + std::to_string(d) + ".dat"), N, 1, full)) != FDTD_OK)
+    std::vector<double> pml[3], vel[3];
+    for (int d = 0; d < 3; ++d)
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "p_pml_" + std::to_string(d) + ".dat"), N, 1, pml[d])) != FDTD_OK)
             return st;
-        comp.assign(std::max<int64_t>(1, S.n_pml), 0.f);
-        for_pml_cells(S, [&](long long c, long long g) { comp[c] = (float)full[g]; });
-        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(h->D.dev.pd[d], comp.data(), S.n_pml * 4, hipMemcpyHostToDevice));
-    }
-    const int vb = (int)(s & 1);
     for (int d = 0; d < 3; ++d) {
         const long long NV = (long long)(S.n[0] + (d == 0)) * (S.n[1] + (d == 1)) * (S.n[2] + (d == 2));
-        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "v_pml_" + std::to_string(d) + ".dat"), NV, 1, full)) != FDTD_OK)
+        if ((st = read_matrix(snap_name(pattern, simulator_id, numbering, "v_pml_" + std::to_string(d) + ".dat"), NV, 1, vel[d])) != FDTD_OK)
             return st;
+    }
+    // split pressures; a merged cell takes pDir_A and the increment of the
+    // transverse sum q, d = sum_b f5/h_b (v_b(high face) - v_b(low face)) at
+    // the newest velocities (exact: with f1 = f3 = 1 on both transverse axes
+    // q changes by that sum every step)
+    std::vector<float> comp[3];
+    for (int d = 0; d < 3; ++d) comp[d].assign(std::max<int64_t>(1, S.n_pml), 0.f);
+    for (const PmlSlab &sl : S.slabs)
+        for (int li = 0; li < sl.e[0]; ++li)
+            for (int lj = 0; lj < sl.e[1]; ++lj)
+                for (int lk = 0; lk < sl.e[2]; ++lk) {
+                    const long long c = sl.cell_base + ((long long)li * sl.e[1] + lj) * sl.e[2] + lk;
+                    const int i = sl.lo[0] + li, j = sl.lo[1] + lj, k = sl.lo[2] + lk;
+                    const long long g = ((long long)i * S.n[1] + j) * S.n[2] + k;
+                    if (!merged_cell(S, sl, j, k)) {
+                        for (int d = 0; d < 3; ++d) comp[d][c] = (float)pml[d][g];
+                        continue;
+                    }
+                    const int A = sl.axis, lower = A == 0 ? 1 : 0;
+                    comp[A][c] = (float)pml[A][g];
+                    double dd = 0.0;
+                    const int ijk[3] = {i, j, k};
+                    for (int b = 0; b < 3; ++b) {
+                        if (b == A) continue;
+                        int hi[3] = {ijk[0], ijk[1], ijk[2]};
+                        hi[b] += 1;
+                        dd += (double)S.f5h[b][ijk[b]] *
+                              (vel[b][face_index(S.n, b, hi[0], hi[1], hi[2])] - vel[b][face_index(S.n, b, i, j, k)]);
+                    }
+                    comp[lower][c] = (float)dd;
+                }
+    for (int d = 0; d < 3; ++d)
+        if (S.n_pml) FDTD_HIP_TRY(hipMemcpy(h->D.dev.pd[d], comp[d].data(), S.n_pml * 4, hipMemcpyHostToDevice));
+    const int vb = (int)(s & 1);
+    std::vector<float> vc;
+    for (int d = 0; d < 3; ++d) {
         if (mac) {
-            comp.assign(N, 0.f);
+            vc.assign(N, 0.f);
             for (long long i = 0; i < S.n[0]; ++i)
                 for (long long j = 0; j < S.n[1]; ++j)
                     for (long long k = 0; k < S.n[2]; ++k)
-                        comp[(i * S.n[1] + j) * S.n[2] + k] = (float)full[face_index(S.n, d, i, j, k)];
-            FDTD_HIP_TRY(h->D.upload_grid(h->D.dev.vel[vb][d], comp.data()));
+                        vc[(i * S.n[1] + j) * S.n[2] + k] = (float)vel[d][face_index(S.n, d, i, j, k)];
+            FDTD_HIP_TRY(h->D.upload_grid(h->D.dev.vel[vb][d], vc.data()));
         } else {
-            comp.assign(std::max<int64_t>(1, S.n_pml_faces[d]), 0.f);
-            for_pml_faces(S, d, [&](long long c, long long g) { comp[c] = (float)full[g]; });
+            vc.assign(std::max<int64_t>(1, S.n_pml_faces[d]), 0.f);
+            for_pml_faces(S, d, [&](long long c, long long g) { vc[c] = (float)vel[d][g]; });
             if (S.n_pml_faces[d])
-                FDTD_HIP_TRY(hipMemcpy(h->D.dev.vel[vb][d], comp.data(), S.n_pml_faces[d] * 4, hipMemcpyHostToDevice));
+                FDTD_HIP_TRY(hipMemcpy(h->D.dev.vel[vb][d], vc.data(), S.n_pml_faces[d] * 4, hipMemcpyHostToDevice));
         }
     }
-    // clock and amplitudes of step s; the series ring starts empty at s
-    const unsigned long long clk[CLK_WORDS] = {(unsigned long long)s, (unsigned long long)LLONG_MAX, 0ULL, 0ULL};
-    FDTD_HIP_TRY(hipMemcpy(h->D.dev.clock, clk, sizeof clk, hipMemcpyHostToDevice));
-    FDTD_HIP_TRY(h->D.fill_amplitudes(h->stream));
-    FDTD_HIP_TRY(hipStreamSynchronize(h->stream));
-    h->step = s;
-    h->flushed = s;
-    h->host_series.assign((size_t)s * S.L, 0.0);
-    return FDTD_OK;
+    return resume_at(h, s);
 }
 
 }  // extern "C"

@@ -400,12 +400,23 @@ fdtd_status fdtd_run_xml(const char *xml_path, const char *simulator_id, const f
  * snapshot_<n>_p_collocated_{0,1,2}.dat, _p_collocated_ind.dat (text),
  * _p_pml_{0,1,2}.dat (split pressures, full grid), _v_pml_{0,1,2}.dat (face
  * velocities, (n_d+1) along axis d), _p_pml_full.dat, plus _step.txt with the
- * step index.  Names follow fdtd_composite_filename. */
+ * step index and _engine_state.bin, this engine's whole state region (bit-
+ * exact resume of the same scene in a handle of the same PML state).  Names
+ * follow fdtd_composite_filename.  A handle with the merged PML state keeps,
+ * per merged cell, pDir_A and the increment of the sum q of its transverse
+ * splits: its p_pml files hold pDir_A, q on the lower transverse axis and
+ * zero on the other, and its v_pml files zero on faces between two merged
+ * cells (velocities that state does not keep). */
 fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering);
-/* Restores a snapshot written by fdtd_save_snapshot into a handle created
- * for the same scene (the reference declares LoadSolverResult,
- * FDTD_AcousticSimulator.h:108, without a definition).  Stepping continues
- * from the saved step bit for bit; listener rows before it read as zero. */
+/* Restores a snapshot into a handle created for the same scene (the
+ * reference declares LoadSolverResult, FDTD_AcousticSimulator.h:108, without
+ * a definition): from the engine state file when it matches the handle (bit
+ * for bit), else from the reference's files -- exact for a split handle; a
+ * merged handle converts the split state (pDir_A, and the transverse
+ * increment sum_b f5/h_b (v_b(high) - v_b(low)) of the stored velocities).  A
+ * split handle refuses files written by a merged one; air viscosity needs the
+ * engine state file (the reference's files lack the previous Laplacian).
+ * Listener rows before the loaded step read as zero. */
 fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *simulator_id, int numbering);
 
 /* Host-only dry run of the device planner (no counterpart in the reference,

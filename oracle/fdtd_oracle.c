@@ -1664,6 +1664,11 @@ void oracle_set_state(oracle *o, const double *p_last, const double *p_curr)
     memcpy(o->pc[o->pc_ind], p_curr, sizeof(double) * o->N);
 }
 
+void oracle_get_state(const oracle *o, double *p_last, double *p_curr)
+{
+    if (p_last) memcpy(p_last, o->pc[(o->pc_ind + 2) % 3], sizeof(double) * o->N);
+    if (p_curr) memcpy(p_curr, o->pc[o->pc_ind], sizeof(double) * o->N);
+}
 void oracle_get_pml_state(const oracle *o, double *pdir, double *v)
 {
     if (pdir) for (int d = 0; d < 3; ++d) memcpy(pdir + d * o->N, o->pdir[d], sizeof(double) * o->N);

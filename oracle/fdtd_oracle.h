@@ -50,6 +50,10 @@ int     oracle_step(oracle *o, int nsteps, double *rows);
 void    oracle_get_pressure(const oracle *o, double *out);          /* newest p, N */
 /* Overwrites p^{n-1} and p^n (N values each) before the next step. */
 void    oracle_set_state(oracle *o, const double *p_last, const double *p_curr);
+/* p^{n-1} and p^n (N values each; either may be NULL). */
+void    oracle_get_state(const oracle *o, double *p_last, double *p_curr);
+/* Split pressures pDir_d (3 N) and face velocities (axis d: (n_d + 1) faces
+ * along d, index ((i ny' + j) nz' + k), one axis after another). */
 void    oracle_get_pml_state(const oracle *o, double *pdir /*3N*/, double *v /*3 faces*/);
 /* counts: [0] N, [1] N_pml cells, [2] N_pml faces, [3] ghost entries,
  *         [4] solid cells, [5] bulk cells, [6] cavity cells,

@@ -40,6 +40,7 @@ def lib():
         L.oracle_get_pressure.argtypes = [C.c_void_p, D]
         L.oracle_get_pml_state.argtypes = [C.c_void_p, D, D]
         L.oracle_set_state.argtypes = [C.c_void_p, D, D]
+        L.oracle_get_state.argtypes = [C.c_void_p, D, D]
         L.oracle_counts.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
         L.oracle_time.restype = C.c_double
         L.oracle_time.argtypes = [C.c_void_p]
@@ -108,6 +109,20 @@ class Oracle:
         b = np.ascontiguousarray(p_curr, dtype=np.float64).reshape(-1)
         assert a.size == self.sb.n_cells and b.size == self.sb.n_cells
         self.L.oracle_set_state(self.h, a.ctypes.data_as(C.POINTER(C.c_double)), b.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def state(self):
+        """(p^{n-1}, p^n, pDir[3], v[3]): the time-stepping state, full grid;
+        v[d] has n_d + 1 faces along axis d."""
+        N = self.sb.n_cells
+        P = C.POINTER(C.c_double)
+        last, curr = np.zeros(N), np.zeros(N)
+        self.L.oracle_get_state(self.h, last.ctypes.data_as(P), curr.ctypes.data_as(P))
+        n = self.sb.divisions
+        nv = [(n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2)) for d in range(3)]
+        pdir, v = np.zeros(3 * N), np.zeros(sum(nv))
+        self.L.oracle_get_pml_state(self.h, pdir.ctypes.data_as(P), v.ctypes.data_as(P))
+        vs = np.split(v, [nv[0], nv[0] + nv[1]])
+        return last, curr, pdir.reshape(3, N), vs
 
     def counts(self):
         c = (C.c_longlong * 9)()

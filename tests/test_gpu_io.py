@@ -14,7 +14,9 @@ import numpy as np
 import pytest
 
 from acoustics_amd import capi, scenes
-from acoustics_amd.solver import Solver
+from acoustics_amd.capi import SceneBuffers
+from acoustics_amd.solver import Solver, rel_l2
+from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -75,6 +77,8 @@ def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
     assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_p_pml_0.dat")).shape == (N, 1)
     nx, ny, nz = sb.divisions
     assert read_matrix_xd_binary(str(tmp_path / "snap_0_snapshot_2_v_pml_1.dat")).shape == (nx * (ny + 1) * nz, 1)
+    # the reference's files alone (without the engine state file) resume a split handle exactly
+    os.remove(str(tmp_path / "snap_0_snapshot_2_engine_state.bin"))
     with Solver(sb, pml_state=split) as b:
         b.load_snapshot(pat, "0", 2)
         b.step(50)
@@ -85,15 +89,100 @@ def test_snapshot_resumes_bit_for_bit(hip_lib, tmp_path, kind):
     assert np.all(ser_b[:70] == 0)
 
 
-def test_snapshot_needs_split_state(hip_lib, tmp_path):
-    """The merged PML state cannot produce the per-axis pDir_d and velocity
-    files, so a merged handle refuses to save or load a snapshot."""
-    sb, _ = scenes.config1(seed=5, n=32)
-    print(f"case: snapshot refused divisions={sb.divisions}", flush=True)
+def _write_matrix(path, v):
+    """IO::writeMatrixX / MATRIX::write layout: int32 rows, int32 cols, doubles."""
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 1)
+    with open(path, "wb") as f:
+        f.write(np.array([v.shape[0], 1], dtype=np.int32).tobytes())
+        f.write(v.tobytes())
+
+
+def _snap_scene():
+    return scenes.config2(seed=24, n=40, radius=0.025, shell=0.06, pml=8, n_listeners=8)[0]
+
+
+def test_snapshot_files_hold_the_oracles_state(hip_lib, tmp_path):
+    """The saved p_collocated (p^n, p^{n-1}), p_pml_d and v_pml_d equal the
+    FP64 oracle's state after the same steps, to FP32 accuracy."""
+    sb = _snap_scene()
+    steps = 90
+    print(f"case: snapshot vs oracle divisions={sb.divisions} steps={steps}", flush=True)
+    pat = str(tmp_path / "s_%s")
+    with Solver(sb, pml_state=capi.FDTD_PML_STATE_SPLIT) as a:
+        a.step(steps)
+        a.save_snapshot(pat, "0", 0)
+    o = Oracle(sb, reclassify_each_step=False)
+    o.step(steps, record=False)
+    last, curr, pdir, vel = o.state()
+    rd = lambda what: read_matrix_xd_binary(str(tmp_path / f"s_0_snapshot_0_{what}.dat"))[:, 0]
+    scale = np.abs(curr).max()
+    assert scale > 0
+    assert np.abs(rd("p_collocated_1") - curr).max() <= 1e-5 * scale
+    assert np.abs(rd("p_collocated_0") - last).max() <= 1e-5 * scale
+    for d in range(3):
+        ps = np.abs(pdir[d]).max()
+        assert np.abs(rd(f"p_pml_{d}") - pdir[d]).max() <= 1e-5 * max(ps, 1e-30), d
+        vs = np.abs(vel[d]).max()
+        assert vs > 0
+        assert np.abs(rd(f"v_pml_{d}") - vel[d]).max() <= 1e-5 * vs, d
+
+
+@pytest.mark.parametrize("pml_state", [capi.FDTD_PML_STATE_SPLIT, capi.FDTD_PML_STATE_MERGED])
+def test_resume_from_oracle_state(hip_lib, tmp_path, pml_state):
+    """A state the FP64 oracle wrote in the reference's snapshot layout, loaded
+    and stepped on, follows the oracle's own continuation (relative L2 <=
+    1e-4 over the listener rows after the load).  A merged handle converts
+    the split state: pDir_A, and the increment of the transverse sum from the
+    face velocities."""
+    sb = _snap_scene()
+    k, m = 80, 120
+    print(f"case: resume from oracle state pml_state={pml_state} divisions={sb.divisions}", flush=True)
+    o = Oracle(sb, reclassify_each_step=False)
+    o.step(k, record=False)
+    last, curr, pdir, vel = o.state()
+    pre = str(tmp_path / "o_0_snapshot_1_")
+    _write_matrix(pre + "p_collocated_1.dat", curr)
+    _write_matrix(pre + "p_collocated_0.dat", last)
+    _write_matrix(pre + "p_collocated_2.dat", np.zeros_like(curr))
+    _write_matrix(pre + "p_pml_full.dat", np.zeros_like(curr))
+    for d in range(3):
+        _write_matrix(pre + f"p_pml_{d}.dat", pdir[d])
+        _write_matrix(pre + f"v_pml_{d}.dat", vel[d])
+    open(pre + "p_collocated_ind.dat", "w").write("1")
+    open(pre + "step.txt", "w").write(f"{k}\n")
+    ref = o.step(m)
+    with Solver(sb, pml_state=pml_state) as b:
+        b.load_snapshot(str(tmp_path / "o_%s"), "0", 1)
+        b.step(m)
+        got = b.series()[k:]
+    assert rel_l2(got, ref) <= 1e-4, rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("kind", ["merged_sphere", "viscous_sphere"])
+def test_engine_state_resumes_bit_for_bit(hip_lib, tmp_path, kind):
+    """The default (merged) PML state and air viscosity, whose state the
+    reference's files cannot hold in full, resume bit for bit from the engine
+    state file every snapshot also writes."""
+    sb = _snap_scene()
+    if kind == "viscous_sphere":
+        sb.settings_dict["air_viscosity_alpha"] = 1e-5
+        sb = SceneBuffers(sb.settings_dict, sb.objects_list, sb.impulse_list, [], sb.listeners_np)
+    print(f"case: engine state {kind} divisions={sb.divisions}", flush=True)
+    pat = str(tmp_path / "e_%s")
     with Solver(sb) as a:
-        assert a.info()["n_pml_merged_cells"] > 0
-        a.step(3)
-        with pytest.raises(RuntimeError, match="FDTD_PML_STATE_SPLIT"):
-            a.save_snapshot(str(tmp_path / "s_%s"), "0", 0)
-        with pytest.raises(RuntimeError, match="FDTD_PML_STATE_SPLIT"):
-            a.load_snapshot(str(tmp_path / "s_%s"), "0", 0)
+        a.step(70)
+        a.save_snapshot(pat, "0", 3)
+        a.step(60)
+        ser_a, p_a = a.series(), a.pressure()
+    with Solver(sb) as b:
+        b.load_snapshot(pat, "0", 3)
+        b.step(60)
+        ser_b, p_b = b.series(), b.pressure()
+    assert np.array_equal(ser_b[70:], ser_a[70:])
+    assert np.array_equal(p_b, p_a)
+    if kind == "merged_sphere":
+        # a split handle refuses the merged handle's reference files, whose
+        # transverse face velocities of merged cells the merged state does not keep
+        with Solver(sb, pml_state=capi.FDTD_PML_STATE_SPLIT) as c:
+            with pytest.raises(RuntimeError, match="merged"):
+                c.load_snapshot(pat, "0", 3)
