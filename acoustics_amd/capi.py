@@ -71,6 +71,10 @@ class SpeakerSource(C.Structure):
                 ("samples", C.POINTER(C.c_double))]
 
 
+class PlaneConstraint(C.Structure):
+    _fields_ = [("direction", C.c_int), ("sign", C.c_int), ("height", C.c_double)]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("settings", Settings),
@@ -79,6 +83,7 @@ class Scene(C.Structure):
         ("n_pressure_sources", C.c_int), ("pressure_sources", C.POINTER(PressureSource)),
         ("n_listeners", C.c_int), ("listeners", C.POINTER(C.c_double)),
         ("n_speaker_sources", C.c_int), ("speaker_sources", C.POINTER(SpeakerSource)),
+        ("n_constraints", C.c_int), ("constraints", C.POINTER(PlaneConstraint)),
     ]
 
 
@@ -208,7 +213,7 @@ class SceneBuffers:
     """Owns the ctypes arrays a Scene points at (keeps them alive)."""
 
     def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None,
-                 speaker_sources=()):
+                 speaker_sources=(), constraints=()):
         self.settings_dict = dict(settings)
         s = Settings()
         for k, v in settings.items():
@@ -273,6 +278,15 @@ class SceneBuffers:
             self._spk[i].samples = self._spk_samples[i].ctypes.data_as(C.POINTER(C.c_double))
         self.scene.n_speaker_sources = len(self.speaker_list)
         self.scene.speaker_sources = C.cast(self._spk, C.POINTER(SpeakerSource))
+        # plane constraints: dict(direction, sign, height)
+        self.constraint_list = list(constraints)
+        self._cons = (PlaneConstraint * max(1, len(self.constraint_list)))()
+        for i, c in enumerate(self.constraint_list):
+            self._cons[i].direction = int(c.get("direction", 1))
+            self._cons[i].sign = int(c.get("sign", 1))
+            self._cons[i].height = float(c.get("height", 0.0))
+        self.scene.n_constraints = len(self.constraint_list)
+        self.scene.constraints = C.cast(self._cons, C.POINTER(PlaneConstraint))
 
     @property
     def n_listeners(self):

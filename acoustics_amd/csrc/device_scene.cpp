@@ -109,7 +109,7 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
         L.coef[d] = c.take<float4>(S.n[d] + 1);
         L.flags[d] = c.take<int>(S.n[d] + 1);
     }
-    L.skip = c.take<uint8_t>(S.objs.empty() ? 0 : (size_t)slot_cells(S));
+    L.skip = c.take<uint8_t>(!S.has_solid ? 0 : (size_t)slot_cells(S));
     L.gtab = c.take<float>(S.gtab.size());
     L.psrc = c.take<PSrcDev>(S.psrc.size());
     L.isrc = c.take<ImpDev>(S.isrc.size());
@@ -205,6 +205,9 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     FDTD_HIP_TRY(hipMemsetAsync(L.field, 0, sizeof(float) * state_floats, st));
     // terms no tile writes (samples of cells that never change) read as zero
     FDTD_HIP_TRY(hipMemsetAsync(L.terms, 0, sizeof(double) * (size_t)cap * std::max<size_t>(1, S.lst_terms.size()), st));
+    // the acceleration ring reads as zero where no object refills it (a
+    // constraint's ghost entries weight it by zero)
+    FDTD_HIP_TRY(hipMemsetAsync(L.acc_tab, 0, sizeof(double) * std::max<size_t>(1, (size_t)AMP_ROWS * S.objs.size()), st));
     std::vector<char> staging;  // pageable copies are synchronised before return
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
@@ -219,7 +222,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
         for (int i = 0; i <= n; ++i) {
             coef[d][i] = make_float4(S.f1[d][i], S.f2[d][i], i < n ? S.f3[d][i] : 0.f, i < n ? S.f5h[d][i] : 0.f);
             flags[d][i] = (i < n && S.cpml[d][i] ? 1 : 0) | (S.fpml[d][i] ? 2 : 0) | (i < n && S.sig0[d][i] ? 4 : 0) |
-                          (axis_plain(S, d, i) ? 8 : 0);
+                          (axis_plain(S, d, i) ? 8 : 0) | (i < n && S.csolid[d][i] ? 16 : 0);
         }
         FDTD_HIP_TRY(up(L.coef[d], coef[d].data(), coef[d].size() * sizeof(float4)));
         FDTD_HIP_TRY(up(L.flags[d], flags[d].data(), flags[d].size() * sizeof(int)));
@@ -227,7 +230,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     // the host's cell indices are dense (k fastest, row length n2); device
     // arrays index padded rows
     std::vector<uint8_t> skip;
-    if (!S.objs.empty()) {
+    if (S.has_solid) {
         skip.assign((size_t)npad, 0);
         for (int64_t i = 0; i < S.N; ++i) skip[to_dev(i)] = S.bulk[i] ? 0 : 1;
         for (const SpecialRec &r : S.special) skip[to_dev(r.cell)] = 1;
@@ -304,7 +307,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
         D.vel[1][d] = L.vel[1][d];
     }
     for (int d = 0; d < 3; ++d) { D.coef[d] = L.coef[d]; D.flags[d] = L.flags[d]; }
-    D.skip = S.objs.empty() ? nullptr : L.skip;
+    D.skip = !S.has_solid ? nullptr : L.skip;
     D.lam2 = S.lam2;
     D.lam2f = (float)S.lam2;
     D.lam = S.c * S.dt / S.h;
@@ -414,7 +417,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
 {
     const size_t base = out.size();
     const int *n = S.n;
-    const int *a = S.inner_lo, *b = S.inner_hi;
+    const int *a = S.tile_lo, *b = S.tile_hi;
     auto src_hit = [&](const int *lo, const int *hi) {
         for (const PSrcHost &p : S.psrc)
             if (box_overlap(lo, hi, p.lo, p.hi)) return 1;

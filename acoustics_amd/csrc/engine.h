@@ -38,6 +38,7 @@ struct GhostEntry {
     // fully-coupled row: x_g = sum_k w_k p[k] + c_acc * a_n(obj, t) + sum_u w_u x_u
     int coupled;      // 1: coupled row, 0: rasterised value
     double c_acc;
+    int type;         // 0: object ghost, 1: plane-constraint ghost (p_g = p[nbr])
 };
 
 struct SpecialRec {      // bulk cell with at least one ghost neighbour
@@ -122,6 +123,14 @@ struct SceneSetup {
     std::vector<float> f3[3], f5h[3];  // cell tables (n_d), f5h = f5 / h
     std::vector<uint8_t> sig0[3];      // cell: sigma_d <= 1e-12 (staggered source goes into pDir_d)
     int inner_lo[3], inner_hi[3];      // non-PML index box (with the Engquist boundary: [2, n - 2))
+    // plane constraints; per axis and index, 1 where a constraint holds the
+    // cells' centres (solid whatever the other coordinates); the interior
+    // tiles cover [tile_lo, tile_hi), the non-PML box without the solid
+    // layers and the fluid layer next to them
+    std::vector<fdtd_plane_constraint> cons;
+    std::vector<uint8_t> csolid[3];
+    int tile_lo[3], tile_hi[3];
+    bool has_solid = false;            // objects or constraints: solid cells, ghosts, a skip mask
     // Engquist boundary (no PML cell anywhere: use_mesh = 0, or a PML width
     // of zero): the layer-1 cells take the absorbing update and layer 0 is
     // never updated (MAC_Grid.cpp:521-919)

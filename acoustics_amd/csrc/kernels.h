@@ -73,7 +73,7 @@ struct SceneDev {
     float *vel[2][3];           // collocated: compact PML faces; staggered: every face, full grid
                                 // (face d of cell c, on its low side, at c's index)
     const float4 *coef[3];      // per axis, index 0..n: {f1(face), f2(face), f3(cell), f5/h(cell)}
-    const int *flags[3];        // per axis, index 0..n: bit0 cell in PML, bit1 face in PML
+    const int *flags[3];        // per axis, index 0..n: 1 cell in PML, 2 face in PML, 4 sigma zero, 8 plain, 16 behind a plane constraint
     const uint8_t *skip;        // per cell: 1 = object cell (solid or ghost-adjacent)
     double lam2, t_start, dt;
     float lam2f;

@@ -472,6 +472,10 @@ __device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc 
     const float4 cyR = cy[min(j + 1, ny)], czR = cz[min(k + 1, nz)];
     const bool cj = fyL & 1, ck = fzL & 1;
     const bool ox = cj || ck;
+    // behind a plane constraint (flag 16 on any axis) a cell is solid: its
+    // pressure stays zero, as the reference's per-step classification resets
+    // it, while its faces and split state update as usual
+    const bool solid_col = (fyL | fzL) & 16;
     // a face is updated when it lies in the PML or its cell is in the PML on
     // another axis (ci, the x flag of the plane, joins per plane); wall faces stay zero
     const bool yl_on = j >= 1 && ((fyL & 2) || ck), yl_ci = j >= 1;
@@ -578,6 +582,7 @@ __device__ __forceinline__ void pml_tile_axis(const SceneDev &S, const TileDesc 
         } else {
             pn = n0 + n1 + n2;
         }
+        if (solid_col || (fxC & 16)) pn = 0.f;
         stb(pnext, po, pn);
         if (t.te > t.tb) emit_cell_terms(S, t, terms, step, (int)(po >> 2), pn);
         pm = pc;

@@ -240,6 +240,10 @@ int64_t fdtd_scene_device_bytes(const fdtd_scene *scene)
     return (int64_t)DeviceScene::bytes_needed(S, 1);
 }
 
+// cell index of a special record or of a bare cell index (plan check lookups)
+static inline int32_t cell_of(const SpecialRec &r) { return r.cell; }
+static inline int32_t cell_of(int32_t c) { return c; }
+
 static fdtd_status fail_early(int plan, const std::string &why)
 {
     set_last_error("plan " + std::to_string(plan) + ": " + why);
@@ -303,21 +307,33 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
                 }
     }
     std::vector<uint8_t> object_cell((size_t)D.S.N, 0);
-    if (!D.S.objs.empty())
+    if (D.S.has_solid)
         for (int64_t c = 0; c < D.S.N; ++c) object_cell[c] = D.S.bulk[c] ? 0 : 1;
     for (const SpecialRec &r : D.S.special) object_cell[r.cell] = 1;
     auto fail = [&](const std::string &why) {
         set_last_error("plan " + std::to_string(plan) + ": " + why);
         return FDTD_ERR_INVALID;
     };
-    // (with the Engquist boundary the outermost layer is in no tile)
+    // (with the Engquist boundary the outermost layer is in no tile; with
+    // plane constraints, the interior layers cut off the tile box -- solid
+    // cells and the fluid cells next to them -- are in none either, and the
+    // PML tiles hold their solid cells at zero)
     const int *n = D.S.n;
+    const SceneSetup &S = D.S;
     for (int64_t c = 0; c < D.S.N; ++c) {
         const int i = (int)(c / ((int64_t)n[1] * n[2])), j = (int)((c / n[2]) % n[1]), k = (int)(c % n[2]);
         const bool outer = i == 0 || j == 0 || k == 0 || i == n[0] - 1 || j == n[1] - 1 || k == n[2] - 1;
-        const int want = (D.S.abc && outer) ? 0 : 1;
+        const bool pml = !S.abc && (S.cpml[0][i] || S.cpml[1][j] || S.cpml[2][k]);
+        const bool cut = !pml && !S.abc && (i < S.tile_lo[0] || i >= S.tile_hi[0] || j < S.tile_lo[1] || j >= S.tile_hi[1] ||
+                                             k < S.tile_lo[2] || k >= S.tile_hi[2]);
+        if (cut && S.bulk.size() && S.bulk[c] && !std::binary_search(S.special.begin(), S.special.end(), (int32_t)c,
+                [](const auto &a, const auto &b) {
+                    return cell_of(a) < cell_of(b);
+                }))
+            return fail("a fluid cell off the interior tiles is no object-adjacent record");
+        const int want = ((D.S.abc && outer) || cut) ? 0 : 1;
         if (hit[c] != want) return fail("a cell is covered " + std::to_string(hit[c]) + " times");
-        if (object_cell[c] && !skips[c]) return fail("an object cell lies in a tile that would overwrite it");
+        if (object_cell[c] && hit[c] && !pml && !skips[c]) return fail("an object cell lies in a tile that would overwrite it");
     }
     for (int h : rec_hits)
         if (h != 1) return fail("an object-adjacent record is covered " + std::to_string(h) + " times");

@@ -361,6 +361,17 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         S.src_steps = k;
     }
     if (st.scheme != FDTD_SCHEME_COLLOCATED && st.scheme != FDTD_SCHEME_STAGGERED) { err = "unknown scheme"; return FDTD_ERR_INVALID; }
+    if (sc.n_constraints < 0 || (sc.n_constraints > 0 && !sc.constraints)) { err = "bad constraint list"; return FDTD_ERR_INVALID; }
+    S.cons.assign(sc.constraints, sc.constraints + sc.n_constraints);
+    for (const fdtd_plane_constraint &c : S.cons)
+        if (c.direction < 0 || c.direction > 2 || (c.sign != 1 && c.sign != -1) || !std::isfinite(c.height)) {
+            err = "a plane constraint needs direction 0, 1 or 2, sign +1 or -1 and a finite height";
+            return FDTD_ERR_INVALID;
+        }
+    if (!S.cons.empty() && (st.scheme != FDTD_SCHEME_COLLOCATED || !st.use_mesh)) {
+        err = "plane constraints are classified with use_mesh = 1 in the collocated scheme";
+        return FDTD_ERR_UNSUPPORTED;
+    }
     if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0)) {
         err = "the staggered scheme runs free-field scenes only (objects need the reference's mesh distance-field "
               "classification of the MAC mode); use the collocated scheme";
@@ -464,6 +475,45 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     }
     if (S.abc)
         for (int d = 0; d < 3; ++d) { S.inner_lo[d] = 2; S.inner_hi[d] = S.n[d] - 2; }
+    // plane constraints: a cell is solid when a constraint holds its centre
+    // (FDTD_PlaneConstraint::Inside), which depends on one coordinate only
+    for (int d = 0; d < 3; ++d) {
+        S.csolid[d].assign(S.n[d], 0);
+        S.tile_lo[d] = S.inner_lo[d];
+        S.tile_hi[d] = S.inner_hi[d];
+    }
+    if (!S.cons.empty() && S.abc) {
+        err = "plane constraints need a PML (the Engquist boundary's layer-1 cells would lie behind the plane)";
+        return FDTD_ERR_UNSUPPORTED;
+    }
+    for (const fdtd_plane_constraint &c : S.cons) {
+        const int d = c.direction;
+        for (int i = 0; i < S.n[d]; ++i) {
+            double x = bmin[d];
+            x += (0.5 + (double)i) * h;
+            if ((x - c.height) * (double)c.sign < 0.0) S.csolid[d][i] = 1;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        // Along an axis the constraints leave the fluid layers [L, U) (each
+        // solid set is a prefix or a suffix).  The interior tiles start past
+        // the solid layers and the fluid layer next to them, whose cells take
+        // the ghost-cell update when they lie outside the PML.
+        int L = 0, U = S.n[d];
+        while (L < S.n[d] && S.csolid[d][L]) ++L;
+        while (U > L && S.csolid[d][U - 1]) --U;
+        for (int i = L; i < U; ++i)
+            if (S.csolid[d][i]) { err = "internal: constraint layers are not a prefix and a suffix"; return FDTD_ERR_INVALID; }
+        int lo = S.inner_lo[d], hi = S.inner_hi[d];
+        if (L > 0 && L >= lo) lo = L + 1;
+        if (U < S.n[d] && U <= hi) hi = U - 1;
+        if (hi - lo < 2) {
+            err = "plane constraints leave fewer than two interior cell layers along an axis";
+            return FDTD_ERR_UNSUPPORTED;
+        }
+        S.tile_lo[d] = lo;
+        S.tile_hi[d] = hi;
+    }
     S.n_pml = S.abc ? 0 : S.N - (int64_t)(S.inner_hi[0] - S.inner_lo[0]) * (S.inner_hi[1] - S.inner_lo[1]) * (S.inner_hi[2] - S.inner_lo[2]);
     S.slabs.clear();
     for (int d = 0; d < 3; ++d) S.n_pml_faces[d] = 0;
@@ -635,11 +685,17 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     S.mv_cls0.push_back(fluid ? 1 : 0);
                 }
     }
-    const bool static_objs = !S.objs.empty() && !S.moving;
-    S.bulk.assign(S.objs.empty() ? 0 : N, 1);
+    if (!S.cons.empty() && S.moving) {
+        err = "plane constraints are classified statically; moving objects are not combined with them";
+        return FDTD_ERR_UNSUPPORTED;
+    }
+    S.has_solid = !S.objs.empty() || !S.cons.empty();
+    const bool static_objs = S.has_solid && !S.moving;
+    S.bulk.assign(S.has_solid ? N : 0, 1);
     S.ghosts.clear(); S.special.clear(); S.boxes.clear(); S.cls.clear();
     S.n_solid = 0; S.n_cavity = 0;
     std::vector<int64_t> solid;
+    std::vector<uint8_t> solid_type;   // per solid cell: 1 when a constraint holds its centre (gctype)
     std::vector<std::array<int, 6>> itbox(S.objs.size());
     if (static_objs) {
         std::vector<uint8_t> classified(N, 0);
@@ -648,9 +704,30 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 if (geom_inside(g, x)) return true;
             return false;
         };
-        for (size_t ob = 0; ob < S.objs.size(); ++ob) {
+        auto constrained = [&](const double *x) {
+            for (const fdtd_plane_constraint &c : S.cons)
+                if ((x[c.direction] - c.height) * (double)c.sign < 0.0) return true;
+            return false;
+        };
+        // iteration boxes (MAC_Grid.cpp:3511-3550): each object's box with
+        // the volume behind every constraint cut away, then each constraint's
+        // solid half-space
+        const size_t nbox = S.objs.size() + S.cons.size();
+        std::vector<std::array<int, 6>> boxes(nbox);
+        for (size_t ob = 0; ob < nbox; ++ob) {
             double mn[3], mx[3];
-            geom_aabb(S.objs[ob], mn, mx);
+            if (ob < S.objs.size()) {
+                geom_aabb(S.objs[ob], mn, mx);
+                for (const fdtd_plane_constraint &c : S.cons) {
+                    if (c.sign > 0) mn[c.direction] = std::max(mn[c.direction], c.height);
+                    else mx[c.direction] = std::min(mx[c.direction], c.height);
+                }
+            } else {
+                const fdtd_plane_constraint &c = S.cons[ob - S.objs.size()];
+                for (int d = 0; d < 3; ++d) { mn[d] = -DBL_MAX; mx[d] = DBL_MAX; }
+                if (c.sign > 0) mx[c.direction] = c.height;
+                else mn[c.direction] = c.height;
+            }
             // ScalarField::GetIterationBox (src/field/ScalarField.cpp:288-314)
             bool in_min = true, in_max = true;
             for (int d = 0; d < 3; ++d) {
@@ -666,19 +743,23 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 rg3[d] = std::max(end - st3[d], 1);
             }
             for (int d = 0; d < 3; ++d) {
-                itbox[ob][d] = std::max(st3[d], 0);
-                itbox[ob][3 + d] = std::min(st3[d] + rg3[d], S.n[d]);
+                boxes[ob][d] = std::max(st3[d], 0);
+                boxes[ob][3 + d] = std::min(st3[d] + rg3[d], S.n[d]);
             }
+            if (ob < S.objs.size()) itbox[ob] = boxes[ob];
             const double q4 = 0.25 * h;
-            for (int i = itbox[ob][0]; i < itbox[ob][3]; ++i)
-                for (int j = itbox[ob][1]; j < itbox[ob][4]; ++j)
-                    for (int k = itbox[ob][2]; k < itbox[ob][5]; ++k) {
+            for (int i = boxes[ob][0]; i < boxes[ob][3]; ++i)
+                for (int j = boxes[ob][1]; j < boxes[ob][4]; ++j)
+                    for (int k = boxes[ob][2]; k < boxes[ob][5]; ++k) {
                         const int64_t ci = flat(i, j, k);
                         if (classified[ci]) continue;
                         classified[ci] = 1;
                         double x[3];
                         cpos(i, j, k, x);
-                        bool fluid = !occupied(x);
+                        // isFluid (MAC_Grid.cpp:3566-3577): the centre outside
+                        // every constraint, all seven samples outside every object
+                        const bool in_con = constrained(x);
+                        bool fluid = !in_con && !occupied(x);
                         for (int d = 0; d < 3 && fluid; ++d)
                             for (int sg = -1; sg <= 1 && fluid; sg += 2) {
                                 double y[3] = {x[0], x[1], x[2]};
@@ -690,10 +771,20 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         }
         std::sort(solid.begin(), solid.end());
         S.n_solid = (int64_t)solid.size();
-        // a solid cell in the PML or on the outer layers is outside what the
-        // device path restates; with the Engquist boundary the cells next to
-        // an object stay off layer 1 (layers 0-2 hold no solid cell)
-        for (int64_t ci : solid) {
+        solid_type.resize(solid.size());
+        for (size_t q = 0; q < solid.size(); ++q) {
+            const int64_t ci = solid[q];
+            const int ijk[3] = {(int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz)};
+            solid_type[q] = (S.csolid[0][ijk[0]] || S.csolid[1][ijk[1]] || S.csolid[2][ijk[2]]) ? 1 : 0;
+        }
+        // a solid cell of an object in the PML or on the outer layers is
+        // outside what the device path restates; with the Engquist boundary
+        // the cells next to an object stay off layer 1 (layers 0-2 hold no
+        // solid cell).  Cells behind a plane constraint may lie in the PML:
+        // their pressure is held at zero there.
+        for (size_t q = 0; q < solid.size(); ++q) {
+            if (solid_type[q]) continue;
+            const int64_t ci = solid[q];
             const int i = (int)(ci / ((int64_t)ny * nz)), j = (int)((ci / nz) % ny), k = (int)(ci % nz);
             if (S.cpml[0][i] || S.cpml[1][j] || S.cpml[2][k]) {
                 err = "an object reaches into the PML; objects must lie inside the non-PML interior";
@@ -709,7 +800,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         }
         // ghost entries (MakeGhostCell), topology order -x,+x,-y,+y,-z,+z
         std::vector<std::pair<uint64_t, GhostEntry>> tmp;
-        for (int64_t ci : solid) {
+        for (size_t q = 0; q < solid.size(); ++q) {
+            const int64_t ci = solid[q];
             const int ijk[3] = {(int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz)};
             for (int d = 0; d < 3; ++d)
                 for (int sg = -1; sg <= 1; sg += 2) {
@@ -720,6 +812,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     if (!S.bulk[nb]) continue;
                     GhostEntry g{};
                     g.owner = (int)ci; g.nbr = (int)nb; g.topology = sg * (d + 1);
+                    g.type = solid_type[q];
                     double x[3];
                     cpos(ijk[0], ijk[1], ijk[2], x);
                     // LowestObjectDistance from the owner centre
@@ -753,7 +846,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         while (first + cnt < G && S.ghosts[first + cnt].owner == owner) ++cnt;
         return cnt;
     };
-    std::vector<uint8_t> isghost(S.objs.empty() ? 0 : N, 0);
+    std::vector<uint8_t> isghost(S.has_solid ? N : 0, 0);
     for (const GhostEntry &g : S.ghosts) isghost[g.owner] = 1;
 
     // ---- ghost values (PML_pressureUpdateGhostCells) -----------------------
@@ -763,6 +856,15 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     const bool coupled = S.boundary == FDTD_BOUNDARY_FULLY_COUPLED;
     for (int g = 0; g < G; ++g) {
         GhostEntry &gc = S.ghosts[g];
+        if (gc.type == 1) {
+            // a plane constraint reflects perfectly: p_g = p[nbr] (MAC_Grid.cpp:1198-1204)
+            gc.k_raster = 0.0;
+            gc.coupled = 0;
+            gc.c_acc = 0.0;
+            S.gk_ptr.push_back((int32_t)S.gk_idx.size());
+            S.gu_ptr.push_back((int32_t)S.gu_idx.size());
+            continue;
+        }
         const Geom &ob = S.objs[gc.obj];
         double bp[3], nb[3];
         const double dist = geom_closest(ob, gc.pos, bp, nb);
@@ -901,6 +1003,13 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         for (int g = 0; g < G; ++g) {
             const GhostEntry &gc = S.ghosts[g];
             const int c_ = gc.nbr;
+            {
+                // a fluid neighbour in the PML takes the PML update, which
+                // reads the solid cell's zero pressure; its ghost serves
+                // listener interpolation only
+                const int i = (int)(c_ / ((int64_t)ny * nz)), j = (int)((c_ / nz) % ny), k = (int)(c_ % nz);
+                if (S.cpml[0][i] || S.cpml[1][j] || S.cpml[2][k]) continue;
+            }
             if (rec_of[c_] < 0) {
                 SpecialRec r{};
                 r.cell = c_; r.flags = 0;
@@ -932,8 +1041,11 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     }
             S.boxes.push_back(b);
         }
-        // every special cell and every solid cell must sit inside a box
-        for (int64_t ci : solid) {
+        // every object's solid cell must sit inside a box (cells behind a
+        // constraint lie outside the interior tiles or in the PML)
+        for (size_t q = 0; q < solid.size(); ++q) {
+            if (solid_type[q]) continue;
+            const int64_t ci = solid[q];
             const int ijk[3] = {(int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz)};
             bool in = false;
             for (const SpecialBox &b : S.boxes)
@@ -982,10 +1094,10 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         std::vector<Pt> pts;
         for (int q = 0; q < ne; ++q) {
             const int64_t ci = cells[q];
-            const bool gh = !S.objs.empty() && isghost[ci];
+            const bool gh = S.has_solid && isghost[ci];
             if (!gh) {
                 Pt P; cpos((int)(ci / ((int64_t)ny * nz)), (int)((ci / nz) % ny), (int)(ci % nz), P.x);
-                const bool isbulk = S.objs.empty() || S.bulk[ci];
+                const bool isbulk = !S.has_solid || S.bulk[ci];
                 P.kind = isbulk ? 0 : -1;  // solid non-ghost cells hold p = 0
                 P.index = (int)ci;
                 pts.push_back(P);
@@ -1006,7 +1118,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                 const double d2 = (x[0] - p[0]) * (x[0] - p[0]) + (x[1] - p[1]) * (x[1] - p[1]) + (x[2] - p[2]) * (x[2] - p[2]);
                 if (d2 < bd) { bd = d2; best = ci; }
             }
-            if (best >= 0 && (S.objs.empty() || S.bulk[best])) S.lst_terms.push_back({0, (int)best, 1.0});
+            if (best >= 0 && (!S.has_solid || S.bulk[best])) S.lst_terms.push_back({0, (int)best, 1.0});
         } else {
             const int m = (int)pts.size();
             // weighted design matrix, columns 1..3 scaled by 1/h for conditioning

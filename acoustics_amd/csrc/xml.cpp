@@ -156,6 +156,7 @@ struct fdtd_parsed {
     std::vector<fdtd_pressure_source> psrc;
     std::vector<fdtd_speaker_source> spk;
     std::vector<std::vector<double>> spk_samples;
+    std::vector<fdtd_plane_constraint> constraints;
     std::vector<double> listeners;
     int num_steps = 0;
     std::string output_pattern = "%s";
@@ -339,6 +340,15 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
         if (const Node *scn = root->child("scene")) {
             for (auto &k : scn->kids) {
                 const Node *o = k.get();
+                if (o->name == "constraint") {
+                    // ImpulseResponseParser.cpp:434-449: direction 1, sign 1, height 0 by default
+                    fdtd_plane_constraint c{};
+                    c.direction = opt_int(o, "direction", 1);
+                    c.sign = opt_int(o, "sign", 1);
+                    c.height = opt_real(o, "height", 0.0);
+                    P->constraints.push_back(c);
+                    continue;
+                }
                 fdtd_object ob{};
                 if (o->name == "analytic_sphere") {
                     ob.type = FDTD_OBJ_SPHERE;
@@ -394,6 +404,8 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
     for (size_t q = 0; q < P->spk.size(); ++q) P->spk[q].samples = P->spk_samples[q].data();
     sc.n_speaker_sources = (int)P->spk.size();
     sc.speaker_sources = P->spk.data();
+    sc.n_constraints = (int)P->constraints.size();
+    sc.constraints = P->constraints.data();
     *out = P.release();
     return FDTD_OK;
 }

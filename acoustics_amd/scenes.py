@@ -331,3 +331,54 @@ def config5_stream(seed=5, n_requests=64, rate=200.0, edge_lo=48, edge_hi=192, d
         dur = float(rng.uniform(dur_lo, dur_hi))
         out.append((t, sc, int(math.ceil(dur / dt))))
     return out
+
+
+def floor_scene(seed=16, n=48, h=0.005, pml=8, direction=1, sign=1, layers=6, frac=0.0, obj="sphere",
+                boundary=FDTD_BOUNDARY_RASTERIZE, steps=300, n_listeners=10, source=True):
+    """A plane constraint (XML <constraint direction sign height>) `layers`
+    cells inside the interior from the PML on the side its sign makes solid,
+    `frac` of a cell past a cell face, with a rigid object resting two cells
+    above it (a sphere or an ellipsoid with a Gaussian impulse), an optional
+    Gaussian pressure pulse (spatial width 2 h) and seeded listeners on the
+    fluid side."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml, boundary)
+    mb = s["min_bound"][direction]
+    if sign > 0:
+        height = mb + (pml + layers + frac) * h
+    else:
+        height = mb + (n - pml - layers - frac) * h
+    cons = [dict(direction=direction, sign=sign, height=height)]
+    up = np.zeros(3)
+    up[direction] = float(sign)
+    objs, imps = [], []
+    r = 0.012
+    c = rng.uniform(-0.2, 0.2, size=3) * h * 4
+    c[direction] = height + sign * (r + 2 * h)
+    if obj == "sphere":
+        objs.append(dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(r, r, r), rotation=np.eye(3)))
+    elif obj == "ellipsoid":
+        objs.append(dict(type=FDTD_OBJ_ELLIPSOID, center=tuple(c), semi_axes=(r, 0.7 * r, 0.85 * r),
+                         rotation=random_rotation(rng)))
+    if objs:
+        imps.append(dict(object=0, amplitude=1e3, width=1e-4, onset=3e-4))
+    ps = []
+    if source:
+        pos = c + up * 0.03 + np.array([0.011, -0.007, 0.004])
+        src = gaussian_pressure_source(pos, 1e-4, offset=4e-4)
+        src["width_space"] = 2 * h
+        src["normalize_constant"] = 1.0 / (math.sqrt(2 * math.pi) * 2 * h) ** 3
+        ps.append(src)
+    lo, hi = interior_bounds(s, 2)
+    if sign > 0:
+        lo[direction] = height + 3 * h
+    else:
+        hi[direction] = height - 3 * h
+    pts = []
+    while len(pts) < n_listeners:
+        p = rng.uniform(lo, hi)
+        if _object_clear(p, objs, h, 3):
+            pts.append(p)
+    sb = SceneBuffers(s, objects=objs, impulse_sources=imps, pressure_sources=ps, listeners=np.array(pts),
+                      constraints=cons)
+    return sb, steps

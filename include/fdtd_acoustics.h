@@ -181,6 +181,19 @@ typedef struct fdtd_speaker_source {
     const double *samples;       /* n_samples values (the caller owns them) */
 } fdtd_speaker_source;
 
+/* Plane constraint (FDTD_PlaneConstraint, FDTD_PlaneConstraint.h:9-45; XML
+ * <constraint direction sign height>, ImpulseResponseParser.cpp:434-449): the
+ * half-space (x[direction] - height) * sign < 0 is solid.  A cell whose
+ * centre lies in it is solid (centre test only, MAC_Grid.cpp:3566-3577), its
+ * pressure is held at zero, and a fluid neighbour's ghost value is the
+ * neighbour's own pressure (a perfectly reflecting wall, MAC_Grid.cpp:
+ * 1198-1204).  Collocated scheme with a PML only. */
+typedef struct fdtd_plane_constraint {
+    int    direction;   /* normal axis 0, 1 or 2 (XML default 1)              */
+    int    sign;        /* +1 or -1 (XML default 1)                           */
+    double height;      /* position of the plane along that axis (m), XML 0  */
+} fdtd_plane_constraint;
+
 typedef struct fdtd_scene {
     fdtd_settings                settings;
     int                          n_objects;
@@ -193,6 +206,8 @@ typedef struct fdtd_scene {
     const double                *listeners;          /* [n_listeners][3] */
     int                          n_speaker_sources;
     const fdtd_speaker_source   *speaker_sources;
+    int                          n_constraints;
+    const fdtd_plane_constraint *constraints;
 } fdtd_scene;
 
 /* Fills every field of a settings block with the parser defaults

@@ -485,6 +485,7 @@ typedef struct { int index, dim; double upd, grad; int pr, pl; } pml_vcell;
 
 typedef struct {
     int owner, neighbour, topology, object;
+    int type;          /* 0: object ghost, 1: plane-constraint ghost (MakeGhostCell's gctype) */
     uint64_t key;
     double position[3];
     double distance;
@@ -526,6 +527,8 @@ struct oracle {
     int evals_off;   /* time-parallel chunk past its stop_boundary_acc_time */
     fdtd_object *objs0;  /* poses at t = 0 of moving objects                */
     fdtd_speaker_source *spk;  /* speaker sources with copied samples       */
+    fdtd_plane_constraint *cons; /* plane constraints                       */
+    long long *solid; long long nsolid;  /* solid cells of the last classification */
     int moving;          /* some object has a velocity or angular velocity  */
     int fresh_error;     /* a fresh cell lay inside an object               */
     /* staggered (MAC) scheme: pFull; pdir[3] and v[3] are the full-grid fields */
@@ -710,6 +713,17 @@ static int cmp_ghost(const void *a, const void *b)
  * bulk = all 7 samples (centre, +-h/4 per axis) outside every object; solid
  * cells with a bulk 6-neighbour become ghost cells, one ghost entry per
  * (solid owner, bulk neighbour) pair via MakeGhostCell (4310-4340). */
+/* FDTD_Objects::OccupyByConstraint (FDTD_Objects.cpp:54-71) with
+ * FDTD_PlaneConstraint::Inside (FDTD_PlaneConstraint.h:38-42) */
+static int occupy_by_constraint(const oracle *o, const double *x)
+{
+    for (int i = 0; i < o->sc.n_constraints; ++i) {
+        const fdtd_plane_constraint *c = &o->cons[i];
+        if ((x[c->direction] - c->height) * (double)c->sign < 0.0) return 1;
+    }
+    return 0;
+}
+
 static void classify_fast(oracle *o)
 {
     const int nobj = o->sc.n_objects;
@@ -721,13 +735,31 @@ static void classify_fast(oracle *o)
     memset(o->ghostc, 0, (size_t)o->N);
     memset(o->classified, 0, (size_t)o->N);
     o->ngh = 0;
-    if (nobj == 0) { hash_build(o); return; }
+    o->nsolid = 0;
+    const int ncons = o->sc.n_constraints;
+    if (nobj == 0 && ncons == 0) { hash_build(o); return; }
     const double q = 0.25 * o->h;
     long long nsolid = 0;
     long long *solid = NULL; long long capsolid = 0;
-    for (int ob = 0; ob < nobj; ++ob) {
+    unsigned char *stype = NULL;
+    /* iteration boxes (MAC_Grid.cpp:3511-3550): each object's box with the
+       volume behind every constraint cut away, then each constraint's solid
+       half-space */
+    for (int box = 0; box < nobj + ncons; ++box) {
         double mn[3], mx[3]; int st[3], rg[3];
-        object_aabb(&o->objs[ob], mn, mx);
+        if (box < nobj) {
+            object_aabb(&o->objs[box], mn, mx);
+            for (int c = 0; c < ncons; ++c) {
+                const fdtd_plane_constraint *pc = &o->cons[c];
+                if (pc->sign > 0) mn[pc->direction] = fmax(mn[pc->direction], pc->height);
+                else              mx[pc->direction] = fmin(mx[pc->direction], pc->height);
+            }
+        } else {
+            const fdtd_plane_constraint *pc = &o->cons[box - nobj];
+            for (int d = 0; d < 3; ++d) { mn[d] = -DBL_MAX; mx[d] = DBL_MAX; }
+            if (pc->sign > 0) mx[pc->direction] = pc->height;
+            else              mn[pc->direction] = pc->height;
+        }
         iteration_box(o, mn, mx, st, rg);
         for (int i = st[0]; i < st[0] + rg[0]; ++i)
         for (int j = st[1]; j < st[1] + rg[1]; ++j)
@@ -737,7 +769,12 @@ static void classify_fast(oracle *o)
             if (o->classified[ci]) continue;
             int ijk[3] = {i, j, k}; double x[3];
             ppos(o, ijk, x);
-            int fluid = occupy_by_object(o, x) < 0;
+            /* isFluid (MAC_Grid.cpp:3566-3577): the centre outside every
+               constraint, all seven samples outside every object; a solid
+               cell is a constraint cell (type 1) when a constraint holds its
+               centre */
+            const int in_con = occupy_by_constraint(o, x);
+            int fluid = !in_con && occupy_by_object(o, x) < 0;
             for (int d = 0; d < 3 && fluid; ++d)
                 for (int sgn = -1; sgn <= 1 && fluid; sgn += 2) {
                     double y[3] = {x[0], x[1], x[2]};
@@ -746,7 +783,12 @@ static void classify_fast(oracle *o)
                 }
             if (!fluid) {
                 o->bulk[ci] = 0;
-                if (nsolid == capsolid) { capsolid = capsolid ? 2*capsolid : 1024; solid = (long long *)realloc(solid, sizeof(long long)*capsolid); }
+                if (nsolid == capsolid) {
+                    capsolid = capsolid ? 2*capsolid : 1024;
+                    solid = (long long *)realloc(solid, sizeof(long long)*capsolid);
+                    stype = (unsigned char *)realloc(stype, (size_t)capsolid);
+                }
+                stype[nsolid] = (unsigned char)in_con;
                 solid[nsolid++] = ci;
             }
             o->classified[ci] = 1;
@@ -769,6 +811,7 @@ static void classify_fast(oracle *o)
                 if (o->ngh == o->capgh) { o->capgh = o->capgh ? 2*o->capgh : 1024; o->gh = (ghost_t *)realloc(o->gh, sizeof(ghost_t)*o->capgh); }
                 ghost_t *g = &o->gh[o->ngh++];
                 g->owner = (int)ci; g->neighbour = (int)nb; g->topology = sgn * (d + 1);
+                g->type = stype[s];
                 g->pressure = 0.0;
                 double x[3]; ppos(o, ijk, x);
                 lowest_object_distance(o, x, &g->distance, &g->object);
@@ -778,7 +821,10 @@ static void classify_fast(oracle *o)
             }
         o->ghostc[ci] = (unsigned char)isg;
     }
-    free(solid);
+    free(o->solid);
+    o->solid = solid;
+    o->nsolid = nsolid;
+    free(stype);
     qsort(o->gh, o->ngh, sizeof(ghost_t), cmp_ghost);
     hash_build(o);
 }
@@ -926,6 +972,12 @@ static void ghost_update(oracle *o, const double *p)
         ghost_t *gc = &o->gh[g];
         trip_t *tr = &trips[(size_t)g * 9];
         const double p_neig = p[gc->neighbour];
+        if (gc->type == 1) {
+            /* a plane constraint reflects perfectly (MAC_Grid.cpp:1198-1204) */
+            tr[0].row = g; tr[0].col = g; tr[0].val = 1.0; ntr[g] = 1;
+            rhs[g] = p_neig;
+            continue;
+        }
         const fdtd_object *obj = &o->objs[gc->object];
         double image[3], bp[3], en[3], nrm_b[3];
         const double distance = reflect_normal(obj, gc->position, image, bp, en, nrm_b);
@@ -1433,6 +1485,17 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
         o->spk[i].samples = copy;
     }
     o->sc.speaker_sources = o->spk;
+    if (scene->n_constraints < 0 || (scene->n_constraints > 0 && !scene->constraints)) { set_err("bad constraint list"); oracle_free(o); return NULL; }
+    for (int i = 0; i < scene->n_constraints; ++i)
+        if (scene->constraints[i].direction < 0 || scene->constraints[i].direction > 2 || (scene->constraints[i].sign != 1 && scene->constraints[i].sign != -1)) {
+            set_err("constraint direction must be 0, 1 or 2 and sign +-1"); oracle_free(o); return NULL;
+        }
+    if (scene->n_constraints > 0 && (!s->use_mesh || s->scheme != FDTD_SCHEME_COLLOCATED)) {
+        set_err("plane constraints are classified with use_mesh = 1 in the collocated scheme"); oracle_free(o); return NULL;
+    }
+    o->cons = (fdtd_plane_constraint *)malloc(sizeof(fdtd_plane_constraint) * ((size_t)scene->n_constraints + 1));
+    if (scene->n_constraints) memcpy(o->cons, scene->constraints, sizeof(fdtd_plane_constraint) * (size_t)scene->n_constraints);
+    o->sc.constraints = o->cons;
     o->objs0 = (fdtd_object *)malloc(sizeof(fdtd_object) * (scene->n_objects + 1));
     memcpy(o->objs0, scene->objects, sizeof(fdtd_object) * scene->n_objects);
     for (int i = 0; i < scene->n_objects; ++i)
@@ -1496,6 +1559,7 @@ void oracle_free(oracle *o)
     free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
     free(o->sys_rp); free(o->sys_ci); free(o->sys_vv); free(o->sys_rhs);
+    free(o->cons); free(o->solid);
     free(o);
 }
 
@@ -1637,6 +1701,14 @@ int oracle_step(oracle *o, int nsteps, double *rows)
         } else {
             memcpy(o->history, o->bulk, (size_t)o->N);
             memset(o->cavity, 0, (size_t)o->N);
+            /* the per-step classification zeroes every solid cell's pressure
+               (MAC_Grid.cpp:3784-3786); only solid PML cells (behind a
+               constraint) are written by the update in between */
+            if (o->sc.n_constraints > 0)
+                for (long long q = 0; q < o->nsolid; ++q) {
+                    const long long ci = o->solid[q];
+                    o->pc[0][ci] = 0.0; o->pc[1][ci] = 0.0; o->pc[2][ci] = 0.0;
+                }
         }
         ghost_update(o, pCurr);
         pml_velocity(o, pCurr);

@@ -121,3 +121,22 @@ def test_noalloc_size_query_and_scratch_limit():
     # too small a state buffer
     st = lib.fdtd_solve_noalloc(sb.ref(), 1, C.c_void_p(256), need.value - 4, None, (C.c_double * 8)(), None)
     assert st == 1
+
+
+def test_xml_plane_constraints(tmp_path):
+    """<constraint> elements of <scene> (ImpulseResponseParser.cpp:434-449):
+    direction, sign and height, defaulting to 1, 1 and 0."""
+    lib = capi.load_library()
+    src = open(os.path.join(ROOT, "tests", "golden", "scene_sphere.xml")).read()
+    src = src.replace("<scene>", '<scene>\n    <constraint id="floor" direction="1" sign="1" height="-0.05"/>\n'
+                                 '    <constraint direction="2" sign="-1" height="0.07"/>\n    <constraint/>')
+    path = tmp_path / "floor.xml"
+    path.write_text(src)
+    p = C.c_void_p()
+    assert lib.fdtd_parse_xml(str(path).encode(), C.byref(p)) == 0
+    sc = lib.fdtd_parsed_scene(p).contents
+    assert sc.n_constraints == 3
+    got = [(sc.constraints[i].direction, sc.constraints[i].sign, sc.constraints[i].height) for i in range(3)]
+    assert got == [(1, 1, -0.05), (2, -1, 0.07), (1, 1, 0.0)]
+    assert sc.n_objects == 1
+    lib.fdtd_parsed_free(p)
