@@ -214,11 +214,17 @@ def test_air_viscosity(hip_lib, case):
     # viscous one far more than the device differs from the oracle
     ref0 = Oracle(sb0, reclassify_each_step=False).step(260)
     assert rel_l2(ref0, ref) > max(10 * TOL, 1000 * err), (rel_l2(ref0, ref), err)
+    # the reference's snapshot files lack the previous Laplacian: a viscous
+    # scene resumes from the engine state file only
+    import os
     import tempfile
     with Solver(sb, pml_state=capi.FDTD_PML_STATE_SPLIT) as a, tempfile.TemporaryDirectory() as d:
         a.step(2)
-        with pytest.raises(RuntimeError, match="Laplacian"):
-            a.save_snapshot(d + "/s_%s", "0", 0)
+        a.save_snapshot(d + "/s_%s", "0", 0)
+        os.remove(d + "/s_0_snapshot_0_engine_state.bin")
+        with Solver(sb, pml_state=capi.FDTD_PML_STATE_SPLIT) as b:
+            with pytest.raises(RuntimeError, match="Laplacian"):
+                b.load_snapshot(d + "/s_%s", "0", 0)
 
 
 def _engquist_scene(case):
