@@ -5,7 +5,8 @@
 // for every resident request (requests admitted at different times are at
 // different steps -- each carries its own device step counter), and retires
 // finished requests (copy the FP64 listener series, check it is finite, free
-// the memory).  Requests arriving while others run join at the next block:
+// the memory); a request whose signal stops being finite is evicted at the
+// next periodic check, and the others run on unaffected.  Requests arriving while others run join at the next block:
 // their allocation and upload run on a second stream while the resident
 // set's block runs, and the work list is rebuilt after that block.
 // Each request's tile list is built once, when it is admitted; a rebuild
@@ -31,6 +32,10 @@ using namespace fdtd;
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// blocks between two finiteness checks of the resident requests (one
+// listener row each, a small synchronous copy per request)
+constexpr int64_t kEvictCheckBlocks = 16;
 
 struct Request {
     int64_t id = 0;
@@ -182,9 +187,29 @@ void fdtd_server::run()
             std::lock_guard<std::mutex> lk(mu);
             if (run_block) {
                 ++stats.blocks_run;
+                // every kEvictCheckBlocks blocks, a request whose latest
+                // listener row is no longer finite (a blown-up scene: NaN
+                // and infinity never leave the field once in it) is evicted
+                // at once instead of holding its slot to the last step
+                const bool check = stats.blocks_run % kEvictCheckBlocks == 0;
+                std::vector<double> row;
                 for (Request *r : active) {
                     r->launched += block;
-                    if (r->launched < r->steps) continue;
+                    if (r->launched < r->steps) {
+                        const int L = r->dev->S.L;
+                        if (!check || !L) continue;
+                        row.resize(L);
+                        fdtd_status s = r->dev->read_series(r->launched - 1, 1, row.data());
+                        for (double v : row)
+                            if (!std::isfinite(v)) { s = FDTD_ERR_NONFINITE; break; }
+                        if (s != FDTD_OK) {
+                            set_last_error("request " + std::to_string(r->id) + " evicted at step " +
+                                           std::to_string(r->launched) + ": listener signal is not finite");
+                            finish(r, s);
+                            dirty = true;
+                        }
+                        continue;
+                    }
                     const int L = r->dev->S.L;
                     r->result.resize((size_t)r->steps * L);
                     fdtd_status s = FDTD_OK;

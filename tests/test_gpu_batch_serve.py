@@ -105,3 +105,67 @@ def test_engquist_scenes_in_batch_and_server(hip_lib):
         ref = _alone(sb, steps[i])
         assert np.array_equal(got[i], ref), i
         assert np.array_equal(served[i], ref), i
+
+
+def _blow_up_scene(seed=9, n=32):
+    """A scene whose time step is twice the FDTD stability limit
+    (c dt / h = 2/sqrt(3)): its highest spatial mode grows about 14-fold per
+    step, so the field overflows to infinity and NaN within ~100 steps."""
+    rng = np.random.default_rng(seed)
+    s = scenes.base_settings((n, n, n), 0.005, 5, dt=scenes.cfl_dt(0.005, cfl=2.0))
+    lo, hi = scenes.interior_bounds(s, 2)
+    src = scenes.gaussian_pressure_source(rng.uniform(lo, hi), 2e-4)
+    return scenes.SceneBuffers(s, pressure_sources=[src], listeners=rng.uniform(lo, hi, size=(4, 3)))
+
+
+def test_server_evicts_a_blown_up_request(hip_lib):
+    """A request that blows up is evicted with FDTD_ERR_NONFINITE at the next
+    periodic check, long before its last step, while the requests sharing its
+    launches finish with the series they give alone."""
+    sbs, steps = _small_batch(seed=45, n=4)
+    bad_steps, block = 20000, 16
+    with Server(steps_per_block=block) as srv:
+        bad = srv.submit(_blow_up_scene(), bad_steps)
+        ids = [srv.submit(sb, st) for sb, st in zip(sbs, steps)]
+        assert srv.wait(bad, 120000) == 7           # FDTD_ERR_NONFINITE
+        for rid in ids:
+            assert srv.wait(rid, 120000) == 0
+        stats = srv.stats()
+        served = [srv.take(rid)[1] for rid in ids]
+        st, ser, _ = srv.take(bad)
+    assert st == 7 and ser is None
+    assert stats["failed"] == 1 and stats["completed"] == len(ids)
+    # evicted within a few checks (16 blocks apart), not after 20000 steps
+    assert stats["blocks_run"] < 100, stats
+    for i, sb in enumerate(sbs):
+        assert np.array_equal(served[i], _alone(sb, steps[i])), i
+
+
+def test_server_take_while_admitting(hip_lib):
+    """Clients that take each result the moment it is ready, while another
+    thread keeps submitting (so the worker admits, finishes and rebuilds
+    concurrently), get every request's own series."""
+    import threading
+    import time
+    sbs, steps = _small_batch(seed=46, n=12)
+    got, ids = {}, {}
+    with Server(steps_per_block=8) as srv:
+        def submitter():
+            for i, (sb, st) in enumerate(zip(sbs, steps)):
+                ids[i] = srv.submit(sb, st)
+        t = threading.Thread(target=submitter)
+        t.start()
+        pending = set(range(len(sbs)))
+        deadline = time.monotonic() + 120
+        while pending:
+            assert time.monotonic() < deadline, pending
+            time.sleep(0.0005)
+            for i in list(pending):
+                if i in ids and srv.poll(ids[i]) != 8:    # not FDTD_ERR_BUSY
+                    st, ser, _ = srv.take(ids[i])
+                    assert st == 0, (i, st)
+                    got[i] = ser
+                    pending.discard(i)
+        t.join()
+    for i, sb in enumerate(sbs):
+        assert np.array_equal(got[i], _alone(sb, steps[i])), i
