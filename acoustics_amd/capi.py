@@ -55,9 +55,24 @@ class Object(C.Structure):
     ]
 
 
+FDTD_VIB_GAUSSIAN, FDTD_VIB_HARMONIC = 0, 1
+
+
 class ImpulseSource(C.Structure):
-    _fields_ = [("object", C.c_int), ("reserved", C.c_int), ("amplitude", C.c_double),
-                ("width", C.c_double), ("onset", C.c_double)]
+    _fields_ = [("object", C.c_int), ("kind", C.c_int), ("amplitude", C.c_double),
+                ("width", C.c_double), ("onset", C.c_double), ("omega", C.c_double), ("phase", C.c_double),
+                ("decay_rate", C.c_double), ("decay_center", C.c_double), ("t0", C.c_double), ("t1", C.c_double)]
+
+
+class Impact(C.Structure):
+    _fields_ = [("timestamp", C.c_double), ("support", C.c_double), ("impulse", C.c_double * 3),
+                ("position", C.c_double * 3)]
+
+
+class AccNoiseSource(C.Structure):
+    _fields_ = [("object", C.c_int), ("n_impacts", C.c_int), ("mass", C.c_double),
+                ("center_of_mass", C.c_double * 3), ("inv_inertia", C.c_double * 9),
+                ("impacts", C.POINTER(Impact))]
 
 
 class PressureSource(C.Structure):
@@ -84,6 +99,7 @@ class Scene(C.Structure):
         ("n_listeners", C.c_int), ("listeners", C.POINTER(C.c_double)),
         ("n_speaker_sources", C.c_int), ("speaker_sources", C.POINTER(SpeakerSource)),
         ("n_constraints", C.c_int), ("constraints", C.POINTER(PlaneConstraint)),
+        ("n_acc_noise_sources", C.c_int), ("acc_noise_sources", C.POINTER(AccNoiseSource)),
     ]
 
 
@@ -100,8 +116,7 @@ class Info(C.Structure):
 
 
 class BatchOptions(C.Structure):
-    _fields_ = [("stream", C.c_void_p), ("steps_per_graph", C.c_int), ("tile_config", C.c_int),
-                ("persistent_groups", C.c_int)]
+    _fields_ = [("stream", C.c_void_p), ("steps_per_graph", C.c_int), ("tile_config", C.c_int)]
 
 
 class ServerOptions(C.Structure):
@@ -204,6 +219,12 @@ def load_library(path: str | None = None):
     return lib
 
 
+def last_error():
+    """Error text of the library's last failed call on this thread."""
+    msg = load_library().fdtd_last_error()
+    return msg.decode(errors="replace") if msg else ""
+
+
 def check(status, what=""):
     if status != 0:
         raise FDTDError(status, what)
@@ -213,7 +234,7 @@ class SceneBuffers:
     """Owns the ctypes arrays a Scene points at (keeps them alive)."""
 
     def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None,
-                 speaker_sources=(), constraints=()):
+                 speaker_sources=(), constraints=(), acc_noise_sources=()):
         self.settings_dict = dict(settings)
         s = Settings()
         for k, v in settings.items():
@@ -239,11 +260,25 @@ class SceneBuffers:
                 self._objs[i].velocity[d] = o.get("velocity", (0.0, 0.0, 0.0))[d]
                 self._objs[i].angular_velocity[d] = o.get("angular_velocity", (0.0, 0.0, 0.0))[d]
         self._isrc = (ImpulseSource * max(1, len(self.impulse_list)))()
+        # Gaussian: dict(object, amplitude, width, onset); harmonic: dict(object,
+        # kind=FDTD_VIB_HARMONIC, omega, [amplitude=1, phase=0, decay_rate=-1,
+        # decay_center=0, t0=0, t1=0])
         for i, s_ in enumerate(self.impulse_list):
+            kind = int(s_.get("kind", FDTD_VIB_GAUSSIAN))
             self._isrc[i].object = s_["object"]
-            self._isrc[i].amplitude = s_["amplitude"]
-            self._isrc[i].width = s_["width"]
-            self._isrc[i].onset = s_["onset"]
+            self._isrc[i].kind = kind
+            if kind == FDTD_VIB_HARMONIC:
+                self._isrc[i].amplitude = s_.get("amplitude", 1.0)
+                self._isrc[i].omega = s_["omega"]
+                self._isrc[i].phase = s_.get("phase", 0.0)
+                self._isrc[i].decay_rate = s_.get("decay_rate", -1.0)
+                self._isrc[i].decay_center = s_.get("decay_center", 0.0)
+                self._isrc[i].t0 = s_.get("t0", 0.0)
+                self._isrc[i].t1 = s_.get("t1", 0.0)
+            else:
+                self._isrc[i].amplitude = s_["amplitude"]
+                self._isrc[i].width = s_["width"]
+                self._isrc[i].onset = s_["onset"]
         self._psrc = (PressureSource * max(1, len(self.pressure_list)))()
         for i, s_ in enumerate(self.pressure_list):
             for d in range(3):
@@ -287,6 +322,33 @@ class SceneBuffers:
             self._cons[i].height = float(c.get("height", 0.0))
         self.scene.n_constraints = len(self.constraint_list)
         self.scene.constraints = C.cast(self._cons, C.POINTER(PlaneConstraint))
+        # acceleration noise: dict(object, mass, center_of_mass, inv_inertia
+        # (3x3), impacts=[dict(timestamp, support, impulse, position)])
+        self.acc_noise_list = list(acc_noise_sources)
+        self._impacts = []
+        self._ans = (AccNoiseSource * max(1, len(self.acc_noise_list)))()
+        for i, a in enumerate(self.acc_noise_list):
+            imps = list(a.get("impacts", ()))
+            arr = (Impact * max(1, len(imps)))()
+            for q, im in enumerate(imps):
+                arr[q].timestamp = im["timestamp"]
+                arr[q].support = im["support"]
+                for d in range(3):
+                    arr[q].impulse[d] = im["impulse"][d]
+                    arr[q].position[d] = im.get("position", (0.0, 0.0, 0.0))[d]
+            self._impacts.append(arr)
+            self._ans[i].object = a["object"]
+            self._ans[i].n_impacts = len(imps)
+            self._ans[i].mass = a["mass"]
+            com = a.get("center_of_mass", (0.0, 0.0, 0.0))
+            inv = np.asarray(a["inv_inertia"], dtype=np.float64).reshape(9)
+            for d in range(3):
+                self._ans[i].center_of_mass[d] = com[d]
+            for q in range(9):
+                self._ans[i].inv_inertia[q] = inv[q]
+            self._ans[i].impacts = C.cast(arr, C.POINTER(Impact))
+        self.scene.n_acc_noise_sources = len(self.acc_noise_list)
+        self.scene.acc_noise_sources = C.cast(self._ans, C.POINTER(AccNoiseSource))
 
     @property
     def n_listeners(self):

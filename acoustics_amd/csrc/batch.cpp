@@ -239,7 +239,6 @@ void fdtd_batch_options_default(fdtd_batch_options *o)
     o->stream = nullptr;
     o->steps_per_graph = 32;
     o->tile_config = -1;
-    o->persistent_groups = 0;
 }
 
 fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, int n, const fdtd_batch_options *opt_in,

@@ -41,9 +41,10 @@ struct Layout {
     float *gtab;
     PSrcDev *psrc;
     ImpDev *isrc;
+    ImpactDev *impacts;
     SpecialRec *recs;
     GhostDev *ghosts;
-    int32_t *op_ptr, *op_idx, *oc_ptr, *oc_obj;
+    int32_t *op_ptr, *op_idx, *oc_ptr, *oc_ch;
     double *op_w, *oc_w;
     double *terms;
     unsigned long long *clock;
@@ -113,17 +114,18 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
     L.gtab = c.take<float>(S.gtab.size());
     L.psrc = c.take<PSrcDev>(S.psrc.size());
     L.isrc = c.take<ImpDev>(S.isrc.size());
+    L.impacts = c.take<ImpactDev>(S.impacts.size());
     L.recs = c.take<SpecialRec>(S.special.size());
     L.ghosts = c.take<GhostDev>(S.ghosts.size());
     L.op_ptr = c.take<int32_t>(S.op_ptr.size());
     L.op_idx = c.take<int32_t>(S.op_idx.size());
     L.op_w = c.take<double>(S.op_w.size());
     L.oc_ptr = c.take<int32_t>(S.oc_ptr.size());
-    L.oc_obj = c.take<int32_t>(S.oc_obj.size());
+    L.oc_ch = c.take<int32_t>(S.oc_ch.size());
     L.oc_w = c.take<double>(S.oc_w.size());
     L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
-    L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size());
+    L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size() * S.n_cpo);
     L.src_tab = c.take<float>((size_t)AMP_ROWS * S.psrc.size());
     L.spk = c.take<double>(8 * S.spk.size());
     L.spk_samples = c.take<double>(S.spk_samples.size());
@@ -207,7 +209,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     FDTD_HIP_TRY(hipMemsetAsync(L.terms, 0, sizeof(double) * (size_t)cap * std::max<size_t>(1, S.lst_terms.size()), st));
     // the acceleration ring reads as zero where no object refills it (a
     // constraint's ghost entries weight it by zero)
-    FDTD_HIP_TRY(hipMemsetAsync(L.acc_tab, 0, sizeof(double) * std::max<size_t>(1, (size_t)AMP_ROWS * S.objs.size()), st));
+    FDTD_HIP_TRY(hipMemsetAsync(L.acc_tab, 0, sizeof(double) * std::max<size_t>(1, (size_t)AMP_ROWS * S.objs.size() * S.n_cpo), st));
     std::vector<char> staging;  // pageable copies are synchronised before return
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         if (!bytes) return hipSuccess;
@@ -271,13 +273,17 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     }
     FDTD_HIP_TRY(up(L.psrc, ps.data(), ps.size() * sizeof(PSrcDev)));
     std::vector<ImpDev> is;
-    for (const ImpulseHost &h : S.isrc) is.push_back({h.obj, 0, h.A, h.w, h.ton});
+    for (const ImpulseHost &h : S.isrc) is.push_back({h.obj, h.kind, h.A, h.w, h.ton, h.omega, h.phase, h.decay, h.dcenter, h.t0, h.t1});
     FDTD_HIP_TRY(up(L.isrc, is.data(), is.size() * sizeof(ImpDev)));
+    std::vector<ImpactDev> im;
+    for (const ImpactHost &h : S.impacts) im.push_back({h.obj, 0, h.t0, h.tau, {h.v[0], h.v[1], h.v[2]}});
+    FDTD_HIP_TRY(up(L.impacts, im.data(), im.size() * sizeof(ImpactDev)));
     std::vector<SpecialRec> recs(S.special);
     for (SpecialRec &r : recs) r.cell = (int32_t)to_dev(r.cell);
     FDTD_HIP_TRY(up(L.recs, recs.data(), recs.size() * sizeof(SpecialRec)));
     std::vector<GhostDev> gd;
-    for (const GhostEntry &g : S.ghosts) gd.push_back({(int)to_dev(g.nbr), g.obj, g.coupled, 0, g.k_raster, g.c_acc});
+    for (const GhostEntry &g : S.ghosts)
+        gd.push_back({(int)to_dev(g.nbr), g.obj, g.coupled, std::abs(g.topology) - 1, g.k_raster, g.k_vec});
     FDTD_HIP_TRY(up(L.ghosts, gd.data(), gd.size() * sizeof(GhostDev)));
     FDTD_HIP_TRY(up(L.op_ptr, S.op_ptr.data(), S.op_ptr.size() * 4));
     std::vector<int32_t> op_idx(S.op_idx);
@@ -285,7 +291,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     FDTD_HIP_TRY(up(L.op_idx, op_idx.data(), op_idx.size() * 4));
     FDTD_HIP_TRY(up(L.op_w, S.op_w.data(), S.op_w.size() * 8));
     FDTD_HIP_TRY(up(L.oc_ptr, S.oc_ptr.data(), S.oc_ptr.size() * 4));
-    FDTD_HIP_TRY(up(L.oc_obj, S.oc_obj.data(), S.oc_obj.size() * 4));
+    FDTD_HIP_TRY(up(L.oc_ch, S.oc_ch.data(), S.oc_ch.size() * 4));
     FDTD_HIP_TRY(up(L.oc_w, S.oc_w.data(), S.oc_w.size() * 8));
     const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
@@ -327,16 +333,20 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     D.psrc = L.psrc;
     D.gtab = L.gtab;
     D.isrc = L.isrc;
+    D.n_impact = (int)S.impacts.size();
+    D.impacts = L.impacts;
     D.recs = L.recs;
     D.ghosts = L.ghosts;
     D.G = (int)S.ghosts.size();
     D.op_ptr = L.op_ptr; D.op_idx = L.op_idx; D.op_w = L.op_w;
-    D.oc_ptr = L.oc_ptr; D.oc_obj = L.oc_obj; D.oc_w = L.oc_w;
+    D.oc_ptr = L.oc_ptr; D.oc_ch = L.oc_ch; D.oc_w = L.oc_w;
     D.T = (int)S.lst_terms.size();
     D.cap = cap;
     D.terms = L.terms;
     D.clock = L.clock;
     D.n_obj = (int)S.objs.size();
+    D.n_cpo = S.n_cpo;
+    D.n_ch = D.n_obj * S.n_cpo;
     D.acc_tab = L.acc_tab;
     D.src_tab = L.src_tab;
     D.n_spk = (int)S.spk.size();

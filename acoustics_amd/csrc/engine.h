@@ -28,16 +28,25 @@ void geom_aabb(const Geom &g, double *mn, double *mx);
 // --------------------------------------------------------------------------
 enum : int { DIR_XM = 0, DIR_XP = 1, DIR_YM = 2, DIR_YP = 3, DIR_ZM = 4, DIR_ZP = 5 };
 
+// The boundary acceleration of object o at a boundary point with outward
+// normal n is a_s(o, t) n + a_v(o, t): a_s sums the sources whose vector is
+// a normal one (Gaussian impulse, harmonic), a_v the acceleration-noise
+// vector.  The device keeps them as amplitude channels, per object
+// [a_s] or, in a scene with acceleration noise, [a_s, a_v.x, a_v.y, a_v.z].
 struct GhostEntry {
     int owner;        // solid cell (flat index)
     int nbr;          // bulk neighbour (flat index)
     int topology;     // +-1, +-2, +-3 (direction owner -> nbr)
     int obj;          // closest object
     double pos[3];    // owner centre + h/4 towards nbr
-    double k_raster;  // p_g = p[nbr] + k_raster * a_n(obj, t)
-    // fully-coupled row: x_g = sum_k w_k p[k] + c_acc * a_n(obj, t) + sum_u w_u x_u
+    // rasterised value p_g = p[nbr] - 0.75 h grad_p . cell_n with
+    // grad_p = -rho (a_s n_b + a_v):  p[nbr] + k_raster a_s + k_vec a_v[|topology| - 1]
+    double k_raster;  // 0.75 h rho (n_b . cell_n)
+    double k_vec;     // 0.75 h rho sign(topology)
+    // fully-coupled row: x_g = sum_k w_k p[k] + c_acc a_s + c_vec . a_v + sum_u w_u x_u
     int coupled;      // 1: coupled row, 0: rasterised value
     double c_acc;
+    double c_vec[3];
     int type;         // 0: object ghost, 1: plane-constraint ghost (p_g = p[nbr])
 };
 
@@ -60,8 +69,17 @@ struct PSrcHost {
 };
 
 struct ImpulseHost {
+    int obj, kind;                 // FDTD_VIB_*
+    double A, w, ton;              // Gaussian
+    double omega, phase, decay, dcenter, t0, t1;   // harmonic (t1 <= t0: no range)
+};
+
+// One acceleration-noise impact: a_v(t) += S(t) v over t0 <= t < t0 + tau,
+// S = sin(pi (t - t0) / tau), v = R pi / (2 tau) (J / m + (I^-1 (r x J)) x r)
+// in the world frame.
+struct ImpactHost {
     int obj;
-    double A, w, ton;
+    double t0, tau, v[3];
 };
 
 struct ListenerTerm {
@@ -143,6 +161,8 @@ struct SceneSetup {
     std::vector<PSrcHost> psrc;
     std::vector<float> gtab;
     std::vector<ImpulseHost> isrc;
+    std::vector<ImpactHost> impacts;
+    int n_cpo = 1;                     // amplitude channels per object: 1, or 4 with acceleration noise
     // speaker sources (collocated scheme): parameters and samples, and the
     // index box within 8 widths of each, where tiles evaluate them
     struct SpeakerHost { double p[3], w, t0, period; int off, n; int lo[3], hi[3]; };
@@ -174,11 +194,12 @@ struct SceneSetup {
     std::vector<int32_t> gk_ptr, gk_idx;  std::vector<double> gk_w;
     std::vector<int32_t> gu_ptr, gu_idx;  std::vector<double> gu_w;
     // the system's solution operator x = M p + m a (fully-coupled scenes):
-    // per ghost, cells and weights of M (CSR, dense cell indices) and objects
-    // and weights of m; op_max_cycle is the largest set of mutually coupled
+    // per ghost, cells and weights of M (CSR, dense cell indices) and
+    // amplitude channels (object * n_cpo + component) and weights of m;
+    // op_max_cycle is the largest set of mutually coupled
     // ghosts (1: the coupling graph is acyclic)
     std::vector<int32_t> op_ptr, op_idx;  std::vector<double> op_w;
-    std::vector<int32_t> oc_ptr, oc_obj;  std::vector<double> oc_w;
+    std::vector<int32_t> oc_ptr, oc_ch;  std::vector<double> oc_w;
     int op_max_cycle = 0;
     int64_t n_coupled = 0;
     // listeners

@@ -23,14 +23,23 @@ struct PSrcDev {
     int pad;
 };
 
+// vibrational source with a normal acceleration vector (FDTD_VIB_*)
 struct ImpDev {
-    int obj, pad;
-    double A, w, ton;
+    int obj, kind;
+    double A, w, ton;                              // Gaussian
+    double omega, phase, decay, dcenter, t0, t1;   // harmonic
 };
 
+// acceleration-noise impact: S(t) v over [t0, t0 + tau), world frame
+struct ImpactDev {
+    int obj, pad;
+    double t0, tau, v[3];
+};
+
+// rasterised ghost value p[nbr] + k_raster a_s(obj) + k_vec a_v(obj)[vaxis]
 struct GhostDev {
-    int nbr, obj, coupled, pad;
-    double k_raster, c_acc;
+    int nbr, obj, coupled, vaxis;
+    double k_raster, k_vec;
 };
 
 // pressure slots inside SceneDev::field, each npad floats
@@ -84,17 +93,18 @@ struct SceneDev {
     float *lapv;
     double kcalp, c2k2, inv_h2;
     float kcalpf, c2k2_kcalpf, inv_h2f;
-    int n_psrc, n_isrc, n_special;
+    int n_psrc, n_isrc, n_special, n_impact;
     const PSrcDev *psrc;
     const float *gtab;
     const ImpDev *isrc;
+    const ImpactDev *impacts;
     const SpecialRec *recs;
     const GhostDev *ghosts;
     // fully-coupled mode: the coupled system's solution operator, per ghost g
     // x_g = sum_q op_w[q] p[op_idx[q]] (q in [op_ptr[g], op_ptr[g+1]))
-    //     + sum_q oc_w[q] a_n(oc_obj[q])  (q in [oc_ptr[g], oc_ptr[g+1]))
+    //     + sum_q oc_w[q] acc[oc_ch[q]]  (q in [oc_ptr[g], oc_ptr[g+1]))
     int G;
-    const int32_t *op_ptr, *op_idx, *oc_ptr, *oc_obj;
+    const int32_t *op_ptr, *op_idx, *oc_ptr, *oc_ch;
     const double *op_w, *oc_w;
     // listener series: row s of the ring holds beta_q * sample_q of every MLS
     // term q of every listener after step s; a listener's value is the sum
@@ -103,8 +113,9 @@ struct SceneDev {
     double *terms;              // [cap][T]
     unsigned long long *clock;  // CLK_WORDS words
     int n_obj;
+    int n_cpo, n_ch;            // amplitude channels per object (1, or 4 with acceleration noise), n_obj n_cpo
     // time-dependent amplitudes, one ring row per step (amp_row):
-    double *acc_tab;            // [AMP_ROWS][n_obj] normal acceleration a_n(obj, t_s)
+    double *acc_tab;            // [AMP_ROWS][n_ch] channel o n_cpo: a_s(o, t_s); o n_cpo + 1 + d: a_v(o, t_s)[d]
     float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
     long long src_steps;        // time-parallel chunk: sources act in steps < src_steps (-1: always)
     // speaker sources: per source {px, py, pz, w, t0, period, off, n} and the samples

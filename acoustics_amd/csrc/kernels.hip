@@ -46,14 +46,38 @@ __device__ __forceinline__ const AS_GLOBAL T *gmem(const T *p) { return (const A
 template <class T>
 __device__ __forceinline__ const AS_CONST T *cmem(const T *p) { return (const AS_CONST T *)p; }
 
+// a_s(obj, t): the sources whose acceleration vector is a_n times the normal
+// (Gaussian impulse; HarmonicVibrationalSource::Evaluate,
+// HarmonicVibrationalSource.cpp:5-28)
 __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int obj, double t)
 {
     double a = 0.0;
     for (int q = 0; q < n; ++q) {
         const ImpDev s = isrc[q];
         if (s.obj != obj) continue;
-        const double u = t - s.ton;
-        a += s.A * exp(-(u * u) / (2.0 * s.w * s.w));
+        if (s.kind == FDTD_VIB_HARMONIC) {
+            if (s.t1 > s.t0 && (t < s.t0 || t > s.t1)) continue;
+            const double v = sin(s.omega * t + s.phase) * (s.A * s.omega);
+            a += s.decay < 0.0 ? v : exp(-s.decay * (t - s.dcenter)) * v;
+        } else {
+            const double u = t - s.ton;
+            a += s.A * exp(-(u * u) / (2.0 * s.w * s.w));
+        }
+    }
+    return a;
+}
+
+// a_v(obj, t)[d]: acceleration noise, the half-sine contact of every impact
+// whose support holds t (AccelerationNoiseVibrationalSource::Evaluate,
+// AccelerationNoiseVibrationalSource.cpp:63-90, and ImpulseSeriesObject::
+// GetImpulseWithinSupport: t0 <= t < t0 + tau)
+__device__ __forceinline__ double noise_accel(const ImpactDev *imp, int n, int obj, int d, double t)
+{
+    double a = 0.0;
+    for (int q = 0; q < n; ++q) {
+        const ImpactDev r = imp[q];
+        if (r.obj != obj || t < r.t0 || !(t < r.t0 + r.tau)) continue;
+        a += sin(3.141592653589793 * (t - r.t0) / r.tau) * r.v[d];
     }
     return a;
 }
@@ -110,6 +134,17 @@ __device__ __forceinline__ float speaker_at(const SceneDev &S, int i, int j, int
 // x_g = sum_q M_gq p_q + sum_o m_go a_n(o), FP64, in row order.  Computed where
 // it is used (the object-adjacent cell next to the ghost, and listener terms),
 // four independent index loads and gathers at a time.
+// -0.75 h grad_p . cell_n of a rasterised ghost, grad_p = -rho (a_s n_b + a_v)
+// (MAC_Grid.cpp:1256-1281): k_raster a_s, plus k_vec a_v along the ghost's
+// axis in a scene with acceleration noise
+__device__ __forceinline__ double raster_accel_term(const SceneDev &S, const GhostDev &gd, long long step)
+{
+    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_ch + gd.obj * S.n_cpo;
+    double v = gd.k_raster * acc[0];
+    if (S.n_cpo > 1) v = fma(gd.k_vec, acc[1 + gd.vaxis], v);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double coupled_ghost(const SceneDev &S, int g, const AS_GLOBAL float *__restrict__ pcur,
                                                 long long step)
@@ -129,9 +164,9 @@ __device__ __forceinline__ double coupled_ghost(const SceneDev &S, int g, const 
         x = fma(w3, (double)v3, x);
     }
     for (; q < q1; ++q) x = fma(w[q], (double)pcur[idx[q]], x);
-    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_obj;
+    const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_ch;
     const int c1 = gmem(S.oc_ptr)[g + 1];
-    for (int c = gmem(S.oc_ptr)[g]; c < c1; ++c) x = fma(gmem(S.oc_w)[c], acc[gmem(S.oc_obj)[c]], x);
+    for (int c = gmem(S.oc_ptr)[g]; c < c1; ++c) x = fma(gmem(S.oc_w)[c], acc[gmem(S.oc_ch)[c]], x);
     return x;
 }
 
@@ -159,7 +194,7 @@ __device__ __forceinline__ void emit_ghost_terms(const SceneDev &S, const TileDe
         double v;
         if (S.raster) {
             const GhostDev gd = gmem(S.ghosts)[r.cell];
-            v = (double)pcur[gd.nbr] + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
+            v = (double)pcur[gd.nbr] + raster_accel_term(S, gd, step);
         } else {
             v = coupled_ghost(S, r.cell, pcur, step);
         }
@@ -810,7 +845,7 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         if (gid >= 0) {
             if (S.raster) {
                 const GhostDev gd = gmem(S.ghosts)[gid];
-                g[q] = c + gd.k_raster * gmem(S.acc_tab)[amp_row(step) * S.n_obj + gd.obj];
+                g[q] = c + raster_accel_term(S, gd, step);
             } else {
                 g[q] = coupled_ghost(S, gid, pcur, step);
             }
@@ -1046,7 +1081,7 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
         mls_weights(nv, pts, image, S.h, 0.018, beta);
         double ml = 0.0, mc = 0.0;
         for (int r = 0; r < nv; ++r) { ml += beta[r] * vl8[r]; mc += beta[r] * vc8[r]; }
-        const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
+        const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
         const double add = S.rho * a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]) * fabs(travelled);
         *vl = ml + add;
         *vc = mc + add;
@@ -1161,7 +1196,7 @@ __global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__res
     for (int e = 0; e < 3; ++e) gp[e] = xo[e];
     gp[d] -= sg * 0.25 * S.h;
     geom_closest(S.pose[id], gp, xb, nb);
-    const double a = gmem(S.acc_tab)[amp_row(step) * S.n_obj + id];
+    const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
     S.ughost[(size_t)q * ncell + c] = 0.75 * S.h * S.rho * a * nb[d] * -(double)sg;
 }
 
@@ -1249,10 +1284,13 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
     const long long base = min(step + (long long)n, target);
     // a time-parallel chunk's sources are off from step src_steps on
     const long long off = S.src_steps < 0 ? LLONG_MAX : S.src_steps;
-    for (int q = threadIdx.x; q < AMP_ROWS * S.n_obj; q += blockDim.x) {
-        const long long s = base + q / S.n_obj;
-        const int o = q % S.n_obj;
-        gmem(S.acc_tab)[amp_row(s) * S.n_obj + o] = s < off ? impulse_accel(S.isrc, S.n_isrc, o, S.t_start + (double)s * S.dt) : 0.0;
+    for (int q = threadIdx.x; q < AMP_ROWS * S.n_ch; q += blockDim.x) {
+        const long long s = base + q / S.n_ch;
+        const int ch = q % S.n_ch, o = ch / S.n_cpo, comp = ch % S.n_cpo;
+        const double t = S.t_start + (double)s * S.dt;
+        double a = 0.0;
+        if (s < off) a = comp == 0 ? impulse_accel(S.isrc, S.n_isrc, o, t) : noise_accel(S.impacts, S.n_impact, o, comp - 1, t);
+        gmem(S.acc_tab)[amp_row(s) * S.n_ch + ch] = a;
     }
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_psrc; q += blockDim.x) {
         const long long s = base + q / S.n_psrc;

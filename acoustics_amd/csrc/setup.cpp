@@ -208,9 +208,10 @@ std::vector<std::vector<int>> coupling_components(const SceneSetup &S)
 
 }  // namespace
 
-// Rows of M (cells) and m (objects) of the coupled ghost operator
+// Rows of M (cells) and m (amplitude channels) of the coupled ghost operator
 // x = M p + m a, from the coupled rows (gk: known cells, gu: unknown ghosts,
-// c_acc) and the rasterized rows (p[nbr] + k_raster a).
+// c_acc a_s + c_vec . a_v) and the rasterized rows (p[nbr] + k_raster a_s +
+// k_vec a_v along the ghost's axis).
 fdtd_status build_coupled_operator(SceneSetup &S, std::string &err)
 {
     const int G = (int)S.ghosts.size();
@@ -222,10 +223,14 @@ fdtd_status build_coupled_operator(SceneSetup &S, std::string &err)
             SparseRow r;
             for (int q = S.gk_ptr[g]; q < S.gk_ptr[g + 1]; ++q) r.push_back({S.gk_idx[q], S.gk_w[q]});
             K[g] = combine({{1.0, &r}});
-            C[g] = {{e.obj, e.c_acc}};
+            C[g] = {{e.obj * S.n_cpo, e.c_acc}};
+            if (S.n_cpo > 1)
+                for (int d = 0; d < 3; ++d)
+                    if (e.c_vec[d] != 0.0) C[g].push_back({e.obj * S.n_cpo + 1 + d, e.c_vec[d]});
         } else {
             K[g] = {{e.nbr, 1.0}};
-            C[g] = {{e.obj, e.k_raster}};
+            C[g] = {{e.obj * S.n_cpo, e.k_raster}};
+            if (S.n_cpo > 1 && e.k_vec != 0.0) C[g].push_back({e.obj * S.n_cpo + 1 + std::abs(e.topology) - 1, e.k_vec});
         }
     }
     const std::vector<std::vector<int>> comps = coupling_components(S);
@@ -303,8 +308,8 @@ fdtd_status build_coupled_operator(SceneSetup &S, std::string &err)
     for (int g = 0; g < G; ++g) {
         for (const auto &e : M[g]) if (e.second != 0.0) { S.op_idx.push_back(e.first); S.op_w.push_back(e.second); }
         S.op_ptr.push_back((int32_t)S.op_idx.size());
-        for (const auto &e : Mo[g]) if (e.second != 0.0) { S.oc_obj.push_back(e.first); S.oc_w.push_back(e.second); }
-        S.oc_ptr.push_back((int32_t)S.oc_obj.size());
+        for (const auto &e : Mo[g]) if (e.second != 0.0) { S.oc_ch.push_back(e.first); S.oc_w.push_back(e.second); }
+        S.oc_ptr.push_back((int32_t)S.oc_ch.size());
     }
     return FDTD_OK;
 }
@@ -372,7 +377,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         err = "plane constraints are classified with use_mesh = 1 in the collocated scheme";
         return FDTD_ERR_UNSUPPORTED;
     }
-    if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0)) {
+    if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0 || sc.n_acc_noise_sources > 0)) {
         err = "the staggered scheme runs free-field scenes only (objects need the reference's mesh distance-field "
               "classification of the MAC mode); use the collocated scheme";
         return FDTD_ERR_UNSUPPORTED;
@@ -609,8 +614,65 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     for (int q = 0; q < sc.n_impulse_sources; ++q) {
         const fdtd_impulse_source &is = sc.impulse_sources[q];
         if (is.object < 0 || is.object >= sc.n_objects) { err = "impulse source names a missing object"; return FDTD_ERR_INVALID; }
-        if (!(is.width > 0)) { err = "impulse width must be positive"; return FDTD_ERR_INVALID; }
-        S.isrc.push_back({is.object, is.amplitude, is.width, is.onset});
+        ImpulseHost H{};
+        H.obj = is.object;
+        H.kind = is.kind;
+        if (is.kind == FDTD_VIB_GAUSSIAN) {
+            if (!(is.width > 0)) { err = "impulse width must be positive"; return FDTD_ERR_INVALID; }
+            H.A = is.amplitude; H.w = is.width; H.ton = is.onset;
+        } else if (is.kind == FDTD_VIB_HARMONIC) {
+            if (!std::isfinite(is.omega) || !std::isfinite(is.phase) || !std::isfinite(is.amplitude)) {
+                err = "a harmonic source needs a finite amplitude, omega and phase";
+                return FDTD_ERR_INVALID;
+            }
+            H.A = is.amplitude; H.omega = is.omega; H.phase = is.phase;
+            H.decay = is.decay_rate; H.dcenter = is.decay_center; H.t0 = is.t0; H.t1 = is.t1;
+        } else {
+            err = "vibrational source kind must be FDTD_VIB_GAUSSIAN or FDTD_VIB_HARMONIC";
+            return FDTD_ERR_INVALID;
+        }
+        S.isrc.push_back(H);
+    }
+    // acceleration noise: the constant vector of each impact, in the world
+    // frame (AccelerationNoiseVibrationalSource.cpp:70-89; impacts with
+    // tau < SMALL_NUM = 1e-12 are skipped there)
+    S.impacts.clear();
+    S.n_cpo = sc.n_acc_noise_sources > 0 ? 4 : 1;
+    if (sc.n_acc_noise_sources < 0) { err = "negative acceleration-noise source count"; return FDTD_ERR_INVALID; }
+    for (int q = 0; q < sc.n_acc_noise_sources; ++q) {
+        const fdtd_acc_noise_source &an = sc.acc_noise_sources[q];
+        if (an.object < 0 || an.object >= sc.n_objects) { err = "acceleration-noise source names a missing object"; return FDTD_ERR_INVALID; }
+        if (an.n_impacts < 0 || (an.n_impacts > 0 && !an.impacts) || !(an.mass > 0)) {
+            err = "an acceleration-noise source needs a positive mass and its impacts";
+            return FDTD_ERR_INVALID;
+        }
+        const fdtd_object &ob = sc.objects[an.object];
+        for (int d = 0; d < 3; ++d)
+            if (ob.velocity[d] != 0.0 || ob.angular_velocity[d] != 0.0) {
+                err = "acceleration noise is supported on static objects only";
+                return FDTD_ERR_UNSUPPORTED;
+            }
+        const double *Iv = an.inv_inertia, *R = ob.rotation;
+        for (int m = 0; m < an.n_impacts; ++m) {
+            const fdtd_impact &im = an.impacts[m];
+            if (!(im.support >= 1e-12)) continue;
+            const double *J = im.impulse;
+            double r[3], rxJ[3], al[3], a[3];
+            for (int d = 0; d < 3; ++d) r[d] = im.position[d] - an.center_of_mass[d];
+            rxJ[0] = r[1] * J[2] - r[2] * J[1]; rxJ[1] = r[2] * J[0] - r[0] * J[2]; rxJ[2] = r[0] * J[1] - r[1] * J[0];
+            for (int d = 0; d < 3; ++d) al[d] = Iv[3 * d] * rxJ[0] + Iv[3 * d + 1] * rxJ[1] + Iv[3 * d + 2] * rxJ[2];
+            // J / m + (I^-1 (r x J)) x r, then pi / (2 tau), in the object frame
+            a[0] = J[0] / an.mass + (al[1] * r[2] - al[2] * r[1]);
+            a[1] = J[1] / an.mass + (al[2] * r[0] - al[0] * r[2]);
+            a[2] = J[2] / an.mass + (al[0] * r[1] - al[1] * r[0]);
+            const double f = 3.141592653589793 / (2.0 * im.support);
+            ImpactHost H{};
+            H.obj = an.object;
+            H.t0 = im.timestamp;
+            H.tau = im.support;
+            for (int d = 0; d < 3; ++d) H.v[d] = f * (R[3 * d] * a[0] + R[3 * d + 1] * a[1] + R[3 * d + 2] * a[2]);
+            S.impacts.push_back(H);
+        }
     }
 
     // ---- objects: rasterisation -------------------------------------------
@@ -633,6 +695,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             err = "moving objects run with the collocated scheme, rasterized ghost cells and no air viscosity";
             return FDTD_ERR_UNSUPPORTED;
         }
+        if (S.n_cpo > 1) { err = "acceleration noise is supported in scenes without moving objects"; return FDTD_ERR_UNSUPPORTED; }
         if (sc.n_objects > MAX_MOVING_OBJ) {
             err = "a scene with moving objects holds at most " + std::to_string((int)MAX_MOVING_OBJ) + " objects";
             return FDTD_ERR_UNSUPPORTED;
@@ -859,8 +922,10 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         if (gc.type == 1) {
             // a plane constraint reflects perfectly: p_g = p[nbr] (MAC_Grid.cpp:1198-1204)
             gc.k_raster = 0.0;
+            gc.k_vec = 0.0;
             gc.coupled = 0;
             gc.c_acc = 0.0;
+            gc.c_vec[0] = gc.c_vec[1] = gc.c_vec[2] = 0.0;
             S.gk_ptr.push_back((int32_t)S.gk_idx.size());
             S.gu_ptr.push_back((int32_t)S.gu_idx.size());
             continue;
@@ -884,8 +949,10 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         const double ndc = nb[0] * cell_n[0] + nb[1] * cell_n[1] + nb[2] * cell_n[2];
         const double nde = nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2];
         gc.k_raster = 0.75 * h * rho * ndc;
+        gc.k_vec = 0.75 * h * rho * (gc.topology > 0 ? 1.0 : -1.0);
         gc.coupled = 0;
         gc.c_acc = 0;
+        gc.c_vec[0] = gc.c_vec[1] = gc.c_vec[2] = 0.0;
         bool done = false;
         if (coupled) {
             int64_t cells[8];
@@ -965,6 +1032,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     const double weights = (dist < 0.0 ? -2.0 * std::fabs(dist) : -std::fabs(dist));
                     gc.coupled = 1;
                     gc.c_acc = -rho * nde * (weights + (s_i >= 0 ? w[s_i] * h : 0.0));
+                    for (int e = 0; e < 3; ++e) gc.c_vec[e] = -rho * en[e] * (weights + (s_i >= 0 ? w[s_i] * h : 0.0));
                     for (int q = 0; q < 8; ++q) {
                         if (kcell[q] >= 0 && S.bulk[kcell[q]]) { S.gk_idx.push_back(kcell[q]); S.gk_w.push_back(w[q]); }
                         if (ughost[q] >= 0) { S.gu_idx.push_back(ughost[q]); S.gu_w.push_back(w[q]); }
@@ -990,7 +1058,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     // with a small dense inverse of I - W over each cycle.  The result is the
     // exact solution (to rounding) whether or not an iteration would converge.
     S.op_ptr.assign(1, 0); S.op_idx.clear(); S.op_w.clear();
-    S.oc_ptr.assign(1, 0); S.oc_obj.clear(); S.oc_w.clear();
+    S.oc_ptr.assign(1, 0); S.oc_ch.clear(); S.oc_w.clear();
     S.op_max_cycle = 0;
     if (S.n_coupled > 0) {
         fdtd_status ost = build_coupled_operator(S, err);

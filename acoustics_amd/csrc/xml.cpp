@@ -5,7 +5,10 @@
 //   <analytic_sphere center_x= center_y= center_z= radius=>
 //   <analytic_ellipsoid center_x=.. semi_axis_x=.. semi_axis_y=.. semi_axis_z=..
 //                       [quat_w= quat_x= quat_y= quat_z=]>
-// each with any number of <gaussian_impulse_source amplitude= width= onset=/>.
+// each with any number of <gaussian_impulse_source amplitude= width= onset=/>,
+// the reference's has_harmonic_source= harmonic_source_f= attributes, and
+// <acc_noise_source mass= [inv_inertia= center_of_mass_x..z=]> holding
+// <impact timestamp= support= impulse_x..z= position_x..z=/> elements.
 // A small attribute-only XML reader replaces the vendored tinyxml.
 #include <cmath>
 #include <cstdio>
@@ -149,8 +152,17 @@ std::string opt_str(const Node *n, const char *a, const char *def)
 }  // namespace
 }  // namespace fdtd
 
+namespace fdtd {
+struct AccNoise {   // one acc_noise_source element and its impacts
+    fdtd_acc_noise_source src{};
+    std::vector<fdtd_impact> impacts;
+};
+}  // namespace fdtd
+
 struct fdtd_parsed {
     fdtd_scene scene;
+    std::vector<std::unique_ptr<fdtd::AccNoise>> acc_noise;
+    std::vector<fdtd_acc_noise_source> acc_noise_src;
     std::vector<fdtd_object> objects;
     std::vector<fdtd_impulse_source> impulses;
     std::vector<fdtd_pressure_source> psrc;
@@ -381,10 +393,52 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
                 for (const Node *g : o->children("gaussian_impulse_source")) {
                     fdtd_impulse_source is{};
                     is.object = oi;
+                    is.kind = FDTD_VIB_GAUSSIAN;
                     is.amplitude = req_real(g, "amplitude");
                     is.width = req_real(g, "width");
                     is.onset = opt_real(g, "onset", 4.0 * is.width);
                     P->impulses.push_back(is);
+                }
+                // ImpulseResponseParser.cpp:266-275: omega = 2 pi f, phase 0,
+                // unit pulsation velocity, no decay, no time range
+                if (opt_real(o, "has_harmonic_source", 0.0) != 0.0) {
+                    fdtd_impulse_source is{};
+                    is.object = oi;
+                    is.kind = FDTD_VIB_HARMONIC;
+                    is.amplitude = 1.0;
+                    is.omega = 2.0 * M_PI * req_real(o, "harmonic_source_f");
+                    is.phase = 0.0;
+                    is.decay_rate = -1.0;
+                    P->impulses.push_back(is);
+                }
+                // acceleration noise of an analytic object: its impacts listed
+                // inline (the reference reads them from a rigid-body simulator's
+                // impulse file against a mesh); mass, centre of mass and inverse
+                // inertia default to a uniform solid of the object's shape
+                for (const Node *g : o->children("acc_noise_source")) {
+                    auto an = std::make_unique<AccNoise>();
+                    an->src.object = oi;
+                    an->src.mass = req_real(g, "mass");
+                    const double m = an->src.mass, a = ob.semi_axes[0], b = ob.semi_axes[1], c = ob.semi_axes[2];
+                    const double I[3] = {m * (b * b + c * c) / 5.0, m * (a * a + c * c) / 5.0, m * (a * a + b * b) / 5.0};
+                    for (int d = 0; d < 9; ++d) an->src.inv_inertia[d] = d % 4 == 0 ? 1.0 / I[d / 4] : 0.0;
+                    if (g->attr.count("inv_inertia")) {
+                        std::istringstream in(g->attr.at("inv_inertia"));
+                        for (int d = 0; d < 9; ++d)
+                            if (!(in >> an->src.inv_inertia[d])) throw ParseError{"acc_noise_source inv_inertia needs 9 numbers"};
+                    }
+                    an->src.center_of_mass[0] = opt_real(g, "center_of_mass_x", 0.0);
+                    an->src.center_of_mass[1] = opt_real(g, "center_of_mass_y", 0.0);
+                    an->src.center_of_mass[2] = opt_real(g, "center_of_mass_z", 0.0);
+                    for (const Node *im : g->children("impact")) {
+                        fdtd_impact r{};
+                        r.timestamp = req_real(im, "timestamp");
+                        r.support = req_real(im, "support");
+                        const char *jn[3] = {"impulse_x", "impulse_y", "impulse_z"}, *pn[3] = {"position_x", "position_y", "position_z"};
+                        for (int d = 0; d < 3; ++d) { r.impulse[d] = req_real(im, jn[d]); r.position[d] = opt_real(im, pn[d], 0.0); }
+                        an->impacts.push_back(r);
+                    }
+                    P->acc_noise.push_back(std::move(an));
                 }
             }
         }
@@ -397,6 +451,13 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
     sc.objects = P->objects.data();
     sc.n_impulse_sources = (int)P->impulses.size();
     sc.impulse_sources = P->impulses.data();
+    for (auto &an : P->acc_noise) {
+        an->src.n_impacts = (int)an->impacts.size();
+        an->src.impacts = an->impacts.data();
+        P->acc_noise_src.push_back(an->src);
+    }
+    sc.n_acc_noise_sources = (int)P->acc_noise_src.size();
+    sc.acc_noise_sources = P->acc_noise_src.data();
     sc.n_pressure_sources = (int)P->psrc.size();
     sc.pressure_sources = P->psrc.data();
     sc.n_listeners = (int)(P->listeners.size() / 3);

@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from .capi import (FDTD_SCHEME_STAGGERED, FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_ELLIPSOID,
-                   FDTD_OBJ_SPHERE, SceneBuffers)
+                   FDTD_OBJ_SPHERE, FDTD_VIB_HARMONIC, SceneBuffers)
 
 C_SOUND = 343.0
 RHO = 1.2
@@ -382,3 +382,45 @@ def floor_scene(seed=16, n=48, h=0.005, pml=8, direction=1, sign=1, layers=6, fr
     sb = SceneBuffers(s, objects=objs, impulse_sources=imps, pressure_sources=ps, listeners=np.array(pts),
                       constraints=cons)
     return sb, steps
+
+
+def uniform_solid_inv_inertia(mass, semi_axes):
+    """Inverse inertia tensor (object frame) of a uniform solid ellipsoid:
+    I = m diag(b^2 + c^2, a^2 + c^2, a^2 + b^2) / 5 (a sphere: 2/5 m r^2)."""
+    a, b, c = semi_axes
+    return np.diag([5.0 / (mass * (b * b + c * c)), 5.0 / (mass * (a * a + c * c)), 5.0 / (mass * (a * a + b * b))])
+
+
+def vibration_scene(seed=17, n=48, h=0.005, pml=8, steps=300, n_obj=2, n_listeners=10, harmonic=True,
+                    acc_noise=True, gaussian=False, boundary=FDTD_BOUNDARY_RASTERIZE):
+    """Rotated ellipsoids whose surfaces carry harmonic pulsations (seeded
+    frequency 1-4 kHz, phase, decay and time range) and acceleration noise
+    (three impacts each at seeded off-centre points, overlapping supports of
+    0.1-0.4 ms), optionally a Gaussian impulse too."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml, boundary)
+    objs = _place_ellipsoids(rng, s, n_obj, 0.012, 0.03)
+    dt = s["time_step"]
+    T = steps * dt
+    isrc, ans = [], []
+    for i, o in enumerate(objs):
+        if harmonic:
+            om = 2 * math.pi * rng.uniform(1e3, 4e3)
+            t0 = float(rng.uniform(0, 0.2)) * T
+            isrc.append(dict(object=i, kind=FDTD_VIB_HARMONIC, omega=om, phase=float(rng.uniform(0, 2 * math.pi)),
+                             amplitude=float(rng.uniform(0.5, 2.0)), decay_rate=float(rng.choice([-1.0, 1.5 / T])),
+                             decay_center=0.3 * T, t0=t0, t1=t0 + 0.6 * T))
+        if gaussian:
+            isrc.append(dict(object=i, amplitude=1e3, width=1e-4, onset=float(rng.uniform(0, 0.4)) * T))
+        if acc_noise:
+            m = float(rng.uniform(0.05, 0.5))
+            imps = []
+            for q in range(3):
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                imps.append(dict(timestamp=float(rng.uniform(0.05, 0.5)) * T, support=float(rng.uniform(1e-4, 4e-4)),
+                                 impulse=tuple(rng.normal(size=3) * 2e-3), position=tuple(np.asarray(o["semi_axes"]) * d)))
+            ans.append(dict(object=i, mass=m, center_of_mass=tuple(rng.normal(size=3) * 1e-3),
+                            inv_inertia=uniform_solid_inv_inertia(m, o["semi_axes"]), impacts=imps))
+    listeners = _random_listeners(rng, s, objs, n_listeners)
+    return SceneBuffers(s, objects=objs, impulse_sources=isrc, listeners=listeners, acc_noise_sources=ans), steps

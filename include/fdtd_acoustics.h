@@ -14,8 +14,14 @@
  *                           (src/wavesolver/FDTD_RigidObject.h:135-147); analytic
  *                           sphere / ellipsoid instead of a mesh + distance field
  *   fdtd_impulse_source  <- VibrationalSource::Evaluate
- *                           (src/wavesolver/VibrationalSource.h:17-37); new
- *                           Gaussian-impulse normal acceleration
+ *                           (src/wavesolver/VibrationalSource.h:17-37): a new
+ *                           Gaussian-impulse normal acceleration, and
+ *                           HarmonicVibrationalSource
+ *                           (src/wavesolver/HarmonicVibrationalSource.cpp:5-28)
+ *   fdtd_acc_noise_source <- AccelerationNoiseVibrationalSource
+ *                           (src/wavesolver/AccelerationNoiseVibrationalSource.cpp:
+ *                           19-90) with its ImpactRecords
+ *                           (src/modal_model/ImpulseSeriesObject.h:23-32)
  *   fdtd_pressure_source <- GaussianPressureSource
  *                           (src/wavesolver/GaussianPressureSource.h:31-39)
  *   fdtd_create          <- PML_WaveSolver(bbox, settings, objects)
@@ -144,16 +150,56 @@ typedef struct fdtd_object {
     double angular_velocity[3]; /* rad/s, world frame, about the centre   */
 } fdtd_object;
 
-/* Normal boundary acceleration a_n(t) = A exp(-(t - t_on)^2 / (2 w^2)) on the
- * whole surface of one object; the acceleration vector is a_n * outward
- * normal at the boundary point. */
+/* Kinds of fdtd_impulse_source. */
+enum { FDTD_VIB_GAUSSIAN = 0, FDTD_VIB_HARMONIC = 1 };
+
+/* Vibrational source on the whole surface of one object (VibrationalSource,
+ * VibrationalSource.h:17-37) whose acceleration vector at a boundary point is
+ * a_n(t) times the outward normal there:
+ *   FDTD_VIB_GAUSSIAN (new): a_n = A exp(-(t - t_on)^2 / (2 w^2));
+ *   FDTD_VIB_HARMONIC (HarmonicVibrationalSource, HarmonicVibrationalSource.
+ *   cpp:5-28, a pulsating surface): a_n = A omega sin(omega t + phase), times
+ *   exp(-decay_rate (t - decay_center)) when decay_rate >= 0, and zero
+ *   outside [t0, t1] when t1 > t0.  A is the amplitude of the pulsation
+ *   velocity (the reference fixes it at 1 m/s; XML has_harmonic_source with
+ *   harmonic_source_f = omega / 2 pi, phase 0, no decay, no range). */
 typedef struct fdtd_impulse_source {
     int    object;        /* index into fdtd_scene.objects               */
-    int    reserved;
-    double amplitude;     /* A (m/s^2)                                   */
-    double width;         /* w (s)                                       */
-    double onset;         /* t_on (s)                                    */
+    int    kind;          /* FDTD_VIB_GAUSSIAN | FDTD_VIB_HARMONIC       */
+    double amplitude;     /* Gaussian: A (m/s^2); harmonic: A (m/s)      */
+    double width;         /* Gaussian: w (s)                             */
+    double onset;         /* Gaussian: t_on (s)                          */
+    double omega;         /* harmonic: angular frequency (rad/s)         */
+    double phase;         /* harmonic: phase (rad)                       */
+    double decay_rate;    /* harmonic: < 0 none (1/s)                    */
+    double decay_center;  /* harmonic (s)                                */
+    double t0, t1;        /* harmonic: active range when t1 > t0 (s)     */
 } fdtd_impulse_source;
+
+/* One impact of an acceleration-noise source (ImpulseSeriesObject::
+ * ImpactRecord, ImpulseSeriesObject.h:23-32): impulse J applied at a point
+ * of the object, spread over a half-sine contact force of duration tau. */
+typedef struct fdtd_impact {
+    double timestamp;     /* t_0 (s)                                     */
+    double support;       /* tau (s); impacts with tau < 1e-12 add nothing */
+    double impulse[3];    /* J (N s), object frame                       */
+    double position[3];   /* impact point (m), object frame              */
+} fdtd_impact;
+
+/* Acceleration noise of a rigid object (AccelerationNoiseVibrationalSource,
+ * AccelerationNoiseVibrationalSource.cpp:63-90): one acceleration vector for
+ * the whole surface, a(t) = R sum over impacts with t_0 <= t < t_0 + tau of
+ *   pi S / (2 tau) (J / m + (I^-1 (r x J)) x r),   r = position - centre of mass,
+ * S = sin(pi (t - t_0) / tau), R the object's rotation (object to world);
+ * the boundary value at a point is a(t) . normal.  Static objects only. */
+typedef struct fdtd_acc_noise_source {
+    int                object;             /* index into fdtd_scene.objects */
+    int                n_impacts;
+    double             mass;               /* m (kg)                        */
+    double             center_of_mass[3];  /* object frame (m)              */
+    double             inv_inertia[9];     /* I^-1, object frame, row-major */
+    const fdtd_impact *impacts;            /* n_impacts (the caller owns them) */
+} fdtd_acc_noise_source;
 
 /* GaussianPressureSource: exp(-|x-x0|^2/(2 ws^2)) exp(-((t-t0)/wt)^2/2) K/3.6836,
  * zero outside the open box |x - x0|_inf < 2 ws. */
@@ -208,6 +254,8 @@ typedef struct fdtd_scene {
     const fdtd_speaker_source   *speaker_sources;
     int                          n_constraints;
     const fdtd_plane_constraint *constraints;
+    int                          n_acc_noise_sources;
+    const fdtd_acc_noise_source *acc_noise_sources;
 } fdtd_scene;
 
 /* Fills every field of a settings block with the parser defaults
@@ -313,7 +361,6 @@ typedef struct fdtd_batch_options {
     void *stream;            /* NULL: batch-owned stream                          */
     int   steps_per_graph;   /* hipGraph step block (0: 32)                       */
     int   tile_config;       /* -1: autotune once, cached per process             */
-    int   persistent_groups; /* 0: 2 * #CUs workgroups                            */
 } fdtd_batch_options;
 
 void fdtd_batch_options_default(fdtd_batch_options *o);

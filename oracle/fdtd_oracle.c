@@ -496,6 +496,7 @@ struct oracle {
     fdtd_scene sc;
     fdtd_object *objs;
     fdtd_impulse_source *isrc;
+    fdtd_acc_noise_source *ans;   /* acceleration-noise sources, impacts copied */
     fdtd_pressure_source *psrc;
     double *listeners;
     double src_bmin[64][3], src_bmax[64][3], src_pos[64][3];
@@ -829,22 +830,85 @@ static void classify_fast(oracle *o)
     hash_build(o);
 }
 
-/* boundary acceleration vector at a surface point: sum of the object's
- * Gaussian-impulse sources times the outward normal */
 static void update_evals(oracle *o);
 static void animate_objects(oracle *o, double t);
 
-static double object_an(const oracle *o, int obj, double t)
+/* HarmonicVibrationalSource::Evaluate (HarmonicVibrationalSource.cpp:5-28):
+ * v_bar omega sin(omega t + phase), decayed when a decay rate is set; zero
+ * outside the source's range (the reference's range is unbounded unless
+ * SetRange is called; here t1 <= t0 means unbounded) */
+static double harmonic_an(const fdtd_impulse_source *s, double t)
 {
+    if (s->t1 > s->t0 && (t < s->t0 || t > s->t1)) return 0.0;
+    const double v_bar = s->amplitude;
+    const double scale = v_bar * s->omega;
+    const double val = sin(s->omega * t + s->phase) * scale;
+    if (s->decay_rate < 0) return val;
+    return exp(-s->decay_rate * (t - s->decay_center)) * val;
+}
+
+/* AccelerationNoiseVibrationalSource::Evaluate(vertexID, time)
+ * (AccelerationNoiseVibrationalSource.cpp:63-90) over the impacts
+ * ImpulseSeriesObject::GetImpulseWithinSupport returns (t_0 <= t < t_0 + tau,
+ * ImpulseSeriesObject.cpp:88-101); ComputeS is the half sine (:19-29);
+ * ObjectToWorldVector applies the object's rotation */
+static void acc_noise(const fdtd_acc_noise_source *an, const fdtd_object *ob, double t, double *out)
+{
+    double acc[3] = {0, 0, 0};
+    for (int q = 0; q < an->n_impacts; ++q) {
+        const fdtd_impact *im = &an->impacts[q];
+        if (!(t >= im->timestamp && t < im->timestamp + im->support)) continue;
+        if (im->support < 1e-12) continue;   /* SMALL_NUM */
+        const double *J = im->impulse;
+        double r[3];
+        for (int d = 0; d < 3; ++d) r[d] = im->position[d] - an->center_of_mass[d];
+        double S = 0.0;
+        if (t <= im->timestamp + im->support && t >= im->timestamp) S = sin(M_PI * (t - im->timestamp) / im->support);
+        /* translational acceleration */
+        for (int d = 0; d < 3; ++d) acc[d] += J[d] * (M_PI * S / (2.0 * im->support * an->mass));
+        /* rotational acceleration: alpha = I^-1 (r x J) pi S / (2 tau), added as alpha x r */
+        const double rxJ[3] = {r[1] * J[2] - r[2] * J[1], r[2] * J[0] - r[0] * J[2], r[0] * J[1] - r[1] * J[0]};
+        double alpha[3];
+        for (int d = 0; d < 3; ++d) {
+            alpha[d] = an->inv_inertia[3 * d] * rxJ[0] + an->inv_inertia[3 * d + 1] * rxJ[1] + an->inv_inertia[3 * d + 2] * rxJ[2];
+            alpha[d] *= (M_PI * S) / (2.0 * im->support);
+        }
+        acc[0] += alpha[1] * r[2] - alpha[2] * r[1];
+        acc[1] += alpha[2] * r[0] - alpha[0] * r[2];
+        acc[2] += alpha[0] * r[1] - alpha[1] * r[0];
+    }
+    for (int d = 0; d < 3; ++d)
+        out[d] += ob->rotation[3 * d] * acc[0] + ob->rotation[3 * d + 1] * acc[1] + ob->rotation[3 * d + 2] * acc[2];
+}
+
+/* FDTD_RigidObject::EvaluateBoundaryAcceleration (FDTD_RigidObject.cpp:
+ * 331-368): the sum over the object's vibrational sources of their
+ * acceleration vectors at a surface point with outward normal nrm -- a_n nrm
+ * for the Gaussian-impulse and harmonic sources, the object's one vector for
+ * acceleration noise; zero once the evaluations are disabled */
+static void object_accel(const oracle *o, int obj, double t, const double *nrm, double *out)
+{
+    out[0] = out[1] = out[2] = 0.0;
+    if (o->evals_off) return;
     double a = 0.0;
-    if (o->evals_off) return 0.0;   /* FDTD_RigidObject::EvaluateBoundaryAcceleration, _disableEvals */
     for (int i = 0; i < o->sc.n_impulse_sources; ++i) {
         const fdtd_impulse_source *s = &o->isrc[i];
         if (s->object != obj) continue;
-        const double u = t - s->onset;
-        a += s->amplitude * exp(-(u * u) / (2.0 * s->width * s->width));
+        if (s->kind == FDTD_VIB_HARMONIC) {
+            a += harmonic_an(s, t);
+        } else {
+            const double u = t - s->onset;
+            a += s->amplitude * exp(-(u * u) / (2.0 * s->width * s->width));
+        }
     }
-    return a;
+    for (int d = 0; d < 3; ++d) out[d] = a * nrm[d];
+    for (int i = 0; i < o->sc.n_acc_noise_sources; ++i)
+        if (o->ans[i].object == obj) acc_noise(&o->ans[i], &o->objs[obj], t, out);
+}
+
+void oracle_object_accel(const oracle *o, int obj, double t, const double *nrm, double *out)
+{
+    object_accel(o, obj, t, nrm, out);
 }
 
 /* ScalarField::enclosingNeighbours (ScalarField.cpp:261-286): sorted unique */
@@ -981,9 +1045,9 @@ static void ghost_update(oracle *o, const double *p)
         const fdtd_object *obj = &o->objs[gc->object];
         double image[3], bp[3], en[3], nrm_b[3];
         const double distance = reflect_normal(obj, gc->position, image, bp, en, nrm_b);
-        const double an = object_an(o, gc->object, t);
         double grad_p[3];
-        for (int i = 0; i < 3; ++i) { grad_p[i] = 0.0; grad_p[i] += an * nrm_b[i]; grad_p[i] *= (-rho); }
+        object_accel(o, gc->object, t, nrm_b, grad_p);
+        for (int i = 0; i < 3; ++i) grad_p[i] *= (-rho);
         double cell_n[3] = {0, 0, 0};
         cell_n[abs(gc->topology) - 1] = gc->topology > 0 ? 1.0 : -1.0;
         const double grad_p_nr = dot3(grad_p, cell_n);
@@ -1472,6 +1536,14 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     memcpy(o->objs, scene->objects, sizeof(fdtd_object) * scene->n_objects);
     o->isrc = (fdtd_impulse_source *)malloc(sizeof(fdtd_impulse_source) * (scene->n_impulse_sources + 1));
     memcpy(o->isrc, scene->impulse_sources, sizeof(fdtd_impulse_source) * scene->n_impulse_sources);
+    o->ans = (fdtd_acc_noise_source *)calloc((size_t)scene->n_acc_noise_sources + 1, sizeof(fdtd_acc_noise_source));
+    for (int i = 0; i < scene->n_acc_noise_sources; ++i) {
+        o->ans[i] = scene->acc_noise_sources[i];
+        fdtd_impact *copy = (fdtd_impact *)malloc(sizeof(fdtd_impact) * ((size_t)o->ans[i].n_impacts + 1));
+        memcpy(copy, scene->acc_noise_sources[i].impacts, sizeof(fdtd_impact) * (size_t)o->ans[i].n_impacts);
+        o->ans[i].impacts = copy;
+    }
+    o->sc.acc_noise_sources = o->ans;
     o->psrc = (fdtd_pressure_source *)malloc(sizeof(fdtd_pressure_source) * (scene->n_pressure_sources + 1));
     memcpy(o->psrc, scene->pressure_sources, sizeof(fdtd_pressure_source) * scene->n_pressure_sources);
     o->listeners = (double *)malloc(sizeof(double) * 3 * (scene->n_listeners + 1));
@@ -1556,6 +1628,8 @@ void oracle_free(oracle *o)
     free(o->pp); free(o->pv); free(o->gh); free(o->hkeys); free(o->hvals);
     for (int i = 0; o->spk && i < o->sc.n_speaker_sources; ++i) free((void *)o->spk[i].samples);
     free(o->spk);
+    for (int i = 0; o->ans && i < o->sc.n_acc_noise_sources; ++i) free((void *)o->ans[i].impacts);
+    free(o->ans);
     free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
     free(o->sys_rp); free(o->sys_ci); free(o->sys_vv); free(o->sys_rhs);
@@ -1658,7 +1732,9 @@ static void fresh_fill(oracle *o, double *p)
             }
         } else {
             const double mls = oracle_mls_lookup(image, nv, pts, vals);
-            const double an = object_an(o, id, o->t) * (nb[0]*en[0] + nb[1]*en[1] + nb[2]*en[2]);
+            double acc[3];
+            object_accel(o, id, o->t, nb, acc);
+            const double an = acc[0]*en[0] + acc[1]*en[1] + acc[2]*en[2];
             p[ci] = mls + o->rho * an * fabs(travelled);
         }
     }

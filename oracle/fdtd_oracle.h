@@ -18,6 +18,9 @@
  *   src/wavesolver/FDTD_Objects.cpp      35-49, 224-257, 356-367
  *   src/wavesolver/FDTD_RigidObject.cpp  256-285, 330-368, 429-536 (semantics;
  *                                        analytic sphere/ellipsoid geometry)
+ *   src/wavesolver/HarmonicVibrationalSource.cpp          5-28
+ *   src/wavesolver/AccelerationNoiseVibrationalSource.cpp 19-90
+ *   src/modal_model/ImpulseSeriesObject.cpp               88-101
  *   src/math/MLSModeInterpolator.hpp     282-429, 588-597, 624-660, 712-738
  * and, for the staggered scheme:
  *   src/wavesolver/PML_WaveSolver.cpp    710-773 (stepLeapfrog)
@@ -71,6 +74,9 @@ int     oracle_svd_values(int m, int n, const double *A, double *sv);
 int     oracle_lsq_solve(int m, int n, const double *A, const double *b, double *x);
 /* Order-1 MLS value at p from npts points (x,y,z) and values. */
 double  oracle_mls_lookup(const double *p, int npts, const double *pts, const double *vals);
+/* Boundary acceleration vector of object obj at time t at a surface point
+ * with outward normal nrm (the sum over its vibrational sources). */
+void    oracle_object_accel(const oracle *o, int obj, double t, const double *nrm, double *out);
 /* Signed distance, closest boundary point and outward unit normal there. */
 double  oracle_object_closest(const fdtd_object *obj, const double *x, double *xb, double *n);
 /* ReflectAgainstBoundary semantics: image point, boundary point, erected

@@ -64,6 +64,8 @@ def lib():
         L.oracle_ghost_system.restype = C.c_int
         L.oracle_ghost_system.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), D, D]
+        L.oracle_object_accel.restype = None
+        L.oracle_object_accel.argtypes = [C.c_void_p, C.c_int, C.c_double, D, D]
         L.oracle_pml_sigma.restype = C.c_double
         L.oracle_pml_sigma.argtypes = [C.POINTER(Settings), D, C.c_int]
         L.oracle_inside_pml.restype = C.c_int
@@ -130,6 +132,14 @@ class Oracle:
         keys = ["n_cells", "n_pml_cells", "n_pml_faces", "n_ghost", "n_solid", "n_bulk", "n_cavity",
                 "n_coupled", "step"]
         return dict(zip(keys, list(c)))
+
+    def object_accel(self, obj, t, normal):
+        """Boundary acceleration vector of object obj at time t at a surface
+        point with outward normal `normal` (sum over its vibrational sources)."""
+        n = np.ascontiguousarray(normal, dtype=np.float64)
+        out = np.zeros(3)
+        self.L.oracle_object_accel(self.h, int(obj), float(t), _d(n), _d(out))
+        return out
 
     def time(self):
         return self.L.oracle_time(self.h)
