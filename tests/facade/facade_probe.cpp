@@ -1,5 +1,7 @@
 // Drives the C++ facade: builds a small free-field scene, steps it and prints
-// the last listener value (GPU), or reports the C-ABI's error (no GPU).
+// the last listener value (GPU), or reports the C-ABI's error (no GPU).  With
+// an XML path it runs the simulator on the scene's own box, and with a second
+// argument also on a box and settings given to InitializeSolver.
 #include <cstdio>
 #include <cmath>
 #include "../../acoustics_amd/facade/wavesolver.hpp"
@@ -8,11 +10,25 @@ int main(int argc, char **argv)
 {
     try {
         if (argc > 1) {  // XML path
-            FDTD_AcousticSimulator sim(argv[1]);
+            FDTD_AcousticSimulator sim(argv[1], new std::string("probe"));
             sim.InitializeSolver();
             sim.Run();
             std::vector<double> series = sim.ListenerSeries();
             printf("xml steps=%d last=%.17g t=%.9g\n", sim.NumberOfTimesteps(), series.empty() ? 0.0 : series.back(), sim.GetSimulationTime());
+            const ScalarField &pf = sim.GetGrid().pressureField();
+            printf("xml grid cells=%d solver cells=%d name=%s\n", pf.numCells(), sim.GetSolver()->numCells(),
+                   sim.CompositeFilename("all_audio.dat", "out/%s").c_str());
+            if (argc > 2) {
+                FDTD_AcousticSimulator boxed(argv[1]);
+                auto st = std::make_shared<PML_WaveSolver_Settings>(*sim.GetSolverSettings());
+                st->PML_width = 5;
+                boxed.InitializeSolver(BoundingBox(st->cellSize, 36, Vector3d(0.0, 0.0, 0.0)), st);
+                boxed.RunForSteps(50);
+                Matrix m;
+                boxed.GetSolver()->FetchPressureData(st->listeningPoints, m);
+                printf("boxed cells=%d pml=%lld rows=%d t=%.9g\n", boxed.GetGrid().numPressureCells(), boxed.GetGrid().numPMLCells(),
+                       m.rows(), boxed.GetSimulationTime());
+            }
             return 0;
         }
         auto settings = std::make_shared<PML_WaveSolver_Settings>();
@@ -26,11 +42,21 @@ int main(int argc, char **argv)
         objects->AddPressureSource(Vector3d(0.001, 0.002, -0.003), 2e-4, ws, 8e-4, 1.0 / std::pow(std::sqrt(2 * M_PI) * ws, 3));
         PML_WaveSolver solver(BoundingBox(0.005, 32, Vector3d(0, 0, 0)), settings, objects);
         for (int i = 0; i < 100; ++i) solver.stepSystem();
-        std::vector<double> out;
+        Matrix out;
         solver.FetchPressureData(settings->listeningPoints, out);
         Tuple3i d = solver.GetGrid().pressureFieldDivisions();
         printf("facade cells=%d divs=%d,%d,%d pml=%lld p=%.17g t=%.9g\n", solver.GetGrid().numPressureCells(), d.x, d.y, d.z,
-               solver.GetGrid().numPMLCells(), out[0], solver.currentSimTime());
+               solver.GetGrid().numPMLCells(), out(0, 0), solver.currentSimTime());
+        const ScalarField &v0 = solver.GetGrid().velocityField(0);
+        const Vector3d c0 = v0.cellPosition(Tuple3i(0, 0, 0));
+        printf("velocity0 divs=%d,%d,%d first=%.6f,%.6f,%.6f\n", v0.cellDivisions().x, v0.cellDivisions().y, v0.cellDivisions().z,
+               c0.x, c0.y, c0.z);
+        try {
+            solver.FetchPressureData(settings->listeningPoints, out, 0);
+            printf("dim 0 accepted\n");
+        } catch (const std::runtime_error &) {
+            printf("dim 0 refused\n");
+        }
         return 0;
     } catch (const std::runtime_error &e) {
         printf("runtime_error: %s\n", e.what());

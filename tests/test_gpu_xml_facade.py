@@ -52,9 +52,14 @@ def test_facade_drives_the_engine(hip_lib, tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0 and "facade cells=32768 divs=32,32,32" in r.stdout
-    r2 = subprocess.run([str(exe), XML], capture_output=True, text=True, timeout=120)
+    # velocity field 0: one more layer along x, starting h/2 below the box
+    assert "velocity0 divs=33,32,32 first=-0.080000,-0.077500,-0.077500" in r.stdout
+    assert "dim 0 refused" in r.stdout
+    r2 = subprocess.run([str(exe), XML, "boxed"], capture_output=True, text=True, timeout=120)
     print(r2.stdout)
     assert r2.returncode == 0 and "xml steps=300" in r2.stdout
+    assert "name=out/probe_all_audio.dat" in r2.stdout
+    assert "boxed cells=46656" in r2.stdout and "rows=" in r2.stdout
     last = float(r2.stdout.split("last=")[1].split()[0])
     ps = _ParsedScene(hip_lib, XML)
     ref = Oracle(ps, reclassify_each_step=False).step(ps.steps)
