@@ -90,6 +90,11 @@ class PlaneConstraint(C.Structure):
     _fields_ = [("direction", C.c_int), ("sign", C.c_int), ("height", C.c_double)]
 
 
+class Kinematics(C.Structure):
+    _fields_ = [("object", C.c_int), ("n_frames", C.c_int), ("step_size", C.c_double),
+                ("frames", C.POINTER(C.c_double))]
+
+
 class Scene(C.Structure):
     _fields_ = [
         ("settings", Settings),
@@ -100,6 +105,7 @@ class Scene(C.Structure):
         ("n_speaker_sources", C.c_int), ("speaker_sources", C.POINTER(SpeakerSource)),
         ("n_constraints", C.c_int), ("constraints", C.POINTER(PlaneConstraint)),
         ("n_acc_noise_sources", C.c_int), ("acc_noise_sources", C.POINTER(AccNoiseSource)),
+        ("n_kinematics", C.c_int), ("kinematics", C.POINTER(Kinematics)),
     ]
 
 
@@ -234,7 +240,7 @@ class SceneBuffers:
     """Owns the ctypes arrays a Scene points at (keeps them alive)."""
 
     def __init__(self, settings: dict, objects=(), impulse_sources=(), pressure_sources=(), listeners=None,
-                 speaker_sources=(), constraints=(), acc_noise_sources=()):
+                 speaker_sources=(), constraints=(), acc_noise_sources=(), kinematics=()):
         self.settings_dict = dict(settings)
         s = Settings()
         for k, v in settings.items():
@@ -349,6 +355,18 @@ class SceneBuffers:
             self._ans[i].impacts = C.cast(arr, C.POINTER(Impact))
         self.scene.n_acc_noise_sources = len(self.acc_noise_list)
         self.scene.acc_noise_sources = C.cast(self._ans, C.POINTER(AccNoiseSource))
+        # keyframe animations: dict(object, step_size, frames=(n, 7) tx ty tz qw qx qy qz)
+        self.kinematics_list = list(kinematics)
+        self._kin_frames = [np.ascontiguousarray(np.asarray(k["frames"], dtype=np.float64).reshape(-1, 7))
+                            for k in self.kinematics_list]
+        self._kin = (Kinematics * max(1, len(self.kinematics_list)))()
+        for i, k in enumerate(self.kinematics_list):
+            self._kin[i].object = k["object"]
+            self._kin[i].n_frames = self._kin_frames[i].shape[0]
+            self._kin[i].step_size = k["step_size"]
+            self._kin[i].frames = self._kin_frames[i].ctypes.data_as(C.POINTER(C.c_double))
+        self.scene.n_kinematics = len(self.kinematics_list)
+        self.scene.kinematics = C.cast(self._kin, C.POINTER(Kinematics))
 
     @property
     def n_listeners(self):

@@ -54,6 +54,7 @@ struct Layout {
     float *spk_field;
     uint8_t *ucls;
     MoveDev *mobj;
+    double *kin;
     Geom *pose;
     double *ughost;
     SceneDev *self;
@@ -132,6 +133,7 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
     L.spk_field = c.take<float>(S.spk.empty() ? 0 : (size_t)(S.spk_hi[0] - S.spk_lo[0]) * (S.spk_hi[1] - S.spk_lo[1]) * (S.spk_hi[2] - S.spk_lo[2]));
     L.ucls = c.take<uint8_t>(2 * moving_cells(S));
     L.mobj = c.take<MoveDev>(S.moving ? S.objs.size() : 0);
+    L.kin = c.take<double>(S.kin.size());
     L.pose = c.take<Geom>(S.moving ? S.objs.size() : 0);
     L.ughost = c.take<double>(6 * moving_cells(S));
     L.self = c.take<SceneDev>(1);
@@ -257,8 +259,12 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
         for (size_t q = 0; q < S.objs.size(); ++q) {
             mv[q].g0 = S.objs[q];
             for (int d = 0; d < 3; ++d) { mv[q].v[d] = S.mv_v[3 * q + d]; mv[q].w[d] = S.mv_w[3 * q + d]; }
+            mv[q].kin_off = S.kin_off[q];
+            mv[q].kin_n = S.kin_n[q];
+            mv[q].kin_dt = S.kin_dt[q];
         }
         FDTD_HIP_TRY(up(L.mobj, mv.data(), mv.size() * sizeof(MoveDev)));
+        FDTD_HIP_TRY(up(L.kin, S.kin.data(), S.kin.size() * sizeof(double)));
         // the flags of step -1 (row 1): the classification at the start pose
         FDTD_HIP_TRY(up(L.ucls + moving_cells(S), S.mv_cls0.data(), S.mv_cls0.size()));
     }
@@ -363,6 +369,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     }
     D.ucls = L.ucls;
     D.mobj = L.mobj;
+    D.kin = L.kin;
     D.pose = L.pose;
     D.ughost = L.ughost;
     D.h = S.h;

@@ -181,6 +181,12 @@ struct SceneSetup {
     bool moving = false;
     int mv_lo[3] = {0, 0, 0}, mv_hi[3] = {0, 0, 0};
     std::vector<double> mv_v, mv_w;     // 3 per object
+    // keyframe animation (kinematics files): per object the offset and count
+    // of its frames in kin (0 frames: constant motion) and the frame step;
+    // kin holds 8 doubles per frame, centre and corrected rotation (w x y z)
+    std::vector<double> kin;
+    std::vector<int32_t> kin_off, kin_n;
+    std::vector<double> kin_dt;
     int64_t mv_steps = 0;               // steps the planned path covers (start to stop_time)
     std::vector<uint8_t> mv_cls0;
     std::vector<uint8_t> bulk;          // full grid (host only)

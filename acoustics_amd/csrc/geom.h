@@ -208,4 +208,83 @@ FDTD_HD inline Geom geom_at(const Geom &g0, const double *v, const double *w, do
     return g;
 }
 
+// Quaternion (w, x, y, z) scaled to unit length unless its squared length
+// is within 1e-12 of one (Quaternion::normalize, Quaternion.hpp:186-195).
+FDTD_HD inline void quat_normalize(double *q)
+{
+    const double len = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (fabs(len - 1.0) > 1e-12) {
+        const double s = 1.0 / sqrt(len);
+        for (int i = 0; i < 4; ++i) q[i] *= s;
+    }
+}
+
+// Quaternion::slerp (Quaternion.hpp:365-420, its active branch): a normalized
+// lerp when the two are within acos(0.9995) (either sign), else the great-arc
+// blend along the shorter way.
+FDTD_HD inline void quat_slerp(const double *a, const double *b, double r, double *out)
+{
+    double v0[4] = {a[0], a[1], a[2], a[3]}, v1[4] = {b[0], b[1], b[2], b[3]};
+    quat_normalize(v0);
+    quat_normalize(v1);
+    double dot = v0[0] * v1[0] + v0[1] * v1[1] + v0[2] * v1[2] + v0[3] * v1[3];
+    if (fabs(dot) > 0.9995) {
+        for (int i = 0; i < 4; ++i) out[i] = v0[i] + (v1[i] - v0[i]) * r;
+        quat_normalize(out);
+        return;
+    }
+    if (dot < 0.0) {
+        for (int i = 0; i < 4; ++i) v1[i] = -v1[i];
+        dot = -dot;
+    }
+    dot = fmin(fmax(dot, -1.0), 1.0);
+    const double th = acos(dot) * r;
+    double v2[4];
+    for (int i = 0; i < 4; ++i) v2[i] = v1[i] - v0[i] * dot;
+    quat_normalize(v2);
+    const double c = cos(th), s = sin(th);
+    for (int i = 0; i < 4; ++i) out[i] = v0[i] * c + v2[i] * s;
+}
+
+// Pose at time t of an object animated by keyframes (8 doubles per frame:
+// centre x y z, rotation w x y z, unused), frame f at time f * step:
+// FDTD_RigidObject_Blender_Animator::GetRigidObjectTransform
+// (FDTD_RigidObject_Animator.cpp:137-160), FDTD_Objects::AnimateObjects
+// (FDTD_Objects.cpp:563-579: the blended quaternion normalized) and
+// FDTD_RigidObject::SetRigidBodyTransform (FDTD_RigidObject.cpp:550-566: the
+// rotation of angle 2 acos(w) about the normalized vector part, the centre
+// moved to the blended one).  w is clamped to [-1, 1] before acos.
+FDTD_HD inline Geom geom_keyframe(const Geom &shape, const double *frames, int n, double step, double t)
+{
+    const double pos = t / step;
+    int f0 = pos > 0.0 ? (int)pos : 0;   // std::max<int>(0, pos) truncates toward zero
+    f0 = f0 < n - 1 ? f0 : n - 1;
+    const int f1 = f0 + 1 < n - 1 ? f0 + 1 : n - 1;
+    const double t0 = f0 * step, t1 = f1 * step;
+    const double alpha = f0 == f1 ? 1.0 : (t - t0) / (t1 - t0);
+    const double *A = frames + 8 * f0, *B = frames + 8 * f1;
+    Geom g = shape;
+    for (int d = 0; d < 3; ++d) g.c[d] = A[d] * (1.0 - alpha) + B[d] * alpha;
+    double q[4];
+    quat_slerp(A + 3, B + 3, alpha, q);
+    quat_normalize(q);
+    const double w = fmin(fmax(q[0], -1.0), 1.0);
+    const double th = 2.0 * acos(w);
+    double k[3] = {0.0, 1.0, 0.0};
+    if (!(th < 1e-12)) {
+        const double vn = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        if (vn > 0.0)
+            for (int d = 0; d < 3; ++d) k[d] = q[1 + d] / vn;
+    }
+    const double s = sin(th), cm = 1.0 - cos(th);
+    const double K[9] = {0, -k[2], k[1], k[2], 0, -k[0], -k[1], k[0], 0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double k2 = 0;
+            for (int m = 0; m < 3; ++m) k2 += K[r * 3 + m] * K[m * 3 + c];
+            g.R[r * 3 + c] = (r == c ? 1.0 : 0.0) + s * K[r * 3 + c] + cm * k2;
+        }
+    return g;
+}
+
 }  // namespace fdtd

@@ -63,9 +63,14 @@ enum : int { AMP_ROWS = 64 };
 __host__ __device__ inline int amp_row(long long s) { return (int)(s & (AMP_ROWS - 1)); }
 
 // a moving object: its pose at t = 0 and its constant velocity and angular velocity
+// Motion of one object: constant velocity v and angular velocity w from
+// the pose g0 at t = 0, or (kin_n > 0) keyframes kin_off .. kin_off + kin_n
+// of SceneDev::kin, kin_dt apart, with g0 giving the shape
 struct MoveDev {
     Geom g0;
     double v[3], w[3];
+    int kin_off, kin_n;
+    double kin_dt;
 };
 
 struct SceneDev {
@@ -132,6 +137,7 @@ struct SceneDev {
     int u_lo[3], u_n[3];
     uint8_t *ucls;              // [2][u_n0 u_n1 u_n2], z fastest: 1 bulk, 0 solid
     const MoveDev *mobj;        // [n_obj]
+    const double *kin;          // keyframes, 8 doubles each
     Geom *pose;                 // [n_obj], written by motion_classify_kernel each step
     double *ughost;             // [6][u cells]: ghost term of each solid neighbour of a fluid cell (-x, +x, ..., +z)
     double h, rho, bmin[3];

@@ -1136,7 +1136,8 @@ __global__ __launch_bounds__(256) void motion_classify_kernel(const SceneDev *__
     __shared__ Geom pose[MAX_MOVING_OBJ];
     const double t = S.t_start + (double)step * S.dt;
     for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) {
-        const Geom g = geom_at(S.mobj[o].g0, S.mobj[o].v, S.mobj[o].w, t);
+        const MoveDev &m = S.mobj[o];
+        const Geom g = m.kin_n > 0 ? geom_keyframe(m.g0, S.kin + m.kin_off, m.kin_n, m.kin_dt, t) : geom_at(m.g0, m.v, m.w, t);
         pose[o] = g;
         if (blockIdx.x == 0) S.pose[o] = g;
     }

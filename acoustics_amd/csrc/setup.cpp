@@ -690,6 +690,49 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     for (int q = 0; q < sc.n_objects; ++q)
         for (int d = 0; d < 3; ++d)
             S.moving = S.moving || sc.objects[q].velocity[d] != 0.0 || sc.objects[q].angular_velocity[d] != 0.0;
+    // keyframe animation (FDTD_RigidObject_Blender_Animator::ReadObjectKinData,
+    // FDTD_RigidObject_Animator.cpp:78-131): qw clamped to [-1, 1], the
+    // rotation turned from Blender's z-up by q <- q * Rot(x, -90 degrees)
+    S.kin.clear();
+    S.kin_off.assign(sc.n_objects, 0);
+    S.kin_n.assign(sc.n_objects, 0);
+    S.kin_dt.assign(sc.n_objects, 0.0);
+    if (sc.n_kinematics < 0 || (sc.n_kinematics > 0 && !sc.kinematics)) { err = "bad kinematics list"; return FDTD_ERR_INVALID; }
+    for (int q = 0; q < sc.n_kinematics; ++q) {
+        const fdtd_kinematics &km = sc.kinematics[q];
+        if (km.object < 0 || km.object >= sc.n_objects) { err = "kinematics names a missing object"; return FDTD_ERR_INVALID; }
+        if (km.n_frames < 1 || !km.frames || !(km.step_size > 0.0)) {
+            err = "kinematics need at least one frame and a positive frame step";
+            return FDTD_ERR_INVALID;
+        }
+        if (S.kin_n[km.object] > 0) { err = "an object has two kinematics"; return FDTD_ERR_INVALID; }
+        for (int d = 0; d < 3; ++d)
+            if (sc.objects[km.object].velocity[d] != 0.0 || sc.objects[km.object].angular_velocity[d] != 0.0) {
+                err = "an object animated by kinematics has no velocity or angular velocity of its own";
+                return FDTD_ERR_INVALID;
+            }
+        const double half = -90.0 * M_PI / 180.0 / 2.0;
+        const double cw = std::cos(half), cx = std::sin(half);   // corr = (cw, cx, 0, 0)
+        S.kin_off[km.object] = (int32_t)S.kin.size();
+        S.kin_n[km.object] = km.n_frames;
+        S.kin_dt[km.object] = km.step_size;
+        for (int f = 0; f < km.n_frames; ++f) {
+            const double *r = km.frames + 7 * (size_t)f;
+            for (int e = 0; e < 7; ++e)
+                if (!std::isfinite(r[e])) { err = "kinematics frames must be finite"; return FDTD_ERR_INVALID; }
+            const double w = std::max(-1.0, std::min(1.0, r[3])), x = r[4], y = r[5], z = r[6];
+            // Hamilton product (w, x, y, z) * (cw, cx, 0, 0) (Quaternion.hpp:82-89)
+            const double qw = w * cw - x * cx, qx = w * cx + x * cw, qy = y * cw + z * cx, qz = z * cw - y * cx;
+            const double row[8] = {r[0], r[1], r[2], qw, qx, qy, qz, 0.0};
+            S.kin.insert(S.kin.end(), row, row + 8);
+        }
+        S.moving = true;
+    }
+    // pose of object q at time t: its keyframes, or its constant motion
+    auto pose_at = [&](int q, double t) {
+        if (S.kin_n[q] > 0) return geom_keyframe(S.objs[q], &S.kin[S.kin_off[q]], S.kin_n[q], S.kin_dt[q], t);
+        return geom_at(S.objs[q], &S.mv_v[3 * q], &S.mv_w[3 * q], t);
+    };
     if (S.moving) {
         if (st.scheme != FDTD_SCHEME_COLLOCATED || S.boundary != FDTD_BOUNDARY_RASTERIZE || S.visc) {
             err = "moving objects run with the collocated scheme, rasterized ghost cells and no air viscosity";
@@ -709,16 +752,26 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         // steps whose start time t_start + s dt lies within [start, stop]
         S.mv_steps = (int64_t)std::floor((st.stop_time - st.start_time) / S.dt) + 1;
         // swept box: the centre's path from start to stop, widened by the
-        // largest semi-axis (the shape turns about its centre)
+        // largest semi-axis (the shape turns about its centre); a keyframed
+        // centre moves piecewise linearly, so its positions at every step
+        // and at every frame inside the run bound it
         double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        auto widen = [&](const double *c, double r) {
+            for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], c[d] - r); hi[d] = std::max(hi[d], c[d] + r); }
+        };
         for (int q = 0; q < sc.n_objects; ++q) {
             const Geom &g = S.objs[q];
             const double r = std::max(g.a[0], std::max(g.a[1], g.a[2]));
-            for (int d = 0; d < 3; ++d) {
-                const double c0 = g.c[d] + S.mv_v[3 * q + d] * st.start_time, c1 = g.c[d] + S.mv_v[3 * q + d] * st.stop_time;
-                lo[d] = std::min(lo[d], std::min(c0, c1) - r);
-                hi[d] = std::max(hi[d], std::max(c0, c1) + r);
+            if (S.kin_n[q] > 0) {
+                for (int64_t s = 0; s < S.mv_steps; ++s) widen(pose_at(q, st.start_time + (double)s * S.dt).c, r);
+                for (int f = 0; f < S.kin_n[q]; ++f)
+                    if (f * S.kin_dt[q] >= st.start_time && f * S.kin_dt[q] <= st.stop_time) widen(&S.kin[S.kin_off[q] + 8 * f], r);
+                continue;
             }
+            double c0[3], c1[3];
+            for (int d = 0; d < 3; ++d) { c0[d] = g.c[d] + S.mv_v[3 * q + d] * st.start_time; c1[d] = g.c[d] + S.mv_v[3 * q + d] * st.stop_time; }
+            widen(c0, r);
+            widen(c1, r);
         }
         const int edge = S.abc ? 3 : 0;
         for (int d = 0; d < 3; ++d) {
@@ -732,7 +785,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         }
         // bulk flags at the start pose (7-sample rule, as the device reclassifies)
         std::vector<Geom> g0(sc.n_objects);
-        for (int q = 0; q < sc.n_objects; ++q) g0[q] = geom_at(S.objs[q], &S.mv_v[3 * q], &S.mv_w[3 * q], st.start_time);
+        for (int q = 0; q < sc.n_objects; ++q) g0[q] = pose_at(q, st.start_time);
         const double q4 = 0.25 * h;
         for (int i = S.mv_lo[0]; i < S.mv_hi[0]; ++i)
             for (int j = S.mv_lo[1]; j < S.mv_hi[1]; ++j)

@@ -8,7 +8,8 @@
 // each with any number of <gaussian_impulse_source amplitude= width= onset=/>,
 // the reference's has_harmonic_source= harmonic_source_f= attributes, and
 // <acc_noise_source mass= [inv_inertia= center_of_mass_x..z=]> holding
-// <impact timestamp= support= impulse_x..z= position_x..z=/> elements.
+// <impact timestamp= support= impulse_x..z= position_x..z=/> elements, and the
+// reference's file_kinematics= kinematics_frame_rate= keyframe animation.
 // A small attribute-only XML reader replaces the vendored tinyxml.
 #include <cmath>
 #include <cstdio>
@@ -163,6 +164,8 @@ struct fdtd_parsed {
     fdtd_scene scene;
     std::vector<std::unique_ptr<fdtd::AccNoise>> acc_noise;
     std::vector<fdtd_acc_noise_source> acc_noise_src;
+    std::vector<std::vector<double>> kin_frames;
+    std::vector<fdtd_kinematics> kinematics;
     std::vector<fdtd_object> objects;
     std::vector<fdtd_impulse_source> impulses;
     std::vector<fdtd_pressure_source> psrc;
@@ -390,6 +393,40 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
                 ob.center[2] = req_real(o, "center_z");
                 const int oi = (int)P->objects.size();
                 P->objects.push_back(ob);
+                // ImpulseResponseParser.cpp:248-263: a kinematics file of rows
+                // "frame tx ty tz qw qx qy qz" at kinematics_frame_rate frames
+                // per second (blank lines are skipped; the reference's reader
+                // would add a zero frame for one)
+                const std::string kfile = opt_str(o, "file_kinematics", "N/A");
+                if (kfile != "N/A") {
+                    const double rate = req_real(o, "kinematics_frame_rate");
+                    if (!(rate > 0)) throw ParseError{"kinematics_frame_rate must be positive"};
+                    std::string where = kfile;
+                    if (!std::ifstream(where).good() && kfile[0] != '/') {
+                        const std::string xml(path);
+                        const size_t cut = xml.find_last_of('/');
+                        if (cut != std::string::npos) where = xml.substr(0, cut + 1) + kfile;
+                    }
+                    std::ifstream kin(where);
+                    if (!kin) throw ParseError{"cannot read object kinematics file " + kfile};
+                    std::vector<double> fr;
+                    std::string line;
+                    while (std::getline(kin, line)) {
+                        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+                        std::istringstream in(line);
+                        double v[8];
+                        for (double &x : v)
+                            if (!(in >> x)) throw ParseError{"kinematics file " + kfile + ": a row needs frame tx ty tz qw qx qy qz"};
+                        fr.insert(fr.end(), v + 1, v + 8);
+                    }
+                    if (fr.empty()) throw ParseError{"kinematics file " + kfile + " holds no frame"};
+                    fdtd_kinematics k{};
+                    k.object = oi;
+                    k.n_frames = (int)(fr.size() / 7);
+                    k.step_size = 1.0 / rate;
+                    P->kin_frames.push_back(std::move(fr));
+                    P->kinematics.push_back(k);
+                }
                 for (const Node *g : o->children("gaussian_impulse_source")) {
                     fdtd_impulse_source is{};
                     is.object = oi;
@@ -458,6 +495,9 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
     }
     sc.n_acc_noise_sources = (int)P->acc_noise_src.size();
     sc.acc_noise_sources = P->acc_noise_src.data();
+    for (size_t q = 0; q < P->kinematics.size(); ++q) P->kinematics[q].frames = P->kin_frames[q].data();
+    sc.n_kinematics = (int)P->kinematics.size();
+    sc.kinematics = P->kinematics.data();
     sc.n_pressure_sources = (int)P->psrc.size();
     sc.pressure_sources = P->psrc.data();
     sc.n_listeners = (int)(P->listeners.size() / 3);

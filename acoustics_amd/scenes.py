@@ -424,3 +424,121 @@ def vibration_scene(seed=17, n=48, h=0.005, pml=8, steps=300, n_obj=2, n_listene
                             inv_inertia=uniform_solid_inv_inertia(m, o["semi_axes"]), impacts=imps))
     listeners = _random_listeners(rng, s, objs, n_listeners)
     return SceneBuffers(s, objects=objs, impulse_sources=isrc, listeners=listeners, acc_noise_sources=ans), steps
+
+
+def quat_mul(a, b):
+    """Hamilton product of quaternions (w, x, y, z)."""
+    aw, ax, ay, az = a
+    bw, bx, by, bz = b
+    return np.array([aw * bw - ax * bx - ay * by - az * bz, aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by + ay * bw + az * bx - ax * bz, aw * bz + az * bw + ax * by - ay * bx])
+
+
+def quat_from_matrix(R):
+    """Unit quaternion (w, x, y, z) of a rotation matrix (Shepperd's method)."""
+    R = np.asarray(R, dtype=np.float64)
+    tr = np.trace(R)
+    if tr > 0:
+        s = 2.0 * math.sqrt(1.0 + tr)
+        q = [0.25 * s, (R[2, 1] - R[1, 2]) / s, (R[0, 2] - R[2, 0]) / s, (R[1, 0] - R[0, 1]) / s]
+    else:
+        i = int(np.argmax(np.diag(R)))
+        j, k = (i + 1) % 3, (i + 2) % 3
+        s = 2.0 * math.sqrt(1.0 + R[i, i] - R[j, j] - R[k, k])
+        q = [0.0] * 4
+        q[0] = (R[k, j] - R[j, k]) / s
+        q[1 + i] = 0.25 * s
+        q[1 + j] = (R[j, i] + R[i, j]) / s
+        q[1 + k] = (R[k, i] + R[i, k]) / s
+    q = np.array(q)
+    return q / np.linalg.norm(q)
+
+
+# a kinematics file's rotation is Blender's z-up one: the reader turns it by
+# -90 degrees about x (q <- q * corr); a pose q is written as q * corr^-1
+BLENDER_CORR = np.array([math.cos(-math.pi / 4), math.sin(-math.pi / 4), 0.0, 0.0])
+
+
+def file_frame(center, R):
+    """One kinematics-file row (tx ty tz qw qx qy qz) that the reader turns
+    into the pose (center, R)."""
+    inv = BLENDER_CORR * np.array([1.0, -1.0, -1.0, -1.0])
+    q = quat_mul(quat_from_matrix(R), inv)
+    return [*center, *q]
+
+
+def rodrigues(axis, angle):
+    k = np.asarray(axis, dtype=np.float64)
+    k = k / np.linalg.norm(k)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+def as_keyframes(sb, frame_rate):
+    """The same scene with every moving object's constant motion written as
+    kinematics-file frames (frame f at time f / frame_rate, to one frame past
+    stop_time) and its own velocities zeroed.  Linear blending of a constant
+    velocity and slerp of a constant-rate turn reproduce the motion."""
+    s = dict(sb.settings_dict)
+    step = 1.0 / frame_rate
+    nf = int(math.ceil(s["stop_time"] / step)) + 2
+    objs, kins = [], []
+    for i, o in enumerate(sb.objects_list):
+        o = dict(o)
+        v = np.asarray(o.get("velocity", (0.0, 0.0, 0.0)), dtype=np.float64)
+        w = np.asarray(o.get("angular_velocity", (0.0, 0.0, 0.0)), dtype=np.float64)
+        if np.any(v != 0) or np.any(w != 0):
+            R0 = np.asarray(o.get("rotation", np.eye(3)), dtype=np.float64).reshape(3, 3)
+            frames, prev = [], None
+            for f in range(nf):
+                t = f * step
+                c = np.asarray(o["center"]) + v * t
+                R = (rodrigues(w, np.linalg.norm(w) * t) if np.any(w != 0) else np.eye(3)) @ R0
+                row = np.array(file_frame(c, R))
+                if prev is not None and np.dot(row[3:], prev[3:]) < 0:
+                    row[3:] = -row[3:]          # the same turn, on the side nearer the last frame
+                frames.append(row)
+                prev = row
+            kins.append(dict(object=i, step_size=step, frames=np.array(frames)))
+            o["velocity"] = (0.0, 0.0, 0.0)
+            o["angular_velocity"] = (0.0, 0.0, 0.0)
+        objs.append(o)
+    return SceneBuffers(s, objects=objs, impulse_sources=sb.impulse_list, pressure_sources=sb.pressure_list,
+                        listeners=sb.listeners_np, kinematics=kins)
+
+
+def keyframed_object(seed=22, n=56, h=0.005, pml=8, steps=360, frame_rate=4000.0, n_listeners=8):
+    """An ellipsoid on a curved, accelerating path with a varying turn, given
+    as kinematics-file frames (a swing: centre on an arc, rotation about a
+    wobbling axis), with a Gaussian impulse; listeners outside the swept box."""
+    rng = np.random.default_rng(seed)
+    s = base_settings((n, n, n), h, pml)
+    dt = s["time_step"]
+    T = steps * dt
+    s["stop_time"] = s["start_time"] + T
+    axes = (0.02, 0.011, 0.015)
+    R0 = random_rotation(rng)
+    step = 1.0 / frame_rate
+    nf = int(math.ceil(T / step)) + 2
+    rad, om = 0.02, 2.5 / T
+    frames, prev = [], None
+    for f in range(nf):
+        t = f * step
+        c = np.array([rad * (math.cos(om * t) - 0.1), 0.4 * rad * math.sin(2 * om * t), 0.2 * rad * t / T])
+        ax = np.array([math.sin(3 * om * t), 1.0, 0.3 * math.cos(om * t)])
+        R = rodrigues(ax, 1.2 * om * t + 0.8 * (om * t) ** 2) @ R0
+        row = np.array(file_frame(c, R))
+        if prev is not None and np.dot(row[3:], prev[3:]) < 0:
+            row[3:] = -row[3:]
+        frames.append(row)
+        prev = row
+    obj = dict(type=FDTD_OBJ_ELLIPSOID, center=(0.0, 0.0, 0.0), semi_axes=axes, rotation=np.eye(3))
+    src = dict(object=0, amplitude=1e3, width=1e-4, onset=3e-4)
+    fr = np.array(frames)
+    lo, hi = fr[:, :3].min(0), fr[:, :3].max(0)
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo) + max(axes) + 4.5 * h
+    pts = fibonacci_shell(mid, float(np.max(half)) + 1.5 * h, 16 * n_listeners)
+    keep = [p for p in pts if np.any(np.abs(p - mid) > half)]
+    listeners = np.array(keep[::max(1, len(keep) // n_listeners)][:n_listeners])
+    return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners,
+                        kinematics=[dict(object=0, step_size=step, frames=fr)]), steps

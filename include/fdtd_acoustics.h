@@ -22,6 +22,8 @@
  *                           (src/wavesolver/AccelerationNoiseVibrationalSource.cpp:
  *                           19-90) with its ImpactRecords
  *                           (src/modal_model/ImpulseSeriesObject.h:23-32)
+ *   fdtd_kinematics      <- FDTD_RigidObject_Blender_Animator (kinematics files,
+ *                           src/wavesolver/FDTD_RigidObject_Animator.cpp:78-160)
  *   fdtd_pressure_source <- GaussianPressureSource
  *                           (src/wavesolver/GaussianPressureSource.h:31-39)
  *   fdtd_create          <- PML_WaveSolver(bbox, settings, objects)
@@ -240,6 +242,24 @@ typedef struct fdtd_plane_constraint {
     double height;      /* position of the plane along that axis (m), XML 0  */
 } fdtd_plane_constraint;
 
+/* Keyframe animation of one object from a kinematics file
+ * (FDTD_RigidObject_Blender_Animator, FDTD_RigidObject_Animator.cpp:78-160;
+ * XML file_kinematics + kinematics_frame_rate, ImpulseResponseParser.cpp:
+ * 248-263).  Frame f holds the object's centre and rotation quaternion as the
+ * file writes them (tx ty tz qw qx qy qz; a Blender z-up rotation, which is
+ * turned by -90 degrees about x, q <- q * Rot(x, -90), with qw clamped to
+ * [-1, 1] first).  At time t the frames f0 = min(max(0, (int)(t / step)),
+ * n - 1) and f1 = min(f0 + 1, n - 1) are blended with alpha = (t - f0 step) /
+ * step (1 when f0 = f1): the centre linearly, the rotation by the reference's
+ * slerp; the pose replaces the object's own centre and rotation (its
+ * velocities must be zero).  Collocated scheme, rasterized ghost cells. */
+typedef struct fdtd_kinematics {
+    int           object;     /* index into fdtd_scene.objects              */
+    int           n_frames;
+    double        step_size;  /* 1 / kinematics_frame_rate (s)              */
+    const double *frames;     /* n_frames x 7 (the caller owns them)        */
+} fdtd_kinematics;
+
 typedef struct fdtd_scene {
     fdtd_settings                settings;
     int                          n_objects;
@@ -256,6 +276,8 @@ typedef struct fdtd_scene {
     const fdtd_plane_constraint *constraints;
     int                          n_acc_noise_sources;
     const fdtd_acc_noise_source *acc_noise_sources;
+    int                          n_kinematics;
+    const fdtd_kinematics       *kinematics;
 } fdtd_scene;
 
 /* Fills every field of a settings block with the parser defaults

@@ -497,6 +497,7 @@ struct oracle {
     fdtd_object *objs;
     fdtd_impulse_source *isrc;
     fdtd_acc_noise_source *ans;   /* acceleration-noise sources, impacts copied */
+    fdtd_kinematics *kin;         /* keyframe animations, frames copied (as in the file) */
     fdtd_pressure_source *psrc;
     double *listeners;
     double src_bmin[64][3], src_bmax[64][3], src_pos[64][3];
@@ -1573,6 +1574,15 @@ oracle *oracle_create(const fdtd_scene *scene, int reclassify_each_step)
     for (int i = 0; i < scene->n_objects; ++i)
         for (int d = 0; d < 3; ++d)
             o->moving |= scene->objects[i].velocity[d] != 0.0 || scene->objects[i].angular_velocity[d] != 0.0;
+    o->kin = (fdtd_kinematics *)calloc((size_t)scene->n_kinematics + 1, sizeof(fdtd_kinematics));
+    for (int i = 0; i < scene->n_kinematics; ++i) {
+        o->kin[i] = scene->kinematics[i];
+        double *copy = (double *)malloc(sizeof(double) * 7 * ((size_t)o->kin[i].n_frames + 1));
+        memcpy(copy, scene->kinematics[i].frames, sizeof(double) * 7 * (size_t)o->kin[i].n_frames);
+        o->kin[i].frames = copy;
+        o->moving = 1;
+    }
+    o->sc.kinematics = o->kin;
     if (o->moving) animate_objects(o, s->start_time);
     for (int i = 0; i < scene->n_pressure_sources; ++i) {
         /* GaussianPressureSource ctor: bbox = pos -/+ ws*2, position = bbox centroid */
@@ -1629,6 +1639,8 @@ void oracle_free(oracle *o)
     for (int i = 0; o->spk && i < o->sc.n_speaker_sources; ++i) free((void *)o->spk[i].samples);
     free(o->spk);
     for (int i = 0; o->ans && i < o->sc.n_acc_noise_sources; ++i) free((void *)o->ans[i].impacts);
+    for (int i = 0; o->kin && i < o->sc.n_kinematics; ++i) free((void *)o->kin[i].frames);
+    free(o->kin);
     free(o->ans);
     free(o->objs); free(o->objs0); free(o->isrc); free(o->psrc); free(o->listeners);
     free(o->pfull);
@@ -1641,11 +1653,114 @@ void oracle_free(oracle *o)
  * centre, from the pose at t = 0: c(t) = c0 + v t, R(t) = Rot(w/|w|, |w| t) R0
  * (Rodrigues' formula); stands in for FDTD_Objects::AnimateObjects with the
  * animator's GetRigidObjectTransform (FDTD_Objects.cpp:565-582). */
+/* Quaternion<double> (w, v) of src/linearalgebra/Quaternion.hpp */
+typedef struct { double w, x, y, z; } quat;
+
+static quat quat_mul(quat a, quat b)   /* operator* (Quaternion.hpp:82-89) */
+{
+    quat r;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z;
+    r.z = a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x;
+    return r;
+}
+
+static void quat_norm(quat *q)   /* normalize (Quaternion.hpp:186-195, EPS 1e-12) */
+{
+    double len = q->w * q->w + q->x * q->x + q->y * q->y + q->z * q->z;
+    if (fabs(len - 1.0) > 1e-12) {
+        len = 1.0 / sqrt(len);
+        q->w *= len; q->x *= len; q->y *= len; q->z *= len;
+    }
+}
+
+static quat quat_slerp(quat a, double r, quat b)   /* slerp (Quaternion.hpp:365-420) */
+{
+    quat v0 = a, v1 = b;
+    quat_norm(&v0);
+    quat_norm(&v1);
+    double dot = v0.w * v1.w + v0.x * v1.x + v0.y * v1.y + v0.z * v1.z;
+    const double DOT_THRESHOLD = 0.9995;
+    if (fabs(dot) > DOT_THRESHOLD) {
+        quat ret = {v0.w + (v1.w - v0.w) * r, v0.x + (v1.x - v0.x) * r, v0.y + (v1.y - v0.y) * r, v0.z + (v1.z - v0.z) * r};
+        quat_norm(&ret);
+        return ret;
+    }
+    if (dot < 0.0) {
+        v1.w = -v1.w; v1.x = -v1.x; v1.y = -v1.y; v1.z = -v1.z;
+        dot = -dot;
+    }
+    if (dot < -1.0) dot = -1.0;
+    if (dot > 1.0) dot = 1.0;
+    const double theta_0 = acos(dot), theta = theta_0 * r;
+    quat v2 = {v1.w - v0.w * dot, v1.x - v0.x * dot, v1.y - v0.y * dot, v1.z - v0.z * dot};
+    quat_norm(&v2);
+    quat ret = {v0.w * cos(theta) + v2.w * sin(theta), v0.x * cos(theta) + v2.x * sin(theta),
+                v0.y * cos(theta) + v2.y * sin(theta), v0.z * cos(theta) + v2.z * sin(theta)};
+    return ret;
+}
+
+/* frame f of a kinematics file as ReadObjectKinData keeps it
+ * (FDTD_RigidObject_Animator.cpp:98-118): w clamped to [-1, 1], then the
+ * Blender z-up rotation turned by -90 degrees about x, q * fromAxisRotD(x, -90) */
+static void kin_frame(const fdtd_kinematics *k, int f, double *c, quat *q)
+{
+    const double *r = k->frames + 7 * (size_t)f;
+    for (int d = 0; d < 3; ++d) c[d] = r[d];
+    double w = r[3];
+    w = fmax(-1.0, fmin(1.0, w));
+    const quat file = {w, r[4], r[5], r[6]};
+    const double rad = -90.0 * M_PI / 180.0;
+    const quat corr = {cos(rad / 2), sin(rad / 2), 0.0, 0.0};   /* fromAxisRotD (Quaternion.hpp:202-208) */
+    *q = quat_mul(file, corr);
+}
+
+/* pose of a keyframed object at time t: GetRigidObjectTransform
+ * (FDTD_RigidObject_Animator.cpp:137-160, frames step apart from time 0),
+ * the normalize of FDTD_Objects::AnimateObjects (FDTD_Objects.cpp:563-579)
+ * and SetRigidBodyTransform (FDTD_RigidObject.cpp:550-566: angle 2 acos(w)
+ * about the normalized vector part, Eigen's angle-axis matrix; w clamped to
+ * [-1, 1] for acos) */
+static void kin_pose(const fdtd_kinematics *k, double t, fdtd_object *b)
+{
+    const int n = k->n_frames;
+    const double pos = t / k->step_size;
+    int idx_0 = pos > 0.0 ? (int)pos : 0;
+    if (idx_0 > n - 1) idx_0 = n - 1;
+    const int idx_1 = idx_0 + 1 < n - 1 ? idx_0 + 1 : n - 1;
+    const double t_0 = idx_0 * k->step_size, t_1 = idx_1 * k->step_size;
+    const double alpha = (idx_0 == idx_1 ? 1.0 : (t - t_0) / (t_1 - t_0));
+    double c0[3], c1[3];
+    quat q0, q1;
+    kin_frame(k, idx_0, c0, &q0);
+    kin_frame(k, idx_1, c1, &q1);
+    for (int d = 0; d < 3; ++d) b->center[d] = c0[d] * (1.0 - alpha) + c1[d] * alpha;
+    quat q = quat_slerp(q0, alpha, q1);
+    quat_norm(&q);
+    const double rad = 2.0 * acos(fmax(-1.0, fmin(1.0, q.w)));
+    double ax[3] = {0.0, 1.0, 0.0};
+    if (!(rad < 1e-12)) {
+        const double l = sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+        ax[0] = q.x / l; ax[1] = q.y / l; ax[2] = q.z / l;
+    }
+    const double cs = cos(rad), sn = sin(rad);
+    /* Eigen AngleAxis::toRotationMatrix: cos I + sin [a]x + (1 - cos) a a^T */
+    const double cross[9] = {0, -ax[2], ax[1], ax[2], 0, -ax[0], -ax[1], ax[0], 0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+            b->rotation[r * 3 + c] = (r == c ? cs : 0.0) + sn * cross[r * 3 + c] + (1.0 - cs) * ax[r] * ax[c];
+}
+
 static void animate_objects(oracle *o, double t)
 {
     for (int i = 0; i < o->sc.n_objects; ++i) {
         const fdtd_object *a = &o->objs0[i];
         fdtd_object *b = &o->objs[i];
+        int keyed = 0;
+        for (int q = 0; q < o->sc.n_kinematics; ++q)
+            if (o->kin[q].object == i) { kin_pose(&o->kin[q], t, b); keyed = 1; }
+        if (keyed) continue;
         for (int d = 0; d < 3; ++d) b->center[d] = a->center[d] + a->velocity[d] * t;
         const double *w = a->angular_velocity;
         const double wn = sqrt(w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
@@ -1666,6 +1781,16 @@ static void animate_objects(oracle *o, double t)
                 b->rotation[r*3+c] = v;
             }
     }
+}
+
+/* pose of a keyframed object at time t (test hook of kin_pose) */
+void oracle_keyframe_pose(const fdtd_kinematics *k, double t, double *center, double *rotation)
+{
+    fdtd_object b;
+    memset(&b, 0, sizeof b);
+    kin_pose(k, t, &b);
+    memcpy(center, b.center, sizeof b.center);
+    memcpy(rotation, b.rotation, sizeof b.rotation);
 }
 
 /* pose of a moving object at time t (test hook of animate_objects) */

@@ -21,6 +21,9 @@
  *   src/wavesolver/HarmonicVibrationalSource.cpp          5-28
  *   src/wavesolver/AccelerationNoiseVibrationalSource.cpp 19-90
  *   src/modal_model/ImpulseSeriesObject.cpp               88-101
+ *   src/wavesolver/FDTD_RigidObject_Animator.cpp          78-160
+ *   src/wavesolver/FDTD_RigidObject.cpp                   550-566
+ *   src/linearalgebra/Quaternion.hpp                      82-89, 186-208, 365-420
  *   src/math/MLSModeInterpolator.hpp     282-429, 588-597, 624-660, 712-738
  * and, for the staggered scheme:
  *   src/wavesolver/PML_WaveSolver.cpp    710-773 (stepLeapfrog)
@@ -74,6 +77,9 @@ int     oracle_svd_values(int m, int n, const double *A, double *sv);
 int     oracle_lsq_solve(int m, int n, const double *A, const double *b, double *x);
 /* Order-1 MLS value at p from npts points (x,y,z) and values. */
 double  oracle_mls_lookup(const double *p, int npts, const double *pts, const double *vals);
+/* Pose (centre, row-major world-from-object rotation) at time t of an object
+ * animated by the keyframes k. */
+void    oracle_keyframe_pose(const fdtd_kinematics *k, double t, double *center, double *rotation);
 /* Boundary acceleration vector of object obj at time t at a surface point
  * with outward normal nrm (the sum over its vibrational sources). */
 void    oracle_object_accel(const oracle *o, int obj, double t, const double *nrm, double *out);

@@ -30,7 +30,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        from acoustics_amd.capi import Scene, Object, Settings
+        from acoustics_amd.capi import Kinematics, Object, Scene, Settings
         L = C.CDLL(LIB)
         D = C.POINTER(C.c_double)
         L.oracle_create.restype = C.c_void_p
@@ -64,6 +64,8 @@ def lib():
         L.oracle_ghost_system.restype = C.c_int
         L.oracle_ghost_system.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int), D, D]
+        L.oracle_keyframe_pose.restype = None
+        L.oracle_keyframe_pose.argtypes = [C.POINTER(Kinematics), C.c_double, D, D]
         L.oracle_object_accel.restype = None
         L.oracle_object_accel.argtypes = [C.c_void_p, C.c_int, C.c_double, D, D]
         L.oracle_pml_sigma.restype = C.c_double
@@ -72,6 +74,18 @@ def lib():
         L.oracle_inside_pml.argtypes = [C.POINTER(Settings), D]
         _lib = L
     return _lib
+
+
+def keyframe_pose(object_index, step_size, frames, t):
+    """(centre, 3x3 rotation) at time t of an object animated by keyframes
+    (frames: (n, 7) tx ty tz qw qx qy qz as a kinematics file holds them)."""
+    from acoustics_amd.capi import Kinematics
+    L = lib()
+    fr = np.ascontiguousarray(np.asarray(frames, dtype=np.float64).reshape(-1, 7))
+    k = Kinematics(object_index, fr.shape[0], step_size, fr.ctypes.data_as(C.POINTER(C.c_double)))
+    c, r = np.zeros(3), np.zeros(9)
+    L.oracle_keyframe_pose(C.byref(k), float(t), _d(c), _d(r))
+    return c, r.reshape(3, 3)
 
 
 def _d(a):

@@ -189,3 +189,28 @@ def test_acc_noise_refusals():
                                acc_noise_sources=sb.acc_noise_list)
     assert lib.fdtd_scene_device_bytes(moving.ref()) == -1
     assert "static" in capi.last_error()
+
+
+def test_xml_kinematics_file(tmp_path):
+    """file_kinematics / kinematics_frame_rate on an object
+    (ImpulseResponseParser.cpp:248-263): rows frame tx ty tz qw qx qy qz read
+    next to the XML file, step 1 / rate."""
+    lib = capi.load_library()
+    (tmp_path / "swing.txt").write_text("0 0.0 0.01 0.0 1 0 0 0\n\n1 0.001 0.01 0.0 0.9 0.1 0.2 0.3\n"
+                                        "2 0.002 0.011 -0.001 0.8 0.2 0.3 0.4\n")
+    src = open(os.path.join(ROOT, "tests", "golden", "scene_sphere.xml")).read()
+    src = src.replace('radius="0.025">', 'radius="0.025" file_kinematics="swing.txt" kinematics_frame_rate="2000">')
+    path = tmp_path / "kin.xml"
+    path.write_text(src)
+    p = C.c_void_p()
+    assert lib.fdtd_parse_xml(str(path).encode(), C.byref(p)) == 0, capi.last_error()
+    sc = lib.fdtd_parsed_scene(p).contents
+    assert sc.n_kinematics == 1
+    k = sc.kinematics[0]
+    assert k.object == 0 and k.n_frames == 3 and k.step_size == 1 / 2000
+    fr = np.array([k.frames[i] for i in range(21)]).reshape(3, 7)
+    assert np.array_equal(fr[1], [0.001, 0.01, 0.0, 0.9, 0.1, 0.2, 0.3])
+    lib.fdtd_parsed_free(p)
+    (tmp_path / "swing.txt").write_text("0 0.0 0.01 0.0 1 0 0\n")
+    assert lib.fdtd_parse_xml(str(path).encode(), C.byref(p)) == 5   # FDTD_ERR_PARSE: a short row
+    assert "row needs" in capi.last_error()
