@@ -19,6 +19,8 @@ FDTD_BOUNDARY_RASTERIZE = 0
 FDTD_BOUNDARY_FULLY_COUPLED = 1
 FDTD_SCHEME_COLLOCATED = 0
 FDTD_SCHEME_STAGGERED = 1
+FDTD_SAMPLE_CENTER = 0
+FDTD_SAMPLE_SEVEN = 1
 FDTD_PML_STATE_MERGED = 0
 FDTD_PML_STATE_SPLIT = 1
 FDTD_TIMING_OFF = 0
@@ -41,7 +43,7 @@ class Settings(C.Structure):
         ("pml_strength", C.c_double), ("air_viscosity_alpha", C.c_double), ("start_time", C.c_double),
         ("boundary_condition_preset", C.c_int), ("boundary_handling", C.c_int), ("use_mesh", C.c_int),
         ("divisions", C.c_int * 3), ("min_bound", C.c_double * 3),
-        ("scheme", C.c_int), ("time_parallel", C.c_int), ("num_time_chunks", C.c_int), ("reserved", C.c_int),
+        ("scheme", C.c_int), ("time_parallel", C.c_int), ("num_time_chunks", C.c_int), ("sampling", C.c_int),
         ("overlap_time", C.c_double), ("early_termination_ratio", C.c_double),
         ("stop_boundary_acc_time", C.c_double),
     ]

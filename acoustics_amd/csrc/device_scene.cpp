@@ -351,6 +351,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     D.terms = L.terms;
     D.clock = L.clock;
     D.n_obj = (int)S.objs.size();
+    D.seven = S.sampling == FDTD_SAMPLE_SEVEN ? 1 : 0;
     D.n_cpo = S.n_cpo;
     D.n_ch = D.n_obj * S.n_cpo;
     D.acc_tab = L.acc_tab;

@@ -134,6 +134,16 @@ struct SceneSetup {
     double kcalp = 0, c2k2 = 0;  // c alpha dt and (c dt)^2
     int boundary = 0;
     int scheme = 0;                 // FDTD_SCHEME_*
+    // FDTD_SAMPLE_*: CENTER classifies by the cell centre, puts a
+    // rasterised ghost on the face (a fully-coupled one at the solid
+    // centre) with p_g = p_nbr - h grad_p . n_cell, and takes ghost values
+    // with the 7-point weights; SEVEN adds the six points h/4 from the
+    // centre, puts ghosts h/4 from it with 0.75 h and the 3h/4 weights
+    int sampling = FDTD_SAMPLE_CENTER;
+    // distance of a ghost from its solid cell's centre, in cells, and the
+    // factor of h in the rasterised ghost value
+    double ghost_offset() const { return sampling == FDTD_SAMPLE_SEVEN ? 0.25 : (boundary == FDTD_BOUNDARY_RASTERIZE ? 0.5 : 0.0); }
+    double raster_factor() const { return sampling == FDTD_SAMPLE_SEVEN ? 0.75 : 1.0; }
     // PML, separable per axis
     std::vector<uint8_t> cpml[3];   // cell predicate, n_d entries
     std::vector<uint8_t> fpml[3];   // face predicate, n_d + 1 entries (walls included)

@@ -118,6 +118,10 @@ struct SceneDev {
     double *terms;              // [cap][T]
     unsigned long long *clock;  // CLK_WORDS words
     int n_obj;
+    // 1: the seven-sample configuration (ghosts at 3h/4 from the fluid cell,
+    // non-uniform Laplacian weights, 0.75 h); 0: centre sampling (ghosts on
+    // the face, plain weights, h)
+    int seven;
     int n_cpo, n_ch;            // amplitude channels per object (1, or 4 with acceleration noise), n_obj n_cpo
     // time-dependent amplitudes, one ring row per step (amp_row):
     double *acc_tab;            // [AMP_ROWS][n_ch] channel o n_cpo: a_s(o, t_s); o n_cpo + 1 + d: a_v(o, t_s)[d]

@@ -134,9 +134,10 @@ __device__ __forceinline__ float speaker_at(const SceneDev &S, int i, int j, int
 // x_g = sum_q M_gq p_q + sum_o m_go a_n(o), FP64, in row order.  Computed where
 // it is used (the object-adjacent cell next to the ghost, and listener terms),
 // four independent index loads and gathers at a time.
-// -0.75 h grad_p . cell_n of a rasterised ghost, grad_p = -rho (a_s n_b + a_v)
-// (MAC_Grid.cpp:1256-1281): k_raster a_s, plus k_vec a_v along the ghost's
-// axis in a scene with acceleration noise
+// -f h grad_p . cell_n of a rasterised ghost (f = 1 with centre sampling,
+// 0.75 with seven samples), grad_p = -rho (a_s n_b + a_v) (MAC_Grid.cpp:
+// 1256-1281): k_raster a_s, plus k_vec a_v along the ghost's axis in a
+// scene with acceleration noise
 __device__ __forceinline__ double raster_accel_term(const SceneDev &S, const GhostDev &gd, long long step)
 {
     const AS_GLOBAL double *acc = gmem(S.acc_tab) + amp_row(step) * S.n_ch + gd.obj * S.n_cpo;
@@ -744,11 +745,12 @@ __device__ __forceinline__ void mac_tile(const SceneDev &S, const TileDesc &t, c
 // group (the fifth is the next lane's first, recomputed the same way), its
 // cells' low x and y faces (the high ones are recomputed by the neighbours
 // that own them) and its four pressures, marching the tile's planes with the
-// x face carried.  Arithmetic as mac_tile<false>, cell by cell.  The host
+// x face carried, and writes the listener terms of its cells.  Arithmetic as
+// mac_tile<false>, cell by cell.  The host
 // builds these rows only where each axis's coefficient table holds a single
 // value over the tile's faces and cells (outside the PML: f1 = 1, sigma = 0),
 // so one entry per axis is held instead of one per face.
-__device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &t, long long step)
+__device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     const int nk4 = (t.k1 - t.k0 + 3) >> 2, R = t.rpw, rem = (t.k1 - t.k0) & 3;
     const int j0w = t.j0 + (int)(threadIdx.x >> 6) * R;
@@ -816,6 +818,12 @@ __device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &
             store4(vo1 + idx, vyL, n, lo);
             store4(vo2 + idx, make_float4(f0, f1, f2, f3), n, lo);
             store4(pnext + idx, pn, n, lo);
+            if (t.te > t.tb) {
+                if (lo == 0) emit_cell_terms(S, t, terms, step, idx, pn.x);
+                if (n > 1 && lo <= 1) emit_cell_terms(S, t, terms, step, idx + 1, pn.y);
+                if (n > 2 && lo <= 2) emit_cell_terms(S, t, terms, step, idx + 2, pn.z);
+                if (n > 3) emit_cell_terms(S, t, terms, step, idx + 3, pn.w);
+            }
         }
         vxL = vxR;
         pm = pc;
@@ -826,6 +834,22 @@ __device__ __forceinline__ void mac_vec_tile(const SceneDev &S, const TileDesc &
 // ---------------------------------------------------------------------------
 // bulk cells next to rigid objects (FP64)
 // ---------------------------------------------------------------------------
+// One axis of the ghost-aware Laplacian (MAC_Grid::
+// pressureFieldLaplacianGhostCell, MAC_Grid.cpp:187-299) at a cell with
+// pressure c, neighbours nm / np and ghost values gm / gp where the
+// neighbour is solid: the plain 3-point stencil with ghost values (centre
+// sampling, 247-266), or the non-uniform one for ghosts 3h/4 away (seven
+// samples, 268-286).
+__device__ __forceinline__ double ghost_lap_axis(int seven, bool ghm, bool ghp, double gm, double gp, double nm, double np,
+                                                 double c)
+{
+    const double m = ghm ? gm : nm, p = ghp ? gp : np;
+    if (!seven || (!ghm && !ghp)) return p + m - 2.0 * c;
+    if (ghm && ghp) return (gp + gm - 2.0 * c) / (9.0 / 16.0);
+    if (ghp) return (gp + 0.75 * nm - (7.0 / 4.0) * c) / (21.0 / 32.0);
+    return (0.75 * np + gm - (7.0 / 4.0) * c) / (21.0 / 32.0);
+}
+
 __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &t, const TermRef *terms, long long step)
 {
     const int r = t.i0 + (int)threadIdx.x;
@@ -856,13 +880,9 @@ __device__ __forceinline__ void special_tile(const SceneDev &S, const TileDesc &
         pn = c;  // one-cell cavity: p^{n+1} = p^n (MAC_Grid.cpp:924-925)
     } else {
         double lap = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const bool gn = rec.gid[2 * d] >= 0, gp = rec.gid[2 * d + 1] >= 0;
-            if (gp && gn)      lap += (g[2 * d + 1] + g[2 * d] - 2.0 * c) / (9.0 / 16.0);
-            else if (gp)       lap += (g[2 * d + 1] + 0.75 * nb[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-            else if (gn)       lap += (0.75 * nb[2 * d + 1] + g[2 * d] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-            else               lap += (nb[2 * d + 1] + nb[2 * d] - 2.0 * c);
-        }
+        for (int d = 0; d < 3; ++d)
+            lap += ghost_lap_axis(S.seven, rec.gid[2 * d] >= 0, rec.gid[2 * d + 1] >= 0, g[2 * d], g[2 * d + 1], nb[2 * d],
+                                  nb[2 * d + 1], c);
         if (S.lapv) {
             // air viscosity: the ghost-aware Laplacian L^n = lap / h^2 (FP64)
             // and the previous step's L^{n-1} of this cell
@@ -991,12 +1011,13 @@ __device__ inline void mv_centre(const SceneDev &S, int i, int j, int k, double 
     for (int d = 0; d < 3; ++d) x[d] = S.bmin[d] + (0.5 + (double)ijk[d]) * S.h;
 }
 
-// fluid by the 7-sample rule: the centre and the six points a quarter cell
-// away along the axes all lie outside every object
+// fluid: the centre, and by the seven-sample rule also the six points a
+// quarter cell away along the axes, lie outside every object
 __device__ inline bool mv_fluid_at(const SceneDev &S, const Geom *pose, const double *x)
 {
     const double q4 = 0.25 * S.h;
-    for (int smp = 0; smp < 7; ++smp) {
+    const int nsmp = S.seven ? 7 : 1;
+    for (int smp = 0; smp < nsmp; ++smp) {
         double y[3] = {x[0], x[1], x[2]};
         if (smp > 0) y[(smp - 1) >> 1] += ((smp - 1) & 1) ? q4 : -q4;
         for (int o = 0; o < S.n_obj; ++o)
@@ -1182,9 +1203,10 @@ __global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__res
         }
     }
     // rasterized ghost term of a fluid cell's solid neighbour in direction q:
-    // 0.75 h rho a_n (n_b . n_cell), the ghost a quarter cell from the solid
-    // cell's centre towards this cell, its object the one nearest the solid
-    // cell's centre, n_b the outward normal at the ghost's closest boundary point
+    // f h rho a_n (n_b . n_cell), the ghost on the face (f = 1) or a quarter
+    // cell from the solid cell's centre (seven samples, f = 0.75) towards
+    // this cell, its object the one nearest the solid cell's centre, n_b the
+    // outward normal at the ghost's closest boundary point
     if (!now) return;
     const int d = q >> 1, sg = (q & 1) ? 1 : -1;
     int m[3] = {i, j, kz};
@@ -1195,10 +1217,10 @@ __global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__res
     mv_centre(S, m[0], m[1], m[2], xo);
     const int id = mv_nearest(S, xo, &dist);
     for (int e = 0; e < 3; ++e) gp[e] = xo[e];
-    gp[d] -= sg * 0.25 * S.h;
+    gp[d] -= sg * (S.seven ? 0.25 : 0.5) * S.h;
     geom_closest(S.pose[id], gp, xb, nb);
     const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
-    S.ughost[(size_t)q * ncell + c] = 0.75 * S.h * S.rho * a * nb[d] * -(double)sg;
+    S.ughost[(size_t)q * ncell + c] = (S.seven ? 0.75 : 1.0) * S.h * S.rho * a * nb[d] * -(double)sg;
 }
 
 // A box of a moving scene's cells (one thread per cell): solid cells are
@@ -1240,10 +1262,8 @@ __device__ void moving_tile(const SceneDev &S, const TileDesc &t, long long step
             if (solid[side]) g[side] = c + S.ughost[(size_t)(2 * d + side) * ncell + u];
         }
         // side 0 is the negative neighbour, side 1 the positive one
-        if (solid[1] && solid[0]) { lap += (g[1] + g[0] - 2.0 * c) / (9.0 / 16.0); ++surrounded; }
-        else if (solid[1])        lap += (g[1] + 0.75 * nbv[0] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-        else if (solid[0])        lap += (0.75 * nbv[1] + g[0] - (7.0 / 4.0) * c) / (21.0 / 32.0);
-        else                      lap += (nbv[1] + nbv[0] - 2.0 * c);
+        lap += ghost_lap_axis(S.seven, solid[0], solid[1], g[0], g[1], nbv[0], nbv[1], c);
+        if (solid[1] && solid[0]) ++surrounded;
     }
     double pn;
     if (surrounded > 1 && prev_f[u]) pn = c;   // MAC_Grid.cpp:924-925
@@ -1322,7 +1342,7 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
         else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
-        else if (t.type == TILE_MAC_VEC) mac_vec_tile(S, t, step);
+        else if (t.type == TILE_MAC_VEC) mac_vec_tile(S, t, terms, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     } else if constexpr (KIND == KIND_PML) {
         if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);

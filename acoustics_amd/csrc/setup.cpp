@@ -352,6 +352,11 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
     S.c2k2 = (S.c * S.dt) * (S.c * S.dt);
     S.boundary = st.boundary_handling;
     S.scheme = st.scheme;
+    if (st.sampling != FDTD_SAMPLE_CENTER && st.sampling != FDTD_SAMPLE_SEVEN) {
+        err = "sampling must be FDTD_SAMPLE_CENTER or FDTD_SAMPLE_SEVEN";
+        return FDTD_ERR_INVALID;
+    }
+    S.sampling = st.sampling;
     S.src_steps = -1;
     if (st.time_parallel) {
         // SimWorld::UpdateObjectState switches every source off once the
@@ -793,7 +798,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     double x[3];
                     cpos(i, j, k, x);
                     bool fluid = true;
-                    for (int smp = 0; smp < 7 && fluid; ++smp) {
+                    const int nsmp = S.sampling == FDTD_SAMPLE_SEVEN ? 7 : 1;
+                    for (int smp = 0; smp < nsmp && fluid; ++smp) {
                         double y[3] = {x[0], x[1], x[2]};
                         if (smp > 0) y[(smp - 1) / 2] += ((smp - 1) % 2 ? q4 : -q4);
                         for (const Geom &g : g0) fluid = fluid && !geom_inside(g, y);
@@ -872,11 +878,12 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                         classified[ci] = 1;
                         double x[3];
                         cpos(i, j, k, x);
-                        // isFluid (MAC_Grid.cpp:3566-3577): the centre outside
-                        // every constraint, all seven samples outside every object
+                        // isFluid (MAC_Grid.cpp:3558-3577): the centre outside
+                        // every constraint and every object, and with the
+                        // seven-sample rule the six points h/4 from it too
                         const bool in_con = constrained(x);
                         bool fluid = !in_con && !occupied(x);
-                        for (int d = 0; d < 3 && fluid; ++d)
+                        for (int d = 0; d < 3 && fluid && S.sampling == FDTD_SAMPLE_SEVEN; ++d)
                             for (int sg = -1; sg <= 1 && fluid; sg += 2) {
                                 double y[3] = {x[0], x[1], x[2]};
                                 y[d] += (double)sg * q4;
@@ -940,7 +947,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     }
                     g.obj = bo;
                     for (int e = 0; e < 3; ++e) g.pos[e] = x[e];
-                    g.pos[d] += (g.topology > 0 ? 1.0 : -1.0) * 0.25 * h;
+                    // MakeGhostCell (MAC_Grid.cpp:4310-4340)
+                    g.pos[d] += (g.topology > 0 ? 1.0 : -1.0) * S.ghost_offset() * h;
                     tmp.push_back({pair_key(ci, nb), g});
                 }
         }
@@ -1001,8 +1009,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         cell_n[std::abs(gc.topology) - 1] = gc.topology > 0 ? 1.0 : -1.0;
         const double ndc = nb[0] * cell_n[0] + nb[1] * cell_n[1] + nb[2] * cell_n[2];
         const double nde = nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2];
-        gc.k_raster = 0.75 * h * rho * ndc;
-        gc.k_vec = 0.75 * h * rho * (gc.topology > 0 ? 1.0 : -1.0);
+        gc.k_raster = S.raster_factor() * h * rho * ndc;
+        gc.k_vec = S.raster_factor() * h * rho * (gc.topology > 0 ? 1.0 : -1.0);
         gc.coupled = 0;
         gc.c_acc = 0;
         gc.c_vec[0] = gc.c_vec[1] = gc.c_vec[2] = 0.0;

@@ -73,6 +73,7 @@ void fdtd_settings_default(fdtd_settings *s)
     for (int d = 0; d < 3; ++d) s->min_bound[d] = -0.005 * 64 / 2.0;
     s->time_parallel = 0;
     s->num_time_chunks = 1;
+    s->sampling = FDTD_SAMPLE_CENTER;   // the reference's compiled default (ONLY_SAMPLE_CENTER)
     s->overlap_time = 6e-3;             // ImpulseResponseParser.cpp:531
     s->early_termination_ratio = 1e-3;  // ImpulseResponseParser.cpp:532
     s->stop_boundary_acc_time = 0.0;

@@ -542,3 +542,12 @@ def keyframed_object(seed=22, n=56, h=0.005, pml=8, steps=360, frame_rate=4000.0
     listeners = np.array(keep[::max(1, len(keep) // n_listeners)][:n_listeners])
     return SceneBuffers(s, objects=[obj], impulse_sources=[src], listeners=listeners,
                         kinematics=[dict(object=0, step_size=step, frames=fr)]), steps
+
+
+def with_settings(sb, **changes):
+    """The same scene with some settings changed (e.g. sampling=FDTD_SAMPLE_SEVEN)."""
+    s = dict(sb.settings_dict)
+    s.update(changes)
+    return SceneBuffers(s, objects=sb.objects_list, impulse_sources=sb.impulse_list, pressure_sources=sb.pressure_list,
+                        listeners=sb.listeners_np, speaker_sources=sb.speaker_list, constraints=sb.constraint_list,
+                        acc_noise_sources=sb.acc_noise_list, kinematics=sb.kinematics_list)

@@ -101,6 +101,24 @@ enum {
     FDTD_SCHEME_STAGGERED  = 1
 };
 
+/* How object cells are sampled: the reference's compile-time choice
+ * (Wavesolver_ConstantsAndTypes.h:46-47).  CENTER is its compiled default
+ * (ONLY_SAMPLE_CENTER and STAGGER_ACC_SAMPLE defined): a cell is solid when
+ * an object holds its centre (MAC_Grid.cpp:3558-3564); a rasterised ghost
+ * sits on the face between the solid cell and its fluid neighbour and takes
+ * p_g = p_nbr - h grad_p . n_cell (MAC_Grid.cpp:1277-1278, 4322-4331), a
+ * fully-coupled ghost sits at the solid cell's centre; the Laplacian takes
+ * ghost values with the plain 7-point weights (MAC_Grid.cpp:247-266).
+ * SEVEN is the other configuration (ONLY_SAMPLE_CENTER undefined): solid
+ * when the centre or any of the six points h/4 from it is inside an object
+ * (3566-3577), ghosts h/4 from the solid cell's centre, p_g = p_nbr -
+ * 0.75 h grad_p . n_cell (1280), and the non-uniform weights for a ghost at
+ * 3h/4 (268-286). */
+enum {
+    FDTD_SAMPLE_CENTER = 0,
+    FDTD_SAMPLE_SEVEN  = 1
+};
+
 typedef struct fdtd_settings {
     double sound_speed;               /* sound_speed (m/s), default 343        */
     double density;                   /* density (kg/m^3)                      */
@@ -128,7 +146,7 @@ typedef struct fdtd_settings {
      * time_step, as the reference's clock runs; SimWorld.cpp:474). */
     int    time_parallel;             /* time_parallel (0: off)                */
     int    num_time_chunks;           /* num_time_chunks                       */
-    int    reserved;
+    int    sampling;                  /* FDTD_SAMPLE_* (object cells, ghosts)  */
     double overlap_time;              /* overlap_time (s), default 6e-3        */
     double early_termination_ratio;   /* early_termination_ratio, default 1e-3 */
     double stop_boundary_acc_time;    /* set per chunk by fdtd_time_chunk      */

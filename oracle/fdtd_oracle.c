@@ -712,9 +712,11 @@ static int cmp_ghost(const void *a, const void *b)
 }
 
 /* MAC_Grid::classifyCells_FAST (MAC_Grid.cpp:3455-3891) for rigid objects:
- * bulk = all 7 samples (centre, +-h/4 per axis) outside every object; solid
- * cells with a bulk 6-neighbour become ghost cells, one ghost entry per
- * (solid owner, bulk neighbour) pair via MakeGhostCell (4310-4340). */
+ * bulk = the centre outside every object (ONLY_SAMPLE_CENTER, the compiled
+ * default, 3558-3564), or all 7 samples (centre, +-h/4 per axis) outside
+ * every object (3566-3577); solid cells with a bulk 6-neighbour become ghost
+ * cells, one ghost entry per (solid owner, bulk neighbour) pair via
+ * MakeGhostCell (4310-4340). */
 /* FDTD_Objects::OccupyByConstraint (FDTD_Objects.cpp:54-71) with
  * FDTD_PlaneConstraint::Inside (FDTD_PlaneConstraint.h:38-42) */
 static int occupy_by_constraint(const oracle *o, const double *x)
@@ -777,7 +779,7 @@ static void classify_fast(oracle *o)
                centre */
             const int in_con = occupy_by_constraint(o, x);
             int fluid = !in_con && occupy_by_object(o, x) < 0;
-            for (int d = 0; d < 3 && fluid; ++d)
+            for (int d = 0; d < 3 && fluid && o->sc.settings.sampling == FDTD_SAMPLE_SEVEN; ++d)
                 for (int sgn = -1; sgn <= 1 && fluid; sgn += 2) {
                     double y[3] = {x[0], x[1], x[2]};
                     y[d] += (double)sgn * q;
@@ -818,7 +820,14 @@ static void classify_fast(oracle *o)
                 double x[3]; ppos(o, ijk, x);
                 lowest_object_distance(o, x, &g->distance, &g->object);
                 g->position[0] = x[0]; g->position[1] = x[1]; g->position[2] = x[2];
-                g->position[d] += (g->topology > 0 ? 1.0 : -1.0) * 0.25 * o->h;
+                /* MakeGhostCell (MAC_Grid.cpp:4322-4331): h/4 towards the
+                   neighbour without ONLY_SAMPLE_CENTER; with it (and
+                   STAGGER_ACC_SAMPLE) h/2 for rasterized ghosts, none for
+                   fully-coupled ones */
+                double off = 0.25;
+                if (o->sc.settings.sampling != FDTD_SAMPLE_SEVEN)
+                    off = o->sc.settings.boundary_handling == FDTD_BOUNDARY_RASTERIZE ? 0.5 : 0.0;
+                g->position[d] += (g->topology > 0 ? 1.0 : -1.0) * off * o->h;
                 g->key = szudzik(ci, nb);
             }
         o->ghostc[ci] = (unsigned char)isg;
@@ -1053,7 +1062,8 @@ static void ghost_update(oracle *o, const double *p)
         cell_n[abs(gc->topology) - 1] = gc->topology > 0 ? 1.0 : -1.0;
         const double grad_p_nr = dot3(grad_p, cell_n);
         const double grad_p_ne = dot3(grad_p, en);
-        const double pg = p_neig - 0.75 * h * grad_p_nr;
+        /* ONLY_SAMPLE_CENTER: pg = p_neig - h grad_p_nr; else 0.75 h (MAC_Grid.cpp:1277-1281) */
+        const double pg = o->sc.settings.sampling == FDTD_SAMPLE_SEVEN ? p_neig - 0.75 * h * grad_p_nr : p_neig - h * grad_p_nr;
         double weights = (object_distance(obj, gc->position) < 0.0 ? -2.0 * fabs(distance) : -fabs(distance));
         double weighted = grad_p_ne * weights;
         long long nbrs[8];
@@ -1199,7 +1209,20 @@ static void laplacian_ghost(oracle *o, const double *val, double *lap)
             if (ijk[d] != 0) neg[d] -= 1;
             const long long ip = pidx(o, pos[0], pos[1], pos[2]), in = pidx(o, neg[0], neg[1], neg[2]);
             const int gp = o->ghostc[ip], gn = o->ghostc[in];
-            if (gp && gn) {
+            if (o->sc.settings.sampling != FDTD_SAMPLE_SEVEN) {
+                /* ONLY_SAMPLE_CENTER (MAC_Grid.cpp:247-266): ghost values
+                   with the plain weights */
+                if (gp && gn) {
+                    acc += (ghost_pressure(o, ci, ip) + ghost_pressure(o, ci, in) - 2.0 * val[ci]);
+                    ++surrounded;
+                } else if (gp && !gn) {
+                    acc += (ghost_pressure(o, ci, ip) + val[in] - 2.0 * val[ci]);
+                } else if (!gp && gn) {
+                    acc += (val[ip] + ghost_pressure(o, ci, in) - 2.0 * val[ci]);
+                } else {
+                    acc += (val[ip] + val[in] - 2.0 * val[ci]);
+                }
+            } else if (gp && gn) {
                 acc += (ghost_pressure(o, ci, ip) + ghost_pressure(o, ci, in) - 2.0 * val[ci]) / (9.0 / 16.0);
                 ++surrounded;
             } else if (gp && !gn) {

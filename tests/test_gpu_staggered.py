@@ -49,3 +49,21 @@ def test_staggered_tile_plans_agree(hip_lib):
         else:
             assert np.array_equal(out[0], base[0]), plan
             assert np.array_equal(out[1], base[1]), plan
+
+
+def test_staggered_float4_rows_carry_listeners(hip_lib):
+    """Rows long enough for the float4 row tiles (the default plan), with
+    listeners whose MLS cells lie in them: every plan gives the same series
+    bit for bit and the oracle's to 1e-4 (a row tile that dropped its cells'
+    listener terms lost part of each listener's sum)."""
+    sb, steps = scenes.staggered_free_field(seed=9, div=(72, 64, 80), pml=6, steps=200, n_listeners=12)
+    print(f"case: staggered float4 rows divisions={sb.divisions} steps={steps}", flush=True)
+    ref = Oracle(sb, reclassify_each_step=False).step(steps)
+    outs = []
+    for plan in (3, 4, 8):
+        with Solver(sb, tile_config=plan) as s:
+            s.step(steps)
+            outs.append(s.series())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert rel_l2(outs[0], ref) <= TOL

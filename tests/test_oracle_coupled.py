@@ -6,14 +6,18 @@ fixes rather than by restating the oracle's formulas:
   the same system, with a residual at rounding level;
 * on the exact field p = a0 + b (|x - c| - R) around a sphere of radius R
   centred at c, with the Neumann data that field satisfies (dp/dn = b, so
-  a_n = -b / rho), the ghost values reproduce the field at the ghost
-  positions to second order: max error <= 0.06 b h at h = 4 mm (measured
-  0.032) and the error falls by at least 0.65x when h halves (measured
-  0.53x).  Planted in the oracle on the CPU, each of these raised the error
-  to 0.48-370 b h: the sign of the image-point correction, the sign of the
-  -1 diagonal, a Neumann row without its h scaling, the factor 2 of a ghost
-  inside the object, the sign of the unknown-ghost weights, and (rasterize)
-  the sign of the 0.75 h term;
+  a_n = -b / rho), the ghost values reproduce the field at the points the
+  Laplacian takes them for -- the ghost position, except that a rasterized
+  ghost of the centre-sampled configuration sits on the face but stands for
+  the solid cell's centre (its value p_nbr - h dp/dn_cell extrapolates a
+  whole cell) -- to second order: max error <= 0.06 b h at h = 4 mm
+  (measured 0.0005-0.032) and the error falls by at least 0.65x when h
+  halves (measured 0.24-0.53x).  Planted in the oracle on the CPU, each of these
+  raised the error to 0.48-370 b h: the sign of the image-point correction,
+  the sign of the -1 diagonal, a Neumann row without its h scaling, the
+  factor 2 of a ghost inside the object, the sign of the unknown-ghost
+  weights, and (rasterize) the sign of the 0.75 h term, or h replaced by
+  0.75 h in the centre-sampled rasterized value;
 * a fluid cell whose two neighbours along each of two axes are solid keeps
   p^{n+1} = p^n, while a cell solid on both sides along one axis only does
   not (the cavity needs two such axes)."""
@@ -22,14 +26,15 @@ import pytest
 
 from acoustics_amd import scenes
 from acoustics_amd.capi import (FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_SPHERE,
-                                SceneBuffers)
+                                FDTD_SAMPLE_CENTER, FDTD_SAMPLE_SEVEN, SceneBuffers)
 from oracle.oracle import Oracle
 
 A0, B = 1.0, 50.0
 
 
-def _radial(boundary, h, R, n):
+def _radial(boundary, h, R, n, sampling=FDTD_SAMPLE_CENTER):
     s = scenes.base_settings((n, n, n), h, 6, boundary)
+    s["sampling"] = sampling
     c = np.array([0.0013, -0.0007, 0.0004])   # off the cell lattice
     rho = s["density"]
     obj = dict(type=FDTD_OBJ_SPHERE, center=tuple(c), semi_axes=(R, R, R), rotation=np.eye(3))
@@ -40,17 +45,20 @@ def _radial(boundary, h, R, n):
     idx = np.indices((n, n, n)).reshape(3, -1).T
     x = np.array(s["min_bound"]) + (idx + 0.5) * h
     p = A0 + B * (np.linalg.norm(x - c, axis=1) - R)
-    val, pos, _ = o.ghost_update_at(p, 1e-3)
+    val, pos, nbr = o.ghost_update_at(p, 1e-3)
+    if sampling == FDTD_SAMPLE_CENTER and boundary == FDTD_BOUNDARY_RASTERIZE:
+        pos = 2.0 * pos - x[nbr]     # the solid cell's centre, mirrored from its neighbour's
     exact = A0 + B * (np.linalg.norm(pos - c, axis=1) - R)
     return o, p, np.abs(val - exact).max() / (B * h)
 
 
+@pytest.mark.parametrize("sampling", [FDTD_SAMPLE_CENTER, FDTD_SAMPLE_SEVEN])
 @pytest.mark.parametrize("boundary", [FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE])
-def test_ghost_values_reproduce_radial_field(boundary):
-    o, _, coarse = _radial(boundary, 0.004, 0.04, 48)
+def test_ghost_values_reproduce_radial_field(boundary, sampling):
+    o, _, coarse = _radial(boundary, 0.004, 0.04, 48, sampling)
     if boundary == FDTD_BOUNDARY_FULLY_COUPLED:
         assert o.counts()["n_coupled"] > 0.95 * o.counts()["n_ghost"]
-    _, _, fine = _radial(boundary, 0.002, 0.04, 80)
+    _, _, fine = _radial(boundary, 0.002, 0.04, 80, sampling)
     assert coarse <= 0.06, coarse
     assert fine <= 0.65 * coarse, (coarse, fine)
 
