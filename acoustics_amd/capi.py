@@ -57,13 +57,14 @@ class Object(C.Structure):
     ]
 
 
-FDTD_VIB_GAUSSIAN, FDTD_VIB_HARMONIC = 0, 1
+FDTD_VIB_GAUSSIAN, FDTD_VIB_HARMONIC, FDTD_VIB_CHIRP = 0, 1, 2
 
 
 class ImpulseSource(C.Structure):
     _fields_ = [("object", C.c_int), ("kind", C.c_int), ("amplitude", C.c_double),
                 ("width", C.c_double), ("onset", C.c_double), ("omega", C.c_double), ("phase", C.c_double),
-                ("decay_rate", C.c_double), ("decay_center", C.c_double), ("t0", C.c_double), ("t1", C.c_double)]
+                ("decay_rate", C.c_double), ("decay_center", C.c_double), ("t0", C.c_double), ("t1", C.c_double),
+                ("omega1", C.c_double)]
 
 
 class Impact(C.Structure):
@@ -275,7 +276,13 @@ class SceneBuffers:
             kind = int(s_.get("kind", FDTD_VIB_GAUSSIAN))
             self._isrc[i].object = s_["object"]
             self._isrc[i].kind = kind
-            if kind == FDTD_VIB_HARMONIC:
+            if kind == FDTD_VIB_CHIRP:
+                # dict(object, kind=FDTD_VIB_CHIRP, omega, omega1, t0, t1)
+                self._isrc[i].omega = s_["omega"]
+                self._isrc[i].omega1 = s_["omega1"]
+                self._isrc[i].t0 = s_["t0"]
+                self._isrc[i].t1 = s_["t1"]
+            elif kind == FDTD_VIB_HARMONIC:
                 self._isrc[i].amplitude = s_.get("amplitude", 1.0)
                 self._isrc[i].omega = s_["omega"]
                 self._isrc[i].phase = s_.get("phase", 0.0)

@@ -279,7 +279,8 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     }
     FDTD_HIP_TRY(up(L.psrc, ps.data(), ps.size() * sizeof(PSrcDev)));
     std::vector<ImpDev> is;
-    for (const ImpulseHost &h : S.isrc) is.push_back({h.obj, h.kind, h.A, h.w, h.ton, h.omega, h.phase, h.decay, h.dcenter, h.t0, h.t1});
+    for (const ImpulseHost &h : S.isrc)
+        is.push_back({h.obj, h.kind, h.A, h.w, h.ton, h.omega, h.phase, h.decay, h.dcenter, h.t0, h.t1, h.omega1});
     FDTD_HIP_TRY(up(L.isrc, is.data(), is.size() * sizeof(ImpDev)));
     std::vector<ImpactDev> im;
     for (const ImpactHost &h : S.impacts) im.push_back({h.obj, 0, h.t0, h.tau, {h.v[0], h.v[1], h.v[2]}});

@@ -71,7 +71,8 @@ struct PSrcHost {
 struct ImpulseHost {
     int obj, kind;                 // FDTD_VIB_*
     double A, w, ton;              // Gaussian
-    double omega, phase, decay, dcenter, t0, t1;   // harmonic (t1 <= t0: no range)
+    double omega, phase, decay, dcenter, t0, t1;   // harmonic (t1 <= t0: no range); chirp: omega, t0, t1
+    double omega1;                                 // chirp
 };
 
 // One acceleration-noise impact: a_v(t) += S(t) v over t0 <= t < t0 + tau,

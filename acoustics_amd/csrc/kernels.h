@@ -27,7 +27,8 @@ struct PSrcDev {
 struct ImpDev {
     int obj, kind;
     double A, w, ton;                              // Gaussian
-    double omega, phase, decay, dcenter, t0, t1;   // harmonic
+    double omega, phase, decay, dcenter, t0, t1;   // harmonic; chirp: omega, t0, t1
+    double omega1;                                 // chirp
 };
 
 // acceleration-noise impact: S(t) v over [t0, t0 + tau), world frame

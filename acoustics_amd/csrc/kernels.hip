@@ -48,7 +48,7 @@ __device__ __forceinline__ const AS_CONST T *cmem(const T *p) { return (const AS
 
 // a_s(obj, t): the sources whose acceleration vector is a_n times the normal
 // (Gaussian impulse; HarmonicVibrationalSource::Evaluate,
-// HarmonicVibrationalSource.cpp:5-28)
+// HarmonicVibrationalSource.cpp:5-28; the chirp)
 __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int obj, double t)
 {
     double a = 0.0;
@@ -59,6 +59,13 @@ __device__ __forceinline__ double impulse_accel(const ImpDev *isrc, int n, int o
             if (s.t1 > s.t0 && (t < s.t0 || t > s.t1)) continue;
             const double v = sin(s.omega * t + s.phase) * (s.A * s.omega);
             a += s.decay < 0.0 ? v : exp(-s.decay * (t - s.dcenter)) * v;
+        } else if (s.kind == FDTD_VIB_CHIRP) {
+            // ChirpVibrationalSource::Evaluate (ChirpVibrationalSource.cpp:5-19)
+            if (t < s.t0) continue;
+            const double al = t < s.t1 ? 0.5 * (t - s.t0) / (s.t1 - s.t0) : 1.0;
+            const double ph = al < 0.52 ? 0.0 : s.t1 * 0.5 * (s.omega1 - s.omega);
+            const double om = (1.0 - al) * s.omega + al * s.omega1;
+            a += sin(om * t + ph);
         } else {
             const double u = t - s.ton;
             a += s.A * exp(-(u * u) / (2.0 * s.w * s.w));

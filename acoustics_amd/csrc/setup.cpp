@@ -632,8 +632,14 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
             }
             H.A = is.amplitude; H.omega = is.omega; H.phase = is.phase;
             H.decay = is.decay_rate; H.dcenter = is.decay_center; H.t0 = is.t0; H.t1 = is.t1;
+        } else if (is.kind == FDTD_VIB_CHIRP) {
+            if (!std::isfinite(is.omega) || !std::isfinite(is.omega1) || !(is.t1 > is.t0)) {
+                err = "a chirp source needs finite frequencies and t1 > t0";
+                return FDTD_ERR_INVALID;
+            }
+            H.omega = is.omega; H.omega1 = is.omega1; H.t0 = is.t0; H.t1 = is.t1;
         } else {
-            err = "vibrational source kind must be FDTD_VIB_GAUSSIAN or FDTD_VIB_HARMONIC";
+            err = "vibrational source kind must be FDTD_VIB_GAUSSIAN, FDTD_VIB_HARMONIC or FDTD_VIB_CHIRP";
             return FDTD_ERR_INVALID;
         }
         S.isrc.push_back(H);

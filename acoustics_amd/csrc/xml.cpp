@@ -448,6 +448,20 @@ fdtd_status fdtd_parse_xml(const char *path, fdtd_parsed **out)
                     is.decay_rate = -1.0;
                     P->impulses.push_back(is);
                 }
+                // ImpulseResponseParser.cpp:278-295: omega_i = 2 pi f_i, the
+                // sweep's times; the amplitudes are read there but unused
+                if (opt_real(o, "has_chirp_source", 0.0) != 0.0) {
+                    fdtd_impulse_source is{};
+                    is.object = oi;
+                    is.kind = FDTD_VIB_CHIRP;
+                    is.omega = 2.0 * M_PI * req_real(o, "chirp_frequency_0");
+                    is.omega1 = 2.0 * M_PI * req_real(o, "chirp_frequency_1");
+                    is.t0 = req_real(o, "chirp_time_0");
+                    is.t1 = req_real(o, "chirp_time_1");
+                    (void)req_real(o, "chirp_amplitude_0");
+                    (void)req_real(o, "chirp_amplitude_1");
+                    P->impulses.push_back(is);
+                }
                 // acceleration noise of an analytic object: its impacts listed
                 // inline (the reference reads them from a rigid-body simulator's
                 // impulse file against a mesh); mass, centre of mass and inverse

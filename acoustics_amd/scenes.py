@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from .capi import (FDTD_SCHEME_STAGGERED, FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_ELLIPSOID,
-                   FDTD_OBJ_SPHERE, FDTD_VIB_HARMONIC, SceneBuffers)
+                   FDTD_OBJ_SPHERE, FDTD_VIB_CHIRP, FDTD_VIB_HARMONIC, SceneBuffers)
 
 C_SOUND = 343.0
 RHO = 1.2
@@ -392,7 +392,7 @@ def uniform_solid_inv_inertia(mass, semi_axes):
 
 
 def vibration_scene(seed=17, n=48, h=0.005, pml=8, steps=300, n_obj=2, n_listeners=10, harmonic=True,
-                    acc_noise=True, gaussian=False, boundary=FDTD_BOUNDARY_RASTERIZE):
+                    acc_noise=True, gaussian=False, chirp=False, boundary=FDTD_BOUNDARY_RASTERIZE):
     """Rotated ellipsoids whose surfaces carry harmonic pulsations (seeded
     frequency 1-4 kHz, phase, decay and time range) and acceleration noise
     (three impacts each at seeded off-centre points, overlapping supports of
@@ -412,6 +412,10 @@ def vibration_scene(seed=17, n=48, h=0.005, pml=8, steps=300, n_obj=2, n_listene
                              decay_center=0.3 * T, t0=t0, t1=t0 + 0.6 * T))
         if gaussian:
             isrc.append(dict(object=i, amplitude=1e3, width=1e-4, onset=float(rng.uniform(0, 0.4)) * T))
+        if chirp:
+            c0 = float(rng.uniform(0.05, 0.2)) * T
+            isrc.append(dict(object=i, kind=FDTD_VIB_CHIRP, omega=2 * math.pi * rng.uniform(1e3, 2e3),
+                             omega1=2 * math.pi * rng.uniform(2.5e3, 4e3), t0=c0, t1=c0 + 0.5 * T))
         if acc_noise:
             m = float(rng.uniform(0.05, 0.5))
             imps = []

@@ -171,7 +171,7 @@ typedef struct fdtd_object {
 } fdtd_object;
 
 /* Kinds of fdtd_impulse_source. */
-enum { FDTD_VIB_GAUSSIAN = 0, FDTD_VIB_HARMONIC = 1 };
+enum { FDTD_VIB_GAUSSIAN = 0, FDTD_VIB_HARMONIC = 1, FDTD_VIB_CHIRP = 2 };
 
 /* Vibrational source on the whole surface of one object (VibrationalSource,
  * VibrationalSource.h:17-37) whose acceleration vector at a boundary point is
@@ -182,18 +182,26 @@ enum { FDTD_VIB_GAUSSIAN = 0, FDTD_VIB_HARMONIC = 1 };
  *   exp(-decay_rate (t - decay_center)) when decay_rate >= 0, and zero
  *   outside [t0, t1] when t1 > t0.  A is the amplitude of the pulsation
  *   velocity (the reference fixes it at 1 m/s; XML has_harmonic_source with
- *   harmonic_source_f = omega / 2 pi, phase 0, no decay, no range). */
+ *   harmonic_source_f = omega / 2 pi, phase 0, no decay, no range);
+ *   FDTD_VIB_CHIRP (ChirpVibrationalSource, ChirpVibrationalSource.cpp:5-19):
+ *   zero before t0, else a_n = sin(w t + phi) with alpha = 0.5 (t - t0) /
+ *   (t1 - t0) before t1 and 1 from t1 on, w = (1 - alpha) omega + alpha
+ *   omega1, phi = 0 while alpha < 0.52 and t1 (omega1 - omega) / 2 after
+ *   (the reference's unit amplitude; XML has_chirp_source with
+ *   chirp_frequency_0/1 = omega / 2 pi, omega1 / 2 pi, chirp_time_0/1). */
 typedef struct fdtd_impulse_source {
     int    object;        /* index into fdtd_scene.objects               */
-    int    kind;          /* FDTD_VIB_GAUSSIAN | FDTD_VIB_HARMONIC       */
+    int    kind;          /* FDTD_VIB_GAUSSIAN | _HARMONIC | _CHIRP      */
     double amplitude;     /* Gaussian: A (m/s^2); harmonic: A (m/s)      */
     double width;         /* Gaussian: w (s)                             */
     double onset;         /* Gaussian: t_on (s)                          */
-    double omega;         /* harmonic: angular frequency (rad/s)         */
+    double omega;         /* harmonic, chirp: angular frequency (rad/s)  */
     double phase;         /* harmonic: phase (rad)                       */
     double decay_rate;    /* harmonic: < 0 none (1/s)                    */
     double decay_center;  /* harmonic (s)                                */
-    double t0, t1;        /* harmonic: active range when t1 > t0 (s)     */
+    double t0, t1;        /* harmonic: active range when t1 > t0 (s);
+                             chirp: the sweep's start and end (s)        */
+    double omega1;        /* chirp: final angular frequency (rad/s)      */
 } fdtd_impulse_source;
 
 /* One impact of an acceleration-noise source (ImpulseSeriesObject::

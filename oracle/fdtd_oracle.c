@@ -857,6 +857,20 @@ static double harmonic_an(const fdtd_impulse_source *s, double t)
     return exp(-s->decay_rate * (t - s->decay_center)) * val;
 }
 
+/* ChirpVibrationalSource::Evaluate (ChirpVibrationalSource.cpp:5-19): zero
+ * before _time[0]; alpha runs to 0.5 at _time[1] and is 1 after; the phase
+ * jumps once alpha reaches 0.52; unit amplitude */
+static double chirp_an(const fdtd_impulse_source *s, double t)
+{
+    const double time0 = s->t0, time1 = s->t1, omega0 = s->omega, omega1 = s->omega1;
+    if (t < time0) return 0.0;
+    const double alpha = (t < time1 ? 0.5 * (t - time0) / (time1 - time0) : 1.0);
+    const double phase = (alpha < 0.52 ? 0 : (time1) * 0.5 * (omega1 - omega0));
+    const double omega = (1.0 - alpha) * omega0 + alpha * omega1;
+    const double amp = 1.0;
+    return sin(omega * t + phase) * amp;
+}
+
 /* AccelerationNoiseVibrationalSource::Evaluate(vertexID, time)
  * (AccelerationNoiseVibrationalSource.cpp:63-90) over the impacts
  * ImpulseSeriesObject::GetImpulseWithinSupport returns (t_0 <= t < t_0 + tau,
@@ -906,6 +920,8 @@ static void object_accel(const oracle *o, int obj, double t, const double *nrm, 
         if (s->object != obj) continue;
         if (s->kind == FDTD_VIB_HARMONIC) {
             a += harmonic_an(s, t);
+        } else if (s->kind == FDTD_VIB_CHIRP) {
+            a += chirp_an(s, t);
         } else {
             const double u = t - s->onset;
             a += s->amplitude * exp(-(u * u) / (2.0 * s->width * s->width));

@@ -149,7 +149,9 @@ def test_xml_harmonic_and_acc_noise_sources(tmp_path):
     whose inverse inertia defaults to the uniform solid of the object's shape."""
     lib = capi.load_library()
     src = open(os.path.join(ROOT, "tests", "golden", "scene_sphere.xml")).read()
-    src = src.replace('radius="0.025">', 'radius="0.025" has_harmonic_source="1" harmonic_source_f="1500">\n'
+    src = src.replace('radius="0.025">', 'radius="0.025" has_harmonic_source="1" harmonic_source_f="1500" '
+                      'has_chirp_source="1" chirp_frequency_0="1000" chirp_frequency_1="3000" chirp_time_0="1e-4" '
+                      'chirp_time_1="9e-4" chirp_amplitude_0="1" chirp_amplitude_1="1">\n'
                       '      <acc_noise_source mass="0.2" center_of_mass_z="0.001">\n'
                       '        <impact timestamp="1e-4" support="2e-4" impulse_x="0.001" impulse_y="0" impulse_z="-0.002"'
                       ' position_x="0.025"/>\n'
@@ -161,7 +163,10 @@ def test_xml_harmonic_and_acc_noise_sources(tmp_path):
     assert lib.fdtd_parse_xml(str(path).encode(), C.byref(p)) == 0, capi.last_error()
     sc = lib.fdtd_parsed_scene(p).contents
     kinds = sorted(sc.impulse_sources[i].kind for i in range(sc.n_impulse_sources))
-    assert kinds == [capi.FDTD_VIB_GAUSSIAN, capi.FDTD_VIB_HARMONIC]
+    assert kinds == [capi.FDTD_VIB_GAUSSIAN, capi.FDTD_VIB_HARMONIC, capi.FDTD_VIB_CHIRP]
+    ch = [sc.impulse_sources[i] for i in range(sc.n_impulse_sources) if sc.impulse_sources[i].kind == capi.FDTD_VIB_CHIRP][0]
+    assert abs(ch.omega - 2 * np.pi * 1000) < 1e-9 and abs(ch.omega1 - 2 * np.pi * 3000) < 1e-9
+    assert ch.t0 == 1e-4 and ch.t1 == 9e-4
     hs = [sc.impulse_sources[i] for i in range(sc.n_impulse_sources) if sc.impulse_sources[i].kind == capi.FDTD_VIB_HARMONIC][0]
     assert hs.object == 0 and abs(hs.omega - 2 * np.pi * 1500) < 1e-9 and hs.phase == 0 and hs.amplitude == 1.0
     assert hs.decay_rate < 0

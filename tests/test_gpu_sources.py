@@ -23,6 +23,7 @@ CASES = {
     "harmonic": dict(acc_noise=False),
     "acc_noise": dict(harmonic=False),
     "all_kinds": dict(gaussian=True, seed=18, n_obj=3),
+    "chirp": dict(harmonic=False, acc_noise=False, chirp=True, seed=26),
 }
 
 

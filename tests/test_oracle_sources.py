@@ -5,7 +5,9 @@
 * a sphere pulsating with normal acceleration a_n(t) radiates exactly
   p(r, t) = F(t - (r - a)/c) / r with F' + (c/a) F = rho c a a_n(t), F(0) = 0
   (the radial momentum equation at r = a).  The harmonic source at 2 kHz,
-  and a decayed and time-windowed one, match it in the oracle to within the
+  a decayed and time-windowed one, and the chirp (ChirpVibrationalSource.cpp:
+  5-19: its sweep, phase jump and final frequency) match it in the oracle to
+  within the
   grid's discretisation error (measured 0.35% rasterised and 2.2% fully
   coupled at h = 5 mm with centre sampling, the default; 4.0% and 5.6% with
   seven samples, falling to 4.1% fully coupled at h = 3.5 mm; bound 5%);
@@ -35,7 +37,7 @@ from scipy.integrate import solve_ivp
 
 from acoustics_amd import scenes
 from acoustics_amd.capi import (FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE, FDTD_OBJ_SPHERE,
-                                FDTD_VIB_HARMONIC, SceneBuffers)
+                                FDTD_VIB_CHIRP, FDTD_VIB_HARMONIC, SceneBuffers)
 from oracle.oracle import Oracle
 
 N, H, PML, STEPS, RADIUS = 72, 0.005, 10, 600, 0.04
@@ -106,6 +108,25 @@ def test_decayed_windowed_harmonic_matches_closed_form():
     def an(t):
         v = 0.7 * om * np.sin(om * t - om * t0) * np.exp(-lam * (t - tc))
         return np.where((t >= t0) & (t <= t1), v, 0.0)
+    ref = _monopole(an, sb.listeners_np, s, STEPS)
+    err = _rel(got, ref)
+    assert err < 0.05, err
+
+
+def test_chirp_pulsating_sphere_matches_closed_form():
+    """The chirp (ChirpVibrationalSource.cpp:5-19) through the same closed
+    form: its frequency sweep, the alpha = 0.5 end of the sweep, the phase
+    jump and the switch to omega1 after t1."""
+    w0, w1, t0, t1 = 2 * math.pi * 1500.0, 2 * math.pi * 3500.0, 2e-4, 1.4e-3
+    src = dict(object=0, kind=FDTD_VIB_CHIRP, omega=w0, omega1=w1, t0=t0, t1=t1)
+    sb, s = _sphere_scene(FDTD_BOUNDARY_RASTERIZE, impulse_sources=[src], seed=3)
+    got = Oracle(sb, reclassify_each_step=False).step(STEPS)
+
+    def an(t):
+        al = np.where(t < t1, 0.5 * (t - t0) / (t1 - t0), 1.0)
+        ph = np.where(al < 0.52, 0.0, t1 * 0.5 * (w1 - w0))
+        om = (1 - al) * w0 + al * w1
+        return np.where(t < t0, 0.0, np.sin(om * t + ph))
     ref = _monopole(an, sb.listeners_np, s, STEPS)
     err = _rel(got, ref)
     assert err < 0.05, err

@@ -88,7 +88,7 @@ def test_engquist_boundary_refusals():
     s = dict(sp.settings_dict, use_mesh=0)
     assert status(s, sp) == (6, b"use_mesh=0 classifies no object cells; objects need use_mesh=1")
     o = dict(sp.objects_list[0])
-    o["center"] = tuple(np.array(sp.settings_dict["min_bound"]) + np.array([0.025 + 2.5 * h, 0.11, 0.11]))
+    o["center"] = tuple(np.array(sp.settings_dict["min_bound"]) + np.array([0.025 + 2.2 * h, 0.11, 0.11]))
     st, msg = status(sp.settings_dict, sp, objects=[o], listeners=np.zeros((0, 3)))
     assert st == 6 and b"three cells" in msg
     mb = np.array(ff.settings_dict["min_bound"])
