@@ -4,9 +4,10 @@
       one plain device-to-device copy of known size (read N bytes, write N
       bytes) per iteration: the FETCH_SIZE / WRITE_SIZE counters of this
       kernel calibrate the counters' units against known bytes.
-  python -m acoustics_amd.profiling step --n 256 --steps 20 [--tile T]
+  python -m acoustics_amd.profiling step --n 256 --steps 20 [--tile T] [--boundary coupled]
       the headline 256^3 scene stepped eagerly (one dispatch per kernel per
-      step) for per-dispatch counters of the step kernels.
+      step) for per-dispatch counters of the step kernels; with coupled
+      ghost cells, the same scene with the fully-coupled boundary.
   python -m acoustics_amd.profiling traffic --step-fetch D1 --step-write D2 \\
           --copy-fetch D3 --copy-write D4 --mib 1024 --out profiles/X.json
       reads the counter CSVs of the four --pmc runs above (rocprofv3
@@ -44,13 +45,15 @@ def copy(mib, iters):
     print(f"copy: {iters} x {mib} MiB read + {mib} MiB written", flush=True)
 
 
-def step(n, steps, tile, scheme="collocated"):
+def step(n, steps, tile, scheme="collocated", boundary="rasterize"):
     from . import scenes
+    from .capi import FDTD_BOUNDARY_FULLY_COUPLED, FDTD_BOUNDARY_RASTERIZE
     from .solver import Solver
     if scheme == "staggered":
         sb, _ = scenes.staggered_free_field(seed=7, div=(n, n, n), h=0.64 / n, pml=16, n_listeners=8)
     else:
-        sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n)
+        bc = FDTD_BOUNDARY_FULLY_COUPLED if boundary == "coupled" else FDTD_BOUNDARY_RASTERIZE
+        sb, _ = scenes.config3(seed=3, n=n, h=0.64 / n, boundary=bc)
     with Solver(sb, tile_config=tile) as s:
         # eager (one dispatch per kernel per step, one advance per 64 steps),
         # no timing events between the kernels
@@ -62,7 +65,8 @@ def step(n, steps, tile, scheme="collocated"):
 
 
 def _short(name):
-    for k in list(STEP_KERNELS) + ["advance_kernel", "ghost_kernel", "copyBuffer", "fillBuffer", "vectorized_elementwise"]:
+    for k in list(STEP_KERNELS) + ["advance_kernel", "motion_classify_kernel", "motion_fresh_kernel", "copyBuffer",
+                                   "fillBuffer", "vectorized_elementwise"]:
         if k in name:
             return k
     return name[:40]
@@ -125,6 +129,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--scheme", choices=["collocated", "staggered"], default="collocated")
+    ap.add_argument("--boundary", choices=["rasterize", "coupled"], default="rasterize")
     ap.add_argument("--step-fetch")
     ap.add_argument("--step-write")
     ap.add_argument("--copy-fetch")
@@ -136,7 +141,7 @@ def main(argv=None):
     if a.what == "copy":
         copy(a.mib, a.iters)
     elif a.what == "step":
-        step(a.n, a.steps, a.tile, a.scheme)
+        step(a.n, a.steps, a.tile, a.scheme, a.boundary)
     elif a.what == "traffic":
         traffic(a)
     else:
