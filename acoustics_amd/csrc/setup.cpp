@@ -383,8 +383,8 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
         return FDTD_ERR_UNSUPPORTED;
     }
     if (st.scheme == FDTD_SCHEME_STAGGERED && (sc.n_objects > 0 || sc.n_impulse_sources > 0 || sc.n_acc_noise_sources > 0)) {
-        err = "the staggered scheme runs free-field scenes only (objects need the reference's mesh distance-field "
-              "classification of the MAC mode); use the collocated scheme";
+        err = "the staggered scheme runs free-field scenes only (the reference's staggered object path mixes two "
+              "cell masks and a stubbed ghost lookup); use the collocated scheme, the reference's default";
         return FDTD_ERR_UNSUPPORTED;
     }
     const double h = S.h, dt = S.dt, rho = S.rho, c = S.c;
