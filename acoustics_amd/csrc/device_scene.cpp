@@ -134,7 +134,7 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
     L.ucls = c.take<uint8_t>(2 * moving_cells(S));
     L.mobj = c.take<MoveDev>(S.moving ? S.objs.size() : 0);
     L.kin = c.take<double>(S.kin.size());
-    L.pose = c.take<Geom>(S.moving ? S.objs.size() : 0);
+    L.pose = c.take<Geom>(S.moving ? (size_t)AMP_ROWS * S.objs.size() : 0);
     L.ughost = c.take<double>(6 * moving_cells(S));
     L.self = c.take<SceneDev>(1);
 }

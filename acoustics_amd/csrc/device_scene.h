@@ -76,7 +76,7 @@ struct DeviceScene {
     // puts the scene back at step 0: fields at rest, empty series ring, clock
     // and per-step amplitudes of step 0 (synchronises st)
     fdtd_status reset_to_start(hipStream_t st);
-    // refills the amplitude ring for the AMP_ROWS steps from the device clock
+    // refills the amplitude (and pose) ring for the AMP_ROWS steps from the device clock
     hipError_t fill_amplitudes(hipStream_t st) const { return launch_advance(self, 1, 0, st); }
 
     // Appends this scene's tiles, tagged with scene_id, and the listener

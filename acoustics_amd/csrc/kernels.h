@@ -137,13 +137,14 @@ struct SceneDev {
     float *spk_field;           // [sp_n0 sp_n1 sp_n2]: the speakers' S dt of the step being run
     // moving objects: the box of cells reclassified every step, their bulk
     // flags by step parity (row s & 1 holds step s's classification), the
-    // objects' motion and their pose at the step being run
+    // objects' motion and a ring of their poses (row amp_row(s) holds the
+    // poses at t_s; the advance kernel refills it with the amplitudes)
     int mv;
     int u_lo[3], u_n[3];
     uint8_t *ucls;              // [2][u_n0 u_n1 u_n2], z fastest: 1 bulk, 0 solid
     const MoveDev *mobj;        // [n_obj]
     const double *kin;          // keyframes, 8 doubles each
-    Geom *pose;                 // [n_obj], written by motion_classify_kernel each step
+    Geom *pose;                 // [AMP_ROWS][n_obj]
     double *ughost;             // [6][u cells]: ghost term of each solid neighbour of a fluid cell (-x, +x, ..., +z)
     double h, rho, bmin[3];
 };
@@ -196,13 +197,14 @@ enum : int { KIND_INTERIOR = 0, KIND_PML = 1, KIND_SPECIAL = 2, NUM_KINDS = 3, K
 hipError_t launch_step(int kind, const SceneDev *scenes, const TileDesc *tiles, int ntiles, int nhead, const TermRef *terms,
                        const unsigned long long *order_clock, int k, hipStream_t s);
 // Moves every scene of the launch n steps on (clamped to its target) and
-// refills its amplitude ring for the AMP_ROWS steps from there; n = 0 only
-// fills the ring.
+// refills its amplitude and pose rings for the AMP_ROWS steps from there;
+// n = 0 only fills the rings.
 hipError_t launch_advance(const SceneDev *scenes, int nscenes, int n, hipStream_t s);
-// Moving objects, before the step kernels of step clock + k: poses, per-step
-// reclassification of each scene's box, zeroed new solid cells and filled
-// fresh cells; workgroups of 256 cells along x (max_cells: the largest box of
-// the list), one scene per y.
+// Moving objects, before the step kernels of step clock + k, in one launch:
+// per-step reclassification of each scene's box, zeroed new solid cells,
+// filled fresh cells and the ghost terms of the box's fluid cells; six
+// threads per cell along x (max_cells: the largest box of the list), one
+// scene per y.
 hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s);
 // Speaker sources, before the step kernels of step clock + k: the speakers'
 // S dt over each listed scene's speaker box.

@@ -1041,34 +1041,44 @@ __device__ inline int mv_uidx(const SceneDev &S, int i, int j, int k)
 }
 
 // LowestObjectDistance (FDTD_Objects.cpp:240-257): the object nearest to x by |distance|
-__device__ inline int mv_nearest(const SceneDev &S, const double *x, double *dist)
+__device__ inline int mv_nearest(const SceneDev &S, const Geom *pose, const double *x, double *dist)
 {
     if (S.n_obj == 1) { *dist = 0.0; return 0; }   // the only object (the distance is not used then)
     double best = DBL_MAX;
     int id = 0;
     for (int o = 0; o < S.n_obj; ++o) {
         double xb[3], n[3];
-        const double d = geom_closest(S.pose[o], x, xb, n);
+        const double d = geom_closest(pose[o], x, xb, n);
         if (fabs(d) < fabs(best)) { best = d; id = o; }
     }
     *dist = best;
     return id;
 }
 
+// this step's classification of cell q (cells outside the box are fluid):
+// the same function of the poses the box's own threads evaluate for their
+// cells, so a thread recomputes it for the neighbours it reads
+__device__ inline bool mv_fluid_now(const SceneDev &S, const Geom *pose, const int *q)
+{
+    if (mv_uidx(S, q[0], q[1], q[2]) < 0) return true;
+    double x[3];
+    mv_centre(S, q[0], q[1], q[2], x);
+    return mv_fluid_at(S, pose, x);
+}
+
 // fresh values of cell (i, j, k) in both pressure slots (p^{s-1} in plast,
-// p^s in pcur; one geometry and one set of MLS weights for both): the cells
-// of the box hold this step's classification in now_f and the previous
-// step's in prev_f (cells outside the box are fluid in both)
-__device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcur, const uint8_t *now_f,
+// p^s in pcur; one geometry and one set of MLS weights for both); prev_f is
+// the box's classification of the previous step
+__device__ void mv_fresh(const SceneDev &S, const Geom *pose, const float *plast, const float *pcur,
                          const uint8_t *prev_f, int i, int j, int k, long long step, double *vl, double *vc)
 {
     double x[3];
     mv_centre(S, i, j, k, x);
     double dist;
-    const int id = mv_nearest(S, x, &dist);
+    const int id = mv_nearest(S, pose, x, &dist);
     // ReflectAgainstBoundary (FDTD_RigidObject.cpp:429-536) for a point outside
     double xb[3], nb[3], en[3], image[3];
-    const double travelled = geom_closest(S.pose[id], x, xb, nb);
+    const double travelled = geom_closest(pose[id], x, xb, nb);
     if (travelled < 0.0) { *vl = *vc = __builtin_nan(""); return; }   // the reference throws: a fresh cell inside an object
     const double dd = sqrt((xb[0] - x[0]) * (xb[0] - x[0]) + (xb[1] - x[1]) * (xb[1] - x[1]) + (xb[2] - x[2]) * (xb[2] - x[2]));
     if (dd > 1e-10) for (int e = 0; e < 3; ++e) { en[e] = x[e] - xb[e]; image[e] = x[e] + en[e]; }
@@ -1076,7 +1086,8 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
     const double enn = sqrt(en[0] * en[0] + en[1] * en[1] + en[2] * en[2]);
     for (int e = 0; e < 3; ++e) en[e] /= enn;
     // enclosing cells of the image point (ScalarField::enclosingNeighbours), in
-    // flat-index order without repeats
+    // flat-index order without repeats, those with valid history; each adds
+    // its row to the MLS sums and competes for the nearest-neighbour fallback
     int lo3[3], hi3[3];
     for (int d = 0; d < 3; ++d) {
         double diff = image[d] - S.bmin[d];
@@ -1086,44 +1097,36 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
         lo3[d] = v0;
         hi3[d] = diff < 0.0 ? 0 : min(v0 + 1, S.n[d] - 1);
     }
-    double pts[24], vl8[8], vc8[8];
-    int nv = 0;
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int c = 0; c < 2; ++c) {
-                const int q[3] = {a ? hi3[0] : lo3[0], b ? hi3[1] : lo3[1], c ? hi3[2] : lo3[2]};
-                if ((a && hi3[0] == lo3[0]) || (b && hi3[1] == lo3[1]) || (c && hi3[2] == lo3[2])) continue;
-                const int u = mv_uidx(S, q[0], q[1], q[2]);
-                if (u >= 0 && !prev_f[u]) continue;   // no valid history
-                mv_centre(S, q[0], q[1], q[2], &pts[3 * nv]);
-                // a cell that turned solid this step was zeroed
-                const bool fluid = u < 0 || now_f[u];
-                const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
-                vl8[nv] = fluid ? (double)plast[qi] : 0.0;
-                vc8[nv] = fluid ? (double)pcur[qi] : 0.0;
-                ++nv;
-            }
-    if (nv >= 4) {
+    MlsSums acc;
+    double bd = DBL_MAX, nl = 0.0, nc = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < 8; ++sl) {
+        const int a = sl >> 2, b = (sl >> 1) & 1, c = sl & 1;
+        const int q[3] = {a ? hi3[0] : lo3[0], b ? hi3[1] : lo3[1], c ? hi3[2] : lo3[2]};
+        if ((a && hi3[0] == lo3[0]) || (b && hi3[1] == lo3[1]) || (c && hi3[2] == lo3[2])) continue;
+        const int u = mv_uidx(S, q[0], q[1], q[2]);
+        if (u >= 0 && !prev_f[u]) continue;   // no valid history
+        double y[3];
+        mv_centre(S, q[0], q[1], q[2], y);
+        // a cell that turned solid this step was zeroed
+        const bool fluid = mv_fluid_now(S, pose, q);
+        const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
+        const double l = fluid ? (double)plast[qi] : 0.0, cv = fluid ? (double)pcur[qi] : 0.0;
+        const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
+        if (d2 < bd) { bd = d2; nl = l; nc = cv; }
+        acc.add(y, image, S.h, 0.018, l, cv);
+    }
+    if (acc.m >= 4) {
         // MLS at the image point plus rho a_n |d|, a_n along the erected normal
-        double beta[8];
-        mls_weights(nv, pts, image, S.h, 0.018, beta);
-        double ml = 0.0, mc = 0.0;
-        for (int r = 0; r < nv; ++r) { ml += beta[r] * vl8[r]; mc += beta[r] * vc8[r]; }
+        double ml, mc;
+        acc.solve(&ml, &mc);
         const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
         const double add = S.rho * a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]) * fabs(travelled);
         *vl = ml + add;
         *vc = mc + add;
         return;
     }
-    if (nv > 0) {
-        double bd = DBL_MAX;
-        for (int r = 0; r < nv; ++r) {
-            const double *y = &pts[3 * r];
-            const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
-            if (d2 < bd) { bd = d2; *vl = vl8[r]; *vc = vc8[r]; }
-        }
-        return;
-    }
+    if (acc.m > 0) { *vl = nl; *vc = nc; return; }
     // no valid enclosing cell: the first fluid axis neighbour (+x, -x, +y, ...);
     // a neighbour that is itself fresh this step counts with the zero it held
     const int ijk[3] = {i, j, k};
@@ -1132,7 +1135,7 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
             int q[3] = {ijk[0], ijk[1], ijk[2]};
             q[d] += sg;
             const int u = mv_uidx(S, q[0], q[1], q[2]);
-            if (u >= 0 && !now_f[u]) continue;
+            if (u >= 0 && !mv_fluid_now(S, pose, q)) continue;
             if (u >= 0 && !prev_f[u]) { *vl = *vc = 0.0; return; }
             const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
             *vl = plast[qi];
@@ -1142,71 +1145,55 @@ __device__ void mv_fresh(const SceneDev &S, const float *plast, const float *pcu
     *vl = *vc = 0.0;
 }
 
-// Moving objects, before the step kernels of step s, in two launches over
-// the box of each moving scene (workgroups along x, scenes along y):
-// motion_classify_kernel takes the objects' poses at t_s (into LDS, and into
-// the scene's pose array for the later kernels) and writes the box's bulk
-// flags by the 7-sample rule into the row of this step's parity;
-// motion_fresh_kernel then zeroes the pressures (both slots, as
-// classifyCells_FAST zeroes every pressure buffer) of cells that turned solid,
-// fills fresh cells (p^{s-1}, then p^s) and writes the rasterized ghost term
-// of every solid neighbour of a fluid cell, six threads per cell.  No thread of the second launch
-// reads a cell another thread writes: fresh values come from cells with valid
-// history that are still fluid, and cells that turned solid count as zero
-// without being read.
-__global__ __launch_bounds__(256) void motion_classify_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
+// Moving objects, before the step kernels of step s, in one launch over the
+// box of each moving scene (six threads per cell along x, scenes along y).
+// Each workgroup takes the objects' poses at t_s from the pose ring into LDS.
+// Thread (q, c), q = 0..5 the directions -x, +x, -y, +y, -z, +z:
+// * q = 0 writes cell c's classification (the seven-sample or centre rule)
+//   into the row of this step's parity, zeroes the pressures (both slots, as
+//   classifyCells_FAST zeroes every pressure buffer) of a cell that turned
+//   solid and fills a fresh cell (p^{s-1}, then p^s);
+// * every q of a fluid cell writes the rasterized ghost term of its solid
+//   neighbour in direction q.
+// Classifications of other cells are recomputed from the poses, not read, so
+// no thread depends on another's writes: fresh values come from cells with
+// valid history that are still fluid, and cells that turned solid count as
+// zero without being read.  The step kernels after this launch read the row.
+__global__ __launch_bounds__(256) void motion_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
 {
     const SceneDev &S = scenes[list[blockIdx.y].scene];
     const long long step = (long long)S.clock[CLK_STEP] + k;
     if (step >= (long long)S.clock[CLK_TARGET]) return;
     const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
-    if ((int)blockIdx.x * (int)blockDim.x >= ncell) return;
+    if ((int)blockIdx.x * (int)blockDim.x >= 6 * ncell) return;
     __shared__ Geom pose[MAX_MOVING_OBJ];
-    const double t = S.t_start + (double)step * S.dt;
-    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) {
-        const MoveDev &m = S.mobj[o];
-        const Geom g = m.kin_n > 0 ? geom_keyframe(m.g0, S.kin + m.kin_off, m.kin_n, m.kin_dt, t) : geom_at(m.g0, m.v, m.w, t);
-        pose[o] = g;
-        if (blockIdx.x == 0) S.pose[o] = g;
-    }
+    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) pose[o] = S.pose[(size_t)amp_row(step) * S.n_obj + o];
     __syncthreads();
-    const int c = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-    if (c >= ncell) return;
-    const int kk = c % S.u_n[2], r = c / S.u_n[2];
-    const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1];
-    double x[3];
-    mv_centre(S, i, j, S.u_lo[2] + kk, x);
-    S.ucls[(size_t)(step & 1) * ncell + c] = mv_fluid_at(S, pose, x) ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__restrict__ scenes, const MotionDesc *__restrict__ list, int k)
-{
-    const SceneDev &S = scenes[list[blockIdx.y].scene];
-    const long long step = (long long)S.clock[CLK_STEP] + k;
-    if (step >= (long long)S.clock[CLK_TARGET]) return;
-    const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
     const int tid = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
     if (tid >= 6 * ncell) return;
-    // thread (q, c): direction q (-x, +x, -y, +y, -z, +z) of cell c; q = 0
-    // also does the cell's fresh fill or zeroing
     const int q = tid / ncell, c = tid - q * ncell;
-    const uint8_t *now_f = S.ucls + (size_t)(step & 1) * ncell;
+    uint8_t *now_row = S.ucls + (size_t)(step & 1) * ncell;
     const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
-    const bool now = now_f[c], prev = prev_f[c];
     const int kk = c % S.u_n[2], r = c / S.u_n[2];
     const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
-    if (q == 0 && now != prev) {
-        const int idx = i * S.sx + j * S.sy + kz;
-        float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
-        float *plast = S.field + (size_t)slot_next(step) * S.npad;
-        if (!now) {
-            pcur[idx] = 0.f;
-            plast[idx] = 0.f;
-        } else {
-            double fl, fc;
-            mv_fresh(S, plast, pcur, now_f, prev_f, i, j, kz, step, &fl, &fc);
-            plast[idx] = (float)fl;
-            pcur[idx] = (float)fc;
+    double x[3];
+    mv_centre(S, i, j, kz, x);
+    const bool now = mv_fluid_at(S, pose, x);
+    if (q == 0) {
+        now_row[c] = now ? 1 : 0;
+        if (now != (bool)prev_f[c]) {
+            const int idx = i * S.sx + j * S.sy + kz;
+            float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
+            float *plast = S.field + (size_t)slot_next(step) * S.npad;
+            if (!now) {
+                pcur[idx] = 0.f;
+                plast[idx] = 0.f;
+            } else {
+                double fl, fc;
+                mv_fresh(S, pose, plast, pcur, prev_f, i, j, kz, step, &fl, &fc);
+                plast[idx] = (float)fl;
+                pcur[idx] = (float)fc;
+            }
         }
     }
     // rasterized ghost term of a fluid cell's solid neighbour in direction q:
@@ -1218,21 +1205,20 @@ __global__ __launch_bounds__(256) void motion_fresh_kernel(const SceneDev *__res
     const int d = q >> 1, sg = (q & 1) ? 1 : -1;
     int m[3] = {i, j, kz};
     m[d] += sg;
-    const int um = mv_uidx(S, m[0], m[1], m[2]);
-    if (um < 0 || now_f[um]) return;
+    if (mv_uidx(S, m[0], m[1], m[2]) < 0 || mv_fluid_now(S, pose, m)) return;
     double xo[3], gp[3], dist, xb[3], nb[3];
     mv_centre(S, m[0], m[1], m[2], xo);
-    const int id = mv_nearest(S, xo, &dist);
+    const int id = mv_nearest(S, pose, xo, &dist);
     for (int e = 0; e < 3; ++e) gp[e] = xo[e];
     gp[d] -= sg * (S.seven ? 0.25 : 0.5) * S.h;
-    geom_closest(S.pose[id], gp, xb, nb);
+    geom_closest(pose[id], gp, xb, nb);
     const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
     S.ughost[(size_t)q * ncell + c] = (S.seven ? 0.75 : 1.0) * S.h * S.rho * a * nb[d] * -(double)sg;
 }
 
 // A box of a moving scene's cells (one thread per cell): solid cells are
 // left alone; a fluid cell takes the ghost-aware Laplacian (MAC_Grid.cpp:
-// 187-299) with rasterized ghost values p + (the ghost term motion_fresh_kernel
+// 187-299) with rasterized ghost values p + (the ghost term motion_kernel
 // wrote for that direction) for its solid neighbours, and a cell with ghosts
 // on both sides along two axes keeps p^n when it was fluid in the previous
 // step.
@@ -1300,9 +1286,10 @@ __device__ __forceinline__ int tile_of(int w, int ntiles, int nhead, const unsig
 
 // Step advance (PML_WaveSolver::stepSystem's time update), one workgroup per
 // scene after a block of steps: the counter moves n steps on (not past the
-// target), and the amplitude ring is refilled for the AMP_ROWS steps from
-// there.  The kernel boundaries order it after every read of the block's
-// amplitudes and counter and before the next block reads them.
+// target), and the amplitude ring (and a moving scene's pose ring) is
+// refilled for the AMP_ROWS steps from there.  The kernel boundaries order it
+// after every read of the block's amplitudes, poses and counter and before
+// the next block reads them.
 __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict__ scenes, int n)
 {
     const SceneDev &S = scenes[blockIdx.x];
@@ -1320,6 +1307,15 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
         if (s < off) a = comp == 0 ? impulse_accel(S.isrc, S.n_isrc, o, t) : noise_accel(S.impacts, S.n_impact, o, comp - 1, t);
         gmem(S.acc_tab)[amp_row(s) * S.n_ch + ch] = a;
     }
+    if (S.mv)
+        for (int q = threadIdx.x; q < AMP_ROWS * S.n_obj; q += blockDim.x) {
+            const long long s = base + q / S.n_obj;
+            const int o = q % S.n_obj;
+            const double t = S.t_start + (double)s * S.dt;
+            const MoveDev &m = S.mobj[o];
+            gmem(S.pose)[amp_row(s) * S.n_obj + o] =
+                m.kin_n > 0 ? geom_keyframe(m.g0, S.kin + m.kin_off, m.kin_n, m.kin_dt, t) : geom_at(m.g0, m.v, m.w, t);
+        }
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_psrc; q += blockDim.x) {
         const long long s = base + q / S.n_psrc;
         const int p = q % S.n_psrc;
@@ -1417,10 +1413,8 @@ hipError_t launch_speaker(const SceneDev *scenes, const MotionDesc *list, int n,
 hipError_t launch_motion(const SceneDev *scenes, const MotionDesc *list, int n, int max_cells, int k, hipStream_t s)
 {
     if (n <= 0 || max_cells <= 0) return hipSuccess;
-    const dim3 grid((unsigned)((max_cells + 255) / 256), (unsigned)n);
     const dim3 grid6((unsigned)((6 * max_cells + 255) / 256), (unsigned)n);
-    hipLaunchKernelGGL(motion_classify_kernel, grid, dim3(256), 0, s, scenes, list, k);
-    hipLaunchKernelGGL(motion_fresh_kernel, grid6, dim3(256), 0, s, scenes, list, k);
+    hipLaunchKernelGGL(motion_kernel, grid6, dim3(256), 0, s, scenes, list, k);
     return hipGetLastError();
 }
 

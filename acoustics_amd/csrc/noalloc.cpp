@@ -112,7 +112,8 @@ extern "C" fdtd_status fdtd_solve_noalloc(const fdtd_scene *scene, int num_steps
     if (st != FDTD_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     // place() zeroes the fields and the term ring and writes every table,
-    // the clock and the amplitude ring over the previous call's contents
+    // the clock and the amplitude and pose rings over the previous call's
+    // contents
     st = D.place(s, cap, LLONG_MAX, (char *)fields, base);
     if (st != FDTD_OK) return st;
     StepProgram prog;

@@ -1083,7 +1083,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                         for (int q = 0; q < 8; ++q) s += V[a * 8 + q] * V[b * 8 + q];
                         M[a * 8 + b] = s;
                     }
-                const double cond = gram_pinv(8, M, 8.0, Mp);
+                const double cond = gram_pinv<8>(M, 8.0, Mp);
                 for (int a = 0; a < 8; ++a) {
                     double s = 0;
                     for (int q = 0; q < 8; ++q) s += V[a * 8 + q] * v[q];
@@ -1272,7 +1272,7 @@ fdtd_status build_setup(const fdtd_scene &sc, SceneSetup &S, std::string &err)
                     for (int i = 0; i < m; ++i) s += A[4 * i + a] * A[4 * i + b];
                     Gm[4 * a + b] = s;
                 }
-            gram_pinv(4, Gm, (double)std::min(m, 4), Gp);
+            gram_pinv<4>(Gm, (double)std::min(m, 4), Gp);
             for (int i = 0; i < m; ++i) {
                 double s = 0;
                 for (int b = 0; b < 4; ++b) s += Gp[b] * A[4 * i + b];  // row 0 of G^+ A^T
