@@ -674,7 +674,7 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
         if (t.tb < 0 || t.te < t.tb || (size_t)t.te > nterms) return "tile term range out of range";
         const bool pml_class = t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC;
         const bool moving = t.type == TILE_MOVING;
-        const bool ok_kind = moving ? kind == KIND_SPECIAL
+        const bool ok_kind = moving ? (kind == KIND_SPECIAL || kind == KIND_ALL)
                                     : (kind == KIND_ALL || (kind == KIND_PML && pml_class) ||
                                        (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL));
         if ((t.type == TILE_MAC || t.type == TILE_MAC_PML || t.type == TILE_MAC_VEC) != (d.scheme == FDTD_SCHEME_STAGGERED))
@@ -735,14 +735,14 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
 void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, std::vector<TileDesc> by_kind[NUM_KINDS],
                         int nhead[NUM_KINDS])
 {
-    std::vector<TileDesc> special;
+    std::vector<TileDesc> special, moving;
     for (int k = 0; k < NUM_KINDS; ++k) by_kind[k].clear();
     if (nhead)
         for (int k = 0; k < NUM_KINDS; ++k) nhead[k] = 0;
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
-        else if (t.type == TILE_MOVING) by_kind[KIND_SPECIAL].push_back(t);   // a launch of their own
+        else if (t.type == TILE_MOVING) moving.push_back(t);
         else by_kind[(t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC) ? KIND_PML : KIND_INTERIOR].push_back(t);
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
@@ -753,7 +753,12 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     // are dispatched first on every step (the kernel keeps this head out of
     // the odd-step reversal), so their latency hides under the streaming
     // tiles; the head is padded to a multiple of eight so every later chunk
-    // still lands on a single XCD
+    // still lands on a single XCD.  The moving-object boxes join that head
+    // in the fused launch (their register budget is below the fused
+    // kernel's); with one launch per class they keep a launch of their own,
+    // which leaves the interior launch's budget to the streaming tiles.
+    if (fused_launch) special.insert(special.end(), moving.begin(), moving.end());
+    else by_kind[KIND_SPECIAL] = moving;
     if (!special.empty()) {
         TileDesc empty{};
         empty.type = TILE_EMPTY;

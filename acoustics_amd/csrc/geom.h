@@ -7,11 +7,12 @@
 // is x_i = a_i^2 y_i / (t + a_i^2), where t is the largest root of
 // F(t) = sum_i (a_i y_i / (t + a_i^2))^2 - 1 (Eberly 2013, "Distance from a
 // Point to an Ellipse, an Ellipsoid, or a Hyperellipsoid").  The root is
-// found with a safeguarded Newton iteration (Newton steps kept inside a
-// shrinking bracket, bisection when a step leaves it); points on principal
-// planes follow the same case analysis as the paper.
+// found with a safeguarded Newton iteration (started from the Newton step at
+// t = 0, steps kept inside a shrinking bracket, bisection when a step leaves
+// it); points on principal planes follow the same case analysis as the paper.
 #pragma once
 
+#include <cfloat>
 #include <cmath>
 
 #if defined(__HIP__)
@@ -68,17 +69,28 @@ FDTD_HD inline double geom_largest_root(int m, const double *e, const double *y)
     hi = fmax(sqrt(nz), 0.0) + 1e-300;
     if (geom_root_f(m, e, y, hi, nullptr) > 0) hi *= 2;
     // inside points (F(0) < 0): root in (lo, 0]; outside: in [0, hi)
-    const double f0 = geom_root_f(m, e, y, 0.0, nullptr);
+    double d0;
+    const double f0 = geom_root_f(m, e, y, 0.0, &d0);
     if (f0 == 0.0) return 0.0;
     if (f0 < 0) hi = 0.0; else lo = 0.0;
-    double t = 0.5 * (lo + hi);
-    if (lo == -emin2) t = hi - 0.5 * (hi - lo);  // stay away from the pole
+    // F is convex and decreasing, so the Newton step from 0 lands at or left
+    // of the root, from where the iteration climbs to it without overshoot;
+    // for points near the surface it starts next to the root
+    double t = -f0 / d0;
+    if (!(t > lo && t < hi)) {
+        t = 0.5 * (lo + hi);
+        if (lo == -emin2) t = hi - 0.5 * (hi - lo);  // stay away from the pole
+    }
     for (int it = 0; it < 400; ++it) {
         double d;
         const double f = geom_root_f(m, e, y, t, &d);
         if (f == 0.0) return t;
         if (f > 0) lo = t; else hi = t;
         double tn = t - f / d;
+        // converged: the Newton correction is within two ulps of the scale
+        // (tested before the bracket check, which would otherwise turn a
+        // converged step that rounds onto the bracket end into bisection)
+        if (fabs(tn - t) <= 2.0 * DBL_EPSILON * fmax(fabs(t), emin2)) return t;
         if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
         if (fabs(tn - t) <= 1e-16 * fmax(fabs(t), emin2)) return tn;
         t = tn;

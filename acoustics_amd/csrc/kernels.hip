@@ -1352,13 +1352,15 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         else if (t.type == TILE_ABC) abc_tile(S, t, terms, step);
         else pml_tile(S, t, terms, step);
     } else if constexpr (KIND == KIND_SPECIAL) {
-        // the moving-object tiles run in this launch of their own, so their
-        // FP64 geometry does not set the register budget of the streaming tiles
+        // with one launch per class the moving-object tiles run in this
+        // launch of their own, so they do not set the register budget of the
+        // streaming tiles
         if (t.type == TILE_MOVING) moving_tile(S, t, step);
         else special_tile(S, t, terms, step);
     } else {
         if (t.type == TILE_PML) pml_tile(S, t, terms, step);
         else if (t.type == TILE_INTERIOR_VEC) interior_vec_tile(S, t, terms, step);
+        else if (t.type == TILE_MOVING) moving_tile(S, t, step);
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
         else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
         else if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
