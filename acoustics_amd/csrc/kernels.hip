@@ -1068,9 +1068,10 @@ __device__ inline bool mv_fluid_now(const SceneDev &S, const Geom *pose, const i
 
 // fresh values of cell (i, j, k) in both pressure slots (p^{s-1} in plast,
 // p^s in pcur; one geometry and one set of MLS weights for both); prev_f is
-// the box's classification of the previous step
-__device__ void mv_fresh(const SceneDev &S, const Geom *pose, const float *plast, const float *pcur,
-                         const uint8_t *prev_f, int i, int j, int k, long long step, double *vl, double *vc)
+// the box's classification of the previous step, acc[o] object o's normal
+// acceleration a_s at this step
+__device__ void mv_fresh(const SceneDev &S, const Geom *pose, const double *acc, const float *plast, const float *pcur,
+                         const uint8_t *prev_f, int i, int j, int k, double *vl, double *vc)
 {
     double x[3];
     mv_centre(S, i, j, k, x);
@@ -1097,7 +1098,7 @@ __device__ void mv_fresh(const SceneDev &S, const Geom *pose, const float *plast
         lo3[d] = v0;
         hi3[d] = diff < 0.0 ? 0 : min(v0 + 1, S.n[d] - 1);
     }
-    MlsSums acc;
+    MlsSums mls;
     double bd = DBL_MAX, nl = 0.0, nc = 0.0;
 #pragma unroll
     for (int sl = 0; sl < 8; ++sl) {
@@ -1114,19 +1115,18 @@ __device__ void mv_fresh(const SceneDev &S, const Geom *pose, const float *plast
         const double l = fluid ? (double)plast[qi] : 0.0, cv = fluid ? (double)pcur[qi] : 0.0;
         const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
         if (d2 < bd) { bd = d2; nl = l; nc = cv; }
-        acc.add(y, image, S.h, 0.018, l, cv);
+        mls.add(y, image, S.h, 0.018, l, cv);
     }
-    if (acc.m >= 4) {
+    if (mls.m >= 4) {
         // MLS at the image point plus rho a_n |d|, a_n along the erected normal
         double ml, mc;
-        acc.solve(&ml, &mc);
-        const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
-        const double add = S.rho * a * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]) * fabs(travelled);
+        mls.solve(&ml, &mc);
+        const double add = S.rho * acc[id] * (nb[0] * en[0] + nb[1] * en[1] + nb[2] * en[2]) * fabs(travelled);
         *vl = ml + add;
         *vc = mc + add;
         return;
     }
-    if (acc.m > 0) { *vl = nl; *vc = nc; return; }
+    if (mls.m > 0) { *vl = nl; *vc = nc; return; }
     // no valid enclosing cell: the first fluid axis neighbour (+x, -x, +y, ...);
     // a neighbour that is itself fresh this step counts with the zero it held
     const int ijk[3] = {i, j, k};
@@ -1166,14 +1166,22 @@ __global__ __launch_bounds__(256) void motion_kernel(const SceneDev *__restrict_
     if (step >= (long long)S.clock[CLK_TARGET]) return;
     const int ncell = S.u_n[0] * S.u_n[1] * S.u_n[2];
     if ((int)blockIdx.x * (int)blockDim.x >= 6 * ncell) return;
+    // the poses and the objects' normal accelerations a_s at t_s, and this
+    // thread's previous classification, all loaded before the barrier so no
+    // dependent load waits at the end of the chain
     __shared__ Geom pose[MAX_MOVING_OBJ];
-    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) pose[o] = S.pose[(size_t)amp_row(step) * S.n_obj + o];
-    __syncthreads();
+    __shared__ double acc[MAX_MOVING_OBJ];
+    for (int o = threadIdx.x; o < S.n_obj; o += blockDim.x) {
+        pose[o] = S.pose[(size_t)amp_row(step) * S.n_obj + o];
+        acc[o] = gmem(S.acc_tab)[amp_row(step) * S.n_ch + o * S.n_cpo];
+    }
     const int tid = (int)blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-    if (tid >= 6 * ncell) return;
     const int q = tid / ncell, c = tid - q * ncell;
     uint8_t *now_row = S.ucls + (size_t)(step & 1) * ncell;
     const uint8_t *prev_f = S.ucls + (size_t)((step + 1) & 1) * ncell;
+    const bool prev = tid < 6 * ncell ? prev_f[c] != 0 : false;
+    __syncthreads();
+    if (tid >= 6 * ncell) return;
     const int kk = c % S.u_n[2], r = c / S.u_n[2];
     const int j = S.u_lo[1] + r % S.u_n[1], i = S.u_lo[0] + r / S.u_n[1], kz = S.u_lo[2] + kk;
     double x[3];
@@ -1181,7 +1189,7 @@ __global__ __launch_bounds__(256) void motion_kernel(const SceneDev *__restrict_
     const bool now = mv_fluid_at(S, pose, x);
     if (q == 0) {
         now_row[c] = now ? 1 : 0;
-        if (now != (bool)prev_f[c]) {
+        if (now != prev) {
             const int idx = i * S.sx + j * S.sy + kz;
             float *pcur = S.field + (size_t)slot_cur(step) * S.npad;
             float *plast = S.field + (size_t)slot_next(step) * S.npad;
@@ -1190,7 +1198,7 @@ __global__ __launch_bounds__(256) void motion_kernel(const SceneDev *__restrict_
                 plast[idx] = 0.f;
             } else {
                 double fl, fc;
-                mv_fresh(S, pose, plast, pcur, prev_f, i, j, kz, step, &fl, &fc);
+                mv_fresh(S, pose, acc, plast, pcur, prev_f, i, j, kz, &fl, &fc);
                 plast[idx] = (float)fl;
                 pcur[idx] = (float)fc;
             }
@@ -1212,8 +1220,7 @@ __global__ __launch_bounds__(256) void motion_kernel(const SceneDev *__restrict_
     for (int e = 0; e < 3; ++e) gp[e] = xo[e];
     gp[d] -= sg * (S.seven ? 0.25 : 0.5) * S.h;
     geom_closest(pose[id], gp, xb, nb);
-    const double a = gmem(S.acc_tab)[amp_row(step) * S.n_ch + id * S.n_cpo];
-    S.ughost[(size_t)q * ncell + c] = (S.seven ? 0.75 : 1.0) * S.h * S.rho * a * nb[d] * -(double)sg;
+    S.ughost[(size_t)q * ncell + c] = (S.seven ? 0.75 : 1.0) * S.h * S.rho * acc[id] * nb[d] * -(double)sg;
 }
 
 // A box of a moving scene's cells (one thread per cell): solid cells are
