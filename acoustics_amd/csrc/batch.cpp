@@ -26,21 +26,69 @@ fdtd_status BatchRunner::rebuild(hipStream_t st)
     std::vector<TermRef> terms;
     std::vector<MotionDesc> motion;
     members.clear();
+    slot_tiles.assign(scenes.size(), 0);
     for (size_t i = 0; i < scenes.size(); ++i) {
         if (!scenes[i] || done[i]) continue;
         const int id = (int)devs.size();
         devs.push_back(scenes[i]->dev);
+        const size_t before = tiles.size();
         scenes[i]->append_tiles(id, tile_plan(plan), tiles, terms);
+        slot_tiles[i] = tiles.size() - before;
         if (scenes[i]->needs_motion_kernel()) motion.push_back({id});
         members.push_back((int)i);
     }
+    prog_tiles = tiles.size();
+    dead_tiles = 0;
     if (devs.empty()) { prog.release(); return FDTD_OK; }
-    return prog.upload(devs, tiles, terms, st, tile_plan(plan).fused, motion);
+    const bool fused = tile_plan(plan).fused;
+    size_t npre = motion.size();
+    for (const SceneDev &d : devs) npre += d.n_spk ? 1 : 0;
+    const size_t need = StepProgram::bytes_needed(devs.size(), tiles, fused, terms.size(), npre);
+    if (need > pbytes) {
+        // the lists only shrink while a batch runs, so this happens on the
+        // first builds (the autotuner's plans), with nothing in flight
+        FDTD_HIP_TRY(hipStreamSynchronize(st));
+        prog.release();
+        free_buffers();
+        for (int b = 0; b < 2; ++b) {
+            FDTD_HIP_TRY(hipMalloc(&pbuf[b], need));
+            FDTD_HIP_TRY(hipHostMalloc(&pstage[b], need));
+            FDTD_HIP_TRY(hipEventCreateWithFlags(&pdone[b], hipEventDisableTiming));
+        }
+        pbytes = need;
+        pnext = 0;
+    }
+    const int b = pnext;
+    pnext ^= 1;
+    FDTD_HIP_TRY(hipEventSynchronize(pdone[b]));   // the stage's previous copy has run
+    prog.release();
+    const fdtd_status s = prog.upload(devs, tiles, terms, st, fused, motion, pbuf[b], pbytes, pstage[b]);
+    if (s != FDTD_OK) return s;
+    FDTD_HIP_TRY(hipEventRecord(pdone[b], st));
+    return FDTD_OK;
 }
 
 void BatchRunner::drop_graph()
 {
-    if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+    if (exec) { retired.push_back(exec); exec = nullptr; }
+}
+
+void BatchRunner::reap()
+{
+    for (hipGraphExec_t e : retired) (void)hipGraphExecDestroy(e);
+    retired.clear();
+}
+
+void BatchRunner::free_buffers()
+{
+    for (int b = 0; b < 2; ++b) {
+        if (pbuf[b]) (void)hipFree(pbuf[b]);
+        if (pstage[b]) (void)hipHostFree(pstage[b]);
+        if (pdone[b]) (void)hipEventDestroy(pdone[b]);
+        pbuf[b] = pstage[b] = nullptr;
+        pdone[b] = nullptr;
+    }
+    pbytes = 0;
 }
 
 fdtd_status BatchRunner::build_graph(hipStream_t st)
@@ -91,19 +139,34 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
     return rebuild(st);
 }
 
+// A finished scene's tiles stay in the work list and return at once on its
+// device clock; the list is rebuilt without them only once they are at least
+// this share of it.  A rebuild drains the stream (the old program's memory is
+// freed and the new one uploaded) and captures a new graph, several
+// milliseconds with the device idle, which the early-returning workgroups of
+// a few finished scenes do not cost.
+constexpr double kRebuildDeadShare = 0.25;
+
 fdtd_status BatchRunner::advance(hipStream_t st, int nsteps)
 {
     int left = nsteps;
     while (left > 0) {
-        bool changed = false;
         long long remaining = 0;
+        bool cut = false;
         for (size_t i = 0; i < scenes.size(); ++i) {
             if (!scenes[i] || done[i]) continue;
-            if (taken >= targets[i]) { done[i] = 1; changed = true; continue; }
+            if (taken >= targets[i]) {
+                done[i] = 1;
+                if (i < slot_tiles.size()) dead_tiles += slot_tiles[i];
+                // a target cut on the host (a time chunk's early termination)
+                // is not on the device clock: only a rebuild stops that scene
+                cut = cut || targets[i] < scenes[i]->target;
+                continue;
+            }
             remaining = std::max(remaining, targets[i] - taken);
         }
         if (remaining == 0) break;
-        if (changed) {
+        if (cut || (dead_tiles > 0 && (double)dead_tiles >= kRebuildDeadShare * (double)prog_tiles)) {
             fdtd_status s = rebuild(st);
             if (s != FDTD_OK) return s;
         }
@@ -130,7 +193,9 @@ fdtd_status BatchRunner::advance(hipStream_t st, int nsteps)
 BatchRunner::~BatchRunner()
 {
     drop_graph();
+    reap();
     prog.release();
+    free_buffers();
 }
 
 }  // namespace fdtd
@@ -297,6 +362,7 @@ fdtd_status fdtd_batch_run(fdtd_batch *b)
     fdtd_status st = fdtd_batch_step(b, (int)std::max<long long>(0, mx - b->R.taken));
     if (st != FDTD_OK) return st;
     FDTD_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->R.reap();
     return FDTD_OK;
 }
 
@@ -311,6 +377,7 @@ fdtd_status fdtd_batch_read_listener_series(fdtd_batch *b, int scene, double *ou
 {
     if (!b || scene < 0 || scene >= (int)b->R.scenes.size()) return FDTD_ERR_INVALID;
     FDTD_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->R.reap();
     const DeviceScene &D = *b->R.scenes[scene];
     const long long rows = std::min<long long>(b->R.taken, b->R.targets[scene]);
     return D.read_series(0, rows, out);
@@ -370,7 +437,9 @@ fdtd_status fdtd_batch_destroy(fdtd_batch *b)
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     b->R.drop_graph();
+    b->R.reap();
     b->R.prog.release();
+    b->R.free_buffers();
     b->R.release_scenes();
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;

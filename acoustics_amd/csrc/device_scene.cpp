@@ -815,7 +815,8 @@ size_t StepProgram::bytes_needed(size_t nscenes, const std::vector<TileDesc> &ti
 
 fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                                 const std::vector<TermRef> &terms, hipStream_t st,
-                                bool fused_launch, const std::vector<MotionDesc> &motion, char *ext_base, size_t ext_bytes)
+                                bool fused_launch, const std::vector<MotionDesc> &motion, char *ext_base, size_t ext_bytes,
+                                char *host_stage)
 {
     // scenes with speaker sources get the speaker pass before their step kernels
     std::vector<MotionDesc> speaker;
@@ -855,14 +856,23 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
     d_scenes = reinterpret_cast<SceneDev *>(base);
     d_terms = reinterpret_cast<TermRef *>(base + bytes[0]);
     d_motion = reinterpret_cast<MotionDesc *>(base + bytes[0] + bytes[1]);
+    // each piece goes straight to the device (pageable copies, then a stream
+    // synchronisation), or into the pinned host stage at the same offset, which
+    // then goes over in one asynchronous copy (the caller keeps the stage
+    // intact until that copy has run)
+    const bool staged = ext_base && host_stage;
+    auto put = [&](void *dst, const void *src, size_t n) -> hipError_t {
+        if (!n) return hipSuccess;
+        if (staged) { std::memcpy(host_stage + ((char *)dst - base), src, n); return hipSuccess; }
+        return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st);
+    };
     size_t off = bytes[0] + bytes[1] + bytes[2];
     for (int k = 0; k < NUM_KINDS; ++k) {
         d_tiles[k] = reinterpret_cast<TileDesc *>(base + off);
         off += bytes[3 + k];
         ntiles[k] = (int)by_kind[k].size();
         head[k] = std::min(nhead_k[k], ntiles[k]);
-        if (ntiles[k])
-            FDTD_HIP_TRY(hipMemcpyAsync(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k], hipMemcpyHostToDevice, st));
+        FDTD_HIP_TRY(put(d_tiles[k], by_kind[k].data(), sizeof(TileDesc) * ntiles[k]));
     }
     nmotion = (int)motion.size();
     motion_cells = 0;
@@ -874,10 +884,11 @@ fdtd_status StepProgram::upload(const std::vector<SceneDev> &scenes, const std::
         speaker_cells = std::max(speaker_cells, devs[m.scene].sp_n[0] * devs[m.scene].sp_n[1] * devs[m.scene].sp_n[2]);
     std::vector<MotionDesc> pre(motion);
     pre.insert(pre.end(), speaker.begin(), speaker.end());
-    if (!pre.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_motion, pre.data(), sizeof(MotionDesc) * pre.size(), hipMemcpyHostToDevice, st));
-    if (!devs.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_scenes, devs.data(), sizeof(SceneDev) * devs.size(), hipMemcpyHostToDevice, st));
-    if (!terms.empty()) FDTD_HIP_TRY(hipMemcpyAsync(d_terms, terms.data(), sizeof(TermRef) * terms.size(), hipMemcpyHostToDevice, st));
-    FDTD_HIP_TRY(hipStreamSynchronize(st));
+    FDTD_HIP_TRY(put(d_motion, pre.data(), sizeof(MotionDesc) * pre.size()));
+    FDTD_HIP_TRY(put(d_scenes, devs.data(), sizeof(SceneDev) * devs.size()));
+    FDTD_HIP_TRY(put(d_terms, terms.data(), sizeof(TermRef) * terms.size()));
+    if (staged) FDTD_HIP_TRY(hipMemcpyAsync(base, host_stage, total, hipMemcpyHostToDevice, st));
+    else FDTD_HIP_TRY(hipStreamSynchronize(st));
     order_clock = scenes.empty() ? nullptr : scenes[0].clock + CLK_STEP;
     return FDTD_OK;
 }

@@ -127,10 +127,13 @@ struct StepProgram {
     // launch when `fused_launch` is set)
     // (ext_base: place the program there, ext_bytes long, instead of
     // allocating it; release() then frees nothing)
+    // (host_stage, with ext_base: pinned host memory of ext_bytes; the pieces
+    // are assembled there and go over in one asynchronous copy on st, with no
+    // synchronisation; the caller keeps the stage until that copy has run)
     fdtd_status upload(const std::vector<SceneDev> &scenes, const std::vector<TileDesc> &tiles,
                        const std::vector<TermRef> &terms, hipStream_t st,
                        bool fused_launch = false, const std::vector<MotionDesc> &motion = {}, char *ext_base = nullptr,
-                       size_t ext_bytes = 0);
+                       size_t ext_bytes = 0, char *host_stage = nullptr);
     // Bytes upload() places for these lists (npre: motion plus speaker entries).
     static size_t bytes_needed(size_t nscenes, const std::vector<TileDesc> &tiles, bool fused_launch, size_t nterms,
                                size_t npre);

@@ -65,7 +65,7 @@ def step(n, steps, tile, scheme="collocated", boundary="rasterize"):
 
 
 def _short(name):
-    for k in list(STEP_KERNELS) + ["advance_kernel", "motion_classify_kernel", "motion_fresh_kernel", "copyBuffer",
+    for k in list(STEP_KERNELS) + ["advance_kernel", "motion_kernel", "copyBuffer",
                                    "fillBuffer", "vectorized_elementwise"]:
         if k in name:
             return k
