@@ -539,7 +539,6 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
     // before a[2] are read and not written); the row stride is a multiple of
     // four too, and a row's last group may be partial
     const int kv0 = a[2] & ~3, lead = a[2] - kv0, kwv = b[2] - kv0;
-    // (the staggered scheme's float4 rows run in the interior launch only)
     // staggered rows take one coefficient entry per axis: every face of the
     // box (a .. b inclusive) and every cell must hold the same table values
     bool mac_uniform = true;
@@ -549,7 +548,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                 mac_uniform = mac_uniform && S.f1[d][i] == S.f1[d][a[d]] && S.f2[d][i] == S.f2[d][a[d]];
                 if (i < b[d]) mac_uniform = mac_uniform && S.f5h[d][i] == S.f5h[d][a[d]];
             }
-    const bool vec = !S.visc && plan.interior_vec && kwv <= 256 && (!mac || (!plan.fused && mac_uniform));
+    const bool vec = !S.visc && plan.interior_vec && kwv <= 256 && (!mac || mac_uniform);
     auto scalar_tiles = [&](int i0, int i1, int jlo, int jhi) {
         for (int j0 = jlo; j0 < jhi; j0 += TJ)
             for (int k0 = a[2]; k0 < b[2]; k0 += TK) {
@@ -679,7 +678,8 @@ static std::string check_tiles(const std::vector<TileDesc> &list, int kind, cons
                                        (kind == KIND_INTERIOR && !pml_class) || (kind == KIND_SPECIAL && t.type == TILE_SPECIAL));
         if ((t.type == TILE_MAC || t.type == TILE_MAC_PML || t.type == TILE_MAC_VEC) != (d.scheme == FDTD_SCHEME_STAGGERED))
             return "tile class does not match the scene's scheme";
-        if (t.type == TILE_MAC_VEC && kind != KIND_INTERIOR) return "staggered float4 rows outside the interior launch";
+        if (t.type == TILE_MAC_VEC && kind != KIND_INTERIOR && kind != KIND_ALL)
+            return "staggered float4 rows outside the interior and fused launches";
         if (!ok_kind) return "tile class does not match its launch";
         if (t.type == TILE_SPECIAL) {
             if (t.i0 < 0 || t.i1 > d.n_special || t.i1 - t.i0 > 256) return "special tile range out of range";

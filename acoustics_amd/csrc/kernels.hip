@@ -1371,6 +1371,7 @@ __global__ __launch_bounds__(256) void step_kernel(const SceneDev *__restrict__ 
         else if (t.type == TILE_SPECIAL) special_tile(S, t, terms, step);
         else if (t.type == TILE_MAC) mac_tile<false>(S, t, terms, step);
         else if (t.type == TILE_MAC_PML) mac_tile<true>(S, t, terms, step);
+        else if (t.type == TILE_MAC_VEC) mac_vec_tile(S, t, terms, step);
         else if (t.type == TILE_ABC) abc_tile(S, t, terms, step);
         else interior_tile(S, t, terms, step, t.type == TILE_INTERIOR_SKIP);
     }
