@@ -765,10 +765,28 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
         while (special.size() % 8) special.push_back(empty);
     }
     if (fused_launch) {
-        // then the PML tiles (the longest); the interior fills in behind them
+        // then the PML and interior tiles interleaved in proportion, a group of
+        // eight (one tile per XCD) at a time, so each XCD walks its chunk of
+        // each class in order and both classes run side by side for the whole
+        // step: each class alone reaches about 3.6 TB/s, together more (at
+        // 256^3, 73.8-75.5 us per step against 77.0 with the PML tiles first;
+        // the 64-scene batch 116.6 Gcell/s against 110.6)
         std::vector<TileDesc> all = special;
-        all.insert(all.end(), by_kind[KIND_PML].begin(), by_kind[KIND_PML].end());
-        all.insert(all.end(), by_kind[KIND_INTERIOR].begin(), by_kind[KIND_INTERIOR].end());
+        const std::vector<TileDesc> &P = by_kind[KIND_PML], &I = by_kind[KIND_INTERIOR];
+        if (P.size() % 8 || I.size() % 8) {
+            all.insert(all.end(), P.begin(), P.end());
+            all.insert(all.end(), I.begin(), I.end());
+        } else {
+            const size_t gP = P.size() / 8, gI = I.size() / 8;
+            size_t p = 0, q = 0;
+            while (p < gP || q < gI) {
+                // a PML group while the PML share so far is at most its overall share
+                const bool takeP = q >= gI || (p < gP && (double)p * (double)(gP + gI) <= (double)(p + q) * (double)gP);
+                const std::vector<TileDesc> &L = takeP ? P : I;
+                const size_t g = takeP ? p++ : q++;
+                all.insert(all.end(), L.begin() + 8 * g, L.begin() + 8 * g + 8);
+            }
+        }
         by_kind[KIND_INTERIOR].swap(all);
         by_kind[KIND_PML].clear();
     } else {
