@@ -48,6 +48,7 @@ struct Layout {
     double *op_w, *oc_w;
     double *terms;
     unsigned long long *clock;
+    double *t_tab;
     double *acc_tab;
     float *src_tab;
     double *spk, *spk_samples;
@@ -126,6 +127,7 @@ void carve(const SceneSetup &S, int cap, Carver &cs, Carver &c, Layout &L)
     L.oc_w = c.take<double>(S.oc_w.size());
     L.terms = c.take<double>((size_t)cap * std::max<size_t>(1, S.lst_terms.size()));
     L.clock = c.take<unsigned long long>(CLK_WORDS);
+    L.t_tab = c.take<double>(AMP_ROWS);
     L.acc_tab = c.take<double>((size_t)AMP_ROWS * S.objs.size() * S.n_cpo);
     L.src_tab = c.take<float>((size_t)AMP_ROWS * S.psrc.size());
     L.spk = c.take<double>(8 * S.spk.size());
@@ -174,7 +176,8 @@ fdtd_status DeviceScene::reset_to_start(hipStream_t st)
 {
     FDTD_HIP_TRY(zero_state(st));
     FDTD_HIP_TRY(hipMemsetAsync(dev.terms, 0, sizeof(double) * (size_t)series_cap * std::max<size_t>(1, S.lst_terms.size()), st));
-    const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
+    unsigned long long clk[CLK_WORDS];
+    clock_words(clk, 0, (unsigned long long)target, S.t_start, S.dt);
     FDTD_HIP_TRY(hipMemcpyAsync(dev.clock, clk, sizeof clk, hipMemcpyHostToDevice, st));
     FDTD_HIP_TRY(fill_amplitudes(st));
     if (S.moving)
@@ -300,7 +303,8 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     FDTD_HIP_TRY(up(L.oc_ptr, S.oc_ptr.data(), S.oc_ptr.size() * 4));
     FDTD_HIP_TRY(up(L.oc_ch, S.oc_ch.data(), S.oc_ch.size() * 4));
     FDTD_HIP_TRY(up(L.oc_w, S.oc_w.data(), S.oc_w.size() * 8));
-    const unsigned long long clk[CLK_WORDS] = {0ULL, (unsigned long long)target, 0ULL, 0ULL};
+    unsigned long long clk[CLK_WORDS];
+    clock_words(clk, 0, (unsigned long long)target, S.t_start, S.dt);
     FDTD_HIP_TRY(up(L.clock, clk, sizeof clk));
     FDTD_HIP_TRY(hipStreamSynchronize(st));
 
@@ -355,6 +359,7 @@ fdtd_status DeviceScene::place(hipStream_t st, int cap, long long target_steps, 
     D.seven = S.sampling == FDTD_SAMPLE_SEVEN ? 1 : 0;
     D.n_cpo = S.n_cpo;
     D.n_ch = D.n_obj * S.n_cpo;
+    D.t_tab = L.t_tab;
     D.acc_tab = L.acc_tab;
     D.src_tab = L.src_tab;
     D.n_spk = (int)S.spk.size();

@@ -52,8 +52,24 @@ enum : int { NUM_P_SLOTS = 2 };
 __host__ __device__ inline int slot_cur(long long s) { return (int)((s + 1) & 1); }
 __host__ __device__ inline int slot_next(long long s) { return (int)(s & 1); }
 
-// clock words of a scene
-enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_WORDS = 4 };
+// clock words of a scene; CLK_TIME holds the bits of the simulation time of
+// step clock[CLK_STEP], accumulated from the start time by adding dt once per
+// step, as the reference's clock runs (_simulationTime += dt)
+enum : int { CLK_STEP = 0, CLK_TARGET = 1, CLK_DONE = 2, CLK_TIME = 3, CLK_WORDS = 4 };
+
+// the clock words of step s of a scene starting at t_start (host side)
+inline void clock_words(unsigned long long *clk, long long s, unsigned long long target, double t_start, double dt)
+{
+    double t = t_start;
+    for (long long q = 0; q < s; ++q) t += dt;
+    unsigned long long tb;
+    static_assert(sizeof tb == sizeof t, "time word");
+    __builtin_memcpy(&tb, &t, sizeof t);
+    clk[CLK_STEP] = (unsigned long long)s;
+    clk[CLK_TARGET] = target;
+    clk[CLK_DONE] = 0ULL;
+    clk[CLK_TIME] = tb;
+}
 
 // Steps are launched in blocks: the kernels of the k-th step of a block run
 // step clock[CLK_STEP] + k, and one advance kernel after the block moves the
@@ -125,6 +141,7 @@ struct SceneDev {
     int seven;
     int n_cpo, n_ch;            // amplitude channels per object (1, or 4 with acceleration noise), n_obj n_cpo
     // time-dependent amplitudes, one ring row per step (amp_row):
+    double *t_tab;              // [AMP_ROWS] the accumulated time of the step (CLK_TIME)
     double *acc_tab;            // [AMP_ROWS][n_ch] channel o n_cpo: a_s(o, t_s); o n_cpo + 1 + d: a_v(o, t_s)[d]
     float *src_tab;             // [AMP_ROWS][n_psrc] S(t_s + dt/2) dt without the spatial factor
     long long src_steps;        // time-parallel chunk: sources act in steps < src_steps (-1: always)

@@ -109,7 +109,7 @@ __device__ __forceinline__ float source_amp(const PSrcDev &ps, double tn, double
 __device__ inline double speaker_terms(const SceneDev &S, long long step, int i, int j, int k)
 {
     if (S.n_spk == 0 || (S.src_steps >= 0 && step >= S.src_steps)) return 0.0;
-    const double t = S.t_start + (double)step * S.dt + 0.5 * S.dt;
+    const double t = gmem(S.t_tab)[amp_row(step)] + 0.5 * S.dt;
     const int ijk[3] = {i, j, k};
     double x[3];
     for (int d = 0; d < 3; ++d) x[d] = S.bmin[d] + (0.5 + (double)ijk[d]) * S.h;
@@ -1304,12 +1304,25 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
     const long long step = (long long)clk[CLK_STEP], target = (long long)clk[CLK_TARGET];
     if (step >= target) return;
     const long long base = min(step + (long long)n, target);
+    // the times of steps base .. base + AMP_ROWS - 1, accumulated from the
+    // clock's time by one addition of dt per step (the reference's clock)
+    __shared__ double tt[AMP_ROWS];
+    if (threadIdx.x == 0) {
+        double t = __longlong_as_double((long long)clk[CLK_TIME]);
+        for (long long q = step; q < base; ++q) t += S.dt;
+        for (int r = 0; r < AMP_ROWS; ++r) {
+            tt[r] = t;
+            t += S.dt;
+        }
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < AMP_ROWS; r += blockDim.x) gmem(S.t_tab)[amp_row(base + r)] = tt[r];
     // a time-parallel chunk's sources are off from step src_steps on
     const long long off = S.src_steps < 0 ? LLONG_MAX : S.src_steps;
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_ch; q += blockDim.x) {
         const long long s = base + q / S.n_ch;
         const int ch = q % S.n_ch, o = ch / S.n_cpo, comp = ch % S.n_cpo;
-        const double t = S.t_start + (double)s * S.dt;
+        const double t = tt[q / S.n_ch];
         double a = 0.0;
         if (s < off) a = comp == 0 ? impulse_accel(S.isrc, S.n_isrc, o, t) : noise_accel(S.impacts, S.n_impact, o, comp - 1, t);
         gmem(S.acc_tab)[amp_row(s) * S.n_ch + ch] = a;
@@ -1318,7 +1331,7 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
         for (int q = threadIdx.x; q < AMP_ROWS * S.n_obj; q += blockDim.x) {
             const long long s = base + q / S.n_obj;
             const int o = q % S.n_obj;
-            const double t = S.t_start + (double)s * S.dt;
+            const double t = tt[q / S.n_obj];
             const MoveDev &m = S.mobj[o];
             gmem(S.pose)[amp_row(s) * S.n_obj + o] =
                 m.kin_n > 0 ? geom_keyframe(m.g0, S.kin + m.kin_off, m.kin_n, m.kin_dt, t) : geom_at(m.g0, m.v, m.w, t);
@@ -1326,10 +1339,13 @@ __global__ __launch_bounds__(256) void advance_kernel(const SceneDev *__restrict
     for (int q = threadIdx.x; q < AMP_ROWS * S.n_psrc; q += blockDim.x) {
         const long long s = base + q / S.n_psrc;
         const int p = q % S.n_psrc;
-        gmem(S.src_tab)[amp_row(s) * S.n_psrc + p] = s < off ? source_amp(S.psrc[p], S.t_start + (double)s * S.dt, S.dt, S.scheme) : 0.f;
+        gmem(S.src_tab)[amp_row(s) * S.n_psrc + p] = s < off ? source_amp(S.psrc[p], tt[q / S.n_psrc], S.dt, S.scheme) : 0.f;
     }
     __syncthreads();
-    if (threadIdx.x == 0) clk[CLK_STEP] = (unsigned long long)base;
+    if (threadIdx.x == 0) {
+        clk[CLK_STEP] = (unsigned long long)base;
+        clk[CLK_TIME] = (unsigned long long)__double_as_longlong(tt[0]);
+    }
 }
 
 // One kernel per tile class, so each class gets its own register allocation

@@ -565,7 +565,8 @@ fdtd_status fdtd_save_snapshot(fdtd_handle *h, const char *pattern, const char *
 // Sets the clock of step s and the host's series bookkeeping after a load.
 static fdtd_status resume_at(fdtd_handle *h, long long s)
 {
-    const unsigned long long clk[CLK_WORDS] = {(unsigned long long)s, (unsigned long long)LLONG_MAX, 0ULL, 0ULL};
+    unsigned long long clk[CLK_WORDS];
+    clock_words(clk, s, (unsigned long long)LLONG_MAX, h->D.S.t_start, h->D.S.dt);
     FDTD_HIP_TRY(hipMemcpy(h->D.dev.clock, clk, sizeof clk, hipMemcpyHostToDevice));
     FDTD_HIP_TRY(h->D.fill_amplitudes(h->stream));
     FDTD_HIP_TRY(hipStreamSynchronize(h->stream));

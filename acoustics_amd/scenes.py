@@ -259,6 +259,26 @@ def speaker_scene(seed=15, n=48, h=0.005, pml=8, steps=300, n_listeners=8, with_
     return SceneBuffers(s, objects=objs, impulse_sources=imps, listeners=listeners, speaker_sources=[spk]), steps
 
 
+def speaker_tie_scene(seed=17, n=40, pml=8, steps=400, n_listeners=8):
+    """A speaker on a cell centre whose sample index lands on (or within an ulp
+    of) an integer at every step: h = 2^-8 m on an even grid, so cell centres
+    are exact in binary and the speaker's distance to its own cell is zero;
+    the sample period is dt / 2 and the samples alternate in sign.  Which
+    sample a step takes then depends on the last bits of the step's time, so
+    the series matches the reference's only when the time accumulates as its
+    clock does (one addition of dt per step)."""
+    rng = np.random.default_rng(seed)
+    h = 2.0 ** -8
+    s = base_settings((n, n, n), h, pml)
+    c = n // 2
+    pos = tuple(float(s["min_bound"][d] + (0.5 + c) * h) for d in range(3))
+    m = 2 * steps + 16
+    samples = np.array([(-1.0) ** i * math.sin(math.pi * (i + 0.5) / m) for i in range(m)])
+    spk = dict(position=pos, width_space=1.5 * h, start_time=0.0, sample_rate=2.0 / s["time_step"], samples=samples)
+    listeners = _random_listeners(rng, s, [], n_listeners)
+    return SceneBuffers(s, objects=[], impulse_sources=[], listeners=listeners, speaker_sources=[spk]), steps
+
+
 def config2(seed=2, n=128, steps=2000, boundary=FDTD_BOUNDARY_RASTERIZE, pml=12, radius=0.05, h=0.005,
             n_listeners=64, shell=0.2):
     """Rigid sphere (radius 0.05 m, seeded centre) with a Gaussian-impulse

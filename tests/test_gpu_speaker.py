@@ -32,6 +32,24 @@ def test_speaker_matches_oracle(hip_lib, with_sphere):
     assert err <= 1e-4
 
 
+def test_speaker_sample_ties_follow_the_accumulated_clock(hip_lib):
+    """A speaker on a cell centre with the sample period dt / 2 and samples of
+    alternating sign: at every step the sample index sits on an integer up to
+    the last bits of the time, and a clock computed as t_start + s dt picks
+    the other sample on about half of the steps (184 of 400 here).  The
+    device accumulates the time as the reference and the oracle do."""
+    sb, steps = scenes.speaker_tie_scene()
+    print(f"case: speaker ties divisions={sb.divisions} steps={steps}", flush=True)
+    ref = Oracle(sb, reclassify_each_step=False).step(steps)
+    with Solver(sb, steps_per_graph=16) as s:
+        s.step(steps)
+        got = s.series()
+    err = rel_l2(got, ref)
+    print("speaker ties rel L2", err, "peak", np.abs(ref).max())
+    assert np.abs(ref).max() > 0
+    assert err <= 1e-4
+
+
 def test_speaker_tile_plans_agree(hip_lib):
     sb, _ = scenes.speaker_scene(seed=16, n=40, with_sphere=True)
     print(f"case: speaker plans divisions={sb.divisions}", flush=True)
