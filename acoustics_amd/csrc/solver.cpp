@@ -341,7 +341,10 @@ fdtd_status fdtd_get_info(fdtd_handle *h, fdtd_info *o)
     o->n_solid_cells = S.n_solid;
     o->n_listeners = S.L;
     o->step_index = h->step;
-    o->time = S.t_start + (double)h->step * S.dt;
+    // the device clock's time: one addition of dt per step from the start
+    double t = S.t_start;
+    for (long long q = 0; q < h->step; ++q) t += S.dt;
+    o->time = t;
     o->bytes_per_step = algorithmic_bytes(S);
     o->device_bytes = (int64_t)h->D.arena_bytes;
     o->n_pml_merged_cells = S.n_pml_merged;

@@ -361,7 +361,7 @@ typedef struct fdtd_info {
     int64_t n_solid_cells;     /* cells occupied by objects                    */
     int64_t n_listeners;
     int64_t step_index;        /* steps taken since fdtd_create                */
-    double  time;              /* simulation time (s)                          */
+    double  time;              /* simulation time (s): start + dt added per step */
     double  bytes_per_step;    /* algorithmic HBM bytes: 12 N + 44 N_pml      */
     int64_t device_bytes;      /* device memory held by the handle             */
     int64_t n_pml_merged_cells;   /* PML cells in the merged state (pml_state)  */
