@@ -150,9 +150,20 @@ TilePlan tile_plan(int id)
     static const TilePlan plans[kNumTilePlans] = {
         {6, 2, 8, true, true, 6},   {6, 4, 4, true, true, 6},   {6, 8, 8, true, true, 6},   {6, 4, 8, true, false, 6},
         {6, 8, 8, false, false, 6}, {6, 4, 8, true, true, 5},   {6, 4, 16, true, true, 6},  {6, 4, 8, true, true, 6},
-        {6, 4, 4, false, false, 6}, {6, 4, 8, true, true, 4},
+        {6, 4, 4, false, false, 6}, {6, 4, 8, true, true, 4},   {6, 2, 2, true, true, 6},
     };
     return plans[(id >= 0 && id < kNumTilePlans) ? id : 0];
+}
+
+// One fused launch; the collocated scheme marches 2 interior and 8 PML
+// planes (plan 0), the staggered scheme 4 and 4 (plan 1: 199 us per step at
+// 256^3, against 207 us with plan 0 and 227 us as two launches).  A grid this
+// small is latency-bound and takes 2-plane PML marches (plan 10: 6.2 us per
+// step at 64^3 against 9.8; the same at 80^3, slower from 128^3 on).
+int default_tile_plan(const SceneSetup &S)
+{
+    if (S.scheme == FDTD_SCHEME_STAGGERED) return 1;
+    return S.N < 400000 ? 10 : 0;
 }
 
 size_t DeviceScene::bytes_needed(const SceneSetup &S, int series_cap)

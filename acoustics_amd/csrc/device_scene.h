@@ -30,8 +30,12 @@ struct TilePlan {
     bool fused = false;        // one launch for all tile classes (PML tiles first)
     int pml_tk_log2 = 6;       // PML slab columns: up to 1 << pml_tk_log2 along k, 256 >> it along j
 };
-constexpr int kNumTilePlans = 10;
+constexpr int kNumTilePlans = 11;
 TilePlan tile_plan(int id);
+struct SceneSetup;
+// The single-scene default: plan 0; plan 1 for the staggered scheme; plan 10
+// for a grid of under 400,000 cells.
+int default_tile_plan(const SceneSetup &S);
 
 // Device buffers and uploaded tables of one scene, all in one hipMalloc arena.
 struct DeviceScene {

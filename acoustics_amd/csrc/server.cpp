@@ -282,7 +282,7 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     std::string err;
     fdtd_status st = build_setup(*scene, D.S, err);
     if (st != FDTD_OK) { set_last_error(err); return st; }
-    const int plan = (tile_config >= 0 && tile_config < kNumTilePlans) ? tile_config : 0;
+    const int plan = (tile_config >= 0 && tile_config < kNumTilePlans) ? tile_config : default_tile_plan(D.S);
     const TilePlan tp = tile_plan(plan);
     // the scene block as the kernels see it, without device arrays
     D.sy_dev = row_stride(D.S);

@@ -116,11 +116,7 @@ fdtd_status fdtd_create(const fdtd_scene *scene, const fdtd_options *opt_in, fdt
     FDTD_HIP_TRY(hipGetDevice(&h->D.device));
     if (opt.stream) h->stream = (hipStream_t)opt.stream;
     else { FDTD_HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)); h->own_stream = true; }
-    // default plan: one fused launch; the collocated scheme marches 2 interior
-    // and 8 PML planes (plan 0), the staggered scheme 4 and 4 (plan 1: 199 us
-    // per step at 256^3, against 207 us with plan 0 and 227 us as two launches)
-    const int def_plan = h->D.S.scheme == FDTD_SCHEME_STAGGERED ? 1 : 0;
-    h->tile = (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) ? opt.tile_config : def_plan;
+    h->tile = (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) ? opt.tile_config : default_tile_plan(h->D.S);
     h->graph_steps = std::max(0, opt.steps_per_graph);
     int cap = opt.series_capacity > 0 ? opt.series_capacity : 1024;
     cap = std::max(cap, h->graph_steps + 1);

@@ -58,7 +58,7 @@ def test_moving_graphs_and_plans_agree(hip_lib):
     with Solver(sb, tile_config=0) as a:
         a.step(150)
         ea, sa = a.pressure(), a.series()
-    for tile in range(10):
+    for tile in range(11):
         with Solver(sb, tile_config=tile, steps_per_graph=16 if tile % 2 else 0) as b:
             b.step(9)
             b.step(141)

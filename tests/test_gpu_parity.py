@@ -84,7 +84,7 @@ def test_deterministic_rerun(hip_lib):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("tile", list(range(10)))
+@pytest.mark.parametrize("tile", list(range(11)))
 def test_tile_configs_agree(hip_lib, tile):
     sb, _ = scenes.config2(seed=23, n=40, radius=0.025, shell=0.06, pml=8, n_listeners=8)
     with Solver(sb, tile_config=0) as a:
@@ -264,7 +264,7 @@ def test_engquist_boundary(hip_lib, case):
     assert np.linalg.norm(inner - qi) <= 1e-4 * np.linalg.norm(qi)
 
 
-@pytest.mark.parametrize("tile", list(range(10)))
+@pytest.mark.parametrize("tile", list(range(11)))
 def test_engquist_tile_plans_and_graphs_agree(hip_lib, tile):
     sb = _engquist_scene("sphere_raster")
     print(f"case: Engquist tile plan {tile}", flush=True)

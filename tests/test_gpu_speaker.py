@@ -56,7 +56,7 @@ def test_speaker_tile_plans_agree(hip_lib):
     with Solver(sb, tile_config=0) as a:
         a.step(80)
         ea = a.pressure()
-    for tile in range(10):
+    for tile in range(11):
         with Solver(sb, tile_config=tile, steps_per_graph=16 if tile % 2 else 0) as b:
             b.step(80)
             assert np.array_equal(ea, b.pressure()), tile

@@ -40,7 +40,7 @@ def test_staggered_tile_plans_agree(hip_lib):
     sb, _ = scenes.staggered_free_field(seed=8, div=(32, 32, 40), pml=5)
     print(f"case: staggered plans divisions={sb.divisions}", flush=True)
     base = None
-    for plan in range(10):
+    for plan in range(11):
         with Solver(sb, tile_config=plan) as s:
             s.step(120)
             out = (s.series(), s.pressure())

@@ -9,11 +9,12 @@ in exactly one tile.  A plan that broke any of these would read or write
 outside its arrays on the device."""
 import ctypes as C
 
+import numpy as np
 import pytest
 
 from acoustics_amd import capi, scenes
 
-N_PLANS = 10
+N_PLANS = 11
 
 
 def _scenes():
@@ -50,9 +51,12 @@ def test_every_plan_is_valid(name, sb):
         assert st == 0, (name, plan, lib.fdtd_last_error())
         assert n.value > 0
         counts.append(n.value)
-    # the default plan is plan 0
+    # the default plan: 1 for the staggered scheme, 10 for a grid of under
+    # 400,000 cells, else 0
+    cells = int(np.prod(sb.divisions))
+    default = 1 if sb.settings_dict.get("scheme", 0) == 1 else (10 if cells < 400000 else 0)
     n = C.c_int64(0)
-    assert lib.fdtd_plan_check(C.byref(sb.scene), -1, C.byref(n)) == 0 and n.value == counts[0]
+    assert lib.fdtd_plan_check(C.byref(sb.scene), -1, C.byref(n)) == 0 and n.value == counts[default]
 
 
 def test_staggered_scheme_rejects_objects():
