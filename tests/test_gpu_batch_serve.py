@@ -107,6 +107,32 @@ def test_engquist_scenes_in_batch_and_server(hip_lib):
         assert np.array_equal(served[i], ref), i
 
 
+def test_staggered_and_speaker_scenes_in_batch_and_server(hip_lib):
+    """Staggered-scheme scenes (their float4 rows in the fused launch) and a
+    speaker scene share launches with collocated scenes in a batch and in the
+    server, and give the single-scene series bit for bit."""
+    sbs, steps = _small_batch(seed=45, n=2)
+    stag_a, _ = scenes.staggered_free_field(seed=63, div=(36, 40, 44), h=0.005, pml=6)
+    stag_b, _ = scenes.staggered_free_field(seed=64, div=(48, 32, 40), h=0.005, pml=8)
+    spk, _ = scenes.speaker_scene(seed=65, n=40)
+    sbs = sbs + [stag_a, stag_b, spk]
+    steps = steps + [110, 140, 100]
+    print("batch:", [sb.divisions for sb in sbs], steps, flush=True)
+    with Batch(sbs, steps, steps_per_graph=16) as b:
+        b.run()
+        got = [b.series(i) for i in range(len(sbs))]
+    with Server(steps_per_block=16) as srv:
+        ids = [srv.submit(sb, st) for sb, st in zip(sbs, steps)]
+        for rid in ids:
+            assert srv.wait(rid, 120000) == 0
+        served = [srv.take(rid)[1] for rid in ids]
+    for i, sb in enumerate(sbs):
+        ref = _alone(sb, steps[i])
+        assert np.abs(ref).max() > 0, i
+        assert np.array_equal(got[i], ref), i
+        assert np.array_equal(served[i], ref), i
+
+
 def _blow_up_scene(seed=9, n=32):
     """A scene whose time step is twice the FDTD stability limit
     (c dt / h = 2/sqrt(3)): its highest spatial mode grows about 14-fold per
