@@ -1100,19 +1100,30 @@ __device__ void mv_fresh(const SceneDev &S, const Geom *pose, const double *acc,
     }
     MlsSums mls;
     double bd = DBL_MAX, nl = 0.0, nc = 0.0;
+    // every corner's history flag and values are loaded first, in one round
+    // trip (the clamped corners are always inside the grid)
+    bool hist[8];
+    float vl8[8], vc8[8];
+#pragma unroll
+    for (int sl = 0; sl < 8; ++sl) {
+        const int q[3] = {(sl >> 2) ? hi3[0] : lo3[0], ((sl >> 1) & 1) ? hi3[1] : lo3[1], (sl & 1) ? hi3[2] : lo3[2]};
+        const int u = mv_uidx(S, q[0], q[1], q[2]);
+        const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
+        hist[sl] = u < 0 || prev_f[u];
+        vl8[sl] = plast[qi];
+        vc8[sl] = pcur[qi];
+    }
 #pragma unroll
     for (int sl = 0; sl < 8; ++sl) {
         const int a = sl >> 2, b = (sl >> 1) & 1, c = sl & 1;
         const int q[3] = {a ? hi3[0] : lo3[0], b ? hi3[1] : lo3[1], c ? hi3[2] : lo3[2]};
         if ((a && hi3[0] == lo3[0]) || (b && hi3[1] == lo3[1]) || (c && hi3[2] == lo3[2])) continue;
-        const int u = mv_uidx(S, q[0], q[1], q[2]);
-        if (u >= 0 && !prev_f[u]) continue;   // no valid history
+        if (!hist[sl]) continue;   // no valid history
         double y[3];
         mv_centre(S, q[0], q[1], q[2], y);
         // a cell that turned solid this step was zeroed
         const bool fluid = mv_fluid_now(S, pose, q);
-        const int qi = q[0] * S.sx + q[1] * S.sy + q[2];
-        const double l = fluid ? (double)plast[qi] : 0.0, cv = fluid ? (double)pcur[qi] : 0.0;
+        const double l = fluid ? (double)vl8[sl] : 0.0, cv = fluid ? (double)vc8[sl] : 0.0;
         const double d2 = (image[0] - y[0]) * (image[0] - y[0]) + (image[1] - y[1]) * (image[1] - y[1]) + (image[2] - y[2]) * (image[2] - y[2]);
         if (d2 < bd) { bd = d2; nl = l; nc = cv; }
         mls.add(y, image, S.h, 0.018, l, cv);
