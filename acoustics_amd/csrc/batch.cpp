@@ -114,27 +114,29 @@ fdtd_status BatchRunner::autotune(hipStream_t st)
 {
     float best = 1e30f;
     int best_plan = 0;
-    hipEvent_t a, b;
-    FDTD_HIP_TRY(hipEventCreate(&a));
-    FDTD_HIP_TRY(hipEventCreate(&b));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    struct Events {
+        hipEvent_t *e;
+        ~Events() { for (int i = 0; i < 2; ++i) if (e[i]) (void)hipEventDestroy(e[i]); }
+    } guard{ev};
+    FDTD_HIP_TRY(hipEventCreate(&ev[0]));
+    FDTD_HIP_TRY(hipEventCreate(&ev[1]));
     for (int p = 0; p < kNumTilePlans; ++p) {
         plan = p;
         fdtd_status s = rebuild(st);
         if (s != FDTD_OK) return s;
         if ((s = prog.enqueue(st, 0)) != FDTD_OK) return s;
-        FDTD_HIP_TRY(hipEventRecord(a, st));
+        FDTD_HIP_TRY(hipEventRecord(ev[0], st));
         if ((s = prog.enqueue(st, 1, 3)) != FDTD_OK) return s;
-        FDTD_HIP_TRY(hipEventRecord(b, st));
-        FDTD_HIP_TRY(hipEventSynchronize(b));
+        FDTD_HIP_TRY(hipEventRecord(ev[1], st));
+        FDTD_HIP_TRY(hipEventSynchronize(ev[1]));
         float ms = 0;
-        FDTD_HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        FDTD_HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
         timings[p] = ms / 3.0f;
         if (ms < best) { best = ms; best_plan = p; }
         for (auto &sc : scenes)
             if (sc) { fdtd_status z = sc->reset_to_start(st); if (z != FDTD_OK) return z; }
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     plan = best_plan;
     return rebuild(st);
 }
@@ -306,18 +308,13 @@ void fdtd_batch_options_default(fdtd_batch_options *o)
     o->tile_config = -1;
 }
 
-fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, int n, const fdtd_batch_options *opt_in,
-                              fdtd_batch **out)
+}  // extern "C"
+
+// Builds the batch's stream, scenes and work list into b; on failure the
+// caller destroys b, which frees whatever was built.
+static fdtd_status batch_fill(fdtd_batch *b, const fdtd_scene *scenes, const int *num_steps, int n,
+                              const fdtd_batch_options &opt)
 {
-    if (!scenes || !num_steps || n <= 0 || !out) { set_last_error("bad batch arguments"); return FDTD_ERR_INVALID; }
-    *out = nullptr;
-    fdtd_batch_options opt;
-    fdtd_batch_options_default(&opt);
-    if (opt_in) opt = *opt_in;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_last_error("no HIP device"); return FDTD_ERR_NO_DEVICE; }
-    std::unique_ptr<fdtd_batch> b(new (std::nothrow) fdtd_batch());
-    if (!b) return FDTD_ERR_OOM;
     FDTD_HIP_TRY(hipGetDevice(&b->device));
     if (opt.stream) b->stream = (hipStream_t)opt.stream;
     else { FDTD_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); b->own_stream = true; }
@@ -334,16 +331,36 @@ fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, in
             return FDTD_ERR_INVALID;
         }
         st = ds->allocate_and_upload(b->stream, std::max(1, num_steps[i]), num_steps[i]);
-        if (st != FDTD_OK) { ds->release(); R.release_scenes(); return st; }
+        if (st != FDTD_OK) { ds->release(); return st; }
         R.scenes.push_back(std::move(ds));
         R.targets.push_back(num_steps[i]);
         R.done.push_back(num_steps[i] == 0);
     }
-    fdtd_status st;
-    if (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) { R.plan = opt.tile_config; st = R.rebuild(b->stream); }
-    else st = R.autotune(b->stream);
-    if (st != FDTD_OK) { R.release_scenes(); return st; }
-    *out = b.release();
+    if (opt.tile_config >= 0 && opt.tile_config < kNumTilePlans) { R.plan = opt.tile_config; return R.rebuild(b->stream); }
+    return R.autotune(b->stream);
+}
+
+extern "C" {
+
+fdtd_status fdtd_batch_create(const fdtd_scene *scenes, const int *num_steps, int n, const fdtd_batch_options *opt_in,
+                              fdtd_batch **out)
+{
+    if (!scenes || !num_steps || n <= 0 || !out) { set_last_error("bad batch arguments"); return FDTD_ERR_INVALID; }
+    *out = nullptr;
+    fdtd_batch_options opt;
+    fdtd_batch_options_default(&opt);
+    if (opt_in) opt = *opt_in;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_last_error("no HIP device"); return FDTD_ERR_NO_DEVICE; }
+    fdtd_batch *b = new (std::nothrow) fdtd_batch();
+    if (!b) return FDTD_ERR_OOM;
+    const fdtd_status st = batch_fill(b, scenes, num_steps, n, opt);
+    if (st != FDTD_OK) {
+        // scenes already uploaded, the program buffers and an owned stream
+        (void)fdtd_batch_destroy(b);
+        return st;
+    }
+    *out = b;
     return FDTD_OK;
 }
 
@@ -435,13 +452,13 @@ fdtd_status fdtd_batch_destroy(fdtd_batch *b)
 {
     if (!b) return FDTD_ERR_INVALID;
     (void)hipSetDevice(b->device);
-    (void)hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->R.drop_graph();
     b->R.reap();
     b->R.prog.release();
     b->R.free_buffers();
     b->R.release_scenes();
-    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return FDTD_OK;
 }

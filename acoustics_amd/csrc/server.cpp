@@ -23,6 +23,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <system_error>
 #include <thread>
 
 #include "device_scene.h"
@@ -385,10 +386,22 @@ fdtd_status fdtd_server_create(const fdtd_server_options *opt_in, fdtd_server **
     if (s->opt.device >= 0) FDTD_HIP_TRY(hipSetDevice(s->opt.device));
     FDTD_HIP_TRY(hipGetDevice(&s->device));
     FDTD_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    FDTD_HIP_TRY(hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking));
-    fdtd_server *raw = s.release();
-    raw->worker = std::thread([raw] { raw->run(); });
-    *out = raw;
+    const hipError_t e = hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s->stream);
+        set_last_error(std::string("upload stream: ") + hipGetErrorString(e));
+        return FDTD_ERR_DEVICE;
+    }
+    fdtd_server *raw = s.get();
+    try {
+        raw->worker = std::thread([raw] { raw->run(); });
+    } catch (const std::system_error &) {
+        (void)hipStreamDestroy(raw->stream);
+        (void)hipStreamDestroy(raw->upload);
+        set_last_error("could not start the server's worker thread");
+        return FDTD_ERR_DEVICE;
+    }
+    *out = s.release();
     return FDTD_OK;
 }
 

@@ -195,3 +195,32 @@ def test_server_take_while_admitting(hip_lib):
         t.join()
     for i, sb in enumerate(sbs):
         assert np.array_equal(got[i], _alone(sb, steps[i])), i
+
+
+def test_batch_create_failure_frees_uploaded_scenes(hip_lib):
+    """A batch whose later scene is refused frees what it already built (the
+    first scene's arena, the program buffers, its stream): five refused
+    creations would otherwise keep five 128^3 arenas (5 x 54.8 MB); the
+    runtime itself keeps about 12 MB per stream created and destroyed."""
+    import ctypes as C
+    from acoustics_amd import capi
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+
+    def free_bytes():
+        f, t = C.c_size_t(), C.c_size_t()
+        assert hip.hipMemGetInfo(C.byref(f), C.byref(t)) == 0
+        return f.value
+
+    sb, _ = scenes.config2(seed=3)
+    need = scene_device_bytes(sb)
+    print("case: refused batch, arena bytes", need, flush=True)
+    with Batch([sb], [8], tile_config=0) as b:   # warm the allocator and the module
+        b.run()
+    before = free_bytes()
+    for _ in range(5):
+        with pytest.raises(capi.FDTDError):
+            Batch([sb, sb], [100, -1], tile_config=0)
+    after = free_bytes()
+    print("free before", before, "after", after, flush=True)
+    assert before - after < 2 * need
