@@ -27,8 +27,9 @@ struct TilePlan {
     int interior_ti = 16;      // planes marched per interior tile
     int pml_ti = 16;           // planes marched per PML tile
     bool interior_vec = false; // whole-row float4 interior tiles where the layout allows
-    bool fused = false;        // one launch for all tile classes (PML tiles first)
+    bool fused = false;        // one launch for all tile classes
     int pml_tk_log2 = 6;       // PML slab columns: up to 1 << pml_tk_log2 along k, 256 >> it along j
+    int band_group = 1;        // float4 row tiles: this many bands of rows advance along x together
 };
 constexpr int kNumTilePlans = 11;
 TilePlan tile_plan(int id);
