@@ -302,6 +302,25 @@ fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n
     build_launch_lists(tiles, tp.fused, lists);
     const std::string e = check_launch_lists(lists, tp.fused, {d}, terms.size());
     if (!e.empty()) { set_last_error("plan " + std::to_string(plan) + ": " + e); return FDTD_ERR_INVALID; }
+    // the launches carry every tile once (the lists only reorder and pad):
+    // the same count and the same sum of per-tile hashes
+    {
+        auto sig = [](const std::vector<TileDesc> &v, uint64_t *sum) {
+            size_t n = 0;
+            for (const TileDesc &t : v) {
+                if (t.type == TILE_EMPTY) continue;
+                uint64_t h = 1469598103934665603ull;
+                for (int x : {t.type, t.i0, t.i1, t.j0, t.j1, t.k0, t.k1, t.slab, t.tb, t.te}) h = (h ^ (uint32_t)x) * 1099511628211ull;
+                *sum += h;
+                ++n;
+            }
+            return n;
+        };
+        uint64_t a = 0, b = 0;
+        size_t na = 0;
+        for (int k = 0; k < NUM_KINDS; ++k) na += sig(lists[k], &a);
+        if (na != sig(tiles, &b) || a != b) return fail_early(plan, "the launch lists do not carry every tile exactly once");
+    }
     // every cell lies in exactly one box tile; object cells (solid or next to
     // an object) only in boxes that skip them, and each record in one
     // object-cell tile.  A moving scene's reclassified box is written by its
