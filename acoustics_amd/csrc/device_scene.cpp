@@ -487,7 +487,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                         const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
                         t.src = src_hit(lo, hi);
                     }
-                    if (sl.axis == 2) zslab.push_back(out.size());
+                    if (sl.axis == 2 && sl.e[2] >= 12) zslab.push_back(out.size());
                     out.push_back(t);
                 }
     }
@@ -604,8 +604,11 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         // the fused launch each z-slab tile follows the row tile that holds
         // its last (i, j) column, so both halves of those lines are fetched
         // once: 256^3 step 72.5 -> 71.0 us, the staggered scheme 0.424 ->
-        // 0.436 of peak.  (Listed after the scene's interior tiles, a slab
-        // tile keeps its place in the launch; build_launch_lists.)
+        // 0.436 of peak, 128^3 with a 12-cell PML 19.5 -> 19.3 us.  Slabs
+        // under 12 cells deep keep their place: the 64-scene batch (8-cell
+        // slabs) measured 124.7 -> 121.8 Gcell/s with them moved.  (Listed
+        // after the scene's interior tiles, a slab tile keeps its place in the
+        // launch; build_launch_lists.)
         auto at = [](int i, int j) { return ((uint64_t)(uint32_t)i << 32) | (uint32_t)j; };
         std::unordered_map<uint64_t, std::vector<TileDesc>> ride;
         if (plan.fused && !zslab.empty()) {
