@@ -149,7 +149,7 @@ TilePlan tile_plan(int id)
     // {interior k threads (log2), interior planes, PML planes, whole-row float4 interior, one launch,
     //  PML k threads (log2)[, bands of float4 rows taken together]}
     static const TilePlan plans[kNumTilePlans] = {
-        {6, 2, 8, true, true, 6, 4}, {6, 4, 4, true, true, 6, 4}, {6, 8, 8, true, true, 6},   {6, 4, 8, true, false, 6},
+        {6, 2, 8, true, true, 6, 4}, {6, 4, 4, true, true, 6, 8}, {6, 8, 8, true, true, 6},   {6, 4, 8, true, false, 6},
         {6, 8, 8, false, false, 6}, {6, 4, 8, true, true, 5},   {6, 4, 16, true, true, 6},  {6, 4, 8, true, true, 6},
         {6, 4, 4, false, false, 6}, {6, 4, 8, true, true, 4},   {6, 2, 2, true, true, 6},
     };
@@ -594,9 +594,10 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         // 46.0 against 51.9).  With PML slabs, G = band_group bands advance
         // along x together (G tiles per x step), so the rows between them are
         // read once for both neighbours' y halos: 72.4 us against 73.2 at
-        // 256^3 with G = 4 (2 and 8: 73.1, 72.9); the staggered scheme 198.0
-        // against 199.0.  The Engquist free field, all rows, measured 46.7
-        // against 45.8 with groups, so it keeps single bands.
+        // 256^3 with G = 4 (2 and 8: 73.1, 72.9); the staggered scheme (plan
+        // 1, marches of four planes) takes G = 8: 189.7 us against 194.2 with
+        // 4 and 207.4 with 2.  The Engquist free field, all rows, measured
+        // 46.7 against 45.8 with groups, so it keeps single bands.
         const int rpw = std::max(1, std::min(8, 64 / ((kwv + 3) / 4)));
         const int G = S.slabs.empty() ? 1 : std::max(1, plan.band_group);
         // A z slab's row segment and the row's interior part share a 128-byte
