@@ -467,7 +467,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         return S.moving && box_overlap(lo, hi, S.mv_lo, S.mv_hi);
     };
     // PML slabs first (heaviest tiles start first)
-    std::vector<size_t> zslab;   // positions in out of the z slabs' tiles
+    std::vector<size_t> row_end;   // positions in out of tiles that ride with the row tiles
     for (int q = 0; q < (int)S.slabs.size(); ++q) {
         const PmlSlab &sl = S.slabs[q];
         const int tk = std::min(plan.pml_tk_log2, pow2_ceil_log2(sl.e[2]));
@@ -487,23 +487,32 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                         const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
                         t.src = src_hit(lo, hi);
                     }
-                    if (sl.axis == 2 && sl.e[2] >= 12) zslab.push_back(out.size());
+                    if (sl.axis == 2 && sl.e[2] >= 12) row_end.push_back(out.size());
                     out.push_back(t);
                 }
     }
     // Engquist boundary: the layer-1 shell as six disjoint boxes (the x faces
     // whole, the y faces inside x, the z faces inside x and y), cut into
-    // tiles of at most 256 cells
+    // tiles of at most 256 cells.  A z-face cell is one float of a row's
+    // first or last line: when the float4 rows run in the fused launch, the
+    // z faces are cut into tiles of 64 planes by 4 rows that follow the row
+    // tile holding their last column, while those lines are in the L2
+    // (256^3 free field 45.8 -> 41.0-41.4 us per step; the same tiles in
+    // the boundary class measured 46.4)
     if (S.abc) {
+        const bool ride_rows = plan.fused && plan.interior_vec && !S.visc && S.scheme != FDTD_SCHEME_STAGGERED &&
+                               b[2] - (a[2] & ~3) <= 256;
         const int boxes[6][6] = {
             {1, 2, 1, n[1] - 1, 1, n[2] - 1},        {n[0] - 2, n[0] - 1, 1, n[1] - 1, 1, n[2] - 1},
             {2, n[0] - 2, 1, 2, 1, n[2] - 1},        {2, n[0] - 2, n[1] - 2, n[1] - 1, 1, n[2] - 1},
             {2, n[0] - 2, 2, n[1] - 2, 1, 2},        {2, n[0] - 2, 2, n[1] - 2, n[2] - 2, n[2] - 1},
         };
-        for (const auto &bx : boxes) {
+        for (int f = 0; f < 6; ++f) {
+            const int *bx = boxes[f];
+            const bool zface = ride_rows && f >= 4;
             const int ek = bx[5] - bx[4], ej = bx[3] - bx[2], ei = bx[1] - bx[0];
-            const int tk = std::min(ek, 256), tj = std::min(ej, std::max(1, 256 / tk));
-            const int ti = std::min(ei, std::max(1, 256 / (tk * tj)));
+            const int tk = std::min(ek, 256), tj = zface ? std::min(ej, 4) : std::min(ej, std::max(1, 256 / tk));
+            const int ti = zface ? std::min(ei, 64) : std::min(ei, std::max(1, 256 / (tk * tj)));
             for (int i0 = bx[0]; i0 < bx[1]; i0 += ti)
                 for (int j0 = bx[2]; j0 < bx[3]; j0 += tj)
                     for (int k0 = bx[4]; k0 < bx[5]; k0 += tk) {
@@ -515,6 +524,7 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
                         t.k0 = k0; t.k1 = std::min(k0 + tk, bx[5]);
                         const int lo[3] = {t.i0, t.j0, t.k0}, hi[3] = {t.i1, t.j1, t.k1};
                         t.src = src_hit(lo, hi);
+                        if (zface) row_end.push_back(out.size());
                         out.push_back(t);
                     }
         }
@@ -602,8 +612,9 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         const int G = S.slabs.empty() ? 1 : std::max(1, plan.band_group);
         // A z slab's row segment and the row's interior part share a 128-byte
         // line at each end of the row (16 PML cells, 64 bytes, at 256^3).  In
-        // the fused launch each z-slab tile follows the row tile that holds
-        // its last (i, j) column, so both halves of those lines are fetched
+        // the fused launch each z-slab tile (and each Engquist z-face tile)
+        // follows the row tile that holds its last (i, j) column, so both
+        // halves of those lines are fetched
         // once: 256^3 step 72.5 -> 71.0 us, the staggered scheme 0.424 ->
         // 0.436 of peak, 128^3 with a 12-cell PML 19.5 -> 19.3 us.  Slabs
         // under 12 cells deep keep their place: the 64-scene batch (8-cell
@@ -612,9 +623,9 @@ void DeviceScene::append_tiles(int scene_id, const TilePlan &plan, std::vector<T
         // launch; build_launch_lists.)
         auto at = [](int i, int j) { return ((uint64_t)(uint32_t)i << 32) | (uint32_t)j; };
         std::unordered_map<uint64_t, std::vector<TileDesc>> ride;
-        if (plan.fused && !zslab.empty()) {
+        if (plan.fused && !row_end.empty()) {
             std::vector<char> moved(out.size(), 0);
-            for (size_t q : zslab) {
+            for (size_t q : row_end) {
                 const TileDesc &z = out[q];
                 const int li = z.i1 - 1, lj = z.j1 - 1;
                 if (li < a[0] || li >= b[0] || lj < a[1] || lj >= b[1]) continue;
@@ -812,16 +823,17 @@ void build_launch_lists(const std::vector<TileDesc> &tiles, bool fused_launch, s
     for (int k = 0; k < NUM_KINDS; ++k) by_kind[k].clear();
     if (nhead)
         for (int k = 0; k < NUM_KINDS; ++k) nhead[k] = 0;
-    // a scene lists its PML slab tiles before its interior tiles; a slab tile
-    // listed after them is a z-slab tile that DeviceScene::append_tiles placed
-    // beside the row tile it shares lines with, and keeps that place
+    // a scene lists its PML slab and boundary tiles before its interior
+    // tiles; one listed after them is a z-slab or z-face tile that
+    // DeviceScene::append_tiles placed beside the row tile it shares lines
+    // with, and keeps that place
     int rows_scene = -1;
     for (const TileDesc &t : tiles) {
         if (t.type == TILE_EMPTY) continue;
         if (t.type == TILE_SPECIAL) special.push_back(t);
         else if (t.type == TILE_MOVING) moving.push_back(t);
         else if (t.type == TILE_PML || t.type == TILE_MAC_PML || t.type == TILE_ABC)
-            by_kind[fused_launch && t.type != TILE_ABC && t.scene == rows_scene ? KIND_INTERIOR : KIND_PML].push_back(t);
+            by_kind[fused_launch && t.scene == rows_scene ? KIND_INTERIOR : KIND_PML].push_back(t);
         else { by_kind[KIND_INTERIOR].push_back(t); rows_scene = t.scene; }
     }
     // neighbouring tiles of a scene share an XCD (and its L2)
