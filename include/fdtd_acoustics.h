@@ -532,8 +532,9 @@ fdtd_status fdtd_load_snapshot(fdtd_handle *h, const char *pattern, const char *
 /* Host-only dry run of the device planner (no counterpart in the reference,
  * which has no tiling): builds the scene's tiles for tile_config (-1: default)
  * and the launch lists of its step, and checks on the host that every tile's
- * class matches its launch and every read stays inside the grid.  No device
- * is touched.  n_tiles (may be NULL) gets the number of workgroups per step. */
+ * class matches its launch, every read stays inside the grid and the launch
+ * lists carry every tile exactly once.  No device is touched.  n_tiles (may
+ * be NULL) gets the number of workgroups per step. */
 fdtd_status fdtd_plan_check(const fdtd_scene *scene, int tile_config, int64_t *n_tiles);
 fdtd_status fdtd_server_destroy(fdtd_server *s);
 
